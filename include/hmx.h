@@ -1,0 +1,201 @@
+/*
+ * hmx.h -- C ABI of the MI355X-native H-matrix builder and H-matvec.
+ *
+ * Every entry point below replaces one free function of the reference
+ * (AdnanJaljuli/KAUST-Hierarchical-Matrices); the reference interface each
+ * one stands in for is cited as file:line.  Plain pointers and sizes only:
+ * device buffers passed in (points, vectors) are caller-owned device memory;
+ * objects returned through an out-handle are owned by the library and
+ * released with the matching hmx_free* call.
+ *
+ * Threading: one hmx_context per stream; calls on one context are not
+ * re-entrant, distinct contexts may be used from distinct threads.
+ * Errors: every call returns hmx_status; hmx_last_error() describes the most
+ * recent failure on that context (the reference asserts/exits instead:
+ * src/constructTLRMatrix.cu:75, src/helperFunctions.cuh:9-17).
+ */
+#ifndef HMX_H
+#define HMX_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum hmx_status {
+    HMX_OK = 0,
+    HMX_EINVAL = 1,       /* bad argument (sizes, null pointers, unsupported value) */
+    HMX_ENOMEM = 2,       /* device or host allocation failed */
+    HMX_EDEVICE = 3,      /* HIP runtime error */
+    HMX_ERANKCAP = 4,     /* reserved: strict mode rank-cap failure */
+    HMX_EUNSUPPORTED = 5, /* valid in the reference but not provided (see hmx_last_error) */
+    HMX_ESTATE = 6        /* object in the wrong state (e.g. TLR not in Morton order) */
+} hmx_status;
+
+/* DIVISION_METHOD (src/config.h:48-52); only FULL_TREE (the default) is built */
+enum { HMX_POWER_OF_TWO_ON_LEFT = 0, HMX_DIVIDE_IN_HALF = 1, HMX_FULL_TREE = 2 };
+/* ADMISSIBILITY_CONDITION (src/config.h:88-91) plus the max-diameter rule */
+enum { HMX_WEAK_ADMISSIBILITY = 0, HMX_BOX_CENTER_ADMISSIBILITY = 1, HMX_MAX_DIAM_ADMISSIBILITY = 2 };
+/* rank caps: 0 = reference schedule (b/2 for TLR tiles, generateMaxRanks for
+   upper levels); 1 = uncapped (tile dimension) */
+enum { HMX_MAXRANK_REF = 0, HMX_MAXRANK_FULL = 1 };
+/* ordering of a TLR matrix (src/TLRMatrix.cuh:8) */
+enum { HMX_COLUMN_MAJOR = 0, HMX_MORTON = 1 };
+
+/* Config (src/config.h:109-117) with the reference defaults (src/config.h:119-127)
+   plus the parameters the reference hard-codes (src/main.cu:107,
+   src/constructTLRMatrix.cu:73, src/HMatrixStructure.cu:7, src/TLRMatrixHelpers.cu:40-42). */
+typedef struct hmx_config {
+    int32_t divMethod;               /* HMX_FULL_TREE */
+    int32_t admissibilityCondition;  /* HMX_WEAK_ADMISSIBILITY */
+    uint32_t numberOfInputPoints;    /* 1024 */
+    uint32_t dimensionOfInputPoints; /* 2 */
+    uint32_t bucketSize;             /* 32 */
+    uint32_t vectorWidth;            /* 16 */
+    float lowestLevelTolerance;      /* 1e-5 (float, as in the reference) */
+    float eta;                       /* 1.0 (DEFAULT_ETA) */
+    double correlationLength;        /* <= 0: 0.2 if d == 3 else 0.1 */
+    int32_t araBlockSize;            /* 16 samples per ARA block */
+    int32_t araR;                    /* 10 (ARA_R) */
+    int32_t maxRankPolicy;           /* HMX_MAXRANK_REF */
+    uint64_t seed;                   /* 1234 */
+} hmx_config;
+
+typedef struct hmx_context hmx_context;
+typedef struct hmx_kdtree hmx_kdtree;
+typedef struct hmx_structure hmx_structure;
+typedef struct hmx_tlr hmx_tlr;
+typedef struct hmx_hmatrix hmx_hmatrix;
+
+/* ---- lifecycle -------------------------------------------------------- */
+/* stream: a hipStream_t (NULL = default stream).  Replaces the implicit
+   device-0/default-stream state of src/config.h:167-168 and magma_init
+   (src/main.cu:62). */
+hmx_status hmx_create(hmx_context** ctx, void* stream);
+hmx_status hmx_destroy(hmx_context* ctx);
+const char* hmx_last_error(const hmx_context* ctx);
+/* parseArgs defaults, src/config.h:119-127 */
+void hmx_config_default(hmx_config* cfg);
+/* library version string */
+const char* hmx_version(void);
+
+/* ---- dataset ---------------------------------------------------------- */
+/* generateDataset, src/generateDataset.cu:15-19.  d_points: n*d doubles, SoA
+   (p[j*n+i]); uniform [0,1) from counter-based Philox4x32-10 keyed by seed. */
+hmx_status hmx_generateDataset(hmx_context* ctx, uint32_t n, uint32_t d, uint64_t seed, double* d_points);
+/* generateRandomVector, src/helperFunctions.cu:59-65: n x vectorWidth
+   column-major N(0,1) doubles (Philox + Box-Muller) */
+hmx_status hmx_generateRandomVector(hmx_context* ctx, uint32_t vectorWidth, uint32_t n, uint64_t seed,
+                                    double* d_out);
+
+/* ---- KD-tree ---------------------------------------------------------- */
+/* allocateKDTree + constructKDTree, src/kDTree.cu:6-31 and
+   src/kDTreeConstruction.cu:14-231 */
+hmx_status hmx_constructKDTree(hmx_context* ctx, uint32_t n, uint32_t d, uint32_t bucketSize, int32_t divMethod,
+                               const double* d_points, hmx_kdtree** tree);
+/* numSegments / segmentSize (src/kDTree.cuh:8-14) and the number of split levels */
+hmx_status hmx_kdtree_info(const hmx_kdtree* tree, uint32_t* numSegments, uint32_t* segmentSize,
+                           uint32_t* numLevels);
+/* host copies: perm[n] (segmentIndices), offsets[numSegments+1] (segmentOffsets),
+   bbmin/bbmax[(2^(numLevels+1)-1)*d] laid out [level][node][dim] (level l at node 2^l-1);
+   any pointer may be NULL */
+hmx_status hmx_kdtree_copy(hmx_context* ctx, const hmx_kdtree* tree, int32_t* perm, int32_t* offsets,
+                           double* bbmin, double* bbmax);
+/* device pointer to segmentIndices (valid while the tree lives) */
+const int32_t* hmx_kdtree_device_perm(const hmx_kdtree* tree);
+/* freeKDTree, src/kDTree.cu:33-36 */
+hmx_status hmx_freeKDTree(hmx_kdtree* tree);
+
+/* ---- H-matrix structure ----------------------------------------------- */
+/* constructHMatrixStructure, src/HMatrixStructure.cu:133-160, called with
+   upperPowerOfTwo(n) as in src/main.cu:89-96 */
+hmx_status hmx_constructHMatrixStructure(hmx_context* ctx, uint32_t n, uint32_t d, uint32_t bucketSize,
+                                         int32_t admissibilityCondition, float eta, const hmx_kdtree* tree,
+                                         hmx_structure** structure);
+/* Same, from host bounding boxes (host only, no device needed) */
+hmx_status hmx_structure_from_boxes(uint32_t n, uint32_t d, uint32_t bucketSize, int32_t admissibilityCondition,
+                                    float eta, const double* bbmin, const double* bbmax,
+                                    hmx_structure** structure);
+/* numLevels and numTiles[numLevels] (src/HMatrixStructure.cuh:8-13); numTiles[l-1] = tiles at level l */
+hmx_status hmx_structure_info(const hmx_structure* s, int32_t* numLevels, int32_t* numTiles);
+/* tileIndices[level-1] (Morton codes, column in even bits) */
+hmx_status hmx_structure_tiles(const hmx_structure* s, int32_t level, int32_t* tileIndices);
+/* JDS of one level (src/JDS.cuh:7-15): rowIndices[2^level], columnIndices[numTiles],
+   tilePos[numTiles], iterPtr[numDiagonals+1] (capacity 2^level+1) */
+hmx_status hmx_structure_jds(const hmx_structure* s, int32_t level, int32_t* rowIndices, int32_t* columnIndices,
+                             int32_t* tilePos, int32_t* iterPtr, int32_t* numDiagonals);
+/* freeHMatrixStructure, src/HMatrixStructure.cu:162-168 */
+hmx_status hmx_freeHMatrixStructure(hmx_structure* s);
+
+/* ---- TLR matrix ------------------------------------------------------- */
+/* createColumnMajorLRMatrix, src/constructTLRMatrix.cu:13-151: batched ARA of
+   all off-diagonal bucketSize x bucketSize tiles, kernel tiles evaluated on
+   the fly; dense diagonal blocks.  rankSum may be NULL. */
+hmx_status hmx_createColumnMajorLRMatrix(hmx_context* ctx, const hmx_config* cfg, const hmx_kdtree* tree,
+                                         const double* d_points, hmx_tlr** matrix, uint64_t* rankSum);
+/* convertColumnMajorToMorton, src/helperFunctions.cu:8-49 */
+hmx_status hmx_convertColumnMajorToMorton(hmx_context* ctx, const hmx_tlr* matrix, hmx_tlr** mortonMatrix);
+/* numSegments, segmentSize, ordering, rankSum, number of rank-cap hits */
+hmx_status hmx_tlr_info(const hmx_tlr* m, uint32_t* numSegments, uint32_t* segmentSize, int32_t* ordering,
+                        uint64_t* rankSum, uint64_t* capHits);
+/* host copies of blockRanks[S*S], blockOffsets[S*S+1] (int64), U/V [rankSum*b],
+   diagonal[S*b*b]; any pointer may be NULL */
+hmx_status hmx_tlr_copy(hmx_context* ctx, const hmx_tlr* m, int32_t* blockRanks, int64_t* blockOffsets, double* U,
+                        double* V, double* diagonal);
+/* freeMatrix, src/TLRMatrix.cu:5-11 */
+hmx_status hmx_freeMatrix(hmx_tlr* m);
+
+/* ---- hierarchical matrix ---------------------------------------------- */
+/* generateMaxRanks, src/helperFunctions.cu:67-74 (maxRanks has numLevels-2 entries) */
+void hmx_generateMaxRanks(uint32_t numLevels, uint32_t bucketSize, uint32_t* maxRanks);
+/* allocateHMatrix + generateHMatrixFromStruct, src/HMatrix.cu:57-103 and
+   src/constructHMatrixFromStruct.cu:13-103: leaf level copied from the Morton
+   TLR, upper levels recompressed by batched low-rank-sum ARA. */
+hmx_status hmx_generateHMatrixFromStruct(hmx_context* ctx, const hmx_config* cfg, const hmx_kdtree* tree,
+                                         const hmx_structure* structure, const hmx_tlr* mortonMatrix,
+                                         hmx_hmatrix** hmatrix);
+/* per level l (1..numLevels-1): tile count, rank sum, max rank, tolerance used, cap hits */
+hmx_status hmx_hmatrix_level_info(const hmx_hmatrix* h, int32_t level, int32_t* numTiles, int64_t* rankSum,
+                                  int32_t* maxRank, float* tolerance, int32_t* capHits);
+/* host copies of one level: ranks[numTiles], U/V[rankSum*m] (m = bus*b rows per column) */
+hmx_status hmx_hmatrix_level_copy(hmx_context* ctx, const hmx_hmatrix* h, int32_t level, int32_t* ranks, double* U,
+                                  double* V);
+/* diagonal blocks [S*b*b] (HMatrix.diagonalBlocks, src/HMatrix.cuh:30-34) */
+hmx_status hmx_hmatrix_diag_copy(hmx_context* ctx, const hmx_hmatrix* h, double* diag);
+/* bytes of factor storage read by one matvec (low-rank U+V and dense blocks) */
+hmx_status hmx_hmatrix_bytes(const hmx_hmatrix* h, uint64_t* lowRankBytes, uint64_t* denseBytes,
+                             uint64_t* factorEntries);
+/* freeHMatrix, src/HMatrix.cu:105-110 */
+hmx_status hmx_freeHMatrix(hmx_hmatrix* h);
+
+/* ---- H-matvec ---------------------------------------------------------- */
+/* HMatrixVecMult, src/HMatrixVectorMultiplication.cu:10-26: out = H * in,
+   in/out are n x vectorWidth column-major (ld n) FP64 device buffers in
+   KD-permuted order; out is overwritten.  Requires full leaves (n = S*b),
+   as the reference does (src/cutlassDiagonalXVector.cu:59-65). */
+hmx_status hmx_HMatrixVecMult(hmx_context* ctx, const hmx_hmatrix* h, uint32_t vectorWidth, const double* d_in,
+                              double* d_out);
+/* out = H^T * in (no reference counterpart; the backward of HMatrixVecMult) */
+hmx_status hmx_HMatrixVecMultTranspose(hmx_context* ctx, const hmx_hmatrix* h, uint32_t vectorWidth,
+                                       const double* d_in, double* d_out);
+/* Times the matvec kernels with hipEvents on the context stream over iters
+   calls: ms[0] = phase 1 (V^T x), ms[1] = partial reduction, ms[2] = phase 2
+   (row owner U z + D x), ms[3] = whole call.  Averages per call. */
+hmx_status hmx_matvec_profile(hmx_context* ctx, const hmx_hmatrix* h, uint32_t vectorWidth, const double* d_in,
+                              double* d_out, int32_t iters, float* ms);
+
+/* ---- dense check (EXPAND_MATRIX oracle on sampled rows) ----------------- */
+/* checkErrorInHmatrixVecMult / ReferenceGemm_kernel, src/expandMatrix.cu:174-234,
+   restricted to nrows sampled rows: Y[r,:] = sum_j K(x_perm[rows[r]], x_perm[j]) X[j,:]
+   (Y nrows x k, X n x k, column-major, device buffers). */
+hmx_status hmx_denseRowsXVector(hmx_context* ctx, uint32_t n, uint32_t d, const double* d_points,
+                                const int32_t* d_perm, double correlationLength, uint32_t nrows,
+                                const int32_t* d_rows, uint32_t k, const double* d_X, double* d_Y);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* HMX_H */

@@ -1,0 +1,428 @@
+// C ABI (include/hmx.h): argument checks, exception -> status translation,
+// and the reference-named entry points.
+#include <cstring>
+#include <new>
+
+#include "hmx_internal.h"
+
+using namespace hmx;
+
+namespace {
+
+thread_local std::string g_noctx_error;
+
+template <class F>
+hmx_status guard(hmx_context* ctx, F&& f) {
+    try {
+        f();
+        if (ctx) ctx->last_error.clear();
+        return HMX_OK;
+    } catch (const Error& e) {
+        if (ctx) ctx->last_error = e.what();
+        else g_noctx_error = e.what();
+        return e.status;
+    } catch (const std::bad_alloc&) {
+        if (ctx) ctx->last_error = "host allocation failed";
+        return HMX_ENOMEM;
+    } catch (const std::exception& e) {
+        if (ctx) ctx->last_error = e.what();
+        return HMX_EDEVICE;
+    }
+}
+
+#define NEED(p)                                                                  \
+    do {                                                                         \
+        if (!(p)) throw Error(HMX_EINVAL, std::string("null argument: ") + #p); \
+    } while (0)
+
+void check_config(const hmx_config* cfg) {
+    NEED(cfg);
+    HMX_CHECK(cfg->divMethod == HMX_FULL_TREE, HMX_EUNSUPPORTED,
+              "only the FULL_TREE division method is provided");
+    HMX_CHECK(cfg->lowestLevelTolerance > 0.f, HMX_EINVAL, "lowestLevelTolerance must be > 0");
+    HMX_CHECK(cfg->maxRankPolicy == HMX_MAXRANK_REF || cfg->maxRankPolicy == HMX_MAXRANK_FULL, HMX_EINVAL,
+              "maxRankPolicy must be 0 (reference) or 1 (full)");
+    HMX_CHECK(cfg->araR >= 1, HMX_EINVAL, "araR must be >= 1");
+}
+
+}  // namespace
+
+extern "C" {
+
+hmx_status hmx_create(hmx_context** ctx, void* stream) {
+    if (!ctx) return HMX_EINVAL;
+    *ctx = nullptr;
+    hmx_context* c = new (std::nothrow) hmx_context();
+    if (!c) return HMX_ENOMEM;
+    c->stream = (hipStream_t)stream;
+    hmx_status st = guard(c, [&] { HMX_HIP(hipGetDevice(&c->device)); });
+    if (st != HMX_OK) {
+        g_noctx_error = c->last_error;
+        delete c;
+        return st;
+    }
+    *ctx = c;
+    return HMX_OK;
+}
+
+hmx_status hmx_destroy(hmx_context* ctx) {
+    delete ctx;
+    return HMX_OK;
+}
+
+const char* hmx_last_error(const hmx_context* ctx) { return ctx ? ctx->last_error.c_str() : g_noctx_error.c_str(); }
+
+void hmx_config_default(hmx_config* cfg) {
+    if (!cfg) return;
+    std::memset(cfg, 0, sizeof(*cfg));
+    cfg->divMethod = HMX_FULL_TREE;
+    cfg->admissibilityCondition = HMX_WEAK_ADMISSIBILITY;
+    cfg->numberOfInputPoints = 1024;
+    cfg->dimensionOfInputPoints = 2;
+    cfg->bucketSize = 32;
+    cfg->vectorWidth = 16;
+    cfg->lowestLevelTolerance = 1e-5f;
+    cfg->eta = 1.0f;
+    cfg->correlationLength = 0.0;
+    cfg->araBlockSize = 16;
+    cfg->araR = 10;
+    cfg->maxRankPolicy = HMX_MAXRANK_REF;
+    cfg->seed = 1234;
+}
+
+const char* hmx_version(void) { return "hmx 0.1 gfx950"; }
+
+hmx_status hmx_generateDataset(hmx_context* ctx, uint32_t n, uint32_t d, uint64_t seed, double* d_points) {
+    if (!ctx) return HMX_EINVAL;
+    return guard(ctx, [&] {
+        NEED(d_points);
+        HMX_CHECK(d >= 1 && d <= 3, HMX_EINVAL, "dimensionOfInputPoints must be 1..3");
+        generate_dataset(*ctx, n, d, seed, d_points);
+    });
+}
+
+hmx_status hmx_generateRandomVector(hmx_context* ctx, uint32_t vectorWidth, uint32_t n, uint64_t seed,
+                                    double* d_out) {
+    if (!ctx) return HMX_EINVAL;
+    return guard(ctx, [&] {
+        NEED(d_out);
+        generate_random_vector(*ctx, vectorWidth, n, seed, d_out);
+    });
+}
+
+hmx_status hmx_constructKDTree(hmx_context* ctx, uint32_t n, uint32_t d, uint32_t bucketSize, int32_t divMethod,
+                               const double* d_points, hmx_kdtree** tree) {
+    if (!ctx) return HMX_EINVAL;
+    return guard(ctx, [&] {
+        NEED(tree);
+        NEED(d_points);
+        *tree = nullptr;
+        HMX_CHECK(divMethod == HMX_FULL_TREE, HMX_EUNSUPPORTED, "only the FULL_TREE division method is provided");
+        hmx_kdtree* t = new hmx_kdtree();
+        t->n = n;
+        t->d = d;
+        t->b = bucketSize;
+        try {
+            build_kdtree(*ctx, *t, d_points);
+        } catch (...) {
+            delete t;
+            throw;
+        }
+        *tree = t;
+    });
+}
+
+hmx_status hmx_kdtree_info(const hmx_kdtree* tree, uint32_t* numSegments, uint32_t* segmentSize,
+                           uint32_t* numLevels) {
+    if (!tree) return HMX_EINVAL;
+    if (numSegments) *numSegments = tree->numSegments;
+    if (segmentSize) *segmentSize = tree->b;
+    if (numLevels) *numLevels = tree->numLevels;
+    return HMX_OK;
+}
+
+hmx_status hmx_kdtree_copy(hmx_context* ctx, const hmx_kdtree* tree, int32_t* perm, int32_t* offsets, double* bbmin,
+                           double* bbmax) {
+    if (!ctx || !tree) return HMX_EINVAL;
+    return guard(ctx, [&] {
+        if (perm)
+            HMX_HIP(hipMemcpyAsync(perm, tree->perm.p, tree->perm.bytes(), hipMemcpyDeviceToHost, ctx->stream));
+        if (offsets) std::memcpy(offsets, tree->h_offsets.data(), tree->h_offsets.size() * sizeof(int32_t));
+        if (bbmin) std::memcpy(bbmin, tree->h_bbmin.data(), tree->h_bbmin.size() * sizeof(double));
+        if (bbmax) std::memcpy(bbmax, tree->h_bbmax.data(), tree->h_bbmax.size() * sizeof(double));
+        HMX_HIP(hipStreamSynchronize(ctx->stream));
+    });
+}
+
+const int32_t* hmx_kdtree_device_perm(const hmx_kdtree* tree) { return tree ? tree->perm.p : nullptr; }
+
+hmx_status hmx_freeKDTree(hmx_kdtree* tree) {
+    delete tree;
+    return HMX_OK;
+}
+
+hmx_status hmx_constructHMatrixStructure(hmx_context* ctx, uint32_t n, uint32_t d, uint32_t bucketSize,
+                                         int32_t admissibilityCondition, float eta, const hmx_kdtree* tree,
+                                         hmx_structure** structure) {
+    if (!ctx) return HMX_EINVAL;
+    return guard(ctx, [&] {
+        NEED(tree);
+        NEED(structure);
+        *structure = nullptr;
+        HMX_CHECK(tree->d == d && tree->b == bucketSize, HMX_EINVAL, "structure arguments do not match the tree");
+        HMX_CHECK(upper_pow2(n) == upper_pow2(tree->n), HMX_EINVAL, "structure n does not match the tree");
+        hmx_structure* s = new hmx_structure();
+        try {
+            build_structure(*s, n, d, bucketSize, admissibilityCondition, eta, tree->h_bbmin.data(),
+                            tree->h_bbmax.data());
+        } catch (...) {
+            delete s;
+            throw;
+        }
+        HMX_CHECK((uint32_t)(1u << (s->numLevels - 1)) == tree->numSegments, HMX_EINVAL,
+                  "structure depth does not match the tree");
+        *structure = s;
+    });
+}
+
+hmx_status hmx_structure_from_boxes(uint32_t n, uint32_t d, uint32_t bucketSize, int32_t admissibilityCondition,
+                                    float eta, const double* bbmin, const double* bbmax, hmx_structure** structure) {
+    return guard(nullptr, [&] {
+        NEED(structure);
+        NEED(bbmin);
+        NEED(bbmax);
+        *structure = nullptr;
+        hmx_structure* s = new hmx_structure();
+        try {
+            build_structure(*s, n, d, bucketSize, admissibilityCondition, eta, bbmin, bbmax);
+        } catch (...) {
+            delete s;
+            throw;
+        }
+        *structure = s;
+    });
+}
+
+hmx_status hmx_structure_info(const hmx_structure* s, int32_t* numLevels, int32_t* numTiles) {
+    if (!s) return HMX_EINVAL;
+    if (numLevels) *numLevels = s->numLevels;
+    if (numTiles) std::memcpy(numTiles, s->numTiles.data(), s->numTiles.size() * sizeof(int32_t));
+    return HMX_OK;
+}
+
+hmx_status hmx_structure_tiles(const hmx_structure* s, int32_t level, int32_t* tileIndices) {
+    if (!s || !tileIndices || level < 1 || level >= s->numLevels) return HMX_EINVAL;
+    const auto& v = s->tileIndices[level - 1];
+    std::memcpy(tileIndices, v.data(), v.size() * sizeof(int32_t));
+    return HMX_OK;
+}
+
+hmx_status hmx_structure_jds(const hmx_structure* s, int32_t level, int32_t* rowIndices, int32_t* columnIndices,
+                             int32_t* tilePos, int32_t* iterPtr, int32_t* numDiagonals) {
+    if (!s || level < 1 || level >= s->numLevels) return HMX_EINVAL;
+    const auto& j = s->jds[level - 1];
+    if (rowIndices) std::memcpy(rowIndices, j.rowIndices.data(), j.rowIndices.size() * sizeof(int32_t));
+    if (columnIndices) std::memcpy(columnIndices, j.columnIndices.data(), j.columnIndices.size() * sizeof(int32_t));
+    if (tilePos) std::memcpy(tilePos, j.tilePos.data(), j.tilePos.size() * sizeof(int32_t));
+    if (iterPtr) std::memcpy(iterPtr, j.iterPtr.data(), j.iterPtr.size() * sizeof(int32_t));
+    if (numDiagonals) *numDiagonals = (int32_t)j.iterPtr.size() - 1;
+    return HMX_OK;
+}
+
+hmx_status hmx_freeHMatrixStructure(hmx_structure* s) {
+    delete s;
+    return HMX_OK;
+}
+
+hmx_status hmx_createColumnMajorLRMatrix(hmx_context* ctx, const hmx_config* cfg, const hmx_kdtree* tree,
+                                         const double* d_points, hmx_tlr** matrix, uint64_t* rankSum) {
+    if (!ctx) return HMX_EINVAL;
+    return guard(ctx, [&] {
+        check_config(cfg);
+        NEED(tree);
+        NEED(d_points);
+        NEED(matrix);
+        *matrix = nullptr;
+        hmx_tlr* m = new hmx_tlr();
+        try {
+            build_tlr_cm(*ctx, *cfg, *tree, d_points, *m);
+        } catch (...) {
+            delete m;
+            throw;
+        }
+        *matrix = m;
+        if (rankSum) *rankSum = m->rankSum;
+    });
+}
+
+hmx_status hmx_convertColumnMajorToMorton(hmx_context* ctx, const hmx_tlr* matrix, hmx_tlr** mortonMatrix) {
+    if (!ctx) return HMX_EINVAL;
+    return guard(ctx, [&] {
+        NEED(matrix);
+        NEED(mortonMatrix);
+        *mortonMatrix = nullptr;
+        hmx_tlr* m = new hmx_tlr();
+        try {
+            convert_to_morton(*ctx, *matrix, *m);
+        } catch (...) {
+            delete m;
+            throw;
+        }
+        *mortonMatrix = m;
+    });
+}
+
+hmx_status hmx_tlr_info(const hmx_tlr* m, uint32_t* numSegments, uint32_t* segmentSize, int32_t* ordering,
+                        uint64_t* rankSum, uint64_t* capHits) {
+    if (!m) return HMX_EINVAL;
+    if (numSegments) *numSegments = m->S;
+    if (segmentSize) *segmentSize = m->b;
+    if (ordering) *ordering = m->ordering;
+    if (rankSum) *rankSum = m->rankSum;
+    if (capHits) *capHits = m->capHits;
+    return HMX_OK;
+}
+
+hmx_status hmx_tlr_copy(hmx_context* ctx, const hmx_tlr* m, int32_t* blockRanks, int64_t* blockOffsets, double* U,
+                        double* V, double* diagonal) {
+    if (!ctx || !m) return HMX_EINVAL;
+    return guard(ctx, [&] {
+        auto cp = [&](void* dst, const void* src, size_t bytes) {
+            if (dst && bytes) HMX_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, ctx->stream));
+        };
+        cp(blockRanks, m->ranks.p, m->ranks.bytes());
+        cp(blockOffsets, m->offsets.p, m->offsets.bytes());
+        cp(U, m->U.p, m->U.bytes());
+        cp(V, m->V.p, m->V.bytes());
+        cp(diagonal, m->diagonal.p, m->diagonal.bytes());
+        HMX_HIP(hipStreamSynchronize(ctx->stream));
+    });
+}
+
+hmx_status hmx_freeMatrix(hmx_tlr* m) {
+    delete m;
+    return HMX_OK;
+}
+
+void hmx_generateMaxRanks(uint32_t numLevels, uint32_t bucketSize, uint32_t* maxRanks) {
+    if (!maxRanks || numLevels < 3) return;
+    for (uint32_t i = 0; i < numLevels - 2; ++i) {
+        maxRanks[i] = bucketSize * (1u << i);
+        if (i > 5) maxRanks[i] /= 4;
+    }
+}
+
+hmx_status hmx_generateHMatrixFromStruct(hmx_context* ctx, const hmx_config* cfg, const hmx_kdtree* tree,
+                                         const hmx_structure* structure, const hmx_tlr* mortonMatrix,
+                                         hmx_hmatrix** hmatrix) {
+    if (!ctx) return HMX_EINVAL;
+    return guard(ctx, [&] {
+        check_config(cfg);
+        NEED(tree);
+        NEED(structure);
+        NEED(mortonMatrix);
+        NEED(hmatrix);
+        *hmatrix = nullptr;
+        hmx_hmatrix* h = new hmx_hmatrix();
+        try {
+            build_hmatrix(*ctx, *cfg, *tree, *structure, *mortonMatrix, *h);
+        } catch (...) {
+            delete h;
+            throw;
+        }
+        *hmatrix = h;
+    });
+}
+
+hmx_status hmx_hmatrix_level_info(const hmx_hmatrix* h, int32_t level, int32_t* numTiles, int64_t* rankSum,
+                                  int32_t* maxRank, float* tolerance, int32_t* capHits) {
+    if (!h || level < 1 || level >= h->numLevels) return HMX_EINVAL;
+    const HLevel& lv = h->levels[level - 1];
+    if (numTiles) *numTiles = lv.numTiles;
+    if (rankSum) *rankSum = lv.rankSum;
+    if (maxRank) *maxRank = lv.maxRank;
+    if (tolerance) *tolerance = lv.tol;
+    if (capHits) *capHits = lv.capHits;
+    return HMX_OK;
+}
+
+hmx_status hmx_hmatrix_level_copy(hmx_context* ctx, const hmx_hmatrix* h, int32_t level, int32_t* ranks, double* U,
+                                  double* V) {
+    if (!ctx || !h || level < 1 || level >= h->numLevels) return HMX_EINVAL;
+    return guard(ctx, [&] {
+        const HLevel& lv = h->levels[level - 1];
+        if (ranks) std::memcpy(ranks, lv.h_ranks.data(), lv.h_ranks.size() * sizeof(int32_t));
+        if (U && lv.U.n) HMX_HIP(hipMemcpyAsync(U, lv.U.p, lv.U.bytes(), hipMemcpyDeviceToHost, ctx->stream));
+        if (V && lv.V.n) HMX_HIP(hipMemcpyAsync(V, lv.V.p, lv.V.bytes(), hipMemcpyDeviceToHost, ctx->stream));
+        HMX_HIP(hipStreamSynchronize(ctx->stream));
+    });
+}
+
+hmx_status hmx_hmatrix_diag_copy(hmx_context* ctx, const hmx_hmatrix* h, double* diag) {
+    if (!ctx || !h || !diag) return HMX_EINVAL;
+    return guard(ctx, [&] {
+        HMX_HIP(hipMemcpyAsync(diag, h->diag.p, h->diag.bytes(), hipMemcpyDeviceToHost, ctx->stream));
+        HMX_HIP(hipStreamSynchronize(ctx->stream));
+    });
+}
+
+hmx_status hmx_hmatrix_bytes(const hmx_hmatrix* h, uint64_t* lowRankBytes, uint64_t* denseBytes,
+                             uint64_t* factorEntries) {
+    if (!h) return HMX_EINVAL;
+    if (lowRankBytes) *lowRankBytes = h->lowRankBytes;
+    if (denseBytes) *denseBytes = h->denseBytes;
+    if (factorEntries) *factorEntries = h->factorEntries;
+    return HMX_OK;
+}
+
+hmx_status hmx_freeHMatrix(hmx_hmatrix* h) {
+    delete h;
+    return HMX_OK;
+}
+
+hmx_status hmx_HMatrixVecMult(hmx_context* ctx, const hmx_hmatrix* h, uint32_t vectorWidth, const double* d_in,
+                              double* d_out) {
+    if (!ctx) return HMX_EINVAL;
+    return guard(ctx, [&] {
+        NEED(h);
+        hmatvec(*ctx, const_cast<hmx_hmatrix&>(*h), vectorWidth, d_in, d_out, false, nullptr, 0);
+    });
+}
+
+hmx_status hmx_HMatrixVecMultTranspose(hmx_context* ctx, const hmx_hmatrix* h, uint32_t vectorWidth,
+                                       const double* d_in, double* d_out) {
+    if (!ctx) return HMX_EINVAL;
+    return guard(ctx, [&] {
+        NEED(h);
+        hmatvec(*ctx, const_cast<hmx_hmatrix&>(*h), vectorWidth, d_in, d_out, true, nullptr, 0);
+    });
+}
+
+hmx_status hmx_matvec_profile(hmx_context* ctx, const hmx_hmatrix* h, uint32_t vectorWidth, const double* d_in,
+                              double* d_out, int32_t iters, float* ms) {
+    if (!ctx) return HMX_EINVAL;
+    return guard(ctx, [&] {
+        NEED(h);
+        NEED(ms);
+        HMX_CHECK(vectorWidth == 1 || vectorWidth == 4 || vectorWidth == 8 || vectorWidth == 16, HMX_EINVAL,
+                  "profiling needs vectorWidth in {1,4,8,16}");
+        hmatvec(*ctx, const_cast<hmx_hmatrix&>(*h), vectorWidth, d_in, d_out, false, ms, iters);
+    });
+}
+
+hmx_status hmx_denseRowsXVector(hmx_context* ctx, uint32_t n, uint32_t d, const double* d_points,
+                                const int32_t* d_perm, double correlationLength, uint32_t nrows,
+                                const int32_t* d_rows, uint32_t k, const double* d_X, double* d_Y) {
+    if (!ctx) return HMX_EINVAL;
+    return guard(ctx, [&] {
+        NEED(d_points);
+        NEED(d_perm);
+        NEED(d_rows);
+        NEED(d_X);
+        NEED(d_Y);
+        dense_rows(*ctx, n, d, d_points, d_perm, correlationLength > 0 ? correlationLength : default_ell(d), nrows,
+                   d_rows, k, d_X, d_Y);
+    });
+}
+
+}  // extern "C"

@@ -1,0 +1,604 @@
+// Hierarchical matrix assembly: allocateHMatrix (src/HMatrix.cu:57-103) and
+// generateHMatrixFromStruct (src/constructHMatrixFromStruct.cu:13-103).
+//
+// The leaf level is gathered from the Morton-ordered TLR matrix.  Every upper
+// level is recompressed by a batched adaptive randomized approximation whose
+// operator is the implicit sum of the tile's bus x bus TLR sub-tiles
+// (Y = sum_s U_s (V_s^T Omega), B = sum_s V_s (U_s^T Q)), i.e. the operator of
+// lr_kblas_ara_batch.  The ARA is the same blocked algorithm as the TLR kernel
+// (and the CPU oracle), split into one launch per step over all tiles of a
+// level: sample, project (x2), Gram, Cholesky with column skipping, triangular
+// solve, project, Gram, Cholesky, triangular solve, commit.
+#include <algorithm>
+
+#include "hmx_device.h"
+#include "hmx_internal.h"
+
+namespace hmx {
+
+double ara_threshold(double tol);
+
+constexpr int kB16 = 16;
+
+struct LrState {
+    int k, small_run, converged, capped, na;
+    int acc[kB16];
+};
+
+struct LrLevel {
+    int T, m, b, bus, maxRank, r;
+    double tau;
+    uint64_t seed;
+    int level;
+    const int64_t* first;   // first Morton sub-tile of each tile
+    const int32_t* ranksMO;
+    const int64_t* offMO;
+    const double* UMO;
+    const double* VMO;
+    LrState* st;
+    double *Q, *Y, *Ya, *Om, *W, *G, *R;
+    int capQ;
+};
+
+__device__ inline bool lr_active(const LrState& s, int maxRank) {
+    return !(s.converged || s.capped || s.k >= maxRank);
+}
+
+__global__ void k_lr_init(int T, LrState* st) {
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= T) return;
+    st[t].k = 0; st[t].small_run = 0; st[t].converged = 0; st[t].capped = 0; st[t].na = 0;
+}
+
+// Om[t][s*m + c]
+__global__ void __launch_bounds__(256) k_lr_omega(LrLevel L, int it) {
+    int t = blockIdx.y;
+    if (!lr_active(L.st[t], L.maxRank)) return;
+    int e = blockIdx.x * 256 + threadIdx.x;
+    if (e >= L.m * kB16) return;
+    int s = e / L.m, c = e % L.m;
+    uint64_t opid = ((uint64_t)(L.level + 1) << 40) | (uint64_t)t;
+    L.Om[(size_t)t * L.m * kB16 + e] = gauss_at(L.seed, kTagOmega, opid, (uint32_t)(it * kB16 + s), (uint32_t)c);
+}
+
+// Y rows of row-leaf i of tile t: sum over column leaves j of U_ij (V_ij^T Om_j)
+template <int BMAX>
+__global__ void __launch_bounds__(256) k_lr_sample(LrLevel L) {
+    __shared__ double sV[BMAX * BMAX], sU[BMAX * BMAX], sO[BMAX * kB16], sT[BMAX * kB16];
+    const int t = blockIdx.y, i = blockIdx.x, b = L.b, m = L.m;
+    if (!lr_active(L.st[t], L.maxRank)) return;
+    const double* Om = L.Om + (size_t)t * m * kB16;
+    double acc[(BMAX * kB16 + 255) / 256];
+#pragma unroll
+    for (int q = 0; q < (BMAX * kB16 + 255) / 256; ++q) acc[q] = 0.0;
+    for (int j = 0; j < L.bus; ++j) {
+        int64_t mo = L.first[t] + (int64_t)morton_xy((uint32_t)j, (uint32_t)i);
+        int rk = L.ranksMO[mo];
+        if (rk == 0) continue;
+        const double* U = L.UMO + (size_t)L.offMO[mo] * b;
+        const double* V = L.VMO + (size_t)L.offMO[mo] * b;
+        for (int e = threadIdx.x; e < rk * b; e += 256) { sU[e] = U[e]; sV[e] = V[e]; }
+        for (int e = threadIdx.x; e < b * kB16; e += 256) {
+            int s = e / b, x = e % b;
+            sO[e] = Om[(size_t)s * m + (size_t)j * b + x];
+        }
+        __syncthreads();
+        for (int e = threadIdx.x; e < rk * kB16; e += 256) {  // sT[s*rk + a] = V_a . Om_s
+            int s = e / rk, a = e % rk;
+            double v = 0.0;
+            for (int x = 0; x < b; ++x) v += sV[a * b + x] * sO[s * b + x];
+            sT[e] = v;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < (BMAX * kB16 + 255) / 256; ++q) {
+            int e = threadIdx.x + q * 256;
+            if (e < b * kB16) {
+                int s = e / b, x = e % b;
+                double v = acc[q];
+                for (int a = 0; a < rk; ++a) v += sU[a * b + x] * sT[s * rk + a];
+                acc[q] = v;
+            }
+        }
+        __syncthreads();
+    }
+    double* Y = L.Y + (size_t)t * m * kB16;
+#pragma unroll
+    for (int q = 0; q < (BMAX * kB16 + 255) / 256; ++q) {
+        int e = threadIdx.x + q * 256;
+        if (e < b * kB16) {
+            int s = e / b, x = e % b;
+            Y[(size_t)s * m + (size_t)i * b + x] = acc[q];
+        }
+    }
+}
+
+// out[t][s*K + a] = sum_i A_t[a*m+i] B_t[s*m+i], for a < K_t, s < C_t.
+// mode 0: A=Q,K=k / B=Y,C=16 -> W ; mode 1: A=B=Y, K=C=16 -> G
+// mode 2: A=Q,K=k / B=Ya,C=na -> W ; mode 3: A=B=Ya, K=C=na -> G
+__global__ void __launch_bounds__(256) k_lr_dot(LrLevel L, int mode) {
+    __shared__ double sA[16][65], sB[16][65];
+    const int t = blockIdx.y;
+    const LrState& S = L.st[t];
+    // before the Cholesky step the tile must still be iterating; after it,
+    // accepted columns are appended even when the tile converged in this block
+    if (mode <= 1 ? !lr_active(S, L.maxRank) : S.na == 0) return;
+    const int K = (mode == 0 || mode == 2) ? S.k : (mode == 1 ? kB16 : S.na);
+    const int C = (mode <= 1) ? kB16 : S.na;
+    const int a0 = blockIdx.x * 16;
+    if (a0 >= K) return;
+    const int m = L.m;
+    const double* A = (mode == 0 || mode == 2) ? L.Q + (size_t)t * m * L.capQ
+                                               : (mode == 1 ? L.Y + (size_t)t * m * kB16 : L.Ya + (size_t)t * m * kB16);
+    const double* Bm = (mode <= 1) ? L.Y + (size_t)t * m * kB16 : L.Ya + (size_t)t * m * kB16;
+    double* out = (mode == 0 || mode == 2) ? L.W + (size_t)t * L.capQ * kB16 : L.G + (size_t)t * kB16 * kB16;
+    const int ta = threadIdx.x & 15, ts = threadIdx.x >> 4;
+    double acc = 0.0;
+    for (int i0 = 0; i0 < m; i0 += 64) {
+        for (int e = threadIdx.x; e < 16 * 64; e += 256) {
+            int c = e >> 6, x = e & 63;
+            int i = i0 + x;
+            sA[c][x] = (a0 + c < K && i < m) ? A[(size_t)(a0 + c) * m + i] : 0.0;
+            sB[c][x] = (c < C && i < m) ? Bm[(size_t)c * m + i] : 0.0;
+        }
+        __syncthreads();
+        int lim = min(64, m - i0);
+        for (int x = 0; x < lim; ++x) acc += sA[ta][x] * sB[ts][x];
+        __syncthreads();
+    }
+    if (a0 + ta < K && ts < C) out[ts * K + a0 + ta] = acc;
+}
+
+// Y[s*m+i] -= sum_a Q[a*m+i] W[s*k+a]  (mode 0: Y, 16 cols; mode 2: Ya, na cols)
+__global__ void __launch_bounds__(256) k_lr_update(LrLevel L, int mode) {
+    __shared__ double sW[64 * kB16];
+    const int t = blockIdx.y;
+    const LrState& S = L.st[t];
+    if (mode == 0 ? !lr_active(S, L.maxRank) : S.na == 0) return;
+    const int k = S.k;
+    const int C = mode == 0 ? kB16 : S.na;
+    if (k == 0 || C == 0) return;
+    const int m = L.m;
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    double* Y = (mode == 0 ? L.Y : L.Ya) + (size_t)t * m * kB16;
+    const double* Q = L.Q + (size_t)t * m * L.capQ;
+    const double* W = L.W + (size_t)t * L.capQ * kB16;
+    double acc[kB16];
+#pragma unroll
+    for (int s = 0; s < kB16; ++s) acc[s] = (s < C && i < m) ? Y[(size_t)s * m + i] : 0.0;
+    for (int a0 = 0; a0 < k; a0 += 64) {
+        int na = min(64, k - a0);
+        __syncthreads();
+        for (int e = threadIdx.x; e < na * C; e += 256) {
+            int s = e / na, a = e % na;
+            sW[a * kB16 + s] = W[s * k + a0 + a];
+        }
+        __syncthreads();
+        if (i < m) {
+            for (int a = 0; a < na; ++a) {
+                double q = Q[(size_t)(a0 + a) * m + i];
+#pragma unroll
+                for (int s = 0; s < kB16; ++s)
+                    if (s < C) acc[s] -= q * sW[a * kB16 + s];
+            }
+        }
+    }
+    if (i < m)
+#pragma unroll
+        for (int s = 0; s < kB16; ++s)
+            if (s < C) Y[(size_t)s * m + i] = acc[s];
+}
+
+__global__ void k_lr_chol_skip(LrLevel L) {
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= L.T) return;
+    LrState& S = L.st[t];
+    S.na = 0;
+    if (!lr_active(S, L.maxRank)) return;
+    const double* G = L.G + (size_t)t * kB16 * kB16;
+    double* R = L.R + (size_t)t * kB16 * kB16;
+    for (int q = 0; q < kB16 * kB16; ++q) R[q] = 0.0;
+    int na = 0, capleft = L.maxRank - S.k;
+    for (int j = 0; j < kB16; ++j) {
+        double s = G[j * kB16 + j];
+        for (int a = 0; a < na; ++a) {
+            double x = R[S.acc[a] * kB16 + j];
+            s -= x * x;
+        }
+        double nu = s > 0 ? sqrt(s) : 0.0;
+        if (nu > L.tau) {
+            if (na >= capleft) { S.capped = 1; break; }
+            R[j * kB16 + j] = nu;
+            for (int l = j + 1; l < kB16; ++l) {
+                double x = G[j * kB16 + l];
+                for (int a = 0; a < na; ++a) x -= R[S.acc[a] * kB16 + j] * R[S.acc[a] * kB16 + l];
+                R[j * kB16 + l] = x / nu;
+            }
+            S.acc[na++] = j;
+            S.small_run = 0;
+        } else {
+            if (++S.small_run >= L.r) { S.converged = 1; break; }
+        }
+    }
+    S.na = na;
+}
+
+// Ya = Y[:, acc] R_acc^{-1}
+__global__ void __launch_bounds__(256) k_lr_trsm1(LrLevel L) {
+    const int t = blockIdx.y;
+    const LrState& S = L.st[t];
+    if (S.na == 0) return;  // na is reset for inactive tiles
+    const int m = L.m, i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= m) return;
+    const double* Y = L.Y + (size_t)t * m * kB16;
+    double* Ya = L.Ya + (size_t)t * m * kB16;
+    const double* R = L.R + (size_t)t * kB16 * kB16;
+    double ya[kB16];
+    for (int bc = 0; bc < S.na; ++bc) {
+        double x = Y[(size_t)S.acc[bc] * m + i];
+        for (int a = 0; a < bc; ++a) x -= ya[a] * R[S.acc[a] * kB16 + S.acc[bc]];
+        ya[bc] = x / R[S.acc[bc] * kB16 + S.acc[bc]];
+        Ya[(size_t)bc * m + i] = ya[bc];
+    }
+}
+
+__global__ void k_lr_chol_plain(LrLevel L) {
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= L.T) return;
+    const LrState& S = L.st[t];
+    const int na = S.na;
+    if (na == 0) return;
+    const double* G = L.G + (size_t)t * kB16 * kB16;
+    double* R = L.R + (size_t)t * kB16 * kB16;
+    for (int j = 0; j < na; ++j) {
+        double s = G[j * na + j];
+        for (int a = 0; a < j; ++a) s -= R[a * na + j] * R[a * na + j];
+        double d2 = s > 0 ? sqrt(s) : 1e-300;
+        R[j * na + j] = d2;
+        for (int l = j + 1; l < na; ++l) {
+            double x = G[j * na + l];
+            for (int a = 0; a < j; ++a) x -= R[a * na + j] * R[a * na + l];
+            R[j * na + l] = x / d2;
+        }
+    }
+}
+
+// Q[:, k+bc] = Ya R2^{-1}
+__global__ void __launch_bounds__(256) k_lr_trsm2(LrLevel L) {
+    const int t = blockIdx.y;
+    const LrState& S = L.st[t];
+    const int na = S.na;
+    if (na == 0) return;
+    const int m = L.m, i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= m) return;
+    const double* Ya = L.Ya + (size_t)t * m * kB16;
+    double* Q = L.Q + (size_t)t * m * L.capQ;
+    const double* R = L.R + (size_t)t * kB16 * kB16;
+    double q[kB16];
+    for (int bc = 0; bc < na; ++bc) {
+        double x = Ya[(size_t)bc * m + i];
+        for (int a = 0; a < bc; ++a) x -= q[a] * R[a * na + bc];
+        q[bc] = x / R[bc * na + bc];
+        Q[(size_t)(S.k + bc) * m + i] = q[bc];
+    }
+}
+
+// k += na; report active count and max k
+__global__ void k_lr_commit(LrLevel L, int* flags) {
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= L.T) return;
+    LrState& S = L.st[t];
+    S.k += S.na;
+    S.na = 0;
+    if (!S.converged && S.k >= L.maxRank) S.capped = 1;
+    if (!(S.converged || S.capped)) atomicAdd(&flags[0], 1);
+    atomicMax(&flags[1], S.k);
+}
+
+// B[t][a*m + j*b + x] = sum_i V_ij (U_ij^T Q_i), 64 Q columns per block
+template <int BMAX>
+__global__ void __launch_bounds__(256) k_lr_sampleT(LrLevel L, double* Bout) {
+    __shared__ double sV[BMAX * BMAX], sU[BMAX * BMAX], sQ[BMAX * 64], sT[BMAX * 64];
+    const int t = blockIdx.z, j = blockIdx.y, c0 = blockIdx.x * 64, b = L.b, m = L.m;
+    const int k = L.st[t].k;
+    if (c0 >= k) return;
+    const int nc = min(64, k - c0);
+    const double* Q = L.Q + (size_t)t * m * L.capQ;
+    constexpr int NACC = (BMAX * 64 + 255) / 256;
+    double acc[NACC];
+#pragma unroll
+    for (int q = 0; q < NACC; ++q) acc[q] = 0.0;
+    for (int i = 0; i < L.bus; ++i) {
+        int64_t mo = L.first[t] + (int64_t)morton_xy((uint32_t)j, (uint32_t)i);
+        int rk = L.ranksMO[mo];
+        if (rk == 0) continue;
+        const double* U = L.UMO + (size_t)L.offMO[mo] * b;
+        const double* V = L.VMO + (size_t)L.offMO[mo] * b;
+        for (int e = threadIdx.x; e < rk * b; e += 256) { sU[e] = U[e]; sV[e] = V[e]; }
+        for (int e = threadIdx.x; e < b * nc; e += 256) {
+            int cc = e / b, x = e % b;
+            sQ[e] = Q[(size_t)(c0 + cc) * m + (size_t)i * b + x];
+        }
+        __syncthreads();
+        for (int e = threadIdx.x; e < rk * nc; e += 256) {  // sT[cc*rk + a] = U_a . Q_cc
+            int cc = e / rk, a = e % rk;
+            double v = 0.0;
+            for (int x = 0; x < b; ++x) v += sU[a * b + x] * sQ[cc * b + x];
+            sT[e] = v;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < NACC; ++q) {
+            int e = threadIdx.x + q * 256;
+            if (e < b * nc) {
+                int cc = e / b, x = e % b;
+                double v = acc[q];
+                for (int a = 0; a < rk; ++a) v += sV[a * b + x] * sT[cc * rk + a];
+                acc[q] = v;
+            }
+        }
+        __syncthreads();
+    }
+    double* B = Bout + (size_t)t * m * L.capQ;
+#pragma unroll
+    for (int q = 0; q < NACC; ++q) {
+        int e = threadIdx.x + q * 256;
+        if (e < b * nc) {
+            int cc = e / b, x = e % b;
+            B[(size_t)(c0 + cc) * m + (size_t)j * b + x] = acc[q];
+        }
+    }
+}
+
+// grow Q capacity: copy each tile's first k columns
+__global__ void __launch_bounds__(256) k_lr_regrow(int T, int m, int oldCap, int newCap, const LrState* st,
+                                                   const double* __restrict__ Qold, double* __restrict__ Qnew) {
+    const int t = blockIdx.y;
+    const size_t cnt = (size_t)st[t].k * m;
+    const double* s = Qold + (size_t)t * m * oldCap;
+    double* d = Qnew + (size_t)t * m * newCap;
+    for (size_t e = (size_t)blockIdx.x * 256 + threadIdx.x; e < cnt; e += (size_t)gridDim.x * 256) d[e] = s[e];
+}
+
+// pack tile factors: dst[off[t]*m ...] = src[t*m*cap ...] (k_t*m elements)
+__global__ void __launch_bounds__(256) k_lr_pack(int T, int m, int cap, const LrState* st,
+                                                 const int64_t* __restrict__ off, const double* __restrict__ src,
+                                                 double* __restrict__ dst) {
+    const int t = blockIdx.y;
+    const size_t cnt = (size_t)st[t].k * m;
+    const double* s = src + (size_t)t * m * cap;
+    double* d = dst + (size_t)off[t] * m;
+    for (size_t e = (size_t)blockIdx.x * 256 + threadIdx.x; e < cnt; e += (size_t)gridDim.x * 256) d[e] = s[e];
+}
+
+// leaf level: gather TLR tiles by Morton tile index
+__global__ void __launch_bounds__(256) k_leaf_gather(int T, int b, const int32_t* __restrict__ tiles,
+                                                     const int32_t* __restrict__ ranksMO,
+                                                     const int64_t* __restrict__ offMO, const int64_t* __restrict__ off,
+                                                     const double* __restrict__ UMO, const double* __restrict__ VMO,
+                                                     double* __restrict__ U, double* __restrict__ V) {
+    int t = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (t >= T) return;
+    int lane = threadIdx.x & 63;
+    int mo = tiles[t];
+    size_t cnt = (size_t)ranksMO[mo] * b;
+    const double* us = UMO + (size_t)offMO[mo] * b;
+    const double* vs = VMO + (size_t)offMO[mo] * b;
+    for (size_t e = lane; e < cnt; e += 64) {
+        U[(size_t)off[t] * b + e] = us[e];
+        V[(size_t)off[t] * b + e] = vs[e];
+    }
+}
+
+__global__ void k_leaf_ranks(int T, const int32_t* tiles, const int32_t* ranksMO, int32_t* out) {
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < T) out[t] = ranksMO[tiles[t]];
+}
+
+static void compress_level(Context& c, const hmx_config& cfg, const TLR& mo, const std::vector<int32_t>& tiles,
+                           int level, int numLevels, float tol, int maxRank, HLevel& out) {
+    const int b = (int)mo.b;
+    const int bus = 1 << (numLevels - (level + 1));
+    const int m = bus * b;
+    const int T = (int)tiles.size();
+    out.level = level;
+    out.numTiles = T;
+    out.m = m;
+    out.maxRank = maxRank;
+    out.tol = tol;
+    out.h_ranks.assign(T, 0);
+    out.h_offsets.assign(T + 1, 0);
+    if (T == 0) return;
+    std::vector<int64_t> first(T);
+    for (int t = 0; t < T; ++t) first[t] = (int64_t)tiles[t] * bus * bus;
+    DBuf<int64_t> d_first(T);
+    HMX_HIP(hipMemcpyAsync(d_first.p, first.data(), T * sizeof(int64_t), hipMemcpyHostToDevice, c.stream));
+    DBuf<LrState> st(T);
+    int capQ = std::min(maxRank, 128);
+    capQ = std::max(capQ, std::min(maxRank, kB16));
+    DBuf<double> Q((size_t)T * m * capQ), W((size_t)T * capQ * kB16);
+    DBuf<double> Y((size_t)T * m * kB16), Ya((size_t)T * m * kB16), Om((size_t)T * m * kB16);
+    DBuf<double> G((size_t)T * kB16 * kB16), R((size_t)T * kB16 * kB16);
+    DBuf<int> flags(2);
+    LrLevel L;
+    L.T = T; L.m = m; L.b = b; L.bus = bus; L.maxRank = maxRank; L.r = cfg.araR;
+    L.tau = ara_threshold((double)tol);
+    L.seed = cfg.seed; L.level = level;
+    L.first = d_first.p; L.ranksMO = mo.ranks.p; L.offMO = mo.offsets.p; L.UMO = mo.U.p; L.VMO = mo.V.p;
+    L.st = st.p; L.Q = Q.p; L.Y = Y.p; L.Ya = Ya.p; L.Om = Om.p; L.W = W.p; L.G = G.p; L.R = R.p; L.capQ = capQ;
+    k_lr_init<<<(T + 255) / 256, 256, 0, c.stream>>>(T, st.p);
+    HMX_LAUNCHED();
+    const unsigned rowBlocks = (unsigned)((m + 255) / 256);
+    int kmax = 0;
+    const int maxIters = (2 * maxRank + cfg.araR) / kB16 + 2;
+    for (int it = 0;; ++it) {
+        HMX_CHECK(it <= maxIters, HMX_EDEVICE, "batched ARA did not converge (non-finite input?)");
+        if (kmax + kB16 > capQ && capQ < maxRank) {  // grow the basis storage
+            int nc = std::min(maxRank, std::max(2 * capQ, kmax + kB16));
+            DBuf<double> Q2((size_t)T * m * nc), W2((size_t)T * nc * kB16);
+            k_lr_regrow<<<dim3(std::min<unsigned>(1024, (unsigned)(((size_t)m * kmax + 255) / 256 + 1)), T), 256, 0,
+                          c.stream>>>(T, m, capQ, nc, st.p, Q.p, Q2.p);
+            HMX_LAUNCHED();
+            HMX_HIP(hipStreamSynchronize(c.stream));
+            Q = std::move(Q2);
+            W = std::move(W2);
+            capQ = nc;
+            L.Q = Q.p; L.W = W.p; L.capQ = capQ;
+        }
+        k_lr_omega<<<dim3((m * kB16 + 255) / 256, T), 256, 0, c.stream>>>(L, it);
+        HMX_LAUNCHED();
+        k_lr_sample<32><<<dim3(bus, T), 256, 0, c.stream>>>(L);
+        HMX_LAUNCHED();
+        const unsigned kb = (unsigned)std::max(1, (kmax + 15) / 16);
+        for (int pass = 0; pass < 2 && kmax > 0; ++pass) {
+            k_lr_dot<<<dim3(kb, T), 256, 0, c.stream>>>(L, 0);
+            HMX_LAUNCHED();
+            k_lr_update<<<dim3(rowBlocks, T), 256, 0, c.stream>>>(L, 0);
+            HMX_LAUNCHED();
+        }
+        k_lr_dot<<<dim3(1, T), 256, 0, c.stream>>>(L, 1);
+        HMX_LAUNCHED();
+        k_lr_chol_skip<<<(T + 63) / 64, 64, 0, c.stream>>>(L);
+        HMX_LAUNCHED();
+        k_lr_trsm1<<<dim3(rowBlocks, T), 256, 0, c.stream>>>(L);
+        HMX_LAUNCHED();
+        if (kmax > 0) {
+            k_lr_dot<<<dim3(kb, T), 256, 0, c.stream>>>(L, 2);
+            HMX_LAUNCHED();
+            k_lr_update<<<dim3(rowBlocks, T), 256, 0, c.stream>>>(L, 2);
+            HMX_LAUNCHED();
+        }
+        k_lr_dot<<<dim3(1, T), 256, 0, c.stream>>>(L, 3);
+        HMX_LAUNCHED();
+        k_lr_chol_plain<<<(T + 63) / 64, 64, 0, c.stream>>>(L);
+        HMX_LAUNCHED();
+        k_lr_trsm2<<<dim3(rowBlocks, T), 256, 0, c.stream>>>(L);
+        HMX_LAUNCHED();
+        HMX_HIP(hipMemsetAsync(flags.p, 0, 2 * sizeof(int), c.stream));
+        k_lr_commit<<<(T + 255) / 256, 256, 0, c.stream>>>(L, flags.p);
+        HMX_LAUNCHED();
+        int hf[2];
+        HMX_HIP(hipMemcpyAsync(hf, flags.p, sizeof(hf), hipMemcpyDeviceToHost, c.stream));
+        HMX_HIP(hipStreamSynchronize(c.stream));
+        kmax = hf[1];
+        if (hf[0] == 0) break;
+    }
+    // B = A^T Q into the Ya-sized buffer reuse: allocate T*m*capQ
+    DBuf<double> Bf((size_t)T * m * capQ);
+    if (kmax > 0) {
+        dim3 g((unsigned)((kmax + 63) / 64), (unsigned)bus, (unsigned)T);
+        k_lr_sampleT<32><<<g, 256, 0, c.stream>>>(L, Bf.p);
+        HMX_LAUNCHED();
+    }
+    std::vector<LrState> hst(T);
+    HMX_HIP(hipMemcpyAsync(hst.data(), st.p, T * sizeof(LrState), hipMemcpyDeviceToHost, c.stream));
+    HMX_HIP(hipStreamSynchronize(c.stream));
+    int64_t acc = 0;
+    int caps = 0;
+    for (int t = 0; t < T; ++t) {
+        out.h_ranks[t] = hst[t].k;
+        out.h_offsets[t] = acc;
+        acc += hst[t].k;
+        caps += hst[t].capped;
+    }
+    out.h_offsets[T] = acc;
+    out.rankSum = acc;
+    out.capHits = caps;
+    out.U.alloc((size_t)acc * m);
+    out.V.alloc((size_t)acc * m);
+    if (acc > 0) {
+        DBuf<int64_t> d_off(T + 1);
+        HMX_HIP(hipMemcpyAsync(d_off.p, out.h_offsets.data(), (T + 1) * sizeof(int64_t), hipMemcpyHostToDevice,
+                               c.stream));
+        dim3 g(std::min<unsigned>(1024, (unsigned)(((size_t)m * kmax + 255) / 256 + 1)), T);
+        k_lr_pack<<<g, 256, 0, c.stream>>>(T, m, capQ, st.p, d_off.p, Q.p, out.U.p);
+        HMX_LAUNCHED();
+        k_lr_pack<<<g, 256, 0, c.stream>>>(T, m, capQ, st.p, d_off.p, Bf.p, out.V.p);
+        HMX_LAUNCHED();
+        HMX_HIP(hipStreamSynchronize(c.stream));
+    }
+}
+
+void build_hmatrix(Context& c, const hmx_config& cfg, const KDTree& t, const Structure& s, const TLR& mo,
+                   HMatrix& h) {
+    HMX_CHECK(mo.ordering == HMX_MORTON, HMX_ESTATE, "generateHMatrixFromStruct: TLR matrix must be Morton-ordered");
+    HMX_CHECK(mo.S == t.numSegments && mo.b == t.b, HMX_EINVAL, "TLR matrix and KD-tree do not match");
+    HMX_CHECK(s.numLevels >= 2 && (uint32_t)(1u << (s.numLevels - 1)) == t.numSegments, HMX_EINVAL,
+              "structure and KD-tree do not match");
+    HMX_CHECK(cfg.araBlockSize == kB16, HMX_EUNSUPPORTED, "ARA block size must be 16");
+    HMX_CHECK(t.b <= 32, HMX_EUNSUPPORTED, "hierarchical recompression: bucketSize must be <= 32");
+    const int L = s.numLevels, b = (int)t.b;
+    h.n = t.n; h.d = t.d; h.b = t.b; h.S = t.numSegments; h.numLevels = L;
+    h.structureCopy = s;
+    h.structure = &h.structureCopy;
+    h.levels.clear();
+    h.levels.resize(L - 1);
+    h.diag.alloc(mo.diagonal.n);
+    HMX_HIP(hipMemcpyAsync(h.diag.p, mo.diagonal.p, mo.diagonal.bytes(), hipMemcpyDeviceToDevice, c.stream));
+    // leaf level (allocateHMatrix, src/HMatrix.cu:57-103)
+    {
+        HLevel& lv = h.levels[L - 2];
+        const std::vector<int32_t>& tiles = s.tileIndices[L - 2];
+        const int T = (int)tiles.size();
+        lv.level = L - 1; lv.numTiles = T; lv.m = b;
+        lv.maxRank = cfg.maxRankPolicy == HMX_MAXRANK_REF ? b / 2 : b;
+        lv.tol = cfg.lowestLevelTolerance;
+        lv.h_ranks.assign(T, 0);
+        lv.h_offsets.assign(T + 1, 0);
+        if (T > 0) {
+            DBuf<int32_t> d_tiles(T), d_r(T);
+            DBuf<int64_t> d_off(T + 1);
+            HMX_HIP(hipMemcpyAsync(d_tiles.p, tiles.data(), T * sizeof(int32_t), hipMemcpyHostToDevice, c.stream));
+            k_leaf_ranks<<<(T + 255) / 256, 256, 0, c.stream>>>(T, d_tiles.p, mo.ranks.p, d_r.p);
+            HMX_LAUNCHED();
+            HMX_HIP(hipMemcpyAsync(lv.h_ranks.data(), d_r.p, T * sizeof(int32_t), hipMemcpyDeviceToHost, c.stream));
+            HMX_HIP(hipStreamSynchronize(c.stream));
+            int64_t acc = 0;
+            for (int q = 0; q < T; ++q) { lv.h_offsets[q] = acc; acc += lv.h_ranks[q]; }
+            lv.h_offsets[T] = acc;
+            lv.rankSum = acc;
+            lv.U.alloc((size_t)acc * b);
+            lv.V.alloc((size_t)acc * b);
+            HMX_HIP(hipMemcpyAsync(d_off.p, lv.h_offsets.data(), (T + 1) * sizeof(int64_t), hipMemcpyHostToDevice,
+                                   c.stream));
+            if (acc > 0) {
+                k_leaf_gather<<<(T + 3) / 4, 256, 0, c.stream>>>(T, b, d_tiles.p, mo.ranks.p, mo.offsets.p, d_off.p,
+                                                                 mo.U.p, mo.V.p, lv.U.p, lv.V.p);
+                HMX_LAUNCHED();
+            }
+            HMX_HIP(hipStreamSynchronize(c.stream));
+        }
+    }
+    // upper levels (generateHMatrixFromStruct): tolerance doubles per
+    // non-empty level (src/constructHMatrixFromStruct.cu:54), rank cap from
+    // generateMaxRanks (src/helperFunctions.cu:67-74)
+    std::vector<uint32_t> maxRanks(L > 2 ? L - 2 : 1, 0);
+    hmx_generateMaxRanks((uint32_t)L, (uint32_t)b, maxRanks.data());
+    float tol = cfg.lowestLevelTolerance;
+    for (int level = L - 2; level > 0; --level) {
+        HLevel& lv = h.levels[level - 1];
+        const int T = s.numTiles[level - 1];
+        const int m = (1 << (L - (level + 1))) * b;
+        if (T == 0) {
+            lv.level = level; lv.numTiles = 0; lv.m = m;
+            lv.h_ranks.clear(); lv.h_offsets.assign(1, 0);
+            continue;
+        }
+        tol *= 2;
+        int maxRank = cfg.maxRankPolicy == HMX_MAXRANK_REF ? (int)maxRanks[L - level - 2] : m;
+        maxRank = std::min(maxRank, m);
+        compress_level(c, cfg, mo, s.tileIndices[level - 1], level, L, tol, maxRank, lv);
+    }
+    // bytes accounting
+    h.lowRankBytes = 0;
+    h.factorEntries = 0;
+    for (auto& lv : h.levels) {
+        h.lowRankBytes += 2ull * (uint64_t)lv.rankSum * lv.m * sizeof(double);
+        h.factorEntries += 2ull * (uint64_t)lv.rankSum * lv.m;
+    }
+    h.denseBytes = (uint64_t)h.S * b * b * sizeof(double);
+    h.factorEntries += (uint64_t)h.S * b * b;
+    // the matvec plan is built on the first HMatrixVecMult call
+}
+
+}  // namespace hmx

@@ -1,0 +1,95 @@
+// Device-side helpers shared by every gfx950 kernel of the library.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace hmx {
+
+// Philox4x32-10 (Salmon, Moraes, Dror & Shaw, SC'11).  Counter-based, so a
+// value depends only on (counter, key): the CPU restatement draws the same bits.
+constexpr uint32_t kPhiloxM0 = 0xD2511F53u;
+constexpr uint32_t kPhiloxM1 = 0xCD9E8D57u;
+constexpr uint32_t kPhiloxW0 = 0x9E3779B9u;
+constexpr uint32_t kPhiloxW1 = 0xBB67AE85u;
+
+constexpr uint32_t kTagPoints = 0x504F0000u;
+constexpr uint32_t kTagOmega = 0x4F4D0000u;
+constexpr uint32_t kTagRhs = 0x52480000u;
+
+__host__ __device__ inline void philox4x32_10(uint32_t c[4], uint32_t k0, uint32_t k1) {
+#pragma unroll
+    for (int r = 0; r < 10; ++r) {
+        uint64_t p0 = (uint64_t)kPhiloxM0 * c[0];
+        uint64_t p1 = (uint64_t)kPhiloxM1 * c[2];
+        uint32_t n0 = (uint32_t)(p1 >> 32) ^ c[1] ^ k0;
+        uint32_t n1 = (uint32_t)p1;
+        uint32_t n2 = (uint32_t)(p0 >> 32) ^ c[3] ^ k1;
+        uint32_t n3 = (uint32_t)p0;
+        c[0] = n0; c[1] = n1; c[2] = n2; c[3] = n3;
+        k0 += kPhiloxW0; k1 += kPhiloxW1;
+    }
+}
+
+// 53-bit uniform in [0, 1) from two 32-bit words
+__host__ __device__ inline double u53(uint32_t a, uint32_t b) {
+    return (double)(((uint64_t)(a >> 5) << 26) | (uint64_t)(b >> 6)) * 0x1.0p-53;
+}
+
+// N(0,1) from one Philox block (Box-Muller, cosine branch).  (a, b, c) is the
+// logical index: a = operation id (64 bit), b = sample column, c = row.
+__device__ inline double gauss_at(uint64_t seed, uint32_t tag, uint64_t a, uint32_t b, uint32_t c) {
+    uint32_t ctr[4] = {c, b, (uint32_t)a, tag ^ (uint32_t)(a >> 32)};
+    philox4x32_10(ctr, (uint32_t)seed, (uint32_t)(seed >> 32));
+    double u1 = u53(ctr[0], ctr[1]) + 0x1.0p-53;
+    double u2 = u53(ctr[2], ctr[3]);
+    return sqrt(-2.0 * log(u1)) * cos(6.283185307179586 * u2);
+}
+
+// Exponential covariance K(x,y) = exp(-||x-y||/ell), same operation order as
+// interaction() in the reference (src/TLRMatrixHelpers.cu:44-58).
+template <int D>
+__device__ inline double kernel_eval(const double* __restrict__ a, const double* __restrict__ b, double ell) {
+    double diff = 0.0;
+#pragma unroll
+    for (int j = 0; j < D; ++j) {
+        double t = a[j] - b[j];
+        diff = __dadd_rn(diff, __dmul_rn(t, t));
+    }
+    return exp(-sqrt(diff) / ell);
+}
+
+__device__ inline double kernel_eval_d(const double* a, const double* b, int d, double ell) {
+    double diff = 0.0;
+    for (int j = 0; j < d; ++j) {
+        double t = a[j] - b[j];
+        diff = __dadd_rn(diff, __dmul_rn(t, t));
+    }
+    return exp(-sqrt(diff) / ell);
+}
+
+// Morton helpers (src/helperKernels.cuh:87-125): column in even bits, row in odd bits
+__host__ __device__ inline uint32_t spread16(uint32_t x) {
+    x = (x | (x << 8)) & 0x00FF00FFu;
+    x = (x | (x << 4)) & 0x0F0F0F0Fu;
+    x = (x | (x << 2)) & 0x33333333u;
+    x = (x | (x << 1)) & 0x55555555u;
+    return x;
+}
+__host__ __device__ inline uint32_t compact16(uint32_t x) {
+    x &= 0x55555555u;
+    x = (x | (x >> 1)) & 0x33333333u;
+    x = (x | (x >> 2)) & 0x0F0F0F0Fu;
+    x = (x | (x >> 4)) & 0x00FF00FFu;
+    x = (x | (x >> 8)) & 0x0000FFFFu;
+    return x;
+}
+__host__ __device__ inline uint32_t morton_xy(uint32_t col, uint32_t row) { return spread16(col) | (spread16(row) << 1); }
+__host__ __device__ inline uint32_t cm_to_mo(uint32_t S, uint32_t n) { return morton_xy(n / S, n % S); }
+
+// order-preserving map of a double onto uint64 (radix-sort key twiddle)
+__device__ inline uint64_t dkey(double v) {
+    uint64_t u = (uint64_t)__double_as_longlong(v);
+    return (u & 0x8000000000000000ull) ? ~u : (u | 0x8000000000000000ull);
+}
+
+}  // namespace hmx

@@ -1,0 +1,202 @@
+// Host-side internals of the H-matrix library: contexts, device buffers and
+// the objects behind the opaque C-ABI handles.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/hmx.h"
+
+namespace hmx {
+
+// Error carrying an hmx_status; thrown by host code, caught at the C-ABI edge.
+struct Error : std::runtime_error {
+    hmx_status status;
+    Error(hmx_status s, const std::string& m) : std::runtime_error(m), status(s) {}
+};
+
+#define HMX_HIP(call)                                                                                  \
+    do {                                                                                               \
+        hipError_t e_ = (call);                                                                        \
+        if (e_ != hipSuccess)                                                                          \
+            throw ::hmx::Error(e_ == hipErrorOutOfMemory ? HMX_ENOMEM : HMX_EDEVICE,                   \
+                               std::string(#call) + ": " + hipGetErrorString(e_) + " at " + __FILE__ + \
+                                   ":" + std::to_string(__LINE__));                                    \
+    } while (0)
+
+#define HMX_CHECK(cond, status, msg)                              \
+    do {                                                          \
+        if (!(cond)) throw ::hmx::Error((status), std::string(msg)); \
+    } while (0)
+
+// Launch check: catches launch-configuration errors right after a launch.
+#define HMX_LAUNCHED() HMX_HIP(hipGetLastError())
+
+// Owning device buffer (hipMalloc / hipFree), movable, not copyable.
+template <class T>
+struct DBuf {
+    T* p = nullptr;
+    size_t n = 0;
+    DBuf() = default;
+    explicit DBuf(size_t count) { alloc(count); }
+    DBuf(const DBuf&) = delete;
+    DBuf& operator=(const DBuf&) = delete;
+    DBuf(DBuf&& o) noexcept : p(o.p), n(o.n) { o.p = nullptr; o.n = 0; }
+    DBuf& operator=(DBuf&& o) noexcept {
+        if (this != &o) { release(); p = o.p; n = o.n; o.p = nullptr; o.n = 0; }
+        return *this;
+    }
+    ~DBuf() { release(); }
+    void alloc(size_t count) {
+        release();
+        n = count;
+        if (count) HMX_HIP(hipMalloc((void**)&p, count * sizeof(T)));
+    }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+    size_t bytes() const { return n * sizeof(T); }
+};
+
+struct Context {
+    hipStream_t stream = nullptr;
+    int device = 0;
+    std::string last_error;
+};
+
+// KD-tree (src/kDTree.cuh:8-14) with device permutation/offsets and
+// per-level bounding boxes (src/boundingBoxes.h:8-23).
+struct KDTree {
+    uint32_t n = 0, d = 0, b = 0;
+    uint32_t numLevels = 0;    // number of split levels (getMaxSegmentSize().second)
+    uint32_t numSegments = 0;  // 2^numLevels leaves
+    DBuf<int32_t> perm;        // segmentIndices
+    DBuf<int32_t> offsets;     // segmentOffsets, numSegments+1
+    DBuf<double> bbmin, bbmax; // [(2^(numLevels+1)-1)][d]
+    std::vector<int32_t> h_offsets;
+    std::vector<double> h_bbmin, h_bbmax;  // host mirror for the structure pass
+};
+
+// HMatrixStructure (src/HMatrixStructure.cuh:8-13) plus the JDS of each level.
+struct Structure {
+    int32_t numLevels = 0;
+    uint32_t n = 0, b = 0;
+    std::vector<int32_t> numTiles;                 // [numLevels]
+    std::vector<std::vector<int32_t>> tileIndices; // [numLevels]; level l at l-1
+    struct JDS {
+        std::vector<int32_t> rowIndices, columnIndices, tilePos, iterPtr;
+    };
+    std::vector<JDS> jds;  // [numLevels-1]; level l at l-1
+};
+
+// TLR_Matrix (src/TLRMatrix.cuh:10-21) with 64-bit offsets.
+struct TLR {
+    int32_t ordering = HMX_COLUMN_MAJOR;
+    uint32_t S = 0, b = 0, n = 0;
+    uint64_t rankSum = 0, capHits = 0;
+    DBuf<int32_t> ranks;     // blockRanks [S*S]
+    DBuf<int64_t> offsets;   // blockOffsets [S*S+1] (exclusive scan, in columns)
+    DBuf<double> U, V;       // [rankSum*b]
+    DBuf<double> diagonal;   // [S*b*b]
+};
+
+// One level of the hierarchical matrix (src/HMatrix.cuh:9-18).
+struct HLevel {
+    int32_t level = 0, numTiles = 0, m = 0, maxRank = 0, capHits = 0;
+    float tol = 0.f;
+    int64_t rankSum = 0;
+    std::vector<int32_t> h_ranks;
+    std::vector<int64_t> h_offsets;  // exclusive scan over tiles (columns)
+    DBuf<double> U, V;               // [rankSum*m]
+};
+
+// Phase-1 work item: a contiguous run of one tile's input-side factor
+// (V for H*x, U for H^T*x).  colLen == 0: the run is one chunk of a single
+// column (len elements, one partial result); colLen > 0: the run holds
+// len/colLen whole columns of length colLen, one result per column.
+struct MvItem {
+    const double* v;
+    int32_t xoff, len, colLen, out;
+};
+// Phase-2 entry: the leaf's rows of one tile's output-side factor.
+struct MvEntry {
+    const double* u;  // first row of this leaf inside the tile's factor
+    int32_t m;        // column stride (tile dimension)
+    int32_t rank;
+    int32_t zbase;    // first global column (index into z)
+    int32_t pad;
+};
+
+// Device-side tables used by the matvec kernels (built once per H-matrix).
+struct MatvecPlan {
+    DBuf<MvItem> items;
+    int32_t numItems = 0;
+    DBuf<int64_t> colPart;   // per global column: first partial slot
+    DBuf<int32_t> colNpart;  // per global column: number of partial slots
+    int64_t numCols = 0, numPart = 0;
+    DBuf<int32_t> leafPtr;   // [S+1] CSR over leaves
+    DBuf<MvEntry> entries;
+    const double* diag = nullptr;  // D (or D^T for the transpose plan)
+    DBuf<double> diagT;            // owned transposed copy
+    bool built = false;
+    // per-call workspaces (grown on demand)
+    DBuf<double> zpart, z;
+    uint32_t wsK = 0;
+};
+
+struct HMatrix {
+    uint32_t n = 0, d = 0, b = 0, S = 0;
+    int32_t numLevels = 0;
+    DBuf<double> diag;              // [S*b*b]
+    std::vector<HLevel> levels;     // index l-1
+    const Structure* structure = nullptr;
+    Structure structureCopy;        // owned copy (the caller may free theirs)
+    // flattened pools for the matvec (all levels)
+    DBuf<double> Upool, Vpool;
+    std::vector<int64_t> levelPoolOff;  // element offset of each level in the pools
+    MatvecPlan plan, planT;
+    uint64_t lowRankBytes = 0, denseBytes = 0, factorEntries = 0;
+};
+
+// ---- internal entry points (implemented in the .hip/.cpp files) ----------
+void generate_dataset(Context& c, uint32_t n, uint32_t d, uint64_t seed, double* d_points);
+void generate_random_vector(Context& c, uint32_t k, uint32_t n, uint64_t seed, double* d_out);
+void build_kdtree(Context& c, KDTree& t, const double* d_points);
+void build_structure(Structure& s, uint32_t n, uint32_t d, uint32_t b, int adm, float eta, const double* bbmin,
+                     const double* bbmax);
+void build_tlr_cm(Context& c, const hmx_config& cfg, const KDTree& t, const double* d_points, TLR& out);
+void convert_to_morton(Context& c, const TLR& cm, TLR& mo);
+void build_hmatrix(Context& c, const hmx_config& cfg, const KDTree& t, const Structure& s, const TLR& mo, HMatrix& h);
+void build_matvec_plan(Context& c, HMatrix& h, bool transpose);
+void hmatvec(Context& c, HMatrix& h, uint32_t k, const double* d_in, double* d_out, bool transpose, float* ms,
+             int iters);
+void dense_rows(Context& c, uint32_t n, uint32_t d, const double* p, const int32_t* perm, double ell, uint32_t nr,
+                const int32_t* rows, uint32_t k, const double* X, double* Y);
+// device exclusive scan int32 -> int64 (out has count+1 entries)
+void exclusive_scan_i32_i64(Context& c, const int32_t* in, int64_t* out, size_t count);
+
+inline double default_ell(uint32_t d) { return d == 3 ? 0.2 : 0.1; }
+inline int ctz32(uint32_t v) { return __builtin_ctz(v); }
+inline uint32_t upper_pow2(uint32_t v) {
+    v--; v |= v >> 1; v |= v >> 2; v |= v >> 4; v |= v >> 8; v |= v >> 16; v++;
+    return v;
+}
+inline uint32_t kd_num_levels(uint32_t n, uint32_t b) {
+    uint32_t it = 0;
+    while (n > b) { n = (n + 1) / 2; ++it; }
+    return it;
+}
+
+}  // namespace hmx
+
+// opaque C handles are the internal objects
+struct hmx_context : hmx::Context {};
+struct hmx_kdtree : hmx::KDTree {};
+struct hmx_structure : hmx::Structure {};
+struct hmx_tlr : hmx::TLR {};
+struct hmx_hmatrix : hmx::HMatrix {};

@@ -1,0 +1,398 @@
+// TLR compression (createColumnMajorLRMatrix, src/constructTLRMatrix.cu:13-151)
+// and Morton reordering (convertColumnMajorToMorton, src/helperFunctions.cu:8-57).
+//
+// One wave per b x b tile: the tile's points are gathered into LDS, the
+// kernel tile exp(-|x-y|/ell) is evaluated once into LDS (no dense matrix in
+// HBM), and the adaptive randomized approximation runs entirely in LDS:
+//   Y = A*Omega (bs Gaussian samples from Philox), Y -= Q(Q^T Y) twice,
+//   Gram Cholesky with column skipping (accept a column when its residual
+//   norm exceeds tau, stop after r consecutive small ones), Y R^-1, one more
+//   projection + Cholesky-QR, append to Q; finally V = A^T Q.
+// The same algorithm, step for step, is the CPU oracle's ara_run().
+#include <algorithm>
+
+#include "hmx_device.h"
+#include "hmx_internal.h"
+
+namespace hmx {
+
+constexpr int kBS = 16;  // ARA samples per block (kblas_ara_batch bs argument)
+
+struct AraState {
+    int k, small_run, converged, capped, na;
+    int acc[kBS];
+};
+
+// Cholesky of the Gram matrix with column skipping (one lane).
+__device__ inline void chol_skip_lane(const double* G, double* R, AraState* st, double tau, int r, int capleft) {
+    int na = 0;
+    for (int i = 0; i < kBS * kBS; ++i) R[i] = 0.0;
+    for (int j = 0; j < kBS; ++j) {
+        double s = G[j * kBS + j];
+        for (int a = 0; a < na; ++a) {
+            double t = R[st->acc[a] * kBS + j];
+            s -= t * t;
+        }
+        double nu = s > 0 ? sqrt(s) : 0.0;
+        if (nu > tau) {
+            if (na >= capleft) { st->capped = 1; break; }
+            R[j * kBS + j] = nu;
+            for (int l = j + 1; l < kBS; ++l) {
+                double t = G[j * kBS + l];
+                for (int a = 0; a < na; ++a) t -= R[st->acc[a] * kBS + j] * R[st->acc[a] * kBS + l];
+                R[j * kBS + l] = t / nu;
+            }
+            st->acc[na++] = j;
+            st->small_run = 0;
+        } else {
+            if (++st->small_run >= r) { st->converged = 1; break; }
+        }
+    }
+    st->na = na;
+}
+
+// plain Cholesky of an na x na Gram matrix (one lane); R upper, R[a*na+s]
+__device__ inline void chol_plain_lane(const double* G, double* R, int na) {
+    for (int j = 0; j < na; ++j) {
+        double s = G[j * na + j];
+        for (int a = 0; a < j; ++a) s -= R[a * na + j] * R[a * na + j];
+        double d2 = s > 0 ? sqrt(s) : 1e-300;
+        R[j * na + j] = d2;
+        for (int l = j + 1; l < na; ++l) {
+            double t = G[j * na + l];
+            for (int a = 0; a < j; ++a) t -= R[a * na + j] * R[a * na + l];
+            R[j * na + l] = t / d2;
+        }
+    }
+}
+
+template <int B>
+struct TileLds {
+    static constexpr int MR = B;  // rank capacity
+    double A[B * B];
+    double Om[B * kBS];  // Omega, later Ya
+    double Y[B * kBS];
+    double Q[B * MR];
+    double W[MR * kBS];
+    double G[kBS * kBS];
+    double R[kBS * kBS];
+    double xr[3 * B], xc[3 * B];
+    AraState st;
+};
+
+template <int B, int D>
+__global__ void __launch_bounds__(64) k_tlr_ara(uint32_t n, uint32_t S, const double* __restrict__ p,
+                                                const int32_t* __restrict__ perm, const int32_t* __restrict__ offsets,
+                                                double ell, double tau, int r, int maxRank, uint64_t seed, uint64_t t0,
+                                                double* __restrict__ Uslot, double* __restrict__ Vslot,
+                                                int32_t* __restrict__ ranks, int32_t* __restrict__ caps,
+                                                double* __restrict__ diag) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    TileLds<B>& L = *reinterpret_cast<TileLds<B>*>(smem);
+    const int tid = threadIdx.x;
+    const uint64_t t = t0 + blockIdx.x;
+    const uint32_t u = (uint32_t)(t / S), v = (uint32_t)(t % S);
+    const int au = offsets[u], su = offsets[u + 1] - au;
+    const int av = offsets[v], sv = offsets[v + 1] - av;
+    for (int e = tid; e < D * B; e += 64) {
+        int j = e / B, c = e % B;
+        L.xc[c * 3 + j] = c < su ? p[(size_t)j * n + perm[au + c]] : 0.0;
+        L.xr[c * 3 + j] = c < sv ? p[(size_t)j * n + perm[av + c]] : 0.0;
+    }
+    __syncthreads();
+    for (int e = tid; e < B * B; e += 64) {
+        int c = e / B, q = e % B;
+        double val;
+        if (c < su && q < sv) val = kernel_eval<D>(&L.xc[c * 3], &L.xr[q * 3], ell);
+        else val = (u == v && c == q) ? 1.0 : 0.0;
+        L.A[e] = val;
+    }
+    __syncthreads();
+    const size_t slot = (size_t)blockIdx.x;
+    if (u == v) {  // dense diagonal block (generateDenseBlockColumn diagonal branch)
+        for (int e = tid; e < B * B; e += 64) diag[(size_t)u * B * B + e] = L.A[e];
+        if (tid == 0) { ranks[slot] = 0; caps[slot] = 0; }
+        return;
+    }
+    if (tid == 0) { L.st.k = 0; L.st.small_run = 0; L.st.converged = 0; L.st.capped = 0; L.st.na = 0; }
+    __syncthreads();
+    // every block either accepts a column (k grows, bounded by maxRank) or
+    // extends the small-column run (bounded by r), so this bound is never hit
+    // by a finite tile; it only guards against NaN-poisoned input
+    const int maxIters = (2 * maxRank + r) / kBS + 2;
+    int it = 0;
+    while (it < maxIters) {
+        int k = L.st.k;
+        if (L.st.converged || L.st.capped || k >= maxRank) break;
+        // Omega
+        for (int e = tid; e < B * kBS; e += 64) {
+            int s = e / B, c = e % B;
+            L.Om[e] = gauss_at(seed, kTagOmega, t, (uint32_t)(it * kBS + s), (uint32_t)c);
+        }
+        __syncthreads();
+        // Y = A Omega
+        for (int e = tid; e < B * kBS; e += 64) {
+            int s = e / B, q = e % B;
+            double acc = 0.0;
+            for (int c = 0; c < B; ++c) acc += L.A[c * B + q] * L.Om[s * B + c];
+            L.Y[e] = acc;
+        }
+        __syncthreads();
+        for (int pass = 0; pass < 2 && k > 0; ++pass) {
+            for (int e = tid; e < k * kBS; e += 64) {
+                int s = e / k, a = e % k;
+                double acc = 0.0;
+                for (int q = 0; q < B; ++q) acc += L.Q[a * B + q] * L.Y[s * B + q];
+                L.W[s * k + a] = acc;
+            }
+            __syncthreads();
+            for (int e = tid; e < B * kBS; e += 64) {
+                int s = e / B, q = e % B;
+                double acc = L.Y[e];
+                for (int a = 0; a < k; ++a) acc -= L.Q[a * B + q] * L.W[s * k + a];
+                L.Y[e] = acc;
+            }
+            __syncthreads();
+        }
+        for (int e = tid; e < kBS * kBS; e += 64) {
+            int a = e / kBS, s = e % kBS;
+            double acc = 0.0;
+            for (int q = 0; q < B; ++q) acc += L.Y[a * B + q] * L.Y[s * B + q];
+            L.G[e] = acc;
+        }
+        __syncthreads();
+        if (tid == 0) chol_skip_lane(L.G, L.R, &L.st, tau, r, maxRank - k);
+        __syncthreads();
+        const int na = L.st.na;
+        if (na > 0) {
+            double* Ya = L.Om;
+            for (int q = tid; q < B; q += 64)
+                for (int bc = 0; bc < na; ++bc) {
+                    double x = L.Y[L.st.acc[bc] * B + q];
+                    for (int a = 0; a < bc; ++a) x -= Ya[a * B + q] * L.R[L.st.acc[a] * kBS + L.st.acc[bc]];
+                    Ya[bc * B + q] = x / L.R[L.st.acc[bc] * kBS + L.st.acc[bc]];
+                }
+            __syncthreads();
+            if (k > 0) {
+                for (int e = tid; e < k * na; e += 64) {
+                    int s = e / k, a = e % k;
+                    double acc = 0.0;
+                    for (int q = 0; q < B; ++q) acc += L.Q[a * B + q] * Ya[s * B + q];
+                    L.W[s * k + a] = acc;
+                }
+                __syncthreads();
+                for (int e = tid; e < B * na; e += 64) {
+                    int s = e / B, q = e % B;
+                    double acc = Ya[e];
+                    for (int a = 0; a < k; ++a) acc -= L.Q[a * B + q] * L.W[s * k + a];
+                    Ya[e] = acc;
+                }
+                __syncthreads();
+            }
+            for (int e = tid; e < na * na; e += 64) {
+                int a = e / na, s = e % na;
+                double acc = 0.0;
+                for (int q = 0; q < B; ++q) acc += Ya[a * B + q] * Ya[s * B + q];
+                L.G[e] = acc;
+            }
+            __syncthreads();
+            if (tid == 0) chol_plain_lane(L.G, L.R, na);
+            __syncthreads();
+            for (int q = tid; q < B; q += 64)
+                for (int bc = 0; bc < na; ++bc) {
+                    double x = Ya[bc * B + q];
+                    for (int a = 0; a < bc; ++a) x -= L.Q[(k + a) * B + q] * L.R[a * na + bc];
+                    L.Q[(k + bc) * B + q] = x / L.R[bc * na + bc];
+                }
+            __syncthreads();
+            if (tid == 0) L.st.k = k + na;
+        }
+        ++it;
+        __syncthreads();
+    }
+    const int k = L.st.k;
+    double* Us = Uslot + slot * (size_t)B * maxRank;
+    double* Vs = Vslot + slot * (size_t)B * maxRank;
+    for (int e = tid; e < k * B; e += 64) {
+        int a = e / B, c = e % B;
+        double acc = 0.0;
+        for (int q = 0; q < B; ++q) acc += L.A[c * B + q] * L.Q[a * B + q];
+        Vs[e] = acc;
+        Us[e] = L.Q[e];
+    }
+    if (tid == 0) {
+        ranks[slot] = k;
+        caps[slot] = (L.st.capped || (!L.st.converged && k >= maxRank)) ? 1 : 0;
+    }
+}
+
+// copy a batch's slots to packed storage: wave per tile
+__global__ void __launch_bounds__(256) k_compact(uint64_t nt, int b, int slotCols, const int32_t* __restrict__ ranks,
+                                                 const int64_t* __restrict__ off, const double* __restrict__ Us,
+                                                 const double* __restrict__ Vs, double* __restrict__ U,
+                                                 double* __restrict__ V) {
+    uint64_t t = (uint64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (t >= nt) return;
+    int lane = threadIdx.x & 63;
+    size_t cnt = (size_t)ranks[t] * b;
+    const double* us = Us + t * (size_t)b * slotCols;
+    const double* vs = Vs + t * (size_t)b * slotCols;
+    double* ud = U + (size_t)off[t] * b;
+    double* vd = V + (size_t)off[t] * b;
+    for (size_t e = lane; e < cnt; e += 64) {
+        ud[e] = us[e];
+        vd[e] = vs[e];
+    }
+}
+
+template <int B, int D>
+static void launch_tlr(Context& c, uint32_t n, uint32_t S, const double* p, const int32_t* perm, const int32_t* off,
+                       double ell, double tau, int r, int maxRank, uint64_t seed, uint64_t t0, uint64_t nt,
+                       double* Us, double* Vs, int32_t* ranks, int32_t* caps, double* diag) {
+    size_t lds = sizeof(TileLds<B>);
+    auto kern = k_tlr_ara<B, D>;
+    if (lds > 64 * 1024) HMX_HIP(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    kern<<<(unsigned)nt, 64, lds, c.stream>>>(n, S, p, perm, off, ell, tau, r, maxRank, seed, t0, Us, Vs, ranks,
+                                              caps, diag);
+    HMX_LAUNCHED();
+}
+
+template <int B>
+static void launch_tlr_d(Context& c, uint32_t d, uint32_t n, uint32_t S, const double* p, const int32_t* perm,
+                         const int32_t* off, double ell, double tau, int r, int maxRank, uint64_t seed, uint64_t t0,
+                         uint64_t nt, double* Us, double* Vs, int32_t* ranks, int32_t* caps, double* diag) {
+    if (d == 1) launch_tlr<B, 1>(c, n, S, p, perm, off, ell, tau, r, maxRank, seed, t0, nt, Us, Vs, ranks, caps, diag);
+    else if (d == 2) launch_tlr<B, 2>(c, n, S, p, perm, off, ell, tau, r, maxRank, seed, t0, nt, Us, Vs, ranks, caps, diag);
+    else launch_tlr<B, 3>(c, n, S, p, perm, off, ell, tau, r, maxRank, seed, t0, nt, Us, Vs, ranks, caps, diag);
+}
+
+double ara_threshold(double tol) { return tol / (10.0 * std::sqrt(2.0 / 3.141592653589793)); }
+
+void build_tlr_cm(Context& c, const hmx_config& cfg, const KDTree& t, const double* d_points, TLR& out) {
+    const uint32_t b = t.b, S = t.numSegments, n = t.n, d = t.d;
+    HMX_CHECK(b == 8 || b == 16 || b == 32 || b == 64, HMX_EUNSUPPORTED,
+              "TLR tiles: bucketSize must be 8, 16, 32 or 64");
+    HMX_CHECK(cfg.araBlockSize == kBS, HMX_EUNSUPPORTED, "ARA block size must be 16");
+    HMX_CHECK(cfg.araR >= 1, HMX_EINVAL, "araR must be >= 1");
+    HMX_CHECK(S >= 2, HMX_EINVAL, "TLR needs at least two segments (n > bucketSize)");
+    const int maxRank = cfg.maxRankPolicy == HMX_MAXRANK_REF ? (int)b / 2 : (int)b;
+    const double ell = cfg.correlationLength > 0 ? cfg.correlationLength : default_ell(d);
+    const double tau = ara_threshold((double)cfg.lowestLevelTolerance);
+    out.ordering = HMX_COLUMN_MAJOR;
+    out.S = S;
+    out.b = b;
+    out.n = n;
+    const uint64_t total = (uint64_t)S * S;
+    out.ranks.alloc(total);
+    out.offsets.alloc(total + 1);
+    out.diagonal.alloc((size_t)S * b * b);
+
+    // batches of tiles sized so the per-tile slots stay under ~2 GiB
+    const size_t slotBytes = 2 * (size_t)b * maxRank * sizeof(double);
+    uint64_t batch = std::max<uint64_t>(1, ((size_t)2 << 30) / std::max<size_t>(slotBytes, 1));
+    batch = std::min<uint64_t>(batch, total);
+    DBuf<double> Us(batch * b * maxRank), Vs(batch * b * maxRank);
+    DBuf<int32_t> caps(batch);
+    DBuf<int64_t> boff(batch + 1);
+    std::vector<DBuf<double>> partsU, partsV;
+    std::vector<uint64_t> partBase;
+    std::vector<int32_t> hcaps(batch);
+    uint64_t running = 0, capHits = 0;
+    for (uint64_t t0 = 0; t0 < total; t0 += batch) {
+        const uint64_t nt = std::min<uint64_t>(batch, total - t0);
+        int32_t* rk = out.ranks.p + t0;
+        switch (b) {
+            case 8: launch_tlr_d<8>(c, d, n, S, d_points, t.perm.p, t.offsets.p, ell, tau, cfg.araR, maxRank, cfg.seed, t0, nt, Us.p, Vs.p, rk, caps.p, out.diagonal.p); break;
+            case 16: launch_tlr_d<16>(c, d, n, S, d_points, t.perm.p, t.offsets.p, ell, tau, cfg.araR, maxRank, cfg.seed, t0, nt, Us.p, Vs.p, rk, caps.p, out.diagonal.p); break;
+            case 32: launch_tlr_d<32>(c, d, n, S, d_points, t.perm.p, t.offsets.p, ell, tau, cfg.araR, maxRank, cfg.seed, t0, nt, Us.p, Vs.p, rk, caps.p, out.diagonal.p); break;
+            default: launch_tlr_d<64>(c, d, n, S, d_points, t.perm.p, t.offsets.p, ell, tau, cfg.araR, maxRank, cfg.seed, t0, nt, Us.p, Vs.p, rk, caps.p, out.diagonal.p); break;
+        }
+        exclusive_scan_i32_i64(c, rk, boff.p, nt);
+        int64_t btotal = 0;
+        HMX_HIP(hipMemcpyAsync(&btotal, boff.p + nt, sizeof(int64_t), hipMemcpyDeviceToHost, c.stream));
+        HMX_HIP(hipMemcpyAsync(hcaps.data(), caps.p, nt * sizeof(int32_t), hipMemcpyDeviceToHost, c.stream));
+        HMX_HIP(hipStreamSynchronize(c.stream));
+        for (uint64_t i = 0; i < nt; ++i) capHits += (uint64_t)hcaps[i];
+        partsU.emplace_back((size_t)btotal * b);
+        partsV.emplace_back((size_t)btotal * b);
+        partBase.push_back(running);
+        if (btotal > 0) {
+            k_compact<<<(unsigned)((nt + 3) / 4), 256, 0, c.stream>>>(nt, (int)b, maxRank, rk, boff.p, Us.p, Vs.p,
+                                                                      partsU.back().p, partsV.back().p);
+            HMX_LAUNCHED();
+        }
+        running += (uint64_t)btotal;
+        HMX_HIP(hipStreamSynchronize(c.stream));
+    }
+    Us.release();
+    Vs.release();
+    out.rankSum = running;
+    out.capHits = capHits;
+    out.U.alloc((size_t)running * b);
+    out.V.alloc((size_t)running * b);
+    for (size_t i = 0; i < partsU.size(); ++i) {
+        if (!partsU[i].n) continue;
+        HMX_HIP(hipMemcpyAsync(out.U.p + partBase[i] * b, partsU[i].p, partsU[i].bytes(), hipMemcpyDeviceToDevice,
+                               c.stream));
+        HMX_HIP(hipMemcpyAsync(out.V.p + partBase[i] * b, partsV[i].p, partsV[i].bytes(), hipMemcpyDeviceToDevice,
+                               c.stream));
+    }
+    exclusive_scan_i32_i64(c, out.ranks.p, out.offsets.p, total);
+    HMX_HIP(hipStreamSynchronize(c.stream));
+}
+
+// ---- column-major -> Morton ------------------------------------------------
+__global__ void __launch_bounds__(256) k_ranks_to_mo(uint32_t S, const int32_t* __restrict__ cm,
+                                                     int32_t* __restrict__ mo) {
+    uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= (uint64_t)S * S) return;
+    mo[cm_to_mo(S, (uint32_t)i)] = cm[i];
+}
+
+__global__ void __launch_bounds__(256) k_tiles_to_mo(uint32_t S, int b, const int32_t* __restrict__ ranks,
+                                                     const int64_t* __restrict__ offCM,
+                                                     const int64_t* __restrict__ offMO, const double* __restrict__ Ucm,
+                                                     const double* __restrict__ Vcm, double* __restrict__ Umo,
+                                                     double* __restrict__ Vmo) {
+    uint64_t i = (uint64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (i >= (uint64_t)S * S) return;
+    int lane = threadIdx.x & 63;
+    size_t cnt = (size_t)ranks[i] * b;
+    uint32_t mo = cm_to_mo(S, (uint32_t)i);
+    const double* us = Ucm + (size_t)offCM[i] * b;
+    const double* vs = Vcm + (size_t)offCM[i] * b;
+    double* ud = Umo + (size_t)offMO[mo] * b;
+    double* vd = Vmo + (size_t)offMO[mo] * b;
+    for (size_t e = lane; e < cnt; e += 64) {
+        ud[e] = us[e];
+        vd[e] = vs[e];
+    }
+}
+
+void convert_to_morton(Context& c, const TLR& cm, TLR& mo) {
+    HMX_CHECK(cm.ordering == HMX_COLUMN_MAJOR, HMX_ESTATE, "convertColumnMajorToMorton: input is not column-major");
+    HMX_CHECK(cm.S <= 65536, HMX_EINVAL, "Morton codes are 32-bit: numSegments must be <= 65536");
+    const uint32_t S = cm.S, b = cm.b;
+    const uint64_t total = (uint64_t)S * S;
+    mo.ordering = HMX_MORTON;
+    mo.S = S;
+    mo.b = b;
+    mo.n = cm.n;
+    mo.rankSum = cm.rankSum;
+    mo.capHits = cm.capHits;
+    mo.ranks.alloc(total);
+    mo.offsets.alloc(total + 1);
+    mo.U.alloc(cm.U.n);
+    mo.V.alloc(cm.V.n);
+    mo.diagonal.alloc(cm.diagonal.n);
+    k_ranks_to_mo<<<(unsigned)((total + 255) / 256), 256, 0, c.stream>>>(S, cm.ranks.p, mo.ranks.p);
+    HMX_LAUNCHED();
+    exclusive_scan_i32_i64(c, mo.ranks.p, mo.offsets.p, total);
+    k_tiles_to_mo<<<(unsigned)((total + 3) / 4), 256, 0, c.stream>>>(S, (int)b, cm.ranks.p, cm.offsets.p,
+                                                                    mo.offsets.p, cm.U.p, cm.V.p, mo.U.p, mo.V.p);
+    HMX_LAUNCHED();
+    HMX_HIP(hipMemcpyAsync(mo.diagonal.p, cm.diagonal.p, cm.diagonal.bytes(), hipMemcpyDeviceToDevice, c.stream));
+    HMX_HIP(hipStreamSynchronize(c.stream));
+}
+
+}  // namespace hmx
