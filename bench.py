@@ -55,6 +55,7 @@ def main():
                         seed=1234, ctx=ctx, timings=timings)
     torch.cuda.synchronize()
     build_s = time.perf_counter() - t0
+    print(f"[bench] build {build_s:.2f}s stages {timings}", file=sys.stderr, flush=True)
     H = pipe.H
     nb = H.bytes()
     # matvec plan (built on first call) is outside the timed region
@@ -75,19 +76,20 @@ def main():
         return (time.perf_counter() - t) / steps
 
     sec1 = timed(x1, y1, args.steps, args.warmup)
+    print(f"[bench] 1-RHS matvec {sec1 * 1e3:.4f} ms", file=sys.stderr, flush=True)
     # per-kernel times with hipEvents on the library's stream
-    ms = hmx.matvec_profile(ctx, H, 1, x1, y1, iters=max(10, args.steps))
+    ms = [float(v) for v in hmx.matvec_profile(ctx, H, 1, x1, y1, iters=max(10, args.steps))]
     # algorithmic bytes: every factor entry once, x read, y written
     fac = nb["lowRankBytes"] + nb["denseBytes"]
     bytes1 = fac + 16 * n
-    gbs1 = bytes1 / sec1 / 1e9
+    gbs1 = float(bytes1 / sec1 / 1e9)
     # dominant kernel: phase 1 reads V and x; phase 2 reads U, D, x and writes y
     ph1_bytes = nb["lowRankBytes"] / 2 + 8 * n
     ph2_bytes = nb["lowRankBytes"] / 2 + nb["denseBytes"] + 16 * n
     dom = 1 if ms[0] >= ms[2] else 2
     dom_ms = ms[0] if dom == 1 else ms[2]
     dom_bytes = ph1_bytes if dom == 1 else ph2_bytes
-    achieved = dom_bytes / (dom_ms / 1e3) / 1e9
+    achieved = float(dom_bytes / (dom_ms / 1e3) / 1e9)
 
     # 16 right-hand sides
     x16 = torch.empty(16, n, dtype=torch.float64, device="cuda")
@@ -115,6 +117,7 @@ def main():
         if li["numTiles"]:
             ranks_per_level[l] = round(li["rankSum"] / li["numTiles"], 1)
 
+    print(f"[bench] 16-RHS {sec16 * 1e3:.3f} ms, rel err {rel_err:.3e} / {rel_err16:.3e}", file=sys.stderr, flush=True)
     cpu = None
     if args.cpu_seconds > 0:
         cpu = cpu_baseline(pipe, ctx, x1, bytes1, args.cpu_seconds)

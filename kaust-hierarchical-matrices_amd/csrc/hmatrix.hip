@@ -7,8 +7,8 @@
 // (Y = sum_s U_s (V_s^T Omega), B = sum_s V_s (U_s^T Q)), i.e. the operator of
 // lr_kblas_ara_batch.  The ARA is the same blocked algorithm as the TLR kernel
 // (and the CPU oracle), split into one launch per step over all tiles of a
-// level: sample, project (x2), Gram, Cholesky with column skipping, triangular
-// solve, project, Gram, Cholesky, triangular solve, commit.
+// level: sample, project (x2), column-by-column decisions, project the
+// accepted columns, re-normalise, commit.
 #include <algorithm>
 
 #include "hmx_device.h"
@@ -22,7 +22,6 @@ constexpr int kB16 = 16;
 
 struct LrState {
     int k, small_run, converged, capped, na;
-    int acc[kB16];
 };
 
 struct LrLevel {
@@ -36,7 +35,7 @@ struct LrLevel {
     const double* UMO;
     const double* VMO;
     LrState* st;
-    double *Q, *Y, *Ya, *Om, *W, *G, *R;
+    double *Q, *Y, *Ya, *Om, *W;
     int capQ;
 };
 
@@ -113,25 +112,23 @@ __global__ void __launch_bounds__(256) k_lr_sample(LrLevel L) {
     }
 }
 
-// out[t][s*K + a] = sum_i A_t[a*m+i] B_t[s*m+i], for a < K_t, s < C_t.
-// mode 0: A=Q,K=k / B=Y,C=16 -> W ; mode 1: A=B=Y, K=C=16 -> G
-// mode 2: A=Q,K=k / B=Ya,C=na -> W ; mode 3: A=B=Ya, K=C=na -> G
+// W[t][s*k + a] = sum_i Q_t[a*m+i] B_t[s*m+i], a < k_t:
+// mode 0: B = Y (16 columns, tile still iterating); mode 2: B = accepted block (na columns)
 __global__ void __launch_bounds__(256) k_lr_dot(LrLevel L, int mode) {
     __shared__ double sA[16][65], sB[16][65];
     const int t = blockIdx.y;
     const LrState& S = L.st[t];
-    // before the Cholesky step the tile must still be iterating; after it,
+    // before the decision step the tile must still be iterating; after it,
     // accepted columns are appended even when the tile converged in this block
-    if (mode <= 1 ? !lr_active(S, L.maxRank) : S.na == 0) return;
-    const int K = (mode == 0 || mode == 2) ? S.k : (mode == 1 ? kB16 : S.na);
-    const int C = (mode <= 1) ? kB16 : S.na;
+    if (mode == 0 ? !lr_active(S, L.maxRank) : S.na == 0) return;
+    const int K = S.k;
+    const int C = mode == 0 ? kB16 : S.na;
     const int a0 = blockIdx.x * 16;
     if (a0 >= K) return;
     const int m = L.m;
-    const double* A = (mode == 0 || mode == 2) ? L.Q + (size_t)t * m * L.capQ
-                                               : (mode == 1 ? L.Y + (size_t)t * m * kB16 : L.Ya + (size_t)t * m * kB16);
-    const double* Bm = (mode <= 1) ? L.Y + (size_t)t * m * kB16 : L.Ya + (size_t)t * m * kB16;
-    double* out = (mode == 0 || mode == 2) ? L.W + (size_t)t * L.capQ * kB16 : L.G + (size_t)t * kB16 * kB16;
+    const double* A = L.Q + (size_t)t * m * L.capQ;
+    const double* Bm = (mode == 0 ? L.Y : L.Ya) + (size_t)t * m * kB16;
+    double* out = L.W + (size_t)t * L.capQ * kB16;
     const int ta = threadIdx.x & 15, ts = threadIdx.x >> 4;
     double acc = 0.0;
     for (int i0 = 0; i0 < m; i0 += 64) {
@@ -189,97 +186,95 @@ __global__ void __launch_bounds__(256) k_lr_update(LrLevel L, int mode) {
             if (s < C) Y[(size_t)s * m + i] = acc[s];
 }
 
-__global__ void k_lr_chol_skip(LrLevel L) {
-    int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= L.T) return;
+__device__ inline double wave_sum64(double v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+// sum over the block; every thread receives the identical value
+__device__ inline double block_sum(double v, double* red) {
+    v = wave_sum64(v);
+    const int w = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0) red[w] = v;
+    __syncthreads();
+    double s = 0.0;
+    for (int q = 0; q < nw; ++q) s += red[q];
+    return s;
+}
+
+// One workgroup per tile: column-by-column decisions on explicitly projected
+// vectors (two modified Gram-Schmidt sweeps against the block's accepted
+// columns, accept when the norm exceeds tau).  Threads own fixed rows, so
+// only the reductions need the workgroup barrier.
+__global__ void __launch_bounds__(1024) k_lr_decide(LrLevel L) {
+    __shared__ double red[16];
+    const int t = blockIdx.x;
     LrState& S = L.st[t];
-    S.na = 0;
-    if (!lr_active(S, L.maxRank)) return;
-    const double* G = L.G + (size_t)t * kB16 * kB16;
-    double* R = L.R + (size_t)t * kB16 * kB16;
-    for (int q = 0; q < kB16 * kB16; ++q) R[q] = 0.0;
-    int na = 0, capleft = L.maxRank - S.k;
+    if (!lr_active(S, L.maxRank)) {
+        if (threadIdx.x == 0) S.na = 0;
+        return;
+    }
+    const int m = L.m, nthr = blockDim.x, tid = threadIdx.x;
+    double* Y = L.Y + (size_t)t * m * kB16;
+    double* Qb = L.Ya + (size_t)t * m * kB16;
+    int na = 0, small = S.small_run, conv = 0, capped = 0;
+    const int capleft = L.maxRank - S.k;
     for (int j = 0; j < kB16; ++j) {
-        double s = G[j * kB16 + j];
-        for (int a = 0; a < na; ++a) {
-            double x = R[S.acc[a] * kB16 + j];
-            s -= x * x;
-        }
-        double nu = s > 0 ? sqrt(s) : 0.0;
-        if (nu > L.tau) {
-            if (na >= capleft) { S.capped = 1; break; }
-            R[j * kB16 + j] = nu;
-            for (int l = j + 1; l < kB16; ++l) {
-                double x = G[j * kB16 + l];
-                for (int a = 0; a < na; ++a) x -= R[S.acc[a] * kB16 + j] * R[S.acc[a] * kB16 + l];
-                R[j * kB16 + l] = x / nu;
+        double* y = Y + (size_t)j * m;
+        for (int pass = 0; pass < 2; ++pass)
+            for (int a = 0; a < na; ++a) {
+                const double* qa = Qb + (size_t)a * m;
+                double p = 0.0;
+                for (int i = tid; i < m; i += nthr) p += qa[i] * y[i];
+                const double d = block_sum(p, red);
+                for (int i = tid; i < m; i += nthr) y[i] -= qa[i] * d;
             }
-            S.acc[na++] = j;
-            S.small_run = 0;
-        } else {
-            if (++S.small_run >= L.r) { S.converged = 1; break; }
+        double p = 0.0;
+        for (int i = tid; i < m; i += nthr) p += y[i] * y[i];
+        const double nu = sqrt(block_sum(p, red));
+        if (nu > L.tau) {
+            if (na >= capleft) { capped = 1; break; }
+            for (int i = tid; i < m; i += nthr) Qb[(size_t)na * m + i] = y[i] / nu;
+            ++na;
+            small = 0;
+        } else if (++small >= L.r) {
+            conv = 1;
+            break;
         }
     }
-    S.na = na;
+    if (tid == 0) { S.na = na; S.small_run = small; S.converged = conv; S.capped = capped; }
 }
 
-// Ya = Y[:, acc] R_acc^{-1}
-__global__ void __launch_bounds__(256) k_lr_trsm1(LrLevel L) {
-    const int t = blockIdx.y;
-    const LrState& S = L.st[t];
-    if (S.na == 0) return;  // na is reset for inactive tiles
-    const int m = L.m, i = blockIdx.x * 256 + threadIdx.x;
-    if (i >= m) return;
-    const double* Y = L.Y + (size_t)t * m * kB16;
-    double* Ya = L.Ya + (size_t)t * m * kB16;
-    const double* R = L.R + (size_t)t * kB16 * kB16;
-    double ya[kB16];
-    for (int bc = 0; bc < S.na; ++bc) {
-        double x = Y[(size_t)S.acc[bc] * m + i];
-        for (int a = 0; a < bc; ++a) x -= ya[a] * R[S.acc[a] * kB16 + S.acc[bc]];
-        ya[bc] = x / R[S.acc[bc] * kB16 + S.acc[bc]];
-        Ya[(size_t)bc * m + i] = ya[bc];
-    }
-}
-
-__global__ void k_lr_chol_plain(LrLevel L) {
-    int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= L.T) return;
+// After the accepted columns were projected against Q: one more modified
+// Gram-Schmidt sweep inside the block, normalise, append to Q.
+__global__ void __launch_bounds__(1024) k_lr_renorm(LrLevel L) {
+    __shared__ double red[16];
+    const int t = blockIdx.x;
     const LrState& S = L.st[t];
     const int na = S.na;
     if (na == 0) return;
-    const double* G = L.G + (size_t)t * kB16 * kB16;
-    double* R = L.R + (size_t)t * kB16 * kB16;
-    for (int j = 0; j < na; ++j) {
-        double s = G[j * na + j];
-        for (int a = 0; a < j; ++a) s -= R[a * na + j] * R[a * na + j];
-        double d2 = s > 0 ? sqrt(s) : 1e-300;
-        R[j * na + j] = d2;
-        for (int l = j + 1; l < na; ++l) {
-            double x = G[j * na + l];
-            for (int a = 0; a < j; ++a) x -= R[a * na + j] * R[a * na + l];
-            R[j * na + l] = x / d2;
-        }
-    }
-}
-
-// Q[:, k+bc] = Ya R2^{-1}
-__global__ void __launch_bounds__(256) k_lr_trsm2(LrLevel L) {
-    const int t = blockIdx.y;
-    const LrState& S = L.st[t];
-    const int na = S.na;
-    if (na == 0) return;
-    const int m = L.m, i = blockIdx.x * 256 + threadIdx.x;
-    if (i >= m) return;
-    const double* Ya = L.Ya + (size_t)t * m * kB16;
+    const int m = L.m, nthr = blockDim.x, tid = threadIdx.x;
+    double* Qb = L.Ya + (size_t)t * m * kB16;
     double* Q = L.Q + (size_t)t * m * L.capQ;
-    const double* R = L.R + (size_t)t * kB16 * kB16;
-    double q[kB16];
-    for (int bc = 0; bc < na; ++bc) {
-        double x = Ya[(size_t)bc * m + i];
-        for (int a = 0; a < bc; ++a) x -= q[a] * R[a * na + bc];
-        q[bc] = x / R[bc * na + bc];
-        Q[(size_t)(S.k + bc) * m + i] = q[bc];
+    for (int j = 0; j < na; ++j) {
+        double* q = Qb + (size_t)j * m;
+        for (int a = 0; a < j; ++a) {
+            const double* qa = Qb + (size_t)a * m;
+            double p = 0.0;
+            for (int i = tid; i < m; i += nthr) p += qa[i] * q[i];
+            const double d = block_sum(p, red);
+            for (int i = tid; i < m; i += nthr) q[i] -= qa[i] * d;
+        }
+        double p = 0.0;
+        for (int i = tid; i < m; i += nthr) p += q[i] * q[i];
+        const double nu = sqrt(block_sum(p, red));
+        for (int i = tid; i < m; i += nthr) {
+            const double v = q[i] / nu;
+            q[i] = v;
+            Q[(size_t)(S.k + j) * m + i] = v;
+        }
     }
 }
 
@@ -418,17 +413,17 @@ static void compress_level(Context& c, const hmx_config& cfg, const TLR& mo, con
     capQ = std::max(capQ, std::min(maxRank, kB16));
     DBuf<double> Q((size_t)T * m * capQ), W((size_t)T * capQ * kB16);
     DBuf<double> Y((size_t)T * m * kB16), Ya((size_t)T * m * kB16), Om((size_t)T * m * kB16);
-    DBuf<double> G((size_t)T * kB16 * kB16), R((size_t)T * kB16 * kB16);
     DBuf<int> flags(2);
     LrLevel L;
     L.T = T; L.m = m; L.b = b; L.bus = bus; L.maxRank = maxRank; L.r = cfg.araR;
     L.tau = ara_threshold((double)tol);
     L.seed = cfg.seed; L.level = level;
     L.first = d_first.p; L.ranksMO = mo.ranks.p; L.offMO = mo.offsets.p; L.UMO = mo.U.p; L.VMO = mo.V.p;
-    L.st = st.p; L.Q = Q.p; L.Y = Y.p; L.Ya = Ya.p; L.Om = Om.p; L.W = W.p; L.G = G.p; L.R = R.p; L.capQ = capQ;
+    L.st = st.p; L.Q = Q.p; L.Y = Y.p; L.Ya = Ya.p; L.Om = Om.p; L.W = W.p; L.capQ = capQ;
     k_lr_init<<<(T + 255) / 256, 256, 0, c.stream>>>(T, st.p);
     HMX_LAUNCHED();
     const unsigned rowBlocks = (unsigned)((m + 255) / 256);
+    const unsigned wgThreads = m >= 4096 ? 1024u : 256u;
     int kmax = 0;
     const int maxIters = (2 * maxRank + cfg.araR) / kB16 + 2;
     for (int it = 0;; ++it) {
@@ -456,11 +451,7 @@ static void compress_level(Context& c, const hmx_config& cfg, const TLR& mo, con
             k_lr_update<<<dim3(rowBlocks, T), 256, 0, c.stream>>>(L, 0);
             HMX_LAUNCHED();
         }
-        k_lr_dot<<<dim3(1, T), 256, 0, c.stream>>>(L, 1);
-        HMX_LAUNCHED();
-        k_lr_chol_skip<<<(T + 63) / 64, 64, 0, c.stream>>>(L);
-        HMX_LAUNCHED();
-        k_lr_trsm1<<<dim3(rowBlocks, T), 256, 0, c.stream>>>(L);
+        k_lr_decide<<<T, wgThreads, 0, c.stream>>>(L);
         HMX_LAUNCHED();
         if (kmax > 0) {
             k_lr_dot<<<dim3(kb, T), 256, 0, c.stream>>>(L, 2);
@@ -468,11 +459,7 @@ static void compress_level(Context& c, const hmx_config& cfg, const TLR& mo, con
             k_lr_update<<<dim3(rowBlocks, T), 256, 0, c.stream>>>(L, 2);
             HMX_LAUNCHED();
         }
-        k_lr_dot<<<dim3(1, T), 256, 0, c.stream>>>(L, 3);
-        HMX_LAUNCHED();
-        k_lr_chol_plain<<<(T + 63) / 64, 64, 0, c.stream>>>(L);
-        HMX_LAUNCHED();
-        k_lr_trsm2<<<dim3(rowBlocks, T), 256, 0, c.stream>>>(L);
+        k_lr_renorm<<<T, wgThreads, 0, c.stream>>>(L);
         HMX_LAUNCHED();
         HMX_HIP(hipMemsetAsync(flags.p, 0, 2 * sizeof(int), c.stream));
         k_lr_commit<<<(T + 255) / 256, 256, 0, c.stream>>>(L, flags.p);

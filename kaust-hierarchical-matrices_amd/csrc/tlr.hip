@@ -4,10 +4,12 @@
 // One wave per b x b tile: the tile's points are gathered into LDS, the
 // kernel tile exp(-|x-y|/ell) is evaluated once into LDS (no dense matrix in
 // HBM), and the adaptive randomized approximation runs entirely in LDS:
-//   Y = A*Omega (bs Gaussian samples from Philox), Y -= Q(Q^T Y) twice,
-//   Gram Cholesky with column skipping (accept a column when its residual
-//   norm exceeds tau, stop after r consecutive small ones), Y R^-1, one more
-//   projection + Cholesky-QR, append to Q; finally V = A^T Q.
+//   Y = A*Omega (bs Gaussian samples from Philox), Y -= Q(Q^T Y) twice, then
+//   column by column two modified Gram-Schmidt sweeps against the columns
+//   already accepted from the block; a column is accepted when the norm of
+//   that explicitly projected vector exceeds tau, and the ARA stops after r
+//   consecutive small ones; accepted columns are projected against Q once
+//   more, re-normalised and appended; finally V = A^T Q.
 // The same algorithm, step for step, is the CPU oracle's ara_run().
 #include <algorithm>
 
@@ -19,63 +21,23 @@ namespace hmx {
 constexpr int kBS = 16;  // ARA samples per block (kblas_ara_batch bs argument)
 
 struct AraState {
-    int k, small_run, converged, capped, na;
-    int acc[kBS];
+    int k, small_run, converged, capped;
 };
 
-// Cholesky of the Gram matrix with column skipping (one lane).
-__device__ inline void chol_skip_lane(const double* G, double* R, AraState* st, double tau, int r, int capleft) {
-    int na = 0;
-    for (int i = 0; i < kBS * kBS; ++i) R[i] = 0.0;
-    for (int j = 0; j < kBS; ++j) {
-        double s = G[j * kBS + j];
-        for (int a = 0; a < na; ++a) {
-            double t = R[st->acc[a] * kBS + j];
-            s -= t * t;
-        }
-        double nu = s > 0 ? sqrt(s) : 0.0;
-        if (nu > tau) {
-            if (na >= capleft) { st->capped = 1; break; }
-            R[j * kBS + j] = nu;
-            for (int l = j + 1; l < kBS; ++l) {
-                double t = G[j * kBS + l];
-                for (int a = 0; a < na; ++a) t -= R[st->acc[a] * kBS + j] * R[st->acc[a] * kBS + l];
-                R[j * kBS + l] = t / nu;
-            }
-            st->acc[na++] = j;
-            st->small_run = 0;
-        } else {
-            if (++st->small_run >= r) { st->converged = 1; break; }
-        }
-    }
-    st->na = na;
-}
-
-// plain Cholesky of an na x na Gram matrix (one lane); R upper, R[a*na+s]
-__device__ inline void chol_plain_lane(const double* G, double* R, int na) {
-    for (int j = 0; j < na; ++j) {
-        double s = G[j * na + j];
-        for (int a = 0; a < j; ++a) s -= R[a * na + j] * R[a * na + j];
-        double d2 = s > 0 ? sqrt(s) : 1e-300;
-        R[j * na + j] = d2;
-        for (int l = j + 1; l < na; ++l) {
-            double t = G[j * na + l];
-            for (int a = 0; a < j; ++a) t -= R[a * na + j] * R[a * na + l];
-            R[j * na + l] = t / d2;
-        }
-    }
+__device__ inline double wave_sum64(double v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;  // butterfly: every lane holds the identical sum
 }
 
 template <int B>
 struct TileLds {
     static constexpr int MR = B;  // rank capacity
     double A[B * B];
-    double Om[B * kBS];  // Omega, later Ya
+    double Om[B * kBS];  // Omega, later the accepted block columns
     double Y[B * kBS];
     double Q[B * MR];
     double W[MR * kBS];
-    double G[kBS * kBS];
-    double R[kBS * kBS];
     double xr[3 * B], xc[3 * B];
     AraState st;
 };
@@ -114,7 +76,7 @@ __global__ void __launch_bounds__(64) k_tlr_ara(uint32_t n, uint32_t S, const do
         if (tid == 0) { ranks[slot] = 0; caps[slot] = 0; }
         return;
     }
-    if (tid == 0) { L.st.k = 0; L.st.small_run = 0; L.st.converged = 0; L.st.capped = 0; L.st.na = 0; }
+    if (tid == 0) { L.st.k = 0; L.st.small_run = 0; L.st.converged = 0; L.st.capped = 0; }
     __syncthreads();
     // every block either accepts a column (k grows, bounded by maxRank) or
     // extends the small-column run (bounded by r), so this bound is never hit
@@ -154,58 +116,69 @@ __global__ void __launch_bounds__(64) k_tlr_ara(uint32_t n, uint32_t S, const do
             }
             __syncthreads();
         }
-        for (int e = tid; e < kBS * kBS; e += 64) {
-            int a = e / kBS, s = e % kBS;
-            double acc = 0.0;
-            for (int q = 0; q < B; ++q) acc += L.Y[a * B + q] * L.Y[s * B + q];
-            L.G[e] = acc;
+        // column-by-column decisions on explicitly projected vectors; lane q owns row q
+        double* Qb = L.Om;
+        int na = 0, small_run = L.st.small_run, converged = 0, capped = 0;
+        const int capleft = maxRank - k;
+        const bool own = tid < B;
+        for (int j = 0; j < kBS; ++j) {
+            double y = own ? L.Y[j * B + tid] : 0.0;
+            for (int pass = 0; pass < 2; ++pass)
+                for (int a = 0; a < na; ++a) {
+                    const double qa = own ? Qb[a * B + tid] : 0.0;
+                    const double d = wave_sum64(qa * y);
+                    y -= qa * d;
+                }
+            const double nu = sqrt(wave_sum64(y * y));
+            if (nu > tau) {
+                if (na >= capleft) { capped = 1; break; }
+                if (own) Qb[na * B + tid] = y / nu;
+                ++na;
+                small_run = 0;
+            } else if (++small_run >= r) {
+                converged = 1;
+                break;
+            }
         }
         __syncthreads();
-        if (tid == 0) chol_skip_lane(L.G, L.R, &L.st, tau, r, maxRank - k);
-        __syncthreads();
-        const int na = L.st.na;
         if (na > 0) {
-            double* Ya = L.Om;
-            for (int q = tid; q < B; q += 64)
-                for (int bc = 0; bc < na; ++bc) {
-                    double x = L.Y[L.st.acc[bc] * B + q];
-                    for (int a = 0; a < bc; ++a) x -= Ya[a * B + q] * L.R[L.st.acc[a] * kBS + L.st.acc[bc]];
-                    Ya[bc * B + q] = x / L.R[L.st.acc[bc] * kBS + L.st.acc[bc]];
-                }
-            __syncthreads();
-            if (k > 0) {
+            if (k > 0) {  // accepted columns -= Q (Q^T accepted)
                 for (int e = tid; e < k * na; e += 64) {
                     int s = e / k, a = e % k;
                     double acc = 0.0;
-                    for (int q = 0; q < B; ++q) acc += L.Q[a * B + q] * Ya[s * B + q];
+                    for (int q = 0; q < B; ++q) acc += L.Q[a * B + q] * Qb[s * B + q];
                     L.W[s * k + a] = acc;
                 }
                 __syncthreads();
                 for (int e = tid; e < B * na; e += 64) {
                     int s = e / B, q = e % B;
-                    double acc = Ya[e];
+                    double acc = Qb[e];
                     for (int a = 0; a < k; ++a) acc -= L.Q[a * B + q] * L.W[s * k + a];
-                    Ya[e] = acc;
+                    Qb[e] = acc;
                 }
                 __syncthreads();
             }
-            for (int e = tid; e < na * na; e += 64) {
-                int a = e / na, s = e % na;
-                double acc = 0.0;
-                for (int q = 0; q < B; ++q) acc += Ya[a * B + q] * Ya[s * B + q];
-                L.G[e] = acc;
-            }
-            __syncthreads();
-            if (tid == 0) chol_plain_lane(L.G, L.R, na);
-            __syncthreads();
-            for (int q = tid; q < B; q += 64)
-                for (int bc = 0; bc < na; ++bc) {
-                    double x = Ya[bc * B + q];
-                    for (int a = 0; a < bc; ++a) x -= L.Q[(k + a) * B + q] * L.R[a * na + bc];
-                    L.Q[(k + bc) * B + q] = x / L.R[bc * na + bc];
+            for (int j = 0; j < na; ++j) {  // re-normalise within the block
+                double q = own ? Qb[j * B + tid] : 0.0;
+                for (int a = 0; a < j; ++a) {
+                    const double qa = own ? Qb[a * B + tid] : 0.0;
+                    const double d = wave_sum64(qa * q);
+                    q -= qa * d;
                 }
-            __syncthreads();
-            if (tid == 0) L.st.k = k + na;
+                const double nu = sqrt(wave_sum64(q * q));
+                if (own) {
+                    q /= nu;
+                    Qb[j * B + tid] = q;
+                    L.Q[(k + j) * B + tid] = q;
+                }
+            }
+        }
+        __syncthreads();
+        if (tid == 0) {
+            L.st.k = k + na;
+            L.st.small_run = small_run;
+            L.st.converged = converged;
+            L.st.capped = capped;
         }
         ++it;
         __syncthreads();

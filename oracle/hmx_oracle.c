@@ -381,36 +381,6 @@ typedef struct {
 
 OR_API double or_ara_threshold(double tol) { return tol / (10.0 * sqrt(2.0 / 3.141592653589793)); }
 
-/*
- * Cholesky of the bs x bs Gram matrix with column skipping.  Column j is
- * accepted when its residual norm (after the accepted columns before it) is
- * above tau; the run of consecutive small columns carries across blocks and
- * the ARA converges when it reaches r.  Returns number of accepted columns.
- */
-static int chol_skip(const double* G, int bs, double tau, int r, int capleft, int* small_run, int* converged,
-                     int* capped, double* R, int* acc) {
-    int na = 0;
-    for (int j = 0; j < bs; ++j) {
-        double s = G[j * bs + j];
-        for (int a = 0; a < na; ++a) { double t = R[acc[a] * bs + j]; s -= t * t; }
-        double nu = s > 0 ? sqrt(s) : 0.0;
-        if (nu > tau) {
-            if (na >= capleft) { *capped = 1; break; }
-            R[j * bs + j] = nu;
-            for (int l = j + 1; l < bs; ++l) {
-                double t = G[j * bs + l];
-                for (int a = 0; a < na; ++a) t -= R[acc[a] * bs + j] * R[acc[a] * bs + l];
-                R[j * bs + l] = t / nu;
-            }
-            acc[na++] = j;
-            *small_run = 0;
-        } else {
-            if (++*small_run >= r) { *converged = 1; break; }
-        }
-    }
-    return na;
-}
-
 /* Q(m x k) ; W(k x c) = Q^T Y(m x c) ; Y -= Q W */
 static void project_out(int m, int k, const double* Q, double* Y, int c, double* W) {
     if (k == 0) return;
@@ -427,65 +397,76 @@ static void project_out(int m, int k, const double* Q, double* Y, int c, double*
         }
 }
 
-static void gram(int m, const double* Y, int c, double* G) {
-    for (int a = 0; a < c; ++a)
-        for (int s = 0; s < c; ++s) {
-            double t = 0;
-            for (int i = 0; i < m; ++i) t += Y[(size_t)a * m + i] * Y[(size_t)s * m + i];
-            G[a * c + s] = t;
-        }
+/* y -= b_a (b_a^T y) for a = 0..nb-1 in turn (modified Gram-Schmidt step) */
+static void mgs_step(int m, const double* B, int nb, double* y) {
+    for (int a = 0; a < nb; ++a) {
+        double t = 0;
+        for (int i = 0; i < m; ++i) t += B[(size_t)a * m + i] * y[i];
+        for (int i = 0; i < m; ++i) y[i] -= B[(size_t)a * m + i] * t;
+    }
 }
 
-/* generic ARA; U (m x maxRank), V (n x maxRank) column-major. Returns rank. */
+static double norm2(int m, const double* y) {
+    double t = 0;
+    for (int i = 0; i < m; ++i) t += y[i] * y[i];
+    return sqrt(t);
+}
+
+/*
+ * generic ARA; U (m x maxRank), V (n x maxRank) column-major.  Returns rank.
+ * Per block of bs samples: Y = A Omega; Y -= Q(Q^T Y) twice; then column by
+ * column, project out the columns already accepted from this block (two
+ * modified Gram-Schmidt sweeps)
+ * and accept the column when its residual norm exceeds tau (HMT Alg. 4.2
+ * test), the run of consecutive small columns carrying across blocks and
+ * ending the ARA at r; the accepted columns are projected against Q once
+ * more and re-normalised before being appended.  Norms are taken of the
+ * explicitly projected vectors (no Gram matrix), so a decision is only as
+ * uncertain as eps*|y| rather than sqrt(eps)*|y|.
+ */
 static int ara_run(const ara_op* op, uint64_t seed, uint64_t opid, double tau, int bs, int r, int maxRank, double* U,
                    double* V, int* capped_out) {
     int m = op->m, n = op->n;
     double* Om = (double*)malloc(sizeof(double) * (size_t)n * bs);
     double* Y = (double*)malloc(sizeof(double) * (size_t)m * bs);
-    double* Ya = (double*)malloc(sizeof(double) * (size_t)m * bs);
+    double* Qb = (double*)malloc(sizeof(double) * (size_t)m * bs);
     double* W = (double*)malloc(sizeof(double) * (size_t)(maxRank > 0 ? maxRank : 1) * bs);
-    double* G = (double*)malloc(sizeof(double) * bs * bs);
-    double* R = (double*)malloc(sizeof(double) * bs * bs);
-    int* acc = (int*)malloc(sizeof(int) * bs);
     int k = 0, small_run = 0, converged = 0, capped = 0, it = 0;
-    while (!converged && !capped && k < maxRank) {
+    const int maxIters = (2 * maxRank + r) / bs + 2;
+    while (!converged && !capped && k < maxRank && it < maxIters) {
         for (int s = 0; s < bs; ++s)
             for (int c = 0; c < n; ++c)
                 Om[(size_t)s * n + c] = gauss_at(seed, TAG_OMEGA, opid, (uint32_t)(it * bs + s), (uint32_t)c);
         op->sample(op->ctx, Om, bs, Y);
         project_out(m, k, U, Y, bs, W);
         project_out(m, k, U, Y, bs, W);
-        gram(m, Y, bs, G);
-        memset(R, 0, sizeof(double) * bs * bs);
-        int na = chol_skip(G, bs, tau, r, maxRank - k, &small_run, &converged, &capped, R, acc);
+        int na = 0, capleft = maxRank - k;
+        for (int j = 0; j < bs; ++j) {
+            double* y = Y + (size_t)j * m;
+            mgs_step(m, Qb, na, y);
+            mgs_step(m, Qb, na, y);
+            double nu = norm2(m, y);
+            if (nu > tau) {
+                if (na >= capleft) { capped = 1; break; }
+                for (int i = 0; i < m; ++i) Qb[(size_t)na * m + i] = y[i] / nu;
+                ++na;
+                small_run = 0;
+            } else if (++small_run >= r) {
+                converged = 1;
+                break;
+            }
+        }
         if (na > 0) {
-            /* Ya = Y[:, acc] R_acc^{-1} (row-wise forward substitution) */
-            for (int i = 0; i < m; ++i)
-                for (int bcol = 0; bcol < na; ++bcol) {
-                    double t = Y[(size_t)acc[bcol] * m + i];
-                    for (int a = 0; a < bcol; ++a) t -= Ya[(size_t)a * m + i] * R[acc[a] * bs + acc[bcol]];
-                    Ya[(size_t)bcol * m + i] = t / R[acc[bcol] * bs + acc[bcol]];
-                }
-            /* second pass: project, Cholesky-QR */
-            project_out(m, k, U, Ya, na, W);
-            gram(m, Ya, na, G);
-            for (int j = 0; j < na; ++j) { /* plain Cholesky G = R2^T R2, R2 upper, stored in R */
-                double s = G[j * na + j];
-                for (int a = 0; a < j; ++a) s -= R[a * na + j] * R[a * na + j];
-                double d2 = s > 0 ? sqrt(s) : 1e-300;
-                R[j * na + j] = d2;
-                for (int l = j + 1; l < na; ++l) {
-                    double t = G[j * na + l];
-                    for (int a = 0; a < j; ++a) t -= R[a * na + j] * R[a * na + l];
-                    R[j * na + l] = t / d2;
+            project_out(m, k, U, Qb, na, W);
+            for (int j = 0; j < na; ++j) {
+                double* q = Qb + (size_t)j * m;
+                mgs_step(m, Qb, j, q);
+                double nu = norm2(m, q);
+                for (int i = 0; i < m; ++i) {
+                    q[i] /= nu;
+                    U[(size_t)(k + j) * m + i] = q[i];
                 }
             }
-            for (int i = 0; i < m; ++i)
-                for (int bcol = 0; bcol < na; ++bcol) {
-                    double t = Ya[(size_t)bcol * m + i];
-                    for (int a = 0; a < bcol; ++a) t -= U[(size_t)(k + a) * m + i] * R[a * na + bcol];
-                    U[(size_t)(k + bcol) * m + i] = t / R[bcol * na + bcol];
-                }
             k += na;
         }
         ++it;
@@ -493,7 +474,7 @@ static int ara_run(const ara_op* op, uint64_t seed, uint64_t opid, double tau, i
     if (!converged && k >= maxRank) capped = 1;
     if (k > 0) op->sampleT(op->ctx, U, k, V);
     if (capped_out) *capped_out = capped;
-    free(Om); free(Y); free(Ya); free(W); free(G); free(R); free(acc);
+    free(Om); free(Y); free(Qb); free(W);
     return k;
 }
 
