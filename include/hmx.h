@@ -186,6 +186,50 @@ hmx_status hmx_HMatrixVecMultTranspose(hmx_context* ctx, const hmx_hmatrix* h, u
 hmx_status hmx_matvec_profile(hmx_context* ctx, const hmx_hmatrix* h, uint32_t vectorWidth, const double* d_in,
                               double* d_out, int32_t iters, float* ms);
 
+/* ---- many right-hand sides on the matrix cores ------------------------- */
+/* hmx_HMatrixVecMult with vectorWidth a multiple of 16 runs on the FP64
+   matrix cores (16 or 64 right-hand sides per pass); other widths use the
+   streaming kernels.  Times the multi-RHS kernels like hmx_matvec_profile
+   (ms[0] phase 1, ms[1] reduce, ms[2] phase 2, ms[3] call); mixed != 0 times
+   hmx_HMatrixVecMultMixed (in/out then point to float). */
+hmx_status hmx_matvec_profile_multi(hmx_context* ctx, const hmx_hmatrix* h, uint32_t vectorWidth, const void* d_in,
+                                    void* d_out, int32_t iters, int32_t mixed, float* ms);
+/* Mixed precision (no reference counterpart; the precision extension of
+   HMatrixVecMult): stores every low-rank column either in FP32 or in FP16
+   with per-column FP32 scales, choosing FP16 for the columns of least energy
+   such that the storage error stays below targetRelErr relative to ||H||_F;
+   dense diagonal blocks in FP32. */
+hmx_status hmx_hmatrix_make_mixed(hmx_context* ctx, hmx_hmatrix* h, double targetRelErr);
+/* columns kept in FP32 / stored in FP16, squared Frobenius norm of H, energy
+   of the FP16 columns, bytes read per call */
+hmx_status hmx_hmatrix_mixed_info(const hmx_hmatrix* h, int64_t* hiCols, int64_t* loCols, double* frobH2,
+                                  double* loEnergy, uint64_t* bytes);
+/* Y = H X with X, Y n x vectorWidth column-major FP32 (vectorWidth a multiple
+   of 16); FP16/FP32 matrix cores with FP32 accumulation */
+hmx_status hmx_HMatrixVecMultMixed(hmx_context* ctx, const hmx_hmatrix* h, uint32_t vectorWidth, const float* d_in,
+                                   float* d_out);
+/* One MFMA of each element type with the lane layout the kernels assume:
+   D(16x16) = A(16xK) B(Kx16), K = 4 (type 0 f64, 1 f32) or 16 (type 2 f16);
+   A, B, D row-major FP64 device buffers.  A self-check of the matrix-core
+   operand layout. */
+hmx_status hmx_selftest_mfma(hmx_context* ctx, int32_t type, const double* d_A, const double* d_B, double* d_D);
+
+/* ---- differentiation (no reference counterpart: the backward of
+   HMatrixVecMult, for autograd callers) ------------------------------------ */
+/* Gradients of a loss with output gradient dY (Y = H X) with respect to every
+   tile's factors, written into per-level buffers laid out like the level's
+   U/V (dU[l-1], dV[l-1] for level l; hmx_hmatrix_level_info gives sizes).
+   X, dY: n x vectorWidth column-major FP64 device buffers.  The input
+   gradient is hmx_HMatrixVecMultTranspose(dY). */
+hmx_status hmx_hmatvec_grad_factors(hmx_context* ctx, const hmx_hmatrix* h, uint32_t vectorWidth, const double* d_X,
+                                    const double* d_dY, double* const* d_dU, double* const* d_dV);
+/* Copy one level's factors into device buffers (same layout) */
+hmx_status hmx_hmatrix_level_get(hmx_context* ctx, const hmx_hmatrix* h, int32_t level, double* d_U, double* d_V);
+/* Overwrite one level's factors from device buffers in the level's layout
+   (e.g. after an optimizer step on them). */
+hmx_status hmx_hmatrix_set_level(hmx_context* ctx, hmx_hmatrix* h, int32_t level, const double* d_U,
+                                 const double* d_V);
+
 /* ---- dense check (EXPAND_MATRIX oracle on sampled rows) ----------------- */
 /* checkErrorInHmatrixVecMult / ReferenceGemm_kernel, src/expandMatrix.cu:174-234,
    restricted to nrows sampled rows: Y[r,:] = sum_j K(x_perm[rows[r]], x_perm[j]) X[j,:]

@@ -385,7 +385,93 @@ hmx_status hmx_HMatrixVecMult(hmx_context* ctx, const hmx_hmatrix* h, uint32_t v
     if (!ctx) return HMX_EINVAL;
     return guard(ctx, [&] {
         NEED(h);
-        hmatvec(*ctx, const_cast<hmx_hmatrix&>(*h), vectorWidth, d_in, d_out, false, nullptr, 0);
+        if (vectorWidth >= 16 && vectorWidth % 16 == 0)
+            hmatvec_multi(*ctx, const_cast<hmx_hmatrix&>(*h), vectorWidth, d_in, d_out, nullptr, 0);
+        else
+            hmatvec(*ctx, const_cast<hmx_hmatrix&>(*h), vectorWidth, d_in, d_out, false, nullptr, 0);
+    });
+}
+
+hmx_status hmx_matvec_profile_multi(hmx_context* ctx, const hmx_hmatrix* h, uint32_t vectorWidth, const void* d_in,
+                                    void* d_out, int32_t iters, int32_t mixed, float* ms) {
+    if (!ctx) return HMX_EINVAL;
+    return guard(ctx, [&] {
+        NEED(h);
+        NEED(ms);
+        HMX_CHECK(vectorWidth == 16 || vectorWidth == 64, HMX_EINVAL, "profiling needs vectorWidth 16 or 64");
+        auto& H = const_cast<hmx_hmatrix&>(*h);
+        if (mixed) hmatvec_mixed(*ctx, H, vectorWidth, (const float*)d_in, (float*)d_out, ms, iters);
+        else hmatvec_multi(*ctx, H, vectorWidth, (const double*)d_in, (double*)d_out, ms, iters);
+    });
+}
+
+hmx_status hmx_hmatrix_make_mixed(hmx_context* ctx, hmx_hmatrix* h, double targetRelErr) {
+    if (!ctx) return HMX_EINVAL;
+    return guard(ctx, [&] {
+        NEED(h);
+        hmatrix_make_mixed(*ctx, *h, targetRelErr);
+    });
+}
+
+hmx_status hmx_hmatrix_mixed_info(const hmx_hmatrix* h, int64_t* hiCols, int64_t* loCols, double* frobH2,
+                                  double* loEnergy, uint64_t* bytes) {
+    if (!h || !h->mfMixed.built) return HMX_ESTATE;
+    if (hiCols) *hiCols = h->mfMixed.hiCols;
+    if (loCols) *loCols = h->mfMixed.loCols;
+    if (frobH2) *frobH2 = h->mfMixed.frobH2;
+    if (loEnergy) *loEnergy = h->mfMixed.loEnergy;
+    if (bytes) *bytes = h->mfMixed.bytes;
+    return HMX_OK;
+}
+
+hmx_status hmx_HMatrixVecMultMixed(hmx_context* ctx, const hmx_hmatrix* h, uint32_t vectorWidth, const float* d_in,
+                                   float* d_out) {
+    if (!ctx) return HMX_EINVAL;
+    return guard(ctx, [&] {
+        NEED(h);
+        hmatvec_mixed(*ctx, const_cast<hmx_hmatrix&>(*h), vectorWidth, d_in, d_out, nullptr, 0);
+    });
+}
+
+hmx_status hmx_hmatvec_grad_factors(hmx_context* ctx, const hmx_hmatrix* h, uint32_t vectorWidth, const double* d_X,
+                                    const double* d_dY, double* const* d_dU, double* const* d_dV) {
+    if (!ctx) return HMX_EINVAL;
+    return guard(ctx, [&] {
+        NEED(h);
+        hmatvec_grad_factors(*ctx, const_cast<hmx_hmatrix&>(*h), vectorWidth, d_X, d_dY, d_dU, d_dV);
+    });
+}
+
+hmx_status hmx_hmatrix_level_get(hmx_context* ctx, const hmx_hmatrix* h, int32_t level, double* d_U, double* d_V) {
+    if (!ctx || !h || level < 1 || level >= h->numLevels) return HMX_EINVAL;
+    return guard(ctx, [&] {
+        const HLevel& lv = h->levels[level - 1];
+        if (lv.U.n) {
+            NEED(d_U);
+            NEED(d_V);
+            HMX_HIP(hipMemcpyAsync(d_U, lv.U.p, lv.U.bytes(), hipMemcpyDeviceToDevice, ctx->stream));
+            HMX_HIP(hipMemcpyAsync(d_V, lv.V.p, lv.V.bytes(), hipMemcpyDeviceToDevice, ctx->stream));
+        }
+        HMX_HIP(hipStreamSynchronize(ctx->stream));
+    });
+}
+
+hmx_status hmx_hmatrix_set_level(hmx_context* ctx, hmx_hmatrix* h, int32_t level, const double* d_U,
+                                 const double* d_V) {
+    if (!ctx) return HMX_EINVAL;
+    return guard(ctx, [&] {
+        NEED(h);
+        hmatrix_set_level(*ctx, *h, level, d_U, d_V);
+    });
+}
+
+hmx_status hmx_selftest_mfma(hmx_context* ctx, int32_t type, const double* d_A, const double* d_B, double* d_D) {
+    if (!ctx) return HMX_EINVAL;
+    return guard(ctx, [&] {
+        NEED(d_A);
+        NEED(d_B);
+        NEED(d_D);
+        mfma_selftest(*ctx, type, d_A, d_B, d_D);
     });
 }
 

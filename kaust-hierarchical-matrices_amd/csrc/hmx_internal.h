@@ -149,6 +149,55 @@ struct MatvecPlan {
     uint32_t wsK = 0;
 };
 
+// ---- multi right-hand-side (MFMA) matvec -----------------------------------
+// Phase-1 item: up to 64 columns x a chunk of rows of one tile's V
+// (column-major, column stride ld), all right-hand sides at once.
+struct MfItem {
+    const void* v;
+    int32_t xoff, rows, ld, ncols;
+    int64_t out;   // partial slot of the item's first column
+    int32_t nch;   // partial slots per column of this tile
+    int32_t pad;
+};
+// Phase-2 entry: the leaf's rows of one tile's U, stored row-major with the
+// tile's padded rank as row stride.
+struct MfEntry {
+    const void* u;
+    int32_t rpad;
+    int32_t zcol;  // first (padded) column of the tile in the group's z
+};
+
+// One precision group (f64, f32 or f16) of low-rank factors.
+struct MfGroup {
+    int type = 0;                  // 0 f64, 1 f32, 2 f16
+    int64_t numColsPad = 0, numPart = 0;
+    DBuf<unsigned char> V, Urm;    // owned converted storage (V empty when it points into the FP64 levels)
+    DBuf<float> colScale;          // per padded column (f16 group: su*sv), else empty
+    DBuf<MfItem> items;
+    int32_t numItems = 0;
+    DBuf<int64_t> colPart;         // per padded column
+    DBuf<int32_t> colNpart;
+    DBuf<int32_t> leafPtr;         // [S+1]
+    DBuf<MfEntry> entries;
+    uint64_t factorBytes = 0;      // V + U bytes of this group (unpadded entries)
+    DBuf<unsigned char> zpart, zt; // workspaces
+    size_t wsNR = 0;
+};
+
+struct MfPlan {
+    bool built = false;
+    int dtype = 0;                 // 0: FP64 plan; 1: mixed FP32/FP16 plan
+    MfGroup g[2];                  // FP64: g[0]; mixed: g[0] = f32 hi, g[1] = f16 lo
+    DBuf<unsigned char> Drm;       // diagonal blocks, row-major (f64 or f32)
+    DBuf<_Float16> X16;            // converted right-hand sides (mixed)
+    DBuf<float> xscale;
+    size_t x16n = 0;
+    // selection statistics (mixed)
+    double frobH2 = 0, loEnergy = 0, target = 0;
+    int64_t hiCols = 0, loCols = 0;
+    uint64_t bytes = 0;            // factor + dense bytes read by one call
+};
+
 struct HMatrix {
     uint32_t n = 0, d = 0, b = 0, S = 0;
     int32_t numLevels = 0;
@@ -160,6 +209,7 @@ struct HMatrix {
     DBuf<double> Upool, Vpool;
     std::vector<int64_t> levelPoolOff;  // element offset of each level in the pools
     MatvecPlan plan, planT;
+    MfPlan mf64, mfMixed;
     uint64_t lowRankBytes = 0, denseBytes = 0, factorEntries = 0;
 };
 
@@ -177,6 +227,13 @@ void hmatvec(Context& c, HMatrix& h, uint32_t k, const double* d_in, double* d_o
              int iters);
 void dense_rows(Context& c, uint32_t n, uint32_t d, const double* p, const int32_t* perm, double ell, uint32_t nr,
                 const int32_t* rows, uint32_t k, const double* X, double* Y);
+void hmatvec_multi(Context& c, HMatrix& h, uint32_t k, const double* d_in, double* d_out, float* ms, int iters);
+void hmatrix_make_mixed(Context& c, HMatrix& h, double targetRelErr);
+void hmatvec_mixed(Context& c, HMatrix& h, uint32_t k, const float* d_in, float* d_out, float* ms, int iters);
+void mfma_selftest(Context& c, int type, const double* A, const double* B, double* D);
+void hmatvec_grad_factors(Context& c, HMatrix& h, uint32_t k, const double* X, const double* dY, double* const* dU,
+                          double* const* dV);
+void hmatrix_set_level(Context& c, HMatrix& h, int level, const double* U, const double* V);
 // device exclusive scan int32 -> int64 (out has count+1 entries)
 void exclusive_scan_i32_i64(Context& c, const int32_t* in, int64_t* out, size_t count);
 

@@ -1,0 +1,749 @@
+// Multi right-hand-side H-matvec on the matrix cores, FP64 and mixed
+// FP32/FP16 (Y = H X with X n x k, k a multiple of 16).
+//
+//   phase 1  Z = V^T X per tile: one wave per (tile, 64 columns, 1024 rows),
+//            D(16 cols x 16 rhs) += A(16 cols x K rows) B(K rows x 16 rhs);
+//            the K index runs along V's contiguous rows, so every lane loads
+//            one 16/32-byte vector per operand per four MFMAs.
+//   reduce   chunk partials summed in a fixed order, scaled, converted to the
+//            compute type and stored transposed ([rhs][column]).
+//   phase 2  one workgroup per leaf: Y_leaf = D X_leaf + sum_t U_t[leaf rows] Z_t,
+//            A = the leaf's rows of a row-major copy of U, B = transposed Z.
+//
+// Mixed precision: the columns of every tile are split by a global energy
+// budget.  A column j of tile t contributes ||u_j v_j^T||_F = ||v_j|| (U has
+// orthonormal columns).  The columns whose squared norms sum to at most
+// (target / (2*sqrt(2)*2^-11/sqrt(3)))^2 * ||H||_F^2 are stored in FP16 with
+// per-column scales (max-abs of u_j and of v_j) and multiplied on the FP16
+// matrix cores with FP32 accumulation, the rest (and the dense diagonal
+// blocks) in FP32 on the FP32 matrix cores.
+#include <algorithm>
+#include <cmath>
+#include <numeric>
+
+#include "hmx_device.h"
+#include "hmx_internal.h"
+#include "mfma.h"
+
+namespace hmx {
+
+constexpr int kMfRows = 1024;  // rows per phase-1 item
+constexpr int kMfCols = 64;    // columns per phase-1 item
+constexpr int kPad = 16;       // rank padding of the row-major U copies
+
+template <class T, int N>
+__device__ inline void vload(const T* __restrict__ p, T (&out)[N]) {
+    static_assert((N * sizeof(T)) % 16 == 0, "vector loads of whole 16-byte words");
+    const uint4* q = reinterpret_cast<const uint4*>(p);
+    uint4 tmp[N * sizeof(T) / 16];
+#pragma unroll
+    for (int i = 0; i < (int)(N * sizeof(T) / 16); ++i) tmp[i] = q[i];
+    __builtin_memcpy(out, tmp, sizeof(tmp));
+}
+
+template <class T, int N>
+__device__ inline void vzero(T (&out)[N]) {
+#pragma unroll
+    for (int i = 0; i < N; ++i) out[i] = (T)0;
+}
+
+// ---- phase 1 ---------------------------------------------------------------
+template <class T, int CG, int NB>
+__global__ void __launch_bounds__(256) k_mf_phase1(const MfItem* __restrict__ items, int nItems,
+                                                   const T* __restrict__ X, int ldx,
+                                                   typename Mfma<T>::acc_t* __restrict__ zpart) {
+    using M = Mfma<T>;
+    constexpr int KL = M::KL, VL = 4 * KL, NR = 16 * NB;
+    const int lane = threadIdx.x & 63;
+    const int wid = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (wid >= nItems) return;
+    const MfItem it = items[wid];
+    const T* __restrict__ V = (const T*)it.v;
+    const T* __restrict__ Xs = X + it.xoff;
+    const int lg = lane >> 4, lc = lane & 15;
+    typename M::accv acc[CG][NB];
+#pragma unroll
+    for (int g = 0; g < CG; ++g)
+#pragma unroll
+        for (int nb = 0; nb < NB; ++nb) acc[g][nb] = typename M::accv{0, 0, 0, 0};
+    for (int r0 = 0; r0 < it.rows; r0 += 16 * KL) {
+        const int rr = r0 + VL * lg;
+        const bool rok = rr < it.rows;
+        T a[CG][VL], bx[NB][VL];
+#pragma unroll
+        for (int g = 0; g < CG; ++g) {
+            const int col = 16 * g + lc;
+            if (rok && col < it.ncols) vload(V + (size_t)col * it.ld + rr, a[g]);
+            else vzero(a[g]);
+        }
+#pragma unroll
+        for (int nb = 0; nb < NB; ++nb) {
+            if (rok) vload(Xs + (size_t)(16 * nb + lc) * ldx + rr, bx[nb]);
+            else vzero(bx[nb]);
+        }
+#pragma unroll
+        for (int e = 0; e < 4; ++e)
+#pragma unroll
+            for (int g = 0; g < CG; ++g)
+#pragma unroll
+                for (int nb = 0; nb < NB; ++nb)
+                    acc[g][nb] = M::op(M::make(&a[g][e * KL]), M::make(&bx[nb][e * KL]), acc[g][nb]);
+    }
+#pragma unroll
+    for (int g = 0; g < CG; ++g)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            const int col = 16 * g + M::out_row(lg, v);
+            if (col < it.ncols)
+#pragma unroll
+                for (int nb = 0; nb < NB; ++nb)
+                    zpart[(it.out + (int64_t)col * it.nch) * NR + 16 * nb + lc] = acc[g][nb][v];
+        }
+}
+
+// ---- reduce partials, scale, convert, transpose ------------------------------
+template <class T, class ACC>
+__global__ void __launch_bounds__(256) k_mf_reduce(int64_t ncols, int NR, const int64_t* __restrict__ colPart,
+                                                   const int32_t* __restrict__ colNpart,
+                                                   const float* __restrict__ colScale,
+                                                   const float* __restrict__ rhsScale, const ACC* __restrict__ zpart,
+                                                   T* __restrict__ Zt) {
+    const int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (e >= ncols * NR) return;
+    const int s = (int)(e / ncols);
+    const int64_t g = e % ncols;
+    const int np = colNpart[g];
+    const int64_t p0 = colPart[g];
+    ACC v = 0;
+    for (int q = 0; q < np; ++q) v += zpart[(p0 + q) * NR + s];
+    if (colScale) v *= (ACC)colScale[g];
+    if (rhsScale) v *= (ACC)rhsScale[s];
+    Zt[(size_t)s * ncols + g] = (T)v;
+}
+
+// ---- phase 2 -----------------------------------------------------------------
+// 4 waves: wave w owns rows 16*(w&1)..+16 of the leaf and every other
+// super-step of the K (column) loop, (w>>1) picks which; partials are folded
+// through LDS at the end.
+template <class TD, class TA, class TB, int NB, class TO>
+__global__ void __launch_bounds__(256) k_mf_phase2(const int32_t* __restrict__ ptrA, const MfEntry* __restrict__ entA,
+                                                   const TA* __restrict__ ZA, int64_t ldzA,
+                                                   const int32_t* __restrict__ ptrB, const MfEntry* __restrict__ entB,
+                                                   const TB* __restrict__ ZB, int64_t ldzB,
+                                                   const TD* __restrict__ Drm, const TD* __restrict__ X, int ldx,
+                                                   TO* __restrict__ Y, int ldy) {
+    using MA = Mfma<TA>;
+    using MB = Mfma<TB>;
+    using MD = Mfma<TD>;
+    using acc_t = typename MA::acc_t;
+    using accv = typename MA::accv;
+    static_assert(sizeof(typename MB::acc_t) == sizeof(acc_t) && sizeof(typename MD::acc_t) == sizeof(acc_t),
+                  "one accumulator type per launch");
+    constexpr int NR = 16 * NB;
+    __shared__ acc_t red[2][32][NR + 1];
+    const int leaf = blockIdx.x;
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int rg = w & 1, kp = w >> 1, lg = lane >> 4, lc = lane & 15;
+    const int row = 16 * rg + lc;
+    accv acc[NB];
+#pragma unroll
+    for (int nb = 0; nb < NB; ++nb) acc[nb] = accv{0, 0, 0, 0};
+    {  // dense diagonal block: 32 columns, two super-steps of 16, kp picks one
+        const int cc = 16 * kp + 4 * lg;
+        TD a[4];
+        vload(Drm + (size_t)leaf * 1024 + row * 32 + cc, a);
+#pragma unroll
+        for (int nb = 0; nb < NB; ++nb) {
+            TD bx[4];
+            vload(X + (size_t)(16 * nb + lc) * ldx + (size_t)leaf * 32 + cc, bx);
+#pragma unroll
+            for (int e = 0; e < 4; ++e) acc[nb] = MD::op(a[e], bx[e], acc[nb]);
+        }
+    }
+    {
+        constexpr int KL = MA::KL, VL = 4 * KL, SS = 16 * KL;
+        for (int e = ptrA[leaf]; e < ptrA[leaf + 1]; ++e) {
+            const MfEntry en = entA[e];
+            const TA* __restrict__ U = (const TA*)en.u + (size_t)row * en.rpad;
+            const TA* __restrict__ Z = ZA + en.zcol;
+            for (int c0 = kp * SS; c0 < en.rpad; c0 += 2 * SS) {
+                const int cc = c0 + VL * lg;
+                TA a[VL], bz[NB][VL];
+                if (cc < en.rpad) {
+                    vload(U + cc, a);
+#pragma unroll
+                    for (int nb = 0; nb < NB; ++nb) vload(Z + (size_t)(16 * nb + lc) * ldzA + cc, bz[nb]);
+                } else {
+                    vzero(a);
+#pragma unroll
+                    for (int nb = 0; nb < NB; ++nb) vzero(bz[nb]);
+                }
+#pragma unroll
+                for (int q = 0; q < 4; ++q)
+#pragma unroll
+                    for (int nb = 0; nb < NB; ++nb)
+                        acc[nb] = MA::op(MA::make(&a[q * KL]), MA::make(&bz[nb][q * KL]), acc[nb]);
+            }
+        }
+    }
+    if (ptrB) {
+        constexpr int KL = MB::KL, VL = 4 * KL, SS = 16 * KL;
+        for (int e = ptrB[leaf]; e < ptrB[leaf + 1]; ++e) {
+            const MfEntry en = entB[e];
+            const TB* __restrict__ U = (const TB*)en.u + (size_t)row * en.rpad;
+            const TB* __restrict__ Z = ZB + en.zcol;
+            for (int c0 = kp * SS; c0 < en.rpad; c0 += 2 * SS) {
+                const int cc = c0 + VL * lg;
+                TB a[VL], bz[NB][VL];
+                if (cc < en.rpad) {
+                    vload(U + cc, a);
+#pragma unroll
+                    for (int nb = 0; nb < NB; ++nb) vload(Z + (size_t)(16 * nb + lc) * ldzB + cc, bz[nb]);
+                } else {
+                    vzero(a);
+#pragma unroll
+                    for (int nb = 0; nb < NB; ++nb) vzero(bz[nb]);
+                }
+#pragma unroll
+                for (int q = 0; q < 4; ++q)
+#pragma unroll
+                    for (int nb = 0; nb < NB; ++nb)
+                        acc[nb] = MB::op(MB::make(&a[q * KL]), MB::make(&bz[nb][q * KL]), acc[nb]);
+            }
+        }
+    }
+#pragma unroll
+    for (int nb = 0; nb < NB; ++nb)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) red[kp][16 * rg + MA::out_row(lg, v)][16 * nb + lc] = acc[nb][v];
+    __syncthreads();
+    for (int o = threadIdx.x; o < 32 * NR; o += 256) {
+        const int s = o >> 5, r = o & 31;
+        Y[(size_t)s * ldy + (size_t)leaf * 32 + r] = (TO)(red[0][r][s] + red[1][r][s]);
+    }
+}
+
+// ---- right-hand-side conversion (mixed) ---------------------------------------
+__global__ void __launch_bounds__(256) k_colmaxabs(int n, const float* __restrict__ X, float* __restrict__ sx) {
+    __shared__ float red[256];
+    const float* x = X + (size_t)blockIdx.x * n;
+    float m = 0.f;
+    for (int i = threadIdx.x; i < n; i += 256) m = fmaxf(m, fabsf(x[i]));
+    red[threadIdx.x] = m;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (threadIdx.x < (unsigned)w) red[threadIdx.x] = fmaxf(red[threadIdx.x], red[threadIdx.x + w]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) sx[blockIdx.x] = red[0] > 0.f ? red[0] : 1.f;
+}
+
+__global__ void __launch_bounds__(256) k_to_f16(int64_t total, int n, const float* __restrict__ X,
+                                                const float* __restrict__ sx, _Float16* __restrict__ X16) {
+    const int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (e >= total) return;
+    X16[e] = (_Float16)(X[e] / sx[e / n]);
+}
+
+// ---- plan construction kernels -----------------------------------------------
+struct ColDesc {
+    const double* u;
+    const double* v;
+    int32_t m, pad;
+};
+
+// per column: ||v||^2, max|u|, max|v|  (one wave per column)
+__global__ void __launch_bounds__(256) k_colstats(int64_t ncols, const ColDesc* __restrict__ cols,
+                                                  double* __restrict__ v2, double* __restrict__ umax,
+                                                  double* __restrict__ vmax) {
+    const int64_t c = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (c >= ncols) return;
+    const int lane = threadIdx.x & 63;
+    const ColDesc d = cols[c];
+    double s = 0, mu = 0, mv = 0;
+    for (int i = lane; i < d.m; i += 64) {
+        const double u = d.u[i], v = d.v[i];
+        s += v * v;
+        mu = fmax(mu, fabs(u));
+        mv = fmax(mv, fabs(v));
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        s += __shfl_xor(s, off, 64);
+        mu = fmax(mu, __shfl_xor(mu, off, 64));
+        mv = fmax(mv, __shfl_xor(mv, off, 64));
+    }
+    if (lane == 0) { v2[c] = s; umax[c] = mu; vmax[c] = mv; }
+}
+
+// V column gather + scale + convert: dst column c = src / scale (one wave per column)
+struct CvCol {
+    const double* src;
+    int64_t dst;   // element offset in the group's V buffer
+    int32_t m;
+    float scale;
+};
+template <class T>
+__global__ void __launch_bounds__(256) k_cvt_cols(int64_t n, const CvCol* __restrict__ cols, T* __restrict__ V) {
+    const int64_t c = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (c >= n) return;
+    const int lane = threadIdx.x & 63;
+    const CvCol d = cols[c];
+    const double inv = 1.0 / (double)d.scale;
+    for (int i = lane; i < d.m; i += 64) V[d.dst + i] = (T)(d.src[i] * inv);
+}
+
+// row-major U copy with zero padding: one block per (tile, 32-row tile, 32-col tile)
+struct TrTile {
+    int64_t colDesc;  // first CvCol of the tile (column c of the tile = colDesc + c)
+    int64_t dst;      // element offset of the tile's row-major copy
+    int32_t m, cnt, rpad, r0, c0, pad;
+};
+template <class T>
+__global__ void __launch_bounds__(256) k_cvt_rowmajor(int64_t n, const TrTile* __restrict__ tiles,
+                                                      const CvCol* __restrict__ cols, T* __restrict__ U) {
+    __shared__ double sh[32][33];
+    const TrTile t = tiles[blockIdx.x];
+    const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;  // 32 x 8
+    for (int cc = ty; cc < 32; cc += 8) {  // read 32 columns (contiguous rows)
+        const int c = t.c0 + cc, r = t.r0 + tx;
+        double v = 0.0;
+        if (c < t.cnt && r < t.m) {
+            const CvCol d = cols[t.colDesc + c];
+            v = d.src[r] / (double)d.scale;
+        }
+        sh[cc][tx] = v;
+    }
+    __syncthreads();
+    for (int rr = ty; rr < 32; rr += 8) {  // write 32 rows (contiguous columns)
+        const int r = t.r0 + rr, c = t.c0 + tx;
+        if (r < t.m && c < t.rpad) U[t.dst + (int64_t)r * t.rpad + c] = (T)sh[tx][rr];
+    }
+}
+
+template <class T>
+__global__ void __launch_bounds__(256) k_diag_rowmajor(int64_t total, const double* __restrict__ D, T* __restrict__ Drm) {
+    const int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (e >= total) return;
+    const int64_t leaf = e >> 10;
+    const int r = (int)((e >> 5) & 31), c = (int)(e & 31);
+    Drm[e] = (T)D[(leaf << 10) + (int64_t)c * 32 + r];  // D is column-major per leaf
+}
+
+// ---- host side ------------------------------------------------------------------
+namespace {
+
+struct TileRef {
+    const HLevel* lv;
+    int level, t, r, m;
+    uint32_t inCl, outCl;
+};
+
+std::vector<TileRef> all_tiles(const HMatrix& h) {
+    std::vector<TileRef> v;
+    for (int l = 1; l < h.numLevels; ++l) {
+        const HLevel& lv = h.levels[l - 1];
+        const auto& tiles = h.structure->tileIndices[l - 1];
+        for (int t = 0; t < lv.numTiles; ++t) {
+            if (lv.h_ranks[t] == 0) continue;
+            v.push_back(TileRef{&lv, l, t, lv.h_ranks[t], lv.m, compact16((uint32_t)tiles[t]),
+                                compact16((uint32_t)tiles[t] >> 1)});
+        }
+    }
+    return v;
+}
+
+template <class T>
+void upload(Context& c, DBuf<T>& d, const std::vector<T>& h) {
+    d.alloc(std::max<size_t>(1, h.size()));
+    if (!h.empty()) HMX_HIP(hipMemcpyAsync(d.p, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice, c.stream));
+}
+
+size_t tsize(int type) { return type == 0 ? 8 : (type == 1 ? 4 : 2); }
+
+// Build one group from a per-tile list of source columns.  When 'vdirect'
+// is set the group's V is the FP64 level storage itself (no copy).
+struct GroupTile {
+    TileRef ref;
+    std::vector<int> cols;  // source column indices in the tile
+};
+
+void build_group(Context& c, HMatrix& h, MfGroup& G, int type, const std::vector<GroupTile>& tiles, bool vdirect,
+                 const std::vector<float>* su, const std::vector<float>* sv,
+                 const std::vector<int64_t>* colBase) {
+    const int b = (int)h.b;
+    const int S = (int)h.S;
+    const size_t es = tsize(type);
+    G.type = type;
+    std::vector<MfItem> items;
+    std::vector<int64_t> colPart;
+    std::vector<int32_t> colNpart;
+    std::vector<float> colScale;
+    std::vector<std::vector<MfEntry>> leafE(S);
+    std::vector<CvCol> vcols, ucols;
+    std::vector<TrTile> trt;
+    int64_t zcol = 0, slot = 0, vElems = 0, uElems = 0;
+    uint64_t fbytes = 0;
+    for (const GroupTile& gt : tiles) {
+        const TileRef& R = gt.ref;
+        const int cnt = (int)gt.cols.size();
+        if (!cnt) continue;
+        const int m = R.m;
+        const int rpad = (cnt + kPad - 1) / kPad * kPad;
+        const double* Usrc = R.lv->U.p + (size_t)R.lv->h_offsets[R.t] * m;
+        const double* Vsrc = R.lv->V.p + (size_t)R.lv->h_offsets[R.t] * m;
+        const int64_t cd0 = (int64_t)ucols.size();
+        for (int q = 0; q < cnt; ++q) {
+            const int j = gt.cols[q];
+            float fu = 1.f, fv = 1.f;
+            if (su) {
+                fu = (*su)[(*colBase)[&gt - &tiles[0]] + j];
+                fv = (*sv)[(*colBase)[&gt - &tiles[0]] + j];
+            }
+            ucols.push_back(CvCol{Usrc + (size_t)j * m, 0, m, fu});
+            if (!vdirect) vcols.push_back(CvCol{Vsrc + (size_t)j * m, vElems + (int64_t)q * m, m, fv});
+            colScale.push_back(fu * fv);
+        }
+        for (int q = cnt; q < rpad; ++q) colScale.push_back(0.f);
+        for (int r0 = 0; r0 < m; r0 += 32)
+            for (int c0 = 0; c0 < rpad; c0 += 32) trt.push_back(TrTile{cd0, uElems, m, cnt, rpad, r0, c0, 0});
+        // phase-1 items
+        const int nch = (m + kMfRows - 1) / kMfRows;
+        const int rows = std::min(m, kMfRows);
+        const char* vbase = vdirect ? (const char*)Vsrc : nullptr;  // patched after allocation otherwise
+        for (int c0 = 0; c0 < cnt; c0 += kMfCols)
+            for (int q = 0; q < nch; ++q) {
+                MfItem it;
+                it.v = vdirect ? (const void*)(vbase + ((size_t)c0 * m + (size_t)q * kMfRows) * es)
+                               : (const void*)(uintptr_t)(((size_t)vElems + (size_t)c0 * m + (size_t)q * kMfRows) * es);
+                it.xoff = (int32_t)(R.inCl * m + q * kMfRows);
+                it.rows = rows;
+                it.ld = m;
+                it.ncols = std::min(kMfCols, cnt - c0);
+                it.out = slot + (int64_t)c0 * nch + q;
+                it.nch = nch;
+                it.pad = 0;
+                items.push_back(it);
+            }
+        for (int q = 0; q < rpad; ++q) {
+            colPart.push_back(q < cnt ? slot + (int64_t)q * nch : 0);
+            colNpart.push_back(q < cnt ? nch : 0);
+        }
+        // phase-2 entries
+        const int bus = m / b;
+        for (int q = 0; q < bus; ++q)
+            leafE[R.outCl * bus + q].push_back(
+                MfEntry{(const void*)(uintptr_t)(((size_t)uElems + (size_t)q * b * rpad) * es), rpad, (int32_t)zcol});
+        slot += (int64_t)cnt * nch;
+        zcol += rpad;
+        vElems += (int64_t)cnt * m;
+        uElems += (int64_t)m * rpad;
+        fbytes += 2ull * (uint64_t)cnt * m * es;
+    }
+    G.numColsPad = zcol;
+    G.numPart = slot;
+    G.factorBytes = fbytes;
+    // storage
+    if (!vdirect) G.V.alloc(std::max<int64_t>(1, vElems) * es);
+    G.Urm.alloc(std::max<int64_t>(1, uElems) * es);
+    for (auto& it : items)
+        if (!vdirect) it.v = (const void*)(G.V.p + (uintptr_t)it.v);
+    std::vector<int32_t> leafPtr(S + 1, 0);
+    std::vector<MfEntry> ents;
+    for (int leaf = 0; leaf < S; ++leaf) {
+        leafPtr[leaf] = (int32_t)ents.size();
+        for (MfEntry e : leafE[leaf]) {
+            e.u = (const void*)(G.Urm.p + (uintptr_t)e.u);
+            ents.push_back(e);
+        }
+    }
+    leafPtr[S] = (int32_t)ents.size();
+    G.numItems = (int32_t)items.size();
+    upload(c, G.items, items);
+    upload(c, G.colPart, colPart);
+    upload(c, G.colNpart, colNpart);
+    upload(c, G.leafPtr, leafPtr);
+    upload(c, G.entries, ents);
+    if (su) upload(c, G.colScale, colScale);
+    DBuf<CvCol> d_u, d_v;
+    DBuf<TrTile> d_t;
+    upload(c, d_u, ucols);
+    upload(c, d_v, vcols);
+    upload(c, d_t, trt);
+    if (!vcols.empty()) {
+        const unsigned g = (unsigned)((vcols.size() + 3) / 4);
+        if (type == 0) k_cvt_cols<double><<<g, 256, 0, c.stream>>>((int64_t)vcols.size(), d_v.p, (double*)G.V.p);
+        else if (type == 1) k_cvt_cols<float><<<g, 256, 0, c.stream>>>((int64_t)vcols.size(), d_v.p, (float*)G.V.p);
+        else k_cvt_cols<_Float16><<<g, 256, 0, c.stream>>>((int64_t)vcols.size(), d_v.p, (_Float16*)G.V.p);
+        HMX_LAUNCHED();
+    }
+    if (!trt.empty()) {
+        const unsigned g = (unsigned)trt.size();
+        if (type == 0) k_cvt_rowmajor<double><<<g, 256, 0, c.stream>>>((int64_t)trt.size(), d_t.p, d_u.p, (double*)G.Urm.p);
+        else if (type == 1) k_cvt_rowmajor<float><<<g, 256, 0, c.stream>>>((int64_t)trt.size(), d_t.p, d_u.p, (float*)G.Urm.p);
+        else k_cvt_rowmajor<_Float16><<<g, 256, 0, c.stream>>>((int64_t)trt.size(), d_t.p, d_u.p, (_Float16*)G.Urm.p);
+        HMX_LAUNCHED();
+    }
+    HMX_HIP(hipStreamSynchronize(c.stream));
+}
+
+void check_shape(const HMatrix& h) {
+    HMX_CHECK(h.n == h.S * h.b, HMX_EUNSUPPORTED, "the multi-RHS matvec requires full leaves (n = numSegments * 32)");
+    HMX_CHECK(h.b == 32, HMX_EUNSUPPORTED, "the multi-RHS matvec kernels are built for bucketSize 32");
+}
+
+void build_fp64_plan(Context& c, HMatrix& h) {
+    check_shape(h);
+    MfPlan& P = h.mf64;
+    std::vector<GroupTile> tiles;
+    for (const TileRef& R : all_tiles(h)) {
+        GroupTile gt{R, {}};
+        gt.cols.resize(R.r);
+        std::iota(gt.cols.begin(), gt.cols.end(), 0);
+        tiles.push_back(std::move(gt));
+    }
+    build_group(c, h, P.g[0], 0, tiles, true, nullptr, nullptr, nullptr);
+    const int64_t tot = (int64_t)h.S * 1024;
+    P.Drm.alloc(tot * sizeof(double));
+    k_diag_rowmajor<double><<<(unsigned)((tot + 255) / 256), 256, 0, c.stream>>>(tot, h.diag.p, (double*)P.Drm.p);
+    HMX_LAUNCHED();
+    HMX_HIP(hipStreamSynchronize(c.stream));
+    P.dtype = 0;
+    P.bytes = P.g[0].factorBytes + (uint64_t)tot * 8;
+    P.built = true;
+}
+
+template <class ACC, class T>
+void reduce_group(Context& c, MfGroup& G, int NR, const float* rhsScale) {
+    if (G.numColsPad == 0) return;
+    const int64_t tot = G.numColsPad * NR;
+    k_mf_reduce<T, ACC><<<(unsigned)((tot + 255) / 256), 256, 0, c.stream>>>(
+        G.numColsPad, NR, G.colPart.p, G.colNpart.p, G.colScale.n ? G.colScale.p : nullptr, rhsScale,
+        (const ACC*)G.zpart.p, (T*)G.zt.p);
+    HMX_LAUNCHED();
+}
+
+void ensure_ws(MfGroup& G, int NR, size_t accBytes) {
+    if (G.wsNR >= (size_t)NR) return;
+    G.zpart.alloc((size_t)std::max<int64_t>(1, G.numPart) * NR * accBytes);
+    G.zt.alloc((size_t)std::max<int64_t>(1, G.numColsPad) * NR * tsize(G.type));
+    G.wsNR = NR;
+}
+
+template <class T, int CG, int NB>
+void phase1(Context& c, MfGroup& G, const T* X, int ldx) {
+    if (G.numItems == 0) return;
+    k_mf_phase1<T, CG, NB><<<(unsigned)((G.numItems + 3) / 4), 256, 0, c.stream>>>(
+        G.items.p, G.numItems, X, ldx, (typename Mfma<T>::acc_t*)G.zpart.p);
+    HMX_LAUNCHED();
+}
+
+struct Events {
+    hipEvent_t e[4] = {nullptr, nullptr, nullptr, nullptr};
+    bool on = false;
+    void create() {
+        for (auto& x : e) HMX_HIP(hipEventCreate(&x));
+        on = true;
+    }
+    void rec(Context& c, int i) {
+        if (on) HMX_HIP(hipEventRecord(e[i], c.stream));
+    }
+    ~Events() {
+        for (auto& x : e)
+            if (x) (void)hipEventDestroy(x);
+    }
+};
+
+template <int NB>
+void run_fp64(Context& c, HMatrix& h, const double* X, double* Y, Events& ev) {
+    MfPlan& P = h.mf64;
+    MfGroup& G = P.g[0];
+    const int ldx = (int)h.n;
+    ev.rec(c, 0);
+    phase1<double, 4, NB>(c, G, X, ldx);  // 4 column groups = the 64 columns of an item
+    ev.rec(c, 1);
+    reduce_group<double, double>(c, G, 16 * NB, nullptr);
+    ev.rec(c, 2);
+    k_mf_phase2<double, double, double, NB, double><<<h.S, 256, 0, c.stream>>>(
+        G.leafPtr.p, G.entries.p, (const double*)G.zt.p, G.numColsPad, nullptr, nullptr, nullptr, 0,
+        (const double*)P.Drm.p, X, ldx, Y, ldx);
+    HMX_LAUNCHED();
+    ev.rec(c, 3);
+}
+
+template <int NB>
+void run_mixed(Context& c, HMatrix& h, const float* X, float* Y, Events& ev) {
+    MfPlan& P = h.mfMixed;
+    MfGroup& Ghi = P.g[0];
+    MfGroup& Glo = P.g[1];
+    const int n = (int)h.n, NR = 16 * NB;
+    ev.rec(c, 0);
+    // right-hand sides to FP16 with one scale per column
+    k_colmaxabs<<<NR, 256, 0, c.stream>>>(n, X, P.xscale.p);
+    HMX_LAUNCHED();
+    const int64_t tot = (int64_t)n * NR;
+    k_to_f16<<<(unsigned)((tot + 255) / 256), 256, 0, c.stream>>>(tot, n, X, P.xscale.p, P.X16.p);
+    HMX_LAUNCHED();
+    phase1<float, 4, NB>(c, Ghi, X, n);
+    phase1<_Float16, 4, NB>(c, Glo, P.X16.p, n);
+    ev.rec(c, 1);
+    reduce_group<float, float>(c, Ghi, NR, nullptr);
+    reduce_group<float, _Float16>(c, Glo, NR, P.xscale.p);
+    ev.rec(c, 2);
+    k_mf_phase2<float, float, _Float16, NB, float><<<h.S, 256, 0, c.stream>>>(
+        Ghi.leafPtr.p, Ghi.entries.p, (const float*)Ghi.zt.p, Ghi.numColsPad, Glo.leafPtr.p, Glo.entries.p,
+        (const _Float16*)Glo.zt.p, Glo.numColsPad, (const float*)P.Drm.p, X, n, Y, n);
+    HMX_LAUNCHED();
+    ev.rec(c, 3);
+}
+
+void finish_timing(Events& ev, float* acc) {
+    if (!ev.on) return;
+    HMX_HIP(hipEventSynchronize(ev.e[3]));
+    float t;
+    for (int q = 0; q < 3; ++q) {
+        HMX_HIP(hipEventElapsedTime(&t, ev.e[q], ev.e[q + 1]));
+        acc[q] += t;
+    }
+    HMX_HIP(hipEventElapsedTime(&t, ev.e[0], ev.e[3]));
+    acc[3] += t;
+}
+
+}  // namespace
+
+void hmatvec_multi(Context& c, HMatrix& h, uint32_t k, const double* d_in, double* d_out, float* ms, int iters) {
+    HMX_CHECK(k >= 16 && k % 16 == 0, HMX_EINVAL, "multi-RHS matvec: vectorWidth must be a multiple of 16");
+    HMX_CHECK(d_in && d_out && d_in != d_out, HMX_EINVAL, "bad vector pointers");
+    if (!h.mf64.built) build_fp64_plan(c, h);
+    MfPlan& P = h.mf64;
+    const int NB = (k % 64 == 0) ? 4 : 1;
+    ensure_ws(P.g[0], 16 * NB, sizeof(double));
+    Events ev;
+    if (ms) ev.create();
+    const int reps = ms ? std::max(1, iters) : 1;
+    float acc[4] = {0, 0, 0, 0};
+    for (int it = 0; it < reps; ++it) {
+        for (uint32_t s0 = 0; s0 < k; s0 += 16 * NB) {
+            const double* X = d_in + (size_t)s0 * h.n;
+            double* Y = d_out + (size_t)s0 * h.n;
+            if (NB == 4) run_fp64<4>(c, h, X, Y, ev);
+            else run_fp64<1>(c, h, X, Y, ev);
+        }
+        finish_timing(ev, acc);
+    }
+    if (ms)
+        for (int q = 0; q < 4; ++q) ms[q] = acc[q] / reps;
+}
+
+void hmatrix_make_mixed(Context& c, HMatrix& h, double target) {
+    check_shape(h);
+    HMX_CHECK(target > 0 && target < 1, HMX_EINVAL, "target relative error must be in (0, 1)");
+    MfPlan& P = h.mfMixed;
+    P = MfPlan();
+    std::vector<TileRef> tiles = all_tiles(h);
+    // per-column statistics
+    std::vector<ColDesc> cd;
+    std::vector<int64_t> colBase;
+    for (const TileRef& R : tiles) {
+        colBase.push_back((int64_t)cd.size());
+        const double* U = R.lv->U.p + (size_t)R.lv->h_offsets[R.t] * R.m;
+        const double* V = R.lv->V.p + (size_t)R.lv->h_offsets[R.t] * R.m;
+        for (int j = 0; j < R.r; ++j) cd.push_back(ColDesc{U + (size_t)j * R.m, V + (size_t)j * R.m, R.m, 0});
+    }
+    const int64_t nc = (int64_t)cd.size();
+    std::vector<double> v2(nc), um(nc), vm(nc);
+    if (nc) {
+        DBuf<ColDesc> d_cd;
+        upload(c, d_cd, cd);
+        DBuf<double> d_v2(nc), d_um(nc), d_vm(nc);
+        k_colstats<<<(unsigned)((nc + 3) / 4), 256, 0, c.stream>>>(nc, d_cd.p, d_v2.p, d_um.p, d_vm.p);
+        HMX_LAUNCHED();
+        HMX_HIP(hipMemcpyAsync(v2.data(), d_v2.p, nc * 8, hipMemcpyDeviceToHost, c.stream));
+        HMX_HIP(hipMemcpyAsync(um.data(), d_um.p, nc * 8, hipMemcpyDeviceToHost, c.stream));
+        HMX_HIP(hipMemcpyAsync(vm.data(), d_vm.p, nc * 8, hipMemcpyDeviceToHost, c.stream));
+        HMX_HIP(hipStreamSynchronize(c.stream));
+    }
+    // ||H||_F^2 = sum of column energies + dense blocks
+    std::vector<double> hd(h.diag.n);
+    HMX_HIP(hipMemcpy(hd.data(), h.diag.p, h.diag.bytes(), hipMemcpyDeviceToHost));
+    double frob = 0;
+    for (double x : hd) frob += x * x;
+    for (double x : v2) frob += x;
+    // FP16 storage error of a column: rounding of u, v, x and z (rms 2^-11/sqrt(3) each)
+    const double delta = std::ldexp(1.0, -11) / std::sqrt(3.0);
+    const double budget = (target / (2.0 * delta)) * (target / (2.0 * delta)) * frob;
+    std::vector<int64_t> order(nc);
+    std::iota(order.begin(), order.end(), 0);
+    std::sort(order.begin(), order.end(), [&](int64_t a, int64_t b2) { return v2[a] < v2[b2]; });
+    std::vector<char> lo(nc, 0);
+    double used = 0;
+    for (int64_t q = 0; q < nc; ++q) {
+        const int64_t j = order[q];
+        if (used + v2[j] > budget) break;
+        used += v2[j];
+        lo[j] = 1;
+    }
+    std::vector<float> su(nc), sv(nc);
+    for (int64_t j = 0; j < nc; ++j) {
+        su[j] = um[j] > 0 ? (float)um[j] : 1.f;
+        sv[j] = vm[j] > 0 ? (float)vm[j] : 1.f;
+    }
+    std::vector<GroupTile> gh, gl;
+    int64_t hi = 0, lc = 0;
+    for (size_t t = 0; t < tiles.size(); ++t) {
+        GroupTile a{tiles[t], {}}, b2{tiles[t], {}};
+        for (int j = 0; j < tiles[t].r; ++j) (lo[colBase[t] + j] ? b2.cols : a.cols).push_back(j);
+        hi += (int64_t)a.cols.size();
+        lc += (int64_t)b2.cols.size();
+        gh.push_back(std::move(a));
+        gl.push_back(std::move(b2));
+    }
+    build_group(c, h, P.g[0], 1, gh, false, nullptr, nullptr, nullptr);
+    build_group(c, h, P.g[1], 2, gl, false, &su, &sv, &colBase);
+    const int64_t tot = (int64_t)h.S * 1024;
+    P.Drm.alloc(tot * sizeof(float));
+    k_diag_rowmajor<float><<<(unsigned)((tot + 255) / 256), 256, 0, c.stream>>>(tot, h.diag.p, (float*)P.Drm.p);
+    HMX_LAUNCHED();
+    HMX_HIP(hipStreamSynchronize(c.stream));
+    P.dtype = 1;
+    P.frobH2 = frob;
+    P.loEnergy = used;
+    P.target = target;
+    P.hiCols = hi;
+    P.loCols = lc;
+    P.bytes = P.g[0].factorBytes + P.g[1].factorBytes + (uint64_t)tot * 4 + (uint64_t)lc * 4;
+    P.built = true;
+}
+
+void hmatvec_mixed(Context& c, HMatrix& h, uint32_t k, const float* d_in, float* d_out, float* ms, int iters) {
+    HMX_CHECK(h.mfMixed.built, HMX_ESTATE, "mixed-precision factors not built (call hmx_hmatrix_make_mixed)");
+    HMX_CHECK(k >= 16 && k % 16 == 0, HMX_EINVAL, "mixed matvec: vectorWidth must be a multiple of 16");
+    HMX_CHECK(d_in && d_out && d_in != d_out, HMX_EINVAL, "bad vector pointers");
+    MfPlan& P = h.mfMixed;
+    const int NB = (k % 64 == 0) ? 4 : 1;
+    const int NR = 16 * NB;
+    ensure_ws(P.g[0], NR, sizeof(float));
+    ensure_ws(P.g[1], NR, sizeof(float));
+    if (P.x16n < (size_t)h.n * NR) {
+        P.X16.alloc((size_t)h.n * NR);
+        P.xscale.alloc(NR);
+        P.x16n = (size_t)h.n * NR;
+    }
+    Events ev;
+    if (ms) ev.create();
+    const int reps = ms ? std::max(1, iters) : 1;
+    float acc[4] = {0, 0, 0, 0};
+    for (int it = 0; it < reps; ++it) {
+        for (uint32_t s0 = 0; s0 < k; s0 += NR) {
+            const float* X = d_in + (size_t)s0 * h.n;
+            float* Y = d_out + (size_t)s0 * h.n;
+            if (NB == 4) run_mixed<4>(c, h, X, Y, ev);
+            else run_mixed<1>(c, h, X, Y, ev);
+        }
+        finish_timing(ev, acc);
+    }
+    if (ms)
+        for (int q = 0; q < 4; ++q) ms[q] = acc[q] / reps;
+}
+
+}  // namespace hmx

@@ -93,6 +93,14 @@ _SIGS = [
     ("hmx_HMatrixVecMultTranspose", _i32, [_vp, _vp, _u32, _vp, _vp]),
     ("hmx_matvec_profile", _i32, [_vp, _vp, _u32, _vp, _vp, _i32, _vp]),
     ("hmx_denseRowsXVector", _i32, [_vp, _u32, _u32, _vp, _vp, _f64, _u32, _vp, _u32, _vp, _vp]),
+    ("hmx_matvec_profile_multi", _i32, [_vp, _vp, _u32, _vp, _vp, _i32, _i32, _vp]),
+    ("hmx_hmatrix_make_mixed", _i32, [_vp, _vp, _f64]),
+    ("hmx_hmatrix_mixed_info", _i32, [_vp, _P(_i64), _P(_i64), _P(_f64), _P(_f64), _P(_u64)]),
+    ("hmx_HMatrixVecMultMixed", _i32, [_vp, _vp, _u32, _vp, _vp]),
+    ("hmx_selftest_mfma", _i32, [_vp, _i32, _vp, _vp, _vp]),
+    ("hmx_hmatvec_grad_factors", _i32, [_vp, _vp, _u32, _vp, _vp, _vp, _vp]),
+    ("hmx_hmatrix_level_get", _i32, [_vp, _vp, _i32, _vp, _vp]),
+    ("hmx_hmatrix_set_level", _i32, [_vp, _vp, _i32, _vp, _vp]),
 ]
 
 
@@ -154,9 +162,12 @@ class Context:
         self.handle = h
 
     def __del__(self):
-        if getattr(self, "handle", None):
-            lib().hmx_destroy(self.handle)
-            self.handle = None
+        try:
+            if getattr(self, "handle", None):
+                lib().hmx_destroy(self.handle)
+                self.handle = None
+        except Exception:  # interpreter shutdown
+            pass
 
 
 class _Obj:
@@ -166,9 +177,12 @@ class _Obj:
         self.handle = handle
 
     def __del__(self):
-        if getattr(self, "handle", None) and self._free:
-            getattr(lib(), self._free)(self.handle)
-            self.handle = None
+        try:
+            if getattr(self, "handle", None) and self._free:
+                getattr(lib(), self._free)(self.handle)
+                self.handle = None
+        except Exception:  # interpreter shutdown
+            pass
 
 
 class KDTree(_Obj):
@@ -369,6 +383,28 @@ def matvec_profile(ctx, H, k, x, y, iters=20):
     return ms
 
 
+def matvec_profile_multi(ctx, H, k, x, y, iters=20, mixed=False):
+    ms = np.zeros(4, dtype=np.float32)
+    _check(lib().hmx_matvec_profile_multi(ctx.handle, H.handle, k, _ptr(x), _ptr(y), iters, int(mixed), _np(ms)), ctx)
+    return ms
+
+
+def make_mixed(ctx, H, target=3e-6):
+    _check(lib().hmx_hmatrix_make_mixed(ctx.handle, H.handle, target), ctx)
+    hi, lo, fr, le, by = _i64(), _i64(), _f64(), _f64(), _u64()
+    _check(lib().hmx_hmatrix_mixed_info(H.handle, ctypes.byref(hi), ctypes.byref(lo), ctypes.byref(fr),
+                                        ctypes.byref(le), ctypes.byref(by)))
+    return dict(hiCols=hi.value, loCols=lo.value, frobH2=fr.value, loEnergy=le.value, bytes=by.value)
+
+
+def HMatrixVecMultMixed(ctx, H, k, x, y):
+    _check(lib().hmx_HMatrixVecMultMixed(ctx.handle, H.handle, k, _ptr(x), _ptr(y)), ctx)
+
+
+def selftest_mfma(ctx, type_, A, B, D):
+    _check(lib().hmx_selftest_mfma(ctx.handle, type_, _ptr(A), _ptr(B), _ptr(D)), ctx)
+
+
 def denseRowsXVector(ctx, n, d, points, perm_ptr, ell, rows, k, X, Y):
     _check(lib().hmx_denseRowsXVector(ctx.handle, n, d, _ptr(points), ctypes.c_void_p(perm_ptr), ell, rows.numel(),
                                       _ptr(rows), k, _ptr(X), _ptr(Y)), ctx)
@@ -414,6 +450,18 @@ class Pipeline:
         self.timings = t
         if timings is not None:
             timings.update(t)
+
+    def make_mixed(self, target=3e-6):
+        self.mixed_info = make_mixed(self.ctx, self.H, target)
+        return self.mixed_info
+
+    def matvec_mixed(self, x, y=None):
+        """x: (k, n) float32 contiguous, k a multiple of 16."""
+        import torch
+        if y is None:
+            y = torch.empty_like(x)
+        HMatrixVecMultMixed(self.ctx, self.H, x.shape[0], x, y)
+        return y
 
     def matvec(self, x, y=None, transpose=False):
         import torch

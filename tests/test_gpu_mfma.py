@@ -1,0 +1,92 @@
+"""Matrix-core (MFMA) H-matvec: operand layout self-check, FP64 multi-RHS
+product against the dense expansion of the same factors, and the mixed
+FP32/FP16 product against the FP64 one and the dense kernel."""
+import numpy as np
+import pytest
+
+import hmx
+import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def torch_cuda():
+    import torch
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    return torch
+
+
+@pytest.fixture(scope="module")
+def ctx(torch_cuda):
+    return hmx.Context()
+
+
+@pytest.mark.parametrize("typ,K", [(0, 4), (1, 4), (2, 16)])
+def test_mfma_operand_layout(torch_cuda, ctx, typ, K):
+    rng = np.random.default_rng(typ)
+    A = rng.integers(-3, 4, size=(16, K)).astype(np.float64)  # exact in every type
+    B = rng.integers(-3, 4, size=(K, 16)).astype(np.float64)
+    dA = torch_cuda.from_numpy(A.copy()).cuda()
+    dB = torch_cuda.from_numpy(B.copy()).cuda()
+    dD = torch_cuda.zeros(16, 16, dtype=torch_cuda.float64, device="cuda")
+    hmx.selftest_mfma(ctx, typ, dA, dB, dD)
+    assert np.array_equal(dD.cpu().numpy(), A @ B)
+
+
+@pytest.fixture(scope="module")
+def pipe(torch_cuda, ctx):
+    n, d, b, tol = 4096, 2, 32, 1e-6
+    p = O.generate_points(n, d, 1234)
+    pp = hmx.Pipeline(n, d, b, tol, max_rank_policy=hmx.MAXRANK_FULL, points=torch_cuda.from_numpy(p).cuda(),
+                      ctx=ctx)
+    pp.host_points = p
+    return pp
+
+
+def expand(ctx, pipe):
+    from test_gpu_pipeline import expand_gpu
+    return expand_gpu(ctx, pipe)
+
+
+@pytest.mark.parametrize("k", [16, 32, 64, 128])
+def test_fp64_multi_rhs_exact(torch_cuda, ctx, pipe, k):
+    n = pipe.n
+    X = O.generate_rhs(n, k, 40 + k)
+    x = torch_cuda.from_numpy(np.ascontiguousarray(X.T)).cuda()
+    y = pipe.matvec(x).cpu().numpy().T
+    E = expand(ctx, pipe)
+    ref = E @ X
+    assert np.linalg.norm(y - ref) / np.linalg.norm(ref) < 1e-13
+
+
+@pytest.mark.parametrize("target", [1e-6, 3e-6, 1e-5])
+def test_mixed_precision_error_and_bytes(torch_cuda, ctx, pipe, target):
+    n = pipe.n
+    info = pipe.make_mixed(target)
+    assert info["hiCols"] + info["loCols"] > 0 and info["loCols"] > 0
+    fp64_bytes = sum(pipe.H.bytes()[k] for k in ("lowRankBytes", "denseBytes"))
+    assert info["bytes"] < fp64_bytes / 2.5
+    X = O.generate_rhs(n, 64, 3).astype(np.float32)
+    x32 = torch_cuda.from_numpy(np.ascontiguousarray(X.T)).cuda()
+    y32 = pipe.matvec_mixed(x32).cpu().numpy().T.astype(np.float64)
+    x64 = x32.double()
+    y64 = pipe.matvec(x64).cpu().numpy().T
+    rel = np.linalg.norm(y32 - y64) / np.linalg.norm(y64)
+    print(f"target {target:g}: hi {info['hiCols']} lo {info['loCols']} bytes {info['bytes']} "
+          f"(fp64 {fp64_bytes}) rel vs FP64 {rel:.3e}")
+    assert rel < 2 * target
+    # against the dense FP64 kernel on the same (FP32-valued) inputs
+    K = O.dense_matrix(pipe.host_points, n, 2, O.kdtree(pipe.host_points, n, 2, 32)[0], 0.1)
+    yd = K @ X.astype(np.float64)
+    assert np.linalg.norm(y32 - yd) / np.linalg.norm(yd) < 2 * target + 1e-7
+
+
+def test_mixed_requires_build(torch_cuda, ctx):
+    n, d, b = 1024, 2, 32
+    pp = hmx.Pipeline(n, d, b, 1e-5, ctx=ctx)
+    x = torch_cuda.zeros(16, n, dtype=torch_cuda.float32, device="cuda")
+    with pytest.raises(hmx.HmxError) as ei:
+        pp.matvec_mixed(x)
+    assert ei.value.status == hmx.HMX_ESTATE
