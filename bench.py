@@ -35,6 +35,7 @@ def main():
     ap.add_argument("--policy", choices=["ref", "full"], default="full")
     ap.add_argument("--rows", type=int, default=1024, help="sampled rows for the dense check")
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="budget of the CPU baseline (0 = skip)")
+    ap.add_argument("--mixed-target", type=float, default=3e-6, help="storage error budget of the mixed factors")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for {args.gpus}); sharding across GPUs is not part of "
@@ -111,6 +112,41 @@ def main():
     pipe.matvec(x16, y16)
     rel_err16 = float(torch.linalg.norm(y16[:, rows.long()] - yd16) / torch.linalg.norm(yd16))
 
+    # 64 right-hand sides: FP64 matrix cores vs mixed FP32/FP16 matrix cores
+    x64 = torch.empty(64, n, dtype=torch.float64, device="cuda")
+    hmx.generateRandomVector(ctx, 64, n, 777, x64)
+    y64 = torch.empty_like(x64)
+    sec64 = timed(x64, y64, max(5, args.steps // 4), 2)
+    ms64 = [float(v) for v in hmx.matvec_profile_multi(ctx, H, 64, x64, y64, iters=10)]
+    t0 = time.perf_counter()
+    minfo = pipe.make_mixed(args.mixed_target)
+    torch.cuda.synchronize()
+    mixed_build_s = time.perf_counter() - t0
+    x32 = x64.float()
+    y32 = torch.empty_like(x32)
+
+    def timed_mixed(steps, warmup):
+        for _ in range(warmup):
+            pipe.matvec_mixed(x32, y32)
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        for _ in range(steps):
+            pipe.matvec_mixed(x32, y32)
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t) / steps
+
+    secm = timed_mixed(max(5, args.steps // 4), 2)
+    msm = [float(v) for v in hmx.matvec_profile_multi(ctx, H, 64, x32, y32, iters=10, mixed=True)]
+    pipe.matvec(x64, y64)
+    pipe.matvec_mixed(x32, y32)
+    # error of the mixed product against the FP64 H-matvec and the dense kernel (sampled rows)
+    rel_m_vs_fp64 = float(torch.linalg.norm(y32.double() - y64) / torch.linalg.norm(y64))
+    yd64 = torch.empty(64, args.rows, dtype=torch.float64, device="cuda")
+    hmx.denseRowsXVector(ctx, n, d, pipe.points, pipe.tree.device_perm(), pipe.ell, rows, 64, x32.double().contiguous(), yd64)
+    rel_m_vs_dense = float(torch.linalg.norm(y32.double()[:, rows.long()] - yd64) / torch.linalg.norm(yd64))
+    print(f"[bench] 64-RHS FP64 {sec64 * 1e3:.3f} ms, mixed {secm * 1e3:.3f} ms, rel {rel_m_vs_fp64:.2e}",
+          file=sys.stderr, flush=True)
+
     ranks_per_level = {}
     for l in range(1, H.num_levels):
         li = H.level_info(l)
@@ -141,6 +177,16 @@ def main():
         "ms_per_rhs": round(sec1 * 1e3, 4),
         "rel_err_vs_dense": rel_err,
         "rhs16": {"ms_per_rhs": round(sec16 * 1e3 / 16, 4), "GBps": round(gbs16, 2), "rel_err_vs_dense": rel_err16},
+        "rhs64": {"fp64_ms_per_rhs": round(sec64 * 1e3 / 64, 5), "fp64_call_ms": round(sec64 * 1e3, 4),
+                  "fp64_kernel_ms": ms64,
+                  "mixed_ms_per_rhs": round(secm * 1e3 / 64, 5), "mixed_call_ms": round(secm * 1e3, 4),
+                  "mixed_kernel_ms": msm,
+                  "speedup_mixed_vs_fp64_64rhs": round(sec64 / secm, 3),
+                  "speedup_mixed_per_rhs_vs_fp64_1rhs": round(sec1 / (secm / 64), 2),
+                  "mixed_rel_err_vs_fp64": rel_m_vs_fp64, "mixed_rel_err_vs_dense": rel_m_vs_dense,
+                  "mixed_target": args.mixed_target, "mixed_bytes": int(minfo["bytes"]),
+                  "mixed_fp32_cols": int(minfo["hiCols"]), "mixed_fp16_cols": int(minfo["loCols"]),
+                  "mixed_build_s": round(mixed_build_s, 3)},
         "build_s": round(build_s, 3),
         "build_stages_s": {k: round(v, 3) for k, v in timings.items()},
         "factor_bytes": int(fac),

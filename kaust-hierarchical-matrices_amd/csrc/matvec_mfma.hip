@@ -66,21 +66,28 @@ __global__ void __launch_bounds__(256) k_mf_phase1(const MfItem* __restrict__ it
     for (int g = 0; g < CG; ++g)
 #pragma unroll
         for (int nb = 0; nb < NB; ++nb) acc[g][nb] = typename M::accv{0, 0, 0, 0};
-    for (int r0 = 0; r0 < it.rows; r0 += 16 * KL) {
+    // software pipelined: the operands of super-step r0 + 16*KL are in flight
+    // while the MFMAs of super-step r0 run
+    T a[CG][VL], bx[NB][VL], an[CG][VL], bn[NB][VL];
+    auto load = [&](int r0, T (&A)[CG][VL], T (&B)[NB][VL]) {
         const int rr = r0 + VL * lg;
         const bool rok = rr < it.rows;
-        T a[CG][VL], bx[NB][VL];
 #pragma unroll
         for (int g = 0; g < CG; ++g) {
             const int col = 16 * g + lc;
-            if (rok && col < it.ncols) vload(V + (size_t)col * it.ld + rr, a[g]);
-            else vzero(a[g]);
+            if (rok && col < it.ncols) vload(V + (size_t)col * it.ld + rr, A[g]);
+            else vzero(A[g]);
         }
 #pragma unroll
         for (int nb = 0; nb < NB; ++nb) {
-            if (rok) vload(Xs + (size_t)(16 * nb + lc) * ldx + rr, bx[nb]);
-            else vzero(bx[nb]);
+            if (rok) vload(Xs + (size_t)(16 * nb + lc) * ldx + rr, B[nb]);
+            else vzero(B[nb]);
         }
+    };
+    load(0, a, bx);
+    for (int r0 = 0; r0 < it.rows; r0 += 16 * KL) {
+        const bool more = r0 + 16 * KL < it.rows;
+        if (more) load(r0 + 16 * KL, an, bn);
 #pragma unroll
         for (int e = 0; e < 4; ++e)
 #pragma unroll
@@ -88,6 +95,16 @@ __global__ void __launch_bounds__(256) k_mf_phase1(const MfItem* __restrict__ it
 #pragma unroll
                 for (int nb = 0; nb < NB; ++nb)
                     acc[g][nb] = M::op(M::make(&a[g][e * KL]), M::make(&bx[nb][e * KL]), acc[g][nb]);
+        if (more) {
+#pragma unroll
+            for (int g = 0; g < CG; ++g)
+#pragma unroll
+                for (int q = 0; q < VL; ++q) a[g][q] = an[g][q];
+#pragma unroll
+            for (int nb = 0; nb < NB; ++nb)
+#pragma unroll
+                for (int q = 0; q < VL; ++q) bx[nb][q] = bn[nb][q];
+        }
     }
 #pragma unroll
     for (int g = 0; g < CG; ++g)
@@ -102,23 +119,34 @@ __global__ void __launch_bounds__(256) k_mf_phase1(const MfItem* __restrict__ it
 }
 
 // ---- reduce partials, scale, convert, transpose ------------------------------
-template <class T, class ACC>
-__global__ void __launch_bounds__(256) k_mf_reduce(int64_t ncols, int NR, const int64_t* __restrict__ colPart,
+template <class T, class ACC, int NR>
+__global__ void __launch_bounds__(256) k_mf_reduce(int64_t ncols, const int64_t* __restrict__ colPart,
                                                    const int32_t* __restrict__ colNpart,
                                                    const float* __restrict__ colScale,
                                                    const float* __restrict__ rhsScale, const ACC* __restrict__ zpart,
                                                    T* __restrict__ Zt) {
-    const int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x;
-    if (e >= ncols * NR) return;
-    const int s = (int)(e / ncols);
-    const int64_t g = e % ncols;
-    const int np = colNpart[g];
-    const int64_t p0 = colPart[g];
-    ACC v = 0;
-    for (int q = 0; q < np; ++q) v += zpart[(p0 + q) * NR + s];
-    if (colScale) v *= (ACC)colScale[g];
-    if (rhsScale) v *= (ACC)rhsScale[s];
-    Zt[(size_t)s * ncols + g] = (T)v;
+    // read along the right-hand sides (contiguous), write along columns (contiguous)
+    __shared__ ACC tile[64][NR + 1];
+    const int64_t g0 = (int64_t)blockIdx.x * 64;
+    for (int o = threadIdx.x; o < 64 * NR; o += 256) {
+        const int c = o / NR, s = o % NR;
+        const int64_t g = g0 + c;
+        ACC v = 0;
+        if (g < ncols) {
+            const int np = colNpart[g];
+            const int64_t p0 = colPart[g];
+            for (int q = 0; q < np; ++q) v += zpart[(p0 + q) * NR + s];
+            if (colScale) v *= (ACC)colScale[g];
+            if (rhsScale) v *= (ACC)rhsScale[s];
+        }
+        tile[c][s] = v;
+    }
+    __syncthreads();
+    for (int o = threadIdx.x; o < 64 * NR; o += 256) {
+        const int s = o / 64, c = o % 64;
+        const int64_t g = g0 + c;
+        if (g < ncols) Zt[(size_t)s * ncols + g] = (T)tile[c][s];
+    }
 }
 
 // ---- phase 2 -----------------------------------------------------------------
@@ -516,10 +544,14 @@ void build_fp64_plan(Context& c, HMatrix& h) {
 template <class ACC, class T>
 void reduce_group(Context& c, MfGroup& G, int NR, const float* rhsScale) {
     if (G.numColsPad == 0) return;
-    const int64_t tot = G.numColsPad * NR;
-    k_mf_reduce<T, ACC><<<(unsigned)((tot + 255) / 256), 256, 0, c.stream>>>(
-        G.numColsPad, NR, G.colPart.p, G.colNpart.p, G.colScale.n ? G.colScale.p : nullptr, rhsScale,
-        (const ACC*)G.zpart.p, (T*)G.zt.p);
+    const unsigned blocks = (unsigned)((G.numColsPad + 63) / 64);
+    const float* cs = G.colScale.n ? G.colScale.p : nullptr;
+    if (NR == 64)
+        k_mf_reduce<T, ACC, 64><<<blocks, 256, 0, c.stream>>>(G.numColsPad, G.colPart.p, G.colNpart.p, cs, rhsScale,
+                                                              (const ACC*)G.zpart.p, (T*)G.zt.p);
+    else
+        k_mf_reduce<T, ACC, 16><<<blocks, 256, 0, c.stream>>>(G.numColsPad, G.colPart.p, G.colNpart.p, cs, rhsScale,
+                                                              (const ACC*)G.zpart.p, (T*)G.zt.p);
     HMX_LAUNCHED();
 }
 
