@@ -156,6 +156,15 @@ void hmx_generateMaxRanks(uint32_t numLevels, uint32_t bucketSize, uint32_t* max
 hmx_status hmx_generateHMatrixFromStruct(hmx_context* ctx, const hmx_config* cfg, const hmx_kdtree* tree,
                                          const hmx_structure* structure, const hmx_tlr* mortonMatrix,
                                          hmx_hmatrix** hmatrix);
+/* createColumnMajorLRMatrix + convertColumnMajorToMorton + generateHMatrixFromStruct
+   (src/constructTLRMatrix.cu:13-151, src/helperFunctions.cu:8-57,
+   src/constructHMatrixFromStruct.cu:13-103) without the S x S TLR matrix, for
+   sizes where it does not fit: diagonal blocks and leaf-level tiles are the
+   TLR path's (same kernel, same random streams, bit for bit), upper levels are
+   compressed by the same ARA and tolerance/rank schedule from on-the-fly
+   kernel evaluations.  bucketSize 32 or 64.  d_points as for the KD-tree. */
+hmx_status hmx_generateHMatrixDirect(hmx_context* ctx, const hmx_config* cfg, const hmx_kdtree* tree,
+                                     const hmx_structure* structure, const double* d_points, hmx_hmatrix** hmatrix);
 /* per level l (1..numLevels-1): tile count, rank sum, max rank, tolerance used, cap hits */
 hmx_status hmx_hmatrix_level_info(const hmx_hmatrix* h, int32_t level, int32_t* numTiles, int64_t* rankSum,
                                   int32_t* maxRank, float* tolerance, int32_t* capHits);

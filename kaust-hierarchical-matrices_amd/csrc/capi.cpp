@@ -334,6 +334,30 @@ hmx_status hmx_generateHMatrixFromStruct(hmx_context* ctx, const hmx_config* cfg
     });
 }
 
+hmx_status hmx_generateHMatrixDirect(hmx_context* ctx, const hmx_config* cfg, const hmx_kdtree* tree,
+                                     const hmx_structure* structure, const double* d_points, hmx_hmatrix** hmatrix) {
+    if (!ctx) return HMX_EINVAL;
+    return guard(ctx, [&] {
+        check_config(cfg);
+        NEED(tree);
+        NEED(structure);
+        NEED(d_points);
+        NEED(hmatrix);
+        *hmatrix = nullptr;
+        HMX_CHECK(tree->n == cfg->numberOfInputPoints && tree->d == cfg->dimensionOfInputPoints &&
+                      tree->b == cfg->bucketSize,
+                  HMX_EINVAL, "configuration and KD-tree do not match");
+        hmx_hmatrix* h = new hmx_hmatrix();
+        try {
+            build_hmatrix_direct(*ctx, *cfg, *tree, *structure, d_points, *h);
+        } catch (...) {
+            delete h;
+            throw;
+        }
+        *hmatrix = h;
+    });
+}
+
 hmx_status hmx_hmatrix_level_info(const hmx_hmatrix* h, int32_t level, int32_t* numTiles, int64_t* rankSum,
                                   int32_t* maxRank, float* tolerance, int32_t* capHits) {
     if (!h || level < 1 || level >= h->numLevels) return HMX_EINVAL;

@@ -37,6 +37,11 @@ struct LrLevel {
     LrState* st;
     double *Q, *Y, *Ya, *Om, *W;
     int capQ;
+    // direct mode: the operator is the kernel itself, evaluated on the fly
+    const double* P;        // padded KD-ordered points, (x, y, z, w) per position; w = 1 real, 0 padding
+    const int32_t* tileMO;  // Morton code of each tile
+    double ell;
+    double* Ypart;          // column-split partial samples [split][T][16][m]
 };
 
 __device__ inline bool lr_active(const LrState& s, int maxRank) {
@@ -49,13 +54,13 @@ __global__ void k_lr_init(int T, LrState* st) {
     st[t].k = 0; st[t].small_run = 0; st[t].converged = 0; st[t].capped = 0; st[t].na = 0;
 }
 
-// Om[t][s*m + c]
-__global__ void __launch_bounds__(256) k_lr_omega(LrLevel L, int it) {
+// Om[t][s*m + c] (transposed == 0) or Om[t][c*16 + s] (transposed == 1)
+__global__ void __launch_bounds__(256) k_lr_omega(LrLevel L, int it, int transposed) {
     int t = blockIdx.y;
     if (!lr_active(L.st[t], L.maxRank)) return;
     int e = blockIdx.x * 256 + threadIdx.x;
     if (e >= L.m * kB16) return;
-    int s = e / L.m, c = e % L.m;
+    int s = transposed ? e % kB16 : e / L.m, c = transposed ? e / kB16 : e % L.m;
     uint64_t opid = ((uint64_t)(L.level + 1) << 40) | (uint64_t)t;
     L.Om[(size_t)t * L.m * kB16 + e] = gauss_at(L.seed, kTagOmega, opid, (uint32_t)(it * kB16 + s), (uint32_t)c);
 }
@@ -110,6 +115,138 @@ __global__ void __launch_bounds__(256) k_lr_sample(LrLevel L) {
             Y[(size_t)s * m + (size_t)i * b + x] = acc[q];
         }
     }
+}
+
+// ---- direct mode: samples from on-the-fly kernel evaluations ----------------
+// Tile t of a level covers row cluster v and column cluster u (Morton code);
+// its padded row q is KD position v*m + q of the padded point array.  Every
+// wave owns 64 consecutive rows (sample) or columns (transposed sample) of
+// one tile (m is a multiple of 64); the other side's points and the dense
+// operand are staged through the wave's slice of LDS 64 at a time.
+
+// padded points: P[(leaf*b + i)*4 + j]
+__global__ void __launch_bounds__(256) k_pad_points(uint32_t n, uint32_t d, int b, uint32_t S,
+                                                    const double* __restrict__ p, const int32_t* __restrict__ perm,
+                                                    const int32_t* __restrict__ offsets, double* __restrict__ P) {
+    const int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (e >= (int64_t)S * b) return;
+    const int leaf = (int)(e / b), i = (int)(e % b);
+    const int a0 = offsets[leaf], cnt = offsets[leaf + 1] - a0;
+    double x[4] = {0.0, 0.0, 0.0, 0.0};
+    if (i < cnt) {
+        for (uint32_t j = 0; j < d; ++j) x[j] = p[(size_t)j * n + perm[a0 + i]];
+        x[3] = 1.0;
+    }
+    double4* o = reinterpret_cast<double4*>(P + e * 4);
+    *o = make_double4(x[0], x[1], x[2], x[3]);
+}
+
+// Y[t][s*m + q] (or the split's partial) = sum_c K(x_q, x_c) Om[t][c*16 + s]
+template <int D>
+__global__ void __launch_bounds__(256) k_dir_sample(LrLevel L, int splits) {
+    __shared__ double4 sP[4][64];
+    __shared__ double sO[4][64][kB16];
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int m = L.m;
+    const int64_t gr = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    const bool inb = gr < (int64_t)L.T * m;  // uniform per wave (T*m is a multiple of 64)
+    const int t = inb ? (int)(gr / m) : 0, q = (int)(gr % m);
+    const bool act = inb && lr_active(L.st[t], L.maxRank);
+    const uint32_t code = (uint32_t)L.tileMO[t];
+    const int64_t rowBase = (int64_t)compact16(code >> 1) * m, colBase = (int64_t)compact16(code) * m;
+    const double4 xr = act ? reinterpret_cast<const double4*>(L.P)[rowBase + q] : make_double4(0, 0, 0, 0);
+    const double xra[3] = {xr.x, xr.y, xr.z};
+    const int chunk = m / splits, c0 = blockIdx.y * chunk;
+    const double* Om = L.Om + (size_t)t * m * kB16;
+    double acc[kB16];
+#pragma unroll
+    for (int s = 0; s < kB16; ++s) acc[s] = 0.0;
+    for (int cb = c0; cb < c0 + chunk; cb += 64) {
+        __syncthreads();
+        if (act) {
+            sP[w][lane] = reinterpret_cast<const double4*>(L.P)[colBase + cb + lane];
+            const double2* src = reinterpret_cast<const double2*>(Om + (size_t)(cb + lane) * kB16);
+#pragma unroll
+            for (int s2 = 0; s2 < kB16 / 2; ++s2) {
+                const double2 v = src[s2];
+                sO[w][lane][2 * s2] = v.x;
+                sO[w][lane][2 * s2 + 1] = v.y;
+            }
+        }
+        __syncthreads();
+        if (act) {
+            for (int cc = 0; cc < 64; ++cc) {
+                const double4 xc = sP[w][cc];
+                const double xca[3] = {xc.x, xc.y, xc.z};
+                const double kv = kernel_eval<D>(xra, xca, L.ell) * xc.w;
+#pragma unroll
+                for (int s = 0; s < kB16; ++s) acc[s] = fma(kv, sO[w][cc][s], acc[s]);
+            }
+        }
+    }
+    if (!act) return;
+    double* Y = splits > 1 ? L.Ypart + ((size_t)blockIdx.y * L.T + t) * m * kB16 : L.Y + (size_t)t * m * kB16;
+#pragma unroll
+    for (int s = 0; s < kB16; ++s) Y[(size_t)s * m + q] = acc[s] * xr.w;
+}
+
+// Y = sum over the splits, in split order
+__global__ void __launch_bounds__(256) k_dir_sum(LrLevel L, int splits) {
+    const int t = blockIdx.y;
+    if (!lr_active(L.st[t], L.maxRank)) return;
+    const int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    const int64_t per = (int64_t)L.m * kB16;
+    if (e >= per) return;
+    double v = 0.0;
+    for (int sp = 0; sp < splits; ++sp) v += L.Ypart[((size_t)sp * L.T + t) * per + e];
+    L.Y[(size_t)t * per + e] = v;
+}
+
+// B[t][a*m + c] = sum_q K(x_q, x_c) Q[t][a*m + q], NQ basis columns per launch row
+template <int D, int NQ>
+__global__ void __launch_bounds__(256) k_dir_sampleT(LrLevel L, double* __restrict__ Bout) {
+    __shared__ double4 sP[4][64];
+    __shared__ double sQ[4][64][NQ];
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int m = L.m;
+    const int64_t gc = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    const bool inb = gc < (int64_t)L.T * m;
+    const int t = inb ? (int)(gc / m) : 0, c = (int)(gc % m);
+    const int k = inb ? L.st[t].k : 0;
+    const int a0 = blockIdx.y * NQ;
+    const bool act = inb && a0 < k;
+    const int na = act ? min(NQ, k - a0) : 0;
+    const uint32_t code = (uint32_t)L.tileMO[t];
+    const int64_t rowBase = (int64_t)compact16(code >> 1) * m, colBase = (int64_t)compact16(code) * m;
+    const double4 xc = act ? reinterpret_cast<const double4*>(L.P)[colBase + c] : make_double4(0, 0, 0, 0);
+    const double xca[3] = {xc.x, xc.y, xc.z};
+    const double* Q = L.Q + (size_t)t * m * L.capQ;
+    double acc[NQ];
+#pragma unroll
+    for (int a = 0; a < NQ; ++a) acc[a] = 0.0;
+    for (int qb = 0; qb < m; qb += 64) {
+        __syncthreads();
+        if (act) {
+            sP[w][lane] = reinterpret_cast<const double4*>(L.P)[rowBase + qb + lane];
+#pragma unroll
+            for (int a = 0; a < NQ; ++a) sQ[w][lane][a] = a < na ? Q[(size_t)(a0 + a) * m + qb + lane] : 0.0;
+        }
+        __syncthreads();
+        if (act) {
+            for (int qq = 0; qq < 64; ++qq) {
+                const double4 xr = sP[w][qq];
+                const double xra[3] = {xr.x, xr.y, xr.z};
+                const double kv = kernel_eval<D>(xra, xca, L.ell) * xr.w;
+#pragma unroll
+                for (int a = 0; a < NQ; ++a) acc[a] = fma(kv, sQ[w][qq][a], acc[a]);
+            }
+        }
+    }
+    if (!act) return;
+    double* B = Bout + (size_t)t * m * L.capQ;
+#pragma unroll
+    for (int a = 0; a < NQ; ++a)
+        if (a < na) B[(size_t)(a0 + a) * m + c] = acc[a] * xc.w;
 }
 
 // W[t][s*k + a] = sum_i Q_t[a*m+i] B_t[s*m+i], a < k_t:
@@ -390,9 +527,50 @@ __global__ void k_leaf_ranks(int T, const int32_t* tiles, const int32_t* ranksMO
     if (t < T) out[t] = ranksMO[tiles[t]];
 }
 
-static void compress_level(Context& c, const hmx_config& cfg, const TLR& mo, const std::vector<int32_t>& tiles,
-                           int level, int numLevels, float tol, int maxRank, HLevel& out) {
-    const int b = (int)mo.b;
+// Source of a level's operator: the Morton TLR sub-tiles, or (direct mode)
+// the kernel evaluated on the fly from the padded points.
+struct LevelSource {
+    const TLR* mo = nullptr;
+    const double* P = nullptr;
+    double ell = 0.0;
+    int d = 0;
+};
+
+template <int BMAX>
+static void launch_sample_tlr(Context& c, const LrLevel& L, int bus, int T) {
+    k_lr_sample<BMAX><<<dim3(bus, T), 256, 0, c.stream>>>(L);
+    HMX_LAUNCHED();
+}
+
+template <int BMAX>
+static void launch_sampleT_tlr(Context& c, const LrLevel& L, int kmax, int bus, int T, double* B) {
+    dim3 g((unsigned)((kmax + 63) / 64), (unsigned)bus, (unsigned)T);
+    k_lr_sampleT<BMAX><<<g, 256, 0, c.stream>>>(L, B);
+    HMX_LAUNCHED();
+}
+
+template <int D>
+static void launch_sample_direct(Context& c, const LrLevel& L, int splits) {
+    const int64_t rows = (int64_t)L.T * L.m;
+    k_dir_sample<D><<<dim3((unsigned)((rows + 255) / 256), (unsigned)splits), 256, 0, c.stream>>>(L, splits);
+    HMX_LAUNCHED();
+    if (splits > 1) {
+        k_dir_sum<<<dim3((unsigned)(((int64_t)L.m * kB16 + 255) / 256), (unsigned)L.T), 256, 0, c.stream>>>(L, splits);
+        HMX_LAUNCHED();
+    }
+}
+
+template <int D>
+static void launch_sampleT_direct(Context& c, const LrLevel& L, int kmax, double* B) {
+    const int64_t cols = (int64_t)L.T * L.m;
+    k_dir_sampleT<D, 32><<<dim3((unsigned)((cols + 255) / 256), (unsigned)((kmax + 31) / 32)), 256, 0, c.stream>>>(L, B);
+    HMX_LAUNCHED();
+}
+
+static void compress_level(Context& c, const hmx_config& cfg, const LevelSource& src, int b,
+                           const std::vector<int32_t>& tiles, int level, int numLevels, float tol, int maxRank,
+                           HLevel& out) {
+    const bool direct = src.mo == nullptr;
     const int bus = 1 << (numLevels - (level + 1));
     const int m = bus * b;
     const int T = (int)tiles.size();
@@ -404,22 +582,37 @@ static void compress_level(Context& c, const hmx_config& cfg, const TLR& mo, con
     out.h_ranks.assign(T, 0);
     out.h_offsets.assign(T + 1, 0);
     if (T == 0) return;
+    HMX_CHECK(!direct || m % 64 == 0, HMX_EUNSUPPORTED, "direct compression needs tile dimensions that are multiples of 64");
     std::vector<int64_t> first(T);
     for (int t = 0; t < T; ++t) first[t] = (int64_t)tiles[t] * bus * bus;
     DBuf<int64_t> d_first(T);
+    DBuf<int32_t> d_tiles(T);
     HMX_HIP(hipMemcpyAsync(d_first.p, first.data(), T * sizeof(int64_t), hipMemcpyHostToDevice, c.stream));
+    HMX_HIP(hipMemcpyAsync(d_tiles.p, tiles.data(), T * sizeof(int32_t), hipMemcpyHostToDevice, c.stream));
     DBuf<LrState> st(T);
     int capQ = std::min(maxRank, 128);
     capQ = std::max(capQ, std::min(maxRank, kB16));
     DBuf<double> Q((size_t)T * m * capQ), W((size_t)T * capQ * kB16);
     DBuf<double> Y((size_t)T * m * kB16), Ya((size_t)T * m * kB16), Om((size_t)T * m * kB16);
+    // direct mode: split the columns of small levels over several workgroups
+    int splits = 1;
+    if (direct) {
+        const int64_t rows = (int64_t)T * m;
+        while (splits * 2 <= m / 64 && rows * splits < ((int64_t)1 << 18)) splits *= 2;
+    }
+    DBuf<double> Ypart(splits > 1 ? (size_t)splits * T * m * kB16 : 0);
     DBuf<int> flags(2);
     LrLevel L;
     L.T = T; L.m = m; L.b = b; L.bus = bus; L.maxRank = maxRank; L.r = cfg.araR;
     L.tau = ara_threshold((double)tol);
     L.seed = cfg.seed; L.level = level;
-    L.first = d_first.p; L.ranksMO = mo.ranks.p; L.offMO = mo.offsets.p; L.UMO = mo.U.p; L.VMO = mo.V.p;
+    L.first = d_first.p;
+    L.ranksMO = direct ? nullptr : src.mo->ranks.p;
+    L.offMO = direct ? nullptr : src.mo->offsets.p;
+    L.UMO = direct ? nullptr : src.mo->U.p;
+    L.VMO = direct ? nullptr : src.mo->V.p;
     L.st = st.p; L.Q = Q.p; L.Y = Y.p; L.Ya = Ya.p; L.Om = Om.p; L.W = W.p; L.capQ = capQ;
+    L.P = src.P; L.tileMO = d_tiles.p; L.ell = src.ell; L.Ypart = Ypart.p;
     k_lr_init<<<(T + 255) / 256, 256, 0, c.stream>>>(T, st.p);
     HMX_LAUNCHED();
     const unsigned rowBlocks = (unsigned)((m + 255) / 256);
@@ -440,10 +633,17 @@ static void compress_level(Context& c, const hmx_config& cfg, const TLR& mo, con
             capQ = nc;
             L.Q = Q.p; L.W = W.p; L.capQ = capQ;
         }
-        k_lr_omega<<<dim3((m * kB16 + 255) / 256, T), 256, 0, c.stream>>>(L, it);
+        k_lr_omega<<<dim3((m * kB16 + 255) / 256, T), 256, 0, c.stream>>>(L, it, direct ? 1 : 0);
         HMX_LAUNCHED();
-        k_lr_sample<32><<<dim3(bus, T), 256, 0, c.stream>>>(L);
-        HMX_LAUNCHED();
+        if (direct) {
+            if (src.d == 1) launch_sample_direct<1>(c, L, splits);
+            else if (src.d == 2) launch_sample_direct<2>(c, L, splits);
+            else launch_sample_direct<3>(c, L, splits);
+        } else if (b <= 32) {
+            launch_sample_tlr<32>(c, L, bus, T);
+        } else {
+            launch_sample_tlr<64>(c, L, bus, T);
+        }
         const unsigned kb = (unsigned)std::max(1, (kmax + 15) / 16);
         for (int pass = 0; pass < 2 && kmax > 0; ++pass) {
             k_lr_dot<<<dim3(kb, T), 256, 0, c.stream>>>(L, 0);
@@ -470,12 +670,22 @@ static void compress_level(Context& c, const hmx_config& cfg, const TLR& mo, con
         kmax = hf[1];
         if (hf[0] == 0) break;
     }
-    // B = A^T Q into the Ya-sized buffer reuse: allocate T*m*capQ
+    Ypart.release();
+    Y.release();
+    Ya.release();
+    Om.release();
+    // B = A^T Q
     DBuf<double> Bf((size_t)T * m * capQ);
     if (kmax > 0) {
-        dim3 g((unsigned)((kmax + 63) / 64), (unsigned)bus, (unsigned)T);
-        k_lr_sampleT<32><<<g, 256, 0, c.stream>>>(L, Bf.p);
-        HMX_LAUNCHED();
+        if (direct) {
+            if (src.d == 1) launch_sampleT_direct<1>(c, L, kmax, Bf.p);
+            else if (src.d == 2) launch_sampleT_direct<2>(c, L, kmax, Bf.p);
+            else launch_sampleT_direct<3>(c, L, kmax, Bf.p);
+        } else if (b <= 32) {
+            launch_sampleT_tlr<32>(c, L, kmax, bus, T, Bf.p);
+        } else {
+            launch_sampleT_tlr<64>(c, L, kmax, bus, T, Bf.p);
+        }
     }
     std::vector<LrState> hst(T);
     HMX_HIP(hipMemcpyAsync(hst.data(), st.p, T * sizeof(LrState), hipMemcpyDeviceToHost, c.stream));
@@ -506,6 +716,42 @@ static void compress_level(Context& c, const hmx_config& cfg, const TLR& mo, con
     }
 }
 
+// upper levels (generateHMatrixFromStruct): tolerance doubles per non-empty
+// level (src/constructHMatrixFromStruct.cu:54), rank cap from
+// generateMaxRanks (src/helperFunctions.cu:67-74)
+static void compress_upper_levels(Context& c, const hmx_config& cfg, const Structure& s, const LevelSource& src, int b,
+                                  HMatrix& h) {
+    const int L = s.numLevels;
+    std::vector<uint32_t> maxRanks(L > 2 ? L - 2 : 1, 0);
+    hmx_generateMaxRanks((uint32_t)L, (uint32_t)b, maxRanks.data());
+    float tol = cfg.lowestLevelTolerance;
+    for (int level = L - 2; level > 0; --level) {
+        HLevel& lv = h.levels[level - 1];
+        const int T = s.numTiles[level - 1];
+        const int m = (1 << (L - (level + 1))) * b;
+        if (T == 0) {
+            lv.level = level; lv.numTiles = 0; lv.m = m;
+            lv.h_ranks.clear(); lv.h_offsets.assign(1, 0);
+            continue;
+        }
+        tol *= 2;
+        int maxRank = cfg.maxRankPolicy == HMX_MAXRANK_REF ? (int)maxRanks[L - level - 2] : m;
+        maxRank = std::min(maxRank, m);
+        compress_level(c, cfg, src, b, s.tileIndices[level - 1], level, L, tol, maxRank, lv);
+    }
+}
+
+static void finish_accounting(HMatrix& h) {
+    h.lowRankBytes = 0;
+    h.factorEntries = 0;
+    for (auto& lv : h.levels) {
+        h.lowRankBytes += 2ull * (uint64_t)lv.rankSum * lv.m * sizeof(double);
+        h.factorEntries += 2ull * (uint64_t)lv.rankSum * lv.m;
+    }
+    h.denseBytes = (uint64_t)h.S * h.b * h.b * sizeof(double);
+    h.factorEntries += (uint64_t)h.S * h.b * h.b;
+}
+
 void build_hmatrix(Context& c, const hmx_config& cfg, const KDTree& t, const Structure& s, const TLR& mo,
                    HMatrix& h) {
     HMX_CHECK(mo.ordering == HMX_MORTON, HMX_ESTATE, "generateHMatrixFromStruct: TLR matrix must be Morton-ordered");
@@ -513,7 +759,7 @@ void build_hmatrix(Context& c, const hmx_config& cfg, const KDTree& t, const Str
     HMX_CHECK(s.numLevels >= 2 && (uint32_t)(1u << (s.numLevels - 1)) == t.numSegments, HMX_EINVAL,
               "structure and KD-tree do not match");
     HMX_CHECK(cfg.araBlockSize == kB16, HMX_EUNSUPPORTED, "ARA block size must be 16");
-    HMX_CHECK(t.b <= 32, HMX_EUNSUPPORTED, "hierarchical recompression: bucketSize must be <= 32");
+    HMX_CHECK(t.b <= 64, HMX_EUNSUPPORTED, "hierarchical recompression: bucketSize must be <= 64");
     const int L = s.numLevels, b = (int)t.b;
     h.n = t.n; h.d = t.d; h.b = t.b; h.S = t.numSegments; h.numLevels = L;
     h.structureCopy = s;
@@ -556,36 +802,56 @@ void build_hmatrix(Context& c, const hmx_config& cfg, const KDTree& t, const Str
             HMX_HIP(hipStreamSynchronize(c.stream));
         }
     }
-    // upper levels (generateHMatrixFromStruct): tolerance doubles per
-    // non-empty level (src/constructHMatrixFromStruct.cu:54), rank cap from
-    // generateMaxRanks (src/helperFunctions.cu:67-74)
-    std::vector<uint32_t> maxRanks(L > 2 ? L - 2 : 1, 0);
-    hmx_generateMaxRanks((uint32_t)L, (uint32_t)b, maxRanks.data());
-    float tol = cfg.lowestLevelTolerance;
-    for (int level = L - 2; level > 0; --level) {
-        HLevel& lv = h.levels[level - 1];
-        const int T = s.numTiles[level - 1];
-        const int m = (1 << (L - (level + 1))) * b;
-        if (T == 0) {
-            lv.level = level; lv.numTiles = 0; lv.m = m;
-            lv.h_ranks.clear(); lv.h_offsets.assign(1, 0);
-            continue;
-        }
-        tol *= 2;
-        int maxRank = cfg.maxRankPolicy == HMX_MAXRANK_REF ? (int)maxRanks[L - level - 2] : m;
-        maxRank = std::min(maxRank, m);
-        compress_level(c, cfg, mo, s.tileIndices[level - 1], level, L, tol, maxRank, lv);
-    }
-    // bytes accounting
-    h.lowRankBytes = 0;
-    h.factorEntries = 0;
-    for (auto& lv : h.levels) {
-        h.lowRankBytes += 2ull * (uint64_t)lv.rankSum * lv.m * sizeof(double);
-        h.factorEntries += 2ull * (uint64_t)lv.rankSum * lv.m;
-    }
-    h.denseBytes = (uint64_t)h.S * b * b * sizeof(double);
-    h.factorEntries += (uint64_t)h.S * b * b;
+    LevelSource src;
+    src.mo = &mo;
+    compress_upper_levels(c, cfg, s, src, b, h);
+    finish_accounting(h);
     // the matvec plan is built on the first HMatrixVecMult call
+}
+
+// Direct assembly for sizes where the TLR matrix of all b x b tiles does not
+// fit in memory (S^2 tiles): the dense diagonal blocks and the leaf level are
+// produced exactly as the TLR path produces them (same kernel, same random
+// streams), every upper level is compressed by the same batched ARA and
+// tolerance/rank schedule, sampling the kernel on the fly instead of the
+// TLR sub-tiles.  Replaces createColumnMajorLRMatrix + convertColumnMajorToMorton
+// + generateHMatrixFromStruct (src/constructTLRMatrix.cu:13-151,
+// src/helperFunctions.cu:8-57, src/constructHMatrixFromStruct.cu:13-103).
+void build_hmatrix_direct(Context& c, const hmx_config& cfg, const KDTree& t, const Structure& s,
+                          const double* d_points, HMatrix& h) {
+    HMX_CHECK(s.numLevels >= 2 && (uint32_t)(1u << (s.numLevels - 1)) == t.numSegments, HMX_EINVAL,
+              "structure and KD-tree do not match");
+    HMX_CHECK(cfg.araBlockSize == kB16, HMX_EUNSUPPORTED, "ARA block size must be 16");
+    HMX_CHECK(t.b == 32 || t.b == 64, HMX_EUNSUPPORTED, "direct assembly: bucketSize must be 32 or 64");
+    HMX_CHECK(t.d >= 1 && t.d <= 3, HMX_EUNSUPPORTED, "direct assembly: dimension must be 1, 2 or 3");
+    HMX_CHECK(d_points != nullptr, HMX_EINVAL, "null points");
+    const int L = s.numLevels, b = (int)t.b;
+    h.n = t.n; h.d = t.d; h.b = t.b; h.S = t.numSegments; h.numLevels = L;
+    h.structureCopy = s;
+    h.structure = &h.structureCopy;
+    h.levels.clear();
+    h.levels.resize(L - 1);
+    h.diag.alloc((size_t)t.numSegments * b * b);
+    build_dense_diagonal(c, cfg, t, d_points, h.diag.p);
+    {
+        HLevel& lv = h.levels[L - 2];
+        lv.level = L - 1;
+        lv.maxRank = cfg.maxRankPolicy == HMX_MAXRANK_REF ? b / 2 : b;
+        lv.tol = cfg.lowestLevelTolerance;
+        compress_leaf_tiles(c, cfg, t, d_points, s.tileIndices[L - 2], lv);
+    }
+    DBuf<double> P((size_t)t.numSegments * b * 4);
+    const int64_t np = (int64_t)t.numSegments * b;
+    k_pad_points<<<(unsigned)((np + 255) / 256), 256, 0, c.stream>>>(t.n, t.d, b, t.numSegments, d_points, t.perm.p,
+                                                                      t.offsets.p, P.p);
+    HMX_LAUNCHED();
+    LevelSource src;
+    src.P = P.p;
+    src.ell = cfg.correlationLength > 0 ? cfg.correlationLength : default_ell(t.d);
+    src.d = (int)t.d;
+    compress_upper_levels(c, cfg, s, src, b, h);
+    HMX_HIP(hipStreamSynchronize(c.stream));
+    finish_accounting(h);
 }
 
 }  // namespace hmx

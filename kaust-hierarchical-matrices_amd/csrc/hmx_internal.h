@@ -222,6 +222,11 @@ void build_structure(Structure& s, uint32_t n, uint32_t d, uint32_t b, int adm, 
 void build_tlr_cm(Context& c, const hmx_config& cfg, const KDTree& t, const double* d_points, TLR& out);
 void convert_to_morton(Context& c, const TLR& cm, TLR& mo);
 void build_hmatrix(Context& c, const hmx_config& cfg, const KDTree& t, const Structure& s, const TLR& mo, HMatrix& h);
+void build_hmatrix_direct(Context& c, const hmx_config& cfg, const KDTree& t, const Structure& s,
+                          const double* d_points, HMatrix& h);
+void compress_leaf_tiles(Context& c, const hmx_config& cfg, const KDTree& t, const double* d_points,
+                         const std::vector<int32_t>& tiles, HLevel& out);
+void build_dense_diagonal(Context& c, const hmx_config& cfg, const KDTree& t, const double* d_points, double* diag);
 void build_matvec_plan(Context& c, HMatrix& h, bool transpose);
 void hmatvec(Context& c, HMatrix& h, uint32_t k, const double* d_in, double* d_out, bool transpose, float* ms,
              int iters);

@@ -9,7 +9,7 @@
 //            chunks that produce partial sums, short columns are packed many
 //            per item and reduced with segmented wave reductions;
 //   reduce   z = sum of the chunk partials (fixed order -> deterministic);
-//   phase 2  one workgroup per leaf: y_leaf = D_leaf x_leaf + sum of the
+//   phase 2  one workgroup per leaf (32 or 64 rows): y_leaf = D_leaf x_leaf + sum of the
 //            leaf's rows of every U_t covering it times z_t; each y element is
 //            written exactly once, no atomics.
 // The factors stay where the build left them (no per-call V transpose as in
@@ -27,6 +27,16 @@ __device__ inline double wave_sum(double v) {
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
     return v;
+}
+
+template <int N>
+__device__ inline void vload_d(const double* __restrict__ p, double (&out)[N]) {
+#pragma unroll
+    for (int i = 0; i < N; i += 2) {
+        const double2 v = *reinterpret_cast<const double2*>(p + i);
+        out[i] = v.x;
+        out[i + 1] = v.y;
+    }
 }
 
 // ---- phase 1 ---------------------------------------------------------------
@@ -111,32 +121,38 @@ __global__ void __launch_bounds__(256) k_mv_reduce(int64_t numCols, int K, const
 }
 
 // ---- phase 2 -----------------------------------------------------------------
-// 256 threads = 16 column groups x 16 lanes; a lane owns rows 2l, 2l+1 of the
-// leaf (b = 32) and a column group walks columns cg, cg+16, ...
-template <int K>
-__global__ void __launch_bounds__(256) k_mv_phase2(int b, const int32_t* __restrict__ leafPtr,
+// 256 threads = 16 column groups x 16 lanes; a lane owns RPL = BL/16
+// consecutive rows of the leaf (BL = bucketSize) and a column group walks
+// columns cg, cg+16, ...
+template <int BL, int K>
+__global__ void __launch_bounds__(256) k_mv_phase2(const int32_t* __restrict__ leafPtr,
                                                    const MvEntry* __restrict__ entries,
                                                    const double* __restrict__ diag, const double* __restrict__ z,
                                                    const double* __restrict__ X, int ldx, double* __restrict__ Y) {
-    __shared__ double red[4][32 * K];
+    constexpr int RPL = BL / 16;
+    static_assert(RPL == 2 || RPL == 4, "bucketSize 32 or 64");
+    __shared__ double red[4][BL * K];
     const int leaf = blockIdx.x;
     const int l16 = threadIdx.x & 15, cg = threadIdx.x >> 4;
-    const int r0 = 2 * l16;
-    double acc0[K], acc1[K];
+    const int r0 = RPL * l16;
+    double acc[RPL][K];
 #pragma unroll
-    for (int s = 0; s < K; ++s) { acc0[s] = 0.0; acc1[s] = 0.0; }
-    // dense diagonal block: columns c = cg, cg+16
-    const double* D = diag + (size_t)leaf * 32 * 32;
-    const double* xl = X + (size_t)leaf * 32;
+    for (int q = 0; q < RPL; ++q)
 #pragma unroll
-    for (int cc = 0; cc < 2; ++cc) {
+        for (int s = 0; s < K; ++s) acc[q][s] = 0.0;
+    // dense diagonal block (column-major): columns c = cg, cg+16, ...
+    const double* D = diag + (size_t)leaf * BL * BL;
+    const double* xl = X + (size_t)leaf * BL;
+#pragma unroll
+    for (int cc = 0; cc < BL / 16; ++cc) {
         const int c = cg + 16 * cc;
-        const double2 dv = *reinterpret_cast<const double2*>(D + c * 32 + r0);
+        double dv[RPL];
+        vload_d<RPL>(D + c * BL + r0, dv);
 #pragma unroll
         for (int s = 0; s < K; ++s) {
             const double xv = xl[(size_t)s * ldx + c];
-            acc0[s] = fma(dv.x, xv, acc0[s]);
-            acc1[s] = fma(dv.y, xv, acc1[s]);
+#pragma unroll
+            for (int q = 0; q < RPL; ++q) acc[q][s] = fma(dv[q], xv, acc[q][s]);
         }
     }
     const int e0 = leafPtr[leaf], e1 = leafPtr[leaf + 1];
@@ -147,48 +163,48 @@ __global__ void __launch_bounds__(256) k_mv_phase2(int b, const int32_t* __restr
         const int m = en.m;
         int j = cg;
         for (; j + 48 < en.rank; j += 64) {
-            double2 u[4];
+            double u[4][RPL];
 #pragma unroll
-            for (int q = 0; q < 4; ++q) u[q] = *reinterpret_cast<const double2*>(U + (size_t)(j + 16 * q) * m + r0);
+            for (int p = 0; p < 4; ++p) vload_d<RPL>(U + (size_t)(j + 16 * p) * m + r0, u[p]);
 #pragma unroll
-            for (int q = 0; q < 4; ++q)
+            for (int p = 0; p < 4; ++p)
 #pragma unroll
                 for (int s = 0; s < K; ++s) {
-                    const double zv = zz[(size_t)(j + 16 * q) * K + s];
-                    acc0[s] = fma(u[q].x, zv, acc0[s]);
-                    acc1[s] = fma(u[q].y, zv, acc1[s]);
+                    const double zv = zz[(size_t)(j + 16 * p) * K + s];
+#pragma unroll
+                    for (int q = 0; q < RPL; ++q) acc[q][s] = fma(u[p][q], zv, acc[q][s]);
                 }
         }
         for (; j < en.rank; j += 16) {
-            const double2 u = *reinterpret_cast<const double2*>(U + (size_t)j * m + r0);
+            double u[RPL];
+            vload_d<RPL>(U + (size_t)j * m + r0, u);
 #pragma unroll
             for (int s = 0; s < K; ++s) {
                 const double zv = zz[(size_t)j * K + s];
-                acc0[s] = fma(u.x, zv, acc0[s]);
-                acc1[s] = fma(u.y, zv, acc1[s]);
+#pragma unroll
+                for (int q = 0; q < RPL; ++q) acc[q][s] = fma(u[q], zv, acc[q][s]);
             }
         }
     }
     // fold the 4 column groups of a wave (lanes l16 + 16g), then the 4 waves
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
 #pragma unroll
-    for (int s = 0; s < K; ++s) {
-        acc0[s] += __shfl_xor(acc0[s], 16, 64);
-        acc0[s] += __shfl_xor(acc0[s], 32, 64);
-        acc1[s] += __shfl_xor(acc1[s], 16, 64);
-        acc1[s] += __shfl_xor(acc1[s], 32, 64);
-    }
-    if (lane < 16)
+    for (int q = 0; q < RPL; ++q)
 #pragma unroll
         for (int s = 0; s < K; ++s) {
-            red[wave][s * 32 + r0] = acc0[s];
-            red[wave][s * 32 + r0 + 1] = acc1[s];
+            acc[q][s] += __shfl_xor(acc[q][s], 16, 64);
+            acc[q][s] += __shfl_xor(acc[q][s], 32, 64);
         }
+    if (lane < 16)
+#pragma unroll
+        for (int s = 0; s < K; ++s)
+#pragma unroll
+            for (int q = 0; q < RPL; ++q) red[wave][s * BL + r0 + q] = acc[q][s];
     __syncthreads();
-    for (int o = threadIdx.x; o < 32 * K; o += 256) {
+    for (int o = threadIdx.x; o < BL * K; o += 256) {
         const double v = (red[0][o] + red[1][o]) + (red[2][o] + red[3][o]);
-        const int s = o / 32, row = o % 32;
-        Y[(size_t)s * ldx + (size_t)leaf * 32 + row] = v;
+        const int s = o / BL, row = o % BL;
+        Y[(size_t)s * ldx + (size_t)leaf * BL + row] = v;
     }
 }
 
@@ -208,7 +224,7 @@ void build_matvec_plan(Context& c, HMatrix& h, bool transpose) {
     const int L = h.numLevels, b = (int)h.b, S = (int)h.S;
     HMX_CHECK(h.n == (uint32_t)S * (uint32_t)b, HMX_EUNSUPPORTED,
               "HMatrixVecMult requires full leaves (n = numSegments * bucketSize), as the reference does");
-    HMX_CHECK(b == 32, HMX_EUNSUPPORTED, "HMatrixVecMult kernels are built for bucketSize 32");
+    HMX_CHECK(b == 32 || b == 64, HMX_EUNSUPPORTED, "HMatrixVecMult kernels are built for bucketSize 32 or 64");
     std::vector<MvItem> items;
     std::vector<int64_t> colPart;
     std::vector<int32_t> colNpart;
@@ -303,7 +319,10 @@ static void run_phases(Context& c, MatvecPlan& P, int S, int b, const double* X,
         HMX_LAUNCHED();
     }
     if (ev) HMX_HIP(hipEventRecord(ev[2], c.stream));
-    k_mv_phase2<K><<<(unsigned)S, 256, 0, c.stream>>>(b, P.leafPtr.p, P.entries.p, P.diag, P.z.p, X, ldx, Y);
+    if (b == 64)
+        k_mv_phase2<64, K><<<(unsigned)S, 256, 0, c.stream>>>(P.leafPtr.p, P.entries.p, P.diag, P.z.p, X, ldx, Y);
+    else
+        k_mv_phase2<32, K><<<(unsigned)S, 256, 0, c.stream>>>(P.leafPtr.p, P.entries.p, P.diag, P.z.p, X, ldx, Y);
     HMX_LAUNCHED();
     if (ev) HMX_HIP(hipEventRecord(ev[3], c.stream));
 }

@@ -7,7 +7,7 @@
 //            one 16/32-byte vector per operand per four MFMAs.
 //   reduce   chunk partials summed in a fixed order, scaled, converted to the
 //            compute type and stored transposed ([rhs][column]).
-//   phase 2  one workgroup per leaf: Y_leaf = D X_leaf + sum_t U_t[leaf rows] Z_t,
+//   phase 2  one workgroup per leaf (32 or 64 rows): Y_leaf = D X_leaf + sum_t U_t[leaf rows] Z_t,
 //            A = the leaf's rows of a row-major copy of U, B = transposed Z.
 //
 // Mixed precision: the columns of every tile are split by a global energy
@@ -150,10 +150,11 @@ __global__ void __launch_bounds__(256) k_mf_reduce(int64_t ncols, const int64_t*
 }
 
 // ---- phase 2 -----------------------------------------------------------------
-// 4 waves: wave w owns rows 16*(w&1)..+16 of the leaf and every other
-// super-step of the K (column) loop, (w>>1) picks which; partials are folded
-// through LDS at the end.
-template <class TD, class TA, class TB, int NB, class TO>
+// One workgroup of 4 waves per leaf of BL rows (BL = 32 or 64).  The leaf's
+// rows form RG = BL/16 row groups of 16; wave w owns row group w % RG and
+// every KP-th super-step of the K (column) loop, KP = 4/RG, with w / RG
+// picking which; partials are folded through LDS at the end.
+template <int BL, class TD, class TA, class TB, int NB, class TO>
 __global__ void __launch_bounds__(256) k_mf_phase2(const int32_t* __restrict__ ptrA, const MfEntry* __restrict__ entA,
                                                    const TA* __restrict__ ZA, int64_t ldzA,
                                                    const int32_t* __restrict__ ptrB, const MfEntry* __restrict__ entB,
@@ -167,23 +168,25 @@ __global__ void __launch_bounds__(256) k_mf_phase2(const int32_t* __restrict__ p
     using accv = typename MA::accv;
     static_assert(sizeof(typename MB::acc_t) == sizeof(acc_t) && sizeof(typename MD::acc_t) == sizeof(acc_t),
                   "one accumulator type per launch");
-    constexpr int NR = 16 * NB;
-    __shared__ acc_t red[2][32][NR + 1];
+    static_assert(BL == 32 || BL == 64, "bucketSize 32 or 64");
+    constexpr int NR = 16 * NB, RG = BL / 16, KP = 4 / RG;
+    __shared__ acc_t red[KP][BL][NR + 1];
     const int leaf = blockIdx.x;
     const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int rg = w & 1, kp = w >> 1, lg = lane >> 4, lc = lane & 15;
+    const int rg = w % RG, kp = w / RG, lg = lane >> 4, lc = lane & 15;
     const int row = 16 * rg + lc;
     accv acc[NB];
 #pragma unroll
     for (int nb = 0; nb < NB; ++nb) acc[nb] = accv{0, 0, 0, 0};
-    {  // dense diagonal block: 32 columns, two super-steps of 16, kp picks one
-        const int cc = 16 * kp + 4 * lg;
+    // dense diagonal block (row-major BL x BL): BL/16 super-steps of 16 columns
+    for (int c0 = 16 * kp; c0 < BL; c0 += 16 * KP) {
+        const int cc = c0 + 4 * lg;
         TD a[4];
-        vload(Drm + (size_t)leaf * 1024 + row * 32 + cc, a);
+        vload(Drm + (size_t)leaf * BL * BL + row * BL + cc, a);
 #pragma unroll
         for (int nb = 0; nb < NB; ++nb) {
             TD bx[4];
-            vload(X + (size_t)(16 * nb + lc) * ldx + (size_t)leaf * 32 + cc, bx);
+            vload(X + (size_t)(16 * nb + lc) * ldx + (size_t)leaf * BL + cc, bx);
 #pragma unroll
             for (int e = 0; e < 4; ++e) acc[nb] = MD::op(a[e], bx[e], acc[nb]);
         }
@@ -194,7 +197,7 @@ __global__ void __launch_bounds__(256) k_mf_phase2(const int32_t* __restrict__ p
             const MfEntry en = entA[e];
             const TA* __restrict__ U = (const TA*)en.u + (size_t)row * en.rpad;
             const TA* __restrict__ Z = ZA + en.zcol;
-            for (int c0 = kp * SS; c0 < en.rpad; c0 += 2 * SS) {
+            for (int c0 = kp * SS; c0 < en.rpad; c0 += KP * SS) {
                 const int cc = c0 + VL * lg;
                 TA a[VL], bz[NB][VL];
                 if (cc < en.rpad) {
@@ -220,7 +223,7 @@ __global__ void __launch_bounds__(256) k_mf_phase2(const int32_t* __restrict__ p
             const MfEntry en = entB[e];
             const TB* __restrict__ U = (const TB*)en.u + (size_t)row * en.rpad;
             const TB* __restrict__ Z = ZB + en.zcol;
-            for (int c0 = kp * SS; c0 < en.rpad; c0 += 2 * SS) {
+            for (int c0 = kp * SS; c0 < en.rpad; c0 += KP * SS) {
                 const int cc = c0 + VL * lg;
                 TB a[VL], bz[NB][VL];
                 if (cc < en.rpad) {
@@ -245,9 +248,12 @@ __global__ void __launch_bounds__(256) k_mf_phase2(const int32_t* __restrict__ p
 #pragma unroll
         for (int v = 0; v < 4; ++v) red[kp][16 * rg + MA::out_row(lg, v)][16 * nb + lc] = acc[nb][v];
     __syncthreads();
-    for (int o = threadIdx.x; o < 32 * NR; o += 256) {
-        const int s = o >> 5, r = o & 31;
-        Y[(size_t)s * ldy + (size_t)leaf * 32 + r] = (TO)(red[0][r][s] + red[1][r][s]);
+    for (int o = threadIdx.x; o < BL * NR; o += 256) {
+        const int s = o / BL, r = o % BL;
+        acc_t v = red[0][r][s];
+#pragma unroll
+        for (int q = 1; q < KP; ++q) v += red[q][r][s];
+        Y[(size_t)s * ldy + (size_t)leaf * BL + r] = (TO)v;
     }
 }
 
@@ -350,12 +356,14 @@ __global__ void __launch_bounds__(256) k_cvt_rowmajor(int64_t n, const TrTile* _
 }
 
 template <class T>
-__global__ void __launch_bounds__(256) k_diag_rowmajor(int64_t total, const double* __restrict__ D, T* __restrict__ Drm) {
+__global__ void __launch_bounds__(256) k_diag_rowmajor(int64_t total, int b, const double* __restrict__ D,
+                                                       T* __restrict__ Drm) {
     const int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x;
     if (e >= total) return;
-    const int64_t leaf = e >> 10;
-    const int r = (int)((e >> 5) & 31), c = (int)(e & 31);
-    Drm[e] = (T)D[(leaf << 10) + (int64_t)c * 32 + r];  // D is column-major per leaf
+    const int64_t bb = (int64_t)b * b;
+    const int64_t leaf = e / bb;
+    const int r = (int)((e % bb) / b), c = (int)(e % b);
+    Drm[e] = (T)D[leaf * bb + (int64_t)c * b + r];  // D is column-major per leaf
 }
 
 // ---- host side ------------------------------------------------------------------
@@ -516,8 +524,9 @@ void build_group(Context& c, HMatrix& h, MfGroup& G, int type, const std::vector
 }
 
 void check_shape(const HMatrix& h) {
-    HMX_CHECK(h.n == h.S * h.b, HMX_EUNSUPPORTED, "the multi-RHS matvec requires full leaves (n = numSegments * 32)");
-    HMX_CHECK(h.b == 32, HMX_EUNSUPPORTED, "the multi-RHS matvec kernels are built for bucketSize 32");
+    HMX_CHECK(h.n == h.S * h.b, HMX_EUNSUPPORTED,
+              "the multi-RHS matvec requires full leaves (n = numSegments * bucketSize)");
+    HMX_CHECK(h.b == 32 || h.b == 64, HMX_EUNSUPPORTED, "the multi-RHS matvec kernels are built for bucketSize 32 or 64");
 }
 
 void build_fp64_plan(Context& c, HMatrix& h) {
@@ -531,9 +540,10 @@ void build_fp64_plan(Context& c, HMatrix& h) {
         tiles.push_back(std::move(gt));
     }
     build_group(c, h, P.g[0], 0, tiles, true, nullptr, nullptr, nullptr);
-    const int64_t tot = (int64_t)h.S * 1024;
+    const int64_t tot = (int64_t)h.S * h.b * h.b;
     P.Drm.alloc(tot * sizeof(double));
-    k_diag_rowmajor<double><<<(unsigned)((tot + 255) / 256), 256, 0, c.stream>>>(tot, h.diag.p, (double*)P.Drm.p);
+    k_diag_rowmajor<double><<<(unsigned)((tot + 255) / 256), 256, 0, c.stream>>>(tot, (int)h.b, h.diag.p,
+                                                                                (double*)P.Drm.p);
     HMX_LAUNCHED();
     HMX_HIP(hipStreamSynchronize(c.stream));
     P.dtype = 0;
@@ -596,9 +606,10 @@ void run_fp64(Context& c, HMatrix& h, const double* X, double* Y, Events& ev) {
     ev.rec(c, 1);
     reduce_group<double, double>(c, G, 16 * NB, nullptr);
     ev.rec(c, 2);
-    k_mf_phase2<double, double, double, NB, double><<<h.S, 256, 0, c.stream>>>(
-        G.leafPtr.p, G.entries.p, (const double*)G.zt.p, G.numColsPad, nullptr, nullptr, nullptr, 0,
-        (const double*)P.Drm.p, X, ldx, Y, ldx);
+    auto k2 = h.b == 64 ? k_mf_phase2<64, double, double, double, NB, double>
+                        : k_mf_phase2<32, double, double, double, NB, double>;
+    k2<<<h.S, 256, 0, c.stream>>>(G.leafPtr.p, G.entries.p, (const double*)G.zt.p, G.numColsPad, nullptr, nullptr,
+                                  nullptr, 0, (const double*)P.Drm.p, X, ldx, Y, ldx);
     HMX_LAUNCHED();
     ev.rec(c, 3);
 }
@@ -622,9 +633,11 @@ void run_mixed(Context& c, HMatrix& h, const float* X, float* Y, Events& ev) {
     reduce_group<float, float>(c, Ghi, NR, nullptr);
     reduce_group<float, _Float16>(c, Glo, NR, P.xscale.p);
     ev.rec(c, 2);
-    k_mf_phase2<float, float, _Float16, NB, float><<<h.S, 256, 0, c.stream>>>(
-        Ghi.leafPtr.p, Ghi.entries.p, (const float*)Ghi.zt.p, Ghi.numColsPad, Glo.leafPtr.p, Glo.entries.p,
-        (const _Float16*)Glo.zt.p, Glo.numColsPad, (const float*)P.Drm.p, X, n, Y, n);
+    auto k2 = h.b == 64 ? k_mf_phase2<64, float, float, _Float16, NB, float>
+                        : k_mf_phase2<32, float, float, _Float16, NB, float>;
+    k2<<<h.S, 256, 0, c.stream>>>(Ghi.leafPtr.p, Ghi.entries.p, (const float*)Ghi.zt.p, Ghi.numColsPad,
+                                  Glo.leafPtr.p, Glo.entries.p, (const _Float16*)Glo.zt.p, Glo.numColsPad,
+                                  (const float*)P.Drm.p, X, n, Y, n);
     HMX_LAUNCHED();
     ev.rec(c, 3);
 }
@@ -732,9 +745,10 @@ void hmatrix_make_mixed(Context& c, HMatrix& h, double target) {
     }
     build_group(c, h, P.g[0], 1, gh, false, nullptr, nullptr, nullptr);
     build_group(c, h, P.g[1], 2, gl, false, &su, &sv, &colBase);
-    const int64_t tot = (int64_t)h.S * 1024;
+    const int64_t tot = (int64_t)h.S * h.b * h.b;
     P.Drm.alloc(tot * sizeof(float));
-    k_diag_rowmajor<float><<<(unsigned)((tot + 255) / 256), 256, 0, c.stream>>>(tot, h.diag.p, (float*)P.Drm.p);
+    k_diag_rowmajor<float><<<(unsigned)((tot + 255) / 256), 256, 0, c.stream>>>(tot, (int)h.b, h.diag.p,
+                                                                               (float*)P.Drm.p);
     HMX_LAUNCHED();
     HMX_HIP(hipStreamSynchronize(c.stream));
     P.dtype = 1;

@@ -48,12 +48,25 @@ __global__ void __launch_bounds__(64) k_tlr_ara(uint32_t n, uint32_t S, const do
                                                 double ell, double tau, int r, int maxRank, uint64_t seed, uint64_t t0,
                                                 double* __restrict__ Uslot, double* __restrict__ Vslot,
                                                 int32_t* __restrict__ ranks, int32_t* __restrict__ caps,
-                                                double* __restrict__ diag) {
+                                                double* __restrict__ diag, const int32_t* __restrict__ tileMO) {
     extern __shared__ __align__(16) unsigned char smem[];
     TileLds<B>& L = *reinterpret_cast<TileLds<B>*>(smem);
     const int tid = threadIdx.x;
-    const uint64_t t = t0 + blockIdx.x;
-    const uint32_t u = (uint32_t)(t / S), v = (uint32_t)(t % S);
+    // the tile: the next one of the column-major enumeration, or the next
+    // entry of a Morton tile list; t (the column-major index u*S + v) is the
+    // random stream id either way, so a tile draws the same Omega in both
+    uint64_t t;
+    uint32_t u, v;
+    if (tileMO) {
+        const uint32_t code = (uint32_t)tileMO[t0 + blockIdx.x];
+        u = compact16(code);
+        v = compact16(code >> 1);
+        t = (uint64_t)u * S + v;
+    } else {
+        t = t0 + blockIdx.x;
+        u = (uint32_t)(t / S);
+        v = (uint32_t)(t % S);
+    }
     const int au = offsets[u], su = offsets[u + 1] - au;
     const int av = offsets[v], sv = offsets[v + 1] - av;
     for (int e = tid; e < D * B; e += 64) {
@@ -72,7 +85,8 @@ __global__ void __launch_bounds__(64) k_tlr_ara(uint32_t n, uint32_t S, const do
     __syncthreads();
     const size_t slot = (size_t)blockIdx.x;
     if (u == v) {  // dense diagonal block (generateDenseBlockColumn diagonal branch)
-        for (int e = tid; e < B * B; e += 64) diag[(size_t)u * B * B + e] = L.A[e];
+        if (diag)
+            for (int e = tid; e < B * B; e += 64) diag[(size_t)u * B * B + e] = L.A[e];
         if (tid == 0) { ranks[slot] = 0; caps[slot] = 0; }
         return;
     }
@@ -218,52 +232,59 @@ __global__ void __launch_bounds__(256) k_compact(uint64_t nt, int b, int slotCol
     }
 }
 
+struct TlrLaunch {
+    uint32_t n, S;
+    const double* p;
+    const int32_t* perm;
+    const int32_t* off;
+    double ell, tau;
+    int r, maxRank;
+    uint64_t seed;
+    const int32_t* tileMO;  // Morton tile list, or null for the column-major enumeration
+};
+
 template <int B, int D>
-static void launch_tlr(Context& c, uint32_t n, uint32_t S, const double* p, const int32_t* perm, const int32_t* off,
-                       double ell, double tau, int r, int maxRank, uint64_t seed, uint64_t t0, uint64_t nt,
-                       double* Us, double* Vs, int32_t* ranks, int32_t* caps, double* diag) {
+static void launch_tlr(Context& c, const TlrLaunch& a, uint64_t t0, uint64_t nt, double* Us, double* Vs,
+                       int32_t* ranks, int32_t* caps, double* diag) {
     size_t lds = sizeof(TileLds<B>);
     auto kern = k_tlr_ara<B, D>;
     if (lds > 64 * 1024) HMX_HIP(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-    kern<<<(unsigned)nt, 64, lds, c.stream>>>(n, S, p, perm, off, ell, tau, r, maxRank, seed, t0, Us, Vs, ranks,
-                                              caps, diag);
+    kern<<<(unsigned)nt, 64, lds, c.stream>>>(a.n, a.S, a.p, a.perm, a.off, a.ell, a.tau, a.r, a.maxRank, a.seed, t0,
+                                              Us, Vs, ranks, caps, diag, a.tileMO);
     HMX_LAUNCHED();
 }
 
 template <int B>
-static void launch_tlr_d(Context& c, uint32_t d, uint32_t n, uint32_t S, const double* p, const int32_t* perm,
-                         const int32_t* off, double ell, double tau, int r, int maxRank, uint64_t seed, uint64_t t0,
-                         uint64_t nt, double* Us, double* Vs, int32_t* ranks, int32_t* caps, double* diag) {
-    if (d == 1) launch_tlr<B, 1>(c, n, S, p, perm, off, ell, tau, r, maxRank, seed, t0, nt, Us, Vs, ranks, caps, diag);
-    else if (d == 2) launch_tlr<B, 2>(c, n, S, p, perm, off, ell, tau, r, maxRank, seed, t0, nt, Us, Vs, ranks, caps, diag);
-    else launch_tlr<B, 3>(c, n, S, p, perm, off, ell, tau, r, maxRank, seed, t0, nt, Us, Vs, ranks, caps, diag);
+static void launch_tlr_d(Context& c, uint32_t d, const TlrLaunch& a, uint64_t t0, uint64_t nt, double* Us, double* Vs,
+                         int32_t* ranks, int32_t* caps, double* diag) {
+    if (d == 1) launch_tlr<B, 1>(c, a, t0, nt, Us, Vs, ranks, caps, diag);
+    else if (d == 2) launch_tlr<B, 2>(c, a, t0, nt, Us, Vs, ranks, caps, diag);
+    else launch_tlr<B, 3>(c, a, t0, nt, Us, Vs, ranks, caps, diag);
+}
+
+static void launch_tlr_b(Context& c, uint32_t b, uint32_t d, const TlrLaunch& a, uint64_t t0, uint64_t nt, double* Us,
+                         double* Vs, int32_t* ranks, int32_t* caps, double* diag) {
+    switch (b) {
+        case 8: launch_tlr_d<8>(c, d, a, t0, nt, Us, Vs, ranks, caps, diag); break;
+        case 16: launch_tlr_d<16>(c, d, a, t0, nt, Us, Vs, ranks, caps, diag); break;
+        case 32: launch_tlr_d<32>(c, d, a, t0, nt, Us, Vs, ranks, caps, diag); break;
+        default: launch_tlr_d<64>(c, d, a, t0, nt, Us, Vs, ranks, caps, diag); break;
+    }
 }
 
 double ara_threshold(double tol) { return tol / (10.0 * std::sqrt(2.0 / 3.141592653589793)); }
 
-void build_tlr_cm(Context& c, const hmx_config& cfg, const KDTree& t, const double* d_points, TLR& out) {
-    const uint32_t b = t.b, S = t.numSegments, n = t.n, d = t.d;
-    HMX_CHECK(b == 8 || b == 16 || b == 32 || b == 64, HMX_EUNSUPPORTED,
-              "TLR tiles: bucketSize must be 8, 16, 32 or 64");
-    HMX_CHECK(cfg.araBlockSize == kBS, HMX_EUNSUPPORTED, "ARA block size must be 16");
-    HMX_CHECK(cfg.araR >= 1, HMX_EINVAL, "araR must be >= 1");
-    HMX_CHECK(S >= 2, HMX_EINVAL, "TLR needs at least two segments (n > bucketSize)");
-    const int maxRank = cfg.maxRankPolicy == HMX_MAXRANK_REF ? (int)b / 2 : (int)b;
-    const double ell = cfg.correlationLength > 0 ? cfg.correlationLength : default_ell(d);
-    const double tau = ara_threshold((double)cfg.lowestLevelTolerance);
-    out.ordering = HMX_COLUMN_MAJOR;
-    out.S = S;
-    out.b = b;
-    out.n = n;
-    const uint64_t total = (uint64_t)S * S;
-    out.ranks.alloc(total);
-    out.offsets.alloc(total + 1);
-    out.diagonal.alloc((size_t)S * b * b);
-
-    // batches of tiles sized so the per-tile slots stay under ~2 GiB
+// Compress 'total' tiles in batches sized so the per-tile slots stay under
+// ~2 GiB; ranks[total] receives the ranks, U/V the packed factors in tile
+// order.  Returns the number of capped tiles.
+static uint64_t compress_tiles(Context& c, const hmx_config& cfg, const KDTree& t, const TlrLaunch& a,
+                               uint64_t total, int32_t* ranks, double* diag, DBuf<double>& U, DBuf<double>& V,
+                               uint64_t& rankSum) {
+    const uint32_t b = t.b;
+    const int maxRank = a.maxRank;
     const size_t slotBytes = 2 * (size_t)b * maxRank * sizeof(double);
     uint64_t batch = std::max<uint64_t>(1, ((size_t)2 << 30) / std::max<size_t>(slotBytes, 1));
-    batch = std::min<uint64_t>(batch, total);
+    batch = std::min<uint64_t>(batch, std::max<uint64_t>(total, 1));
     DBuf<double> Us(batch * b * maxRank), Vs(batch * b * maxRank);
     DBuf<int32_t> caps(batch);
     DBuf<int64_t> boff(batch + 1);
@@ -273,13 +294,8 @@ void build_tlr_cm(Context& c, const hmx_config& cfg, const KDTree& t, const doub
     uint64_t running = 0, capHits = 0;
     for (uint64_t t0 = 0; t0 < total; t0 += batch) {
         const uint64_t nt = std::min<uint64_t>(batch, total - t0);
-        int32_t* rk = out.ranks.p + t0;
-        switch (b) {
-            case 8: launch_tlr_d<8>(c, d, n, S, d_points, t.perm.p, t.offsets.p, ell, tau, cfg.araR, maxRank, cfg.seed, t0, nt, Us.p, Vs.p, rk, caps.p, out.diagonal.p); break;
-            case 16: launch_tlr_d<16>(c, d, n, S, d_points, t.perm.p, t.offsets.p, ell, tau, cfg.araR, maxRank, cfg.seed, t0, nt, Us.p, Vs.p, rk, caps.p, out.diagonal.p); break;
-            case 32: launch_tlr_d<32>(c, d, n, S, d_points, t.perm.p, t.offsets.p, ell, tau, cfg.araR, maxRank, cfg.seed, t0, nt, Us.p, Vs.p, rk, caps.p, out.diagonal.p); break;
-            default: launch_tlr_d<64>(c, d, n, S, d_points, t.perm.p, t.offsets.p, ell, tau, cfg.araR, maxRank, cfg.seed, t0, nt, Us.p, Vs.p, rk, caps.p, out.diagonal.p); break;
-        }
+        int32_t* rk = ranks + t0;
+        launch_tlr_b(c, b, t.d, a, t0, nt, Us.p, Vs.p, rk, caps.p, diag);
         exclusive_scan_i32_i64(c, rk, boff.p, nt);
         int64_t btotal = 0;
         HMX_HIP(hipMemcpyAsync(&btotal, boff.p + nt, sizeof(int64_t), hipMemcpyDeviceToHost, c.stream));
@@ -299,19 +315,128 @@ void build_tlr_cm(Context& c, const hmx_config& cfg, const KDTree& t, const doub
     }
     Us.release();
     Vs.release();
-    out.rankSum = running;
-    out.capHits = capHits;
-    out.U.alloc((size_t)running * b);
-    out.V.alloc((size_t)running * b);
+    rankSum = running;
+    U.alloc((size_t)running * b);
+    V.alloc((size_t)running * b);
     for (size_t i = 0; i < partsU.size(); ++i) {
         if (!partsU[i].n) continue;
-        HMX_HIP(hipMemcpyAsync(out.U.p + partBase[i] * b, partsU[i].p, partsU[i].bytes(), hipMemcpyDeviceToDevice,
+        HMX_HIP(hipMemcpyAsync(U.p + partBase[i] * b, partsU[i].p, partsU[i].bytes(), hipMemcpyDeviceToDevice,
                                c.stream));
-        HMX_HIP(hipMemcpyAsync(out.V.p + partBase[i] * b, partsV[i].p, partsV[i].bytes(), hipMemcpyDeviceToDevice,
+        HMX_HIP(hipMemcpyAsync(V.p + partBase[i] * b, partsV[i].p, partsV[i].bytes(), hipMemcpyDeviceToDevice,
                                c.stream));
     }
+    HMX_HIP(hipStreamSynchronize(c.stream));
+    return capHits;
+}
+
+static TlrLaunch tlr_args(const hmx_config& cfg, const KDTree& t, const double* d_points, const int32_t* tileMO) {
+    TlrLaunch a;
+    a.n = t.n;
+    a.S = t.numSegments;
+    a.p = d_points;
+    a.perm = t.perm.p;
+    a.off = t.offsets.p;
+    a.ell = cfg.correlationLength > 0 ? cfg.correlationLength : default_ell(t.d);
+    a.tau = ara_threshold((double)cfg.lowestLevelTolerance);
+    a.r = cfg.araR;
+    a.maxRank = cfg.maxRankPolicy == HMX_MAXRANK_REF ? (int)t.b / 2 : (int)t.b;
+    a.seed = cfg.seed;
+    a.tileMO = tileMO;
+    return a;
+}
+
+static void check_tlr_args(const hmx_config& cfg, const KDTree& t) {
+    const uint32_t b = t.b;
+    HMX_CHECK(b == 8 || b == 16 || b == 32 || b == 64, HMX_EUNSUPPORTED,
+              "TLR tiles: bucketSize must be 8, 16, 32 or 64");
+    HMX_CHECK(cfg.araBlockSize == kBS, HMX_EUNSUPPORTED, "ARA block size must be 16");
+    HMX_CHECK(cfg.araR >= 1, HMX_EINVAL, "araR must be >= 1");
+    HMX_CHECK(t.numSegments >= 2, HMX_EINVAL, "TLR needs at least two segments (n > bucketSize)");
+}
+
+void build_tlr_cm(Context& c, const hmx_config& cfg, const KDTree& t, const double* d_points, TLR& out) {
+    check_tlr_args(cfg, t);
+    const uint32_t b = t.b, S = t.numSegments;
+    out.ordering = HMX_COLUMN_MAJOR;
+    out.S = S;
+    out.b = b;
+    out.n = t.n;
+    const uint64_t total = (uint64_t)S * S;
+    out.ranks.alloc(total);
+    out.offsets.alloc(total + 1);
+    out.diagonal.alloc((size_t)S * b * b);
+    const TlrLaunch a = tlr_args(cfg, t, d_points, nullptr);
+    uint64_t rankSum = 0;
+    out.capHits = compress_tiles(c, cfg, t, a, total, out.ranks.p, out.diagonal.p, out.U, out.V, rankSum);
+    out.rankSum = rankSum;
     exclusive_scan_i32_i64(c, out.ranks.p, out.offsets.p, total);
     HMX_HIP(hipStreamSynchronize(c.stream));
+}
+
+// Leaf-level tiles of a structure compressed straight from the kernel (the
+// TLR ARA over a Morton tile list): the same factors, bit for bit, as the
+// TLR matrix's tiles at those positions.
+void compress_leaf_tiles(Context& c, const hmx_config& cfg, const KDTree& t, const double* d_points,
+                         const std::vector<int32_t>& tiles, HLevel& out) {
+    check_tlr_args(cfg, t);
+    const int T = (int)tiles.size();
+    out.numTiles = T;
+    out.m = (int)t.b;
+    out.h_ranks.assign(T, 0);
+    out.h_offsets.assign(T + 1, 0);
+    out.rankSum = 0;
+    out.capHits = 0;
+    if (T == 0) return;
+    DBuf<int32_t> d_tiles(T), d_r(T);
+    HMX_HIP(hipMemcpyAsync(d_tiles.p, tiles.data(), T * sizeof(int32_t), hipMemcpyHostToDevice, c.stream));
+    const TlrLaunch a = tlr_args(cfg, t, d_points, d_tiles.p);
+    uint64_t rankSum = 0;
+    out.capHits = (int32_t)compress_tiles(c, cfg, t, a, (uint64_t)T, d_r.p, nullptr, out.U, out.V, rankSum);
+    HMX_HIP(hipMemcpyAsync(out.h_ranks.data(), d_r.p, T * sizeof(int32_t), hipMemcpyDeviceToHost, c.stream));
+    HMX_HIP(hipStreamSynchronize(c.stream));
+    int64_t acc = 0;
+    for (int q = 0; q < T; ++q) { out.h_offsets[q] = acc; acc += out.h_ranks[q]; }
+    out.h_offsets[T] = acc;
+    out.rankSum = acc;
+    HMX_CHECK((uint64_t)acc == rankSum, HMX_EDEVICE, "leaf tile compression: rank sum mismatch");
+}
+
+// Dense diagonal blocks, exactly as the TLR kernel writes them: D[leaf] is
+// column-major b x b, K(x_c, x_q) for real points, identity on the padding.
+template <int D>
+__global__ void __launch_bounds__(256) k_dense_diag(uint32_t n, int b, uint32_t S, const double* __restrict__ p,
+                                                    const int32_t* __restrict__ perm,
+                                                    const int32_t* __restrict__ offsets, double ell,
+                                                    double* __restrict__ diag) {
+    const int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    const int64_t bb = (int64_t)b * b;
+    if (e >= (int64_t)S * bb) return;
+    const int leaf = (int)(e / bb);
+    const int c = (int)((e % bb) / b), q = (int)(e % b);
+    const int a0 = offsets[leaf], cnt = offsets[leaf + 1] - a0;
+    double val;
+    if (c < cnt && q < cnt) {
+        double xc[3], xq[3];
+#pragma unroll
+        for (int j = 0; j < D; ++j) {
+            xc[j] = p[(size_t)j * n + perm[a0 + c]];
+            xq[j] = p[(size_t)j * n + perm[a0 + q]];
+        }
+        val = kernel_eval<D>(xc, xq, ell);
+    } else {
+        val = c == q ? 1.0 : 0.0;
+    }
+    diag[e] = val;
+}
+
+void build_dense_diagonal(Context& c, const hmx_config& cfg, const KDTree& t, const double* d_points, double* diag) {
+    const double ell = cfg.correlationLength > 0 ? cfg.correlationLength : default_ell(t.d);
+    const int64_t tot = (int64_t)t.numSegments * t.b * t.b;
+    const unsigned g = (unsigned)((tot + 255) / 256);
+    if (t.d == 1) k_dense_diag<1><<<g, 256, 0, c.stream>>>(t.n, (int)t.b, t.numSegments, d_points, t.perm.p, t.offsets.p, ell, diag);
+    else if (t.d == 2) k_dense_diag<2><<<g, 256, 0, c.stream>>>(t.n, (int)t.b, t.numSegments, d_points, t.perm.p, t.offsets.p, ell, diag);
+    else k_dense_diag<3><<<g, 256, 0, c.stream>>>(t.n, (int)t.b, t.numSegments, d_points, t.perm.p, t.offsets.p, ell, diag);
+    HMX_LAUNCHED();
 }
 
 // ---- column-major -> Morton ------------------------------------------------

@@ -84,6 +84,7 @@ _SIGS = [
     ("hmx_freeMatrix", _i32, [_vp]),
     ("hmx_generateMaxRanks", None, [_u32, _u32, _vp]),
     ("hmx_generateHMatrixFromStruct", _i32, [_vp, _P(Config), _vp, _vp, _vp, _P(_vp)]),
+    ("hmx_generateHMatrixDirect", _i32, [_vp, _P(Config), _vp, _vp, _vp, _P(_vp)]),
     ("hmx_hmatrix_level_info", _i32, [_vp, _i32, _P(_i32), _P(_i64), _P(_i32), _P(_f32), _P(_i32)]),
     ("hmx_hmatrix_level_copy", _i32, [_vp, _vp, _i32, _vp, _vp, _vp]),
     ("hmx_hmatrix_diag_copy", _i32, [_vp, _vp, _vp]),
@@ -372,6 +373,15 @@ def generateHMatrixFromStruct(ctx, cfg, tree, structure, morton_tlr):
     return HMatrix(h, structure.num_levels(), cfg.numberOfInputPoints, cfg.bucketSize)
 
 
+def generateHMatrixDirect(ctx, cfg, tree, structure, points):
+    """Whole H-matrix without the S x S TLR matrix (diagonal and leaf level as
+    the TLR path produces them, upper levels sampled from the kernel)."""
+    h = ctypes.c_void_p()
+    _check(lib().hmx_generateHMatrixDirect(ctx.handle, ctypes.byref(cfg), tree.handle, structure.handle,
+                                           _ptr(points), ctypes.byref(h)), ctx)
+    return HMatrix(h, structure.num_levels(), cfg.numberOfInputPoints, cfg.bucketSize)
+
+
 def HMatrixVecMult(ctx, H, k, x, y, transpose=False):
     f = lib().hmx_HMatrixVecMultTranspose if transpose else lib().hmx_HMatrixVecMult
     _check(f(ctx.handle, H.handle, k, _ptr(x), _ptr(y)), ctx)
@@ -414,7 +424,7 @@ class Pipeline:
     """The reference driver sequence (src/main.cu:46-173) on the GPU."""
 
     def __init__(self, n, d, b, tol, adm="weak", eta=1.0, ell=0.0, max_rank_policy=MAXRANK_REF, seed=1234,
-                 points=None, ctx=None, timings=None):
+                 points=None, ctx=None, timings=None, direct=False):
         import time
         import torch
         self.ctx = ctx or Context()
@@ -441,12 +451,20 @@ class Pipeline:
         t0 = mark("KDTREE", t0)
         self.structure = constructHMatrixStructure(self.ctx, n, d, b, adm, eta, self.tree)
         t0 = mark("STRUCTURE", t0)
-        self.tlr_cm, self.rank_sum = createColumnMajorLRMatrix(self.ctx, self.cfg, self.tree, self.points)
-        t0 = mark("TLR_MATRIX", t0)
-        self.tlr_mo = convertColumnMajorToMorton(self.ctx, self.tlr_cm)
-        t0 = mark("CMTOMO", t0)
-        self.H = generateHMatrixFromStruct(self.ctx, self.cfg, self.tree, self.structure, self.tlr_mo)
-        t0 = mark("HMATRIX", t0)
+        self.direct = direct
+        if direct:
+            # no S x S TLR matrix: diagonal + leaf tiles as the TLR path, upper levels from the kernel
+            self.tlr_cm = self.tlr_mo = None
+            self.rank_sum = None
+            self.H = generateHMatrixDirect(self.ctx, self.cfg, self.tree, self.structure, self.points)
+            t0 = mark("HMATRIX_DIRECT", t0)
+        else:
+            self.tlr_cm, self.rank_sum = createColumnMajorLRMatrix(self.ctx, self.cfg, self.tree, self.points)
+            t0 = mark("TLR_MATRIX", t0)
+            self.tlr_mo = convertColumnMajorToMorton(self.ctx, self.tlr_cm)
+            t0 = mark("CMTOMO", t0)
+            self.H = generateHMatrixFromStruct(self.ctx, self.cfg, self.tree, self.structure, self.tlr_mo)
+            t0 = mark("HMATRIX", t0)
         self.timings = t
         if timings is not None:
             timings.update(t)

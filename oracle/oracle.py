@@ -235,6 +235,8 @@ class OracleHMatrix:
 
     def level(self, l):
         """(ranks, U list, V list) for H level l (1..num_levels-1); U_t is m x r_t."""
+        if not 1 <= l < self.num_levels:
+            raise IndexError(f"level {l} outside 1..{self.num_levels - 1}")
         L = lib()
         nt = len(self.tiles[l])
         ranks = np.ctypeslib.as_array(L.or_hm_level_ranks(self.h, l), shape=(max(nt, 1),))[:nt].copy()
@@ -252,6 +254,8 @@ class OracleHMatrix:
         return ranks, Us, Vs
 
     def level_info(self, l):
+        if not 1 <= l < self.num_levels:
+            raise IndexError(f"level {l} outside 1..{self.num_levels - 1}")
         L = lib()
         return L.or_hm_level_max_rank(self.h, l), L.or_hm_level_tol(self.h, l), L.or_hm_level_cap_hits(self.h, l)
 
