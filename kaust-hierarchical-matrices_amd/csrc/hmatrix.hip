@@ -10,6 +10,9 @@
 // level: sample, project (x2), column-by-column decisions, project the
 // accepted columns, re-normalise, commit.
 #include <algorithm>
+#include <chrono>
+#include <cstdio>
+#include <cstdlib>
 
 #include "hmx_device.h"
 #include "hmx_internal.h"
@@ -567,10 +570,18 @@ static void launch_sampleT_direct(Context& c, const LrLevel& L, int kmax, double
     HMX_LAUNCHED();
 }
 
+// progress lines on stderr when HMX_LOG is set (long builds)
+static bool log_on() {
+    static const bool on = std::getenv("HMX_LOG") != nullptr;
+    return on;
+}
+
 static void compress_level(Context& c, const hmx_config& cfg, const LevelSource& src, int b,
                            const std::vector<int32_t>& tiles, int level, int numLevels, float tol, int maxRank,
                            HLevel& out) {
     const bool direct = src.mo == nullptr;
+    const auto tstart = std::chrono::steady_clock::now();
+    int iters = 0;
     const int bus = 1 << (numLevels - (level + 1));
     const int m = bus * b;
     const int T = (int)tiles.size();
@@ -668,6 +679,7 @@ static void compress_level(Context& c, const hmx_config& cfg, const LevelSource&
         HMX_HIP(hipMemcpyAsync(hf, flags.p, sizeof(hf), hipMemcpyDeviceToHost, c.stream));
         HMX_HIP(hipStreamSynchronize(c.stream));
         kmax = hf[1];
+        iters = it + 1;
         if (hf[0] == 0) break;
     }
     Ypart.release();
@@ -713,6 +725,12 @@ static void compress_level(Context& c, const hmx_config& cfg, const LevelSource&
         k_lr_pack<<<g, 256, 0, c.stream>>>(T, m, capQ, st.p, d_off.p, Bf.p, out.V.p);
         HMX_LAUNCHED();
         HMX_HIP(hipStreamSynchronize(c.stream));
+    }
+    if (log_on()) {
+        const double ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - tstart).count();
+        std::fprintf(stderr, "[hmx] level %d (%s): %d tiles m=%d rank sum %lld max %d, %d ARA blocks, %.1f ms\n", level,
+                     direct ? "direct" : "TLR sum", T, m, (long long)acc, kmax, iters, ms);
+        std::fflush(stderr);
     }
 }
 
@@ -838,7 +856,14 @@ void build_hmatrix_direct(Context& c, const hmx_config& cfg, const KDTree& t, co
         lv.level = L - 1;
         lv.maxRank = cfg.maxRankPolicy == HMX_MAXRANK_REF ? b / 2 : b;
         lv.tol = cfg.lowestLevelTolerance;
+        const auto t0 = std::chrono::steady_clock::now();
         compress_leaf_tiles(c, cfg, t, d_points, s.tileIndices[L - 2], lv);
+        if (log_on()) {
+            const double ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+            std::fprintf(stderr, "[hmx] level %d (leaf tiles): %d tiles rank sum %lld, %.1f ms\n", L - 1, lv.numTiles,
+                         (long long)lv.rankSum, ms);
+            std::fflush(stderr);
+        }
     }
     DBuf<double> P((size_t)t.numSegments * b * 4);
     const int64_t np = (int64_t)t.numSegments * b;

@@ -155,16 +155,20 @@ struct MatvecPlan {
 struct MfItem {
     const void* v;
     int32_t xoff, rows, ld, ncols;
-    int64_t out;   // partial slot of the item's first column
-    int32_t nch;   // partial slots per column of this tile
+    int64_t out;   // partial slot of the item's first column (tiles split into several row chunks)
+    int64_t zcol;  // padded column of the item's first column in the group's z
+    int32_t nch;   // row chunks per column of this tile; 1: the item writes z itself
     int32_t pad;
 };
-// Phase-2 entry: the leaf's rows of one tile's U, stored row-major with the
-// tile's padded rank as row stride.
+// Phase-2 entry: the leaf's rows of one tile's U, either a row-major copy with
+// the tile's padded rank as row stride, or (FP64) the level's column-major U
+// in place with column stride ld.
 struct MfEntry {
     const void* u;
     int32_t rpad;
     int32_t zcol;  // first (padded) column of the tile in the group's z
+    int32_t rank;  // columns present in U (column-major form)
+    int32_t ld;    // column stride of the column-major form
 };
 
 // One precision group (f64, f32 or f16) of low-rank factors.
@@ -175,8 +179,10 @@ struct MfGroup {
     DBuf<float> colScale;          // per padded column (f16 group: su*sv), else empty
     DBuf<MfItem> items;
     int32_t numItems = 0;
-    DBuf<int64_t> colPart;         // per padded column
-    DBuf<int32_t> colNpart;
+    int64_t numRed = 0;            // columns summed from row-chunk partials
+    DBuf<int64_t> redCol;          // their padded column
+    DBuf<int64_t> colPart;         // their first partial slot
+    DBuf<int32_t> colNpart;        // their partial count
     DBuf<int32_t> leafPtr;         // [S+1]
     DBuf<MfEntry> entries;
     uint64_t factorBytes = 0;      // V + U bytes of this group (unpadded entries)

@@ -51,7 +51,10 @@ __device__ inline void vzero(T (&out)[N]) {
 template <class T, int CG, int NB>
 __global__ void __launch_bounds__(256) k_mf_phase1(const MfItem* __restrict__ items, int nItems,
                                                    const T* __restrict__ X, int ldx,
-                                                   typename Mfma<T>::acc_t* __restrict__ zpart) {
+                                                   typename Mfma<T>::acc_t* __restrict__ zpart,
+                                                   T* __restrict__ Zt, int64_t ldz,
+                                                   const float* __restrict__ colScale,
+                                                   const float* __restrict__ rhsScale) {
     using M = Mfma<T>;
     constexpr int KL = M::KL, VL = 4 * KL, NR = 16 * NB;
     const int lane = threadIdx.x & 63;
@@ -106,6 +109,26 @@ __global__ void __launch_bounds__(256) k_mf_phase1(const MfItem* __restrict__ it
                 for (int q = 0; q < VL; ++q) bx[nb][q] = bn[nb][q];
         }
     }
+    if (it.nch == 1) {  // whole columns: scaled, converted, stored transposed ([rhs][column])
+        using ACC = typename M::acc_t;
+        ACC rs[NB];
+#pragma unroll
+        for (int nb = 0; nb < NB; ++nb) rs[nb] = rhsScale ? (ACC)rhsScale[16 * nb + lc] : (ACC)1;
+#pragma unroll
+        for (int g = 0; g < CG; ++g)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int col = 16 * g + M::out_row(lg, v);
+                if (col < it.ncols) {
+                    const int64_t zc = it.zcol + col;
+                    const ACC cs = colScale ? (ACC)colScale[zc] : (ACC)1;
+#pragma unroll
+                    for (int nb = 0; nb < NB; ++nb)
+                        Zt[(size_t)(16 * nb + lc) * ldz + zc] = (T)(acc[g][nb][v] * cs * rs[nb]);
+                }
+            }
+        return;
+    }
 #pragma unroll
     for (int g = 0; g < CG; ++g)
 #pragma unroll
@@ -119,24 +142,27 @@ __global__ void __launch_bounds__(256) k_mf_phase1(const MfItem* __restrict__ it
 }
 
 // ---- reduce partials, scale, convert, transpose ------------------------------
+// nred columns split into row chunks: g-th such column is padded column
+// redCol[g] with partials colPart[g] .. + colNpart[g]
 template <class T, class ACC, int NR>
-__global__ void __launch_bounds__(256) k_mf_reduce(int64_t ncols, const int64_t* __restrict__ colPart,
+__global__ void __launch_bounds__(256) k_mf_reduce(int64_t nred, int64_t ldz, const int64_t* __restrict__ redCol,
+                                                   const int64_t* __restrict__ colPart,
                                                    const int32_t* __restrict__ colNpart,
                                                    const float* __restrict__ colScale,
                                                    const float* __restrict__ rhsScale, const ACC* __restrict__ zpart,
                                                    T* __restrict__ Zt) {
-    // read along the right-hand sides (contiguous), write along columns (contiguous)
+    // read along the right-hand sides (contiguous), write along columns
     __shared__ ACC tile[64][NR + 1];
     const int64_t g0 = (int64_t)blockIdx.x * 64;
     for (int o = threadIdx.x; o < 64 * NR; o += 256) {
         const int c = o / NR, s = o % NR;
         const int64_t g = g0 + c;
         ACC v = 0;
-        if (g < ncols) {
+        if (g < nred) {
             const int np = colNpart[g];
             const int64_t p0 = colPart[g];
             for (int q = 0; q < np; ++q) v += zpart[(p0 + q) * NR + s];
-            if (colScale) v *= (ACC)colScale[g];
+            if (colScale) v *= (ACC)colScale[redCol[g]];
             if (rhsScale) v *= (ACC)rhsScale[s];
         }
         tile[c][s] = v;
@@ -145,7 +171,7 @@ __global__ void __launch_bounds__(256) k_mf_reduce(int64_t ncols, const int64_t*
     for (int o = threadIdx.x; o < 64 * NR; o += 256) {
         const int s = o / 64, c = o % 64;
         const int64_t g = g0 + c;
-        if (g < ncols) Zt[(size_t)s * ncols + g] = (T)tile[c][s];
+        if (g < nred) Zt[(size_t)s * ldz + redCol[g]] = (T)tile[c][s];
     }
 }
 
@@ -154,7 +180,7 @@ __global__ void __launch_bounds__(256) k_mf_reduce(int64_t ncols, const int64_t*
 // rows form RG = BL/16 row groups of 16; wave w owns row group w % RG and
 // every KP-th super-step of the K (column) loop, KP = 4/RG, with w / RG
 // picking which; partials are folded through LDS at the end.
-template <int BL, class TD, class TA, class TB, int NB, class TO>
+template <int BL, bool UCM, class TD, class TA, class TB, int NB, class TO>
 __global__ void __launch_bounds__(256) k_mf_phase2(const int32_t* __restrict__ ptrA, const MfEntry* __restrict__ entA,
                                                    const TA* __restrict__ ZA, int64_t ldzA,
                                                    const int32_t* __restrict__ ptrB, const MfEntry* __restrict__ entB,
@@ -195,13 +221,18 @@ __global__ void __launch_bounds__(256) k_mf_phase2(const int32_t* __restrict__ p
         constexpr int KL = MA::KL, VL = 4 * KL, SS = 16 * KL;
         for (int e = ptrA[leaf]; e < ptrA[leaf + 1]; ++e) {
             const MfEntry en = entA[e];
-            const TA* __restrict__ U = (const TA*)en.u + (size_t)row * en.rpad;
+            const TA* __restrict__ U = UCM ? (const TA*)en.u + row : (const TA*)en.u + (size_t)row * en.rpad;
             const TA* __restrict__ Z = ZA + en.zcol;
             for (int c0 = kp * SS; c0 < en.rpad; c0 += KP * SS) {
                 const int cc = c0 + VL * lg;
                 TA a[VL], bz[NB][VL];
                 if (cc < en.rpad) {
-                    vload(U + cc, a);
+                    if (UCM) {  // column-major U in place: one element per column, zero past the rank
+#pragma unroll
+                        for (int q = 0; q < VL; ++q) a[q] = cc + q < en.rank ? U[(size_t)(cc + q) * en.ld] : (TA)0;
+                    } else {
+                        vload(U + cc, a);
+                    }
 #pragma unroll
                     for (int nb = 0; nb < NB; ++nb) vload(Z + (size_t)(16 * nb + lc) * ldzA + cc, bz[nb]);
                 } else {
@@ -407,12 +438,14 @@ struct GroupTile {
 void build_group(Context& c, HMatrix& h, MfGroup& G, int type, const std::vector<GroupTile>& tiles, bool vdirect,
                  const std::vector<float>* su, const std::vector<float>* sv,
                  const std::vector<int64_t>* colBase) {
+    // FP64: both factors are used where the build left them (V and U column-major)
+    const bool udirect = vdirect;
     const int b = (int)h.b;
     const int S = (int)h.S;
     const size_t es = tsize(type);
     G.type = type;
     std::vector<MfItem> items;
-    std::vector<int64_t> colPart;
+    std::vector<int64_t> redCol, colPart;
     std::vector<int32_t> colNpart;
     std::vector<float> colScale;
     std::vector<std::vector<MfEntry>> leafE(S);
@@ -441,8 +474,9 @@ void build_group(Context& c, HMatrix& h, MfGroup& G, int type, const std::vector
             colScale.push_back(fu * fv);
         }
         for (int q = cnt; q < rpad; ++q) colScale.push_back(0.f);
-        for (int r0 = 0; r0 < m; r0 += 32)
-            for (int c0 = 0; c0 < rpad; c0 += 32) trt.push_back(TrTile{cd0, uElems, m, cnt, rpad, r0, c0, 0});
+        if (!udirect)
+            for (int r0 = 0; r0 < m; r0 += 32)
+                for (int c0 = 0; c0 < rpad; c0 += 32) trt.push_back(TrTile{cd0, uElems, m, cnt, rpad, r0, c0, 0});
         // phase-1 items
         const int nch = (m + kMfRows - 1) / kMfRows;
         const int rows = std::min(m, kMfRows);
@@ -456,21 +490,29 @@ void build_group(Context& c, HMatrix& h, MfGroup& G, int type, const std::vector
                 it.rows = rows;
                 it.ld = m;
                 it.ncols = std::min(kMfCols, cnt - c0);
-                it.out = slot + (int64_t)c0 * nch + q;
+                it.out = nch > 1 ? slot + (int64_t)c0 * nch + q : -1;
+                it.zcol = zcol + c0;
                 it.nch = nch;
                 it.pad = 0;
                 items.push_back(it);
             }
-        for (int q = 0; q < rpad; ++q) {
-            colPart.push_back(q < cnt ? slot + (int64_t)q * nch : 0);
-            colNpart.push_back(q < cnt ? nch : 0);
-        }
+        if (nch > 1)
+            for (int q = 0; q < cnt; ++q) {
+                redCol.push_back(zcol + q);
+                colPart.push_back(slot + (int64_t)q * nch);
+                colNpart.push_back(nch);
+            }
         // phase-2 entries
         const int bus = m / b;
-        for (int q = 0; q < bus; ++q)
-            leafE[R.outCl * bus + q].push_back(
-                MfEntry{(const void*)(uintptr_t)(((size_t)uElems + (size_t)q * b * rpad) * es), rpad, (int32_t)zcol});
-        slot += (int64_t)cnt * nch;
+        for (int q = 0; q < bus; ++q) {
+            if (udirect)  // the leaf's first row inside the tile's column-major U
+                leafE[R.outCl * bus + q].push_back(
+                    MfEntry{(const void*)(Usrc + (size_t)q * b), rpad, (int32_t)zcol, cnt, m});
+            else
+                leafE[R.outCl * bus + q].push_back(MfEntry{
+                    (const void*)(uintptr_t)(((size_t)uElems + (size_t)q * b * rpad) * es), rpad, (int32_t)zcol, cnt, 0});
+        }
+        if (nch > 1) slot += (int64_t)cnt * nch;
         zcol += rpad;
         vElems += (int64_t)cnt * m;
         uElems += (int64_t)m * rpad;
@@ -481,7 +523,7 @@ void build_group(Context& c, HMatrix& h, MfGroup& G, int type, const std::vector
     G.factorBytes = fbytes;
     // storage
     if (!vdirect) G.V.alloc(std::max<int64_t>(1, vElems) * es);
-    G.Urm.alloc(std::max<int64_t>(1, uElems) * es);
+    if (!udirect) G.Urm.alloc(std::max<int64_t>(1, uElems) * es);
     for (auto& it : items)
         if (!vdirect) it.v = (const void*)(G.V.p + (uintptr_t)it.v);
     std::vector<int32_t> leafPtr(S + 1, 0);
@@ -489,13 +531,15 @@ void build_group(Context& c, HMatrix& h, MfGroup& G, int type, const std::vector
     for (int leaf = 0; leaf < S; ++leaf) {
         leafPtr[leaf] = (int32_t)ents.size();
         for (MfEntry e : leafE[leaf]) {
-            e.u = (const void*)(G.Urm.p + (uintptr_t)e.u);
+            if (!udirect) e.u = (const void*)(G.Urm.p + (uintptr_t)e.u);
             ents.push_back(e);
         }
     }
     leafPtr[S] = (int32_t)ents.size();
     G.numItems = (int32_t)items.size();
     upload(c, G.items, items);
+    G.numRed = (int64_t)redCol.size();
+    upload(c, G.redCol, redCol);
     upload(c, G.colPart, colPart);
     upload(c, G.colNpart, colNpart);
     upload(c, G.leafPtr, leafPtr);
@@ -553,30 +597,36 @@ void build_fp64_plan(Context& c, HMatrix& h) {
 
 template <class ACC, class T>
 void reduce_group(Context& c, MfGroup& G, int NR, const float* rhsScale) {
-    if (G.numColsPad == 0) return;
-    const unsigned blocks = (unsigned)((G.numColsPad + 63) / 64);
+    if (G.numRed == 0) return;
+    const unsigned blocks = (unsigned)((G.numRed + 63) / 64);
     const float* cs = G.colScale.n ? G.colScale.p : nullptr;
     if (NR == 64)
-        k_mf_reduce<T, ACC, 64><<<blocks, 256, 0, c.stream>>>(G.numColsPad, G.colPart.p, G.colNpart.p, cs, rhsScale,
-                                                              (const ACC*)G.zpart.p, (T*)G.zt.p);
+        k_mf_reduce<T, ACC, 64><<<blocks, 256, 0, c.stream>>>(G.numRed, G.numColsPad, G.redCol.p, G.colPart.p,
+                                                              G.colNpart.p, cs, rhsScale, (const ACC*)G.zpart.p,
+                                                              (T*)G.zt.p);
     else
-        k_mf_reduce<T, ACC, 16><<<blocks, 256, 0, c.stream>>>(G.numColsPad, G.colPart.p, G.colNpart.p, cs, rhsScale,
-                                                              (const ACC*)G.zpart.p, (T*)G.zt.p);
+        k_mf_reduce<T, ACC, 16><<<blocks, 256, 0, c.stream>>>(G.numRed, G.numColsPad, G.redCol.p, G.colPart.p,
+                                                              G.colNpart.p, cs, rhsScale, (const ACC*)G.zpart.p,
+                                                              (T*)G.zt.p);
     HMX_LAUNCHED();
 }
 
-void ensure_ws(MfGroup& G, int NR, size_t accBytes) {
+// Workspaces.  The padding columns of z (rank rounded up to 16) are never
+// written by the kernels and stay at the zero set here.
+void ensure_ws(Context& c, MfGroup& G, int NR, size_t accBytes) {
     if (G.wsNR >= (size_t)NR) return;
     G.zpart.alloc((size_t)std::max<int64_t>(1, G.numPart) * NR * accBytes);
     G.zt.alloc((size_t)std::max<int64_t>(1, G.numColsPad) * NR * tsize(G.type));
+    HMX_HIP(hipMemsetAsync(G.zt.p, 0, G.zt.bytes(), c.stream));
     G.wsNR = NR;
 }
 
 template <class T, int CG, int NB>
-void phase1(Context& c, MfGroup& G, const T* X, int ldx) {
+void phase1(Context& c, MfGroup& G, const T* X, int ldx, const float* rhsScale) {
     if (G.numItems == 0) return;
+    const float* cs = G.colScale.n ? G.colScale.p : nullptr;
     k_mf_phase1<T, CG, NB><<<(unsigned)((G.numItems + 3) / 4), 256, 0, c.stream>>>(
-        G.items.p, G.numItems, X, ldx, (typename Mfma<T>::acc_t*)G.zpart.p);
+        G.items.p, G.numItems, X, ldx, (typename Mfma<T>::acc_t*)G.zpart.p, (T*)G.zt.p, G.numColsPad, cs, rhsScale);
     HMX_LAUNCHED();
 }
 
@@ -602,12 +652,12 @@ void run_fp64(Context& c, HMatrix& h, const double* X, double* Y, Events& ev) {
     MfGroup& G = P.g[0];
     const int ldx = (int)h.n;
     ev.rec(c, 0);
-    phase1<double, 4, NB>(c, G, X, ldx);  // 4 column groups = the 64 columns of an item
+    phase1<double, 4, NB>(c, G, X, ldx, nullptr);  // 4 column groups = the 64 columns of an item
     ev.rec(c, 1);
     reduce_group<double, double>(c, G, 16 * NB, nullptr);
     ev.rec(c, 2);
-    auto k2 = h.b == 64 ? k_mf_phase2<64, double, double, double, NB, double>
-                        : k_mf_phase2<32, double, double, double, NB, double>;
+    auto k2 = h.b == 64 ? k_mf_phase2<64, true, double, double, double, NB, double>
+                        : k_mf_phase2<32, true, double, double, double, NB, double>;
     k2<<<h.S, 256, 0, c.stream>>>(G.leafPtr.p, G.entries.p, (const double*)G.zt.p, G.numColsPad, nullptr, nullptr,
                                   nullptr, 0, (const double*)P.Drm.p, X, ldx, Y, ldx);
     HMX_LAUNCHED();
@@ -627,14 +677,14 @@ void run_mixed(Context& c, HMatrix& h, const float* X, float* Y, Events& ev) {
     const int64_t tot = (int64_t)n * NR;
     k_to_f16<<<(unsigned)((tot + 255) / 256), 256, 0, c.stream>>>(tot, n, X, P.xscale.p, P.X16.p);
     HMX_LAUNCHED();
-    phase1<float, 4, NB>(c, Ghi, X, n);
-    phase1<_Float16, 4, NB>(c, Glo, P.X16.p, n);
+    phase1<float, 4, NB>(c, Ghi, X, n, nullptr);
+    phase1<_Float16, 4, NB>(c, Glo, P.X16.p, n, P.xscale.p);
     ev.rec(c, 1);
     reduce_group<float, float>(c, Ghi, NR, nullptr);
     reduce_group<float, _Float16>(c, Glo, NR, P.xscale.p);
     ev.rec(c, 2);
-    auto k2 = h.b == 64 ? k_mf_phase2<64, float, float, _Float16, NB, float>
-                        : k_mf_phase2<32, float, float, _Float16, NB, float>;
+    auto k2 = h.b == 64 ? k_mf_phase2<64, false, float, float, _Float16, NB, float>
+                        : k_mf_phase2<32, false, float, float, _Float16, NB, float>;
     k2<<<h.S, 256, 0, c.stream>>>(Ghi.leafPtr.p, Ghi.entries.p, (const float*)Ghi.zt.p, Ghi.numColsPad,
                                   Glo.leafPtr.p, Glo.entries.p, (const _Float16*)Glo.zt.p, Glo.numColsPad,
                                   (const float*)P.Drm.p, X, n, Y, n);
@@ -662,7 +712,7 @@ void hmatvec_multi(Context& c, HMatrix& h, uint32_t k, const double* d_in, doubl
     if (!h.mf64.built) build_fp64_plan(c, h);
     MfPlan& P = h.mf64;
     const int NB = (k % 64 == 0) ? 4 : 1;
-    ensure_ws(P.g[0], 16 * NB, sizeof(double));
+    ensure_ws(c, P.g[0], 16 * NB, sizeof(double));
     Events ev;
     if (ms) ev.create();
     const int reps = ms ? std::max(1, iters) : 1;
@@ -768,8 +818,8 @@ void hmatvec_mixed(Context& c, HMatrix& h, uint32_t k, const float* d_in, float*
     MfPlan& P = h.mfMixed;
     const int NB = (k % 64 == 0) ? 4 : 1;
     const int NR = 16 * NB;
-    ensure_ws(P.g[0], NR, sizeof(float));
-    ensure_ws(P.g[1], NR, sizeof(float));
+    ensure_ws(c, P.g[0], NR, sizeof(float));
+    ensure_ws(c, P.g[1], NR, sizeof(float));
     if (P.x16n < (size_t)h.n * NR) {
         P.X16.alloc((size_t)h.n * NR);
         P.xscale.alloc(NR);

@@ -291,6 +291,23 @@ class HMatrix(_Obj):
             o += r
         return ranks, Us, Vs
 
+    def level_ranks(self, ctx, level):
+        """Tile ranks of one level (host array, no factor copy)."""
+        li = self.level_info(level)
+        ranks = np.zeros(max(li["numTiles"], 1), dtype=np.int32)
+        _check(lib().hmx_hmatrix_level_copy(ctx.handle, self.handle, level, _np(ranks), None, None), ctx)
+        return ranks[:li["numTiles"]]
+
+    def level_device(self, ctx, level):
+        """Device copies (torch float64, flat) of one level's U and V."""
+        import torch
+        li = self.level_info(level)
+        m = (1 << (self.num_levels - 1 - level)) * self.b
+        u = torch.empty(max(li["rankSum"] * m, 1), dtype=torch.float64, device="cuda")
+        v = torch.empty_like(u)
+        _check(lib().hmx_hmatrix_level_get(ctx.handle, self.handle, level, _ptr(u), _ptr(v)), ctx)
+        return u, v
+
     def level_raw(self, ctx, level):
         """(ranks, exclusive column offsets, flat U, flat V) of one level."""
         li = self.level_info(level)
