@@ -67,6 +67,7 @@ typedef struct hmx_context hmx_context;
 typedef struct hmx_kdtree hmx_kdtree;
 typedef struct hmx_structure hmx_structure;
 typedef struct hmx_tlr hmx_tlr;
+typedef struct hmx_batch hmx_batch;
 typedef struct hmx_hmatrix hmx_hmatrix;
 
 /* ---- lifecycle -------------------------------------------------------- */
@@ -165,6 +166,12 @@ hmx_status hmx_generateHMatrixFromStruct(hmx_context* ctx, const hmx_config* cfg
    kernel evaluations.  bucketSize 32 or 64.  d_points as for the KD-tree. */
 hmx_status hmx_generateHMatrixDirect(hmx_context* ctx, const hmx_config* cfg, const hmx_kdtree* tree,
                                      const hmx_structure* structure, const double* d_points, hmx_hmatrix** hmatrix);
+/* Binary H-matrix file (no counterpart in the reference, which only prints
+   text dumps, src/helperFunctions.cu:76-169): save an assembled H-matrix,
+   load it back on this context's device for matvecs without rebuilding.
+   The loaded handle owns its copy of the structure. */
+hmx_status hmx_hmatrix_save(hmx_context* ctx, const hmx_hmatrix* h, const char* path);
+hmx_status hmx_hmatrix_load(hmx_context* ctx, const char* path, hmx_hmatrix** hmatrix);
 /* per level l (1..numLevels-1): tile count, rank sum, max rank, tolerance used, cap hits */
 hmx_status hmx_hmatrix_level_info(const hmx_hmatrix* h, int32_t level, int32_t* numTiles, int64_t* rankSum,
                                   int32_t* maxRank, float* tolerance, int32_t* capHits);
@@ -213,6 +220,12 @@ hmx_status hmx_hmatrix_make_mixed(hmx_context* ctx, hmx_hmatrix* h, double targe
    of the FP16 columns, bytes read per call */
 hmx_status hmx_hmatrix_mixed_info(const hmx_hmatrix* h, int64_t* hiCols, int64_t* loCols, double* frobH2,
                                   double* loEnergy, uint64_t* bytes);
+/* Matrix-core plan summary (plan 0: FP64 multi-RHS, built on the first
+   multi-RHS call; plan 1: mixed, built by hmx_hmatrix_make_mixed): bytes read
+   per call (factors + dense blocks), leaf-level tiles applied as dense blocks
+   (rank above bucketSize/2), low-rank columns.  HMX_ESTATE if not built. */
+hmx_status hmx_hmatrix_plan_info(const hmx_hmatrix* h, int32_t plan, uint64_t* bytes, int64_t* denseBlocks,
+                                 int64_t* lowRankColumns);
 /* Y = H X with X, Y n x vectorWidth column-major FP32 (vectorWidth a multiple
    of 16); FP16/FP32 matrix cores with FP32 accumulation */
 hmx_status hmx_HMatrixVecMultMixed(hmx_context* ctx, const hmx_hmatrix* h, uint32_t vectorWidth, const float* d_in,
@@ -246,6 +259,27 @@ hmx_status hmx_hmatrix_set_level(hmx_context* ctx, hmx_hmatrix* h, int32_t level
 hmx_status hmx_denseRowsXVector(hmx_context* ctx, uint32_t n, uint32_t d, const double* d_points,
                                 const int32_t* d_perm, double correlationLength, uint32_t nrows,
                                 const int32_t* d_rows, uint32_t k, const double* d_X, double* d_Y);
+
+/* ---- many point clouds (no counterpart in the reference, whose driver
+   handles one point cloud per run, src/main.cu:46-173) ------------------- */
+/* Concatenate the matvec plans of `count` H-matrices (same bucketSize, any n):
+   the batch applies Y = blockdiag(H_0, H_1, ...) X with one launch per phase.
+   Instance i's entries sit at positions [sum_{j<i} n_j, sum_{j<=i} n_j) of every
+   right-hand side (length sum n_j, KD order of each instance).  The H-matrices
+   must outlive the batch. */
+hmx_status hmx_batch_create(hmx_context* ctx, int32_t count, hmx_hmatrix* const* hmatrices, hmx_batch** batch);
+hmx_status hmx_batch_info(const hmx_batch* batch, int64_t* totalPoints, int64_t* totalLeaves, int32_t* chunk);
+/* FP64, vectorWidth 1 or a multiple of 4/8/16.  useGraph != 0: the launch
+   sequence is captured in a hipGraph on first use (again when d_in, d_out or
+   vectorWidth change) and replayed, ordered after prior work on the context
+   stream. */
+hmx_status hmx_batch_matvec(hmx_context* ctx, hmx_batch* batch, uint32_t vectorWidth, const double* d_in,
+                            double* d_out, int32_t useGraph);
+/* Autotuner: times the phase-1 chunk lengths 512, 1024, 2048, 4096 at
+   vectorWidth on the batch and keeps the fastest. */
+hmx_status hmx_batch_tune(hmx_context* ctx, hmx_batch* batch, uint32_t vectorWidth, int32_t* bestChunk,
+                          float* bestMs);
+hmx_status hmx_batch_free(hmx_batch* batch);
 
 #ifdef __cplusplus
 }

@@ -1,0 +1,118 @@
+// Batched H-matvec over many independent H-matrices (point clouds of uneven
+// sizes): one concatenated plan applied by one launch per phase, optionally
+// captured once in a hipGraph and replayed, with the phase-1 chunk length
+// picked by timing.  No counterpart in the reference, whose driver handles one
+// point cloud per run (src/main.cu:46-173).
+#include <algorithm>
+
+#include "hmx_internal.h"
+
+namespace hmx {
+
+void Batch::drop_graph() {
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    exec = nullptr;
+    graph = nullptr;
+    gX = nullptr;
+    gY = nullptr;
+    gk = 0;
+}
+
+Batch::~Batch() {
+    drop_graph();
+    if (evIn) (void)hipEventDestroy(evIn);
+    if (evOut) (void)hipEventDestroy(evOut);
+    if (stream) (void)hipStreamDestroy(stream);
+}
+
+void batch_create(Context& c, const std::vector<HMatrix*>& hs, Batch& B) {
+    for (HMatrix* h : hs) HMX_CHECK(h != nullptr, HMX_EINVAL, "null H-matrix in batch");
+    B.hs = hs;
+    B.b = (int)hs.at(0)->b;
+    build_batch_plan(c, hs, 2048, B.P, B.diag);
+    HMX_HIP(hipStreamCreate(&B.stream));  // default priority
+    HMX_HIP(hipEventCreateWithFlags(&B.evIn, hipEventDisableTiming));
+    HMX_HIP(hipEventCreateWithFlags(&B.evOut, hipEventDisableTiming));
+}
+
+void batch_matvec(Context& c, Batch& B, uint32_t k, const double* X, double* Y, bool useGraph) {
+    HMX_CHECK(k >= 1, HMX_EINVAL, "vectorWidth must be >= 1");
+    HMX_CHECK(X && Y && X != Y, HMX_EINVAL, "bad vector pointers");
+    if (!useGraph) {
+        run_matvec_plan(c, B.P, B.b, k, X, B.P.n, Y, nullptr);
+        return;
+    }
+    if (!B.exec || B.gX != X || B.gY != Y || B.gk != k) {
+        B.drop_graph();
+        // workspaces are allocated here, outside the capture; the captured
+        // sequence holds kernel launches only (gather, phase 1, reduce,
+        // phase 2, scatter per pass)
+        const uint32_t K = k == 1 ? 1 : (k <= 4 ? 4 : (k <= 8 ? 8 : 16));
+        HMX_CHECK(k == 1 || k % K == 0, HMX_EINVAL, "vectorWidth must be 1 or a multiple of 4/8/16");
+        ensure_matvec_ws(B.P, K);
+        HMX_HIP(hipStreamSynchronize(c.stream));
+        Context cap;
+        cap.stream = B.stream;
+        cap.device = c.device;
+        HMX_HIP(hipStreamBeginCapture(B.stream, hipStreamCaptureModeThreadLocal));
+        try {
+            run_matvec_plan(cap, B.P, B.b, k, X, B.P.n, Y, nullptr);
+        } catch (...) {
+            hipGraph_t g = nullptr;
+            (void)hipStreamEndCapture(B.stream, &g);
+            if (g) (void)hipGraphDestroy(g);
+            throw;
+        }
+        HMX_HIP(hipStreamEndCapture(B.stream, &B.graph));
+        HMX_HIP(hipGraphInstantiate(&B.exec, B.graph, nullptr, nullptr, 0));
+        B.gX = X;
+        B.gY = Y;
+        B.gk = k;
+    }
+    // order the replay after the caller's prior work and the caller's later work after it
+    HMX_HIP(hipEventRecord(B.evIn, c.stream));
+    HMX_HIP(hipStreamWaitEvent(B.stream, B.evIn, 0));
+    HMX_HIP(hipGraphLaunch(B.exec, B.stream));
+    HMX_HIP(hipEventRecord(B.evOut, B.stream));
+    HMX_HIP(hipStreamWaitEvent(c.stream, B.evOut, 0));
+}
+
+void batch_tune(Context& c, Batch& B, uint32_t k, int* bestChunk, float* bestMs) {
+    const int cands[] = {512, 1024, 2048, 4096};
+    DBuf<double> X((size_t)B.P.n * k), Y((size_t)B.P.n * k);
+    HMX_HIP(hipMemsetAsync(X.p, 0, X.bytes(), c.stream));
+    hipEvent_t e0, e1;
+    HMX_HIP(hipEventCreate(&e0));
+    HMX_HIP(hipEventCreate(&e1));
+    int best = 2048;
+    float bms = 1e30f;
+    try {
+        for (int ch : cands) {
+            B.drop_graph();
+            build_batch_plan(c, B.hs, ch, B.P, B.diag);
+            for (int w = 0; w < 2; ++w) run_matvec_plan(c, B.P, B.b, k, X.p, B.P.n, Y.p, nullptr);
+            HMX_HIP(hipEventRecord(e0, c.stream));
+            const int reps = 5;
+            for (int r = 0; r < reps; ++r) run_matvec_plan(c, B.P, B.b, k, X.p, B.P.n, Y.p, nullptr);
+            HMX_HIP(hipEventRecord(e1, c.stream));
+            HMX_HIP(hipEventSynchronize(e1));
+            float t = 0;
+            HMX_HIP(hipEventElapsedTime(&t, e0, e1));
+            t /= reps;
+            if (t < bms) { bms = t; best = ch; }
+        }
+        B.drop_graph();
+        build_batch_plan(c, B.hs, best, B.P, B.diag);
+    } catch (...) {
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        throw;
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (bestChunk) *bestChunk = best;
+    if (bestMs) *bestMs = bms;
+}
+
+}  // namespace hmx

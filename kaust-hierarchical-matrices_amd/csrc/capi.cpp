@@ -409,7 +409,7 @@ hmx_status hmx_HMatrixVecMult(hmx_context* ctx, const hmx_hmatrix* h, uint32_t v
     if (!ctx) return HMX_EINVAL;
     return guard(ctx, [&] {
         NEED(h);
-        if (vectorWidth >= 16 && vectorWidth % 16 == 0)
+        if (vectorWidth >= 16 && vectorWidth % 16 == 0 && h->n == h->S * h->b)
             hmatvec_multi(*ctx, const_cast<hmx_hmatrix&>(*h), vectorWidth, d_in, d_out, nullptr, 0);
         else
             hmatvec(*ctx, const_cast<hmx_hmatrix&>(*h), vectorWidth, d_in, d_out, false, nullptr, 0);
@@ -533,6 +533,98 @@ hmx_status hmx_denseRowsXVector(hmx_context* ctx, uint32_t n, uint32_t d, const 
         dense_rows(*ctx, n, d, d_points, d_perm, correlationLength > 0 ? correlationLength : default_ell(d), nrows,
                    d_rows, k, d_X, d_Y);
     });
+}
+
+hmx_status hmx_hmatrix_save(hmx_context* ctx, const hmx_hmatrix* h, const char* path) {
+    if (!ctx) return HMX_EINVAL;
+    return guard(ctx, [&] {
+        NEED(h);
+        hmatrix_save(*ctx, *h, path);
+    });
+}
+
+hmx_status hmx_hmatrix_load(hmx_context* ctx, const char* path, hmx_hmatrix** hmatrix) {
+    if (!ctx) return HMX_EINVAL;
+    return guard(ctx, [&] {
+        NEED(hmatrix);
+        *hmatrix = nullptr;
+        hmx_hmatrix* h = new hmx_hmatrix();
+        try {
+            hmatrix_load(*ctx, path, *h);
+        } catch (...) {
+            delete h;
+            throw;
+        }
+        *hmatrix = h;
+    });
+}
+
+hmx_status hmx_hmatrix_plan_info(const hmx_hmatrix* h, int32_t plan, uint64_t* bytes, int64_t* denseBlocks,
+                                 int64_t* lowRankColumns) {
+    if (!h || (plan != 0 && plan != 1)) return HMX_EINVAL;
+    const MfPlan& P = plan == 0 ? h->mf64 : h->mfMixed;
+    if (!P.built) return HMX_ESTATE;
+    if (bytes) *bytes = P.bytes;
+    if (denseBlocks) *denseBlocks = P.numNear;
+    if (lowRankColumns) *lowRankColumns = plan == 0 ? P.g[0].numColsPad : P.hiCols + P.loCols;
+    return HMX_OK;
+}
+
+hmx_status hmx_batch_create(hmx_context* ctx, int32_t count, hmx_hmatrix* const* hmatrices, hmx_batch** batch) {
+    if (!ctx) return HMX_EINVAL;
+    return guard(ctx, [&] {
+        NEED(hmatrices);
+        NEED(batch);
+        HMX_CHECK(count >= 1, HMX_EINVAL, "batch needs at least one H-matrix");
+        *batch = nullptr;
+        std::vector<HMatrix*> hs;
+        for (int32_t i = 0; i < count; ++i) {
+            HMX_CHECK(hmatrices[i] != nullptr, HMX_EINVAL, "null H-matrix in batch");
+            hs.push_back(hmatrices[i]);
+        }
+        hmx_batch* b = new hmx_batch();
+        try {
+            batch_create(*ctx, hs, *b);
+        } catch (...) {
+            delete b;
+            throw;
+        }
+        *batch = b;
+    });
+}
+
+hmx_status hmx_batch_info(const hmx_batch* batch, int64_t* totalPoints, int64_t* totalLeaves, int32_t* chunk) {
+    if (!batch) return HMX_EINVAL;
+    if (totalPoints) *totalPoints = batch->P.n;
+    if (totalLeaves) *totalLeaves = batch->P.numLeaves;
+    if (chunk) *chunk = batch->P.chunk;
+    return HMX_OK;
+}
+
+hmx_status hmx_batch_matvec(hmx_context* ctx, hmx_batch* batch, uint32_t vectorWidth, const double* d_in,
+                            double* d_out, int32_t useGraph) {
+    if (!ctx) return HMX_EINVAL;
+    return guard(ctx, [&] {
+        NEED(batch);
+        batch_matvec(*ctx, *batch, vectorWidth, d_in, d_out, useGraph != 0);
+    });
+}
+
+hmx_status hmx_batch_tune(hmx_context* ctx, hmx_batch* batch, uint32_t vectorWidth, int32_t* bestChunk,
+                          float* bestMs) {
+    if (!ctx) return HMX_EINVAL;
+    return guard(ctx, [&] {
+        NEED(batch);
+        HMX_CHECK(vectorWidth >= 1, HMX_EINVAL, "vectorWidth must be >= 1");
+        int bc = 0;
+        batch_tune(*ctx, *batch, vectorWidth, &bc, bestMs);
+        if (bestChunk) *bestChunk = bc;
+    });
+}
+
+hmx_status hmx_batch_free(hmx_batch* batch) {
+    delete batch;
+    return HMX_OK;
 }
 
 }  // extern "C"

@@ -46,6 +46,7 @@ void hmatvec_grad_factors(Context& c, HMatrix& h, uint32_t k, const double* X, c
                           double* const* dU, double* const* dV) {
     HMX_CHECK(k >= 1, HMX_EINVAL, "vectorWidth must be >= 1");
     HMX_CHECK(X && dY && dU && dV, HMX_EINVAL, "null pointer");
+    HMX_CHECK(h.n == h.S * h.b, HMX_EUNSUPPORTED, "factor gradients need full leaves (n = numSegments * bucketSize)");
     std::vector<GradTile> tiles;
     int64_t g = 0;
     for (int l = 1; l < h.numLevels; ++l) {

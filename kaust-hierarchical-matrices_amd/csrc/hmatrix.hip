@@ -780,6 +780,7 @@ void build_hmatrix(Context& c, const hmx_config& cfg, const KDTree& t, const Str
     HMX_CHECK(t.b <= 64, HMX_EUNSUPPORTED, "hierarchical recompression: bucketSize must be <= 64");
     const int L = s.numLevels, b = (int)t.b;
     h.n = t.n; h.d = t.d; h.b = t.b; h.S = t.numSegments; h.numLevels = L;
+    h.leafOffsets = t.h_offsets;
     h.structureCopy = s;
     h.structure = &h.structureCopy;
     h.levels.clear();
@@ -845,6 +846,7 @@ void build_hmatrix_direct(Context& c, const hmx_config& cfg, const KDTree& t, co
     HMX_CHECK(d_points != nullptr, HMX_EINVAL, "null points");
     const int L = s.numLevels, b = (int)t.b;
     h.n = t.n; h.d = t.d; h.b = t.b; h.S = t.numSegments; h.numLevels = L;
+    h.leafOffsets = t.h_offsets;
     h.structureCopy = s;
     h.structure = &h.structureCopy;
     h.levels.clear();

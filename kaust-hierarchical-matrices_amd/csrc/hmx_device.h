@@ -86,6 +86,15 @@ __host__ __device__ inline uint32_t compact16(uint32_t x) {
 __host__ __device__ inline uint32_t morton_xy(uint32_t col, uint32_t row) { return spread16(col) | (spread16(row) << 1); }
 __host__ __device__ inline uint32_t cm_to_mo(uint32_t S, uint32_t n) { return morton_xy(n / S, n % S); }
 
+// Workgroup i of a launch starts on XCD i % 8 (8 XCDs, each with its own L2).
+// Renumber so that every XCD walks one contiguous range of the work list:
+// neighbouring leaves / tiles (which share right-hand-side blocks) then meet
+// in the same L2.  A bijection on [0, n).
+__device__ inline int xcd_local(int bid, int n) {
+    const int q = n >> 3, r = n & 7, x = bid & 7, k = bid >> 3;
+    return x * q + min(x, r) + k;
+}
+
 // order-preserving map of a double onto uint64 (radix-sort key twiddle)
 __device__ inline uint64_t dkey(double v) {
     uint64_t u = (uint64_t)__double_as_longlong(v);

@@ -132,6 +132,17 @@ struct MvEntry {
     int32_t pad;
 };
 
+// Host form of a matvec plan over the padded vector space (position
+// leaf*b + i for point i of a leaf; padded positions hold zeros).
+struct PlanHost {
+    std::vector<MvItem> items;
+    std::vector<int64_t> colPart;
+    std::vector<int32_t> colNpart;
+    std::vector<int32_t> leafPtr;   // [S+1]
+    std::vector<MvEntry> entries;
+    int64_t numCols = 0, numPart = 0;
+};
+
 // Device-side tables used by the matvec kernels (built once per H-matrix).
 struct MatvecPlan {
     DBuf<MvItem> items;
@@ -139,13 +150,19 @@ struct MatvecPlan {
     DBuf<int64_t> colPart;   // per global column: first partial slot
     DBuf<int32_t> colNpart;  // per global column: number of partial slots
     int64_t numCols = 0, numPart = 0;
+    int32_t numLeaves = 0;
     DBuf<int32_t> leafPtr;   // [S+1] CSR over leaves
     DBuf<MvEntry> entries;
     const double* diag = nullptr;  // D (or D^T for the transpose plan)
     DBuf<double> diagT;            // owned transposed copy
+    int chunk = 2048;              // elements of one single-column phase-1 item
     bool built = false;
+    // partial leaves: vectors are gathered into / scattered from the padded space
+    int64_t n = 0, npad = 0;
+    bool gather = false;
+    DBuf<int32_t> gmap;            // padded position -> vector index, -1 for padding
     // per-call workspaces (grown on demand)
-    DBuf<double> zpart, z;
+    DBuf<double> zpart, z, Xp, Yp;
     uint32_t wsK = 0;
 };
 
@@ -190,11 +207,23 @@ struct MfGroup {
     size_t wsNR = 0;
 };
 
+// Leaf-level tile stored dense in a matrix-core plan (rank above b/2: the
+// b x b block is smaller than its two factors and needs no z).
+struct MfNear {
+    const void* d;     // row-major b x b block (plan's dense type)
+    int32_t colLeaf;   // leaf of the block's columns
+    int32_t pad;
+};
+
 struct MfPlan {
     bool built = false;
     int dtype = 0;                 // 0: FP64 plan; 1: mixed FP32/FP16 plan
     MfGroup g[2];                  // FP64: g[0]; mixed: g[0] = f32 hi, g[1] = f16 lo
     DBuf<unsigned char> Drm;       // diagonal blocks, row-major (f64 or f32)
+    DBuf<unsigned char> nearStore; // dense leaf-level blocks, row-major (f64 or f32)
+    DBuf<int32_t> nearPtr;         // [S+1] CSR over row leaves
+    DBuf<MfNear> near;
+    int64_t numNear = 0;
     DBuf<_Float16> X16;            // converted right-hand sides (mixed)
     DBuf<float> xscale;
     size_t x16n = 0;
@@ -207,6 +236,7 @@ struct MfPlan {
 struct HMatrix {
     uint32_t n = 0, d = 0, b = 0, S = 0;
     int32_t numLevels = 0;
+    std::vector<int32_t> leafOffsets;  // KD-tree segment offsets [S+1]
     DBuf<double> diag;              // [S*b*b]
     std::vector<HLevel> levels;     // index l-1
     const Structure* structure = nullptr;
@@ -217,6 +247,24 @@ struct HMatrix {
     MatvecPlan plan, planT;
     MfPlan mf64, mfMixed;
     uint64_t lowRankBytes = 0, denseBytes = 0, factorEntries = 0;
+};
+
+// Many H-matrices applied as one block-diagonal operator: one concatenated
+// plan, its own stream for hipGraph capture and replay.
+struct Batch {
+    std::vector<HMatrix*> hs;
+    MatvecPlan P;
+    DBuf<double> diag;
+    int b = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t evIn = nullptr, evOut = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    const double* gX = nullptr;  // arguments the graph was captured with
+    double* gY = nullptr;
+    uint32_t gk = 0;
+    void drop_graph();
+    ~Batch();
 };
 
 // ---- internal entry points (implemented in the .hip/.cpp files) ----------
@@ -233,7 +281,18 @@ void build_hmatrix_direct(Context& c, const hmx_config& cfg, const KDTree& t, co
 void compress_leaf_tiles(Context& c, const hmx_config& cfg, const KDTree& t, const double* d_points,
                          const std::vector<int32_t>& tiles, HLevel& out);
 void build_dense_diagonal(Context& c, const hmx_config& cfg, const KDTree& t, const double* d_points, double* diag);
+void hmatrix_save(Context& c, const HMatrix& h, const char* path);
+void hmatrix_load(Context& c, const char* path, HMatrix& h);
 void build_matvec_plan(Context& c, HMatrix& h, bool transpose);
+PlanHost plan_host(const HMatrix& h, bool transpose, int chunk);
+void upload_plan(Context& c, MatvecPlan& P, const PlanHost& H);
+void ensure_matvec_ws(MatvecPlan& P, uint32_t K);
+void run_matvec_plan(Context& c, MatvecPlan& P, int b, uint32_t k, const double* X, int64_t ldx, double* Y,
+                     hipEvent_t* ev);
+void build_batch_plan(Context& c, const std::vector<HMatrix*>& hs, int chunk, MatvecPlan& P, DBuf<double>& diag);
+void batch_create(Context& c, const std::vector<HMatrix*>& hs, Batch& B);
+void batch_matvec(Context& c, Batch& B, uint32_t k, const double* X, double* Y, bool useGraph);
+void batch_tune(Context& c, Batch& B, uint32_t k, int* bestChunk, float* bestMs);
 void hmatvec(Context& c, HMatrix& h, uint32_t k, const double* d_in, double* d_out, bool transpose, float* ms,
              int iters);
 void dense_rows(Context& c, uint32_t n, uint32_t d, const double* p, const int32_t* perm, double ell, uint32_t nr,
@@ -268,3 +327,4 @@ struct hmx_kdtree : hmx::KDTree {};
 struct hmx_structure : hmx::Structure {};
 struct hmx_tlr : hmx::TLR {};
 struct hmx_hmatrix : hmx::HMatrix {};
+struct hmx_batch : hmx::Batch {};

@@ -5,9 +5,10 @@
 // y = D x + sum over all levels and tiles t of U_t (V_t^T x_col(t)), for any
 // admissibility structure, in three launches (all levels at once):
 //   phase 1  z_t = V_t^T x : one wave per work item, a contiguous run of V
-//            read with 16-byte loads; long columns are split into 2048-element
-//            chunks that produce partial sums, short columns are packed many
-//            per item and reduced with segmented wave reductions;
+//            read with 16-byte loads; long columns are split into chunks
+//            (2048 elements by default, tunable) that produce partial sums,
+//            short columns are packed many per item and reduced with segmented
+//            wave reductions;
 //   reduce   z = sum of the chunk partials (fixed order -> deterministic);
 //   phase 2  one workgroup per leaf (32 or 64 rows): y_leaf = D_leaf x_leaf + sum of the
 //            leaf's rows of every U_t covering it times z_t; each y element is
@@ -20,8 +21,6 @@
 #include "hmx_internal.h"
 
 namespace hmx {
-
-constexpr int kChunk = 2048;  // elements of one single-column work item
 
 __device__ inline double wave_sum(double v) {
 #pragma unroll
@@ -219,15 +218,33 @@ __global__ void __launch_bounds__(256) k_transpose_blocks(int S, int b, const do
     DT[leaf * per + (int64_t)q * b + c] = D[e];
 }
 
-void build_matvec_plan(Context& c, HMatrix& h, bool transpose) {
-    MatvecPlan& P = transpose ? h.planT : h.plan;
+// vectors (ldx apart) <-> the padded space (npad apart), K right-hand sides
+__global__ void __launch_bounds__(256) k_gather(int64_t npad, int K, const int32_t* __restrict__ gmap,
+                                                const double* __restrict__ X, int64_t ldx, double* __restrict__ Xp) {
+    const int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (e >= npad * K) return;
+    const int64_t s = e / npad, p = e % npad;
+    const int g = gmap[p];
+    Xp[e] = g >= 0 ? X[s * ldx + g] : 0.0;
+}
+
+__global__ void __launch_bounds__(256) k_scatter(int64_t npad, int K, const int32_t* __restrict__ gmap,
+                                                 const double* __restrict__ Yp, double* __restrict__ Y, int64_t ldy) {
+    const int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (e >= npad * K) return;
+    const int64_t s = e / npad, p = e % npad;
+    const int g = gmap[p];
+    if (g >= 0) Y[s * ldy + g] = Yp[e];
+}
+
+// Work lists of one H-matrix: phase-1 items over the input-side factors,
+// partial-slot bookkeeping, phase-2 entries per output leaf.  Positions are
+// in the padded space (leaf*b + i).
+PlanHost plan_host(const HMatrix& h, bool transpose, int chunk) {
     const int L = h.numLevels, b = (int)h.b, S = (int)h.S;
-    HMX_CHECK(h.n == (uint32_t)S * (uint32_t)b, HMX_EUNSUPPORTED,
-              "HMatrixVecMult requires full leaves (n = numSegments * bucketSize), as the reference does");
     HMX_CHECK(b == 32 || b == 64, HMX_EUNSUPPORTED, "HMatrixVecMult kernels are built for bucketSize 32 or 64");
-    std::vector<MvItem> items;
-    std::vector<int64_t> colPart;
-    std::vector<int32_t> colNpart;
+    HMX_CHECK(chunk >= 128 && (chunk & (chunk - 1)) == 0, HMX_EINVAL, "phase-1 chunk must be a power of two >= 128");
+    PlanHost H;
     std::vector<std::vector<MvEntry>> leafEntries(S);
     int64_t slot = 0, g = 0;
     for (int l = 1; l < L; ++l) {
@@ -242,21 +259,21 @@ void build_matvec_plan(Context& c, HMatrix& h, bool transpose) {
             const uint32_t inCl = transpose ? v : u, outCl = transpose ? u : v;
             const double* Pin = (transpose ? lv.U.p : lv.V.p) + (size_t)lv.h_offsets[t] * m;
             const double* Pout = (transpose ? lv.V.p : lv.U.p) + (size_t)lv.h_offsets[t] * m;
-            const int nch = m >= kChunk ? m / kChunk : 1;
+            const int nch = m >= chunk ? m / chunk : 1;
             for (int j = 0; j < r; ++j) {
-                colPart.push_back(slot + (int64_t)j * nch);
-                colNpart.push_back(nch);
+                H.colPart.push_back(slot + (int64_t)j * nch);
+                H.colNpart.push_back(nch);
             }
-            if (m >= kChunk) {
+            if (m >= chunk) {
                 for (int j = 0; j < r; ++j)
                     for (int q = 0; q < nch; ++q)
-                        items.push_back(MvItem{Pin + (size_t)j * m + (size_t)q * kChunk, (int32_t)(inCl * m + q * kChunk),
-                                               kChunk, 0, (int32_t)(slot + (int64_t)j * nch + q)});
+                        H.items.push_back(MvItem{Pin + (size_t)j * m + (size_t)q * chunk, (int32_t)(inCl * m + q * chunk),
+                                                 chunk, 0, (int32_t)(slot + (int64_t)j * nch + q)});
             } else {
-                const int cpi = kChunk / m;
+                const int cpi = chunk / m;
                 for (int j0 = 0; j0 < r; j0 += cpi) {
                     const int nc = std::min(cpi, r - j0);
-                    items.push_back(MvItem{Pin + (size_t)j0 * m, (int32_t)(inCl * m), nc * m, m, (int32_t)(slot + j0)});
+                    H.items.push_back(MvItem{Pin + (size_t)j0 * m, (int32_t)(inCl * m), nc * m, m, (int32_t)(slot + j0)});
                 }
             }
             for (int q = 0; q < bus; ++q)
@@ -266,30 +283,60 @@ void build_matvec_plan(Context& c, HMatrix& h, bool transpose) {
         }
     }
     HMX_CHECK(slot < (int64_t)INT32_MAX && g < (int64_t)INT32_MAX, HMX_EUNSUPPORTED, "matvec plan too large");
-    std::vector<int32_t> leafPtr(S + 1, 0);
-    std::vector<MvEntry> entries;
+    H.leafPtr.assign(S + 1, 0);
     for (int leaf = 0; leaf < S; ++leaf) {
-        leafPtr[leaf] = (int32_t)entries.size();
-        entries.insert(entries.end(), leafEntries[leaf].begin(), leafEntries[leaf].end());
+        H.leafPtr[leaf] = (int32_t)H.entries.size();
+        H.entries.insert(H.entries.end(), leafEntries[leaf].begin(), leafEntries[leaf].end());
     }
-    leafPtr[S] = (int32_t)entries.size();
-    P.numItems = (int32_t)items.size();
-    P.numCols = g;
-    P.numPart = slot;
-    P.items.alloc(std::max<size_t>(1, items.size()));
-    P.colPart.alloc(std::max<size_t>(1, colPart.size()));
-    P.colNpart.alloc(std::max<size_t>(1, colNpart.size()));
-    P.leafPtr.alloc(S + 1);
-    P.entries.alloc(std::max<size_t>(1, entries.size()));
-    HMX_HIP(hipMemcpyAsync(P.items.p, items.data(), items.size() * sizeof(MvItem), hipMemcpyHostToDevice, c.stream));
-    HMX_HIP(hipMemcpyAsync(P.colPart.p, colPart.data(), colPart.size() * sizeof(int64_t), hipMemcpyHostToDevice,
-                           c.stream));
-    HMX_HIP(hipMemcpyAsync(P.colNpart.p, colNpart.data(), colNpart.size() * sizeof(int32_t), hipMemcpyHostToDevice,
-                           c.stream));
-    HMX_HIP(hipMemcpyAsync(P.leafPtr.p, leafPtr.data(), leafPtr.size() * sizeof(int32_t), hipMemcpyHostToDevice,
-                           c.stream));
-    HMX_HIP(hipMemcpyAsync(P.entries.p, entries.data(), entries.size() * sizeof(MvEntry), hipMemcpyHostToDevice,
-                           c.stream));
+    H.leafPtr[S] = (int32_t)H.entries.size();
+    H.numCols = g;
+    H.numPart = slot;
+    return H;
+}
+
+template <class T>
+static void put(Context& c, DBuf<T>& d, const std::vector<T>& h) {
+    d.alloc(std::max<size_t>(1, h.size()));
+    if (!h.empty()) HMX_HIP(hipMemcpyAsync(d.p, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice, c.stream));
+}
+
+void upload_plan(Context& c, MatvecPlan& P, const PlanHost& H) {
+    P.numItems = (int32_t)H.items.size();
+    P.numCols = H.numCols;
+    P.numPart = H.numPart;
+    P.numLeaves = (int32_t)H.leafPtr.size() - 1;
+    put(c, P.items, H.items);
+    put(c, P.colPart, H.colPart);
+    put(c, P.colNpart, H.colNpart);
+    put(c, P.leafPtr, H.leafPtr);
+    put(c, P.entries, H.entries);
+    P.wsK = 0;  // workspaces are sized for the previous plan
+    HMX_HIP(hipStreamSynchronize(c.stream));
+}
+
+// padded position -> index of the KD-ordered vector (-1 on padding)
+static std::vector<int32_t> padding_map(const HMatrix& h, int64_t base) {
+    const int S = (int)h.S, b = (int)h.b;
+    std::vector<int32_t> gm((size_t)S * b, -1);
+    for (int leaf = 0; leaf < S; ++leaf) {
+        const int a0 = h.leafOffsets[leaf], cnt = h.leafOffsets[leaf + 1] - a0;
+        for (int i = 0; i < cnt; ++i) gm[(size_t)leaf * b + i] = (int32_t)(base + a0 + i);
+    }
+    return gm;
+}
+
+void build_matvec_plan(Context& c, HMatrix& h, bool transpose) {
+    MatvecPlan& P = transpose ? h.planT : h.plan;
+    const int b = (int)h.b, S = (int)h.S;
+    HMX_CHECK(h.leafOffsets.size() == (size_t)S + 1, HMX_ESTATE, "H-matrix without leaf offsets");
+    upload_plan(c, P, plan_host(h, transpose, P.chunk));
+    P.n = h.n;
+    P.npad = (int64_t)S * b;
+    P.gather = h.n != (uint32_t)P.npad;
+    if (P.gather) {
+        put(c, P.gmap, padding_map(h, 0));
+        HMX_HIP(hipStreamSynchronize(c.stream));
+    }
     if (transpose) {
         P.diagT.alloc(h.diag.n);
         int64_t tot = (int64_t)h.diag.n;
@@ -304,11 +351,10 @@ void build_matvec_plan(Context& c, HMatrix& h, bool transpose) {
 }
 
 template <int K>
-static void run_phases(Context& c, MatvecPlan& P, int S, int b, const double* X, int ldx, double* Y,
-                       hipEvent_t* ev) {
+static void run_phases(Context& c, MatvecPlan& P, int b, const double* X, int64_t ldx, double* Y, hipEvent_t* ev) {
     if (ev) HMX_HIP(hipEventRecord(ev[0], c.stream));
     if (P.numItems > 0) {
-        k_mv_phase1<K><<<(unsigned)((P.numItems + 3) / 4), 256, 0, c.stream>>>(P.items.p, P.numItems, X, ldx,
+        k_mv_phase1<K><<<(unsigned)((P.numItems + 3) / 4), 256, 0, c.stream>>>(P.items.p, P.numItems, X, (int)ldx,
                                                                                  P.zpart.p);
         HMX_LAUNCHED();
     }
@@ -320,18 +366,60 @@ static void run_phases(Context& c, MatvecPlan& P, int S, int b, const double* X,
     }
     if (ev) HMX_HIP(hipEventRecord(ev[2], c.stream));
     if (b == 64)
-        k_mv_phase2<64, K><<<(unsigned)S, 256, 0, c.stream>>>(P.leafPtr.p, P.entries.p, P.diag, P.z.p, X, ldx, Y);
+        k_mv_phase2<64, K><<<(unsigned)P.numLeaves, 256, 0, c.stream>>>(P.leafPtr.p, P.entries.p, P.diag, P.z.p, X,
+                                                                         (int)ldx, Y);
     else
-        k_mv_phase2<32, K><<<(unsigned)S, 256, 0, c.stream>>>(P.leafPtr.p, P.entries.p, P.diag, P.z.p, X, ldx, Y);
+        k_mv_phase2<32, K><<<(unsigned)P.numLeaves, 256, 0, c.stream>>>(P.leafPtr.p, P.entries.p, P.diag, P.z.p, X,
+                                                                         (int)ldx, Y);
     HMX_LAUNCHED();
     if (ev) HMX_HIP(hipEventRecord(ev[3], c.stream));
 }
 
+void ensure_matvec_ws(MatvecPlan& P, uint32_t K) {
+    if (P.wsK >= K) return;
+    P.zpart.alloc((size_t)std::max<int64_t>(1, P.numPart) * K);
+    P.z.alloc((size_t)std::max<int64_t>(1, P.numCols) * K);
+    if (P.gather) {
+        P.Xp.alloc((size_t)P.npad * K);
+        P.Yp.alloc((size_t)P.npad * K);
+    }
+    P.wsK = K;
+}
+
+// K = kernel width (1, 4, 8 or 16); k right-hand sides in passes of K.  With
+// partial leaves every pass gathers into and scatters from the padded space.
 template <int K>
-static void run_k(Context& c, MatvecPlan& P, HMatrix& h, uint32_t k, const double* X, double* Y, hipEvent_t* ev) {
-    const int ldx = (int)h.n;
-    for (uint32_t s0 = 0; s0 < k; s0 += K)
-        run_phases<K>(c, P, (int)h.S, (int)h.b, X + (size_t)s0 * ldx, ldx, Y + (size_t)s0 * ldx, ev);
+static void run_k(Context& c, MatvecPlan& P, int b, uint32_t k, const double* X, int64_t ldx, double* Y,
+                  hipEvent_t* ev) {
+    for (uint32_t s0 = 0; s0 < k; s0 += K) {
+        const double* Xs = X + (size_t)s0 * ldx;
+        double* Ys = Y + (size_t)s0 * ldx;
+        if (!P.gather) {
+            run_phases<K>(c, P, b, Xs, ldx, Ys, ev);
+            continue;
+        }
+        const int64_t tot = P.npad * K;
+        k_gather<<<(unsigned)((tot + 255) / 256), 256, 0, c.stream>>>(P.npad, K, P.gmap.p, Xs, ldx, P.Xp.p);
+        HMX_LAUNCHED();
+        run_phases<K>(c, P, b, P.Xp.p, P.npad, P.Yp.p, ev);
+        k_scatter<<<(unsigned)((tot + 255) / 256), 256, 0, c.stream>>>(P.npad, K, P.gmap.p, P.Yp.p, Ys, ldx);
+        HMX_LAUNCHED();
+    }
+}
+
+static uint32_t kernel_width(uint32_t k) { return k == 1 ? 1 : (k <= 4 ? 4 : (k <= 8 ? 8 : 16)); }
+
+void run_matvec_plan(Context& c, MatvecPlan& P, int b, uint32_t k, const double* X, int64_t ldx, double* Y,
+                     hipEvent_t* ev) {
+    const uint32_t K = kernel_width(k);
+    HMX_CHECK(k == 1 || k % K == 0, HMX_EINVAL, "vectorWidth must be 1 or a multiple of 4/8/16");
+    ensure_matvec_ws(P, K);
+    switch (K) {
+        case 1: run_k<1>(c, P, b, k, X, ldx, Y, ev); break;
+        case 4: run_k<4>(c, P, b, k, X, ldx, Y, ev); break;
+        case 8: run_k<8>(c, P, b, k, X, ldx, Y, ev); break;
+        default: run_k<16>(c, P, b, k, X, ldx, Y, ev); break;
+    }
 }
 
 void hmatvec(Context& c, HMatrix& h, uint32_t k, const double* d_in, double* d_out, bool transpose, float* ms,
@@ -341,14 +429,6 @@ void hmatvec(Context& c, HMatrix& h, uint32_t k, const double* d_in, double* d_o
     HMX_CHECK(d_in != d_out, HMX_EINVAL, "in and out must not alias");
     MatvecPlan& P = transpose ? h.planT : h.plan;
     if (!P.built) build_matvec_plan(c, h, transpose);
-    // kernel width: 1, 4, 8 or 16 right-hand sides per pass
-    const uint32_t K = k == 1 ? 1 : (k <= 4 ? 4 : (k <= 8 ? 8 : 16));
-    HMX_CHECK(k == 1 || k % K == 0, HMX_EINVAL, "vectorWidth must be 1 or a multiple of 4/8/16");
-    if (P.wsK < K) {
-        P.zpart.alloc((size_t)std::max<int64_t>(1, P.numPart) * K);
-        P.z.alloc((size_t)std::max<int64_t>(1, P.numCols) * K);
-        P.wsK = K;
-    }
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
     const bool prof = ms != nullptr;
     if (prof)
@@ -356,12 +436,7 @@ void hmatvec(Context& c, HMatrix& h, uint32_t k, const double* d_in, double* d_o
     const int reps = prof ? std::max(1, iters) : 1;
     float acc[4] = {0, 0, 0, 0};
     for (int it = 0; it < reps; ++it) {
-        switch (K) {
-            case 1: run_k<1>(c, P, h, k, d_in, d_out, prof ? ev : nullptr); break;
-            case 4: run_k<4>(c, P, h, k, d_in, d_out, prof ? ev : nullptr); break;
-            case 8: run_k<8>(c, P, h, k, d_in, d_out, prof ? ev : nullptr); break;
-            default: run_k<16>(c, P, h, k, d_in, d_out, prof ? ev : nullptr); break;
-        }
+        run_matvec_plan(c, P, (int)h.b, k, d_in, (int64_t)h.n, d_out, prof ? ev : nullptr);
         if (prof) {
             HMX_HIP(hipEventSynchronize(ev[3]));
             float t;
@@ -377,6 +452,64 @@ void hmatvec(Context& c, HMatrix& h, uint32_t k, const double* d_in, double* d_o
         for (int q = 0; q < 4; ++q) ms[q] = acc[q] / reps;
         for (auto& e : ev) (void)hipEventDestroy(e);
     }
+}
+
+// ---- many H-matrices in one set of launches ----------------------------------
+// The plans of all instances are concatenated: instance i's padded vector
+// space starts at padBase_i, its leaves at leafBase_i, its z columns and
+// partial slots after those of the instances before it; its vectors start at
+// vecBase_i = sum of the previous instances' n inside every right-hand side.
+void build_batch_plan(Context& c, const std::vector<HMatrix*>& hs, int chunk, MatvecPlan& P, DBuf<double>& diag) {
+    HMX_CHECK(!hs.empty(), HMX_EINVAL, "empty batch");
+    const int b = (int)hs[0]->b;
+    PlanHost B;
+    B.leafPtr.push_back(0);
+    std::vector<int32_t> gm;
+    int64_t padBase = 0, vecBase = 0, colBase = 0, slotBase = 0, leafBase = 0;
+    for (HMatrix* h : hs) {
+        HMX_CHECK(h->b == (uint32_t)b, HMX_EINVAL, "all instances of a batch need the same bucketSize");
+        HMX_CHECK(h->leafOffsets.size() == (size_t)h->S + 1, HMX_ESTATE, "H-matrix without leaf offsets");
+        PlanHost H = plan_host(*h, false, chunk);
+        for (MvItem it : H.items) {
+            it.xoff = (int32_t)(it.xoff + padBase);
+            it.out = (int32_t)(it.out + slotBase);
+            B.items.push_back(it);
+        }
+        for (int64_t p : H.colPart) B.colPart.push_back(p + slotBase);
+        B.colNpart.insert(B.colNpart.end(), H.colNpart.begin(), H.colNpart.end());
+        const int32_t e0 = (int32_t)B.entries.size();
+        for (MvEntry en : H.entries) {
+            en.zbase = (int32_t)(en.zbase + colBase);
+            B.entries.push_back(en);
+        }
+        for (size_t q = 1; q < H.leafPtr.size(); ++q) B.leafPtr.push_back(e0 + H.leafPtr[q]);
+        std::vector<int32_t> g = padding_map(*h, vecBase);
+        gm.insert(gm.end(), g.begin(), g.end());
+        padBase += (int64_t)h->S * b;
+        vecBase += h->n;
+        colBase += H.numCols;
+        slotBase += H.numPart;
+        leafBase += h->S;
+    }
+    HMX_CHECK(padBase < (int64_t)INT32_MAX && slotBase < (int64_t)INT32_MAX && colBase < (int64_t)INT32_MAX,
+              HMX_EUNSUPPORTED, "batch too large for 32-bit plan indices");
+    B.numCols = colBase;
+    B.numPart = slotBase;
+    upload_plan(c, P, B);
+    P.chunk = chunk;
+    P.n = vecBase;
+    P.npad = padBase;
+    P.gather = true;
+    put(c, P.gmap, gm);
+    diag.alloc((size_t)leafBase * b * b);
+    size_t off = 0;
+    for (HMatrix* h : hs) {
+        HMX_HIP(hipMemcpyAsync(diag.p + off, h->diag.p, h->diag.bytes(), hipMemcpyDeviceToDevice, c.stream));
+        off += h->diag.n;
+    }
+    P.diag = diag.p;
+    HMX_HIP(hipStreamSynchronize(c.stream));
+    P.built = true;
 }
 
 }  // namespace hmx

@@ -58,7 +58,7 @@ __global__ void __launch_bounds__(256) k_mf_phase1(const MfItem* __restrict__ it
     using M = Mfma<T>;
     constexpr int KL = M::KL, VL = 4 * KL, NR = 16 * NB;
     const int lane = threadIdx.x & 63;
-    const int wid = blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int wid = xcd_local(blockIdx.x, gridDim.x) * 4 + (threadIdx.x >> 6);
     if (wid >= nItems) return;
     const MfItem it = items[wid];
     const T* __restrict__ V = (const T*)it.v;
@@ -185,7 +185,8 @@ __global__ void __launch_bounds__(256) k_mf_phase2(const int32_t* __restrict__ p
                                                    const TA* __restrict__ ZA, int64_t ldzA,
                                                    const int32_t* __restrict__ ptrB, const MfEntry* __restrict__ entB,
                                                    const TB* __restrict__ ZB, int64_t ldzB,
-                                                   const TD* __restrict__ Drm, const TD* __restrict__ X, int ldx,
+                                                   const TD* __restrict__ Drm, const int32_t* __restrict__ nearPtr,
+                                                   const MfNear* __restrict__ nearB, const TD* __restrict__ X, int ldx,
                                                    TO* __restrict__ Y, int ldy) {
     using MA = Mfma<TA>;
     using MB = Mfma<TB>;
@@ -197,24 +198,30 @@ __global__ void __launch_bounds__(256) k_mf_phase2(const int32_t* __restrict__ p
     static_assert(BL == 32 || BL == 64, "bucketSize 32 or 64");
     constexpr int NR = 16 * NB, RG = BL / 16, KP = 4 / RG;
     __shared__ acc_t red[KP][BL][NR + 1];
-    const int leaf = blockIdx.x;
+    const int leaf = xcd_local(blockIdx.x, gridDim.x);
     const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int rg = w % RG, kp = w / RG, lg = lane >> 4, lc = lane & 15;
     const int row = 16 * rg + lc;
     accv acc[NB];
 #pragma unroll
     for (int nb = 0; nb < NB; ++nb) acc[nb] = accv{0, 0, 0, 0};
-    // dense diagonal block (row-major BL x BL): BL/16 super-steps of 16 columns
-    for (int c0 = 16 * kp; c0 < BL; c0 += 16 * KP) {
-        const int cc = c0 + 4 * lg;
-        TD a[4];
-        vload(Drm + (size_t)leaf * BL * BL + row * BL + cc, a);
+    // dense blocks (row-major BL x BL): the diagonal block, then the leaf's
+    // dense near-field blocks; BL/16 super-steps of 16 columns each
+    const int n0 = nearPtr ? nearPtr[leaf] : 0, n1 = nearPtr ? nearPtr[leaf + 1] : 0;
+    for (int q = n0 - 1; q < n1; ++q) {
+        const TD* Dq = q < n0 ? Drm + (size_t)leaf * BL * BL : (const TD*)nearB[q].d;
+        const int cl = q < n0 ? leaf : nearB[q].colLeaf;
+        for (int c0 = 16 * kp; c0 < BL; c0 += 16 * KP) {
+            const int cc = c0 + 4 * lg;
+            TD a[4];
+            vload(Dq + row * BL + cc, a);
 #pragma unroll
-        for (int nb = 0; nb < NB; ++nb) {
-            TD bx[4];
-            vload(X + (size_t)(16 * nb + lc) * ldx + (size_t)leaf * BL + cc, bx);
+            for (int nb = 0; nb < NB; ++nb) {
+                TD bx[4];
+                vload(X + (size_t)(16 * nb + lc) * ldx + (size_t)cl * BL + cc, bx);
 #pragma unroll
-            for (int e = 0; e < 4; ++e) acc[nb] = MD::op(a[e], bx[e], acc[nb]);
+                for (int e = 0; e < 4; ++e) acc[nb] = MD::op(a[e], bx[e], acc[nb]);
+            }
         }
     }
     {
@@ -397,6 +404,24 @@ __global__ void __launch_bounds__(256) k_diag_rowmajor(int64_t total, int b, con
     Drm[e] = (T)D[leaf * bb + (int64_t)c * b + r];  // D is column-major per leaf
 }
 
+// dense leaf-level blocks: out[dst + row*b + col] = sum_a U[a*b + row] V[a*b + col]
+struct NearSrc {
+    const double* U;
+    const double* V;
+    int64_t dst;
+    int32_t r, pad;
+};
+template <class T>
+__global__ void __launch_bounds__(256) k_near_dense(int b, const NearSrc* __restrict__ src, T* __restrict__ out) {
+    const NearSrc t = src[blockIdx.x];
+    for (int e = threadIdx.x; e < b * b; e += 256) {
+        const int row = e / b, col = e % b;
+        double v = 0.0;
+        for (int a = 0; a < t.r; ++a) v = fma(t.U[(size_t)a * b + row], t.V[(size_t)a * b + col], v);
+        out[t.dst + e] = (T)v;
+    }
+}
+
 // ---- host side ------------------------------------------------------------------
 namespace {
 
@@ -573,11 +598,51 @@ void check_shape(const HMatrix& h) {
     HMX_CHECK(h.b == 32 || h.b == 64, HMX_EUNSUPPORTED, "the multi-RHS matvec kernels are built for bucketSize 32 or 64");
 }
 
+// leaf-level tiles of rank above b/2 are applied as dense blocks
+bool is_near(const HMatrix& h, const TileRef& R) { return R.m == (int)h.b && 2 * R.r > (int)h.b; }
+
+template <class T>
+void build_near(Context& c, HMatrix& h, MfPlan& P, const std::vector<TileRef>& tiles) {
+    const int b = (int)h.b, S = (int)h.S;
+    std::vector<std::vector<std::pair<int32_t, int64_t>>> perLeaf(S);
+    std::vector<NearSrc> src;
+    for (const TileRef& R : tiles) {
+        if (!is_near(h, R)) continue;
+        const double* U = R.lv->U.p + (size_t)R.lv->h_offsets[R.t] * R.m;
+        const double* V = R.lv->V.p + (size_t)R.lv->h_offsets[R.t] * R.m;
+        perLeaf[R.outCl].push_back({(int32_t)R.inCl, (int64_t)src.size()});
+        src.push_back(NearSrc{U, V, (int64_t)src.size() * b * b, R.r, 0});
+    }
+    P.numNear = (int64_t)src.size();
+    P.nearStore.alloc((size_t)std::max<int64_t>(1, P.numNear) * b * b * sizeof(T));
+    if (P.numNear) {
+        DBuf<NearSrc> d_src;
+        upload(c, d_src, src);
+        k_near_dense<T><<<(unsigned)P.numNear, 256, 0, c.stream>>>(b, d_src.p, (T*)P.nearStore.p);
+        HMX_LAUNCHED();
+        HMX_HIP(hipStreamSynchronize(c.stream));
+    }
+    std::vector<int32_t> ptr(S + 1, 0);
+    std::vector<MfNear> list;
+    for (int leaf = 0; leaf < S; ++leaf) {
+        ptr[leaf] = (int32_t)list.size();
+        for (auto& e : perLeaf[leaf])
+            list.push_back(MfNear{(const void*)((const T*)P.nearStore.p + e.second * b * b), e.first, 0});
+    }
+    ptr[S] = (int32_t)list.size();
+    upload(c, P.nearPtr, ptr);
+    upload(c, P.near, list);
+    HMX_HIP(hipStreamSynchronize(c.stream));
+}
+
 void build_fp64_plan(Context& c, HMatrix& h) {
     check_shape(h);
     MfPlan& P = h.mf64;
+    std::vector<TileRef> all = all_tiles(h);
+    build_near<double>(c, h, P, all);
     std::vector<GroupTile> tiles;
-    for (const TileRef& R : all_tiles(h)) {
+    for (const TileRef& R : all) {
+        if (is_near(h, R)) continue;
         GroupTile gt{R, {}};
         gt.cols.resize(R.r);
         std::iota(gt.cols.begin(), gt.cols.end(), 0);
@@ -591,7 +656,7 @@ void build_fp64_plan(Context& c, HMatrix& h) {
     HMX_LAUNCHED();
     HMX_HIP(hipStreamSynchronize(c.stream));
     P.dtype = 0;
-    P.bytes = P.g[0].factorBytes + (uint64_t)tot * 8;
+    P.bytes = P.g[0].factorBytes + (uint64_t)tot * 8 + (uint64_t)P.numNear * h.b * h.b * 8;
     P.built = true;
 }
 
@@ -659,7 +724,7 @@ void run_fp64(Context& c, HMatrix& h, const double* X, double* Y, Events& ev) {
     auto k2 = h.b == 64 ? k_mf_phase2<64, true, double, double, double, NB, double>
                         : k_mf_phase2<32, true, double, double, double, NB, double>;
     k2<<<h.S, 256, 0, c.stream>>>(G.leafPtr.p, G.entries.p, (const double*)G.zt.p, G.numColsPad, nullptr, nullptr,
-                                  nullptr, 0, (const double*)P.Drm.p, X, ldx, Y, ldx);
+                                  nullptr, 0, (const double*)P.Drm.p, P.nearPtr.p, P.near.p, X, ldx, Y, ldx);
     HMX_LAUNCHED();
     ev.rec(c, 3);
 }
@@ -687,7 +752,7 @@ void run_mixed(Context& c, HMatrix& h, const float* X, float* Y, Events& ev) {
                         : k_mf_phase2<32, false, float, float, _Float16, NB, float>;
     k2<<<h.S, 256, 0, c.stream>>>(Ghi.leafPtr.p, Ghi.entries.p, (const float*)Ghi.zt.p, Ghi.numColsPad,
                                   Glo.leafPtr.p, Glo.entries.p, (const _Float16*)Glo.zt.p, Glo.numColsPad,
-                                  (const float*)P.Drm.p, X, n, Y, n);
+                                  (const float*)P.Drm.p, P.nearPtr.p, P.near.p, X, n, Y, n);
     HMX_LAUNCHED();
     ev.rec(c, 3);
 }
@@ -770,10 +835,16 @@ void hmatrix_make_mixed(Context& c, HMatrix& h, double target) {
     std::vector<int64_t> order(nc);
     std::iota(order.begin(), order.end(), 0);
     std::sort(order.begin(), order.end(), [&](int64_t a, int64_t b2) { return v2[a] < v2[b2]; });
+    // columns of tiles applied as dense blocks stay out of the selection
+    std::vector<char> dense(nc, 0);
+    for (size_t t = 0; t < tiles.size(); ++t)
+        if (is_near(h, tiles[t]))
+            for (int j = 0; j < tiles[t].r; ++j) dense[colBase[t] + j] = 1;
     std::vector<char> lo(nc, 0);
     double used = 0;
     for (int64_t q = 0; q < nc; ++q) {
         const int64_t j = order[q];
+        if (dense[j]) continue;
         if (used + v2[j] > budget) break;
         used += v2[j];
         lo[j] = 1;
@@ -783,10 +854,16 @@ void hmatrix_make_mixed(Context& c, HMatrix& h, double target) {
         su[j] = um[j] > 0 ? (float)um[j] : 1.f;
         sv[j] = vm[j] > 0 ? (float)vm[j] : 1.f;
     }
+    build_near<float>(c, h, P, tiles);
     std::vector<GroupTile> gh, gl;
     int64_t hi = 0, lc = 0;
     for (size_t t = 0; t < tiles.size(); ++t) {
         GroupTile a{tiles[t], {}}, b2{tiles[t], {}};
+        if (is_near(h, tiles[t])) {  // keep the group lists aligned with colBase (empty entries)
+            gh.push_back(std::move(a));
+            gl.push_back(std::move(b2));
+            continue;
+        }
         for (int j = 0; j < tiles[t].r; ++j) (lo[colBase[t] + j] ? b2.cols : a.cols).push_back(j);
         hi += (int64_t)a.cols.size();
         lc += (int64_t)b2.cols.size();
@@ -807,7 +884,8 @@ void hmatrix_make_mixed(Context& c, HMatrix& h, double target) {
     P.target = target;
     P.hiCols = hi;
     P.loCols = lc;
-    P.bytes = P.g[0].factorBytes + P.g[1].factorBytes + (uint64_t)tot * 4 + (uint64_t)lc * 4;
+    P.bytes = P.g[0].factorBytes + P.g[1].factorBytes + (uint64_t)tot * 4 + (uint64_t)lc * 4 +
+              (uint64_t)P.numNear * h.b * h.b * 4;
     P.built = true;
 }
 

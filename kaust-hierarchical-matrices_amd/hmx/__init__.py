@@ -102,6 +102,14 @@ _SIGS = [
     ("hmx_hmatvec_grad_factors", _i32, [_vp, _vp, _u32, _vp, _vp, _vp, _vp]),
     ("hmx_hmatrix_level_get", _i32, [_vp, _vp, _i32, _vp, _vp]),
     ("hmx_hmatrix_set_level", _i32, [_vp, _vp, _i32, _vp, _vp]),
+    ("hmx_hmatrix_plan_info", _i32, [_vp, _i32, _P(_u64), _P(_i64), _P(_i64)]),
+    ("hmx_hmatrix_save", _i32, [_vp, _vp, ctypes.c_char_p]),
+    ("hmx_hmatrix_load", _i32, [_vp, ctypes.c_char_p, _P(_vp)]),
+    ("hmx_batch_create", _i32, [_vp, _i32, _vp, _P(_vp)]),
+    ("hmx_batch_info", _i32, [_vp, _P(_i64), _P(_i64), _P(_i32)]),
+    ("hmx_batch_matvec", _i32, [_vp, _vp, _u32, _vp, _vp, _i32]),
+    ("hmx_batch_tune", _i32, [_vp, _vp, _u32, _P(_i32), _P(_f32)]),
+    ("hmx_batch_free", _i32, [_vp]),
 ]
 
 
@@ -326,10 +334,63 @@ class HMatrix(_Obj):
         _check(lib().hmx_hmatrix_diag_copy(ctx.handle, self.handle, _np(D)), ctx)
         return D
 
+    def save(self, ctx, path):
+        _check(lib().hmx_hmatrix_save(ctx.handle, self.handle, os.fsencode(path)), ctx)
+
+    @staticmethod
+    def load(ctx, path):
+        h = ctypes.c_void_p()
+        _check(lib().hmx_hmatrix_load(ctx.handle, os.fsencode(path), ctypes.byref(h)), ctx)
+        # magic, version, n, d, b, S, numLevels
+        with open(path, "rb") as f:
+            hdr = np.frombuffer(f.read(28), dtype=np.uint32)
+        return HMatrix(h, int(hdr[6]), int(hdr[2]), int(hdr[4]))
+
+    def plan_info(self, plan):
+        """plan 0: FP64 matrix-core plan, 1: mixed plan (bytes per call, dense blocks, low-rank columns)."""
+        by, nd, nc = _u64(), _i64(), _i64()
+        _check(lib().hmx_hmatrix_plan_info(self.handle, plan, ctypes.byref(by), ctypes.byref(nd), ctypes.byref(nc)))
+        return dict(bytes=by.value, denseBlocks=nd.value, lowRankColumns=nc.value)
+
     def bytes(self):
         lr, dn, fe = _u64(), _u64(), _u64()
         _check(lib().hmx_hmatrix_bytes(self.handle, ctypes.byref(lr), ctypes.byref(dn), ctypes.byref(fe)))
         return dict(lowRankBytes=lr.value, denseBytes=dn.value, factorEntries=fe.value)
+
+
+class Batch(_Obj):
+    """Many H-matrices applied as one block-diagonal operator (one launch per
+    phase, optional hipGraph replay).  Keeps the H-matrices alive."""
+    _free = "hmx_batch_free"
+
+    def __init__(self, ctx, hmatrices):
+        arr = (ctypes.c_void_p * len(hmatrices))(*[h.handle.value for h in hmatrices])
+        h = ctypes.c_void_p()
+        _check(lib().hmx_batch_create(ctx.handle, len(hmatrices), arr, ctypes.byref(h)), ctx)
+        super().__init__(h)
+        self.ctx = ctx
+        self._keep = list(hmatrices)
+        self.sizes = [hm.n for hm in hmatrices]
+        self.offsets = np.concatenate([[0], np.cumsum(self.sizes)]).astype(np.int64)
+
+    def info(self):
+        n, l, c = _i64(), _i64(), _i32()
+        _check(lib().hmx_batch_info(self.handle, ctypes.byref(n), ctypes.byref(l), ctypes.byref(c)))
+        return dict(totalPoints=n.value, totalLeaves=l.value, chunk=c.value)
+
+    def matvec(self, x, y=None, graph=False):
+        """x: (k, total) or (total,) float64 CUDA tensor."""
+        import torch
+        k = 1 if x.dim() == 1 else x.shape[0]
+        if y is None:
+            y = torch.empty_like(x)
+        _check(lib().hmx_batch_matvec(self.ctx.handle, self.handle, k, _ptr(x), _ptr(y), int(graph)), self.ctx)
+        return y
+
+    def tune(self, k):
+        c, ms = _i32(), _f32()
+        _check(lib().hmx_batch_tune(self.ctx.handle, self.handle, k, ctypes.byref(c), ctypes.byref(ms)), self.ctx)
+        return c.value, ms.value
 
 
 # ---- reference-named entry points --------------------------------------------

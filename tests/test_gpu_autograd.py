@@ -105,3 +105,60 @@ def test_factor_gradient_tile_outer_product(torch_cuda, setup):
         ref = Wn[v * m:(v + 1) * m] @ (Vs[t].T @ Xn[u * m:(u + 1) * m]).T
         got = dU[off[t] * m:(off[t] + r) * m].reshape(r, m).T
         assert np.allclose(got, ref, rtol=1e-10, atol=1e-10)
+
+
+# ---- full size: N = 2^18 points in [0,1]^2, l = 0.1, eps = 1e-6, 32 right-hand sides
+@pytest.fixture(scope="module")
+def big(torch_cuda):
+    from hmx.autograd import HMatrixOperator
+    n, d, b, tol = 1 << 18, 2, 32, 1e-6
+    print(f"[autograd] building n={n}", flush=True)
+    pipe = hmx.Pipeline(n, d, b, tol, max_rank_policy=hmx.MAXRANK_FULL, direct=True)
+    return pipe, HMatrixOperator(pipe)
+
+
+def test_full_size_input_gradient_vs_dense(torch_cuda, big):
+    torch = torch_cuda
+    pipe, op = big
+    n = pipe.n
+    g = torch.Generator(device="cuda").manual_seed(5)
+    X = torch.randn(32, n, generator=g, device="cuda", dtype=torch.float32, requires_grad=True)
+    W = torch.randn(32, n, generator=g, device="cuda", dtype=torch.float32)
+    (op(X) * W).sum().backward()
+    rows = torch.randperm(n, generator=torch.Generator().manual_seed(6))[:512].to(torch.int32).cuda()
+    ref = torch.empty(32, 512, dtype=torch.float64, device="cuda")
+    # K is symmetric: (K^T W)[rows] = K[rows, :] W
+    hmx.denseRowsXVector(pipe.ctx, n, 2, pipe.points, pipe.tree.device_perm(), pipe.ell, rows, 32,
+                         W.double().contiguous(), ref)
+    got = X.grad.double()[:, rows.long()]
+    rel = float(torch.linalg.norm(got - ref) / torch.linalg.norm(ref))
+    print(f"dX vs dense K^T W on 512 rows: {rel:.3e}")
+    assert rel < 1e-5
+
+
+def test_full_size_factor_gradients_fd(torch_cuda, big):
+    torch = torch_cuda
+    pipe, op = big
+    n = pipe.n
+    g = torch.Generator(device="cuda").manual_seed(7)
+    X = torch.randn(32, n, generator=g, device="cuda", dtype=torch.float64)
+    W = torch.randn(32, n, generator=g, device="cuda", dtype=torch.float64)
+    for prm in op.parameters():
+        prm.grad = None
+    (op(X) * W).sum().backward()
+    h = 1e-3
+    rng = torch.Generator(device="cuda").manual_seed(8)
+    for l in (op.levels[0], op.levels[len(op.levels) // 2], op.levels[-1]):
+        for P in (op.U[l], op.V[l]):
+            E = torch.randn(P.shape, generator=rng, device="cuda", dtype=torch.float64)
+            analytic = float((P.grad * E).sum())
+            with torch.no_grad():
+                P.add_(h * E)
+                lp = float((op(X) * W).sum())
+                P.sub_(2 * h * E)
+                lm = float((op(X) * W).sum())
+                P.add_(h * E)
+            fd = (lp - lm) / (2 * h)
+            print(f"level {l}: analytic {analytic:.6e} fd {fd:.6e}")
+            assert abs(fd - analytic) <= 1e-6 * max(1.0, abs(analytic)), (l, fd, analytic)
+    op.sync()

@@ -1,0 +1,145 @@
+"""Point clouds of uneven, non-power-of-two sizes and the batched H-matvec.
+
+Bars:
+  * partial leaves: the H-matvec of an H-matrix over n points that do not fill
+    the leaves equals the dense expansion of its own factors within 1e-13 and
+    the dense kernel product within the oracle's compression error;
+  * a batch of independent H-matrices (2D and 3D clouds, per-cloud
+    correlation length) applied in one launch per phase equals, bit for bit,
+    each instance's own H-matvec; hipGraph replays equal the eager launches
+    bit for bit, with eager work between the replays; after autotuning
+    (another chunk length, another summation order) within 1e-14;
+  * every instance against sampled rows of its dense FP64 kernel.
+"""
+import numpy as np
+import pytest
+
+import hmx
+import oracle as O
+from test_gpu_pipeline import expand_gpu, to_dev
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def torch_cuda():
+    import torch
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    return torch
+
+
+@pytest.fixture(scope="module")
+def ctx(torch_cuda):
+    return hmx.Context()
+
+
+@pytest.mark.parametrize("n,d,direct", [(3000, 2, False), (5001, 3, True), (2049, 2, True)])
+def test_partial_leaves(torch_cuda, ctx, n, d, direct):
+    b, tol = 32, 1e-6
+    p = O.generate_points(n, d, 77)
+    pipe = hmx.Pipeline(n, d, b, tol, adm="weak", max_rank_policy=hmx.MAXRANK_FULL, points=to_dev(torch_cuda, p),
+                        ctx=ctx, direct=direct)
+    X = O.generate_rhs(n, 4, 3)
+    x = to_dev(torch_cuda, X.T)
+    Y = pipe.matvec(x).cpu().numpy().T
+    E = expand_gpu(ctx, pipe)
+    assert np.linalg.norm(Y - E @ X) / np.linalg.norm(E @ X) < 1e-13
+    yt = pipe.matvec(x, transpose=True).cpu().numpy().T
+    assert np.linalg.norm(yt - E.T @ X) / np.linalg.norm(E.T @ X) < 1e-13
+    K = O.dense_matrix(p, n, d, O.kdtree(p, n, d, b)[0], pipe.ell)
+    err = np.linalg.norm(Y - K @ X) / np.linalg.norm(K @ X)
+    ref = O.OracleHMatrix(p, n, d, b, tol=tol, max_rank_policy=1, seed=1234)
+    err_o = np.linalg.norm(ref.matvec(X) - K @ X) / np.linalg.norm(K @ X)
+    print(f"n={n} d={d} direct={direct}: rel err {err:.3e} (oracle {err_o:.3e})")
+    assert err <= 2 * err_o + 1e-9
+    # 16 right-hand sides with partial leaves take the streaming path
+    X16 = O.generate_rhs(n, 16, 5)
+    Y16 = pipe.matvec(to_dev(torch_cuda, X16.T)).cpu().numpy().T
+    assert np.linalg.norm(Y16 - E @ X16) / np.linalg.norm(E @ X16) < 1e-13
+
+
+def cloud_specs(count, seed, nmin=2048, nmax=16384):
+    """Sizes log-uniform in [nmin, nmax], first half 2D, second half 3D,
+    correlation length uniform in [0.05, 0.3]."""
+    rng = np.random.default_rng(seed)
+    ns = np.floor(np.exp(rng.uniform(np.log(nmin), np.log(nmax), count))).astype(int)
+    ells = rng.uniform(0.05, 0.3, count)
+    return [(int(ns[i]), 2 if i < count // 2 else 3, float(ells[i])) for i in range(count)]
+
+
+@pytest.fixture(scope="module")
+def batch(torch_cuda, ctx):
+    specs = cloud_specs(8, 11)
+    pipes, pts = [], []
+    for i, (n, d, ell) in enumerate(specs):
+        p = O.generate_points(n, d, 100 + i)
+        pts.append(p)
+        pipes.append(hmx.Pipeline(n, d, 32, 1e-6, adm="weak", ell=ell, max_rank_policy=hmx.MAXRANK_FULL,
+                                  points=to_dev(torch_cuda, p), ctx=ctx, direct=True))
+    B = hmx.Batch(ctx, [pp.H for pp in pipes])
+    return specs, pipes, pts, B
+
+
+def test_batch_equals_instances(torch_cuda, ctx, batch):
+    specs, pipes, pts, B = batch
+    tot = B.info()["totalPoints"]
+    assert tot == sum(n for n, _, _ in specs)
+    k = 8
+    X = torch_cuda.randn(k, tot, dtype=torch_cuda.float64, device="cuda")
+    Y = B.matvec(X)
+    for i, pp in enumerate(pipes):
+        a, e = B.offsets[i], B.offsets[i + 1]
+        yi = pp.matvec(X[:, a:e].contiguous())
+        assert torch_cuda.equal(Y[:, a:e], yi), i
+
+
+def test_batch_graph_replay(torch_cuda, ctx, batch):
+    specs, pipes, pts, B = batch
+    tot = B.info()["totalPoints"]
+    k = 8
+    X = torch_cuda.randn(k, tot, dtype=torch_cuda.float64, device="cuda")
+    ref = B.matvec(X)
+    Y = torch_cuda.empty_like(X)
+    for rep in range(4):
+        Y.fill_(float("nan"))
+        B.matvec(X, Y, graph=True)
+        torch_cuda.cuda.synchronize()
+        assert torch_cuda.equal(Y, ref), rep
+        Z = B.matvec(X)  # eager work between replays
+        assert torch_cuda.equal(Z, ref)
+    # new inputs through the same captured pointers
+    X.copy_(torch_cuda.randn(k, tot, dtype=torch_cuda.float64, device="cuda"))
+    B.matvec(X, Y, graph=True)
+    assert torch_cuda.equal(Y, B.matvec(X))
+
+
+def test_batch_tune_and_dense_error(torch_cuda, ctx, batch):
+    specs, pipes, pts, B = batch
+    tot = B.info()["totalPoints"]
+    k = 8
+    X = torch_cuda.randn(k, tot, dtype=torch_cuda.float64, device="cuda")
+    before = B.matvec(X)
+    chunk, ms = B.tune(k)
+    assert chunk in (512, 1024, 2048, 4096) and ms > 0
+    assert B.info()["chunk"] == chunk
+    after = B.matvec(X)
+    assert float(torch_cuda.linalg.norm(after - before) / torch_cuda.linalg.norm(before)) < 1e-14
+    g = torch_cuda.Generator().manual_seed(0)
+    for i, ((n, d, ell), pp) in enumerate(zip(specs, pipes)):
+        a, e = B.offsets[i], B.offsets[i + 1]
+        rows = torch_cuda.randperm(n, generator=g)[:256].to(torch_cuda.int32).cuda()
+        xi = X[:, a:e].contiguous()
+        yd = torch_cuda.empty(k, 256, dtype=torch_cuda.float64, device="cuda")
+        hmx.denseRowsXVector(ctx, n, d, pp.points, pp.tree.device_perm(), ell, rows, k, xi, yd)
+        rel = float(torch_cuda.linalg.norm(after[:, a:e][:, rows.long()] - yd) / torch_cuda.linalg.norm(yd))
+        print(f"cloud {i}: n={n} d={d} l={ell:.3f} rel err {rel:.2e}")
+        assert rel < 1e-5
+
+
+def test_batch_rejects_mixed_bucket_sizes(torch_cuda, ctx):
+    a = hmx.Pipeline(2048, 2, 32, 1e-5, ctx=ctx, direct=True)
+    b = hmx.Pipeline(4096, 3, 64, 1e-5, ctx=ctx, direct=True)
+    with pytest.raises(hmx.HmxError) as ei:
+        hmx.Batch(ctx, [a.H, b.H])
+    assert ei.value.status == hmx.HMX_EINVAL

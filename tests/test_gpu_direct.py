@@ -120,6 +120,8 @@ def test_matvec_paths_vs_expansion(trio, ctx, torch_cuda, k):
         if k == 1:
             yt = pipe.matvec(x, transpose=True).cpu().numpy().T
             assert np.linalg.norm(yt - E.T @ X) / np.linalg.norm(E.T @ X) < 1e-13
+        elif tlr.d == 3:  # 3D leaf-level tiles have ranks above b/2: applied as dense blocks
+            assert pipe.H.plan_info(0)["denseBlocks"] > 0
 
 
 def test_b64_ranks_vs_oracle(torch_cuda, ctx):

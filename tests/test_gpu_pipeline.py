@@ -225,15 +225,6 @@ def test_dense_rows_vs_oracle(pair, ctx, torch_cuda):
     assert np.allclose(Y.cpu().numpy().T, Yo, rtol=1e-12, atol=1e-12)
 
 
-def test_partial_leaves_rejected_by_matvec(torch_cuda, ctx):
-    n, d, b = 3000, 2, 32
-    pipe = hmx.Pipeline(n, d, b, 1e-5, ctx=ctx)
-    x = torch_cuda.zeros(n, dtype=torch_cuda.float64, device="cuda")
-    with pytest.raises(hmx.HmxError) as ei:
-        pipe.matvec(x)
-    assert ei.value.status == hmx.HMX_EUNSUPPORTED
-
-
 def test_bad_config_rejected(torch_cuda, ctx):
     with pytest.raises(hmx.HmxError) as ei:
         hmx.Pipeline(1024, 2, 32, 1e-5, ctx=ctx, adm=5)
