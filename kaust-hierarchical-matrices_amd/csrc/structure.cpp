@@ -83,6 +83,8 @@ struct Walker {
     }
 };
 
+}  // namespace
+
 void build_jds(Structure::JDS& j, int level, const std::vector<int32_t>& tiles) {
     const uint32_t numRows = 1u << level;
     std::vector<int32_t> cnt(numRows, 0), start(numRows + 1, 0), fill(numRows, 0), rowTiles(tiles.size());
@@ -116,6 +118,8 @@ void build_jds(Structure::JDS& j, int level, const std::vector<int32_t>& tiles) 
     }
     j.iterPtr[maxc] = pos;
 }
+
+namespace {
 
 }  // namespace
 
