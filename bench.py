@@ -46,6 +46,8 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="budget of the CPU baseline (0 = skip)")
     ap.add_argument("--cpu-stride", type=int, default=32, help="CPU baseline samples every n-th tile")
     ap.add_argument("--mixed-target", type=float, default=1e-5, help="error budget of the mixed factors")
+    ap.add_argument("--counters-run", action="store_true",
+                    help="build, one 1 GiB device copy (HBM counter calibration), 3 FP64 1-RHS matvecs, exit")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for {args.gpus}); sharding across GPUs is not part of "
@@ -88,6 +90,20 @@ def main():
         torch.cuda.synchronize()
         return (time.perf_counter() - t) / steps
 
+    if args.counters_run:
+        # a plain copy of known size: FETCH_SIZE / WRITE_SIZE of its dispatch
+        # against 2^30 bytes gives the scale of the counters
+        src = torch.empty(1 << 27, dtype=torch.float64, device="cuda")
+        src.fill_(1.0)
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        for _ in range(3):
+            pipe.matvec(x1, y1)
+        torch.cuda.synchronize()
+        log("counters run done")
+        return
     sec1 = timed(lambda: pipe.matvec(x1, y1), args.steps, args.warmup)
     bytes1 = fac + 16 * n
     gbs1 = float(bytes1 / sec1 / 1e9)

@@ -151,6 +151,7 @@ __global__ void __launch_bounds__(256) k_dir_sample(LrLevel L, int splits) {
     __shared__ double sO[4][64][kB16];
     const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int m = L.m;
+    const double nil = -1.0 / L.ell;
     const int64_t gr = (int64_t)blockIdx.x * 256 + threadIdx.x;
     const bool inb = gr < (int64_t)L.T * m;  // uniform per wave (T*m is a multiple of 64)
     const int t = inb ? (int)(gr / m) : 0, q = (int)(gr % m);
@@ -181,7 +182,7 @@ __global__ void __launch_bounds__(256) k_dir_sample(LrLevel L, int splits) {
             for (int cc = 0; cc < 64; ++cc) {
                 const double4 xc = sP[w][cc];
                 const double xca[3] = {xc.x, xc.y, xc.z};
-                const double kv = kernel_eval<D>(xra, xca, L.ell) * xc.w;
+                const double kv = kernel_eval_scaled<D>(xra, xca, nil) * xc.w;
 #pragma unroll
                 for (int s = 0; s < kB16; ++s) acc[s] = fma(kv, sO[w][cc][s], acc[s]);
             }
@@ -212,6 +213,7 @@ __global__ void __launch_bounds__(256) k_dir_sampleT(LrLevel L, double* __restri
     __shared__ double sQ[4][64][NQ];
     const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int m = L.m;
+    const double nil = -1.0 / L.ell;
     const int64_t gc = (int64_t)blockIdx.x * 256 + threadIdx.x;
     const bool inb = gc < (int64_t)L.T * m;
     const int t = inb ? (int)(gc / m) : 0, c = (int)(gc % m);
@@ -239,7 +241,7 @@ __global__ void __launch_bounds__(256) k_dir_sampleT(LrLevel L, double* __restri
             for (int qq = 0; qq < 64; ++qq) {
                 const double4 xr = sP[w][qq];
                 const double xra[3] = {xr.x, xr.y, xr.z};
-                const double kv = kernel_eval<D>(xra, xca, L.ell) * xr.w;
+                const double kv = kernel_eval_scaled<D>(xra, xca, nil) * xr.w;
 #pragma unroll
                 for (int a = 0; a < NQ; ++a) acc[a] = fma(kv, sQ[w][qq][a], acc[a]);
             }

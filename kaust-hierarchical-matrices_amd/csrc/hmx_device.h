@@ -58,6 +58,19 @@ __device__ inline double kernel_eval(const double* __restrict__ a, const double*
     return exp(-sqrt(diff) / ell);
 }
 
+// Same kernel with the division folded into a multiply by nil = -1/ell: for
+// sampling loops whose values are not compared bit for bit.
+template <int D>
+__device__ inline double kernel_eval_scaled(const double* __restrict__ a, const double* __restrict__ b, double nil) {
+    double diff = 0.0;
+#pragma unroll
+    for (int j = 0; j < D; ++j) {
+        const double t = a[j] - b[j];
+        diff = fma(t, t, diff);
+    }
+    return exp(nil * sqrt(diff));
+}
+
 __device__ inline double kernel_eval_d(const double* a, const double* b, int d, double ell) {
     double diff = 0.0;
     for (int j = 0; j < d; ++j) {

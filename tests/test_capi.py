@@ -67,3 +67,19 @@ def test_structure_rejects_bad_arguments():
     assert ei.value.status == hmx.HMX_EINVAL
     with pytest.raises(hmx.HmxError):
         hmx.structure_from_boxes(100, 2, 24, "weak", 1.0, bmin, bmin)  # upper2(n)/b = 5, not a power of two
+
+
+def test_null_handles_rejected_without_a_device():
+    """Argument checks that run before any device call (CPU-only here)."""
+    import ctypes
+    L = hmx.lib()
+    out = ctypes.c_void_p()
+    assert L.hmx_batch_create(None, 1, None, ctypes.byref(out)) == hmx.HMX_EINVAL
+    assert L.hmx_batch_info(None, None, None, None) == hmx.HMX_EINVAL
+    assert L.hmx_batch_matvec(None, None, 1, None, None, 0) == hmx.HMX_EINVAL
+    assert L.hmx_batch_tune(None, None, 1, None, None) == hmx.HMX_EINVAL
+    assert L.hmx_batch_free(None) == hmx.HMX_OK
+    assert L.hmx_hmatrix_plan_info(None, 0, None, None, None) == hmx.HMX_EINVAL
+    assert L.hmx_hmatrix_save(None, None, b"x") == hmx.HMX_EINVAL
+    assert L.hmx_hmatrix_load(None, b"x", ctypes.byref(out)) == hmx.HMX_EINVAL
+    assert L.hmx_generateHMatrixDirect(None, None, None, None, None, ctypes.byref(out)) == hmx.HMX_EINVAL
