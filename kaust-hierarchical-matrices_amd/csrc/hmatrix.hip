@@ -194,6 +194,137 @@ __global__ void __launch_bounds__(256) k_dir_sample(LrLevel L, int splits) {
     for (int s = 0; s < kB16; ++s) Y[(size_t)s * m + q] = acc[s] * xr.w;
 }
 
+// The same two products on the FP64 matrix cores (v_mfma_f64_16x16x4f64).
+// A wave owns 64 rows (AΩ) or 64 columns (AᵀQ) of one tile as four blocks of
+// 16; per step of 4 along the inner index a lane evaluates one kernel entry
+// per block -- the A operand, lane l holding A[l%16][l/16] -- and reads one
+// entry of Ω or Q from LDS as the B operand (B[l/16][l%16]).  The 16x16
+// result of a block sits in register v of lane l at row 4v + l/16, column l%16.
+typedef double v4d_t __attribute__((ext_vector_type(4)));
+
+template <int D>
+__global__ void __launch_bounds__(256) k_dir_sample_mf(LrLevel L, int splits) {
+    __shared__ double4 sP[4][64];
+    __shared__ double sO[4][64][kB16];
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63, lg = lane >> 4, lc = lane & 15;
+    const int m = L.m;
+    const double nil = -1.0 / L.ell;
+    const int64_t r0g = ((int64_t)blockIdx.x * 4 + w) * 64;  // first row of this wave in the level's row space
+    const bool inb = r0g < (int64_t)L.T * m;                 // m is a multiple of 64
+    const int t = inb ? (int)(r0g / m) : 0, q0 = (int)(r0g % m);
+    const bool act = inb && lr_active(L.st[t], L.maxRank);
+    const uint32_t code = (uint32_t)L.tileMO[t];
+    const int64_t rowBase = (int64_t)compact16(code >> 1) * m, colBase = (int64_t)compact16(code) * m;
+    double xr[4][3], wr[4];
+#pragma unroll
+    for (int rb = 0; rb < 4; ++rb) {
+        const double4 p = act ? reinterpret_cast<const double4*>(L.P)[rowBase + q0 + 16 * rb + lc]
+                              : make_double4(0, 0, 0, 0);
+        xr[rb][0] = p.x; xr[rb][1] = p.y; xr[rb][2] = p.z; wr[rb] = p.w;
+    }
+    const int chunk = m / splits, c0 = blockIdx.y * chunk;
+    const double* Om = L.Om + (size_t)t * m * kB16;
+    v4d_t acc[4];
+#pragma unroll
+    for (int rb = 0; rb < 4; ++rb) acc[rb] = v4d_t{0.0, 0.0, 0.0, 0.0};
+    for (int cb = c0; cb < c0 + chunk; cb += 64) {
+        __syncthreads();
+        if (act) {
+            sP[w][lane] = reinterpret_cast<const double4*>(L.P)[colBase + cb + lane];
+            const double2* src = reinterpret_cast<const double2*>(Om + (size_t)(cb + lane) * kB16);
+#pragma unroll
+            for (int s2 = 0; s2 < kB16 / 2; ++s2) {
+                const double2 v = src[s2];
+                sO[w][lane][2 * s2] = v.x;
+                sO[w][lane][2 * s2 + 1] = v.y;
+            }
+        }
+        __syncthreads();
+        if (act) {
+            for (int c4 = 0; c4 < 64; c4 += 4) {
+                const double4 xc = sP[w][c4 + lg];
+                const double xca[3] = {xc.x, xc.y, xc.z};
+                const double om = sO[w][c4 + lg][lc];
+#pragma unroll
+                for (int rb = 0; rb < 4; ++rb) {
+                    const double a = kernel_eval_scaled<D>(xr[rb], xca, nil) * (xc.w * wr[rb]);
+                    acc[rb] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, om, acc[rb], 0, 0, 0);
+                }
+            }
+        }
+    }
+    if (!act) return;
+    double* Y = splits > 1 ? L.Ypart + ((size_t)blockIdx.y * L.T + t) * m * kB16 : L.Y + (size_t)t * m * kB16;
+#pragma unroll
+    for (int rb = 0; rb < 4; ++rb)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) Y[(size_t)lc * m + q0 + 16 * rb + 4 * v + lg] = acc[rb][v];
+}
+
+// B[t][a*m + c] = sum_q K(x_q, x_c) Q[t][a*m + q] for 32 basis columns a0.. per launch row
+template <int D>
+__global__ void __launch_bounds__(256) k_dir_sampleT_mf(LrLevel L, double* __restrict__ Bout) {
+    constexpr int NQ = 32;
+    __shared__ double4 sP[4][64];
+    __shared__ double sQ[4][64][NQ];
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63, lg = lane >> 4, lc = lane & 15;
+    const int m = L.m;
+    const double nil = -1.0 / L.ell;
+    const int64_t c0g = ((int64_t)blockIdx.x * 4 + w) * 64;
+    const bool inb = c0g < (int64_t)L.T * m;
+    const int t = inb ? (int)(c0g / m) : 0, cw = (int)(c0g % m);
+    const int k = inb ? L.st[t].k : 0;
+    const int a0 = blockIdx.y * NQ;
+    const bool act = inb && a0 < k;
+    const int na = act ? min(NQ, k - a0) : 0;
+    const uint32_t code = (uint32_t)L.tileMO[t];
+    const int64_t rowBase = (int64_t)compact16(code >> 1) * m, colBase = (int64_t)compact16(code) * m;
+    double xc[4][3], wc[4];
+#pragma unroll
+    for (int cb = 0; cb < 4; ++cb) {
+        const double4 p = act ? reinterpret_cast<const double4*>(L.P)[colBase + cw + 16 * cb + lc]
+                              : make_double4(0, 0, 0, 0);
+        xc[cb][0] = p.x; xc[cb][1] = p.y; xc[cb][2] = p.z; wc[cb] = p.w;
+    }
+    const double* Q = L.Q + (size_t)t * m * L.capQ;
+    v4d_t acc[4][2];
+#pragma unroll
+    for (int cb = 0; cb < 4; ++cb) { acc[cb][0] = v4d_t{0.0, 0.0, 0.0, 0.0}; acc[cb][1] = acc[cb][0]; }
+    for (int qb = 0; qb < m; qb += 64) {
+        __syncthreads();
+        if (act) {
+            sP[w][lane] = reinterpret_cast<const double4*>(L.P)[rowBase + qb + lane];
+#pragma unroll
+            for (int a = 0; a < NQ; ++a) sQ[w][lane][a] = a < na ? Q[(size_t)(a0 + a) * m + qb + lane] : 0.0;
+        }
+        __syncthreads();
+        if (act) {
+            for (int q4 = 0; q4 < 64; q4 += 4) {
+                const double4 xr = sP[w][q4 + lg];
+                const double xra[3] = {xr.x, xr.y, xr.z};
+                const double b0 = sQ[w][q4 + lg][lc], b1 = sQ[w][q4 + lg][16 + lc];
+#pragma unroll
+                for (int cb = 0; cb < 4; ++cb) {
+                    const double a = kernel_eval_scaled<D>(xra, xc[cb], nil) * (xr.w * wc[cb]);
+                    acc[cb][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b0, acc[cb][0], 0, 0, 0);
+                    acc[cb][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b1, acc[cb][1], 0, 0, 0);
+                }
+            }
+        }
+    }
+    if (!act) return;
+    double* B = Bout + (size_t)t * m * L.capQ;
+#pragma unroll
+    for (int cb = 0; cb < 4; ++cb)
+#pragma unroll
+        for (int ab = 0; ab < 2; ++ab)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int a = 16 * ab + lc;
+                if (a < na) B[(size_t)(a0 + a) * m + cw + 16 * cb + 4 * v + lg] = acc[cb][ab][v];
+            }
+}
+
 // Y = sum over the splits, in split order
 __global__ void __launch_bounds__(256) k_dir_sum(LrLevel L, int splits) {
     const int t = blockIdx.y;
@@ -557,7 +688,7 @@ static void launch_sampleT_tlr(Context& c, const LrLevel& L, int kmax, int bus, 
 template <int D>
 static void launch_sample_direct(Context& c, const LrLevel& L, int splits) {
     const int64_t rows = (int64_t)L.T * L.m;
-    k_dir_sample<D><<<dim3((unsigned)((rows + 255) / 256), (unsigned)splits), 256, 0, c.stream>>>(L, splits);
+    k_dir_sample_mf<D><<<dim3((unsigned)((rows + 255) / 256), (unsigned)splits), 256, 0, c.stream>>>(L, splits);
     HMX_LAUNCHED();
     if (splits > 1) {
         k_dir_sum<<<dim3((unsigned)(((int64_t)L.m * kB16 + 255) / 256), (unsigned)L.T), 256, 0, c.stream>>>(L, splits);
@@ -568,7 +699,7 @@ static void launch_sample_direct(Context& c, const LrLevel& L, int splits) {
 template <int D>
 static void launch_sampleT_direct(Context& c, const LrLevel& L, int kmax, double* B) {
     const int64_t cols = (int64_t)L.T * L.m;
-    k_dir_sampleT<D, 32><<<dim3((unsigned)((cols + 255) / 256), (unsigned)((kmax + 31) / 32)), 256, 0, c.stream>>>(L, B);
+    k_dir_sampleT_mf<D><<<dim3((unsigned)((cols + 255) / 256), (unsigned)((kmax + 31) / 32)), 256, 0, c.stream>>>(L, B);
     HMX_LAUNCHED();
 }
 
