@@ -262,6 +262,16 @@ hmx_status hmx_denseRowsXVector(hmx_context* ctx, uint32_t n, uint32_t d, const 
 
 /* ---- many point clouds (no counterpart in the reference, whose driver
    handles one point cloud per run, src/main.cu:46-173) ------------------- */
+/* Assemble `count` H-matrices directly (as hmx_generateHMatrixDirect) in one
+   go: per instance the diagonal blocks, leaf level and padded points, then the
+   upper levels of all instances together -- one batched ARA per (tile
+   dimension, point dimension) group, each tile with its own instance's
+   tolerance schedule, rank cap, random stream and correlation length.
+   cfgs, trees, structures, d_points and hmatrices are arrays of `count`;
+   araR and seed must agree across the batch. */
+hmx_status hmx_generateHMatrixDirectBatch(hmx_context* ctx, int32_t count, const hmx_config* cfgs,
+                                          hmx_kdtree* const* trees, hmx_structure* const* structures,
+                                          const double* const* d_points, hmx_hmatrix** hmatrices);
 /* Concatenate the matvec plans of `count` H-matrices (same bucketSize, any n):
    the batch applies Y = blockdiag(H_0, H_1, ...) X with one launch per phase.
    Instance i's entries sit at positions [sum_{j<i} n_j, sum_{j<=i} n_j) of every

@@ -1,6 +1,7 @@
 // C ABI (include/hmx.h): argument checks, exception -> status translation,
 // and the reference-named entry points.
 #include <cstring>
+#include <memory>
 #include <new>
 
 #include "hmx_internal.h"
@@ -568,6 +569,44 @@ hmx_status hmx_hmatrix_plan_info(const hmx_hmatrix* h, int32_t plan, uint64_t* b
     if (denseBlocks) *denseBlocks = P.numNear;
     if (lowRankColumns) *lowRankColumns = plan == 0 ? P.g[0].numColsPad : P.hiCols + P.loCols;
     return HMX_OK;
+}
+
+hmx_status hmx_generateHMatrixDirectBatch(hmx_context* ctx, int32_t count, const hmx_config* cfgs,
+                                          hmx_kdtree* const* trees, hmx_structure* const* structures,
+                                          const double* const* d_points, hmx_hmatrix** hmatrices) {
+    if (!ctx) return HMX_EINVAL;
+    return guard(ctx, [&] {
+        NEED(cfgs);
+        NEED(trees);
+        NEED(structures);
+        NEED(d_points);
+        NEED(hmatrices);
+        HMX_CHECK(count >= 1, HMX_EINVAL, "batch needs at least one instance");
+        for (int32_t i = 0; i < count; ++i) hmatrices[i] = nullptr;
+        std::vector<const hmx_config*> cf;
+        std::vector<const KDTree*> tr;
+        std::vector<const Structure*> st;
+        std::vector<const double*> pt;
+        for (int32_t i = 0; i < count; ++i) {
+            check_config(&cfgs[i]);
+            HMX_CHECK(trees[i] && structures[i] && d_points[i], HMX_EINVAL, "null instance argument");
+            HMX_CHECK(trees[i]->n == cfgs[i].numberOfInputPoints && trees[i]->d == cfgs[i].dimensionOfInputPoints &&
+                          trees[i]->b == cfgs[i].bucketSize,
+                      HMX_EINVAL, "configuration and KD-tree do not match");
+            cf.push_back(&cfgs[i]);
+            tr.push_back(trees[i]);
+            st.push_back(structures[i]);
+            pt.push_back(d_points[i]);
+        }
+        std::vector<std::unique_ptr<hmx_hmatrix>> own;
+        std::vector<HMatrix*> hs;
+        for (int32_t i = 0; i < count; ++i) {
+            own.emplace_back(new hmx_hmatrix());
+            hs.push_back(own.back().get());
+        }
+        build_hmatrix_direct_batch(*ctx, cf, tr, st, pt, hs);
+        for (int32_t i = 0; i < count; ++i) hmatrices[i] = own[i].release();
+    });
 }
 
 hmx_status hmx_batch_create(hmx_context* ctx, int32_t count, hmx_hmatrix* const* hmatrices, hmx_batch** batch) {

@@ -27,11 +27,21 @@ struct LrState {
     int k, small_run, converged, capped, na;
 };
 
+// Per-tile parameters of a batched ARA launch (tiles of one launch share m
+// but may come from different levels or different H-matrices).
+struct TileParam {
+    double tau;       // acceptance threshold
+    double nil;       // -1 / correlation length (direct mode)
+    int64_t pbase;    // first padded position of the tile's H-matrix in P (direct mode)
+    uint64_t opid;    // random stream id of Omega
+    int32_t maxRank;
+    int32_t code;     // Morton code of the tile within its level
+};
+
 struct LrLevel {
-    int T, m, b, bus, maxRank, r;
-    double tau;
+    int T, m, b, bus, r;
     uint64_t seed;
-    int level;
+    const TileParam* tp;
     const int64_t* first;   // first Morton sub-tile of each tile
     const int32_t* ranksMO;
     const int64_t* offMO;
@@ -42,8 +52,6 @@ struct LrLevel {
     int capQ;
     // direct mode: the operator is the kernel itself, evaluated on the fly
     const double* P;        // padded KD-ordered points, (x, y, z, w) per position; w = 1 real, 0 padding
-    const int32_t* tileMO;  // Morton code of each tile
-    double ell;
     double* Ypart;          // column-split partial samples [split][T][16][m]
 };
 
@@ -60,12 +68,11 @@ __global__ void k_lr_init(int T, LrState* st) {
 // Om[t][s*m + c] (transposed == 0) or Om[t][c*16 + s] (transposed == 1)
 __global__ void __launch_bounds__(256) k_lr_omega(LrLevel L, int it, int transposed) {
     int t = blockIdx.y;
-    if (!lr_active(L.st[t], L.maxRank)) return;
+    if (!lr_active(L.st[t], L.tp[t].maxRank)) return;
     int e = blockIdx.x * 256 + threadIdx.x;
     if (e >= L.m * kB16) return;
     int s = transposed ? e % kB16 : e / L.m, c = transposed ? e / kB16 : e % L.m;
-    uint64_t opid = ((uint64_t)(L.level + 1) << 40) | (uint64_t)t;
-    L.Om[(size_t)t * L.m * kB16 + e] = gauss_at(L.seed, kTagOmega, opid, (uint32_t)(it * kB16 + s), (uint32_t)c);
+    L.Om[(size_t)t * L.m * kB16 + e] = gauss_at(L.seed, kTagOmega, L.tp[t].opid, (uint32_t)(it * kB16 + s), (uint32_t)c);
 }
 
 // Y rows of row-leaf i of tile t: sum over column leaves j of U_ij (V_ij^T Om_j)
@@ -73,7 +80,7 @@ template <int BMAX>
 __global__ void __launch_bounds__(256) k_lr_sample(LrLevel L) {
     __shared__ double sV[BMAX * BMAX], sU[BMAX * BMAX], sO[BMAX * kB16], sT[BMAX * kB16];
     const int t = blockIdx.y, i = blockIdx.x, b = L.b, m = L.m;
-    if (!lr_active(L.st[t], L.maxRank)) return;
+    if (!lr_active(L.st[t], L.tp[t].maxRank)) return;
     const double* Om = L.Om + (size_t)t * m * kB16;
     double acc[(BMAX * kB16 + 255) / 256];
 #pragma unroll
@@ -151,13 +158,15 @@ __global__ void __launch_bounds__(256) k_dir_sample(LrLevel L, int splits) {
     __shared__ double sO[4][64][kB16];
     const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int m = L.m;
-    const double nil = -1.0 / L.ell;
     const int64_t gr = (int64_t)blockIdx.x * 256 + threadIdx.x;
     const bool inb = gr < (int64_t)L.T * m;  // uniform per wave (T*m is a multiple of 64)
     const int t = inb ? (int)(gr / m) : 0, q = (int)(gr % m);
-    const bool act = inb && lr_active(L.st[t], L.maxRank);
-    const uint32_t code = (uint32_t)L.tileMO[t];
-    const int64_t rowBase = (int64_t)compact16(code >> 1) * m, colBase = (int64_t)compact16(code) * m;
+    const bool act = inb && lr_active(L.st[t], L.tp[t].maxRank);
+    const TileParam tpt = L.tp[t];
+    const uint32_t code = (uint32_t)tpt.code;
+    const double nil = tpt.nil;
+    const int64_t rowBase = tpt.pbase + (int64_t)compact16(code >> 1) * m;
+    const int64_t colBase = tpt.pbase + (int64_t)compact16(code) * m;
     const double4 xr = act ? reinterpret_cast<const double4*>(L.P)[rowBase + q] : make_double4(0, 0, 0, 0);
     const double xra[3] = {xr.x, xr.y, xr.z};
     const int chunk = m / splits, c0 = blockIdx.y * chunk;
@@ -208,13 +217,15 @@ __global__ void __launch_bounds__(256) k_dir_sample_mf(LrLevel L, int splits) {
     __shared__ double sO[4][64][kB16];
     const int w = threadIdx.x >> 6, lane = threadIdx.x & 63, lg = lane >> 4, lc = lane & 15;
     const int m = L.m;
-    const double nil = -1.0 / L.ell;
     const int64_t r0g = ((int64_t)blockIdx.x * 4 + w) * 64;  // first row of this wave in the level's row space
     const bool inb = r0g < (int64_t)L.T * m;                 // m is a multiple of 64
     const int t = inb ? (int)(r0g / m) : 0, q0 = (int)(r0g % m);
-    const bool act = inb && lr_active(L.st[t], L.maxRank);
-    const uint32_t code = (uint32_t)L.tileMO[t];
-    const int64_t rowBase = (int64_t)compact16(code >> 1) * m, colBase = (int64_t)compact16(code) * m;
+    const bool act = inb && lr_active(L.st[t], L.tp[t].maxRank);
+    const TileParam tpt = L.tp[t];
+    const uint32_t code = (uint32_t)tpt.code;
+    const double nil = tpt.nil;
+    const int64_t rowBase = tpt.pbase + (int64_t)compact16(code >> 1) * m;
+    const int64_t colBase = tpt.pbase + (int64_t)compact16(code) * m;
     double xr[4][3], wr[4];
 #pragma unroll
     for (int rb = 0; rb < 4; ++rb) {
@@ -269,7 +280,6 @@ __global__ void __launch_bounds__(256) k_dir_sampleT_mf(LrLevel L, double* __res
     __shared__ double sQ[4][64][NQ];
     const int w = threadIdx.x >> 6, lane = threadIdx.x & 63, lg = lane >> 4, lc = lane & 15;
     const int m = L.m;
-    const double nil = -1.0 / L.ell;
     const int64_t c0g = ((int64_t)blockIdx.x * 4 + w) * 64;
     const bool inb = c0g < (int64_t)L.T * m;
     const int t = inb ? (int)(c0g / m) : 0, cw = (int)(c0g % m);
@@ -277,8 +287,11 @@ __global__ void __launch_bounds__(256) k_dir_sampleT_mf(LrLevel L, double* __res
     const int a0 = blockIdx.y * NQ;
     const bool act = inb && a0 < k;
     const int na = act ? min(NQ, k - a0) : 0;
-    const uint32_t code = (uint32_t)L.tileMO[t];
-    const int64_t rowBase = (int64_t)compact16(code >> 1) * m, colBase = (int64_t)compact16(code) * m;
+    const TileParam tpt = L.tp[t];
+    const uint32_t code = (uint32_t)tpt.code;
+    const double nil = tpt.nil;
+    const int64_t rowBase = tpt.pbase + (int64_t)compact16(code >> 1) * m;
+    const int64_t colBase = tpt.pbase + (int64_t)compact16(code) * m;
     double xc[4][3], wc[4];
 #pragma unroll
     for (int cb = 0; cb < 4; ++cb) {
@@ -328,7 +341,7 @@ __global__ void __launch_bounds__(256) k_dir_sampleT_mf(LrLevel L, double* __res
 // Y = sum over the splits, in split order
 __global__ void __launch_bounds__(256) k_dir_sum(LrLevel L, int splits) {
     const int t = blockIdx.y;
-    if (!lr_active(L.st[t], L.maxRank)) return;
+    if (!lr_active(L.st[t], L.tp[t].maxRank)) return;
     const int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x;
     const int64_t per = (int64_t)L.m * kB16;
     if (e >= per) return;
@@ -344,7 +357,6 @@ __global__ void __launch_bounds__(256) k_dir_sampleT(LrLevel L, double* __restri
     __shared__ double sQ[4][64][NQ];
     const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int m = L.m;
-    const double nil = -1.0 / L.ell;
     const int64_t gc = (int64_t)blockIdx.x * 256 + threadIdx.x;
     const bool inb = gc < (int64_t)L.T * m;
     const int t = inb ? (int)(gc / m) : 0, c = (int)(gc % m);
@@ -352,8 +364,11 @@ __global__ void __launch_bounds__(256) k_dir_sampleT(LrLevel L, double* __restri
     const int a0 = blockIdx.y * NQ;
     const bool act = inb && a0 < k;
     const int na = act ? min(NQ, k - a0) : 0;
-    const uint32_t code = (uint32_t)L.tileMO[t];
-    const int64_t rowBase = (int64_t)compact16(code >> 1) * m, colBase = (int64_t)compact16(code) * m;
+    const TileParam tpt = L.tp[t];
+    const uint32_t code = (uint32_t)tpt.code;
+    const double nil = tpt.nil;
+    const int64_t rowBase = tpt.pbase + (int64_t)compact16(code >> 1) * m;
+    const int64_t colBase = tpt.pbase + (int64_t)compact16(code) * m;
     const double4 xc = act ? reinterpret_cast<const double4*>(L.P)[colBase + c] : make_double4(0, 0, 0, 0);
     const double xca[3] = {xc.x, xc.y, xc.z};
     const double* Q = L.Q + (size_t)t * m * L.capQ;
@@ -393,7 +408,7 @@ __global__ void __launch_bounds__(256) k_lr_dot(LrLevel L, int mode) {
     const LrState& S = L.st[t];
     // before the decision step the tile must still be iterating; after it,
     // accepted columns are appended even when the tile converged in this block
-    if (mode == 0 ? !lr_active(S, L.maxRank) : S.na == 0) return;
+    if (mode == 0 ? !lr_active(S, L.tp[t].maxRank) : S.na == 0) return;
     const int K = S.k;
     const int C = mode == 0 ? kB16 : S.na;
     const int a0 = blockIdx.x * 16;
@@ -424,7 +439,7 @@ __global__ void __launch_bounds__(256) k_lr_update(LrLevel L, int mode) {
     __shared__ double sW[64 * kB16];
     const int t = blockIdx.y;
     const LrState& S = L.st[t];
-    if (mode == 0 ? !lr_active(S, L.maxRank) : S.na == 0) return;
+    if (mode == 0 ? !lr_active(S, L.tp[t].maxRank) : S.na == 0) return;
     const int k = S.k;
     const int C = mode == 0 ? kB16 : S.na;
     if (k == 0 || C == 0) return;
@@ -485,7 +500,7 @@ __global__ void __launch_bounds__(1024) k_lr_decide(LrLevel L) {
     __shared__ double red[16];
     const int t = blockIdx.x;
     LrState& S = L.st[t];
-    if (!lr_active(S, L.maxRank)) {
+    if (!lr_active(S, L.tp[t].maxRank)) {
         if (threadIdx.x == 0) S.na = 0;
         return;
     }
@@ -493,7 +508,8 @@ __global__ void __launch_bounds__(1024) k_lr_decide(LrLevel L) {
     double* Y = L.Y + (size_t)t * m * kB16;
     double* Qb = L.Ya + (size_t)t * m * kB16;
     int na = 0, small = S.small_run, conv = 0, capped = 0;
-    const int capleft = L.maxRank - S.k;
+    const int capleft = L.tp[t].maxRank - S.k;
+    const double tau = L.tp[t].tau;
     for (int j = 0; j < kB16; ++j) {
         double* y = Y + (size_t)j * m;
         for (int pass = 0; pass < 2; ++pass)
@@ -507,7 +523,7 @@ __global__ void __launch_bounds__(1024) k_lr_decide(LrLevel L) {
         double p = 0.0;
         for (int i = tid; i < m; i += nthr) p += y[i] * y[i];
         const double nu = sqrt(block_sum(p, red));
-        if (nu > L.tau) {
+        if (nu > tau) {
             if (na >= capleft) { capped = 1; break; }
             for (int i = tid; i < m; i += nthr) Qb[(size_t)na * m + i] = y[i] / nu;
             ++na;
@@ -558,7 +574,7 @@ __global__ void k_lr_commit(LrLevel L, int* flags) {
     LrState& S = L.st[t];
     S.k += S.na;
     S.na = 0;
-    if (!S.converged && S.k >= L.maxRank) S.capped = 1;
+    if (!S.converged && S.k >= L.tp[t].maxRank) S.capped = 1;
     if (!(S.converged || S.capped)) atomicAdd(&flags[0], 1);
     atomicMax(&flags[1], S.k);
 }
@@ -709,36 +725,42 @@ static bool log_on() {
     return on;
 }
 
-static void compress_level(Context& c, const hmx_config& cfg, const LevelSource& src, int b,
-                           const std::vector<int32_t>& tiles, int level, int numLevels, float tol, int maxRank,
-                           HLevel& out) {
+// Result of one batched ARA launch sequence: ranks, exclusive column
+// offsets and the packed factors of its tiles, in tile order.
+struct LrOut {
+    std::vector<int32_t> ranks;
+    std::vector<char> capped;
+    std::vector<int64_t> offsets;
+    DBuf<double> U, V;
+    int64_t rankSum = 0;
+    int caps = 0, kmax = 0, iters = 0;
+};
+
+// Batched ARA over T tiles of dimension m (bus = m/b leaves per side), each
+// with its own threshold, rank cap, random stream and (direct mode) point
+// base and correlation length.
+static void compress_batch(Context& c, int araR, uint64_t seed, const LevelSource& src, int b, int m,
+                           const std::vector<TileParam>& tp, const std::vector<int64_t>& first, LrOut& out) {
     const bool direct = src.mo == nullptr;
-    const auto tstart = std::chrono::steady_clock::now();
-    int iters = 0;
-    const int bus = 1 << (numLevels - (level + 1));
-    const int m = bus * b;
-    const int T = (int)tiles.size();
-    out.level = level;
-    out.numTiles = T;
-    out.m = m;
-    out.maxRank = maxRank;
-    out.tol = tol;
-    out.h_ranks.assign(T, 0);
-    out.h_offsets.assign(T + 1, 0);
+    const int bus = m / b;
+    const int T = (int)tp.size();
+    out.ranks.assign(T, 0);
+    out.offsets.assign(T + 1, 0);
     if (T == 0) return;
     HMX_CHECK(!direct || m % 64 == 0, HMX_EUNSUPPORTED, "direct compression needs tile dimensions that are multiples of 64");
-    std::vector<int64_t> first(T);
-    for (int t = 0; t < T; ++t) first[t] = (int64_t)tiles[t] * bus * bus;
-    DBuf<int64_t> d_first(T);
-    DBuf<int32_t> d_tiles(T);
-    HMX_HIP(hipMemcpyAsync(d_first.p, first.data(), T * sizeof(int64_t), hipMemcpyHostToDevice, c.stream));
-    HMX_HIP(hipMemcpyAsync(d_tiles.p, tiles.data(), T * sizeof(int32_t), hipMemcpyHostToDevice, c.stream));
+    int maxRank = 0;
+    for (const TileParam& q : tp) maxRank = std::max(maxRank, (int)q.maxRank);
+    DBuf<TileParam> d_tp(T);
+    HMX_HIP(hipMemcpyAsync(d_tp.p, tp.data(), T * sizeof(TileParam), hipMemcpyHostToDevice, c.stream));
+    DBuf<int64_t> d_first(direct ? 0 : T);
+    if (!direct)
+        HMX_HIP(hipMemcpyAsync(d_first.p, first.data(), T * sizeof(int64_t), hipMemcpyHostToDevice, c.stream));
     DBuf<LrState> st(T);
     int capQ = std::min(maxRank, 128);
     capQ = std::max(capQ, std::min(maxRank, kB16));
     DBuf<double> Q((size_t)T * m * capQ), W((size_t)T * capQ * kB16);
     DBuf<double> Y((size_t)T * m * kB16), Ya((size_t)T * m * kB16), Om((size_t)T * m * kB16);
-    // direct mode: split the columns of small levels over several workgroups
+    // direct mode: split the columns of small launches over several workgroups
     int splits = 1;
     if (direct) {
         const int64_t rows = (int64_t)T * m;
@@ -747,22 +769,22 @@ static void compress_level(Context& c, const hmx_config& cfg, const LevelSource&
     DBuf<double> Ypart(splits > 1 ? (size_t)splits * T * m * kB16 : 0);
     DBuf<int> flags(2);
     LrLevel L;
-    L.T = T; L.m = m; L.b = b; L.bus = bus; L.maxRank = maxRank; L.r = cfg.araR;
-    L.tau = ara_threshold((double)tol);
-    L.seed = cfg.seed; L.level = level;
+    L.T = T; L.m = m; L.b = b; L.bus = bus; L.r = araR;
+    L.seed = seed;
+    L.tp = d_tp.p;
     L.first = d_first.p;
     L.ranksMO = direct ? nullptr : src.mo->ranks.p;
     L.offMO = direct ? nullptr : src.mo->offsets.p;
     L.UMO = direct ? nullptr : src.mo->U.p;
     L.VMO = direct ? nullptr : src.mo->V.p;
     L.st = st.p; L.Q = Q.p; L.Y = Y.p; L.Ya = Ya.p; L.Om = Om.p; L.W = W.p; L.capQ = capQ;
-    L.P = src.P; L.tileMO = d_tiles.p; L.ell = src.ell; L.Ypart = Ypart.p;
+    L.P = src.P; L.Ypart = Ypart.p;
     k_lr_init<<<(T + 255) / 256, 256, 0, c.stream>>>(T, st.p);
     HMX_LAUNCHED();
     const unsigned rowBlocks = (unsigned)((m + 255) / 256);
     const unsigned wgThreads = m >= 4096 ? 1024u : 256u;
     int kmax = 0;
-    const int maxIters = (2 * maxRank + cfg.araR) / kB16 + 2;
+    const int maxIters = (2 * maxRank + araR) / kB16 + 2;
     for (int it = 0;; ++it) {
         HMX_CHECK(it <= maxIters, HMX_EDEVICE, "batched ARA did not converge (non-finite input?)");
         if (kmax + kB16 > capQ && capQ < maxRank) {  // grow the basis storage
@@ -812,7 +834,7 @@ static void compress_level(Context& c, const hmx_config& cfg, const LevelSource&
         HMX_HIP(hipMemcpyAsync(hf, flags.p, sizeof(hf), hipMemcpyDeviceToHost, c.stream));
         HMX_HIP(hipStreamSynchronize(c.stream));
         kmax = hf[1];
-        iters = it + 1;
+        out.iters = it + 1;
         if (hf[0] == 0) break;
     }
     Ypart.release();
@@ -837,20 +859,23 @@ static void compress_level(Context& c, const hmx_config& cfg, const LevelSource&
     HMX_HIP(hipStreamSynchronize(c.stream));
     int64_t acc = 0;
     int caps = 0;
+    out.capped.assign(T, 0);
     for (int t = 0; t < T; ++t) {
-        out.h_ranks[t] = hst[t].k;
-        out.h_offsets[t] = acc;
+        out.ranks[t] = hst[t].k;
+        out.offsets[t] = acc;
         acc += hst[t].k;
         caps += hst[t].capped;
+        out.capped[t] = (char)hst[t].capped;
     }
-    out.h_offsets[T] = acc;
+    out.offsets[T] = acc;
     out.rankSum = acc;
-    out.capHits = caps;
+    out.caps = caps;
+    out.kmax = kmax;
     out.U.alloc((size_t)acc * m);
     out.V.alloc((size_t)acc * m);
     if (acc > 0) {
         DBuf<int64_t> d_off(T + 1);
-        HMX_HIP(hipMemcpyAsync(d_off.p, out.h_offsets.data(), (T + 1) * sizeof(int64_t), hipMemcpyHostToDevice,
+        HMX_HIP(hipMemcpyAsync(d_off.p, out.offsets.data(), (T + 1) * sizeof(int64_t), hipMemcpyHostToDevice,
                                c.stream));
         dim3 g(std::min<unsigned>(1024, (unsigned)(((size_t)m * kmax + 255) / 256 + 1)), T);
         k_lr_pack<<<g, 256, 0, c.stream>>>(T, m, capQ, st.p, d_off.p, Q.p, out.U.p);
@@ -859,10 +884,53 @@ static void compress_level(Context& c, const hmx_config& cfg, const LevelSource&
         HMX_LAUNCHED();
         HMX_HIP(hipStreamSynchronize(c.stream));
     }
+}
+
+static TileParam tile_param(double tol, double ell, int64_t pbase, int level, int t, int maxRank, int32_t code) {
+    TileParam q;
+    q.tau = ara_threshold(tol);
+    q.nil = ell > 0 ? -1.0 / ell : 0.0;
+    q.pbase = pbase;
+    q.opid = ((uint64_t)(level + 1) << 40) | (uint64_t)t;
+    q.maxRank = maxRank;
+    q.code = code;
+    return q;
+}
+
+static void compress_level(Context& c, const hmx_config& cfg, const LevelSource& src, int b,
+                           const std::vector<int32_t>& tiles, int level, int numLevels, float tol, int maxRank,
+                           HLevel& out) {
+    const bool direct = src.mo == nullptr;
+    const auto tstart = std::chrono::steady_clock::now();
+    const int bus = 1 << (numLevels - (level + 1));
+    const int m = bus * b;
+    const int T = (int)tiles.size();
+    out.level = level;
+    out.numTiles = T;
+    out.m = m;
+    out.maxRank = maxRank;
+    out.tol = tol;
+    out.h_ranks.assign(T, 0);
+    out.h_offsets.assign(T + 1, 0);
+    if (T == 0) return;
+    std::vector<TileParam> tp(T);
+    std::vector<int64_t> first(T);
+    for (int t = 0; t < T; ++t) {
+        tp[t] = tile_param((double)tol, src.ell, 0, level, t, maxRank, tiles[t]);
+        first[t] = (int64_t)tiles[t] * bus * bus;
+    }
+    LrOut r;
+    compress_batch(c, cfg.araR, cfg.seed, src, b, m, tp, first, r);
+    out.h_ranks = std::move(r.ranks);
+    out.h_offsets = std::move(r.offsets);
+    out.rankSum = r.rankSum;
+    out.capHits = r.caps;
+    out.U = std::move(r.U);
+    out.V = std::move(r.V);
     if (log_on()) {
         const double ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - tstart).count();
         std::fprintf(stderr, "[hmx] level %d (%s): %d tiles m=%d rank sum %lld max %d, %d ARA blocks, %.1f ms\n", level,
-                     direct ? "direct" : "TLR sum", T, m, (long long)acc, kmax, iters, ms);
+                     direct ? "direct" : "TLR sum", T, m, (long long)out.rankSum, r.kmax, r.iters, ms);
         std::fflush(stderr);
     }
 }
@@ -1012,6 +1080,150 @@ void build_hmatrix_direct(Context& c, const hmx_config& cfg, const KDTree& t, co
     compress_upper_levels(c, cfg, s, src, b, h);
     HMX_HIP(hipStreamSynchronize(c.stream));
     finish_accounting(h);
+}
+
+// Many H-matrices assembled directly at once: per instance the diagonal
+// blocks, the leaf level and the padded points; then the upper levels of all
+// instances together, one batched ARA per (tile dimension, point dimension)
+// group, each tile with its own instance's threshold schedule, rank cap,
+// random stream and correlation length.  Per tile the same algorithm as
+// build_hmatrix_direct; only the column split of the sampler (and so the
+// summation order of the samples) can differ.
+void build_hmatrix_direct_batch(Context& c, const std::vector<const hmx_config*>& cfgs,
+                                const std::vector<const KDTree*>& trees, const std::vector<const Structure*>& ss,
+                                const std::vector<const double*>& pts, const std::vector<HMatrix*>& hs) {
+    const size_t N = hs.size();
+    HMX_CHECK(N >= 1 && cfgs.size() == N && trees.size() == N && ss.size() == N && pts.size() == N, HMX_EINVAL,
+              "batch arrays of different lengths");
+    const int b = (int)trees[0]->b;
+    std::vector<int64_t> pbase(N);
+    int64_t ptot = 0;
+    for (size_t i = 0; i < N; ++i) {
+        const KDTree& t = *trees[i];
+        const Structure& s = *ss[i];
+        const hmx_config& cfg = *cfgs[i];
+        HMX_CHECK((int)t.b == b, HMX_EINVAL, "all instances of a batch need the same bucketSize");
+        HMX_CHECK(cfg.araR == cfgs[0]->araR && cfg.seed == cfgs[0]->seed && cfg.araBlockSize == kB16, HMX_EINVAL,
+                  "all instances of a batch need the same araR, seed and ARA block size 16");
+        HMX_CHECK(s.numLevels >= 2 && (uint32_t)(1u << (s.numLevels - 1)) == t.numSegments, HMX_EINVAL,
+                  "structure and KD-tree do not match");
+        HMX_CHECK(b == 32 || b == 64, HMX_EUNSUPPORTED, "direct assembly: bucketSize must be 32 or 64");
+        HMX_CHECK(t.d >= 1 && t.d <= 3, HMX_EUNSUPPORTED, "direct assembly: dimension must be 1, 2 or 3");
+        HMX_CHECK(pts[i] != nullptr, HMX_EINVAL, "null points");
+        pbase[i] = ptot;
+        ptot += (int64_t)t.numSegments * b;
+    }
+    DBuf<double> P((size_t)ptot * 4);
+    for (size_t i = 0; i < N; ++i) {
+        const KDTree& t = *trees[i];
+        const Structure& s = *ss[i];
+        HMatrix& h = *hs[i];
+        const int L = s.numLevels;
+        h.n = t.n; h.d = t.d; h.b = t.b; h.S = t.numSegments; h.numLevels = L;
+        h.leafOffsets = t.h_offsets;
+        h.structureCopy = s;
+        h.structure = &h.structureCopy;
+        h.levels.clear();
+        h.levels.resize(L - 1);
+        h.diag.alloc((size_t)t.numSegments * b * b);
+        build_dense_diagonal(c, *cfgs[i], t, pts[i], h.diag.p);
+        HLevel& lv = h.levels[L - 2];
+        lv.level = L - 1;
+        lv.maxRank = cfgs[i]->maxRankPolicy == HMX_MAXRANK_REF ? b / 2 : b;
+        lv.tol = cfgs[i]->lowestLevelTolerance;
+        compress_leaf_tiles(c, *cfgs[i], t, pts[i], s.tileIndices[L - 2], lv);
+        const int64_t np = (int64_t)t.numSegments * b;
+        k_pad_points<<<(unsigned)((np + 255) / 256), 256, 0, c.stream>>>(t.n, t.d, b, t.numSegments, pts[i],
+                                                                          t.perm.p, t.offsets.p, P.p + pbase[i] * 4);
+        HMX_LAUNCHED();
+    }
+    // upper levels: the reference's schedule per instance, grouped across instances
+    struct Job {
+        int inst, level;
+    };
+    std::vector<std::pair<std::pair<int, int>, std::vector<Job>>> groups;  // ((m, d), jobs)
+    for (size_t i = 0; i < N; ++i) {
+        const Structure& s = *ss[i];
+        const hmx_config& cfg = *cfgs[i];
+        HMatrix& h = *hs[i];
+        const int L = s.numLevels;
+        std::vector<uint32_t> maxRanks(L > 2 ? L - 2 : 1, 0);
+        hmx_generateMaxRanks((uint32_t)L, (uint32_t)b, maxRanks.data());
+        float tol = cfg.lowestLevelTolerance;
+        for (int level = L - 2; level > 0; --level) {
+            HLevel& lv = h.levels[level - 1];
+            const int T = s.numTiles[level - 1];
+            const int m = (1 << (L - (level + 1))) * b;
+            lv.level = level;
+            lv.numTiles = T;
+            lv.m = m;
+            lv.h_ranks.assign(T, 0);
+            lv.h_offsets.assign(T + 1, 0);
+            if (T == 0) continue;
+            tol *= 2;
+            int maxRank = cfg.maxRankPolicy == HMX_MAXRANK_REF ? (int)maxRanks[L - level - 2] : m;
+            lv.maxRank = std::min(maxRank, m);
+            lv.tol = tol;
+            const std::pair<int, int> key(m, (int)h.d);
+            size_t g = 0;
+            while (g < groups.size() && groups[g].first != key) ++g;
+            if (g == groups.size()) groups.push_back({key, {}});
+            groups[g].second.push_back(Job{(int)i, level});
+        }
+    }
+    for (auto& grp : groups) {
+        const int m = grp.first.first, d = grp.first.second;
+        const auto tstart = std::chrono::steady_clock::now();
+        std::vector<TileParam> tp;
+        for (const Job& j : grp.second) {
+            const hmx_config& cfg = *cfgs[j.inst];
+            const HLevel& lv = hs[j.inst]->levels[j.level - 1];
+            const double ell = cfg.correlationLength > 0 ? cfg.correlationLength : default_ell(trees[j.inst]->d);
+            const auto& tiles = ss[j.inst]->tileIndices[j.level - 1];
+            for (int t = 0; t < (int)tiles.size(); ++t)
+                tp.push_back(tile_param((double)lv.tol, ell, pbase[j.inst], j.level, t, lv.maxRank, tiles[t]));
+        }
+        LevelSource src;
+        src.P = P.p;
+        src.d = d;
+        LrOut r;
+        compress_batch(c, cfgs[0]->araR, cfgs[0]->seed, src, b, m, tp, {}, r);
+        size_t t0 = 0;
+        for (const Job& j : grp.second) {
+            HLevel& lv = hs[j.inst]->levels[j.level - 1];
+            const int T = lv.numTiles;
+            int64_t acc = 0;
+            int caps = 0;
+            for (int t = 0; t < T; ++t) {
+                lv.h_ranks[t] = r.ranks[t0 + t];
+                lv.h_offsets[t] = acc;
+                acc += r.ranks[t0 + t];
+                caps += r.capped[t0 + t];
+            }
+            lv.h_offsets[T] = acc;
+            lv.rankSum = acc;
+            lv.capHits = caps;
+            lv.U.alloc((size_t)acc * m);
+            lv.V.alloc((size_t)acc * m);
+            if (acc > 0) {
+                const size_t o = (size_t)r.offsets[t0] * m;
+                HMX_HIP(hipMemcpyAsync(lv.U.p, r.U.p + o, (size_t)acc * m * sizeof(double), hipMemcpyDeviceToDevice,
+                                       c.stream));
+                HMX_HIP(hipMemcpyAsync(lv.V.p, r.V.p + o, (size_t)acc * m * sizeof(double), hipMemcpyDeviceToDevice,
+                                       c.stream));
+            }
+            t0 += T;
+        }
+        HMX_HIP(hipStreamSynchronize(c.stream));
+        if (log_on()) {
+            const double ms =
+                std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - tstart).count();
+            std::fprintf(stderr, "[hmx] batch group m=%d d=%d: %zu tiles from %zu levels, %d ARA blocks, %.1f ms\n", m,
+                         d, tp.size(), grp.second.size(), r.iters, ms);
+            std::fflush(stderr);
+        }
+    }
+    for (HMatrix* h : hs) finish_accounting(*h);
 }
 
 }  // namespace hmx

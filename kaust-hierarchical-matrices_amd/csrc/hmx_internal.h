@@ -278,6 +278,9 @@ void convert_to_morton(Context& c, const TLR& cm, TLR& mo);
 void build_hmatrix(Context& c, const hmx_config& cfg, const KDTree& t, const Structure& s, const TLR& mo, HMatrix& h);
 void build_hmatrix_direct(Context& c, const hmx_config& cfg, const KDTree& t, const Structure& s,
                           const double* d_points, HMatrix& h);
+void build_hmatrix_direct_batch(Context& c, const std::vector<const hmx_config*>& cfgs,
+                                const std::vector<const KDTree*>& trees, const std::vector<const Structure*>& ss,
+                                const std::vector<const double*>& pts, const std::vector<HMatrix*>& hs);
 void compress_leaf_tiles(Context& c, const hmx_config& cfg, const KDTree& t, const double* d_points,
                          const std::vector<int32_t>& tiles, HLevel& out);
 void build_dense_diagonal(Context& c, const hmx_config& cfg, const KDTree& t, const double* d_points, double* diag);

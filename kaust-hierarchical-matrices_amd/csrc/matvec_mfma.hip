@@ -543,6 +543,12 @@ void build_group(Context& c, HMatrix& h, MfGroup& G, int type, const std::vector
         uElems += (int64_t)m * rpad;
         fbytes += 2ull * (uint64_t)cnt * m * es;
     }
+    // items that read the same right-hand-side block (same tile dimension,
+    // column cluster and row chunk) next to each other: with XCD-local block
+    // numbering they run on one XCD and share that block in its L2
+    std::stable_sort(items.begin(), items.end(), [](const MfItem& a, const MfItem& b2) {
+        return a.ld != b2.ld ? a.ld > b2.ld : a.xoff < b2.xoff;
+    });
     G.numColsPad = zcol;
     G.numPart = slot;
     G.factorBytes = fbytes;

@@ -105,6 +105,7 @@ _SIGS = [
     ("hmx_hmatrix_plan_info", _i32, [_vp, _i32, _P(_u64), _P(_i64), _P(_i64)]),
     ("hmx_hmatrix_save", _i32, [_vp, _vp, ctypes.c_char_p]),
     ("hmx_hmatrix_load", _i32, [_vp, ctypes.c_char_p, _P(_vp)]),
+    ("hmx_generateHMatrixDirectBatch", _i32, [_vp, _i32, _vp, _vp, _vp, _vp, _vp]),
     ("hmx_batch_create", _i32, [_vp, _i32, _vp, _P(_vp)]),
     ("hmx_batch_info", _i32, [_vp, _P(_i64), _P(_i64), _P(_i32)]),
     ("hmx_batch_matvec", _i32, [_vp, _vp, _u32, _vp, _vp, _i32]),
@@ -458,6 +459,58 @@ def generateHMatrixDirect(ctx, cfg, tree, structure, points):
     _check(lib().hmx_generateHMatrixDirect(ctx.handle, ctypes.byref(cfg), tree.handle, structure.handle,
                                            _ptr(points), ctypes.byref(h)), ctx)
     return HMatrix(h, structure.num_levels(), cfg.numberOfInputPoints, cfg.bucketSize)
+
+
+def generateHMatrixDirectBatch(ctx, cfgs, trees, structures, points):
+    """Direct assembly of many H-matrices, upper levels compressed together."""
+    n = len(cfgs)
+    carr = (Config * n)(*cfgs)
+    tarr = (ctypes.c_void_p * n)(*[t.handle.value for t in trees])
+    sarr = (ctypes.c_void_p * n)(*[s.handle.value for s in structures])
+    parr = (ctypes.c_void_p * n)(*[p.data_ptr() for p in points])
+    harr = (ctypes.c_void_p * n)()
+    _check(lib().hmx_generateHMatrixDirectBatch(ctx.handle, n, carr, tarr, sarr, parr, harr), ctx)
+    return [HMatrix(ctypes.c_void_p(harr[i]), structures[i].num_levels(), cfgs[i].numberOfInputPoints,
+                    cfgs[i].bucketSize) for i in range(n)]
+
+
+class BatchPipeline:
+    """Many point clouds through the whole path: per cloud the dataset,
+    KD-tree and structure, then one batched direct assembly and one batched
+    matvec operator.  clouds: list of (n, d, ell)."""
+
+    def __init__(self, clouds, b=32, tol=1e-6, adm="weak", max_rank_policy=MAXRANK_FULL, seed=1234, ctx=None,
+                 points=None):
+        import time
+        import torch
+        self.ctx = ctx or Context()
+        self.clouds = list(clouds)
+        t = {}
+        t0 = time.perf_counter()
+        self.points, self.trees, self.structures, self.cfgs = [], [], [], []
+        for i, (n, d, ell) in enumerate(self.clouds):
+            if points is None:
+                p = torch.empty(n * d, dtype=torch.float64, device="cuda")
+                generateDataset(self.ctx, n, d, seed + i, p)
+            else:
+                p = points[i]
+            self.points.append(p)
+            self.trees.append(constructKDTree(self.ctx, n, d, b, p))
+            self.structures.append(constructHMatrixStructure(self.ctx, n, d, b, adm, 1.0, self.trees[-1]))
+            self.cfgs.append(default_config(numberOfInputPoints=n, dimensionOfInputPoints=d, bucketSize=b,
+                                            lowestLevelTolerance=tol, admissibilityCondition=ADMISSIBILITY.get(adm, adm),
+                                            correlationLength=ell, maxRankPolicy=max_rank_policy, seed=seed))
+        torch.cuda.synchronize()
+        t["TREES_AND_STRUCTURES"] = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        self.H = generateHMatrixDirectBatch(self.ctx, self.cfgs, self.trees, self.structures, self.points)
+        torch.cuda.synchronize()
+        t["HMATRIX_BATCH"] = time.perf_counter() - t0
+        self.batch = Batch(self.ctx, self.H)
+        self.timings = t
+
+    def matvec(self, x, y=None, graph=False):
+        return self.batch.matvec(x, y, graph)
 
 
 def HMatrixVecMult(ctx, H, k, x, y, transpose=False):

@@ -143,3 +143,48 @@ def test_batch_rejects_mixed_bucket_sizes(torch_cuda, ctx):
     with pytest.raises(hmx.HmxError) as ei:
         hmx.Batch(ctx, [a.H, b.H])
     assert ei.value.status == hmx.HMX_EINVAL
+
+
+def test_batched_build_matches_instances(torch_cuda, ctx):
+    """Upper levels of several clouds compressed together equal, tile for
+    tile, each cloud's own direct build: same diagonal and leaf level bit for
+    bit, same ranks up to threshold-borderline columns, products within the
+    level tolerance; and the batch's matvec against dense kernel rows."""
+    specs = cloud_specs(6, 21, nmin=2048, nmax=8192)
+    pts = [to_dev(torch_cuda, O.generate_points(n, d, 300 + i)) for i, (n, d, ell) in enumerate(specs)]
+    bp = hmx.BatchPipeline(specs, b=32, tol=1e-6, ctx=ctx, points=pts)
+    print("batched build timings", bp.timings)
+    diffs = []
+    for i, ((n, d, ell), H) in enumerate(zip(specs, bp.H)):
+        single = hmx.Pipeline(n, d, 32, 1e-6, ell=ell, max_rank_policy=hmx.MAXRANK_FULL, points=pts[i], ctx=ctx,
+                              direct=True)
+        S = single.tree.info()[0]
+        assert np.array_equal(H.diag(ctx, S), single.H.diag(ctx, S))
+        L = single.structure.num_levels()
+        for l in range(1, L):
+            rb, Ub, Vb = H.level(ctx, l)
+            rs, Us, Vs = single.H.level(ctx, l)
+            assert H.level_info(l)["tol"] == single.H.level_info(l)["tol"]
+            if l == L - 1:
+                assert np.array_equal(rb, rs)
+                continue
+            diffs.extend(np.abs(rb.astype(int) - rs.astype(int)).tolist())
+            tol = H.level_info(l)["tol"]
+            for t in range(0, len(rb), max(1, len(rb) // 4)):
+                if rb[t] == rs[t]:
+                    assert np.linalg.norm(Ub[t] @ Vb[t].T - Us[t] @ Vs[t].T, 2) <= 2 * tol
+    diffs = np.array(diffs)
+    print(f"batched vs single upper-level ranks: {len(diffs)} tiles, mismatches {(diffs != 0).sum()}")
+    assert diffs.max() <= 3 and (diffs != 0).mean() <= 0.05
+    tot = bp.batch.info()["totalPoints"]
+    X = torch_cuda.randn(8, tot, dtype=torch_cuda.float64, device="cuda")
+    Y = bp.matvec(X, graph=True)
+    g = torch_cuda.Generator().manual_seed(3)
+    for i, (n, d, ell) in enumerate(specs):
+        a, e = bp.batch.offsets[i], bp.batch.offsets[i + 1]
+        rows = torch_cuda.randperm(n, generator=g)[:256].to(torch_cuda.int32).cuda()
+        yd = torch_cuda.empty(8, 256, dtype=torch_cuda.float64, device="cuda")
+        hmx.denseRowsXVector(ctx, n, d, bp.points[i], bp.trees[i].device_perm(), ell, rows, 8,
+                             X[:, a:e].contiguous(), yd)
+        rel = float(torch_cuda.linalg.norm(Y[:, a:e][:, rows.long()] - yd) / torch_cuda.linalg.norm(yd))
+        assert rel < 1e-5, (i, rel)

@@ -152,3 +152,35 @@ def test_b64_mixed(trio, ctx, torch_cuda):
     rel = np.linalg.norm(y32 - y64) / np.linalg.norm(y64)
     print(f"mixed b={tlr.b}: hi {info['hiCols']} lo {info['loCols']} rel vs FP64 {rel:.3e}")
     assert rel < 2 * target
+
+
+def test_direct_ranks_vs_oracle_ara(trio, ctx):
+    """Upper-level tiles of the direct build against the oracle's ARA run on the
+    dense kernel tile with the same random stream (seed, operation id
+    ((level+1) << 40) | tile, column index): same ranks up to threshold-
+    borderline columns, products within the level tolerance."""
+    tlr, direct, p, K = trio
+    n, b = tlr.n, tlr.b
+    L = tlr.structure.num_levels()
+    _, off, _, _ = direct.tree.copy(ctx, n, direct.d)
+    rng = np.random.default_rng(5)
+    diffs = []
+    for l in range(1, L - 1):
+        info = direct.H.level_info(l)
+        if info["numTiles"] == 0:
+            continue
+        ranks, Us, Vs = direct.H.level(ctx, l)
+        bus = 1 << (L - 1 - l)
+        m = bus * b
+        tiles = direct.structure.tiles(l)
+        for t in rng.choice(len(tiles), min(6, len(tiles)), replace=False):
+            u, v = O._mo_to_xy(int(tiles[t]))
+            A = K[v * m:(v + 1) * m, u * m:(u + 1) * m]  # full leaves: padded space == KD order
+            Uo, Vo, k, _ = O.ara_dense(A, info["tol"], bs=16, r=10, max_rank=info["maxRank"], seed=1234,
+                                       opid=((l + 1) << 40) | int(t))
+            diffs.append(abs(int(ranks[t]) - k))
+            if ranks[t] == k:
+                assert np.linalg.norm(Us[t] @ Vs[t].T - Uo @ Vo.T, 2) <= 2 * info["tol"]
+    diffs = np.array(diffs)
+    print(f"direct vs oracle ARA: {len(diffs)} tiles, rank mismatches {(diffs != 0).sum()}, max {diffs.max()}")
+    assert diffs.max() <= 3 and (diffs != 0).mean() <= 0.1
