@@ -39,10 +39,14 @@ __device__ inline void vload_d(const double* __restrict__ p, double (&out)[N]) {
 }
 
 // ---- phase 1 ---------------------------------------------------------------
+// A wave walks its item in steps of 128 elements (2 per lane, 16-byte loads),
+// U steps at a time: the V and x loads of all U steps are issued before any
+// of their arithmetic, so every lane has U loads in flight.
 template <int K>
 __global__ void __launch_bounds__(256) k_mv_phase1(const MvItem* __restrict__ items, int numItems,
                                                    const double* __restrict__ X, int ldx,
                                                    double* __restrict__ zpart) {
+    constexpr int U = K <= 4 ? 4 : (K == 8 ? 2 : 1);  // keep the register file for occupancy at wide K
     const int lane = threadIdx.x & 63;
     const int wid = blockIdx.x * 4 + (threadIdx.x >> 6);
     if (wid >= numItems) return;
@@ -50,55 +54,73 @@ __global__ void __launch_bounds__(256) k_mv_phase1(const MvItem* __restrict__ it
     const int len = it.len;
     const int m = it.colLen ? it.colLen : len;  // column length inside this item (power of two)
     const double* __restrict__ xs = X + it.xoff;
-    double acc[K];
-#pragma unroll
-    for (int s = 0; s < K; ++s) acc[s] = 0.0;
     if (m >= 128) {
         // every 128-element step lies inside one column
-        int col = 0;
-        for (int e0 = 0; e0 < len; e0 += 128) {
-            const int e = e0 + 2 * lane;
-            const double2 vv = *reinterpret_cast<const double2*>(it.v + e);
-            const int xi = e & (m - 1);
+        double acc[K];
 #pragma unroll
-            for (int s = 0; s < K; ++s) {
-                const double2 xx = *reinterpret_cast<const double2*>(xs + (size_t)s * ldx + xi);
-                acc[s] = fma(vv.x, xx.x, acc[s]);
-                acc[s] = fma(vv.y, xx.y, acc[s]);
+        for (int s = 0; s < K; ++s) acc[s] = 0.0;
+        int col = 0;
+        for (int e0 = 0; e0 < len; e0 += 128 * U) {
+            double2 vv[U], xx[U][K];
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const int e = e0 + 128 * u + 2 * lane;
+                const bool ok = e0 + 128 * u < len;  // uniform: len is a multiple of 128
+                vv[u] = ok ? *reinterpret_cast<const double2*>(it.v + e) : make_double2(0.0, 0.0);
+                const int xi = e & (m - 1);
+#pragma unroll
+                for (int s = 0; s < K; ++s)
+                    xx[u][s] = ok ? *reinterpret_cast<const double2*>(xs + (size_t)s * ldx + xi) : make_double2(0.0, 0.0);
             }
-            if (((e0 + 128) & (m - 1)) == 0) {
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                if (e0 + 128 * u >= len) break;
 #pragma unroll
                 for (int s = 0; s < K; ++s) {
-                    double r = wave_sum(acc[s]);
-                    if (lane == 0) zpart[(size_t)(it.out + col) * K + s] = r;
-                    acc[s] = 0.0;
+                    acc[s] = fma(vv[u].x, xx[u][s].x, acc[s]);
+                    acc[s] = fma(vv[u].y, xx[u][s].y, acc[s]);
                 }
-                ++col;
+                if (((e0 + 128 * u + 128) & (m - 1)) == 0) {
+#pragma unroll
+                    for (int s = 0; s < K; ++s) {
+                        const double r = wave_sum(acc[s]);
+                        if (lane == 0) zpart[(size_t)(it.out + col) * K + s] = r;
+                        acc[s] = 0.0;
+                    }
+                    ++col;
+                }
             }
         }
     } else {
         // m in {32, 64}: one step covers 128/m columns; m/2 lanes per column
         const int g = m >> 1;
-        for (int e0 = 0; e0 < len; e0 += 128) {
-            const int e = e0 + 2 * lane;
-            const bool ok = e < len;
-            double2 vv = make_double2(0.0, 0.0);
-            if (ok) vv = *reinterpret_cast<const double2*>(it.v + e);
-            const int xi = e & (m - 1);
-            double p[K];
+        for (int e0 = 0; e0 < len; e0 += 128 * U) {
+            double2 vv[U], xx[U][K];
 #pragma unroll
-            for (int s = 0; s < K; ++s) {
-                double2 xx = make_double2(0.0, 0.0);
-                if (ok) xx = *reinterpret_cast<const double2*>(xs + (size_t)s * ldx + xi);
-                p[s] = fma(vv.y, xx.y, vv.x * xx.x);
+            for (int u = 0; u < U; ++u) {
+                const int e = e0 + 128 * u + 2 * lane;
+                const bool ok = e < len;
+                vv[u] = ok ? *reinterpret_cast<const double2*>(it.v + e) : make_double2(0.0, 0.0);
+                const int xi = e & (m - 1);
+#pragma unroll
+                for (int s = 0; s < K; ++s)
+                    xx[u][s] = ok ? *reinterpret_cast<const double2*>(xs + (size_t)s * ldx + xi) : make_double2(0.0, 0.0);
             }
-            for (int off = 1; off < g; off <<= 1)
 #pragma unroll
-                for (int s = 0; s < K; ++s) p[s] += __shfl_xor(p[s], off, 64);
-            if (ok && (lane & (g - 1)) == 0) {
-                const int col = e / m;
+            for (int u = 0; u < U; ++u) {
+                if (e0 + 128 * u >= len) break;
+                const int e = e0 + 128 * u + 2 * lane;
+                double p[K];
 #pragma unroll
-                for (int s = 0; s < K; ++s) zpart[(size_t)(it.out + col) * K + s] = p[s];
+                for (int s = 0; s < K; ++s) p[s] = fma(vv[u].y, xx[u][s].y, vv[u].x * xx[u][s].x);
+                for (int off = 1; off < g; off <<= 1)
+#pragma unroll
+                    for (int s = 0; s < K; ++s) p[s] += __shfl_xor(p[s], off, 64);
+                if (e < len && (lane & (g - 1)) == 0) {
+                    const int col = e / m;
+#pragma unroll
+                    for (int s = 0; s < K; ++s) zpart[(size_t)(it.out + col) * K + s] = p[s];
+                }
             }
         }
     }
