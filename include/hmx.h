@@ -34,7 +34,10 @@ typedef enum hmx_status {
     HMX_ESTATE = 6        /* object in the wrong state (e.g. TLR not in Morton order) */
 } hmx_status;
 
-/* DIVISION_METHOD (src/config.h:48-52); only FULL_TREE (the default) is built */
+/* DIVISION_METHOD (src/config.h:48-52).  FULL_TREE (the default) and
+   DIVIDE_IN_HALF run the same construction, as in the reference;
+   POWER_OF_TWO_ON_LEFT cuts at multiples of halving power-of-two sizes.  The
+   H-matrix structure needs 2^levels leaves. */
 enum { HMX_POWER_OF_TWO_ON_LEFT = 0, HMX_DIVIDE_IN_HALF = 1, HMX_FULL_TREE = 2 };
 /* ADMISSIBILITY_CONDITION (src/config.h:88-91) plus the max-diameter rule */
 enum { HMX_WEAK_ADMISSIBILITY = 0, HMX_BOX_CENTER_ADMISSIBILITY = 1, HMX_MAX_DIAM_ADMISSIBILITY = 2 };
@@ -48,7 +51,7 @@ enum { HMX_COLUMN_MAJOR = 0, HMX_MORTON = 1 };
    plus the parameters the reference hard-codes (src/main.cu:107,
    src/constructTLRMatrix.cu:73, src/HMatrixStructure.cu:7, src/TLRMatrixHelpers.cu:40-42). */
 typedef struct hmx_config {
-    int32_t divMethod;               /* HMX_FULL_TREE */
+    int32_t divMethod;               /* HMX_FULL_TREE (default), HMX_DIVIDE_IN_HALF, HMX_POWER_OF_TWO_ON_LEFT */
     int32_t admissibilityCondition;  /* HMX_WEAK_ADMISSIBILITY */
     uint32_t numberOfInputPoints;    /* 1024 */
     uint32_t dimensionOfInputPoints; /* 2 */

@@ -38,8 +38,9 @@ hmx_status guard(hmx_context* ctx, F&& f) {
 
 void check_config(const hmx_config* cfg) {
     NEED(cfg);
-    HMX_CHECK(cfg->divMethod == HMX_FULL_TREE, HMX_EUNSUPPORTED,
-              "only the FULL_TREE division method is provided");
+    HMX_CHECK(cfg->divMethod == HMX_FULL_TREE || cfg->divMethod == HMX_DIVIDE_IN_HALF ||
+                  cfg->divMethod == HMX_POWER_OF_TWO_ON_LEFT,
+              HMX_EINVAL, "divMethod must be POWER_OF_TWO_ON_LEFT, DIVIDE_IN_HALF or FULL_TREE");
     HMX_CHECK(cfg->lowestLevelTolerance > 0.f, HMX_EINVAL, "lowestLevelTolerance must be > 0");
     HMX_CHECK(cfg->maxRankPolicy == HMX_MAXRANK_REF || cfg->maxRankPolicy == HMX_MAXRANK_FULL, HMX_EINVAL,
               "maxRankPolicy must be 0 (reference) or 1 (full)");
@@ -118,11 +119,13 @@ hmx_status hmx_constructKDTree(hmx_context* ctx, uint32_t n, uint32_t d, uint32_
         NEED(tree);
         NEED(d_points);
         *tree = nullptr;
-        HMX_CHECK(divMethod == HMX_FULL_TREE, HMX_EUNSUPPORTED, "only the FULL_TREE division method is provided");
+        HMX_CHECK(divMethod == HMX_FULL_TREE || divMethod == HMX_DIVIDE_IN_HALF || divMethod == HMX_POWER_OF_TWO_ON_LEFT,
+                  HMX_EINVAL, "divMethod must be POWER_OF_TWO_ON_LEFT, DIVIDE_IN_HALF or FULL_TREE");
         hmx_kdtree* t = new hmx_kdtree();
         t->n = n;
         t->d = d;
         t->b = bucketSize;
+        t->divMethod = divMethod;
         try {
             build_kdtree(*ctx, *t, d_points);
         } catch (...) {
@@ -172,17 +175,15 @@ hmx_status hmx_constructHMatrixStructure(hmx_context* ctx, uint32_t n, uint32_t 
         *structure = nullptr;
         HMX_CHECK(tree->d == d && tree->b == bucketSize, HMX_EINVAL, "structure arguments do not match the tree");
         HMX_CHECK(upper_pow2(n) == upper_pow2(tree->n), HMX_EINVAL, "structure n does not match the tree");
-        hmx_structure* s = new hmx_structure();
-        try {
-            build_structure(*s, n, d, bucketSize, admissibilityCondition, eta, tree->h_bbmin.data(),
-                            tree->h_bbmax.data());
-        } catch (...) {
-            delete s;
-            throw;
-        }
+        HMX_CHECK(tree->numSegments == (1u << tree->numLevels), HMX_EUNSUPPORTED,
+                  "the H-matrix structure needs 2^levels leaves (a POWER_OF_TWO_ON_LEFT tree whose last "
+                  "power-of-two block is partly empty has fewer)");
+        std::unique_ptr<hmx_structure> s(new hmx_structure());
+        build_structure(*s, n, d, bucketSize, admissibilityCondition, eta, tree->h_bbmin.data(),
+                        tree->h_bbmax.data());
         HMX_CHECK((uint32_t)(1u << (s->numLevels - 1)) == tree->numSegments, HMX_EINVAL,
                   "structure depth does not match the tree");
-        *structure = s;
+        *structure = s.release();
     });
 }
 

@@ -73,6 +73,7 @@ struct Context {
 // per-level bounding boxes (src/boundingBoxes.h:8-23).
 struct KDTree {
     uint32_t n = 0, d = 0, b = 0;
+    int32_t divMethod = HMX_FULL_TREE;
     uint32_t numLevels = 0;    // number of split levels (getMaxSegmentSize().second)
     uint32_t numSegments = 0;  // 2^numLevels leaves
     DBuf<int32_t> perm;        // segmentIndices

@@ -230,9 +230,23 @@ void build_kdtree(Context& c, KDTree& t, const double* d_points) {
     HMX_CHECK(d >= 1 && d <= 3, HMX_EINVAL, "dimensionOfInputPoints must be 1, 2 or 3");
     HMX_CHECK(b >= 1, HMX_EINVAL, "bucketSize must be >= 1");
     HMX_CHECK(n >= 1, HMX_EINVAL, "numberOfInputPoints must be >= 1");
-    const uint32_t L = kd_num_levels(n, b);
+    // FULL_TREE and DIVIDE_IN_HALF run the same construction in the reference
+    // (the shared branches of src/kDTreeConstruction.cu); POWER_OF_TWO_ON_LEFT
+    // cuts every level at multiples of a power-of-two size that starts at
+    // upperPowerOfTwo(n) and halves while it exceeds b (fillOffsets,
+    // src/kDTreeHelpers.cu:29-43)
+    const bool p2l = t.divMethod == HMX_POWER_OF_TWO_ON_LEFT;
+    uint32_t L, size = 0;
+    if (p2l) {
+        size = upper_pow2(n);
+        L = 0;
+        while (size > b) { size >>= 1; ++L; }
+        t.numSegments = (n + size - 1) / size;
+    } else {
+        L = kd_num_levels(n, b);
+        t.numSegments = 1u << L;
+    }
     t.numLevels = L;
-    t.numSegments = 1u << L;
     const size_t nbox = ((size_t)2 << L) - 1;
     t.perm.alloc(n);
     t.offsets.alloc(t.numSegments + 1);
@@ -251,7 +265,14 @@ void build_kdtree(Context& c, KDTree& t, const double* d_points) {
     DBuf<int32_t> d_off(t.numSegments + 1);
 
     for (uint32_t level = 0; level < L; ++level) {
-        const uint32_t nseg = 1u << level;
+        const uint32_t lsize = p2l ? size << (L - level) : 0;
+        if (p2l) {  // segments [s*lsize, min((s+1)*lsize, n))
+            const uint32_t ns = (n + lsize - 1) / lsize;
+            off.assign(ns + 1, 0);
+            for (uint32_t q = 0; q < ns; ++q) off[q] = (int32_t)(q * lsize);
+            off[ns] = (int32_t)n;
+        }
+        const uint32_t nseg = (uint32_t)off.size() - 1;
         // ---- bounding boxes
         std::vector<int2> bchunks;
         std::vector<int32_t> segChunk(nseg + 1);
@@ -333,6 +354,10 @@ void build_kdtree(Context& c, KDTree& t, const double* d_points) {
             HMX_LAUNCHED();
             HMX_HIP(hipStreamSynchronize(c.stream));
         }
+        if (p2l) {  // the next level's cuts come from the halved size
+            HMX_HIP(hipStreamSynchronize(c.stream));
+            continue;
+        }
         // ---- split (fillOffsetsSort, src/kDTreeHelpers.cu:81-93)
         std::vector<int32_t> noff(2 * nseg + 1);
         for (uint32_t s = 0; s < nseg; ++s) {
@@ -342,6 +367,11 @@ void build_kdtree(Context& c, KDTree& t, const double* d_points) {
         noff[2 * nseg] = (int32_t)n;
         off.swap(noff);
         HMX_HIP(hipStreamSynchronize(c.stream));  // per-level host vectors go out of scope
+    }
+    if (p2l) {
+        off.assign(t.numSegments + 1, 0);
+        for (uint32_t q = 0; q < t.numSegments; ++q) off[q] = (int32_t)(q * size);
+        off[t.numSegments] = (int32_t)n;
     }
     HMX_HIP(hipMemcpyAsync(t.offsets.p, off.data(), off.size() * sizeof(int32_t), hipMemcpyHostToDevice, c.stream));
     t.h_offsets = off;

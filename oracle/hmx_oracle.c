@@ -213,6 +213,60 @@ OR_API int or_kdtree(int n, int d, int b, const double* p, int* perm, int* offse
     return levels;
 }
 
+/*
+ * POWER_OF_TWO_ON_LEFT division (src/kDTreeConstruction.cu:77-82, 90-95,
+ * 101-105, 154-156, 180-183, 206-208; fillOffsets src/kDTreeHelpers.cu:29-43):
+ * the segments of a level are [s*size, min((s+1)*size, n)) with size starting
+ * at upperPowerOfTwo(n) and halving while size > b; each segment is sorted by
+ * its widest coordinate as in FULL_TREE.  Returns the number of levels; the
+ * leaf count ceil(n/size) goes to *numSegments.
+ */
+OR_API int or_kdtree_pow2left(int n, int d, int b, const double* p, int* perm, int* offsets, double* bbmin,
+                              double* bbmax, int* split_dims, int* numSegments) {
+    kd_item* items = (kd_item*)malloc(sizeof(kd_item) * (size_t)(n > 0 ? n : 1));
+    for (int i = 0; i < n; ++i) perm[i] = i;
+    int size = or_upper_pow2(n);
+    int level = 0;
+    while (size > b) {
+        int nseg = (n + size - 1) / size;
+        int base = (1 << level) - 1;
+        for (int s = 0; s < nseg; ++s) {
+            int a = s * size, e = (s + 1) * size < n ? (s + 1) * size : n;
+            double* mn = bbmin + (size_t)(base + s) * d;
+            double* mx = bbmax + (size_t)(base + s) * d;
+            for (int j = 0; j < d; ++j) { mn[j] = DBL_MAX; mx[j] = -DBL_MAX; }
+            for (int i = a; i < e; ++i)
+                for (int j = 0; j < d; ++j) {
+                    double v = p[(size_t)j * n + perm[i]];
+                    if (v < mn[j]) mn[j] = v;
+                    if (v > mx[j]) mx[j] = v;
+                }
+            int dim = 0;
+            double best = mx[0] - mn[0];
+            for (int j = 1; j < d; ++j) {
+                double sp = mx[j] - mn[j];
+                if (sp > best) { best = sp; dim = j; }
+            }
+            if (split_dims) split_dims[base + s] = dim;
+            for (int i = a; i < e; ++i) {
+                items[i - a].key = dkey(p[(size_t)dim * n + perm[i]]);
+                items[i - a].pos = i - a;
+                items[i - a].val = perm[i];
+            }
+            qsort(items, (size_t)(e - a), sizeof(kd_item), kd_cmp);
+            for (int i = a; i < e; ++i) perm[i] = items[i - a].val;
+        }
+        size >>= 1;
+        ++level;
+    }
+    int nseg = (n + size - 1) / size;
+    for (int s = 0; s < nseg; ++s) offsets[s] = s * size;
+    offsets[nseg] = n;
+    *numSegments = nseg;
+    free(items);
+    return level;
+}
+
 /* ------------------------------------------------------------------------- */
 /* Admissibility (src/admissibilityCondition.cu:7-68)                        */
 /* ------------------------------------------------------------------------- */

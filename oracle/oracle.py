@@ -50,6 +50,7 @@ def lib():
         L.or_kd_num_levels.argtypes = [c_i, c_i]
         L.or_upper_pow2.argtypes = [c_i]
         L.or_kdtree.argtypes = [c_i, c_i, c_i, vp, vp, vp, vp, vp, vp]
+        L.or_kdtree_pow2left.argtypes = [c_i, c_i, c_i, vp, vp, vp, vp, vp, vp, vp]
         L.or_morton_from_xy.argtypes = [c_u32, c_u32]
         L.or_cm_to_mo.argtypes = [c_i, c_i]
         L.or_structure.argtypes = [c_i, c_i, c_i, c_i, c_f, vp, vp, vp, vp, vp]
@@ -146,6 +147,26 @@ def kdtree(points, n, d, b):
     p = np.ascontiguousarray(points, dtype=np.float64)
     lib().or_kdtree(n, d, b, _p(p), _p(perm), _p(off), _p(bmin), _p(bmax), _p(sd))
     return perm[:n], off, bmin.reshape(nb, d), bmax.reshape(nb, d), sd
+
+
+def kdtree_pow2left(points, n, d, b):
+    """POWER_OF_TWO_ON_LEFT tree: (perm, offsets, bbmin, bbmax, split_dims, levels)."""
+    p = np.ascontiguousarray(points, dtype=np.float64)
+    size, L = 1 << max(0, int(n - 1).bit_length()), 0
+    while size > b:
+        size >>= 1
+        L += 1
+    nbox = (2 << L) - 1
+    perm = np.zeros(n, np.int32)
+    nseg = (n + size - 1) // size
+    off = np.zeros(nseg + 1, np.int32)
+    bmin = np.zeros(nbox * d)
+    bmax = np.zeros(nbox * d)
+    sd = np.zeros(nbox, np.int32)
+    ns = ctypes.c_int(0)
+    lv = lib().or_kdtree_pow2left(n, d, b, _p(p), _p(perm), _p(off), _p(bmin), _p(bmax), _p(sd), ctypes.byref(ns))
+    assert ns.value == nseg and lv == L
+    return perm, off, bmin.reshape(nbox, d), bmax.reshape(nbox, d), sd, L
 
 
 def structure(n, d, b, adm, eta, bbmin, bbmax):

@@ -229,3 +229,42 @@ def test_bad_config_rejected(torch_cuda, ctx):
     with pytest.raises(hmx.HmxError) as ei:
         hmx.Pipeline(1024, 2, 32, 1e-5, ctx=ctx, adm=5)
     assert ei.value.status == hmx.HMX_EINVAL
+
+
+@pytest.mark.parametrize("n,d,b,quant", [(1000, 3, 32, 0), (5000, 2, 24, 16), (65536, 2, 32, 0), (100000, 3, 64, 0)])
+def test_kdtree_division_methods(torch_cuda, ctx, n, d, b, quant):
+    """POWER_OF_TWO_ON_LEFT bitwise against the oracle; DIVIDE_IN_HALF is the
+    FULL_TREE construction, as in the reference."""
+    p = O.generate_points(n, d, 21 + n)
+    if quant:
+        p = np.floor(p * quant) / quant
+    pts = to_dev(torch_cuda, p)
+    tree = hmx.constructKDTree(ctx, n, d, b, pts, hmx.POWER_OF_TWO_ON_LEFT)
+    perm, off, bmin, bmax = tree.copy(ctx, n, d)
+    rperm, roff, rmin, rmax, _, L = O.kdtree_pow2left(p, n, d, b)
+    assert np.array_equal(perm, rperm) and np.array_equal(off, roff)
+    size = 1 << L
+    top = 1 << max(0, (n - 1).bit_length())
+    for l in range(L):
+        lsize = top >> l
+        for s in range(-(-n // lsize)):
+            q = (1 << l) - 1 + s
+            assert np.array_equal(bmin[q], rmin[q]) and np.array_equal(bmax[q], rmax[q])
+    half = hmx.constructKDTree(ctx, n, d, b, pts, hmx.DIVIDE_IN_HALF)
+    full = hmx.constructKDTree(ctx, n, d, b, pts, hmx.FULL_TREE)
+    for a, c in zip(half.copy(ctx, n, d), full.copy(ctx, n, d)):
+        assert np.array_equal(a, c)
+
+
+def test_pow2left_tree_structure_guard(torch_cuda, ctx):
+    # 1000 points, leaves of 32: 32 = 2^5 leaves -> a structure can be built
+    p = O.generate_points(1000, 2, 5)
+    t = hmx.constructKDTree(ctx, 1000, 2, 32, to_dev(torch_cuda, p), hmx.POWER_OF_TWO_ON_LEFT)
+    s = hmx.constructHMatrixStructure(ctx, 1000, 2, 32, "weak", 1.0, t)
+    assert s.num_levels() == 6
+    # 900 points: 29 leaves, not 2^levels -> refused
+    p = O.generate_points(900, 2, 5)
+    t = hmx.constructKDTree(ctx, 900, 2, 32, to_dev(torch_cuda, p), hmx.POWER_OF_TWO_ON_LEFT)
+    with pytest.raises(hmx.HmxError) as ei:
+        hmx.constructHMatrixStructure(ctx, 900, 2, 32, "weak", 1.0, t)
+    assert ei.value.status == hmx.HMX_EUNSUPPORTED

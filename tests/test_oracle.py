@@ -106,6 +106,46 @@ def test_kdtree_invariants_large():
             assert P[dim, perm[a:h]].max() <= P[dim, perm[h:e]].min()
 
 
+def kd_pow2left_np(p, n, d, b):
+    """numpy restatement of the POWER_OF_TWO_ON_LEFT division."""
+    P = p.reshape(d, n)
+    perm = np.arange(n)
+    size = 1 << max(0, (n - 1).bit_length())
+    dims = []
+    while size > b:
+        lv = []
+        for s in range(-(-n // size)):
+            a, e = s * size, min((s + 1) * size, n)
+            seg = perm[a:e]
+            spans = P[:, seg].max(1) - P[:, seg].min(1)
+            dim = int(np.argmax(spans))  # first maximum
+            perm[a:e] = seg[np.argsort(P[dim, seg], kind="stable")]
+            lv.append(dim)
+        dims.append(lv)
+        size >>= 1
+    nseg = -(-n // size)
+    off = np.array([s * size for s in range(nseg)] + [n])
+    return perm, off, dims
+
+
+@pytest.mark.parametrize("n,d,b,quant", [(13, 2, 2, 0), (1000, 3, 32, 0), (1000, 2, 24, 16), (4096, 2, 32, 0),
+                                         (5000, 3, 64, 8)])
+def test_kdtree_pow2left_matches_numpy(n, d, b, quant):
+    p = O.generate_points(n, d, 7 + n)
+    if quant:
+        p = np.floor(p * quant) / quant
+    perm, off, bmin, bmax, sd, L = O.kdtree_pow2left(p, n, d, b)
+    rp, roff, dims = kd_pow2left_np(p, n, d, b)
+    assert np.array_equal(perm, rp) and np.array_equal(off, roff)
+    for l, lv in enumerate(dims):
+        for s, dim in enumerate(lv):
+            assert sd[(1 << l) - 1 + s] == dim
+    # a power-of-two n gives the FULL_TREE tree
+    if n & (n - 1) == 0:
+        fp, foff, _, _, _ = O.kdtree(p, n, d, b)
+        assert np.array_equal(perm, fp) and np.array_equal(off, foff)
+
+
 # --------------------------------------------------------------- Morton ----
 def morton_np(x, y):
     def spread(v):
