@@ -223,6 +223,12 @@ hmx_status hmx_hmatrix_make_mixed(hmx_context* ctx, hmx_hmatrix* h, double targe
    of the FP16 columns, bytes read per call */
 hmx_status hmx_hmatrix_mixed_info(const hmx_hmatrix* h, int64_t* hiCols, int64_t* loCols, double* frobH2,
                                   double* loEnergy, uint64_t* bytes);
+/* Matrix-core plan choices (no counterpart in the reference); each 0 or 1,
+   all 0 by default.  denseNear: leaf-level tiles of rank above bucketSize/2
+   applied as dense blocks.  directZ: phase 1 stores whole-column z itself
+   instead of partial sums for the reduce.  locality: phase-1 work ordered by
+   right-hand-side block.  Drops the built matrix-core plans. */
+hmx_status hmx_hmatrix_set_plan_options(hmx_hmatrix* h, int32_t denseNear, int32_t directZ, int32_t locality);
 /* Matrix-core plan summary (plan 0: FP64 multi-RHS, built on the first
    multi-RHS call; plan 1: mixed, built by hmx_hmatrix_make_mixed): bytes read
    per call (factors + dense blocks), leaf-level tiles applied as dense blocks

@@ -561,6 +561,17 @@ hmx_status hmx_hmatrix_load(hmx_context* ctx, const char* path, hmx_hmatrix** hm
     });
 }
 
+hmx_status hmx_hmatrix_set_plan_options(hmx_hmatrix* h, int32_t denseNear, int32_t directZ, int32_t locality) {
+    if (!h) return HMX_EINVAL;
+    if ((denseNear & ~1) || (directZ & ~1) || (locality & ~1)) return HMX_EINVAL;
+    h->planDenseNear = denseNear;
+    h->planDirectZ = directZ;
+    h->planLocality = locality;
+    h->mf64 = MfPlan();
+    h->mfMixed = MfPlan();
+    return HMX_OK;
+}
+
 hmx_status hmx_hmatrix_plan_info(const hmx_hmatrix* h, int32_t plan, uint64_t* bytes, int64_t* denseBlocks,
                                  int64_t* lowRankColumns) {
     if (!h || (plan != 0 && plan != 1)) return HMX_EINVAL;

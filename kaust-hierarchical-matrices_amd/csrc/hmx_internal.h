@@ -175,8 +175,8 @@ struct MfItem {
     int32_t xoff, rows, ld, ncols;
     int64_t out;   // partial slot of the item's first column (tiles split into several row chunks)
     int64_t zcol;  // padded column of the item's first column in the group's z
-    int32_t nch;   // row chunks per column of this tile; 1: the item writes z itself
-    int32_t pad;
+    int32_t nch;   // row chunks per column of this tile
+    int32_t direct;  // 1: whole columns, the item stores z itself; 0: partial sums for the reduce
 };
 // Phase-2 entry: the leaf's rows of one tile's U, either a row-major copy with
 // the tile's padded rank as row stride, or (FP64) the level's column-major U
@@ -247,6 +247,10 @@ struct HMatrix {
     std::vector<int64_t> levelPoolOff;  // element offset of each level in the pools
     MatvecPlan plan, planT;
     MfPlan mf64, mfMixed;
+    // matrix-core plan choices (hmx_hmatrix_set_plan_options)
+    int planDenseNear = 0;   // leaf-level tiles of rank > b/2 as dense blocks
+    int planDirectZ = 0;     // phase 1 stores whole-column z itself (else partials + reduce)
+    int planLocality = 0;    // phase-1 items ordered by right-hand-side block
     uint64_t lowRankBytes = 0, denseBytes = 0, factorEntries = 0;
 };
 

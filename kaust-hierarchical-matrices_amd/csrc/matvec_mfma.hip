@@ -109,7 +109,7 @@ __global__ void __launch_bounds__(256) k_mf_phase1(const MfItem* __restrict__ it
                 for (int q = 0; q < VL; ++q) bx[nb][q] = bn[nb][q];
         }
     }
-    if (it.nch == 1) {  // whole columns: scaled, converted, stored transposed ([rhs][column])
+    if (it.direct) {  // whole columns: scaled, converted, stored transposed ([rhs][column])
         using ACC = typename M::acc_t;
         ACC rs[NB];
 #pragma unroll
@@ -505,6 +505,7 @@ void build_group(Context& c, HMatrix& h, MfGroup& G, int type, const std::vector
         // phase-1 items
         const int nch = (m + kMfRows - 1) / kMfRows;
         const int rows = std::min(m, kMfRows);
+        const bool partial = nch > 1 || !h.planDirectZ;
         const char* vbase = vdirect ? (const char*)Vsrc : nullptr;  // patched after allocation otherwise
         for (int c0 = 0; c0 < cnt; c0 += kMfCols)
             for (int q = 0; q < nch; ++q) {
@@ -515,13 +516,13 @@ void build_group(Context& c, HMatrix& h, MfGroup& G, int type, const std::vector
                 it.rows = rows;
                 it.ld = m;
                 it.ncols = std::min(kMfCols, cnt - c0);
-                it.out = nch > 1 ? slot + (int64_t)c0 * nch + q : -1;
+                it.out = partial ? slot + (int64_t)c0 * nch + q : -1;
                 it.zcol = zcol + c0;
                 it.nch = nch;
-                it.pad = 0;
+                it.direct = partial ? 0 : 1;
                 items.push_back(it);
             }
-        if (nch > 1)
+        if (partial)
             for (int q = 0; q < cnt; ++q) {
                 redCol.push_back(zcol + q);
                 colPart.push_back(slot + (int64_t)q * nch);
@@ -537,7 +538,7 @@ void build_group(Context& c, HMatrix& h, MfGroup& G, int type, const std::vector
                 leafE[R.outCl * bus + q].push_back(MfEntry{
                     (const void*)(uintptr_t)(((size_t)uElems + (size_t)q * b * rpad) * es), rpad, (int32_t)zcol, cnt, 0});
         }
-        if (nch > 1) slot += (int64_t)cnt * nch;
+        if (partial) slot += (int64_t)cnt * nch;
         zcol += rpad;
         vElems += (int64_t)cnt * m;
         uElems += (int64_t)m * rpad;
@@ -546,9 +547,10 @@ void build_group(Context& c, HMatrix& h, MfGroup& G, int type, const std::vector
     // items that read the same right-hand-side block (same tile dimension,
     // column cluster and row chunk) next to each other: with XCD-local block
     // numbering they run on one XCD and share that block in its L2
-    std::stable_sort(items.begin(), items.end(), [](const MfItem& a, const MfItem& b2) {
-        return a.ld != b2.ld ? a.ld > b2.ld : a.xoff < b2.xoff;
-    });
+    if (h.planLocality)
+        std::stable_sort(items.begin(), items.end(), [](const MfItem& a, const MfItem& b2) {
+            return a.ld != b2.ld ? a.ld > b2.ld : a.xoff < b2.xoff;
+        });
     G.numColsPad = zcol;
     G.numPart = slot;
     G.factorBytes = fbytes;
@@ -605,7 +607,9 @@ void check_shape(const HMatrix& h) {
 }
 
 // leaf-level tiles of rank above b/2 are applied as dense blocks
-bool is_near(const HMatrix& h, const TileRef& R) { return R.m == (int)h.b && 2 * R.r > (int)h.b; }
+bool is_near(const HMatrix& h, const TileRef& R) {
+    return h.planDenseNear && R.m == (int)h.b && 2 * R.r > (int)h.b;
+}
 
 template <class T>
 void build_near(Context& c, HMatrix& h, MfPlan& P, const std::vector<TileRef>& tiles) {

@@ -103,6 +103,7 @@ _SIGS = [
     ("hmx_hmatrix_level_get", _i32, [_vp, _vp, _i32, _vp, _vp]),
     ("hmx_hmatrix_set_level", _i32, [_vp, _vp, _i32, _vp, _vp]),
     ("hmx_hmatrix_plan_info", _i32, [_vp, _i32, _P(_u64), _P(_i64), _P(_i64)]),
+    ("hmx_hmatrix_set_plan_options", _i32, [_vp, _i32, _i32, _i32]),
     ("hmx_hmatrix_save", _i32, [_vp, _vp, ctypes.c_char_p]),
     ("hmx_hmatrix_load", _i32, [_vp, ctypes.c_char_p, _P(_vp)]),
     ("hmx_generateHMatrixDirectBatch", _i32, [_vp, _i32, _vp, _vp, _vp, _vp, _vp]),
@@ -346,6 +347,9 @@ class HMatrix(_Obj):
         with open(path, "rb") as f:
             hdr = np.frombuffer(f.read(28), dtype=np.uint32)
         return HMatrix(h, int(hdr[6]), int(hdr[2]), int(hdr[4]))
+
+    def set_plan_options(self, dense_near=0, direct_z=0, locality=0):
+        _check(lib().hmx_hmatrix_set_plan_options(self.handle, int(dense_near), int(direct_z), int(locality)))
 
     def plan_info(self, plan):
         """plan 0: FP64 matrix-core plan, 1: mixed plan (bytes per call, dense blocks, low-rank columns)."""

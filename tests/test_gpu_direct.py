@@ -120,8 +120,15 @@ def test_matvec_paths_vs_expansion(trio, ctx, torch_cuda, k):
         if k == 1:
             yt = pipe.matvec(x, transpose=True).cpu().numpy().T
             assert np.linalg.norm(yt - E.T @ X) / np.linalg.norm(E.T @ X) < 1e-13
-        elif tlr.d == 3:  # 3D leaf-level tiles have ranks above b/2: applied as dense blocks
-            assert pipe.H.plan_info(0)["denseBlocks"] > 0
+            continue
+        # every matrix-core plan variant gives the same product
+        for opts in ((1, 0, 0), (0, 1, 0), (0, 0, 1), (1, 1, 1)):
+            pipe.H.set_plan_options(*opts)
+            Y = pipe.matvec(x).cpu().numpy().T
+            assert np.linalg.norm(Y - E @ X) / np.linalg.norm(E @ X) < 1e-13, opts
+            if opts[0] and tlr.d == 3:  # 3D leaf-level tiles have ranks above b/2
+                assert pipe.H.plan_info(0)["denseBlocks"] > 0
+        pipe.H.set_plan_options(0, 0, 0)
 
 
 def test_b64_ranks_vs_oracle(torch_cuda, ctx):
@@ -144,13 +151,20 @@ def test_b64_mixed(trio, ctx, torch_cuda):
     tlr, direct, p, K = trio
     n = tlr.n
     target = 1e-5
+    direct.H.set_plan_options(1, 1, 1)
     info = direct.make_mixed(target)
     X = O.generate_rhs(n, 64, 8).astype(np.float32)
     x32 = to_dev(torch_cuda, X.T)
     y32 = direct.matvec_mixed(x32).cpu().numpy().T.astype(np.float64)
     y64 = direct.matvec(x32.double()).cpu().numpy().T
     rel = np.linalg.norm(y32 - y64) / np.linalg.norm(y64)
-    print(f"mixed b={tlr.b}: hi {info['hiCols']} lo {info['loCols']} rel vs FP64 {rel:.3e}")
+    print(f"mixed b={tlr.b} (all plan options): hi {info['hiCols']} lo {info['loCols']} rel vs FP64 {rel:.3e}")
+    assert rel < 2 * target
+    direct.H.set_plan_options(0, 0, 0)
+    info = direct.make_mixed(target)
+    y32 = direct.matvec_mixed(x32).cpu().numpy().T.astype(np.float64)
+    rel = np.linalg.norm(y32 - y64) / np.linalg.norm(y64)
+    print(f"mixed b={tlr.b} (default plan): rel vs FP64 {rel:.3e}")
     assert rel < 2 * target
 
 
