@@ -58,7 +58,9 @@ __global__ void __launch_bounds__(256) k_mf_phase1(const MfItem* __restrict__ it
     using M = Mfma<T>;
     constexpr int KL = M::KL, VL = 4 * KL, NR = 16 * NB;
     const int lane = threadIdx.x & 63;
-    const int wid = xcd_local(blockIdx.x, gridDim.x) * 4 + (threadIdx.x >> 6);
+    // items differ in cost (1024-row chunks of upper levels, 64-row leaf
+    // tiles): keep the hardware's round-robin spread over the XCDs
+    const int wid = blockIdx.x * 4 + (threadIdx.x >> 6);
     if (wid >= nItems) return;
     const MfItem it = items[wid];
     const T* __restrict__ V = (const T*)it.v;
