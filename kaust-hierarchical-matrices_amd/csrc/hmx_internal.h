@@ -248,9 +248,9 @@ struct HMatrix {
     MatvecPlan plan, planT;
     MfPlan mf64, mfMixed;
     // matrix-core plan choices (hmx_hmatrix_set_plan_options)
-    int planDenseNear = 0;   // leaf-level tiles of rank > b/2 as dense blocks
-    int planDirectZ = 0;     // phase 1 stores whole-column z itself (else partials + reduce)
-    int planLocality = 0;    // phase-1 items ordered by right-hand-side block
+    int planDenseNear = 1;   // leaf-level tiles of rank > b/2 as dense blocks
+    int planDirectZ = 1;     // phase 1 stores whole-column z itself (else partials + reduce)
+    int planLocality = 1;    // phase-1 items ordered by right-hand-side block
     uint64_t lowRankBytes = 0, denseBytes = 0, factorEntries = 0;
 };
 

@@ -208,21 +208,38 @@ __global__ void __launch_bounds__(256) k_mf_phase2(const int32_t* __restrict__ p
 #pragma unroll
     for (int nb = 0; nb < NB; ++nb) acc[nb] = accv{0, 0, 0, 0};
     // dense blocks (row-major BL x BL): the diagonal block, then the leaf's
-    // dense near-field blocks; BL/16 super-steps of 16 columns each
-    const int n0 = nearPtr ? nearPtr[leaf] : 0, n1 = nearPtr ? nearPtr[leaf + 1] : 0;
-    for (int q = n0 - 1; q < n1; ++q) {
-        const TD* Dq = q < n0 ? Drm + (size_t)leaf * BL * BL : (const TD*)nearB[q].d;
-        const int cl = q < n0 ? leaf : nearB[q].colLeaf;
-        for (int c0 = 16 * kp; c0 < BL; c0 += 16 * KP) {
-            const int cc = c0 + 4 * lg;
-            TD a[4];
-            vload(Dq + row * BL + cc, a);
+    // dense near-field blocks.  Each wave takes BL/(16*KP) super-steps of 16
+    // columns per block; all super-steps of all blocks form one sequence whose
+    // next step's operands are loaded while the current step's MFMAs run.
+    {
+        constexpr int SPB = BL / (16 * KP);  // super-steps per block for this wave
+        const int n0 = nearPtr ? nearPtr[leaf] : 0, n1 = nearPtr ? nearPtr[leaf + 1] : 0;
+        const int total = (n1 - n0 + 1) * SPB;
+        TD a[4], bx[NB][4], an[4], bn[NB][4];
+        auto dload = [&](int idx, TD (&A)[4], TD (&B)[NB][4]) {
+            const int q = n0 - 1 + idx / SPB;
+            const int cc = 16 * kp + (idx % SPB) * 16 * KP + 4 * lg;
+            const TD* Dq = q < n0 ? Drm + (size_t)leaf * BL * BL : (const TD*)nearB[q].d;
+            const int cl = q < n0 ? leaf : nearB[q].colLeaf;
+            vload(Dq + row * BL + cc, A);
 #pragma unroll
-            for (int nb = 0; nb < NB; ++nb) {
-                TD bx[4];
-                vload(X + (size_t)(16 * nb + lc) * ldx + (size_t)cl * BL + cc, bx);
+            for (int nb = 0; nb < NB; ++nb) vload(X + (size_t)(16 * nb + lc) * ldx + (size_t)cl * BL + cc, B[nb]);
+        };
+        dload(0, a, bx);
+        for (int idx = 0; idx < total; ++idx) {
+            const bool more = idx + 1 < total;
+            if (more) dload(idx + 1, an, bn);
 #pragma unroll
-                for (int e = 0; e < 4; ++e) acc[nb] = MD::op(a[e], bx[e], acc[nb]);
+            for (int nb = 0; nb < NB; ++nb)
+#pragma unroll
+                for (int e = 0; e < 4; ++e) acc[nb] = MD::op(a[e], bx[nb][e], acc[nb]);
+            if (more) {
+#pragma unroll
+                for (int e = 0; e < 4; ++e) a[e] = an[e];
+#pragma unroll
+                for (int nb = 0; nb < NB; ++nb)
+#pragma unroll
+                    for (int e = 0; e < 4; ++e) bx[nb][e] = bn[nb][e];
             }
         }
     }
