@@ -178,74 +178,6 @@ __global__ void __launch_bounds__(256) k_mf_reduce(int64_t nred, int64_t ldz, co
 }
 
 // ---- phase 2 -----------------------------------------------------------------
-constexpr int kMaxStagedEntries = 128;
-
-// Low-rank part of phase 2 for one precision group: the leaf's entries are
-// staged in LDS (kMaxStagedEntries at a time), then every (entry, super-step)
-// pair of this wave forms one sequence whose next operands are loaded while
-// the current MFMAs run.  A = the leaf's 16 rows of U (row-major copy, or the
-// column-major U in place when UCM), B = the tile's z (transposed, ldz apart).
-template <class T, bool UCM, int NB, int KP, class ACCV>
-__device__ inline void mf_lowrank(const int32_t* __restrict__ ptr, const MfEntry* __restrict__ ent,
-                                  const T* __restrict__ Z, int64_t ldz, int leaf, int row, int kp, int lg, int lc,
-                                  ACCV (&acc)[NB], MfEntry* sE) {
-    using M = Mfma<T>;
-    constexpr int KL = M::KL, VL = 4 * KL, SS = 16 * KL;
-    const int e0 = ptr[leaf], e1 = ptr[leaf + 1];
-    for (int base = e0; base < e1; base += kMaxStagedEntries) {
-        const int ne = min(kMaxStagedEntries, e1 - base);
-        __syncthreads();
-        for (int i = threadIdx.x; i < ne; i += blockDim.x) sE[i] = ent[base + i];
-        __syncthreads();
-        auto norm = [&](int& ei, int& c0) {
-            while (ei < ne && c0 >= sE[ei].rpad) { ++ei; c0 = kp * SS; }
-        };
-        auto load = [&](int ei, int c0, T (&A)[VL], T (&B)[NB][VL]) {
-            const MfEntry en = sE[ei];
-            const int cc = c0 + VL * lg;
-            if (cc < en.rpad) {
-                if (UCM) {
-                    const T* __restrict__ U = (const T*)en.u + row;
-#pragma unroll
-                    for (int q = 0; q < VL; ++q) A[q] = cc + q < en.rank ? U[(size_t)(cc + q) * en.ld] : (T)0;
-                } else {
-                    vload((const T*)en.u + (size_t)row * en.rpad + cc, A);
-                }
-#pragma unroll
-                for (int nb = 0; nb < NB; ++nb) vload(Z + en.zcol + (size_t)(16 * nb + lc) * ldz + cc, B[nb]);
-            } else {
-                vzero(A);
-#pragma unroll
-                for (int nb = 0; nb < NB; ++nb) vzero(B[nb]);
-            }
-        };
-        int ei = 0, c0 = kp * SS;
-        norm(ei, c0);
-        T a[VL], bz[NB][VL], an[VL], bn[NB][VL];
-        if (ei < ne) load(ei, c0, a, bz);
-        while (ei < ne) {
-            int ni = ei, nc = c0 + KP * SS;
-            norm(ni, nc);
-            const bool more = ni < ne;
-            if (more) load(ni, nc, an, bn);
-#pragma unroll
-            for (int q = 0; q < 4; ++q)
-#pragma unroll
-                for (int nb = 0; nb < NB; ++nb) acc[nb] = M::op(M::make(&a[q * KL]), M::make(&bz[nb][q * KL]), acc[nb]);
-            if (more) {
-#pragma unroll
-                for (int q = 0; q < VL; ++q) a[q] = an[q];
-#pragma unroll
-                for (int nb = 0; nb < NB; ++nb)
-#pragma unroll
-                    for (int q = 0; q < VL; ++q) bz[nb][q] = bn[nb][q];
-            }
-            ei = ni;
-            c0 = nc;
-        }
-    }
-}
-
 // One workgroup of 4 waves per leaf of BL rows (BL = 32 or 64).  The leaf's
 // rows form RG = BL/16 row groups of 16; wave w owns row group w % RG and
 // every KP-th super-step of the K (column) loop, KP = 4/RG, with w / RG
@@ -311,9 +243,63 @@ __global__ void __launch_bounds__(256) k_mf_phase2(const int32_t* __restrict__ p
             }
         }
     }
-    __shared__ MfEntry sE[kMaxStagedEntries];
-    mf_lowrank<TA, UCM, NB, KP>(ptrA, entA, ZA, ldzA, leaf, row, kp, lg, lc, acc, sE);
-    if (ptrB) mf_lowrank<TB, false, NB, KP>(ptrB, entB, ZB, ldzB, leaf, row, kp, lg, lc, acc, sE);
+    {
+        constexpr int KL = MA::KL, VL = 4 * KL, SS = 16 * KL;
+        for (int e = ptrA[leaf]; e < ptrA[leaf + 1]; ++e) {
+            const MfEntry en = entA[e];
+            const TA* __restrict__ U = UCM ? (const TA*)en.u + row : (const TA*)en.u + (size_t)row * en.rpad;
+            const TA* __restrict__ Z = ZA + en.zcol;
+            for (int c0 = kp * SS; c0 < en.rpad; c0 += KP * SS) {
+                const int cc = c0 + VL * lg;
+                TA a[VL], bz[NB][VL];
+                if (cc < en.rpad) {
+                    if (UCM) {  // column-major U in place: one element per column, zero past the rank
+#pragma unroll
+                        for (int q = 0; q < VL; ++q) a[q] = cc + q < en.rank ? U[(size_t)(cc + q) * en.ld] : (TA)0;
+                    } else {
+                        vload(U + cc, a);
+                    }
+#pragma unroll
+                    for (int nb = 0; nb < NB; ++nb) vload(Z + (size_t)(16 * nb + lc) * ldzA + cc, bz[nb]);
+                } else {
+                    vzero(a);
+#pragma unroll
+                    for (int nb = 0; nb < NB; ++nb) vzero(bz[nb]);
+                }
+#pragma unroll
+                for (int q = 0; q < 4; ++q)
+#pragma unroll
+                    for (int nb = 0; nb < NB; ++nb)
+                        acc[nb] = MA::op(MA::make(&a[q * KL]), MA::make(&bz[nb][q * KL]), acc[nb]);
+            }
+        }
+    }
+    if (ptrB) {
+        constexpr int KL = MB::KL, VL = 4 * KL, SS = 16 * KL;
+        for (int e = ptrB[leaf]; e < ptrB[leaf + 1]; ++e) {
+            const MfEntry en = entB[e];
+            const TB* __restrict__ U = (const TB*)en.u + (size_t)row * en.rpad;
+            const TB* __restrict__ Z = ZB + en.zcol;
+            for (int c0 = kp * SS; c0 < en.rpad; c0 += KP * SS) {
+                const int cc = c0 + VL * lg;
+                TB a[VL], bz[NB][VL];
+                if (cc < en.rpad) {
+                    vload(U + cc, a);
+#pragma unroll
+                    for (int nb = 0; nb < NB; ++nb) vload(Z + (size_t)(16 * nb + lc) * ldzB + cc, bz[nb]);
+                } else {
+                    vzero(a);
+#pragma unroll
+                    for (int nb = 0; nb < NB; ++nb) vzero(bz[nb]);
+                }
+#pragma unroll
+                for (int q = 0; q < 4; ++q)
+#pragma unroll
+                    for (int nb = 0; nb < NB; ++nb)
+                        acc[nb] = MB::op(MB::make(&a[q * KL]), MB::make(&bz[nb][q * KL]), acc[nb]);
+            }
+        }
+    }
 #pragma unroll
     for (int nb = 0; nb < NB; ++nb)
 #pragma unroll
