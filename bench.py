@@ -220,12 +220,25 @@ def main():
             "mixed_fp32_cols": int(minfo["hiCols"]), "mixed_fp16_cols": int(minfo["loCols"]),
             "mixed_build_s": round(mixed_build_s, 3)},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": PEAK_HBM_GBS, "unit": "GB/s",
-                     "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": None,
+                     "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": traffic_bytes(args, dom),
                      "kernel": "k_mv_phase1" if dom == 1 else "k_mv_phase2", "kernel_bytes": int(dom_bytes),
                      "kernel_ms": dom_ms},
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
+
+
+def traffic_bytes(args, dom):
+    """HBM bytes per launch of the dominant kernel, from the committed
+    rocprofv3 FETCH_SIZE / WRITE_SIZE passes (scaled by a 1 GiB copy) of this
+    workload; None for other workloads or when the summary is absent."""
+    path = os.path.join(ROOT, "profiles", "hmatvec_fp64_n1048576_3d_hbm_traffic.json")
+    if not os.path.exists(path) or (args.n, args.d, args.b, args.tol, args.adm) != (1 << 20, 3, 64, 1e-5, "maxDiam"):
+        return None
+    with open(path) as f:
+        t = json.load(f)
+    key = "k_mv_phase1<1>" if dom == 1 else "k_mv_phase2<64, 1>"
+    return int(t[key]["traffic_bytes"]) if key in t else None
 
 
 def cpu_baseline(pipe, ctx, x1, budget_s, stride):
