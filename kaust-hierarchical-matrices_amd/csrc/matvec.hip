@@ -54,7 +54,30 @@ __global__ void __launch_bounds__(256) k_mv_phase1(const MvItem* __restrict__ it
     const int len = it.len;
     const int m = it.colLen ? it.colLen : len;  // column length inside this item (power of two)
     const double* __restrict__ xs = X + it.xoff;
-    if (m >= 128) {
+    if (m == 128) {
+        // one column per step, and a lane meets the same two x entries in every step
+        double2 xr[K];
+#pragma unroll
+        for (int s = 0; s < K; ++s) xr[s] = *reinterpret_cast<const double2*>(xs + (size_t)s * ldx + 2 * lane);
+        int col = 0;
+        for (int e0 = 0; e0 < len; e0 += 128 * U) {
+            double2 vv[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u)
+                vv[u] = e0 + 128 * u < len ? *reinterpret_cast<const double2*>(it.v + e0 + 128 * u + 2 * lane)
+                                           : make_double2(0.0, 0.0);
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                if (e0 + 128 * u >= len) break;
+#pragma unroll
+                for (int s = 0; s < K; ++s) {
+                    const double r = wave_sum(fma(vv[u].y, xr[s].y, vv[u].x * xr[s].x));
+                    if (lane == 0) zpart[(size_t)(it.out + col) * K + s] = r;
+                }
+                ++col;
+            }
+        }
+    } else if (m > 128) {
         // every 128-element step lies inside one column
         double acc[K];
 #pragma unroll
@@ -92,19 +115,19 @@ __global__ void __launch_bounds__(256) k_mv_phase1(const MvItem* __restrict__ it
             }
         }
     } else {
-        // m in {32, 64}: one step covers 128/m columns; m/2 lanes per column
+        // m in {32, 64}: one step covers 128/m columns; m/2 lanes per column.
+        // A lane meets the same two x entries in every step: load them once.
         const int g = m >> 1;
+        const int xi = (2 * lane) & (m - 1);
+        double2 xr[K];
+#pragma unroll
+        for (int s = 0; s < K; ++s) xr[s] = *reinterpret_cast<const double2*>(xs + (size_t)s * ldx + xi);
         for (int e0 = 0; e0 < len; e0 += 128 * U) {
-            double2 vv[U], xx[U][K];
+            double2 vv[U];
 #pragma unroll
             for (int u = 0; u < U; ++u) {
                 const int e = e0 + 128 * u + 2 * lane;
-                const bool ok = e < len;
-                vv[u] = ok ? *reinterpret_cast<const double2*>(it.v + e) : make_double2(0.0, 0.0);
-                const int xi = e & (m - 1);
-#pragma unroll
-                for (int s = 0; s < K; ++s)
-                    xx[u][s] = ok ? *reinterpret_cast<const double2*>(xs + (size_t)s * ldx + xi) : make_double2(0.0, 0.0);
+                vv[u] = e < len ? *reinterpret_cast<const double2*>(it.v + e) : make_double2(0.0, 0.0);
             }
 #pragma unroll
             for (int u = 0; u < U; ++u) {
@@ -112,7 +135,7 @@ __global__ void __launch_bounds__(256) k_mv_phase1(const MvItem* __restrict__ it
                 const int e = e0 + 128 * u + 2 * lane;
                 double p[K];
 #pragma unroll
-                for (int s = 0; s < K; ++s) p[s] = fma(vv[u].y, xx[u][s].y, vv[u].x * xx[u][s].x);
+                for (int s = 0; s < K; ++s) p[s] = fma(vv[u].y, xr[s].y, vv[u].x * xr[s].x);
                 for (int off = 1; off < g; off <<= 1)
 #pragma unroll
                     for (int s = 0; s < K; ++s) p[s] += __shfl_xor(p[s], off, 64);
