@@ -53,7 +53,8 @@ def main():
     ap.add_argument("--batch-nmax", type=int, default=16384,
                     help="largest cloud (sizes up to 65536 need ~280 GB of FP64 factors at eps 1e-6: more than one GPU)")
     ap.add_argument("--counters-run", action="store_true",
-                    help="build, one 1 GiB device copy (HBM counter calibration), 3 FP64 1-RHS matvecs, exit")
+                    help="build, one 1 GiB device copy (HBM counter calibration), 3 FP64 1-RHS matvecs, one FP64 and "
+                         "one mixed --rhs product, exit")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for {args.gpus}); sharding across GPUs is not part of "
@@ -111,6 +112,16 @@ def main():
         for _ in range(3):
             pipe.matvec(x1, y1)
         torch.cuda.synchronize()
+        if args.rhs:  # one FP64 and one mixed multi-RHS product as well
+            X = torch.empty(args.rhs, n, dtype=torch.float64, device="cuda")
+            hmx.generateRandomVector(ctx, args.rhs, n, 777, X)
+            Y = torch.empty_like(X)
+            pipe.matvec(X, Y)
+            pipe.make_mixed(args.mixed_target)
+            X32 = X.float()
+            Y32 = torch.empty_like(X32)
+            pipe.matvec_mixed(X32, Y32)
+            torch.cuda.synchronize()
         log("counters run done")
         return
     sec1 = timed(lambda: pipe.matvec(x1, y1), args.steps, args.warmup)
