@@ -28,11 +28,18 @@ __device__ inline double wave_sum(double v) {
     return v;
 }
 
+// factor data is read once per call: non-temporal loads keep x and z in L2
+typedef double f64x2_t __attribute__((ext_vector_type(2)));
+__device__ inline double2 ld_nt2(const double* __restrict__ p) {
+    const f64x2_t v = __builtin_nontemporal_load(reinterpret_cast<const f64x2_t*>(p));
+    return make_double2(v.x, v.y);
+}
+
 template <int N>
 __device__ inline void vload_d(const double* __restrict__ p, double (&out)[N]) {
 #pragma unroll
     for (int i = 0; i < N; i += 2) {
-        const double2 v = *reinterpret_cast<const double2*>(p + i);
+        const double2 v = ld_nt2(p + i);
         out[i] = v.x;
         out[i + 1] = v.y;
     }
@@ -64,7 +71,7 @@ __global__ void __launch_bounds__(256) k_mv_phase1(const MvItem* __restrict__ it
             double2 vv[U];
 #pragma unroll
             for (int u = 0; u < U; ++u)
-                vv[u] = e0 + 128 * u < len ? *reinterpret_cast<const double2*>(it.v + e0 + 128 * u + 2 * lane)
+                vv[u] = e0 + 128 * u < len ? ld_nt2(it.v + e0 + 128 * u + 2 * lane)
                                            : make_double2(0.0, 0.0);
 #pragma unroll
             for (int u = 0; u < U; ++u) {
@@ -89,7 +96,7 @@ __global__ void __launch_bounds__(256) k_mv_phase1(const MvItem* __restrict__ it
             for (int u = 0; u < U; ++u) {
                 const int e = e0 + 128 * u + 2 * lane;
                 const bool ok = e0 + 128 * u < len;  // uniform: len is a multiple of 128
-                vv[u] = ok ? *reinterpret_cast<const double2*>(it.v + e) : make_double2(0.0, 0.0);
+                vv[u] = ok ? ld_nt2(it.v + e) : make_double2(0.0, 0.0);
                 const int xi = e & (m - 1);
 #pragma unroll
                 for (int s = 0; s < K; ++s)
@@ -127,7 +134,7 @@ __global__ void __launch_bounds__(256) k_mv_phase1(const MvItem* __restrict__ it
 #pragma unroll
             for (int u = 0; u < U; ++u) {
                 const int e = e0 + 128 * u + 2 * lane;
-                vv[u] = e < len ? *reinterpret_cast<const double2*>(it.v + e) : make_double2(0.0, 0.0);
+                vv[u] = e < len ? ld_nt2(it.v + e) : make_double2(0.0, 0.0);
             }
 #pragma unroll
             for (int u = 0; u < U; ++u) {

@@ -41,6 +41,19 @@ __device__ inline void vload(const T* __restrict__ p, T (&out)[N]) {
     __builtin_memcpy(out, tmp, sizeof(tmp));
 }
 
+// the same for data read once (factors): non-temporal, so the streamed
+// factors do not push the reused right-hand-side blocks and z out of L2
+typedef unsigned int u32x4_t __attribute__((ext_vector_type(4)));
+template <class T, int N>
+__device__ inline void vload_nt(const T* __restrict__ p, T (&out)[N]) {
+    static_assert((N * sizeof(T)) % 16 == 0, "vector loads of whole 16-byte words");
+    const u32x4_t* q = reinterpret_cast<const u32x4_t*>(p);
+    u32x4_t tmp[N * sizeof(T) / 16];
+#pragma unroll
+    for (int i = 0; i < (int)(N * sizeof(T) / 16); ++i) tmp[i] = __builtin_nontemporal_load(q + i);
+    __builtin_memcpy(out, tmp, sizeof(tmp));
+}
+
 template <class T, int N>
 __device__ inline void vzero(T (&out)[N]) {
 #pragma unroll
@@ -80,7 +93,7 @@ __global__ void __launch_bounds__(256) k_mf_phase1(const MfItem* __restrict__ it
 #pragma unroll
         for (int g = 0; g < CG; ++g) {
             const int col = 16 * g + lc;
-            if (rok && col < it.ncols) vload(V + (size_t)col * it.ld + rr, A[g]);
+            if (rok && col < it.ncols) vload_nt(V + (size_t)col * it.ld + rr, A[g]);
             else vzero(A[g]);
         }
 #pragma unroll
@@ -221,7 +234,7 @@ __global__ void __launch_bounds__(256) k_mf_phase2(const int32_t* __restrict__ p
             const int cc = 16 * kp + (idx % SPB) * 16 * KP + 4 * lg;
             const TD* Dq = q < n0 ? Drm + (size_t)leaf * BL * BL : (const TD*)nearB[q].d;
             const int cl = q < n0 ? leaf : nearB[q].colLeaf;
-            vload(Dq + row * BL + cc, A);
+            vload_nt(Dq + row * BL + cc, A);
 #pragma unroll
             for (int nb = 0; nb < NB; ++nb) vload(X + (size_t)(16 * nb + lc) * ldx + (size_t)cl * BL + cc, B[nb]);
         };
@@ -255,9 +268,10 @@ __global__ void __launch_bounds__(256) k_mf_phase2(const int32_t* __restrict__ p
                 if (cc < en.rpad) {
                     if (UCM) {  // column-major U in place: one element per column, zero past the rank
 #pragma unroll
-                        for (int q = 0; q < VL; ++q) a[q] = cc + q < en.rank ? U[(size_t)(cc + q) * en.ld] : (TA)0;
+                        for (int q = 0; q < VL; ++q)
+                            a[q] = cc + q < en.rank ? __builtin_nontemporal_load(U + (size_t)(cc + q) * en.ld) : (TA)0;
                     } else {
-                        vload(U + cc, a);
+                        vload_nt(U + cc, a);
                     }
 #pragma unroll
                     for (int nb = 0; nb < NB; ++nb) vload(Z + (size_t)(16 * nb + lc) * ldzA + cc, bz[nb]);
@@ -284,7 +298,7 @@ __global__ void __launch_bounds__(256) k_mf_phase2(const int32_t* __restrict__ p
                 const int cc = c0 + VL * lg;
                 TB a[VL], bz[NB][VL];
                 if (cc < en.rpad) {
-                    vload(U + cc, a);
+                    vload_nt(U + cc, a);
 #pragma unroll
                     for (int nb = 0; nb < NB; ++nb) vload(Z + (size_t)(16 * nb + lc) * ldzB + cc, bz[nb]);
                 } else {
