@@ -60,6 +60,16 @@ __device__ inline void vzero(T (&out)[N]) {
     for (int i = 0; i < N; ++i) out[i] = (T)0;
 }
 
+// z of a precision group: column c (padded, tiles start at multiples of 16)
+// and right-hand side s at block c/VL of NR*VL elements, element s*VL + c%VL.
+// A lane's VL consecutive columns of one right-hand side (its B operand in
+// phase 2) are contiguous, and a 16-column super-step of a tile is one
+// contiguous NR*16-element block.
+template <int VL, int NR>
+__device__ inline size_t zt_index(int64_t c, int s) {
+    return (size_t)(c / VL) * (NR * VL) + (size_t)s * VL + (size_t)(c % VL);
+}
+
 // ---- phase 1 ---------------------------------------------------------------
 template <class T, int CG, int NB>
 __global__ void __launch_bounds__(256) k_mf_phase1(const MfItem* __restrict__ items, int nItems,
@@ -139,7 +149,7 @@ __global__ void __launch_bounds__(256) k_mf_phase1(const MfItem* __restrict__ it
                     const ACC cs = colScale ? (ACC)colScale[zc] : (ACC)1;
 #pragma unroll
                     for (int nb = 0; nb < NB; ++nb)
-                        Zt[(size_t)(16 * nb + lc) * ldz + zc] = (T)(acc[g][nb][v] * cs * rs[nb]);
+                        Zt[zt_index<VL, NR>(zc, 16 * nb + lc)] = (T)(acc[g][nb][v] * cs * rs[nb]);
                 }
             }
         return;
@@ -183,10 +193,11 @@ __global__ void __launch_bounds__(256) k_mf_reduce(int64_t nred, int64_t ldz, co
         tile[c][s] = v;
     }
     __syncthreads();
+    constexpr int VL = 4 * Mfma<T>::KL;
     for (int o = threadIdx.x; o < 64 * NR; o += 256) {
         const int s = o / 64, c = o % 64;
         const int64_t g = g0 + c;
-        if (g < nred) Zt[(size_t)s * ldz + redCol[g]] = (T)tile[c][s];
+        if (g < nred) Zt[zt_index<VL, NR>(redCol[g], s)] = (T)tile[c][s];
     }
 }
 
@@ -261,7 +272,7 @@ __global__ void __launch_bounds__(256) k_mf_phase2(const int32_t* __restrict__ p
         for (int e = ptrA[leaf]; e < ptrA[leaf + 1]; ++e) {
             const MfEntry en = entA[e];
             const TA* __restrict__ U = UCM ? (const TA*)en.u + row : (const TA*)en.u + (size_t)row * en.rpad;
-            const TA* __restrict__ Z = ZA + en.zcol;
+            const TA* __restrict__ Z = ZA;
             for (int c0 = kp * SS; c0 < en.rpad; c0 += KP * SS) {
                 const int cc = c0 + VL * lg;
                 TA a[VL], bz[NB][VL];
@@ -274,7 +285,7 @@ __global__ void __launch_bounds__(256) k_mf_phase2(const int32_t* __restrict__ p
                         vload_nt(U + cc, a);
                     }
 #pragma unroll
-                    for (int nb = 0; nb < NB; ++nb) vload(Z + (size_t)(16 * nb + lc) * ldzA + cc, bz[nb]);
+                    for (int nb = 0; nb < NB; ++nb) vload(Z + zt_index<VL, NR>(en.zcol + cc, 16 * nb + lc), bz[nb]);
                 } else {
                     vzero(a);
 #pragma unroll
@@ -293,14 +304,14 @@ __global__ void __launch_bounds__(256) k_mf_phase2(const int32_t* __restrict__ p
         for (int e = ptrB[leaf]; e < ptrB[leaf + 1]; ++e) {
             const MfEntry en = entB[e];
             const TB* __restrict__ U = (const TB*)en.u + (size_t)row * en.rpad;
-            const TB* __restrict__ Z = ZB + en.zcol;
+            const TB* __restrict__ Z = ZB;
             for (int c0 = kp * SS; c0 < en.rpad; c0 += KP * SS) {
                 const int cc = c0 + VL * lg;
                 TB a[VL], bz[NB][VL];
                 if (cc < en.rpad) {
                     vload_nt(U + cc, a);
 #pragma unroll
-                    for (int nb = 0; nb < NB; ++nb) vload(Z + (size_t)(16 * nb + lc) * ldzB + cc, bz[nb]);
+                    for (int nb = 0; nb < NB; ++nb) vload(Z + zt_index<VL, NR>(en.zcol + cc, 16 * nb + lc), bz[nb]);
                 } else {
                     vzero(a);
 #pragma unroll
