@@ -52,7 +52,7 @@ struct LrLevel {
     int capQ;
     // direct mode: the operator is the kernel itself, evaluated on the fly
     const double* P;        // padded KD-ordered points, (x, y, z, w) per position; w = 1 real, 0 padding
-    double* Ypart;          // column-split partial samples [split][T][16][m]
+    const int64_t* boff;    // A^T Q of tile t goes to column boff[t] of the packed V factors
 };
 
 __device__ inline bool lr_active(const LrState& s, int maxRank) {
@@ -65,14 +65,18 @@ __global__ void k_lr_init(int T, LrState* st) {
     st[t].k = 0; st[t].small_run = 0; st[t].converged = 0; st[t].capped = 0; st[t].na = 0;
 }
 
-// Om[t][s*m + c] (transposed == 0) or Om[t][c*16 + s] (transposed == 1)
-__global__ void __launch_bounds__(256) k_lr_omega(LrLevel L, int it, int transposed) {
+// Omega of the nb blocks it .. it+nb-1: Om[t][s*m + c] (transposed == 0,
+// nb == 1) or Om[t][c*16nb + 16p + s] for sample s of block it+p
+// (transposed == 1).  Entry (block, s, c) is the same number however many
+// blocks one call generates.
+__global__ void __launch_bounds__(256) k_lr_omega(LrLevel L, int it, int nb, int transposed) {
     int t = blockIdx.y;
     if (!lr_active(L.st[t], L.tp[t].maxRank)) return;
+    const int wd = kB16 * nb;
     int e = blockIdx.x * 256 + threadIdx.x;
-    if (e >= L.m * kB16) return;
-    int s = transposed ? e % kB16 : e / L.m, c = transposed ? e / kB16 : e % L.m;
-    L.Om[(size_t)t * L.m * kB16 + e] = gauss_at(L.seed, kTagOmega, L.tp[t].opid, (uint32_t)(it * kB16 + s), (uint32_t)c);
+    if (e >= L.m * wd) return;
+    int j = transposed ? e % wd : e / L.m, c = transposed ? e / wd : e % L.m;
+    L.Om[(size_t)t * L.m * wd + e] = gauss_at(L.seed, kTagOmega, L.tp[t].opid, (uint32_t)(it * kB16 + j), (uint32_t)c);
 }
 
 // Y rows of row-leaf i of tile t: sum over column leaves j of U_ij (V_ij^T Om_j)
@@ -151,253 +155,176 @@ __global__ void __launch_bounds__(256) k_pad_points(uint32_t n, uint32_t d, int 
     *o = make_double4(x[0], x[1], x[2], x[3]);
 }
 
-// Y[t][s*m + q] (or the split's partial) = sum_c K(x_q, x_c) Om[t][c*16 + s]
-template <int D>
-__global__ void __launch_bounds__(256) k_dir_sample(LrLevel L, int splits) {
-    __shared__ double4 sP[4][64];
-    __shared__ double sO[4][64][kB16];
-    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int m = L.m;
-    const int64_t gr = (int64_t)blockIdx.x * 256 + threadIdx.x;
-    const bool inb = gr < (int64_t)L.T * m;  // uniform per wave (T*m is a multiple of 64)
-    const int t = inb ? (int)(gr / m) : 0, q = (int)(gr % m);
-    const bool act = inb && lr_active(L.st[t], L.tp[t].maxRank);
-    const TileParam tpt = L.tp[t];
-    const uint32_t code = (uint32_t)tpt.code;
-    const double nil = tpt.nil;
-    const int64_t rowBase = tpt.pbase + (int64_t)compact16(code >> 1) * m;
-    const int64_t colBase = tpt.pbase + (int64_t)compact16(code) * m;
-    const double4 xr = act ? reinterpret_cast<const double4*>(L.P)[rowBase + q] : make_double4(0, 0, 0, 0);
-    const double xra[3] = {xr.x, xr.y, xr.z};
-    const int chunk = m / splits, c0 = blockIdx.y * chunk;
-    const double* Om = L.Om + (size_t)t * m * kB16;
-    double acc[kB16];
-#pragma unroll
-    for (int s = 0; s < kB16; ++s) acc[s] = 0.0;
-    for (int cb = c0; cb < c0 + chunk; cb += 64) {
-        __syncthreads();
-        if (act) {
-            sP[w][lane] = reinterpret_cast<const double4*>(L.P)[colBase + cb + lane];
-            const double2* src = reinterpret_cast<const double2*>(Om + (size_t)(cb + lane) * kB16);
-#pragma unroll
-            for (int s2 = 0; s2 < kB16 / 2; ++s2) {
-                const double2 v = src[s2];
-                sO[w][lane][2 * s2] = v.x;
-                sO[w][lane][2 * s2 + 1] = v.y;
-            }
-        }
-        __syncthreads();
-        if (act) {
-            for (int cc = 0; cc < 64; ++cc) {
-                const double4 xc = sP[w][cc];
-                const double xca[3] = {xc.x, xc.y, xc.z};
-                const double kv = kernel_eval_scaled<D>(xra, xca, nil) * xc.w;
-#pragma unroll
-                for (int s = 0; s < kB16; ++s) acc[s] = fma(kv, sO[w][cc][s], acc[s]);
-            }
-        }
-    }
-    if (!act) return;
-    double* Y = splits > 1 ? L.Ypart + ((size_t)blockIdx.y * L.T + t) * m * kB16 : L.Y + (size_t)t * m * kB16;
-#pragma unroll
-    for (int s = 0; s < kB16; ++s) Y[(size_t)s * m + q] = acc[s] * xr.w;
-}
-
-// The same two products on the FP64 matrix cores (v_mfma_f64_16x16x4f64).
-// A wave owns 64 rows (AΩ) or 64 columns (AᵀQ) of one tile as four blocks of
-// 16; per step of 4 along the inner index a lane evaluates one kernel entry
-// per block -- the A operand, lane l holding A[l%16][l/16] -- and reads one
-// entry of Ω or Q from LDS as the B operand (B[l/16][l%16]).  The 16x16
-// result of a block sits in register v of lane l at row 4v + l/16, column l%16.
+// The samplers run on the FP64 matrix cores (v_mfma_f64_16x16x4f64).  A
+// workgroup of nw = min(4, m/64) waves covers 64*nw consecutive rows (AΩ) or
+// columns (AᵀQ) of one tile; its waves share the staged points of the other
+// side and the staged rows of Ω or Q.  A wave owns 64 rows (columns) as four
+// blocks of 16; per step of 4 along the inner index a lane evaluates one
+// kernel entry per block -- the A operand, lane l holding A[l%16][l/16] -- and
+// uses it for every 16-column block of the dense operand, read from LDS as
+// the B operand (B[l/16][l%16]).  The 16x16 result of a block sits in
+// register v of lane l at row 4v + l/16, column l%16.  Kernel evaluations
+// (exp, sqrt on the FP64 vector units) dominate, so the more dense columns a
+// sweep carries, the fewer evaluations a tile costs.
 typedef double v4d_t __attribute__((ext_vector_type(4)));
 
-template <int D>
-__global__ void __launch_bounds__(256) k_dir_sample_mf(LrLevel L, int splits) {
-    __shared__ double4 sP[4][64];
-    __shared__ double sO[4][64][kB16];
+// NB consecutive ARA blocks in one sweep over the tile's columns: block p of
+// the pass is Y_p[t][s*m + q] = sum_c K(x_q, x_c) Om[t][c*16NB + 16p + s],
+// written to Yout + p*bstride + slot*16m with slot = t (one split) or
+// split*T + t (column splits, summed by k_dir_sum).  Per block the
+// accumulation order over c is that of a one-block sweep, so every block's
+// samples are the same bits whatever NB is.
+template <int D, int NB>
+__global__ void __launch_bounds__(256) k_dir_sample_mf(LrLevel L, int splits, double* __restrict__ Yout,
+                                                       size_t bstride) {
+    constexpr int NC = kB16 * NB;
+    __shared__ double4 sP[64];
+    __shared__ double sO[64 * NC];
+    const int nw = blockDim.x >> 6;
     const int w = threadIdx.x >> 6, lane = threadIdx.x & 63, lg = lane >> 4, lc = lane & 15;
     const int m = L.m;
-    const int64_t r0g = ((int64_t)blockIdx.x * 4 + w) * 64;  // first row of this wave in the level's row space
-    const bool inb = r0g < (int64_t)L.T * m;                 // m is a multiple of 64
-    const int t = inb ? (int)(r0g / m) : 0, q0 = (int)(r0g % m);
-    const bool act = inb && lr_active(L.st[t], L.tp[t].maxRank);
+    const int wgPerTile = m / (64 * nw);
+    const int t = blockIdx.x / wgPerTile;
+    if (!lr_active(L.st[t], L.tp[t].maxRank)) return;  // uniform over the workgroup
+    const int q0 = (blockIdx.x % wgPerTile) * 64 * nw + 64 * w;
     const TileParam tpt = L.tp[t];
     const uint32_t code = (uint32_t)tpt.code;
     const double nil = tpt.nil;
     const int64_t rowBase = tpt.pbase + (int64_t)compact16(code >> 1) * m;
     const int64_t colBase = tpt.pbase + (int64_t)compact16(code) * m;
+    const double4* P4 = reinterpret_cast<const double4*>(L.P);
     double xr[4][3], wr[4];
 #pragma unroll
     for (int rb = 0; rb < 4; ++rb) {
-        const double4 p = act ? reinterpret_cast<const double4*>(L.P)[rowBase + q0 + 16 * rb + lc]
-                              : make_double4(0, 0, 0, 0);
+        const double4 p = P4[rowBase + q0 + 16 * rb + lc];
         xr[rb][0] = p.x; xr[rb][1] = p.y; xr[rb][2] = p.z; wr[rb] = p.w;
     }
     const int chunk = m / splits, c0 = blockIdx.y * chunk;
-    const double* Om = L.Om + (size_t)t * m * kB16;
-    v4d_t acc[4];
-#pragma unroll
-    for (int rb = 0; rb < 4; ++rb) acc[rb] = v4d_t{0.0, 0.0, 0.0, 0.0};
-    for (int cb = c0; cb < c0 + chunk; cb += 64) {
-        __syncthreads();
-        if (act) {
-            sP[w][lane] = reinterpret_cast<const double4*>(L.P)[colBase + cb + lane];
-            const double2* src = reinterpret_cast<const double2*>(Om + (size_t)(cb + lane) * kB16);
-#pragma unroll
-            for (int s2 = 0; s2 < kB16 / 2; ++s2) {
-                const double2 v = src[s2];
-                sO[w][lane][2 * s2] = v.x;
-                sO[w][lane][2 * s2 + 1] = v.y;
-            }
-        }
-        __syncthreads();
-        if (act) {
-            for (int c4 = 0; c4 < 64; c4 += 4) {
-                const double4 xc = sP[w][c4 + lg];
-                const double xca[3] = {xc.x, xc.y, xc.z};
-                const double om = sO[w][c4 + lg][lc];
-#pragma unroll
-                for (int rb = 0; rb < 4; ++rb) {
-                    const double a = kernel_eval_scaled<D>(xr[rb], xca, nil) * (xc.w * wr[rb]);
-                    acc[rb] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, om, acc[rb], 0, 0, 0);
-                }
-            }
-        }
-    }
-    if (!act) return;
-    double* Y = splits > 1 ? L.Ypart + ((size_t)blockIdx.y * L.T + t) * m * kB16 : L.Y + (size_t)t * m * kB16;
+    const double* Om = L.Om + (size_t)t * m * NC;
+    v4d_t acc[4][NB];
 #pragma unroll
     for (int rb = 0; rb < 4; ++rb)
 #pragma unroll
-        for (int v = 0; v < 4; ++v) Y[(size_t)lc * m + q0 + 16 * rb + 4 * v + lg] = acc[rb][v];
+        for (int p = 0; p < NB; ++p) acc[rb][p] = v4d_t{0.0, 0.0, 0.0, 0.0};
+    for (int cb = c0; cb < c0 + chunk; cb += 64) {
+        __syncthreads();
+        if (threadIdx.x < 64) sP[threadIdx.x] = P4[colBase + cb + threadIdx.x];
+        const double2* src = reinterpret_cast<const double2*>(Om + (size_t)cb * NC);
+        for (int e = threadIdx.x; e < 32 * NC; e += blockDim.x) {
+            const double2 v = src[e];
+            sO[2 * e] = v.x;
+            sO[2 * e + 1] = v.y;
+        }
+        __syncthreads();
+        for (int c4 = 0; c4 < 64; c4 += 4) {
+            const double4 xc = sP[c4 + lg];
+            const double xca[3] = {xc.x, xc.y, xc.z};
+            double om[NB];
+#pragma unroll
+            for (int p = 0; p < NB; ++p) om[p] = sO[(c4 + lg) * NC + 16 * p + lc];
+#pragma unroll
+            for (int rb = 0; rb < 4; ++rb) {
+                const double a = kernel_eval_scaled<D>(xr[rb], xca, nil) * (xc.w * wr[rb]);
+#pragma unroll
+                for (int p = 0; p < NB; ++p) acc[rb][p] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, om[p], acc[rb][p], 0, 0, 0);
+            }
+        }
+    }
+    const size_t slot = splits > 1 ? (size_t)blockIdx.y * L.T + t : (size_t)t;
+#pragma unroll
+    for (int p = 0; p < NB; ++p) {
+        double* Y = Yout + (size_t)p * bstride + slot * m * kB16;
+#pragma unroll
+        for (int rb = 0; rb < 4; ++rb)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) Y[(size_t)lc * m + q0 + 16 * rb + 4 * v + lg] = acc[rb][p][v];
+    }
 }
 
-// B[t][a*m + c] = sum_q K(x_q, x_c) Q[t][a*m + q] for 32 basis columns a0.. per launch row
-template <int D>
+// B[t][a*m + c] = sum_q K(x_q, x_c) Q[t][a*m + q] for the 16NB basis columns
+// a0 = 16NB*blockIdx.y ... of every tile with k > a0; the blocks past the
+// tile's k are skipped.  A column's sum runs over q in the same order for any
+// NB, so B does not depend on NB.
+template <int D, int NB>
 __global__ void __launch_bounds__(256) k_dir_sampleT_mf(LrLevel L, double* __restrict__ Bout) {
-    constexpr int NQ = 32;
-    __shared__ double4 sP[4][64];
-    __shared__ double sQ[4][64][NQ];
+    constexpr int NQ = kB16 * NB;
+    __shared__ double4 sP[64];
+    __shared__ double sQ[64][NQ + 1];
+    const int nw = blockDim.x >> 6;
     const int w = threadIdx.x >> 6, lane = threadIdx.x & 63, lg = lane >> 4, lc = lane & 15;
     const int m = L.m;
-    const int64_t c0g = ((int64_t)blockIdx.x * 4 + w) * 64;
-    const bool inb = c0g < (int64_t)L.T * m;
-    const int t = inb ? (int)(c0g / m) : 0, cw = (int)(c0g % m);
-    const int k = inb ? L.st[t].k : 0;
+    const int wgPerTile = m / (64 * nw);
+    const int t = blockIdx.x / wgPerTile;
+    const int k = L.st[t].k;
     const int a0 = blockIdx.y * NQ;
-    const bool act = inb && a0 < k;
-    const int na = act ? min(NQ, k - a0) : 0;
+    if (a0 >= k) return;  // uniform over the workgroup
+    const int na = min(NQ, k - a0);
+    const int cw = (blockIdx.x % wgPerTile) * 64 * nw + 64 * w;
     const TileParam tpt = L.tp[t];
     const uint32_t code = (uint32_t)tpt.code;
     const double nil = tpt.nil;
     const int64_t rowBase = tpt.pbase + (int64_t)compact16(code >> 1) * m;
     const int64_t colBase = tpt.pbase + (int64_t)compact16(code) * m;
+    const double4* P4 = reinterpret_cast<const double4*>(L.P);
     double xc[4][3], wc[4];
 #pragma unroll
     for (int cb = 0; cb < 4; ++cb) {
-        const double4 p = act ? reinterpret_cast<const double4*>(L.P)[colBase + cw + 16 * cb + lc]
-                              : make_double4(0, 0, 0, 0);
+        const double4 p = P4[colBase + cw + 16 * cb + lc];
         xc[cb][0] = p.x; xc[cb][1] = p.y; xc[cb][2] = p.z; wc[cb] = p.w;
     }
-    const double* Q = L.Q + (size_t)t * m * L.capQ;
-    v4d_t acc[4][2];
-#pragma unroll
-    for (int cb = 0; cb < 4; ++cb) { acc[cb][0] = v4d_t{0.0, 0.0, 0.0, 0.0}; acc[cb][1] = acc[cb][0]; }
-    for (int qb = 0; qb < m; qb += 64) {
-        __syncthreads();
-        if (act) {
-            sP[w][lane] = reinterpret_cast<const double4*>(L.P)[rowBase + qb + lane];
-#pragma unroll
-            for (int a = 0; a < NQ; ++a) sQ[w][lane][a] = a < na ? Q[(size_t)(a0 + a) * m + qb + lane] : 0.0;
-        }
-        __syncthreads();
-        if (act) {
-            for (int q4 = 0; q4 < 64; q4 += 4) {
-                const double4 xr = sP[w][q4 + lg];
-                const double xra[3] = {xr.x, xr.y, xr.z};
-                const double b0 = sQ[w][q4 + lg][lc], b1 = sQ[w][q4 + lg][16 + lc];
-#pragma unroll
-                for (int cb = 0; cb < 4; ++cb) {
-                    const double a = kernel_eval_scaled<D>(xra, xc[cb], nil) * (xr.w * wc[cb]);
-                    acc[cb][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b0, acc[cb][0], 0, 0, 0);
-                    acc[cb][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b1, acc[cb][1], 0, 0, 0);
-                }
-            }
-        }
-    }
-    if (!act) return;
-    double* B = Bout + (size_t)t * m * L.capQ;
+    const double* Q = L.Q + (size_t)t * m * L.capQ + (size_t)a0 * m;
+    v4d_t acc[4][NB];
 #pragma unroll
     for (int cb = 0; cb < 4; ++cb)
 #pragma unroll
-        for (int ab = 0; ab < 2; ++ab)
+        for (int p = 0; p < NB; ++p) acc[cb][p] = v4d_t{0.0, 0.0, 0.0, 0.0};
+    for (int qb = 0; qb < m; qb += 64) {
+        __syncthreads();
+        if (threadIdx.x < 64) sP[threadIdx.x] = P4[rowBase + qb + threadIdx.x];
+        for (int e = threadIdx.x; e < 64 * NQ; e += blockDim.x) {
+            const int a = e >> 6, r = e & 63;
+            sQ[r][a] = a < na ? Q[(size_t)a * m + qb + r] : 0.0;
+        }
+        __syncthreads();
+        for (int q4 = 0; q4 < 64; q4 += 4) {
+            const double4 xr = sP[q4 + lg];
+            const double xra[3] = {xr.x, xr.y, xr.z};
+            double bq[NB];
+#pragma unroll
+            for (int p = 0; p < NB; ++p) bq[p] = sQ[q4 + lg][16 * p + lc];
+#pragma unroll
+            for (int cb = 0; cb < 4; ++cb) {
+                const double a = kernel_eval_scaled<D>(xra, xc[cb], nil) * (xr.w * wc[cb]);
+#pragma unroll
+                for (int p = 0; p < NB; ++p)
+                    if (16 * p < na) acc[cb][p] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, bq[p], acc[cb][p], 0, 0, 0);
+            }
+        }
+    }
+    double* B = Bout + (size_t)L.boff[t] * m;
+#pragma unroll
+    for (int cb = 0; cb < 4; ++cb)
+#pragma unroll
+        for (int p = 0; p < NB; ++p)
 #pragma unroll
             for (int v = 0; v < 4; ++v) {
-                const int a = 16 * ab + lc;
-                if (a < na) B[(size_t)(a0 + a) * m + cw + 16 * cb + 4 * v + lg] = acc[cb][ab][v];
+                const int a = 16 * p + lc;
+                if (a < na) B[(size_t)(a0 + a) * m + cw + 16 * cb + 4 * v + lg] = acc[cb][p][v];
             }
 }
 
-// Y = sum over the splits, in split order
-__global__ void __launch_bounds__(256) k_dir_sum(LrLevel L, int splits) {
-    const int t = blockIdx.y;
+// Y_p = sum over the splits, in split order, for the NB blocks of a pass
+__global__ void __launch_bounds__(256) k_dir_sum(LrLevel L, int splits, const double* __restrict__ Ypart,
+                                                 double* __restrict__ Yout, size_t bstride, size_t pstride) {
+    const int t = blockIdx.y, p = blockIdx.z;
     if (!lr_active(L.st[t], L.tp[t].maxRank)) return;
     const int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x;
     const int64_t per = (int64_t)L.m * kB16;
     if (e >= per) return;
+    const double* src = Ypart + (size_t)p * pstride;
     double v = 0.0;
-    for (int sp = 0; sp < splits; ++sp) v += L.Ypart[((size_t)sp * L.T + t) * per + e];
-    L.Y[(size_t)t * per + e] = v;
-}
-
-// B[t][a*m + c] = sum_q K(x_q, x_c) Q[t][a*m + q], NQ basis columns per launch row
-template <int D, int NQ>
-__global__ void __launch_bounds__(256) k_dir_sampleT(LrLevel L, double* __restrict__ Bout) {
-    __shared__ double4 sP[4][64];
-    __shared__ double sQ[4][64][NQ];
-    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int m = L.m;
-    const int64_t gc = (int64_t)blockIdx.x * 256 + threadIdx.x;
-    const bool inb = gc < (int64_t)L.T * m;
-    const int t = inb ? (int)(gc / m) : 0, c = (int)(gc % m);
-    const int k = inb ? L.st[t].k : 0;
-    const int a0 = blockIdx.y * NQ;
-    const bool act = inb && a0 < k;
-    const int na = act ? min(NQ, k - a0) : 0;
-    const TileParam tpt = L.tp[t];
-    const uint32_t code = (uint32_t)tpt.code;
-    const double nil = tpt.nil;
-    const int64_t rowBase = tpt.pbase + (int64_t)compact16(code >> 1) * m;
-    const int64_t colBase = tpt.pbase + (int64_t)compact16(code) * m;
-    const double4 xc = act ? reinterpret_cast<const double4*>(L.P)[colBase + c] : make_double4(0, 0, 0, 0);
-    const double xca[3] = {xc.x, xc.y, xc.z};
-    const double* Q = L.Q + (size_t)t * m * L.capQ;
-    double acc[NQ];
-#pragma unroll
-    for (int a = 0; a < NQ; ++a) acc[a] = 0.0;
-    for (int qb = 0; qb < m; qb += 64) {
-        __syncthreads();
-        if (act) {
-            sP[w][lane] = reinterpret_cast<const double4*>(L.P)[rowBase + qb + lane];
-#pragma unroll
-            for (int a = 0; a < NQ; ++a) sQ[w][lane][a] = a < na ? Q[(size_t)(a0 + a) * m + qb + lane] : 0.0;
-        }
-        __syncthreads();
-        if (act) {
-            for (int qq = 0; qq < 64; ++qq) {
-                const double4 xr = sP[w][qq];
-                const double xra[3] = {xr.x, xr.y, xr.z};
-                const double kv = kernel_eval_scaled<D>(xra, xca, nil) * xr.w;
-#pragma unroll
-                for (int a = 0; a < NQ; ++a) acc[a] = fma(kv, sQ[w][qq][a], acc[a]);
-            }
-        }
-    }
-    if (!act) return;
-    double* B = Bout + (size_t)t * m * L.capQ;
-#pragma unroll
-    for (int a = 0; a < NQ; ++a)
-        if (a < na) B[(size_t)(a0 + a) * m + c] = acc[a] * xc.w;
+    for (int sp = 0; sp < splits; ++sp) v += src[((size_t)sp * L.T + t) * per + e];
+    Yout[(size_t)p * bstride + (size_t)t * per + e] = v;
 }
 
 // W[t][s*k + a] = sum_i Q_t[a*m+i] B_t[s*m+i], a < k_t:
@@ -623,7 +550,7 @@ __global__ void __launch_bounds__(256) k_lr_sampleT(LrLevel L, double* Bout) {
         }
         __syncthreads();
     }
-    double* B = Bout + (size_t)t * m * L.capQ;
+    double* B = Bout + (size_t)L.boff[t] * m;
 #pragma unroll
     for (int q = 0; q < NACC; ++q) {
         int e = threadIdx.x + q * 256;
@@ -701,28 +628,129 @@ static void launch_sampleT_tlr(Context& c, const LrLevel& L, int kmax, int bus, 
     HMX_LAUNCHED();
 }
 
-template <int D>
-static void launch_sample_direct(Context& c, const LrLevel& L, int splits) {
-    const int64_t rows = (int64_t)L.T * L.m;
-    k_dir_sample_mf<D><<<dim3((unsigned)((rows + 255) / 256), (unsigned)splits), 256, 0, c.stream>>>(L, splits);
-    HMX_LAUNCHED();
+// direct samplers: workgroups of nw waves, all in one tile (m is 64 * 2^j)
+static int sampler_threads(int m) { return 64 * (m >= 256 ? 4 : m / 64); }
+
+template <int D, int NB>
+static void launch_sample_direct_nb(Context& c, const LrLevel& L, int splits, double* Ypre, double* Ypart) {
+    const int thr = sampler_threads(L.m);
+    const unsigned wgs = (unsigned)((int64_t)L.T * L.m / thr);
+    const size_t per = (size_t)L.T * L.m * kB16;
     if (splits > 1) {
-        k_dir_sum<<<dim3((unsigned)(((int64_t)L.m * kB16 + 255) / 256), (unsigned)L.T), 256, 0, c.stream>>>(L, splits);
+        k_dir_sample_mf<D, NB><<<dim3(wgs, (unsigned)splits), thr, 0, c.stream>>>(L, splits, Ypart, per * splits);
+        HMX_LAUNCHED();
+        k_dir_sum<<<dim3((unsigned)(((int64_t)L.m * kB16 + 255) / 256), (unsigned)L.T, (unsigned)NB), 256, 0,
+                    c.stream>>>(L, splits, Ypart, Ypre, per, per * splits);
+        HMX_LAUNCHED();
+    } else {
+        k_dir_sample_mf<D, NB><<<dim3(wgs, 1u), thr, 0, c.stream>>>(L, 1, Ypre, per);
         HMX_LAUNCHED();
     }
 }
 
+// nb consecutive ARA blocks from one sweep over every active tile
+template <int D>
+static void launch_sample_direct(Context& c, const LrLevel& L, int nb, int splits, double* Ypre, double* Ypart) {
+    switch (nb) {
+        case 1: launch_sample_direct_nb<D, 1>(c, L, splits, Ypre, Ypart); break;
+        case 2: launch_sample_direct_nb<D, 2>(c, L, splits, Ypre, Ypart); break;
+        case 3: launch_sample_direct_nb<D, 3>(c, L, splits, Ypre, Ypart); break;
+        default: launch_sample_direct_nb<D, 4>(c, L, splits, Ypre, Ypart); break;
+    }
+}
+
+template <int D, int NB>
+static void launch_sampleT_direct_nb(Context& c, const LrLevel& L, int kmax, double* B) {
+    const int thr = sampler_threads(L.m);
+    const unsigned wgs = (unsigned)((int64_t)L.T * L.m / thr);
+    k_dir_sampleT_mf<D, NB><<<dim3(wgs, (unsigned)((kmax + kB16 * NB - 1) / (kB16 * NB))), thr, 0, c.stream>>>(L, B);
+    HMX_LAUNCHED();
+}
+
+// one sweep carries up to 64 basis columns
 template <int D>
 static void launch_sampleT_direct(Context& c, const LrLevel& L, int kmax, double* B) {
-    const int64_t cols = (int64_t)L.T * L.m;
-    k_dir_sampleT_mf<D><<<dim3((unsigned)((cols + 255) / 256), (unsigned)((kmax + 31) / 32)), 256, 0, c.stream>>>(L, B);
-    HMX_LAUNCHED();
+    switch (std::min(4, (kmax + kB16 - 1) / kB16)) {
+        case 1: launch_sampleT_direct_nb<D, 1>(c, L, kmax, B); break;
+        case 2: launch_sampleT_direct_nb<D, 2>(c, L, kmax, B); break;
+        case 3: launch_sampleT_direct_nb<D, 3>(c, L, kmax, B); break;
+        default: launch_sampleT_direct_nb<D, 4>(c, L, kmax, B); break;
+    }
 }
 
 // progress lines on stderr when HMX_LOG is set (long builds)
 static bool log_on() {
     static const bool on = std::getenv("HMX_LOG") != nullptr;
     return on;
+}
+
+// Scratch of the batched ARA.  One build keeps it across its levels and
+// sizes it once for the largest of them: allocating tens of GB afresh for
+// every level (right after freeing the previous level's) stalled the build
+// for a second and more per level.
+struct AraScratch {
+    DBuf<double> Q, W, Ya, Om, Ypre, Ypart, Y;
+    DBuf<LrState> st;
+    DBuf<TileParam> tp;
+    DBuf<int64_t> first, off;
+    DBuf<int> flags;
+};
+
+// Buffer sizes of one batched ARA launch sequence.  Direct mode: one
+// sampler sweep yields up to nbMax consecutive blocks (each kernel entry
+// evaluated once for all of them), kept in Ypre until the ARA reaches them;
+// small launches split their columns over several workgroups.
+struct AraShape {
+    int capQ = 0, nbMax = 1, splits = 1;
+    size_t blockElems = 0;
+};
+
+static AraShape ara_shape(int T, int m, int maxRank, bool direct) {
+    AraShape s;
+    s.capQ = std::max(std::min(maxRank, 128), std::min(maxRank, kB16));
+    s.blockElems = (size_t)T * m * kB16;
+    if (direct) {
+        s.nbMax = (int)std::max<size_t>(1, std::min<size_t>(4, (size_t(16) << 30) / (2 * s.blockElems * 8)));
+        const int64_t rows = (int64_t)T * m;
+        while (s.splits * 2 <= m / 64 && rows * s.splits < ((int64_t)1 << 18)) s.splits *= 2;
+    }
+    return s;
+}
+
+struct AraSizes {
+    size_t q = 0, w = 0, blk = 0, om = 0, ypre = 0, ypart = 0, y = 0, tiles = 0;
+    void add(int T, int m, int maxRank, bool direct) {
+        const AraShape s = ara_shape(T, m, maxRank, direct);
+        q = std::max(q, (size_t)T * m * s.capQ);
+        w = std::max(w, (size_t)T * s.capQ * kB16);
+        blk = std::max(blk, s.blockElems);
+        om = std::max(om, s.blockElems * s.nbMax);
+        if (direct) ypre = std::max(ypre, s.blockElems * s.nbMax);
+        else y = std::max(y, s.blockElems);
+        if (s.splits > 1) ypart = std::max(ypart, (size_t)s.splits * s.blockElems * s.nbMax);
+        tiles = std::max(tiles, (size_t)T);
+    }
+};
+
+template <class X>
+static X* need(DBuf<X>& b, size_t n) {
+    if (b.n < n) b.alloc(n);
+    return b.p;
+}
+
+static void ara_reserve(AraScratch& ws, const AraSizes& z) {
+    need(ws.Q, z.q);
+    need(ws.W, z.w);
+    need(ws.Ya, z.blk);
+    need(ws.Om, z.om);
+    need(ws.Ypre, z.ypre);
+    need(ws.Y, z.y);
+    need(ws.Ypart, z.ypart);
+    need(ws.st, z.tiles);
+    need(ws.tp, z.tiles);
+    need(ws.first, z.y ? z.tiles : 0);
+    need(ws.off, z.tiles + 1);
+    need(ws.flags, 2);
 }
 
 // Result of one batched ARA launch sequence: ranks, exclusive column
@@ -733,57 +761,64 @@ struct LrOut {
     std::vector<int64_t> offsets;
     DBuf<double> U, V;
     int64_t rankSum = 0;
-    int caps = 0, kmax = 0, iters = 0;
+    int caps = 0, kmax = 0, iters = 0, sweeps = 0;
 };
 
 // Batched ARA over T tiles of dimension m (bus = m/b leaves per side), each
 // with its own threshold, rank cap, random stream and (direct mode) point
-// base and correlation length.
+// base and correlation length.  Scratch comes from ws (grown if too small).
 static void compress_batch(Context& c, int araR, uint64_t seed, const LevelSource& src, int b, int m,
-                           const std::vector<TileParam>& tp, const std::vector<int64_t>& first, LrOut& out) {
+                           const std::vector<TileParam>& tp, const std::vector<int64_t>& first, int firstBlocks,
+                           AraScratch& ws, LrOut& out) {
     const bool direct = src.mo == nullptr;
     const int bus = m / b;
     const int T = (int)tp.size();
+    const auto tA = std::chrono::steady_clock::now();
     out.ranks.assign(T, 0);
     out.offsets.assign(T + 1, 0);
     if (T == 0) return;
     HMX_CHECK(!direct || m % 64 == 0, HMX_EUNSUPPORTED, "direct compression needs tile dimensions that are multiples of 64");
     int maxRank = 0;
     for (const TileParam& q : tp) maxRank = std::max(maxRank, (int)q.maxRank);
-    DBuf<TileParam> d_tp(T);
-    HMX_HIP(hipMemcpyAsync(d_tp.p, tp.data(), T * sizeof(TileParam), hipMemcpyHostToDevice, c.stream));
-    DBuf<int64_t> d_first(direct ? 0 : T);
-    if (!direct)
-        HMX_HIP(hipMemcpyAsync(d_first.p, first.data(), T * sizeof(int64_t), hipMemcpyHostToDevice, c.stream));
-    DBuf<LrState> st(T);
-    int capQ = std::min(maxRank, 128);
-    capQ = std::max(capQ, std::min(maxRank, kB16));
-    DBuf<double> Q((size_t)T * m * capQ), W((size_t)T * capQ * kB16);
-    DBuf<double> Y((size_t)T * m * kB16), Ya((size_t)T * m * kB16), Om((size_t)T * m * kB16);
-    // direct mode: split the columns of small launches over several workgroups
-    int splits = 1;
-    if (direct) {
-        const int64_t rows = (int64_t)T * m;
-        while (splits * 2 <= m / 64 && rows * splits < ((int64_t)1 << 18)) splits *= 2;
+    const AraShape shape = ara_shape(T, m, maxRank, direct);
+    {
+        AraSizes z;
+        z.add(T, m, maxRank, direct);
+        ara_reserve(ws, z);
     }
-    DBuf<double> Ypart(splits > 1 ? (size_t)splits * T * m * kB16 : 0);
-    DBuf<int> flags(2);
+    int capQ = shape.capQ;
+    const size_t blockElems = shape.blockElems;
+    const int nbMax = shape.nbMax, splits = shape.splits;
+    HMX_HIP(hipMemcpyAsync(ws.tp.p, tp.data(), T * sizeof(TileParam), hipMemcpyHostToDevice, c.stream));
+    if (!direct)
+        HMX_HIP(hipMemcpyAsync(ws.first.p, first.data(), T * sizeof(int64_t), hipMemcpyHostToDevice, c.stream));
+    int nbFirst = std::max(1, std::min(nbMax, firstBlocks)), nbLater = std::min(nbMax, 2);
+    if (const char* e = std::getenv("HMX_SAMPLER_BLOCKS")) {  // tuning: "first,later"
+        int f = 0, l = 0;
+        if (std::sscanf(e, "%d,%d", &f, &l) == 2 && f >= 1 && l >= 1) {
+            nbFirst = std::min(nbMax, f);
+            nbLater = std::min(nbMax, l);
+        }
+    }
     LrLevel L;
     L.T = T; L.m = m; L.b = b; L.bus = bus; L.r = araR;
     L.seed = seed;
-    L.tp = d_tp.p;
-    L.first = d_first.p;
+    L.tp = ws.tp.p;
+    L.first = direct ? nullptr : ws.first.p;
     L.ranksMO = direct ? nullptr : src.mo->ranks.p;
     L.offMO = direct ? nullptr : src.mo->offsets.p;
     L.UMO = direct ? nullptr : src.mo->U.p;
     L.VMO = direct ? nullptr : src.mo->V.p;
-    L.st = st.p; L.Q = Q.p; L.Y = Y.p; L.Ya = Ya.p; L.Om = Om.p; L.W = W.p; L.capQ = capQ;
-    L.P = src.P; L.Ypart = Ypart.p;
-    k_lr_init<<<(T + 255) / 256, 256, 0, c.stream>>>(T, st.p);
+    L.st = ws.st.p; L.Q = ws.Q.p; L.Y = direct ? ws.Ypre.p : ws.Y.p; L.Ya = ws.Ya.p; L.Om = ws.Om.p; L.W = ws.W.p;
+    L.capQ = capQ;
+    L.P = src.P;
+    L.boff = nullptr;
+    k_lr_init<<<(T + 255) / 256, 256, 0, c.stream>>>(T, ws.st.p);
     HMX_LAUNCHED();
+    const auto tB = std::chrono::steady_clock::now();
     const unsigned rowBlocks = (unsigned)((m + 255) / 256);
     const unsigned wgThreads = m >= 4096 ? 1024u : 256u;
-    int kmax = 0;
+    int kmax = 0, preStart = 0, preN = 0, sweeps = 0;
     const int maxIters = (2 * maxRank + araR) / kB16 + 2;
     for (int it = 0;; ++it) {
         HMX_CHECK(it <= maxIters, HMX_EDEVICE, "batched ARA did not converge (non-finite input?)");
@@ -791,24 +826,32 @@ static void compress_batch(Context& c, int araR, uint64_t seed, const LevelSourc
             int nc = std::min(maxRank, std::max(2 * capQ, kmax + kB16));
             DBuf<double> Q2((size_t)T * m * nc), W2((size_t)T * nc * kB16);
             k_lr_regrow<<<dim3(std::min<unsigned>(1024, (unsigned)(((size_t)m * kmax + 255) / 256 + 1)), T), 256, 0,
-                          c.stream>>>(T, m, capQ, nc, st.p, Q.p, Q2.p);
+                          c.stream>>>(T, m, capQ, nc, ws.st.p, ws.Q.p, Q2.p);
             HMX_LAUNCHED();
             HMX_HIP(hipStreamSynchronize(c.stream));
-            Q = std::move(Q2);
-            W = std::move(W2);
+            ws.Q = std::move(Q2);
+            ws.W = std::move(W2);
             capQ = nc;
-            L.Q = Q.p; L.W = W.p; L.capQ = capQ;
+            L.Q = ws.Q.p; L.W = ws.W.p; L.capQ = capQ;
         }
-        k_lr_omega<<<dim3((m * kB16 + 255) / 256, T), 256, 0, c.stream>>>(L, it, direct ? 1 : 0);
-        HMX_LAUNCHED();
         if (direct) {
-            if (src.d == 1) launch_sample_direct<1>(c, L, splits);
-            else if (src.d == 2) launch_sample_direct<2>(c, L, splits);
-            else launch_sample_direct<3>(c, L, splits);
-        } else if (b <= 32) {
-            launch_sample_tlr<32>(c, L, bus, T);
+            if (it >= preStart + preN) {  // sweep for the next nb blocks
+                const int nb = it == 0 ? nbFirst : nbLater;
+                k_lr_omega<<<dim3((m * kB16 * nb + 255) / 256, T), 256, 0, c.stream>>>(L, it, nb, 1);
+                HMX_LAUNCHED();
+                if (src.d == 1) launch_sample_direct<1>(c, L, nb, splits, ws.Ypre.p, ws.Ypart.p);
+                else if (src.d == 2) launch_sample_direct<2>(c, L, nb, splits, ws.Ypre.p, ws.Ypart.p);
+                else launch_sample_direct<3>(c, L, nb, splits, ws.Ypre.p, ws.Ypart.p);
+                preStart = it;
+                preN = nb;
+                ++sweeps;
+            }
+            L.Y = ws.Ypre.p + (size_t)(it - preStart) * blockElems;
         } else {
-            launch_sample_tlr<64>(c, L, bus, T);
+            k_lr_omega<<<dim3((m * kB16 + 255) / 256, T), 256, 0, c.stream>>>(L, it, 1, 0);
+            HMX_LAUNCHED();
+            if (b <= 32) launch_sample_tlr<32>(c, L, bus, T);
+            else launch_sample_tlr<64>(c, L, bus, T);
         }
         const unsigned kb = (unsigned)std::max(1, (kmax + 15) / 16);
         for (int pass = 0; pass < 2 && kmax > 0; ++pass) {
@@ -827,35 +870,21 @@ static void compress_batch(Context& c, int araR, uint64_t seed, const LevelSourc
         }
         k_lr_renorm<<<T, wgThreads, 0, c.stream>>>(L);
         HMX_LAUNCHED();
-        HMX_HIP(hipMemsetAsync(flags.p, 0, 2 * sizeof(int), c.stream));
-        k_lr_commit<<<(T + 255) / 256, 256, 0, c.stream>>>(L, flags.p);
+        HMX_HIP(hipMemsetAsync(ws.flags.p, 0, 2 * sizeof(int), c.stream));
+        k_lr_commit<<<(T + 255) / 256, 256, 0, c.stream>>>(L, ws.flags.p);
         HMX_LAUNCHED();
         int hf[2];
-        HMX_HIP(hipMemcpyAsync(hf, flags.p, sizeof(hf), hipMemcpyDeviceToHost, c.stream));
+        HMX_HIP(hipMemcpyAsync(hf, ws.flags.p, sizeof(hf), hipMemcpyDeviceToHost, c.stream));
         HMX_HIP(hipStreamSynchronize(c.stream));
         kmax = hf[1];
         out.iters = it + 1;
         if (hf[0] == 0) break;
     }
-    Ypart.release();
-    Y.release();
-    Ya.release();
-    Om.release();
-    // B = A^T Q
-    DBuf<double> Bf((size_t)T * m * capQ);
-    if (kmax > 0) {
-        if (direct) {
-            if (src.d == 1) launch_sampleT_direct<1>(c, L, kmax, Bf.p);
-            else if (src.d == 2) launch_sampleT_direct<2>(c, L, kmax, Bf.p);
-            else launch_sampleT_direct<3>(c, L, kmax, Bf.p);
-        } else if (b <= 32) {
-            launch_sampleT_tlr<32>(c, L, kmax, bus, T, Bf.p);
-        } else {
-            launch_sampleT_tlr<64>(c, L, kmax, bus, T, Bf.p);
-        }
-    }
+    const auto tC = std::chrono::steady_clock::now();
+    out.sweeps = sweeps;
+    // ranks and packed offsets, then B = A^T Q straight into the packed V
     std::vector<LrState> hst(T);
-    HMX_HIP(hipMemcpyAsync(hst.data(), st.p, T * sizeof(LrState), hipMemcpyDeviceToHost, c.stream));
+    HMX_HIP(hipMemcpyAsync(hst.data(), ws.st.p, T * sizeof(LrState), hipMemcpyDeviceToHost, c.stream));
     HMX_HIP(hipStreamSynchronize(c.stream));
     int64_t acc = 0;
     int caps = 0;
@@ -874,15 +903,30 @@ static void compress_batch(Context& c, int araR, uint64_t seed, const LevelSourc
     out.U.alloc((size_t)acc * m);
     out.V.alloc((size_t)acc * m);
     if (acc > 0) {
-        DBuf<int64_t> d_off(T + 1);
-        HMX_HIP(hipMemcpyAsync(d_off.p, out.offsets.data(), (T + 1) * sizeof(int64_t), hipMemcpyHostToDevice,
+        HMX_HIP(hipMemcpyAsync(ws.off.p, out.offsets.data(), (T + 1) * sizeof(int64_t), hipMemcpyHostToDevice,
                                c.stream));
+        L.boff = ws.off.p;
+        if (direct) {
+            if (src.d == 1) launch_sampleT_direct<1>(c, L, kmax, out.V.p);
+            else if (src.d == 2) launch_sampleT_direct<2>(c, L, kmax, out.V.p);
+            else launch_sampleT_direct<3>(c, L, kmax, out.V.p);
+        } else if (b <= 32) {
+            launch_sampleT_tlr<32>(c, L, kmax, bus, T, out.V.p);
+        } else {
+            launch_sampleT_tlr<64>(c, L, kmax, bus, T, out.V.p);
+        }
         dim3 g(std::min<unsigned>(1024, (unsigned)(((size_t)m * kmax + 255) / 256 + 1)), T);
-        k_lr_pack<<<g, 256, 0, c.stream>>>(T, m, capQ, st.p, d_off.p, Q.p, out.U.p);
-        HMX_LAUNCHED();
-        k_lr_pack<<<g, 256, 0, c.stream>>>(T, m, capQ, st.p, d_off.p, Bf.p, out.V.p);
+        k_lr_pack<<<g, 256, 0, c.stream>>>(T, m, capQ, ws.st.p, ws.off.p, ws.Q.p, out.U.p);
         HMX_LAUNCHED();
         HMX_HIP(hipStreamSynchronize(c.stream));
+    }
+    if (log_on()) {
+        auto ms = [](std::chrono::steady_clock::time_point a, std::chrono::steady_clock::time_point b) {
+            return std::chrono::duration<double, std::milli>(b - a).count();
+        };
+        const auto tD = std::chrono::steady_clock::now();
+        std::fprintf(stderr, "[hmx]   T=%d m=%d: setup %.1f ms, ARA loop %.1f ms, A^T Q + pack %.1f ms\n", T, m,
+                     ms(tA, tB), ms(tB, tC), ms(tC, tD));
     }
 }
 
@@ -897,9 +941,20 @@ static TileParam tile_param(double tol, double ell, int64_t pbase, int level, in
     return q;
 }
 
+// Blocks a tile of the next level up is likely to need in its first sweep:
+// the ARA stops r samples after its last accepted column, so a tile of rank
+// k takes about (k + r)/16 blocks; the 10th-percentile rank of the level
+// below stands in for k (ranks grow slowly towards the root).
+static int first_sweep_blocks(std::vector<int32_t> ranks, int araR) {
+    if (ranks.empty()) return 2;
+    const size_t q = ranks.size() / 10;
+    std::nth_element(ranks.begin(), ranks.begin() + q, ranks.end());
+    return std::max(1, (ranks[q] + araR) / kB16);
+}
+
 static void compress_level(Context& c, const hmx_config& cfg, const LevelSource& src, int b,
                            const std::vector<int32_t>& tiles, int level, int numLevels, float tol, int maxRank,
-                           HLevel& out) {
+                           int firstBlocks, AraScratch& ws, HLevel& out) {
     const bool direct = src.mo == nullptr;
     const auto tstart = std::chrono::steady_clock::now();
     const int bus = 1 << (numLevels - (level + 1));
@@ -920,7 +975,7 @@ static void compress_level(Context& c, const hmx_config& cfg, const LevelSource&
         first[t] = (int64_t)tiles[t] * bus * bus;
     }
     LrOut r;
-    compress_batch(c, cfg.araR, cfg.seed, src, b, m, tp, first, r);
+    compress_batch(c, cfg.araR, cfg.seed, src, b, m, tp, first, firstBlocks, ws, r);
     out.h_ranks = std::move(r.ranks);
     out.h_offsets = std::move(r.offsets);
     out.rankSum = r.rankSum;
@@ -929,8 +984,8 @@ static void compress_level(Context& c, const hmx_config& cfg, const LevelSource&
     out.V = std::move(r.V);
     if (log_on()) {
         const double ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - tstart).count();
-        std::fprintf(stderr, "[hmx] level %d (%s): %d tiles m=%d rank sum %lld max %d, %d ARA blocks, %.1f ms\n", level,
-                     direct ? "direct" : "TLR sum", T, m, (long long)out.rankSum, r.kmax, r.iters, ms);
+        std::fprintf(stderr, "[hmx] level %d (%s): %d tiles m=%d rank sum %lld max %d, %d ARA blocks in %d sweeps, %.1f ms\n",
+                     level, direct ? "direct" : "TLR sum", T, m, (long long)out.rankSum, r.kmax, r.iters, r.sweeps, ms);
         std::fflush(stderr);
     }
 }
@@ -943,7 +998,21 @@ static void compress_upper_levels(Context& c, const hmx_config& cfg, const Struc
     const int L = s.numLevels;
     std::vector<uint32_t> maxRanks(L > 2 ? L - 2 : 1, 0);
     hmx_generateMaxRanks((uint32_t)L, (uint32_t)b, maxRanks.data());
+    auto level_cap = [&](int level) {
+        const int m = (1 << (L - (level + 1))) * b;
+        const int maxRank = cfg.maxRankPolicy == HMX_MAXRANK_REF ? (int)maxRanks[L - level - 2] : m;
+        return std::min(maxRank, m);
+    };
+    AraScratch ws;
+    {
+        AraSizes z;
+        for (int level = L - 2; level > 0; --level)
+            if (s.numTiles[level - 1] > 0)
+                z.add(s.numTiles[level - 1], (1 << (L - (level + 1))) * b, level_cap(level), src.mo == nullptr);
+        ara_reserve(ws, z);
+    }
     float tol = cfg.lowestLevelTolerance;
+    const std::vector<int32_t>* below = &h.levels[L - 2].h_ranks;
     for (int level = L - 2; level > 0; --level) {
         HLevel& lv = h.levels[level - 1];
         const int T = s.numTiles[level - 1];
@@ -954,9 +1023,9 @@ static void compress_upper_levels(Context& c, const hmx_config& cfg, const Struc
             continue;
         }
         tol *= 2;
-        int maxRank = cfg.maxRankPolicy == HMX_MAXRANK_REF ? (int)maxRanks[L - level - 2] : m;
-        maxRank = std::min(maxRank, m);
-        compress_level(c, cfg, src, b, s.tileIndices[level - 1], level, L, tol, maxRank, lv);
+        compress_level(c, cfg, src, b, s.tileIndices[level - 1], level, L, tol, level_cap(level),
+                       first_sweep_blocks(*below, cfg.araR), ws, lv);
+        below = &lv.h_ranks;
     }
 }
 
@@ -1138,6 +1207,11 @@ void build_hmatrix_direct_batch(Context& c, const std::vector<const hmx_config*>
         HMX_LAUNCHED();
     }
     // upper levels: the reference's schedule per instance, grouped across instances
+    std::vector<int32_t> leafRanks;
+    for (size_t i = 0; i < N; ++i) {
+        const auto& r = hs[i]->levels[ss[i]->numLevels - 2].h_ranks;
+        leafRanks.insert(leafRanks.end(), r.begin(), r.end());
+    }
     struct Job {
         int inst, level;
     };
@@ -1171,6 +1245,20 @@ void build_hmatrix_direct_batch(Context& c, const std::vector<const hmx_config*>
             groups[g].second.push_back(Job{(int)i, level});
         }
     }
+    AraScratch ws;
+    {
+        AraSizes z;
+        for (auto& grp : groups) {
+            int T = 0, cap = 0;
+            for (const Job& j : grp.second) {
+                const HLevel& lv = hs[j.inst]->levels[j.level - 1];
+                T += lv.numTiles;
+                cap = std::max(cap, lv.maxRank);
+            }
+            z.add(T, grp.first.first, cap, true);
+        }
+        ara_reserve(ws, z);
+    }
     for (auto& grp : groups) {
         const int m = grp.first.first, d = grp.first.second;
         const auto tstart = std::chrono::steady_clock::now();
@@ -1187,7 +1275,8 @@ void build_hmatrix_direct_batch(Context& c, const std::vector<const hmx_config*>
         src.P = P.p;
         src.d = d;
         LrOut r;
-        compress_batch(c, cfgs[0]->araR, cfgs[0]->seed, src, b, m, tp, {}, r);
+        compress_batch(c, cfgs[0]->araR, cfgs[0]->seed, src, b, m, tp, {}, first_sweep_blocks(leafRanks, cfgs[0]->araR),
+                       ws, r);
         size_t t0 = 0;
         for (const Job& j : grp.second) {
             HLevel& lv = hs[j.inst]->levels[j.level - 1];
@@ -1218,8 +1307,8 @@ void build_hmatrix_direct_batch(Context& c, const std::vector<const hmx_config*>
         if (log_on()) {
             const double ms =
                 std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - tstart).count();
-            std::fprintf(stderr, "[hmx] batch group m=%d d=%d: %zu tiles from %zu levels, %d ARA blocks, %.1f ms\n", m,
-                         d, tp.size(), grp.second.size(), r.iters, ms);
+            std::fprintf(stderr, "[hmx] batch group m=%d d=%d: %zu tiles from %zu levels, %d ARA blocks in %d sweeps, %.1f ms\n",
+                         m, d, tp.size(), grp.second.size(), r.iters, r.sweeps, ms);
             std::fflush(stderr);
         }
     }
