@@ -108,6 +108,17 @@ __device__ inline int xcd_local(int bid, int n) {
     return x * q + min(x, r) + k;
 }
 
+// Workgroup numbering in chunks: the hardware deals workgroup b to XCD b % 8;
+// this remap hands XCD x the logical chunks x, x+8, x+16, ... of C
+// consecutive workgroups, so neighbouring work (sharing right-hand-side
+// rows) meets in one L2 while every XCD still draws from the whole range.
+// A bijection on [0, 8*C*k); the grid is padded to such a size and logical
+// numbers past the real count exit.
+__device__ inline int xcd_chunked(int bid, int C) {
+    const int x = bid & 7, slot = bid >> 3;
+    return ((slot / C) * 8 + x) * C + slot % C;
+}
+
 // order-preserving map of a double onto uint64 (radix-sort key twiddle)
 __device__ inline uint64_t dkey(double v) {
     uint64_t u = (uint64_t)__double_as_longlong(v);

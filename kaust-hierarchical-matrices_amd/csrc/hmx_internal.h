@@ -174,7 +174,7 @@ struct MfItem {
     const void* v;
     int32_t xoff, rows, ld, ncols;
     int64_t out;   // partial slot of the item's first column (tiles split into several row chunks)
-    int64_t zcol;  // padded column of the item's first column in the group's z
+    int64_t zidx;  // the item's first column in the group's column order (index into zmap)
     int32_t nch;   // row chunks per column of this tile
     int32_t direct;  // 1: whole columns, the item stores z itself; 0: partial sums for the reduce
 };
@@ -197,6 +197,7 @@ struct MfGroup {
     DBuf<float> colScale;          // per padded column (f16 group: su*sv), else empty
     DBuf<MfItem> items;
     int32_t numItems = 0;
+    DBuf<int32_t> zmap;            // group column (V order) -> padded z column
     int64_t numRed = 0;            // columns summed from row-chunk partials
     DBuf<int64_t> redCol;          // their padded column
     DBuf<int64_t> colPart;         // their first partial slot
@@ -226,7 +227,7 @@ struct MfPlan {
     DBuf<MfNear> near;
     int64_t numNear = 0;
     DBuf<_Float16> X16;            // converted right-hand sides (mixed)
-    DBuf<float> xscale;
+    DBuf<float> xscale, xpart;     // per right-hand-side scale and its per-slice partial maxima
     size_t x16n = 0;
     // selection statistics (mixed)
     double frobH2 = 0, loEnergy = 0, target = 0;

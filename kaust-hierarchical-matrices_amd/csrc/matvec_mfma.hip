@@ -19,6 +19,7 @@
 // blocks) in FP32 on the FP32 matrix cores.
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <numeric>
 
 #include "hmx_device.h"
@@ -30,6 +31,7 @@ namespace hmx {
 constexpr int kMfRows = 1024;  // rows per phase-1 item
 constexpr int kMfCols = 64;    // columns per phase-1 item
 constexpr int kPad = 16;       // rank padding of the row-major U copies
+constexpr int kXSlices = 64;   // row slices of the right-hand-side max-abs scan
 
 template <class T, int N>
 __device__ inline void vload(const T* __restrict__ p, T (&out)[N]) {
@@ -72,18 +74,21 @@ __device__ inline size_t zt_index(int64_t c, int s) {
 
 // ---- phase 1 ---------------------------------------------------------------
 template <class T, int CG, int NB>
-__global__ void __launch_bounds__(256) k_mf_phase1(const MfItem* __restrict__ items, int nItems,
+__global__ void __launch_bounds__(256) k_mf_phase1(const MfItem* __restrict__ items, int nItems, int chunk,
                                                    const T* __restrict__ X, int ldx,
                                                    typename Mfma<T>::acc_t* __restrict__ zpart,
-                                                   T* __restrict__ Zt, int64_t ldz,
+                                                   T* __restrict__ Zt, const int32_t* __restrict__ zmap,
                                                    const float* __restrict__ colScale,
                                                    const float* __restrict__ rhsScale) {
     using M = Mfma<T>;
     constexpr int KL = M::KL, VL = 4 * KL, NR = 16 * NB;
     const int lane = threadIdx.x & 63;
-    // items differ in cost (1024-row chunks of upper levels, 64-row leaf
-    // tiles): keep the hardware's round-robin spread over the XCDs
-    const int wid = blockIdx.x * 4 + (threadIdx.x >> 6);
+    // Items are sorted by the right-hand-side rows they read.  Chunks of
+    // consecutive workgroups share an XCD (and its L2), the chunks spread
+    // over all XCDs: items differ in cost (1024-row chunks of upper levels,
+    // 64-row leaf tiles), so one contiguous range per XCD would not balance.
+    const int wg = chunk > 0 ? xcd_chunked(blockIdx.x, chunk) : (int)blockIdx.x;
+    const int wid = wg * 4 + (threadIdx.x >> 6);
     if (wid >= nItems) return;
     const MfItem it = items[wid];
     const T* __restrict__ V = (const T*)it.v;
@@ -145,7 +150,7 @@ __global__ void __launch_bounds__(256) k_mf_phase1(const MfItem* __restrict__ it
             for (int v = 0; v < 4; ++v) {
                 const int col = 16 * g + M::out_row(lg, v);
                 if (col < it.ncols) {
-                    const int64_t zc = it.zcol + col;
+                    const int64_t zc = zmap[it.zidx + col];
                     const ACC cs = colScale ? (ACC)colScale[zc] : (ACC)1;
 #pragma unroll
                     for (int nb = 0; nb < NB; ++nb)
@@ -340,18 +345,31 @@ __global__ void __launch_bounds__(256) k_mf_phase2(const int32_t* __restrict__ p
 }
 
 // ---- right-hand-side conversion (mixed) ---------------------------------------
-__global__ void __launch_bounds__(256) k_colmaxabs(int n, const float* __restrict__ X, float* __restrict__ sx) {
+// max |X[s][i]| over slice p of the rows: part[s*P + p], grid (P, right-hand sides)
+__global__ void __launch_bounds__(256) k_colmaxabs_part(int n, const float* __restrict__ X, float* __restrict__ part) {
     __shared__ float red[256];
-    const float* x = X + (size_t)blockIdx.x * n;
+    const int P = gridDim.x;
+    const int64_t per = ((int64_t)n + P - 1) / P;
+    const int64_t i0 = (int64_t)blockIdx.x * per, i1 = min((int64_t)n, i0 + per);
+    const float* x = X + (size_t)blockIdx.y * n;
     float m = 0.f;
-    for (int i = threadIdx.x; i < n; i += 256) m = fmaxf(m, fabsf(x[i]));
+    for (int64_t i = i0 + threadIdx.x; i < i1; i += 256) m = fmaxf(m, fabsf(x[i]));
     red[threadIdx.x] = m;
     __syncthreads();
     for (int w = 128; w > 0; w >>= 1) {
         if (threadIdx.x < (unsigned)w) red[threadIdx.x] = fmaxf(red[threadIdx.x], red[threadIdx.x + w]);
         __syncthreads();
     }
-    if (threadIdx.x == 0) sx[blockIdx.x] = red[0] > 0.f ? red[0] : 1.f;
+    if (threadIdx.x == 0) part[(size_t)blockIdx.y * P + blockIdx.x] = red[0];
+}
+
+// sx[s] = max over the P slices (1 for an all-zero right-hand side); one wave per s
+__global__ void __launch_bounds__(64) k_colmaxabs_fin(int P, const float* __restrict__ part, float* __restrict__ sx) {
+    float m = 0.f;
+    for (int p = threadIdx.x; p < P; p += 64) m = fmaxf(m, part[(size_t)blockIdx.x * P + p]);
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) m = fmaxf(m, __shfl_xor(m, off, 64));
+    if (threadIdx.x == 0) sx[blockIdx.x] = m > 0.f ? m : 1.f;
 }
 
 __global__ void __launch_bounds__(256) k_to_f16(int64_t total, int n, const float* __restrict__ X,
@@ -513,20 +531,74 @@ void build_group(Context& c, HMatrix& h, MfGroup& G, int type, const std::vector
     const int S = (int)h.S;
     const size_t es = tsize(type);
     G.type = type;
+    // Copied factors are laid out by (tile dimension, column cluster): the
+    // columns of all tiles that read the same right-hand-side rows are
+    // contiguous in V, so a phase-1 item of 64 columns spans tiles and one
+    // load of those rows serves all of them.  V used in place (FP64) keeps
+    // the build's per-tile layout and per-tile items.
+    std::vector<int> ord(tiles.size());
+    std::iota(ord.begin(), ord.end(), 0);
+    if (!vdirect)
+        std::stable_sort(ord.begin(), ord.end(), [&](int x, int y) {
+            const TileRef &A = tiles[x].ref, &B = tiles[y].ref;
+            return A.m != B.m ? A.m > B.m : A.inCl < B.inCl;
+        });
     std::vector<MfItem> items;
     std::vector<int64_t> redCol, colPart;
-    std::vector<int32_t> colNpart;
+    std::vector<int32_t> colNpart, zmap;
     std::vector<float> colScale;
     std::vector<std::vector<MfEntry>> leafE(S);
     std::vector<CvCol> vcols, ucols;
     std::vector<TrTile> trt;
     int64_t zcol = 0, slot = 0, vElems = 0, uElems = 0;
     uint64_t fbytes = 0;
-    for (const GroupTile& gt : tiles) {
+    // phase-1 items over group columns [k0, k1), all of tile dimension m and
+    // column cluster inCl; V at vptr (in place) or element offset vOff (copy)
+    auto emit_items = [&](int64_t k0, int64_t k1, int m, uint32_t inCl, const char* vptr, int64_t vOff) {
+        const int nch = (m + kMfRows - 1) / kMfRows;
+        const int rows = std::min(m, kMfRows);
+        const bool partial = nch > 1 || !h.planDirectZ;
+        const int64_t cnt = k1 - k0;
+        for (int64_t c0 = 0; c0 < cnt; c0 += kMfCols)
+            for (int q = 0; q < nch; ++q) {
+                MfItem it;
+                const size_t e = (size_t)c0 * m + (size_t)q * kMfRows;
+                it.v = vptr ? (const void*)(vptr + e * es) : (const void*)(uintptr_t)(((size_t)vOff + e) * es);
+                it.xoff = (int32_t)(inCl * m + q * kMfRows);
+                it.rows = rows;
+                it.ld = m;
+                it.ncols = (int32_t)std::min<int64_t>(kMfCols, cnt - c0);
+                it.out = partial ? slot + c0 * nch + q : -1;
+                it.zidx = k0 + c0;
+                it.nch = nch;
+                it.direct = partial ? 0 : 1;
+                items.push_back(it);
+            }
+        if (partial) {
+            for (int64_t k = k0; k < k1; ++k) {
+                redCol.push_back(zmap[k]);
+                colPart.push_back(slot + (k - k0) * nch);
+                colNpart.push_back(nch);
+            }
+            slot += cnt * nch;
+        }
+    };
+    int64_t runK0 = 0, runV0 = 0;  // open run of copied columns sharing (m, inCl)
+    int runM = 0;
+    uint32_t runCl = 0;
+    for (int oi : ord) {
+        const GroupTile& gt = tiles[oi];
         const TileRef& R = gt.ref;
         const int cnt = (int)gt.cols.size();
         if (!cnt) continue;
         const int m = R.m;
+        if (!vdirect && (m != runM || R.inCl != runCl)) {
+            if ((int64_t)zmap.size() > runK0) emit_items(runK0, (int64_t)zmap.size(), runM, runCl, nullptr, runV0);
+            runK0 = (int64_t)zmap.size();
+            runV0 = vElems;
+            runM = m;
+            runCl = R.inCl;
+        }
         const int rpad = (cnt + kPad - 1) / kPad * kPad;
         const double* Usrc = R.lv->U.p + (size_t)R.lv->h_offsets[R.t] * m;
         const double* Vsrc = R.lv->V.p + (size_t)R.lv->h_offsets[R.t] * m;
@@ -535,8 +607,8 @@ void build_group(Context& c, HMatrix& h, MfGroup& G, int type, const std::vector
             const int j = gt.cols[q];
             float fu = 1.f, fv = 1.f;
             if (su) {
-                fu = (*su)[(*colBase)[&gt - &tiles[0]] + j];
-                fv = (*sv)[(*colBase)[&gt - &tiles[0]] + j];
+                fu = (*su)[(*colBase)[oi] + j];
+                fv = (*sv)[(*colBase)[oi] + j];
             }
             ucols.push_back(CvCol{Usrc + (size_t)j * m, 0, m, fu});
             if (!vdirect) vcols.push_back(CvCol{Vsrc + (size_t)j * m, vElems + (int64_t)q * m, m, fv});
@@ -546,32 +618,9 @@ void build_group(Context& c, HMatrix& h, MfGroup& G, int type, const std::vector
         if (!udirect)
             for (int r0 = 0; r0 < m; r0 += 32)
                 for (int c0 = 0; c0 < rpad; c0 += 32) trt.push_back(TrTile{cd0, uElems, m, cnt, rpad, r0, c0, 0});
-        // phase-1 items
-        const int nch = (m + kMfRows - 1) / kMfRows;
-        const int rows = std::min(m, kMfRows);
-        const bool partial = nch > 1 || !h.planDirectZ;
-        const char* vbase = vdirect ? (const char*)Vsrc : nullptr;  // patched after allocation otherwise
-        for (int c0 = 0; c0 < cnt; c0 += kMfCols)
-            for (int q = 0; q < nch; ++q) {
-                MfItem it;
-                it.v = vdirect ? (const void*)(vbase + ((size_t)c0 * m + (size_t)q * kMfRows) * es)
-                               : (const void*)(uintptr_t)(((size_t)vElems + (size_t)c0 * m + (size_t)q * kMfRows) * es);
-                it.xoff = (int32_t)(R.inCl * m + q * kMfRows);
-                it.rows = rows;
-                it.ld = m;
-                it.ncols = std::min(kMfCols, cnt - c0);
-                it.out = partial ? slot + (int64_t)c0 * nch + q : -1;
-                it.zcol = zcol + c0;
-                it.nch = nch;
-                it.direct = partial ? 0 : 1;
-                items.push_back(it);
-            }
-        if (partial)
-            for (int q = 0; q < cnt; ++q) {
-                redCol.push_back(zcol + q);
-                colPart.push_back(slot + (int64_t)q * nch);
-                colNpart.push_back(nch);
-            }
+        const int64_t k0 = (int64_t)zmap.size();
+        for (int q = 0; q < cnt; ++q) zmap.push_back((int32_t)(zcol + q));
+        if (vdirect) emit_items(k0, k0 + cnt, m, R.inCl, (const char*)Vsrc, 0);
         // phase-2 entries
         const int bus = m / b;
         for (int q = 0; q < bus; ++q) {
@@ -582,15 +631,16 @@ void build_group(Context& c, HMatrix& h, MfGroup& G, int type, const std::vector
                 leafE[R.outCl * bus + q].push_back(MfEntry{
                     (const void*)(uintptr_t)(((size_t)uElems + (size_t)q * b * rpad) * es), rpad, (int32_t)zcol, cnt, 0});
         }
-        if (partial) slot += (int64_t)cnt * nch;
         zcol += rpad;
+        HMX_CHECK(zcol < ((int64_t)1 << 31), HMX_EUNSUPPORTED, "matrix-core plan: more than 2^31 columns in one group");
         vElems += (int64_t)cnt * m;
         uElems += (int64_t)m * rpad;
         fbytes += 2ull * (uint64_t)cnt * m * es;
     }
+    if (!vdirect && (int64_t)zmap.size() > runK0) emit_items(runK0, (int64_t)zmap.size(), runM, runCl, nullptr, runV0);
     // items that read the same right-hand-side block (same tile dimension,
-    // column cluster and row chunk) next to each other: with XCD-local block
-    // numbering they run on one XCD and share that block in its L2
+    // column cluster and row chunk) next to each other (already so for the
+    // copied layout)
     if (h.planLocality)
         std::stable_sort(items.begin(), items.end(), [](const MfItem& a, const MfItem& b2) {
             return a.ld != b2.ld ? a.ld > b2.ld : a.xoff < b2.xoff;
@@ -615,6 +665,7 @@ void build_group(Context& c, HMatrix& h, MfGroup& G, int type, const std::vector
     leafPtr[S] = (int32_t)ents.size();
     G.numItems = (int32_t)items.size();
     upload(c, G.items, items);
+    upload(c, G.zmap, zmap);
     G.numRed = (int64_t)redCol.size();
     upload(c, G.redCol, redCol);
     upload(c, G.colPart, colPart);
@@ -740,12 +791,25 @@ void ensure_ws(Context& c, MfGroup& G, int NR, size_t accBytes) {
     G.wsNR = NR;
 }
 
+// workgroups per XCD chunk in phase 1 (0: hardware order)
+int phase1_chunk() {
+    static const int v = [] {
+        const char* e = std::getenv("HMX_P1_CHUNK");
+        return e ? std::max(0, std::atoi(e)) : 16;
+    }();
+    return v;
+}
+
 template <class T, int CG, int NB>
 void phase1(Context& c, MfGroup& G, const T* X, int ldx, const float* rhsScale) {
     if (G.numItems == 0) return;
     const float* cs = G.colScale.n ? G.colScale.p : nullptr;
-    k_mf_phase1<T, CG, NB><<<(unsigned)((G.numItems + 3) / 4), 256, 0, c.stream>>>(
-        G.items.p, G.numItems, X, ldx, (typename Mfma<T>::acc_t*)G.zpart.p, (T*)G.zt.p, G.numColsPad, cs, rhsScale);
+    const int chunk = phase1_chunk();
+    unsigned blocks = (unsigned)((G.numItems + 3) / 4);
+    if (chunk > 0) blocks = (blocks + 8 * chunk - 1) / (8 * chunk) * (8 * chunk);
+    k_mf_phase1<T, CG, NB><<<blocks, 256, 0, c.stream>>>(G.items.p, G.numItems, chunk, X, ldx,
+                                                         (typename Mfma<T>::acc_t*)G.zpart.p, (T*)G.zt.p, G.zmap.p,
+                                                         cs, rhsScale);
     HMX_LAUNCHED();
 }
 
@@ -791,7 +855,9 @@ void run_mixed(Context& c, HMatrix& h, const float* X, float* Y, Events& ev) {
     const int n = (int)h.n, NR = 16 * NB;
     ev.rec(c, 0);
     // right-hand sides to FP16 with one scale per column
-    k_colmaxabs<<<NR, 256, 0, c.stream>>>(n, X, P.xscale.p);
+    k_colmaxabs_part<<<dim3(kXSlices, NR), 256, 0, c.stream>>>(n, X, P.xpart.p);
+    HMX_LAUNCHED();
+    k_colmaxabs_fin<<<NR, 64, 0, c.stream>>>(kXSlices, P.xpart.p, P.xscale.p);
     HMX_LAUNCHED();
     const int64_t tot = (int64_t)n * NR;
     k_to_f16<<<(unsigned)((tot + 255) / 256), 256, 0, c.stream>>>(tot, n, X, P.xscale.p, P.X16.p);
@@ -955,6 +1021,7 @@ void hmatvec_mixed(Context& c, HMatrix& h, uint32_t k, const float* d_in, float*
     if (P.x16n < (size_t)h.n * NR) {
         P.X16.alloc((size_t)h.n * NR);
         P.xscale.alloc(NR);
+        P.xpart.alloc((size_t)NR * kXSlices);
         P.x16n = (size_t)h.n * NR;
     }
     Events ev;
