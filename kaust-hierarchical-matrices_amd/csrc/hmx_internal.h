@@ -226,7 +226,8 @@ struct MfPlan {
     DBuf<int32_t> nearPtr;         // [S+1] CSR over row leaves
     DBuf<MfNear> near;
     int64_t numNear = 0;
-    DBuf<_Float16> X16;            // converted right-hand sides (mixed)
+    DBuf<unsigned char> Xb;        // right-hand sides in the blocked layout (plan's dense type)
+    DBuf<_Float16> X16;            // the same in FP16, scaled per right-hand side (mixed)
     DBuf<float> xscale, xpart;     // per right-hand-side scale and its per-slice partial maxima
     size_t x16n = 0;
     // selection statistics (mixed)

@@ -72,10 +72,19 @@ __device__ inline size_t zt_index(int64_t c, int s) {
     return (size_t)(c / VL) * (NR * VL) + (size_t)s * VL + (size_t)(c % VL);
 }
 
+// Right-hand sides in blocks of 2^sh rows (the leaf size): block k holds the
+// rows of leaf k for all NR right-hand sides, [s][row % 2^sh].  A leaf's (or
+// a tile's) slice of X is then a few contiguous NR*2^sh-element blocks
+// instead of NR strips n elements apart, one per right-hand side.
+template <int NR>
+__device__ inline size_t xb_index(int64_t row, int s, int sh) {
+    return ((((size_t)(row >> sh)) * NR + (size_t)s) << sh) + (size_t)(row & ((1 << sh) - 1));
+}
+
 // ---- phase 1 ---------------------------------------------------------------
 template <class T, int CG, int NB>
 __global__ void __launch_bounds__(256) k_mf_phase1(const MfItem* __restrict__ items, int nItems, int chunk,
-                                                   const T* __restrict__ X, int ldx,
+                                                   const T* __restrict__ Xb, int sh,
                                                    typename Mfma<T>::acc_t* __restrict__ zpart,
                                                    T* __restrict__ Zt, const int32_t* __restrict__ zmap,
                                                    const float* __restrict__ colScale,
@@ -92,7 +101,6 @@ __global__ void __launch_bounds__(256) k_mf_phase1(const MfItem* __restrict__ it
     if (wid >= nItems) return;
     const MfItem it = items[wid];
     const T* __restrict__ V = (const T*)it.v;
-    const T* __restrict__ Xs = X + it.xoff;
     const int lg = lane >> 4, lc = lane & 15;
     typename M::accv acc[CG][NB];
 #pragma unroll
@@ -113,7 +121,7 @@ __global__ void __launch_bounds__(256) k_mf_phase1(const MfItem* __restrict__ it
         }
 #pragma unroll
         for (int nb = 0; nb < NB; ++nb) {
-            if (rok) vload(Xs + (size_t)(16 * nb + lc) * ldx + rr, B[nb]);
+            if (rok) vload(Xb + xb_index<NR>((int64_t)it.xoff + rr, 16 * nb + lc, sh), B[nb]);
             else vzero(B[nb]);
         }
     };
@@ -217,7 +225,7 @@ __global__ void __launch_bounds__(256) k_mf_phase2(const int32_t* __restrict__ p
                                                    const int32_t* __restrict__ ptrB, const MfEntry* __restrict__ entB,
                                                    const TB* __restrict__ ZB, int64_t ldzB,
                                                    const TD* __restrict__ Drm, const int32_t* __restrict__ nearPtr,
-                                                   const MfNear* __restrict__ nearB, const TD* __restrict__ X, int ldx,
+                                                   const MfNear* __restrict__ nearB, const TD* __restrict__ Xb,
                                                    TO* __restrict__ Y, int ldy) {
     using MA = Mfma<TA>;
     using MB = Mfma<TB>;
@@ -252,7 +260,7 @@ __global__ void __launch_bounds__(256) k_mf_phase2(const int32_t* __restrict__ p
             const int cl = q < n0 ? leaf : nearB[q].colLeaf;
             vload_nt(Dq + row * BL + cc, A);
 #pragma unroll
-            for (int nb = 0; nb < NB; ++nb) vload(X + (size_t)(16 * nb + lc) * ldx + (size_t)cl * BL + cc, B[nb]);
+            for (int nb = 0; nb < NB; ++nb) vload(Xb + ((size_t)cl * NR + 16 * nb + lc) * BL + cc, B[nb]);
         };
         dload(0, a, bx);
         for (int idx = 0; idx < total; ++idx) {
@@ -372,11 +380,17 @@ __global__ void __launch_bounds__(64) k_colmaxabs_fin(int P, const float* __rest
     if (threadIdx.x == 0) sx[blockIdx.x] = m > 0.f ? m : 1.f;
 }
 
-__global__ void __launch_bounds__(256) k_to_f16(int64_t total, int n, const float* __restrict__ X,
-                                                const float* __restrict__ sx, _Float16* __restrict__ X16) {
+// X ([s][n], one right-hand side per row of n) into the blocked layout,
+// divided by the per-right-hand-side scale sx when one is given
+template <class TI, class TO>
+__global__ void __launch_bounds__(256) k_x_blocked(int64_t total, int n, int NR, int sh, const TI* __restrict__ X,
+                                                   const float* __restrict__ sx, TO* __restrict__ Xb) {
     const int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x;
     if (e >= total) return;
-    X16[e] = (_Float16)(X[e] / sx[e / n]);
+    const int s = (int)(e / n), row = (int)(e % n);
+    const size_t o = ((((size_t)(row >> sh)) * NR + (size_t)s) << sh) + (size_t)(row & ((1 << sh) - 1));
+    if (sx) Xb[o] = (TO)(X[e] / sx[s]);
+    else Xb[o] = (TO)X[e];
 }
 
 // ---- plan construction kernels -----------------------------------------------
@@ -801,13 +815,13 @@ int phase1_chunk() {
 }
 
 template <class T, int CG, int NB>
-void phase1(Context& c, MfGroup& G, const T* X, int ldx, const float* rhsScale) {
+void phase1(Context& c, MfGroup& G, const T* Xb, int sh, const float* rhsScale) {
     if (G.numItems == 0) return;
     const float* cs = G.colScale.n ? G.colScale.p : nullptr;
     const int chunk = phase1_chunk();
     unsigned blocks = (unsigned)((G.numItems + 3) / 4);
     if (chunk > 0) blocks = (blocks + 8 * chunk - 1) / (8 * chunk) * (8 * chunk);
-    k_mf_phase1<T, CG, NB><<<blocks, 256, 0, c.stream>>>(G.items.p, G.numItems, chunk, X, ldx,
+    k_mf_phase1<T, CG, NB><<<blocks, 256, 0, c.stream>>>(G.items.p, G.numItems, chunk, Xb, sh,
                                                          (typename Mfma<T>::acc_t*)G.zpart.p, (T*)G.zt.p, G.zmap.p,
                                                          cs, rhsScale);
     HMX_LAUNCHED();
@@ -829,20 +843,33 @@ struct Events {
     }
 };
 
+int leaf_shift(const HMatrix& h) { return h.b == 64 ? 6 : 5; }
+
+// X ([s][n]) -> blocked copy, optionally divided by a per-right-hand-side scale
+template <class TI, class TO>
+void to_blocked(Context& c, const HMatrix& h, int NR, const TI* X, const float* sx, TO* Xb) {
+    const int64_t tot = (int64_t)h.n * NR;
+    k_x_blocked<TI, TO><<<(unsigned)((tot + 255) / 256), 256, 0, c.stream>>>(tot, (int)h.n, NR, leaf_shift(h), X, sx,
+                                                                             Xb);
+    HMX_LAUNCHED();
+}
+
 template <int NB>
 void run_fp64(Context& c, HMatrix& h, const double* X, double* Y, Events& ev) {
     MfPlan& P = h.mf64;
     MfGroup& G = P.g[0];
     const int ldx = (int)h.n;
+    const double* Xb = (const double*)P.Xb.p;
     ev.rec(c, 0);
-    phase1<double, 4, NB>(c, G, X, ldx, nullptr);  // 4 column groups = the 64 columns of an item
+    to_blocked<double, double>(c, h, 16 * NB, X, nullptr, (double*)P.Xb.p);
+    phase1<double, 4, NB>(c, G, Xb, leaf_shift(h), nullptr);  // 4 column groups = the 64 columns of an item
     ev.rec(c, 1);
     reduce_group<double, double>(c, G, 16 * NB, nullptr);
     ev.rec(c, 2);
     auto k2 = h.b == 64 ? k_mf_phase2<64, true, double, double, double, NB, double>
                         : k_mf_phase2<32, true, double, double, double, NB, double>;
     k2<<<h.S, 256, 0, c.stream>>>(G.leafPtr.p, G.entries.p, (const double*)G.zt.p, G.numColsPad, nullptr, nullptr,
-                                  nullptr, 0, (const double*)P.Drm.p, P.nearPtr.p, P.near.p, X, ldx, Y, ldx);
+                                  nullptr, 0, (const double*)P.Drm.p, P.nearPtr.p, P.near.p, Xb, Y, ldx);
     HMX_LAUNCHED();
     ev.rec(c, 3);
 }
@@ -859,11 +886,11 @@ void run_mixed(Context& c, HMatrix& h, const float* X, float* Y, Events& ev) {
     HMX_LAUNCHED();
     k_colmaxabs_fin<<<NR, 64, 0, c.stream>>>(kXSlices, P.xpart.p, P.xscale.p);
     HMX_LAUNCHED();
-    const int64_t tot = (int64_t)n * NR;
-    k_to_f16<<<(unsigned)((tot + 255) / 256), 256, 0, c.stream>>>(tot, n, X, P.xscale.p, P.X16.p);
-    HMX_LAUNCHED();
-    phase1<float, 4, NB>(c, Ghi, X, n, nullptr);
-    phase1<_Float16, 4, NB>(c, Glo, P.X16.p, n, P.xscale.p);
+    const float* Xb = (const float*)P.Xb.p;
+    to_blocked<float, _Float16>(c, h, NR, X, P.xscale.p, P.X16.p);
+    to_blocked<float, float>(c, h, NR, X, nullptr, (float*)P.Xb.p);
+    phase1<float, 4, NB>(c, Ghi, Xb, leaf_shift(h), nullptr);
+    phase1<_Float16, 4, NB>(c, Glo, P.X16.p, leaf_shift(h), P.xscale.p);
     ev.rec(c, 1);
     reduce_group<float, float>(c, Ghi, NR, nullptr);
     reduce_group<float, _Float16>(c, Glo, NR, P.xscale.p);
@@ -872,7 +899,7 @@ void run_mixed(Context& c, HMatrix& h, const float* X, float* Y, Events& ev) {
                         : k_mf_phase2<32, false, float, float, _Float16, NB, float>;
     k2<<<h.S, 256, 0, c.stream>>>(Ghi.leafPtr.p, Ghi.entries.p, (const float*)Ghi.zt.p, Ghi.numColsPad,
                                   Glo.leafPtr.p, Glo.entries.p, (const _Float16*)Glo.zt.p, Glo.numColsPad,
-                                  (const float*)P.Drm.p, P.nearPtr.p, P.near.p, X, n, Y, n);
+                                  (const float*)P.Drm.p, P.nearPtr.p, P.near.p, Xb, Y, n);
     HMX_LAUNCHED();
     ev.rec(c, 3);
 }
@@ -898,6 +925,7 @@ void hmatvec_multi(Context& c, HMatrix& h, uint32_t k, const double* d_in, doubl
     MfPlan& P = h.mf64;
     const int NB = (k % 64 == 0) ? 4 : 1;
     ensure_ws(c, P.g[0], 16 * NB, sizeof(double));
+    if (P.Xb.n < (size_t)h.n * 16 * NB * sizeof(double)) P.Xb.alloc((size_t)h.n * 16 * NB * sizeof(double));
     Events ev;
     if (ms) ev.create();
     const int reps = ms ? std::max(1, iters) : 1;
@@ -1020,6 +1048,7 @@ void hmatvec_mixed(Context& c, HMatrix& h, uint32_t k, const float* d_in, float*
     ensure_ws(c, P.g[1], NR, sizeof(float));
     if (P.x16n < (size_t)h.n * NR) {
         P.X16.alloc((size_t)h.n * NR);
+        P.Xb.alloc((size_t)h.n * NR * sizeof(float));
         P.xscale.alloc(NR);
         P.xpart.alloc((size_t)NR * kXSlices);
         P.x16n = (size_t)h.n * NR;
