@@ -204,6 +204,10 @@ struct MfGroup {
     DBuf<int32_t> colNpart;        // their partial count
     DBuf<int32_t> leafPtr;         // [S+1]
     DBuf<MfEntry> entries;
+    bool grouped = false;          // 64-row leaves in groups of 4: entries hold only tiles of at most two
+                                   // leaves, tiles spanning a group are in grpPtr/grpEntries
+    DBuf<int32_t> grpPtr;          // [S/4+1]
+    DBuf<MfEntry> grpEntries;      // U at the group's first row
     uint64_t factorBytes = 0;      // V + U bytes of this group (unpadded entries)
     DBuf<unsigned char> zpart, zt; // workspaces
     size_t wsNR = 0;

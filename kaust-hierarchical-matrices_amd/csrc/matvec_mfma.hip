@@ -352,6 +352,167 @@ __global__ void __launch_bounds__(256) k_mf_phase2(const int32_t* __restrict__ p
     }
 }
 
+// ---- phase 2, grouped (64-row leaves) ------------------------------------------
+// One wave's 64 rows (one leaf) times a list of low-rank entries:
+// acc[rg][nb] += U[16rg.., cols] Z[cols, rhs 16nb..].  The operands of the
+// next 16*KL-column step (possibly of the next entry) load while the current
+// step's MFMAs run.  rowOff: the wave's first row inside the entries' U.
+template <bool UCM, class T, int NB>
+__device__ inline void lr_rows64(const MfEntry* __restrict__ ents, int e0, int e1, const T* __restrict__ Z,
+                                 int rowOff, int lg, int lc, typename Mfma<T>::accv (&acc)[4][NB]) {
+    using M = Mfma<T>;
+    constexpr int KL = M::KL, VL = 4 * KL, SS = 16 * KL, NR = 16 * NB;
+    if (e0 >= e1) return;
+    T a[4][VL], bz[NB][VL], an[4][VL], bn[NB][VL];
+    auto load = [&](const MfEntry& en, int c0, T (&A)[4][VL], T (&B)[NB][VL]) {
+        const int cc = c0 + VL * lg;
+        if (cc < en.rpad) {
+#pragma unroll
+            for (int rg = 0; rg < 4; ++rg) {
+                const int row = rowOff + 16 * rg + lc;
+                if (UCM) {  // column-major U in place: one element per column, zero past the rank
+                    const T* U = (const T*)en.u + row;
+#pragma unroll
+                    for (int q = 0; q < VL; ++q)
+                        A[rg][q] = cc + q < en.rank ? __builtin_nontemporal_load(U + (size_t)(cc + q) * en.ld) : (T)0;
+                } else {
+                    vload_nt((const T*)en.u + (size_t)row * en.rpad + cc, A[rg]);
+                }
+            }
+#pragma unroll
+            for (int nb = 0; nb < NB; ++nb) vload(Z + zt_index<VL, NR>(en.zcol + cc, 16 * nb + lc), B[nb]);
+        } else {
+#pragma unroll
+            for (int rg = 0; rg < 4; ++rg) vzero(A[rg]);
+#pragma unroll
+            for (int nb = 0; nb < NB; ++nb) vzero(B[nb]);
+        }
+    };
+    int e = e0, c0 = 0;
+    MfEntry en = ents[e];
+    load(en, 0, a, bz);
+    while (true) {
+        int ne = e, nc = c0 + SS;
+        MfEntry nen = en;
+        if (nc >= en.rpad) {
+            ne = e + 1;
+            nc = 0;
+            if (ne < e1) nen = ents[ne];
+        }
+        const bool more = ne < e1;
+        if (more) load(nen, nc, an, bn);
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+#pragma unroll
+            for (int rg = 0; rg < 4; ++rg)
+#pragma unroll
+                for (int nb = 0; nb < NB; ++nb)
+                    acc[rg][nb] = M::op(M::make(&a[rg][q * KL]), M::make(&bz[nb][q * KL]), acc[rg][nb]);
+        if (!more) break;
+#pragma unroll
+        for (int rg = 0; rg < 4; ++rg)
+#pragma unroll
+            for (int q = 0; q < VL; ++q) a[rg][q] = an[rg][q];
+#pragma unroll
+        for (int nb = 0; nb < NB; ++nb)
+#pragma unroll
+            for (int q = 0; q < VL; ++q) bz[nb][q] = bn[nb][q];
+        e = ne;
+        c0 = nc;
+        en = nen;
+    }
+}
+
+// One wave's leaf times its dense 64x64 blocks (the diagonal block, then its
+// near-field blocks), software pipelined over 16-column steps.
+template <class TD, int NB>
+__device__ inline void dense_rows64(int leaf, const TD* __restrict__ Drm, const int32_t* __restrict__ nearPtr,
+                                    const MfNear* __restrict__ nearB, const TD* __restrict__ Xb, int lg, int lc,
+                                    typename Mfma<TD>::accv (&acc)[4][NB]) {
+    using MD = Mfma<TD>;
+    constexpr int NR = 16 * NB;
+    const int n0 = nearPtr ? nearPtr[leaf] : 0, n1 = nearPtr ? nearPtr[leaf + 1] : 0;
+    const int total = (n1 - n0 + 1) * 4;
+    TD a[4][4], bx[NB][4], an[4][4], bn[NB][4];
+    auto dload = [&](int idx, TD (&A)[4][4], TD (&B)[NB][4]) {
+        const int q = n0 - 1 + idx / 4;
+        const int cc = (idx % 4) * 16 + 4 * lg;
+        const TD* Dq = q < n0 ? Drm + (size_t)leaf * 64 * 64 : (const TD*)nearB[q].d;
+        const int cl = q < n0 ? leaf : nearB[q].colLeaf;
+#pragma unroll
+        for (int rg = 0; rg < 4; ++rg) vload_nt(Dq + (16 * rg + lc) * 64 + cc, A[rg]);
+#pragma unroll
+        for (int nb = 0; nb < NB; ++nb) vload(Xb + ((size_t)cl * NR + 16 * nb + lc) * 64 + cc, B[nb]);
+    };
+    dload(0, a, bx);
+    for (int idx = 0; idx < total; ++idx) {
+        const bool more = idx + 1 < total;
+        if (more) dload(idx + 1, an, bn);
+#pragma unroll
+        for (int e = 0; e < 4; ++e)
+#pragma unroll
+            for (int rg = 0; rg < 4; ++rg)
+#pragma unroll
+                for (int nb = 0; nb < NB; ++nb) acc[rg][nb] = MD::op(a[rg][e], bx[nb][e], acc[rg][nb]);
+        if (more) {
+#pragma unroll
+            for (int rg = 0; rg < 4; ++rg)
+#pragma unroll
+                for (int e = 0; e < 4; ++e) a[rg][e] = an[rg][e];
+#pragma unroll
+            for (int nb = 0; nb < NB; ++nb)
+#pragma unroll
+                for (int e = 0; e < 4; ++e) bx[nb][e] = bn[nb][e];
+        }
+    }
+}
+
+// One workgroup of 4 waves per group of 4 consecutive 64-row leaves; wave w
+// owns leaf 4g+w -- all its rows, all right-hand sides, every contribution:
+// its dense blocks and the low-rank tiles of at most two leaves (private
+// entries, per leaf), then the tiles spanning the whole group (shared
+// entries, per group).  The four waves walk the shared list in the same
+// order, so a tile's z block comes from HBM once per group and reaches the
+// other waves from the caches.
+template <bool UCM, class TD, class TA, class TB, int NB, class TO>
+__global__ void __launch_bounds__(256) k_mf_phase2g(const int32_t* __restrict__ ptrA, const MfEntry* __restrict__ entA,
+                                                    const int32_t* __restrict__ gptrA,
+                                                    const MfEntry* __restrict__ gentA, const TA* __restrict__ ZA,
+                                                    const int32_t* __restrict__ ptrB, const MfEntry* __restrict__ entB,
+                                                    const int32_t* __restrict__ gptrB,
+                                                    const MfEntry* __restrict__ gentB, const TB* __restrict__ ZB,
+                                                    const TD* __restrict__ Drm, const int32_t* __restrict__ nearPtr,
+                                                    const MfNear* __restrict__ nearB, const TD* __restrict__ Xb,
+                                                    TO* __restrict__ Y, int ldy) {
+    using MA = Mfma<TA>;
+    using accv = typename MA::accv;
+    static_assert(sizeof(typename Mfma<TB>::acc_t) == sizeof(typename MA::acc_t) &&
+                      sizeof(typename Mfma<TD>::acc_t) == sizeof(typename MA::acc_t),
+                  "one accumulator type per launch");
+    const int grp = xcd_local(blockIdx.x, gridDim.x);
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63, lg = lane >> 4, lc = lane & 15;
+    const int leaf = 4 * grp + w;
+    accv acc[4][NB];
+#pragma unroll
+    for (int rg = 0; rg < 4; ++rg)
+#pragma unroll
+        for (int nb = 0; nb < NB; ++nb) acc[rg][nb] = accv{0, 0, 0, 0};
+    dense_rows64<TD, NB>(leaf, Drm, nearPtr, nearB, Xb, lg, lc, acc);
+    lr_rows64<UCM, TA, NB>(entA, ptrA[leaf], ptrA[leaf + 1], ZA, 0, lg, lc, acc);
+    lr_rows64<UCM, TA, NB>(gentA, gptrA[grp], gptrA[grp + 1], ZA, 64 * w, lg, lc, acc);
+    if (ptrB) {
+        lr_rows64<false, TB, NB>(entB, ptrB[leaf], ptrB[leaf + 1], ZB, 0, lg, lc, acc);
+        lr_rows64<false, TB, NB>(gentB, gptrB[grp], gptrB[grp + 1], ZB, 64 * w, lg, lc, acc);
+    }
+#pragma unroll
+    for (int rg = 0; rg < 4; ++rg)
+#pragma unroll
+        for (int nb = 0; nb < NB; ++nb)
+#pragma unroll
+            for (int v = 0; v < 4; ++v)
+                Y[(size_t)(16 * nb + lc) * ldy + (size_t)leaf * 64 + 16 * rg + MA::out_row(lg, v)] = (TO)acc[rg][nb][v];
+}
+
 // ---- right-hand-side conversion (mixed) ---------------------------------------
 // max |X[s][i]| over slice p of the rows: part[s*P + p], grid (P, right-hand sides)
 __global__ void __launch_bounds__(256) k_colmaxabs_part(int n, const float* __restrict__ X, float* __restrict__ part) {
@@ -562,6 +723,10 @@ void build_group(Context& c, HMatrix& h, MfGroup& G, int type, const std::vector
     std::vector<int32_t> colNpart, zmap;
     std::vector<float> colScale;
     std::vector<std::vector<MfEntry>> leafE(S);
+    // 64-row leaves: phase 2 runs one workgroup per 4 leaves, and tiles
+    // spanning all 4 are listed once per group
+    const bool grouped = b == 64 && S >= 4;
+    std::vector<std::vector<MfEntry>> grpE(grouped ? S / 4 : 0);
     std::vector<CvCol> vcols, ucols;
     std::vector<TrTile> trt;
     int64_t zcol = 0, slot = 0, vElems = 0, uElems = 0;
@@ -635,16 +800,18 @@ void build_group(Context& c, HMatrix& h, MfGroup& G, int type, const std::vector
         const int64_t k0 = (int64_t)zmap.size();
         for (int q = 0; q < cnt; ++q) zmap.push_back((int32_t)(zcol + q));
         if (vdirect) emit_items(k0, k0 + cnt, m, R.inCl, (const char*)Vsrc, 0);
-        // phase-2 entries
+        // phase-2 entries: U at the first row of the leaf (or group) inside the
+        // tile -- the column-major U in place, or the row-major copy
         const int bus = m / b;
-        for (int q = 0; q < bus; ++q) {
-            if (udirect)  // the leaf's first row inside the tile's column-major U
-                leafE[R.outCl * bus + q].push_back(
-                    MfEntry{(const void*)(Usrc + (size_t)q * b), rpad, (int32_t)zcol, cnt, m});
-            else
-                leafE[R.outCl * bus + q].push_back(MfEntry{
-                    (const void*)(uintptr_t)(((size_t)uElems + (size_t)q * b * rpad) * es), rpad, (int32_t)zcol, cnt, 0});
-        }
+        auto entry_at = [&](int r0) {
+            return udirect ? MfEntry{(const void*)(Usrc + (size_t)r0), rpad, (int32_t)zcol, cnt, m}
+                           : MfEntry{(const void*)(uintptr_t)(((size_t)uElems + (size_t)r0 * rpad) * es), rpad,
+                                     (int32_t)zcol, cnt, 0};
+        };
+        if (grouped && bus >= 4)
+            for (int q = 0; q < bus / 4; ++q) grpE[(R.outCl * bus) / 4 + q].push_back(entry_at(q * 4 * b));
+        else
+            for (int q = 0; q < bus; ++q) leafE[R.outCl * bus + q].push_back(entry_at(q * b));
         zcol += rpad;
         HMX_CHECK(zcol < ((int64_t)1 << 31), HMX_EUNSUPPORTED, "matrix-core plan: more than 2^31 columns in one group");
         vElems += (int64_t)cnt * m;
@@ -677,6 +844,21 @@ void build_group(Context& c, HMatrix& h, MfGroup& G, int type, const std::vector
         }
     }
     leafPtr[S] = (int32_t)ents.size();
+    G.grouped = grouped;
+    if (grouped) {
+        std::vector<int32_t> gptr(S / 4 + 1, 0);
+        std::vector<MfEntry> gents;
+        for (int g = 0; g < S / 4; ++g) {
+            gptr[g] = (int32_t)gents.size();
+            for (MfEntry e : grpE[g]) {
+                if (!udirect) e.u = (const void*)(G.Urm.p + (uintptr_t)e.u);
+                gents.push_back(e);
+            }
+        }
+        gptr[S / 4] = (int32_t)gents.size();
+        upload(c, G.grpPtr, gptr);
+        upload(c, G.grpEntries, gents);
+    }
     G.numItems = (int32_t)items.size();
     upload(c, G.items, items);
     upload(c, G.zmap, zmap);
@@ -866,10 +1048,16 @@ void run_fp64(Context& c, HMatrix& h, const double* X, double* Y, Events& ev) {
     ev.rec(c, 1);
     reduce_group<double, double>(c, G, 16 * NB, nullptr);
     ev.rec(c, 2);
-    auto k2 = h.b == 64 ? k_mf_phase2<64, true, double, double, double, NB, double>
-                        : k_mf_phase2<32, true, double, double, double, NB, double>;
-    k2<<<h.S, 256, 0, c.stream>>>(G.leafPtr.p, G.entries.p, (const double*)G.zt.p, G.numColsPad, nullptr, nullptr,
-                                  nullptr, 0, (const double*)P.Drm.p, P.nearPtr.p, P.near.p, Xb, Y, ldx);
+    if (G.grouped) {
+        k_mf_phase2g<true, double, double, double, NB, double><<<h.S / 4, 256, 0, c.stream>>>(
+            G.leafPtr.p, G.entries.p, G.grpPtr.p, G.grpEntries.p, (const double*)G.zt.p, nullptr, nullptr, nullptr,
+            nullptr, nullptr, (const double*)P.Drm.p, P.nearPtr.p, P.near.p, Xb, Y, ldx);
+    } else {
+        auto k2 = h.b == 64 ? k_mf_phase2<64, true, double, double, double, NB, double>
+                            : k_mf_phase2<32, true, double, double, double, NB, double>;
+        k2<<<h.S, 256, 0, c.stream>>>(G.leafPtr.p, G.entries.p, (const double*)G.zt.p, G.numColsPad, nullptr,
+                                      nullptr, nullptr, 0, (const double*)P.Drm.p, P.nearPtr.p, P.near.p, Xb, Y, ldx);
+    }
     HMX_LAUNCHED();
     ev.rec(c, 3);
 }
@@ -895,11 +1083,18 @@ void run_mixed(Context& c, HMatrix& h, const float* X, float* Y, Events& ev) {
     reduce_group<float, float>(c, Ghi, NR, nullptr);
     reduce_group<float, _Float16>(c, Glo, NR, P.xscale.p);
     ev.rec(c, 2);
-    auto k2 = h.b == 64 ? k_mf_phase2<64, false, float, float, _Float16, NB, float>
-                        : k_mf_phase2<32, false, float, float, _Float16, NB, float>;
-    k2<<<h.S, 256, 0, c.stream>>>(Ghi.leafPtr.p, Ghi.entries.p, (const float*)Ghi.zt.p, Ghi.numColsPad,
-                                  Glo.leafPtr.p, Glo.entries.p, (const _Float16*)Glo.zt.p, Glo.numColsPad,
-                                  (const float*)P.Drm.p, P.nearPtr.p, P.near.p, Xb, Y, n);
+    if (Ghi.grouped && Glo.grouped) {
+        k_mf_phase2g<false, float, float, _Float16, NB, float><<<h.S / 4, 256, 0, c.stream>>>(
+            Ghi.leafPtr.p, Ghi.entries.p, Ghi.grpPtr.p, Ghi.grpEntries.p, (const float*)Ghi.zt.p, Glo.leafPtr.p,
+            Glo.entries.p, Glo.grpPtr.p, Glo.grpEntries.p, (const _Float16*)Glo.zt.p, (const float*)P.Drm.p,
+            P.nearPtr.p, P.near.p, Xb, Y, n);
+    } else {
+        auto k2 = h.b == 64 ? k_mf_phase2<64, false, float, float, _Float16, NB, float>
+                            : k_mf_phase2<32, false, float, float, _Float16, NB, float>;
+        k2<<<h.S, 256, 0, c.stream>>>(Ghi.leafPtr.p, Ghi.entries.p, (const float*)Ghi.zt.p, Ghi.numColsPad,
+                                      Glo.leafPtr.p, Glo.entries.p, (const _Float16*)Glo.zt.p, Glo.numColsPad,
+                                      (const float*)P.Drm.p, P.nearPtr.p, P.near.p, Xb, Y, n);
+    }
     HMX_LAUNCHED();
     ev.rec(c, 3);
 }
