@@ -691,7 +691,17 @@ void upload(Context& c, DBuf<T>& d, const std::vector<T>& h) {
 size_t tsize(int type) { return type == 0 ? 8 : (type == 1 ? 4 : 2); }
 
 // Build one group from a per-tile list of source columns.  When 'vdirect'
-// is set the group's V is the FP64 level storage itself (no copy).
+// is set the group's factors are the FP64 level storage itself (no copy):
+// V and U column-major in place, one run of z columns per tile (rank padded
+// to 16).  Otherwise both are copied and converted, in two layouts:
+//   * U (row-major) and z by (tile dimension, row cluster): the columns of
+//     all tiles that share rows form one run, padded to 16 once, so phase 2
+//     takes one entry per run and leaf (or group of leaves) instead of one
+//     per tile, however few columns each tile has in this group;
+//   * V by (tile dimension, column cluster): the columns of all tiles that
+//     read the same right-hand-side rows are contiguous, so a phase-1 item
+//     of 64 columns spans tiles and one load of those rows serves all of
+//     them; a per-column map gives each V column its z column.
 struct GroupTile {
     TileRef ref;
     std::vector<int> cols;  // source column indices in the tile
@@ -700,39 +710,107 @@ struct GroupTile {
 void build_group(Context& c, HMatrix& h, MfGroup& G, int type, const std::vector<GroupTile>& tiles, bool vdirect,
                  const std::vector<float>* su, const std::vector<float>* sv,
                  const std::vector<int64_t>* colBase) {
-    // FP64: both factors are used where the build left them (V and U column-major)
     const bool udirect = vdirect;
     const int b = (int)h.b;
     const int S = (int)h.S;
     const size_t es = tsize(type);
     G.type = type;
-    // Copied factors are laid out by (tile dimension, column cluster): the
-    // columns of all tiles that read the same right-hand-side rows are
-    // contiguous in V, so a phase-1 item of 64 columns spans tiles and one
-    // load of those rows serves all of them.  V used in place (FP64) keeps
-    // the build's per-tile layout and per-tile items.
-    std::vector<int> ord(tiles.size());
-    std::iota(ord.begin(), ord.end(), 0);
-    if (!vdirect)
-        std::stable_sort(ord.begin(), ord.end(), [&](int x, int y) {
-            const TileRef &A = tiles[x].ref, &B = tiles[y].ref;
-            return A.m != B.m ? A.m > B.m : A.inCl < B.inCl;
-        });
+    const int T = (int)tiles.size();
+    auto cnt_of = [&](int t) { return (int)tiles[t].cols.size(); };
+    auto scales = [&](int t, int q, float& fu, float& fv) {
+        fu = fv = 1.f;
+        if (su) {
+            const int j = tiles[t].cols[q];
+            fu = (*su)[(*colBase)[t] + j];
+            fv = (*sv)[(*colBase)[t] + j];
+        }
+    };
+    auto sorted_by = [&](bool byRow) {
+        std::vector<int> ord;
+        for (int t = 0; t < T; ++t)
+            if (cnt_of(t)) ord.push_back(t);
+        if (!vdirect)
+            std::stable_sort(ord.begin(), ord.end(), [&](int x, int y) {
+                const TileRef &A = tiles[x].ref, &B = tiles[y].ref;
+                const uint32_t ca = byRow ? A.outCl : A.inCl, cb = byRow ? B.outCl : B.inCl;
+                return A.m != B.m ? A.m > B.m : ca < cb;
+            });
+        return ord;
+    };
     std::vector<MfItem> items;
     std::vector<int64_t> redCol, colPart;
     std::vector<int32_t> colNpart, zmap;
     std::vector<float> colScale;
     std::vector<std::vector<MfEntry>> leafE(S);
-    // 64-row leaves: phase 2 runs one workgroup per 4 leaves, and tiles
+    // 64-row leaves: phase 2 runs one workgroup per 4 leaves, and runs
     // spanning all 4 are listed once per group
     const bool grouped = b == 64 && S >= 4;
     std::vector<std::vector<MfEntry>> grpE(grouped ? S / 4 : 0);
     std::vector<CvCol> vcols, ucols;
     std::vector<TrTile> trt;
+    std::vector<int64_t> zb(T, -1);  // z column of each tile's first column in this group
     int64_t zcol = 0, slot = 0, vElems = 0, uElems = 0;
     uint64_t fbytes = 0;
-    // phase-1 items over group columns [k0, k1), all of tile dimension m and
-    // column cluster inCl; V at vptr (in place) or element offset vOff (copy)
+    auto add_entries = [&](int m, uint32_t outCl, const MfEntry& proto, const double* ucm) {
+        // U at the first row of the leaf (or group) inside the run's U
+        const int bus = m / b;
+        auto at = [&](int r0) {
+            MfEntry e = proto;
+            e.u = ucm ? (const void*)(ucm + (size_t)r0)
+                      : (const void*)(uintptr_t)(((size_t)(uintptr_t)proto.u + (size_t)r0 * proto.rpad) * es);
+            return e;
+        };
+        if (grouped && bus >= 4)
+            for (int q = 0; q < bus / 4; ++q) grpE[(outCl * bus) / 4 + q].push_back(at(q * 4 * b));
+        else
+            for (int q = 0; q < bus; ++q) leafE[outCl * bus + q].push_back(at(q * b));
+    };
+    // ---- U and z ----
+    {
+        const std::vector<int> ord = sorted_by(true);
+        size_t i = 0;
+        while (i < ord.size()) {
+            const TileRef& R0 = tiles[ord[i]].ref;
+            size_t j = i + 1;
+            if (!udirect)
+                while (j < ord.size() && tiles[ord[j]].ref.m == R0.m && tiles[ord[j]].ref.outCl == R0.outCl) ++j;
+            const int m = R0.m;
+            int cntRun = 0;
+            for (size_t k = i; k < j; ++k) cntRun += cnt_of(ord[k]);
+            const int rpad = (cntRun + kPad - 1) / kPad * kPad;
+            const int64_t cd0 = (int64_t)ucols.size();
+            for (size_t k = i; k < j; ++k) {
+                const int t = ord[k];
+                const TileRef& R = tiles[t].ref;
+                const double* Usrc = R.lv->U.p + (size_t)R.lv->h_offsets[R.t] * m;
+                zb[t] = zcol + (int64_t)(ucols.size() - cd0);
+                for (int q = 0; q < cnt_of(t); ++q) {
+                    float fu, fv;
+                    scales(t, q, fu, fv);
+                    ucols.push_back(CvCol{Usrc + (size_t)tiles[t].cols[q] * m, 0, m, fu});
+                    colScale.push_back(fu * fv);
+                }
+            }
+            for (int q = cntRun; q < rpad; ++q) colScale.push_back(0.f);
+            if (udirect) {  // one tile, U column-major in place (all its columns, in order)
+                const TileRef& R = R0;
+                add_entries(m, R.outCl, MfEntry{nullptr, rpad, (int32_t)zcol, cntRun, m},
+                            R.lv->U.p + (size_t)R.lv->h_offsets[R.t] * m);
+            } else {
+                for (int r0 = 0; r0 < m; r0 += 32)
+                    for (int c0 = 0; c0 < rpad; c0 += 32) trt.push_back(TrTile{cd0, uElems, m, cntRun, rpad, r0, c0, 0});
+                add_entries(m, R0.outCl, MfEntry{(const void*)(uintptr_t)uElems, rpad, (int32_t)zcol, cntRun, 0},
+                            nullptr);
+                uElems += (int64_t)m * rpad;
+            }
+            zcol += rpad;
+            HMX_CHECK(zcol < ((int64_t)1 << 31), HMX_EUNSUPPORTED, "matrix-core plan: more than 2^31 columns in one group");
+            i = j;
+        }
+    }
+    // ---- V and the phase-1 items ----
+    // items over group columns [k0, k1), all of tile dimension m and column
+    // cluster inCl; V at vptr (in place) or element offset vOff (copy)
     auto emit_items = [&](int64_t k0, int64_t k1, int m, uint32_t inCl, const char* vptr, int64_t vOff) {
         const int nch = (m + kMfRows - 1) / kMfRows;
         const int rows = std::min(m, kMfRows);
@@ -762,63 +840,37 @@ void build_group(Context& c, HMatrix& h, MfGroup& G, int type, const std::vector
             slot += cnt * nch;
         }
     };
-    int64_t runK0 = 0, runV0 = 0;  // open run of copied columns sharing (m, inCl)
-    int runM = 0;
-    uint32_t runCl = 0;
-    for (int oi : ord) {
-        const GroupTile& gt = tiles[oi];
-        const TileRef& R = gt.ref;
-        const int cnt = (int)gt.cols.size();
-        if (!cnt) continue;
-        const int m = R.m;
-        if (!vdirect && (m != runM || R.inCl != runCl)) {
-            if ((int64_t)zmap.size() > runK0) emit_items(runK0, (int64_t)zmap.size(), runM, runCl, nullptr, runV0);
-            runK0 = (int64_t)zmap.size();
-            runV0 = vElems;
-            runM = m;
-            runCl = R.inCl;
-        }
-        const int rpad = (cnt + kPad - 1) / kPad * kPad;
-        const double* Usrc = R.lv->U.p + (size_t)R.lv->h_offsets[R.t] * m;
-        const double* Vsrc = R.lv->V.p + (size_t)R.lv->h_offsets[R.t] * m;
-        const int64_t cd0 = (int64_t)ucols.size();
-        for (int q = 0; q < cnt; ++q) {
-            const int j = gt.cols[q];
-            float fu = 1.f, fv = 1.f;
-            if (su) {
-                fu = (*su)[(*colBase)[oi] + j];
-                fv = (*sv)[(*colBase)[oi] + j];
+    {
+        const std::vector<int> ord = sorted_by(false);
+        size_t i = 0;
+        while (i < ord.size()) {
+            const TileRef& R0 = tiles[ord[i]].ref;
+            size_t j = i + 1;
+            if (!vdirect)
+                while (j < ord.size() && tiles[ord[j]].ref.m == R0.m && tiles[ord[j]].ref.inCl == R0.inCl) ++j;
+            const int m = R0.m;
+            const int64_t k0 = (int64_t)zmap.size(), v0 = vElems;
+            for (size_t k = i; k < j; ++k) {
+                const int t = ord[k];
+                const TileRef& R = tiles[t].ref;
+                const double* Vsrc = R.lv->V.p + (size_t)R.lv->h_offsets[R.t] * m;
+                for (int q = 0; q < cnt_of(t); ++q) {
+                    float fu, fv;
+                    scales(t, q, fu, fv);
+                    if (!vdirect) vcols.push_back(CvCol{Vsrc + (size_t)tiles[t].cols[q] * m, vElems, m, fv});
+                    zmap.push_back((int32_t)(zb[t] + q));
+                    vElems += m;
+                }
+                fbytes += 2ull * (uint64_t)cnt_of(t) * m * es;
             }
-            ucols.push_back(CvCol{Usrc + (size_t)j * m, 0, m, fu});
-            if (!vdirect) vcols.push_back(CvCol{Vsrc + (size_t)j * m, vElems + (int64_t)q * m, m, fv});
-            colScale.push_back(fu * fv);
+            if (vdirect)
+                emit_items(k0, (int64_t)zmap.size(), m, R0.inCl,
+                           (const char*)(R0.lv->V.p + (size_t)R0.lv->h_offsets[R0.t] * m), 0);
+            else
+                emit_items(k0, (int64_t)zmap.size(), m, R0.inCl, nullptr, v0);
+            i = j;
         }
-        for (int q = cnt; q < rpad; ++q) colScale.push_back(0.f);
-        if (!udirect)
-            for (int r0 = 0; r0 < m; r0 += 32)
-                for (int c0 = 0; c0 < rpad; c0 += 32) trt.push_back(TrTile{cd0, uElems, m, cnt, rpad, r0, c0, 0});
-        const int64_t k0 = (int64_t)zmap.size();
-        for (int q = 0; q < cnt; ++q) zmap.push_back((int32_t)(zcol + q));
-        if (vdirect) emit_items(k0, k0 + cnt, m, R.inCl, (const char*)Vsrc, 0);
-        // phase-2 entries: U at the first row of the leaf (or group) inside the
-        // tile -- the column-major U in place, or the row-major copy
-        const int bus = m / b;
-        auto entry_at = [&](int r0) {
-            return udirect ? MfEntry{(const void*)(Usrc + (size_t)r0), rpad, (int32_t)zcol, cnt, m}
-                           : MfEntry{(const void*)(uintptr_t)(((size_t)uElems + (size_t)r0 * rpad) * es), rpad,
-                                     (int32_t)zcol, cnt, 0};
-        };
-        if (grouped && bus >= 4)
-            for (int q = 0; q < bus / 4; ++q) grpE[(R.outCl * bus) / 4 + q].push_back(entry_at(q * 4 * b));
-        else
-            for (int q = 0; q < bus; ++q) leafE[R.outCl * bus + q].push_back(entry_at(q * b));
-        zcol += rpad;
-        HMX_CHECK(zcol < ((int64_t)1 << 31), HMX_EUNSUPPORTED, "matrix-core plan: more than 2^31 columns in one group");
-        vElems += (int64_t)cnt * m;
-        uElems += (int64_t)m * rpad;
-        fbytes += 2ull * (uint64_t)cnt * m * es;
     }
-    if (!vdirect && (int64_t)zmap.size() > runK0) emit_items(runK0, (int64_t)zmap.size(), runM, runCl, nullptr, runV0);
     // items that read the same right-hand-side block (same tile dimension,
     // column cluster and row chunk) next to each other (already so for the
     // copied layout)
