@@ -162,37 +162,52 @@ def main():
     # what torch's caching allocator still holds from the staging copies
     torch.cuda.empty_cache()
 
-    # ---- k right-hand sides: FP64 on the matrix cores
-    X = torch.empty(k, n, dtype=torch.float64, device="cuda")
-    hmx.generateRandomVector(ctx, k, n, 777, X)
-    Y = torch.empty_like(X)
-    pipe.matvec(X, Y)
-    secK = timed(lambda: pipe.matvec(X, Y), 5, 1)
-    msK = [float(v) for v in hmx.matvec_profile_multi(ctx, H, k, X, Y, iters=5)]
-    pipe.matvec(X, Y)
-    ydK = dense_rows(X, k)
-    rel_errK = float(torch.linalg.norm(Y[:, rl] - ydK) / torch.linalg.norm(ydK))
-    pinfo = H.plan_info(0)
-    log(f"FP64 {k}-RHS {secK * 1e3:.2f} ms, rel err {rel_errK:.3e}, plan {pinfo}")
+    rhs_block = None
+    if k:
+        # ---- k right-hand sides: FP64 on the matrix cores
+        X = torch.empty(k, n, dtype=torch.float64, device="cuda")
+        hmx.generateRandomVector(ctx, k, n, 777, X)
+        Y = torch.empty_like(X)
+        pipe.matvec(X, Y)
+        secK = timed(lambda: pipe.matvec(X, Y), 5, 1)
+        msK = [float(v) for v in hmx.matvec_profile_multi(ctx, H, k, X, Y, iters=5)]
+        pipe.matvec(X, Y)
+        ydK = dense_rows(X, k)
+        rel_errK = float(torch.linalg.norm(Y[:, rl] - ydK) / torch.linalg.norm(ydK))
+        pinfo = H.plan_info(0)
+        log(f"FP64 {k}-RHS {secK * 1e3:.2f} ms, rel err {rel_errK:.3e}, plan {pinfo}")
 
-    # ---- k right-hand sides: mixed FP32/FP16 on the matrix cores (FP32 inputs and outputs)
-    torch.cuda.empty_cache()
-    t0 = time.perf_counter()
-    minfo = pipe.make_mixed(args.mixed_target)
-    torch.cuda.synchronize()
-    mixed_build_s = time.perf_counter() - t0
-    X32 = X.float()
-    Y32 = torch.empty_like(X32)
-    pipe.matvec_mixed(X32, Y32)
-    secM = timed(lambda: pipe.matvec_mixed(X32, Y32), 5, 1)
-    msM = [float(v) for v in hmx.matvec_profile_multi(ctx, H, k, X32, Y32, iters=5, mixed=True)]
-    pipe.matvec_mixed(X32, Y32)
-    pipe.matvec(X, Y)
-    rel_m_vs_fp64 = float(torch.linalg.norm(Y32.double() - Y) / torch.linalg.norm(Y))
-    ydM = dense_rows(X32.double().contiguous(), k)
-    rel_m_vs_dense = float(torch.linalg.norm(Y32.double()[:, rl] - ydM) / torch.linalg.norm(ydM))
-    bytesM = int(minfo["bytes"]) + 8 * n * k
-    log(f"mixed {k}-RHS {secM * 1e3:.2f} ms, rel vs FP64 {rel_m_vs_fp64:.2e}, vs dense {rel_m_vs_dense:.2e}")
+        # ---- k right-hand sides: mixed FP32/FP16 on the matrix cores (FP32 inputs and outputs)
+        torch.cuda.empty_cache()
+        t0 = time.perf_counter()
+        minfo = pipe.make_mixed(args.mixed_target)
+        torch.cuda.synchronize()
+        mixed_build_s = time.perf_counter() - t0
+        X32 = X.float()
+        Y32 = torch.empty_like(X32)
+        pipe.matvec_mixed(X32, Y32)
+        secM = timed(lambda: pipe.matvec_mixed(X32, Y32), 5, 1)
+        msM = [float(v) for v in hmx.matvec_profile_multi(ctx, H, k, X32, Y32, iters=5, mixed=True)]
+        pipe.matvec_mixed(X32, Y32)
+        pipe.matvec(X, Y)
+        rel_m_vs_fp64 = float(torch.linalg.norm(Y32.double() - Y) / torch.linalg.norm(Y))
+        ydM = dense_rows(X32.double().contiguous(), k)
+        rel_m_vs_dense = float(torch.linalg.norm(Y32.double()[:, rl] - ydM) / torch.linalg.norm(ydM))
+        bytesM = int(minfo["bytes"]) + 8 * n * k
+        log(f"mixed {k}-RHS {secM * 1e3:.2f} ms, rel vs FP64 {rel_m_vs_fp64:.2e}, vs dense {rel_m_vs_dense:.2e}")
+        rhs_block = {
+                "fp64_ms_per_rhs": round(secK * 1e3 / k, 5), "fp64_call_ms": round(secK * 1e3, 3),
+                "fp64_kernel_ms": msK, "fp64_rel_err_vs_dense": rel_errK,
+                "fp64_GBps": round((pinfo["bytes"] + 16 * n * k) / secK / 1e9, 1),
+                "fp64_plan_bytes": int(pinfo["bytes"]), "dense_near_blocks": int(pinfo["denseBlocks"]),
+                "mixed_ms_per_rhs": round(secM * 1e3 / k, 5), "mixed_call_ms": round(secM * 1e3, 3),
+                "mixed_kernel_ms": msM, "mixed_GBps": round(bytesM / secM / 1e9, 1),
+                "mixed_rel_err_vs_fp64": rel_m_vs_fp64, "mixed_rel_err_vs_dense": rel_m_vs_dense,
+                "speedup_mixed_vs_fp64_same_rhs": round(secK / secM, 3),
+                "speedup_mixed_per_rhs_vs_fp64_1rhs": round(sec1 / (secM / k), 2),
+                "mixed_target": args.mixed_target, "mixed_bytes": int(minfo["bytes"]),
+                "mixed_fp32_cols": int(minfo["hiCols"]), "mixed_fp16_cols": int(minfo["loCols"]),
+                "mixed_build_s": round(mixed_build_s, 3)}
 
     ranks_per_level = {}
     for l in range(1, H.num_levels):
@@ -226,25 +241,14 @@ def main():
         "factor_bytes": int(fac),
         "levels": ranks_per_level,
         "kernel_ms": {"phase1_VTx": ms[0], "reduce": ms[1], "phase2_Uz_Dx": ms[2], "call": ms[3]},
-        f"rhs{k}": {
-            "fp64_ms_per_rhs": round(secK * 1e3 / k, 5), "fp64_call_ms": round(secK * 1e3, 3),
-            "fp64_kernel_ms": msK, "fp64_rel_err_vs_dense": rel_errK,
-            "fp64_GBps": round((pinfo["bytes"] + 16 * n * k) / secK / 1e9, 1),
-            "fp64_plan_bytes": int(pinfo["bytes"]), "dense_near_blocks": int(pinfo["denseBlocks"]),
-            "mixed_ms_per_rhs": round(secM * 1e3 / k, 5), "mixed_call_ms": round(secM * 1e3, 3),
-            "mixed_kernel_ms": msM, "mixed_GBps": round(bytesM / secM / 1e9, 1),
-            "mixed_rel_err_vs_fp64": rel_m_vs_fp64, "mixed_rel_err_vs_dense": rel_m_vs_dense,
-            "speedup_mixed_vs_fp64_same_rhs": round(secK / secM, 3),
-            "speedup_mixed_per_rhs_vs_fp64_1rhs": round(sec1 / (secM / k), 2),
-            "mixed_target": args.mixed_target, "mixed_bytes": int(minfo["bytes"]),
-            "mixed_fp32_cols": int(minfo["hiCols"]), "mixed_fp16_cols": int(minfo["loCols"]),
-            "mixed_build_s": round(mixed_build_s, 3)},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": PEAK_HBM_GBS, "unit": "GB/s",
                      "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": traffic_bytes(args, dom),
                      "kernel": "k_mv_phase1" if dom == 1 else "k_mv_phase2", "kernel_bytes": int(dom_bytes),
                      "kernel_ms": dom_ms},
         "cpu_baseline": cpu,
     }
+    if rhs_block:
+        out[f"rhs{k}"] = rhs_block
     print(json.dumps(out), flush=True)
 
 

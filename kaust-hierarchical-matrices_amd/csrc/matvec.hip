@@ -212,29 +212,27 @@ __global__ void __launch_bounds__(256) k_mv_phase2(const int32_t* __restrict__ l
         const double* __restrict__ U = en.u;
         const double* __restrict__ zz = z + (size_t)en.zbase * K;
         const int m = en.m;
-        int j = cg;
-        for (; j + 48 < en.rank; j += 64) {
-            double u[4][RPL];
+        // 64 columns per step: a column group's four columns (cg, cg+16,
+        // cg+32, cg+48) load together, the ones past the rank masked off, so
+        // a tile of rank <= 64 costs one round trip instead of one per column
+        for (int j0 = cg; j0 < en.rank; j0 += 64) {
+            double u[4][RPL], zv[4][K];
 #pragma unroll
-            for (int p = 0; p < 4; ++p) vload_d<RPL>(U + (size_t)(j + 16 * p) * m + r0, u[p]);
+            for (int p = 0; p < 4; ++p) {
+                const int j = j0 + 16 * p;
+                if (j < en.rank) {
+                    vload_d<RPL>(U + (size_t)j * m + r0, u[p]);
+#pragma unroll
+                    for (int s = 0; s < K; ++s) zv[p][s] = zz[(size_t)j * K + s];
+                }
+            }
 #pragma unroll
             for (int p = 0; p < 4; ++p)
+                if (j0 + 16 * p < en.rank)
 #pragma unroll
-                for (int s = 0; s < K; ++s) {
-                    const double zv = zz[(size_t)(j + 16 * p) * K + s];
+                    for (int s = 0; s < K; ++s)
 #pragma unroll
-                    for (int q = 0; q < RPL; ++q) acc[q][s] = fma(u[p][q], zv, acc[q][s]);
-                }
-        }
-        for (; j < en.rank; j += 16) {
-            double u[RPL];
-            vload_d<RPL>(U + (size_t)j * m + r0, u);
-#pragma unroll
-            for (int s = 0; s < K; ++s) {
-                const double zv = zz[(size_t)j * K + s];
-#pragma unroll
-                for (int q = 0; q < RPL; ++q) acc[q][s] = fma(u[q], zv, acc[q][s]);
-            }
+                        for (int q = 0; q < RPL; ++q) acc[q][s] = fma(u[p][q], zv[p][s], acc[q][s]);
         }
     }
     // fold the 4 column groups of a wave (lanes l16 + 16g), then the 4 waves
