@@ -107,8 +107,11 @@ __global__ void __launch_bounds__(256) k_mf_phase1(const MfItem* __restrict__ it
     for (int g = 0; g < CG; ++g)
 #pragma unroll
         for (int nb = 0; nb < NB; ++nb) acc[g][nb] = typename M::accv{0, 0, 0, 0};
-    // software pipelined: the operands of super-step r0 + 16*KL are in flight
-    // while the MFMAs of super-step r0 run
+    // FP64: software pipelined, the operands of super-step r0 + 16*KL are in
+    // flight while the MFMAs of super-step r0 run.  FP32/FP16: no register
+    // double buffer -- the registers it frees buy a third wave per SIMD,
+    // which hides the load latency better (measured).
+    constexpr bool PF = sizeof(T) == 8;
     T a[CG][VL], bx[NB][VL], an[CG][VL], bn[NB][VL];
     auto load = [&](int r0, T (&A)[CG][VL], T (&B)[NB][VL]) {
         const int rr = r0 + VL * lg;
@@ -125,9 +128,10 @@ __global__ void __launch_bounds__(256) k_mf_phase1(const MfItem* __restrict__ it
             else vzero(B[nb]);
         }
     };
-    load(0, a, bx);
+    if (PF) load(0, a, bx);
     for (int r0 = 0; r0 < it.rows; r0 += 16 * KL) {
-        const bool more = r0 + 16 * KL < it.rows;
+        const bool more = PF && r0 + 16 * KL < it.rows;
+        if (!PF) load(r0, a, bx);
         if (more) load(r0 + 16 * KL, an, bn);
 #pragma unroll
         for (int e = 0; e < 4; ++e)
@@ -181,7 +185,10 @@ __global__ void __launch_bounds__(256) k_mf_phase1(const MfItem* __restrict__ it
 
 // ---- reduce partials, scale, convert, transpose ------------------------------
 // nred columns split into row chunks: g-th such column is padded column
-// redCol[g] with partials colPart[g] .. + colNpart[g]
+// redCol[g] with partials colPart[g] .. + colNpart[g].  One wave per column,
+// lane = right-hand side (NR = 64; NR = 16 uses 16 lanes): the partials are
+// summed by 8 interleaved accumulators folded in a fixed order, so the long
+// columns of the top levels (up to m/1024 partials) are not one serial chain.
 template <class T, class ACC, int NR>
 __global__ void __launch_bounds__(256) k_mf_reduce(int64_t nred, int64_t ldz, const int64_t* __restrict__ redCol,
                                                    const int64_t* __restrict__ colPart,
@@ -189,29 +196,27 @@ __global__ void __launch_bounds__(256) k_mf_reduce(int64_t nred, int64_t ldz, co
                                                    const float* __restrict__ colScale,
                                                    const float* __restrict__ rhsScale, const ACC* __restrict__ zpart,
                                                    T* __restrict__ Zt) {
-    // read along the right-hand sides (contiguous), write along columns
-    __shared__ ACC tile[64][NR + 1];
-    const int64_t g0 = (int64_t)blockIdx.x * 64;
-    for (int o = threadIdx.x; o < 64 * NR; o += 256) {
-        const int c = o / NR, s = o % NR;
-        const int64_t g = g0 + c;
-        ACC v = 0;
-        if (g < nred) {
-            const int np = colNpart[g];
-            const int64_t p0 = colPart[g];
-            for (int q = 0; q < np; ++q) v += zpart[(p0 + q) * NR + s];
-            if (colScale) v *= (ACC)colScale[redCol[g]];
-            if (rhsScale) v *= (ACC)rhsScale[s];
-        }
-        tile[c][s] = v;
-    }
-    __syncthreads();
+    const int64_t g = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int s = threadIdx.x & 63;
+    if (g >= nred || s >= NR) return;
+    const int np = colNpart[g];
+    const ACC* p = zpart + colPart[g] * NR + s;
+    ACC acc[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) acc[u] = 0;
+    int q = 0;
+    for (; q + 8 <= np; q += 8)
+#pragma unroll
+        for (int u = 0; u < 8; ++u) acc[u] += p[(size_t)(q + u) * NR];
+#pragma unroll
+    for (int u = 0; u < 8; ++u)
+        if (q + u < np) acc[u] += p[(size_t)(q + u) * NR];
+    ACC v = ((acc[0] + acc[1]) + (acc[2] + acc[3])) + ((acc[4] + acc[5]) + (acc[6] + acc[7]));
+    const int64_t zc = redCol[g];
+    if (colScale) v *= (ACC)colScale[zc];
+    if (rhsScale) v *= (ACC)rhsScale[s];
     constexpr int VL = 4 * Mfma<T>::KL;
-    for (int o = threadIdx.x; o < 64 * NR; o += 256) {
-        const int s = o / 64, c = o % 64;
-        const int64_t g = g0 + c;
-        if (g < nred) Zt[zt_index<VL, NR>(redCol[g], s)] = (T)tile[c][s];
-    }
+    Zt[zt_index<VL, NR>(zc, s)] = (T)v;
 }
 
 // ---- phase 2 -----------------------------------------------------------------
@@ -1016,7 +1021,7 @@ void build_fp64_plan(Context& c, HMatrix& h) {
 template <class ACC, class T>
 void reduce_group(Context& c, MfGroup& G, int NR, const float* rhsScale) {
     if (G.numRed == 0) return;
-    const unsigned blocks = (unsigned)((G.numRed + 63) / 64);
+    const unsigned blocks = (unsigned)((G.numRed + 3) / 4);
     const float* cs = G.colScale.n ? G.colScale.p : nullptr;
     if (NR == 64)
         k_mf_reduce<T, ACC, 64><<<blocks, 256, 0, c.stream>>>(G.numRed, G.numColsPad, G.redCol.p, G.colPart.p,
