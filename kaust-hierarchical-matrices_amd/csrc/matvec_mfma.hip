@@ -429,23 +429,28 @@ __device__ inline void lr_rows64(const MfEntry* __restrict__ ents, int e0, int e
 }
 
 // One wave's leaf times its dense 64x64 blocks (the diagonal block, then its
-// near-field blocks), software pipelined over 16-column steps.
+// near-field blocks), software pipelined over steps of 4*EPL columns.  A lane
+// loads EPL = 32 bytes of a block row and of an X row per step, so the four
+// lane groups of a row read one whole 128-byte line at once (an FP32 row
+// read 64 bytes at a time was fetched twice).  Temporal loads: a block's
+// lines are reused by the next step.
 template <class TD, int NB>
 __device__ inline void dense_rows64(int leaf, const TD* __restrict__ Drm, const int32_t* __restrict__ nearPtr,
                                     const MfNear* __restrict__ nearB, const TD* __restrict__ Xb, int lg, int lc,
                                     typename Mfma<TD>::accv (&acc)[4][NB]) {
     using MD = Mfma<TD>;
-    constexpr int NR = 16 * NB;
+    static_assert(MD::KL == 1, "dense blocks on the 16x16x4 shapes");
+    constexpr int NR = 16 * NB, EPL = 32 / sizeof(TD), SPB = 64 / (4 * EPL);
     const int n0 = nearPtr ? nearPtr[leaf] : 0, n1 = nearPtr ? nearPtr[leaf + 1] : 0;
-    const int total = (n1 - n0 + 1) * 4;
-    TD a[4][4], bx[NB][4], an[4][4], bn[NB][4];
-    auto dload = [&](int idx, TD (&A)[4][4], TD (&B)[NB][4]) {
-        const int q = n0 - 1 + idx / 4;
-        const int cc = (idx % 4) * 16 + 4 * lg;
+    const int total = (n1 - n0 + 1) * SPB;
+    TD a[4][EPL], bx[NB][EPL], an[4][EPL], bn[NB][EPL];
+    auto dload = [&](int idx, TD (&A)[4][EPL], TD (&B)[NB][EPL]) {
+        const int q = n0 - 1 + idx / SPB;
+        const int cc = (idx % SPB) * 4 * EPL + EPL * lg;
         const TD* Dq = q < n0 ? Drm + (size_t)leaf * 64 * 64 : (const TD*)nearB[q].d;
         const int cl = q < n0 ? leaf : nearB[q].colLeaf;
 #pragma unroll
-        for (int rg = 0; rg < 4; ++rg) vload_nt(Dq + (16 * rg + lc) * 64 + cc, A[rg]);
+        for (int rg = 0; rg < 4; ++rg) vload(Dq + (16 * rg + lc) * 64 + cc, A[rg]);
 #pragma unroll
         for (int nb = 0; nb < NB; ++nb) vload(Xb + ((size_t)cl * NR + 16 * nb + lc) * 64 + cc, B[nb]);
     };
@@ -454,7 +459,7 @@ __device__ inline void dense_rows64(int leaf, const TD* __restrict__ Drm, const 
         const bool more = idx + 1 < total;
         if (more) dload(idx + 1, an, bn);
 #pragma unroll
-        for (int e = 0; e < 4; ++e)
+        for (int e = 0; e < EPL; ++e)
 #pragma unroll
             for (int rg = 0; rg < 4; ++rg)
 #pragma unroll
@@ -463,11 +468,11 @@ __device__ inline void dense_rows64(int leaf, const TD* __restrict__ Drm, const 
 #pragma unroll
             for (int rg = 0; rg < 4; ++rg)
 #pragma unroll
-                for (int e = 0; e < 4; ++e) a[rg][e] = an[rg][e];
+                for (int e = 0; e < EPL; ++e) a[rg][e] = an[rg][e];
 #pragma unroll
             for (int nb = 0; nb < NB; ++nb)
 #pragma unroll
-                for (int e = 0; e < 4; ++e) bx[nb][e] = bn[nb][e];
+                for (int e = 0; e < EPL; ++e) bx[nb][e] = bn[nb][e];
         }
     }
 }
