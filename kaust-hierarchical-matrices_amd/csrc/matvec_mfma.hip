@@ -56,6 +56,16 @@ __device__ inline void vload_nt(const T* __restrict__ p, T (&out)[N]) {
     __builtin_memcpy(out, tmp, sizeof(tmp));
 }
 
+// factor loads: non-temporal when a lane's vector is 32 bytes, so the four
+// lane groups of a row or column read whole 128-byte lines; a narrower
+// vector leaves half a line that the next step reads, and a non-temporal
+// load would let that line go and fetch it twice (measured on FP32 blocks)
+template <class T, int N>
+__device__ inline void vload_f(const T* __restrict__ p, T (&out)[N]) {
+    if constexpr (N * sizeof(T) >= 32) vload_nt(p, out);
+    else vload(p, out);
+}
+
 template <class T, int N>
 __device__ inline void vzero(T (&out)[N]) {
 #pragma unroll
@@ -119,7 +129,7 @@ __global__ void __launch_bounds__(256) k_mf_phase1(const MfItem* __restrict__ it
 #pragma unroll
         for (int g = 0; g < CG; ++g) {
             const int col = 16 * g + lc;
-            if (rok && col < it.ncols) vload_nt(V + (size_t)col * it.ld + rr, A[g]);
+            if (rok && col < it.ncols) vload_f(V + (size_t)col * it.ld + rr, A[g]);
             else vzero(A[g]);
         }
 #pragma unroll
@@ -263,7 +273,7 @@ __global__ void __launch_bounds__(256) k_mf_phase2(const int32_t* __restrict__ p
             const int cc = 16 * kp + (idx % SPB) * 16 * KP + 4 * lg;
             const TD* Dq = q < n0 ? Drm + (size_t)leaf * BL * BL : (const TD*)nearB[q].d;
             const int cl = q < n0 ? leaf : nearB[q].colLeaf;
-            vload_nt(Dq + row * BL + cc, A);
+            vload_f(Dq + row * BL + cc, A);
 #pragma unroll
             for (int nb = 0; nb < NB; ++nb) vload(Xb + ((size_t)cl * NR + 16 * nb + lc) * BL + cc, B[nb]);
         };
@@ -300,7 +310,7 @@ __global__ void __launch_bounds__(256) k_mf_phase2(const int32_t* __restrict__ p
                         for (int q = 0; q < VL; ++q)
                             a[q] = cc + q < en.rank ? __builtin_nontemporal_load(U + (size_t)(cc + q) * en.ld) : (TA)0;
                     } else {
-                        vload_nt(U + cc, a);
+                        vload_f(U + cc, a);
                     }
 #pragma unroll
                     for (int nb = 0; nb < NB; ++nb) vload(Z + zt_index<VL, NR>(en.zcol + cc, 16 * nb + lc), bz[nb]);
@@ -327,7 +337,7 @@ __global__ void __launch_bounds__(256) k_mf_phase2(const int32_t* __restrict__ p
                 const int cc = c0 + VL * lg;
                 TB a[VL], bz[NB][VL];
                 if (cc < en.rpad) {
-                    vload_nt(U + cc, a);
+                    vload_f(U + cc, a);
 #pragma unroll
                     for (int nb = 0; nb < NB; ++nb) vload(Z + zt_index<VL, NR>(en.zcol + cc, 16 * nb + lc), bz[nb]);
                 } else {
@@ -381,7 +391,7 @@ __device__ inline void lr_rows64(const MfEntry* __restrict__ ents, int e0, int e
                     for (int q = 0; q < VL; ++q)
                         A[rg][q] = cc + q < en.rank ? __builtin_nontemporal_load(U + (size_t)(cc + q) * en.ld) : (T)0;
                 } else {
-                    vload_nt((const T*)en.u + (size_t)row * en.rpad + cc, A[rg]);
+                    vload_f((const T*)en.u + (size_t)row * en.rpad + cc, A[rg]);
                 }
             }
 #pragma unroll
