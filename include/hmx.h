@@ -109,6 +109,15 @@ hmx_status hmx_kdtree_copy(hmx_context* ctx, const hmx_kdtree* tree, int32_t* pe
                            double* bbmin, double* bbmax);
 /* device pointer to segmentIndices (valid while the tree lives) */
 const int32_t* hmx_kdtree_device_perm(const hmx_kdtree* tree);
+/* vectorWidth vectors (n x vectorWidth, column-major, ld n) between the
+   caller's original point order and the KD-tree order every other entry
+   point uses (position i of the KD order holds original point
+   segmentIndices[i]): toOriginal = 0 gathers (out[i] = in[perm[i]]),
+   toOriginal = 1 scatters (out[perm[i]] = in[i]).  Out of place.  The
+   reference takes vectors in KD order only (src/main.cu:158-173); this is the
+   original-order option of the drop-in contract. */
+hmx_status hmx_reorderVectors(hmx_context* ctx, const hmx_kdtree* tree, uint32_t vectorWidth, int32_t toOriginal,
+                              const double* d_in, double* d_out);
 /* freeKDTree, src/kDTree.cu:33-36 */
 hmx_status hmx_freeKDTree(hmx_kdtree* tree);
 

@@ -112,6 +112,18 @@ hmx_status hmx_generateRandomVector(hmx_context* ctx, uint32_t vectorWidth, uint
     });
 }
 
+hmx_status hmx_reorderVectors(hmx_context* ctx, const hmx_kdtree* tree, uint32_t vectorWidth, int32_t toOriginal,
+                              const double* d_in, double* d_out) {
+    if (!ctx) return HMX_EINVAL;
+    return guard(ctx, [&] {
+        NEED(tree);
+        NEED(d_in);
+        NEED(d_out);
+        HMX_CHECK(d_in != d_out, HMX_EINVAL, "reorderVectors works out of place");
+        reorder_vectors(*ctx, tree->n, vectorWidth, toOriginal ? 1 : 0, tree->perm.p, d_in, d_out);
+    });
+}
+
 hmx_status hmx_constructKDTree(hmx_context* ctx, uint32_t n, uint32_t d, uint32_t bucketSize, int32_t divMethod,
                                const double* d_points, hmx_kdtree** tree) {
     if (!ctx) return HMX_EINVAL;

@@ -30,6 +30,27 @@ void generate_dataset(Context& c, uint32_t n, uint32_t d, uint64_t seed, double*
     HMX_LAUNCHED();
 }
 
+// k vectors (column-major, ld n) between original point order and the
+// KD-tree order: position i of the KD order holds original point perm[i]
+__global__ void __launch_bounds__(256) k_reorder(uint32_t n, uint32_t k, int toOriginal,
+                                                 const int32_t* __restrict__ perm, const double* __restrict__ in,
+                                                 double* __restrict__ out) {
+    const uint64_t e = (uint64_t)blockIdx.x * 256 + threadIdx.x;
+    if (e >= (uint64_t)n * k) return;
+    const uint64_t s = e / n, i = e % n;
+    const uint64_t o = s * n + (uint64_t)perm[i];
+    if (toOriginal) out[o] = in[e];
+    else out[e] = in[o];
+}
+
+void reorder_vectors(Context& c, uint32_t n, uint32_t k, int toOriginal, const int32_t* d_perm, const double* d_in,
+                     double* d_out) {
+    const uint64_t total = (uint64_t)n * k;
+    if (total == 0) return;
+    k_reorder<<<(unsigned)((total + 255) / 256), 256, 0, c.stream>>>(n, k, toOriginal, d_perm, d_in, d_out);
+    HMX_LAUNCHED();
+}
+
 void generate_random_vector(Context& c, uint32_t k, uint32_t n, uint64_t seed, double* d_out) {
     uint64_t total = (uint64_t)n * k;
     if (!total) return;

@@ -70,6 +70,7 @@ _SIGS = [
     ("hmx_kdtree_info", _i32, [_vp, _P(_u32), _P(_u32), _P(_u32)]),
     ("hmx_kdtree_copy", _i32, [_vp, _vp, _vp, _vp, _vp, _vp]),
     ("hmx_kdtree_device_perm", _vp, [_vp]),
+    ("hmx_reorderVectors", _i32, [_vp, _vp, _u32, _i32, _vp, _vp]),
     ("hmx_freeKDTree", _i32, [_vp]),
     ("hmx_constructHMatrixStructure", _i32, [_vp, _u32, _u32, _u32, _i32, _f32, _vp, _P(_vp)]),
     ("hmx_structure_from_boxes", _i32, [_u32, _u32, _u32, _i32, _f32, _vp, _vp, _P(_vp)]),
@@ -216,6 +217,16 @@ class KDTree(_Obj):
 
     def device_perm(self):
         return lib().hmx_kdtree_device_perm(self.handle)
+
+    def reorder(self, ctx, x, to_original, out=None):
+        """k vectors ((n,) or (k, n) FP64 on the GPU) from the original point
+        order to the KD order (to_original=False) or back."""
+        import torch
+        k = 1 if x.dim() == 1 else x.shape[0]
+        if out is None:
+            out = torch.empty_like(x)
+        _check(lib().hmx_reorderVectors(ctx.handle, self.handle, k, 1 if to_original else 0, _ptr(x), _ptr(out)), ctx)
+        return out
 
 
 class Structure(_Obj):
@@ -616,11 +627,20 @@ class Pipeline:
         HMatrixVecMultMixed(self.ctx, self.H, x.shape[0], x, y)
         return y
 
-    def matvec(self, x, y=None, transpose=False):
+    def matvec(self, x, y=None, transpose=False, original_order=False):
+        """y = H x (or H^T x).  x: (n,) or (k, n) contiguous, i.e. an n x k
+        column-major block, in KD-tree order like the reference; with
+        original_order=True x and y are in the caller's point order and are
+        permuted through the tree on the way in and out."""
         import torch
-        # x: (n,) or (k, n) contiguous, i.e. an n x k column-major block
         k = 1 if x.dim() == 1 else x.shape[0]
         if y is None:
             y = torch.empty_like(x)
+        if original_order:
+            xk = self.tree.reorder(self.ctx, x, False)
+            yk = torch.empty_like(xk)
+            HMatrixVecMult(self.ctx, self.H, k, xk, yk, transpose)
+            self.tree.reorder(self.ctx, yk, True, out=y)
+            return y
         HMatrixVecMult(self.ctx, self.H, k, x, y, transpose)
         return y

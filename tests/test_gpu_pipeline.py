@@ -213,6 +213,24 @@ def test_matvec_deterministic(pair, torch_cuda):
     assert torch_cuda.equal(y1, y2)
 
 
+def test_matvec_in_original_order(pair, ctx, torch_cuda):
+    """Vectors in the caller's point order: gathered into the KD order through
+    the tree's permutation and scattered back; equal, bit for bit, to the
+    KD-order product with the permutation applied by hand (numpy)."""
+    pipe, ref, _ = pair
+    n = ref.n
+    perm, _, _, _ = pipe.tree.copy(ctx, n, ref.d)
+    x = torch_cuda.randn(3, n, dtype=torch_cuda.float64, device="cuda")
+    xk = pipe.tree.reorder(ctx, x, False)
+    assert np.array_equal(xk.cpu().numpy(), x.cpu().numpy()[:, perm])
+    assert torch_cuda.equal(pipe.tree.reorder(ctx, xk, True), x)
+    y = pipe.matvec(x, original_order=True)
+    yk = pipe.matvec(torch_cuda.from_numpy(x.cpu().numpy()[:, perm].copy()).cuda())
+    expect = np.empty_like(yk.cpu().numpy())
+    expect[:, perm] = yk.cpu().numpy()
+    assert np.array_equal(y.cpu().numpy(), expect)
+
+
 def test_dense_rows_vs_oracle(pair, ctx, torch_cuda):
     pipe, ref, p = pair
     n = ref.n
