@@ -253,6 +253,11 @@ hmx_status hmx_HMatrixVecMultMixed(hmx_context* ctx, const hmx_hmatrix* h, uint3
    A, B, D row-major FP64 device buffers.  A self-check of the matrix-core
    operand layout. */
 hmx_status hmx_selftest_mfma(hmx_context* ctx, int32_t type, const double* d_A, const double* d_B, double* d_D);
+/* Diagnostic, no reference counterpart: average ms of one pass that reads
+   the caller's device buffer (bytes >= 1 MiB, a multiple of 16) once with
+   16-byte non-temporal loads -- the measured streaming-read ceiling the
+   H-matvec bandwidth is compared against beside the spec peak. */
+hmx_status hmx_stream_read_probe(hmx_context* ctx, const void* d_buf, uint64_t bytes, int32_t iters, float* ms);
 
 /* ---- differentiation (no reference counterpart: the backward of
    HMatrixVecMult, for autograd callers) ------------------------------------ */

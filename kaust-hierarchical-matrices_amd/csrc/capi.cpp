@@ -503,6 +503,15 @@ hmx_status hmx_hmatrix_set_level(hmx_context* ctx, hmx_hmatrix* h, int32_t level
     });
 }
 
+hmx_status hmx_stream_read_probe(hmx_context* ctx, const void* d_buf, uint64_t bytes, int32_t iters, float* ms) {
+    if (!ctx) return HMX_EINVAL;
+    return guard(ctx, [&] {
+        NEED(d_buf);
+        NEED(ms);
+        stream_read(*ctx, d_buf, bytes, iters, ms);
+    });
+}
+
 hmx_status hmx_selftest_mfma(hmx_context* ctx, int32_t type, const double* d_A, const double* d_B, double* d_D) {
     if (!ctx) return HMX_EINVAL;
     return guard(ctx, [&] {

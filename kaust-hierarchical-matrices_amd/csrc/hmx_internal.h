@@ -281,6 +281,7 @@ struct Batch {
 // ---- internal entry points (implemented in the .hip/.cpp files) ----------
 void generate_dataset(Context& c, uint32_t n, uint32_t d, uint64_t seed, double* d_points);
 void generate_random_vector(Context& c, uint32_t k, uint32_t n, uint64_t seed, double* d_out);
+void stream_read(Context& c, const void* buf, uint64_t bytes, int iters, float* ms);
 void reorder_vectors(Context& c, uint32_t n, uint32_t k, int toOriginal, const int32_t* d_perm, const double* d_in,
                      double* d_out);
 void build_kdtree(Context& c, KDTree& t, const double* d_points);

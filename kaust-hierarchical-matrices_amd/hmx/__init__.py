@@ -100,6 +100,7 @@ _SIGS = [
     ("hmx_hmatrix_mixed_info", _i32, [_vp, _P(_i64), _P(_i64), _P(_f64), _P(_f64), _P(_u64)]),
     ("hmx_HMatrixVecMultMixed", _i32, [_vp, _vp, _u32, _vp, _vp]),
     ("hmx_selftest_mfma", _i32, [_vp, _i32, _vp, _vp, _vp]),
+    ("hmx_stream_read_probe", _i32, [_vp, _vp, _u64, _i32, _P(_f32)]),
     ("hmx_hmatvec_grad_factors", _i32, [_vp, _vp, _u32, _vp, _vp, _vp, _vp]),
     ("hmx_hmatrix_level_get", _i32, [_vp, _vp, _i32, _vp, _vp]),
     ("hmx_hmatrix_set_level", _i32, [_vp, _vp, _i32, _vp, _vp]),
@@ -555,6 +556,15 @@ def make_mixed(ctx, H, target=3e-6):
 
 def HMatrixVecMultMixed(ctx, H, k, x, y):
     _check(lib().hmx_HMatrixVecMultMixed(ctx.handle, H.handle, k, _ptr(x), _ptr(y)), ctx)
+
+
+def stream_read_probe(ctx, buf, iters=5):
+    """GB/s of a pass that reads the device tensor buf once (16-byte
+    non-temporal loads): the measured streaming-read ceiling."""
+    ms = ctypes.c_float(0)
+    nbytes = buf.numel() * buf.element_size()
+    _check(lib().hmx_stream_read_probe(ctx.handle, _ptr(buf), nbytes, iters, ctypes.byref(ms)), ctx)
+    return nbytes / (ms.value * 1e-3) / 1e9
 
 
 def selftest_mfma(ctx, type_, A, B, D):

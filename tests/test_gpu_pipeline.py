@@ -220,7 +220,7 @@ def test_matvec_in_original_order(pair, ctx, torch_cuda):
     pipe, ref, _ = pair
     n = ref.n
     perm, _, _, _ = pipe.tree.copy(ctx, n, ref.d)
-    x = torch_cuda.randn(3, n, dtype=torch_cuda.float64, device="cuda")
+    x = torch_cuda.randn(4, n, dtype=torch_cuda.float64, device="cuda")
     xk = pipe.tree.reorder(ctx, x, False)
     assert np.array_equal(xk.cpu().numpy(), x.cpu().numpy()[:, perm])
     assert torch_cuda.equal(pipe.tree.reorder(ctx, xk, True), x)
