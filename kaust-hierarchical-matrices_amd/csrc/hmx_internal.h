@@ -195,6 +195,7 @@ struct MfGroup {
     int64_t numColsPad = 0, numPart = 0;
     DBuf<unsigned char> V, Urm;    // owned converted storage (V empty when it points into the FP64 levels)
     DBuf<float> colScale;          // per padded column (f16 group: su*sv), else empty
+    DBuf<float> colScaleV;         // the same by V column (phase 1 reads it beside zmap)
     DBuf<MfItem> items;
     int32_t numItems = 0;
     DBuf<int32_t> zmap;            // group column (V order) -> padded z column

@@ -97,7 +97,7 @@ __global__ void __launch_bounds__(256) k_mf_phase1(const MfItem* __restrict__ it
                                                    const T* __restrict__ Xb, int sh,
                                                    typename Mfma<T>::acc_t* __restrict__ zpart,
                                                    T* __restrict__ Zt, const int32_t* __restrict__ zmap,
-                                                   const float* __restrict__ colScale,
+                                                   const float* __restrict__ colScaleV,
                                                    const float* __restrict__ rhsScale) {
     using M = Mfma<T>;
     constexpr int KL = M::KL, VL = 4 * KL, NR = 16 * NB;
@@ -109,9 +109,13 @@ __global__ void __launch_bounds__(256) k_mf_phase1(const MfItem* __restrict__ it
     const int wg = chunk > 0 ? xcd_chunked(blockIdx.x, chunk) : (int)blockIdx.x;
     const int wid = wg * 4 + (threadIdx.x >> 6);
     if (wid >= nItems) return;
+    const int lg = lane >> 4, lc = lane & 15;
+    using ACC = typename M::acc_t;
+    ACC rs[NB];  // right-hand-side scales: independent of the item, load them first
+#pragma unroll
+    for (int nb = 0; nb < NB; ++nb) rs[nb] = rhsScale ? (ACC)rhsScale[16 * nb + lc] : (ACC)1;
     const MfItem it = items[wid];
     const T* __restrict__ V = (const T*)it.v;
-    const int lg = lane >> 4, lc = lane & 15;
     typename M::accv acc[CG][NB];
 #pragma unroll
     for (int g = 0; g < CG; ++g)
@@ -162,18 +166,15 @@ __global__ void __launch_bounds__(256) k_mf_phase1(const MfItem* __restrict__ it
         }
     }
     if (it.direct) {  // whole columns: scaled, converted, stored transposed ([rhs][column])
-        using ACC = typename M::acc_t;
-        ACC rs[NB];
-#pragma unroll
-        for (int nb = 0; nb < NB; ++nb) rs[nb] = rhsScale ? (ACC)rhsScale[16 * nb + lc] : (ACC)1;
 #pragma unroll
         for (int g = 0; g < CG; ++g)
 #pragma unroll
             for (int v = 0; v < 4; ++v) {
                 const int col = 16 * g + M::out_row(lg, v);
                 if (col < it.ncols) {
+                    // column scale by V order: no load waits on the z-column map
                     const int64_t zc = zmap[it.zidx + col];
-                    const ACC cs = colScale ? (ACC)colScale[zc] : (ACC)1;
+                    const ACC cs = colScaleV ? (ACC)colScaleV[it.zidx + col] : (ACC)1;
 #pragma unroll
                     for (int nb = 0; nb < NB; ++nb)
                         Zt[zt_index<VL, NR>(zc, 16 * nb + lc)] = (T)(acc[g][nb][v] * cs * rs[nb]);
@@ -940,7 +941,12 @@ void build_group(Context& c, HMatrix& h, MfGroup& G, int type, const std::vector
     upload(c, G.colNpart, colNpart);
     upload(c, G.leafPtr, leafPtr);
     upload(c, G.entries, ents);
-    if (su) upload(c, G.colScale, colScale);
+    if (su) {
+        upload(c, G.colScale, colScale);
+        std::vector<float> colScaleV(zmap.size());
+        for (size_t k = 0; k < zmap.size(); ++k) colScaleV[k] = colScale[zmap[k]];
+        upload(c, G.colScaleV, colScaleV);
+    }
     DBuf<CvCol> d_u, d_v;
     DBuf<TrTile> d_t;
     upload(c, d_u, ucols);
@@ -1071,7 +1077,7 @@ int phase1_chunk() {
 template <class T, int CG, int NB>
 void phase1(Context& c, MfGroup& G, const T* Xb, int sh, const float* rhsScale) {
     if (G.numItems == 0) return;
-    const float* cs = G.colScale.n ? G.colScale.p : nullptr;
+    const float* cs = G.colScaleV.n ? G.colScaleV.p : nullptr;
     const int chunk = phase1_chunk();
     unsigned blocks = (unsigned)((G.numItems + 3) / 4);
     if (chunk > 0) blocks = (blocks + 8 * chunk - 1) / (8 * chunk) * (8 * chunk);
