@@ -57,8 +57,8 @@ def main():
                          "one mixed --rhs product, exit")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py measures one GPU only (asked for {args.gpus}); sharding across GPUs is not part of "
-              "this build.", file=sys.stderr)
+        print(f"bench.py measures one GPU only (asked for {args.gpus}): the library runs on one GPU; how the "
+              "path shards across GPUs is a design, not an implementation.", file=sys.stderr)
         sys.exit(2)
 
     import torch
