@@ -201,8 +201,11 @@ hmx_status hmx_freeHMatrix(hmx_hmatrix* h);
 /* ---- H-matvec ---------------------------------------------------------- */
 /* HMatrixVecMult, src/HMatrixVectorMultiplication.cu:10-26: out = H * in,
    in/out are n x vectorWidth column-major (ld n) FP64 device buffers in
-   KD-permuted order; out is overwritten.  Requires full leaves (n = S*b),
-   as the reference does (src/cutlassDiagonalXVector.cu:59-65). */
+   KD-permuted order; out is overwritten.  Any n: with partial leaves (the
+   reference needs n = S*b, src/cutlassDiagonalXVector.cu:59-65) the vectors
+   are gathered into the padded leaf space and the product scattered back.
+   One right-hand side runs the FP64 streaming kernels, two or more the FP64
+   matrix cores (passes of 16, or 64 when vectorWidth is a multiple of 64). */
 hmx_status hmx_HMatrixVecMult(hmx_context* ctx, const hmx_hmatrix* h, uint32_t vectorWidth, const double* d_in,
                               double* d_out);
 /* out = H^T * in (no reference counterpart; the backward of HMatrixVecMult) */

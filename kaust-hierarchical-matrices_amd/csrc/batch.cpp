@@ -36,28 +36,43 @@ void batch_create(Context& c, const std::vector<HMatrix*>& hs, Batch& B) {
     HMX_HIP(hipEventCreateWithFlags(&B.evOut, hipEventDisableTiming));
 }
 
+// One right-hand side: the streaming kernels.  Two or more: the FP64 matrix
+// cores over the batch's block-diagonal operator (passes of 16 or 64, the
+// last padded with zero columns).
+static void prepare(Context& c, Batch& B, uint32_t k) {
+    if (k >= 2) {
+        batch_mf_prepare(c, B, k);
+    } else {
+        ensure_matvec_ws(B.P, 1);
+    }
+}
+
+static void run_batch(Context& c, Batch& B, uint32_t k, const double* X, double* Y) {
+    if (k >= 2) batch_mf_run(c, B, k, X, Y);
+    else run_matvec_plan(c, B.P, B.b, k, X, B.P.n, Y, nullptr);
+}
+
 void batch_matvec(Context& c, Batch& B, uint32_t k, const double* X, double* Y, bool useGraph) {
     HMX_CHECK(k >= 1, HMX_EINVAL, "vectorWidth must be >= 1");
     HMX_CHECK(X && Y && X != Y, HMX_EINVAL, "bad vector pointers");
     if (!useGraph) {
-        run_matvec_plan(c, B.P, B.b, k, X, B.P.n, Y, nullptr);
+        prepare(c, B, k);
+        run_batch(c, B, k, X, Y);
         return;
     }
     if (!B.exec || B.gX != X || B.gY != Y || B.gk != k) {
         B.drop_graph();
         // workspaces are allocated here, outside the capture; the captured
-        // sequence holds kernel launches only (gather, phase 1, reduce,
-        // phase 2, scatter per pass)
-        const uint32_t K = k == 1 ? 1 : (k <= 4 ? 4 : (k <= 8 ? 8 : 16));
-        HMX_CHECK(k == 1 || k % K == 0, HMX_EINVAL, "vectorWidth must be 1 or a multiple of 4/8/16");
-        ensure_matvec_ws(B.P, K);
+        // sequence holds kernel launches only (gather or blocked copy,
+        // phase 1, reduce, phase 2, scatter per pass)
+        prepare(c, B, k);
         HMX_HIP(hipStreamSynchronize(c.stream));
         Context cap;
         cap.stream = B.stream;
         cap.device = c.device;
         HMX_HIP(hipStreamBeginCapture(B.stream, hipStreamCaptureModeThreadLocal));
         try {
-            run_matvec_plan(cap, B.P, B.b, k, X, B.P.n, Y, nullptr);
+            run_batch(cap, B, k, X, Y);
         } catch (...) {
             hipGraph_t g = nullptr;
             (void)hipStreamEndCapture(B.stream, &g);
@@ -78,6 +93,9 @@ void batch_matvec(Context& c, Batch& B, uint32_t k, const double* X, double* Y, 
     HMX_HIP(hipStreamWaitEvent(c.stream, B.evOut, 0));
 }
 
+// Times the candidates of the path k right-hand sides take: the streaming
+// phase-1 chunk length (reported; it also serves one right-hand side) and,
+// for k >= 2, the matrix-core phase-1 workgroups per XCD chunk.
 void batch_tune(Context& c, Batch& B, uint32_t k, int* bestChunk, float* bestMs) {
     const int cands[] = {512, 1024, 2048, 4096};
     DBuf<double> X((size_t)B.P.n * k), Y((size_t)B.P.n * k);
@@ -104,6 +122,27 @@ void batch_tune(Context& c, Batch& B, uint32_t k, int* bestChunk, float* bestMs)
         }
         B.drop_graph();
         build_batch_plan(c, B.hs, best, B.P, B.diag);
+        if (k >= 2) {
+            batch_mf_prepare(c, B, k);
+            const int xc[] = {0, 8, 16, 32};
+            int bx = default_xcd_chunk();
+            float bmx = 1e30f;
+            for (int q : xc) {
+                B.mf.xcdChunk = q;
+                for (int w = 0; w < 2; ++w) batch_mf_run(c, B, k, X.p, Y.p);
+                HMX_HIP(hipEventRecord(e0, c.stream));
+                const int reps = 5;
+                for (int r = 0; r < reps; ++r) batch_mf_run(c, B, k, X.p, Y.p);
+                HMX_HIP(hipEventRecord(e1, c.stream));
+                HMX_HIP(hipEventSynchronize(e1));
+                float t = 0;
+                HMX_HIP(hipEventElapsedTime(&t, e0, e1));
+                t /= reps;
+                if (t < bmx) { bmx = t; bx = q; }
+            }
+            B.mf.xcdChunk = bx;
+            bms = bmx;
+        }
     } catch (...) {
         (void)hipEventDestroy(e0);
         (void)hipEventDestroy(e1);

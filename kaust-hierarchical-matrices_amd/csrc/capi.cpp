@@ -423,7 +423,10 @@ hmx_status hmx_HMatrixVecMult(hmx_context* ctx, const hmx_hmatrix* h, uint32_t v
     if (!ctx) return HMX_EINVAL;
     return guard(ctx, [&] {
         NEED(h);
-        if (vectorWidth >= 16 && vectorWidth % 16 == 0 && h->n == h->S * h->b)
+        // two or more right-hand sides: the FP64 matrix cores (passes of 16
+        // or 64, partial leaves through the padded space); one: the
+        // streaming kernels
+        if (vectorWidth >= 2)
             hmatvec_multi(*ctx, const_cast<hmx_hmatrix&>(*h), vectorWidth, d_in, d_out, nullptr, 0);
         else
             hmatvec(*ctx, const_cast<hmx_hmatrix&>(*h), vectorWidth, d_in, d_out, false, nullptr, 0);

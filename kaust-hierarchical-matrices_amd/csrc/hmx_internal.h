@@ -231,6 +231,15 @@ struct MfPlan {
     DBuf<int32_t> nearPtr;         // [S+1] CSR over row leaves
     DBuf<MfNear> near;
     int64_t numNear = 0;
+    // leaf space of the plan (one H-matrix, or a batch's block-diagonal
+    // operator): S leaves of b rows, npad = S*b positions for n vector
+    // entries; gmap maps a position to its entry (-1 on padding) when gather
+    int S = 0, b = 0;
+    int64_t n = 0, npad = 0;
+    bool gather = false;
+    DBuf<int32_t> gmap;
+    DBuf<unsigned char> Yp;        // product in the padded space before the scatter
+    int xcdChunk = -1;             // phase-1 workgroups per XCD chunk (-1: the library default)
     DBuf<unsigned char> Xb;        // right-hand sides in the blocked layout (plan's dense type)
     DBuf<_Float16> X16;            // the same in FP16, scaled per right-hand side (mixed)
     DBuf<float> xscale, xpart;     // per right-hand-side scale and its per-slice partial maxima
@@ -265,7 +274,8 @@ struct HMatrix {
 // plan, its own stream for hipGraph capture and replay.
 struct Batch {
     std::vector<HMatrix*> hs;
-    MatvecPlan P;
+    MatvecPlan P;                // streaming kernels (one right-hand side)
+    MfPlan mf;                   // FP64 matrix cores (two or more), built on first use
     DBuf<double> diag;
     int b = 0;
     hipStream_t stream = nullptr;
@@ -316,6 +326,10 @@ void hmatvec(Context& c, HMatrix& h, uint32_t k, const double* d_in, double* d_o
 void dense_rows(Context& c, uint32_t n, uint32_t d, const double* p, const int32_t* perm, double ell, uint32_t nr,
                 const int32_t* rows, uint32_t k, const double* X, double* Y);
 void hmatvec_multi(Context& c, HMatrix& h, uint32_t k, const double* d_in, double* d_out, float* ms, int iters);
+std::vector<int32_t> padding_map(const HMatrix& h, int64_t base);
+void batch_mf_prepare(Context& c, Batch& B, uint32_t k);
+void batch_mf_run(Context& c, Batch& B, uint32_t k, const double* d_in, double* d_out);
+int default_xcd_chunk();
 void hmatrix_make_mixed(Context& c, HMatrix& h, double targetRelErr);
 void hmatvec_mixed(Context& c, HMatrix& h, uint32_t k, const float* d_in, float* d_out, float* ms, int iters);
 void mfma_selftest(Context& c, int type, const double* A, const double* B, double* D);

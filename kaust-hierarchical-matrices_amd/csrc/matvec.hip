@@ -365,7 +365,7 @@ void upload_plan(Context& c, MatvecPlan& P, const PlanHost& H) {
 }
 
 // padded position -> index of the KD-ordered vector (-1 on padding)
-static std::vector<int32_t> padding_map(const HMatrix& h, int64_t base) {
+std::vector<int32_t> padding_map(const HMatrix& h, int64_t base) {
     const int S = (int)h.S, b = (int)h.b;
     std::vector<int32_t> gm((size_t)S * b, -1);
     for (int leaf = 0; leaf < S; ++leaf) {

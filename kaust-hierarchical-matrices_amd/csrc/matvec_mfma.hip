@@ -242,7 +242,7 @@ __global__ void __launch_bounds__(256) k_mf_phase2(const int32_t* __restrict__ p
                                                    const TB* __restrict__ ZB, int64_t ldzB,
                                                    const TD* __restrict__ Drm, const int32_t* __restrict__ nearPtr,
                                                    const MfNear* __restrict__ nearB, const TD* __restrict__ Xb,
-                                                   TO* __restrict__ Y, int ldy) {
+                                                   TO* __restrict__ Y, int ldy, int kvalid) {
     using MA = Mfma<TA>;
     using MB = Mfma<TB>;
     using MD = Mfma<TD>;
@@ -364,7 +364,7 @@ __global__ void __launch_bounds__(256) k_mf_phase2(const int32_t* __restrict__ p
         acc_t v = red[0][r][s];
 #pragma unroll
         for (int q = 1; q < KP; ++q) v += red[q][r][s];
-        Y[(size_t)s * ldy + (size_t)leaf * BL + r] = (TO)v;
+        if (s < kvalid) Y[(size_t)s * ldy + (size_t)leaf * BL + r] = (TO)v;
     }
 }
 
@@ -504,7 +504,7 @@ __global__ void __launch_bounds__(256) k_mf_phase2g(const int32_t* __restrict__ 
                                                     const MfEntry* __restrict__ gentB, const TB* __restrict__ ZB,
                                                     const TD* __restrict__ Drm, const int32_t* __restrict__ nearPtr,
                                                     const MfNear* __restrict__ nearB, const TD* __restrict__ Xb,
-                                                    TO* __restrict__ Y, int ldy) {
+                                                    TO* __restrict__ Y, int ldy, int kvalid) {
     using MA = Mfma<TA>;
     using accv = typename MA::accv;
     static_assert(sizeof(typename Mfma<TB>::acc_t) == sizeof(typename MA::acc_t) &&
@@ -531,7 +531,8 @@ __global__ void __launch_bounds__(256) k_mf_phase2g(const int32_t* __restrict__ 
         for (int nb = 0; nb < NB; ++nb)
 #pragma unroll
             for (int v = 0; v < 4; ++v)
-                Y[(size_t)(16 * nb + lc) * ldy + (size_t)leaf * 64 + 16 * rg + MA::out_row(lg, v)] = (TO)acc[rg][nb][v];
+                if (16 * nb + lc < kvalid)
+                    Y[(size_t)(16 * nb + lc) * ldy + (size_t)leaf * 64 + 16 * rg + MA::out_row(lg, v)] = (TO)acc[rg][nb][v];
 }
 
 // ---- right-hand-side conversion (mixed) ---------------------------------------
@@ -562,17 +563,37 @@ __global__ void __launch_bounds__(64) k_colmaxabs_fin(int P, const float* __rest
     if (threadIdx.x == 0) sx[blockIdx.x] = m > 0.f ? m : 1.f;
 }
 
-// X ([s][n], one right-hand side per row of n) into the blocked layout,
-// divided by the per-right-hand-side scale sx when one is given
+// X (right-hand side s at X + s*ldx) into the blocked layout of the padded
+// leaf space (npad positions): position p takes entry gmap[p] (p itself
+// without a map; zero on padding, gmap[p] < 0), right-hand sides s >= kvalid
+// are zero; divided by the per-right-hand-side scale sx when one is given
 template <class TI, class TO>
-__global__ void __launch_bounds__(256) k_x_blocked(int64_t total, int n, int NR, int sh, const TI* __restrict__ X,
-                                                   const float* __restrict__ sx, TO* __restrict__ Xb) {
+__global__ void __launch_bounds__(256) k_x_blocked(int64_t total, int64_t npad, int NR, int sh,
+                                                   const TI* __restrict__ X, int64_t ldx, int kvalid,
+                                                   const int32_t* __restrict__ gmap, const float* __restrict__ sx,
+                                                   TO* __restrict__ Xb) {
     const int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x;
     if (e >= total) return;
-    const int s = (int)(e / n), row = (int)(e % n);
-    const size_t o = ((((size_t)(row >> sh)) * NR + (size_t)s) << sh) + (size_t)(row & ((1 << sh) - 1));
-    if (sx) Xb[o] = (TO)(X[e] / sx[s]);
-    else Xb[o] = (TO)X[e];
+    const int s = (int)(e / npad);
+    const int64_t p = e % npad;
+    const size_t o = ((((size_t)(p >> sh)) * NR + (size_t)s) << sh) + (size_t)(p & ((1 << sh) - 1));
+    TO v = (TO)0;
+    if (s < kvalid) {
+        const int64_t g = gmap ? (int64_t)gmap[p] : p;
+        if (g >= 0) v = sx ? (TO)(X[s * ldx + g] / sx[s]) : (TO)X[s * ldx + g];
+    }
+    Xb[o] = v;
+}
+
+// product in the padded space (Yp, ld npad) back to the caller's entries
+template <class T>
+__global__ void __launch_bounds__(256) k_y_scatter(int64_t npad, int k, const int32_t* __restrict__ gmap,
+                                                   const T* __restrict__ Yp, T* __restrict__ Y, int64_t ldy) {
+    const int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (e >= npad * k) return;
+    const int64_t s = e / npad, p = e % npad;
+    const int g = gmap[p];
+    if (g >= 0) Y[s * ldy + g] = Yp[e];
 }
 
 // ---- plan construction kernels -----------------------------------------------
@@ -687,9 +708,10 @@ struct TileRef {
     const HLevel* lv;
     int level, t, r, m;
     uint32_t inCl, outCl;
+    int64_t leafBase;  // first leaf of the tile's H-matrix in the plan's leaf space
 };
 
-std::vector<TileRef> all_tiles(const HMatrix& h) {
+std::vector<TileRef> all_tiles(const HMatrix& h, int64_t leafBase = 0) {
     std::vector<TileRef> v;
     for (int l = 1; l < h.numLevels; ++l) {
         const HLevel& lv = h.levels[l - 1];
@@ -697,7 +719,7 @@ std::vector<TileRef> all_tiles(const HMatrix& h) {
         for (int t = 0; t < lv.numTiles; ++t) {
             if (lv.h_ranks[t] == 0) continue;
             v.push_back(TileRef{&lv, l, t, lv.h_ranks[t], lv.m, compact16((uint32_t)tiles[t]),
-                                compact16((uint32_t)tiles[t] >> 1)});
+                                compact16((uint32_t)tiles[t] >> 1), leafBase});
         }
     }
     return v;
@@ -728,12 +750,11 @@ struct GroupTile {
     std::vector<int> cols;  // source column indices in the tile
 };
 
-void build_group(Context& c, HMatrix& h, MfGroup& G, int type, const std::vector<GroupTile>& tiles, bool vdirect,
-                 const std::vector<float>* su, const std::vector<float>* sv,
-                 const std::vector<int64_t>* colBase) {
+void build_group(Context& c, const HMatrix& h, int S, bool grouped, MfGroup& G, int type,
+                 const std::vector<GroupTile>& tiles, bool vdirect, const std::vector<float>* su,
+                 const std::vector<float>* sv, const std::vector<int64_t>* colBase) {
     const bool udirect = vdirect;
     const int b = (int)h.b;
-    const int S = (int)h.S;
     const size_t es = tsize(type);
     G.type = type;
     const int T = (int)tiles.size();
@@ -746,6 +767,9 @@ void build_group(Context& c, HMatrix& h, MfGroup& G, int type, const std::vector
             fv = (*sv)[(*colBase)[t] + j];
         }
     };
+    // first padded row of a tile's row / column cluster in the plan's space
+    auto row_start = [&](const TileRef& R) { return R.leafBase * b + (int64_t)R.outCl * R.m; };
+    auto col_start = [&](const TileRef& R) { return R.leafBase * b + (int64_t)R.inCl * R.m; };
     auto sorted_by = [&](bool byRow) {
         std::vector<int> ord;
         for (int t = 0; t < T; ++t)
@@ -753,7 +777,7 @@ void build_group(Context& c, HMatrix& h, MfGroup& G, int type, const std::vector
         if (!vdirect)
             std::stable_sort(ord.begin(), ord.end(), [&](int x, int y) {
                 const TileRef &A = tiles[x].ref, &B = tiles[y].ref;
-                const uint32_t ca = byRow ? A.outCl : A.inCl, cb = byRow ? B.outCl : B.inCl;
+                const int64_t ca = byRow ? row_start(A) : col_start(A), cb = byRow ? row_start(B) : col_start(B);
                 return A.m != B.m ? A.m > B.m : ca < cb;
             });
         return ord;
@@ -763,17 +787,17 @@ void build_group(Context& c, HMatrix& h, MfGroup& G, int type, const std::vector
     std::vector<int32_t> colNpart, zmap;
     std::vector<float> colScale;
     std::vector<std::vector<MfEntry>> leafE(S);
-    // 64-row leaves: phase 2 runs one workgroup per 4 leaves, and runs
-    // spanning all 4 are listed once per group
-    const bool grouped = b == 64 && S >= 4;
+    // grouped (64-row leaves): phase 2 runs one workgroup per 4 leaves, and
+    // runs spanning all 4 are listed once per group
     std::vector<std::vector<MfEntry>> grpE(grouped ? S / 4 : 0);
     std::vector<CvCol> vcols, ucols;
     std::vector<TrTile> trt;
     std::vector<int64_t> zb(T, -1);  // z column of each tile's first column in this group
     int64_t zcol = 0, slot = 0, vElems = 0, uElems = 0;
     uint64_t fbytes = 0;
-    auto add_entries = [&](int m, uint32_t outCl, const MfEntry& proto, const double* ucm) {
-        // U at the first row of the leaf (or group) inside the run's U
+    auto add_entries = [&](int m, int64_t leaf0, const MfEntry& proto, const double* ucm) {
+        // U at the first row of the leaf (or group) inside the run's U;
+        // leaf0 = the run's first leaf in the plan's leaf space
         const int bus = m / b;
         auto at = [&](int r0) {
             MfEntry e = proto;
@@ -782,9 +806,9 @@ void build_group(Context& c, HMatrix& h, MfGroup& G, int type, const std::vector
             return e;
         };
         if (grouped && bus >= 4)
-            for (int q = 0; q < bus / 4; ++q) grpE[(outCl * bus) / 4 + q].push_back(at(q * 4 * b));
+            for (int q = 0; q < bus / 4; ++q) grpE[leaf0 / 4 + q].push_back(at(q * 4 * b));
         else
-            for (int q = 0; q < bus; ++q) leafE[outCl * bus + q].push_back(at(q * b));
+            for (int q = 0; q < bus; ++q) leafE[leaf0 + q].push_back(at(q * b));
     };
     // ---- U and z ----
     {
@@ -794,7 +818,8 @@ void build_group(Context& c, HMatrix& h, MfGroup& G, int type, const std::vector
             const TileRef& R0 = tiles[ord[i]].ref;
             size_t j = i + 1;
             if (!udirect)
-                while (j < ord.size() && tiles[ord[j]].ref.m == R0.m && tiles[ord[j]].ref.outCl == R0.outCl) ++j;
+                while (j < ord.size() && tiles[ord[j]].ref.m == R0.m && row_start(tiles[ord[j]].ref) == row_start(R0))
+                    ++j;
             const int m = R0.m;
             int cntRun = 0;
             for (size_t k = i; k < j; ++k) cntRun += cnt_of(ord[k]);
@@ -813,14 +838,15 @@ void build_group(Context& c, HMatrix& h, MfGroup& G, int type, const std::vector
                 }
             }
             for (int q = cntRun; q < rpad; ++q) colScale.push_back(0.f);
+            const int64_t leaf0 = row_start(R0) / b;
             if (udirect) {  // one tile, U column-major in place (all its columns, in order)
                 const TileRef& R = R0;
-                add_entries(m, R.outCl, MfEntry{nullptr, rpad, (int32_t)zcol, cntRun, m},
+                add_entries(m, leaf0, MfEntry{nullptr, rpad, (int32_t)zcol, cntRun, m},
                             R.lv->U.p + (size_t)R.lv->h_offsets[R.t] * m);
             } else {
                 for (int r0 = 0; r0 < m; r0 += 32)
                     for (int c0 = 0; c0 < rpad; c0 += 32) trt.push_back(TrTile{cd0, uElems, m, cntRun, rpad, r0, c0, 0});
-                add_entries(m, R0.outCl, MfEntry{(const void*)(uintptr_t)uElems, rpad, (int32_t)zcol, cntRun, 0},
+                add_entries(m, leaf0, MfEntry{(const void*)(uintptr_t)uElems, rpad, (int32_t)zcol, cntRun, 0},
                             nullptr);
                 uElems += (int64_t)m * rpad;
             }
@@ -832,7 +858,7 @@ void build_group(Context& c, HMatrix& h, MfGroup& G, int type, const std::vector
     // ---- V and the phase-1 items ----
     // items over group columns [k0, k1), all of tile dimension m and column
     // cluster inCl; V at vptr (in place) or element offset vOff (copy)
-    auto emit_items = [&](int64_t k0, int64_t k1, int m, uint32_t inCl, const char* vptr, int64_t vOff) {
+    auto emit_items = [&](int64_t k0, int64_t k1, int m, int64_t colStart, const char* vptr, int64_t vOff) {
         const int nch = (m + kMfRows - 1) / kMfRows;
         const int rows = std::min(m, kMfRows);
         const bool partial = nch > 1 || !h.planDirectZ;
@@ -842,7 +868,7 @@ void build_group(Context& c, HMatrix& h, MfGroup& G, int type, const std::vector
                 MfItem it;
                 const size_t e = (size_t)c0 * m + (size_t)q * kMfRows;
                 it.v = vptr ? (const void*)(vptr + e * es) : (const void*)(uintptr_t)(((size_t)vOff + e) * es);
-                it.xoff = (int32_t)(inCl * m + q * kMfRows);
+                it.xoff = (int32_t)(colStart + q * kMfRows);
                 it.rows = rows;
                 it.ld = m;
                 it.ncols = (int32_t)std::min<int64_t>(kMfCols, cnt - c0);
@@ -868,7 +894,8 @@ void build_group(Context& c, HMatrix& h, MfGroup& G, int type, const std::vector
             const TileRef& R0 = tiles[ord[i]].ref;
             size_t j = i + 1;
             if (!vdirect)
-                while (j < ord.size() && tiles[ord[j]].ref.m == R0.m && tiles[ord[j]].ref.inCl == R0.inCl) ++j;
+                while (j < ord.size() && tiles[ord[j]].ref.m == R0.m && col_start(tiles[ord[j]].ref) == col_start(R0))
+                    ++j;
             const int m = R0.m;
             const int64_t k0 = (int64_t)zmap.size(), v0 = vElems;
             for (size_t k = i; k < j; ++k) {
@@ -885,10 +912,10 @@ void build_group(Context& c, HMatrix& h, MfGroup& G, int type, const std::vector
                 fbytes += 2ull * (uint64_t)cnt_of(t) * m * es;
             }
             if (vdirect)
-                emit_items(k0, (int64_t)zmap.size(), m, R0.inCl,
+                emit_items(k0, (int64_t)zmap.size(), m, col_start(R0),
                            (const char*)(R0.lv->V.p + (size_t)R0.lv->h_offsets[R0.t] * m), 0);
             else
-                emit_items(k0, (int64_t)zmap.size(), m, R0.inCl, nullptr, v0);
+                emit_items(k0, (int64_t)zmap.size(), m, col_start(R0), nullptr, v0);
             i = j;
         }
     }
@@ -969,10 +996,14 @@ void build_group(Context& c, HMatrix& h, MfGroup& G, int type, const std::vector
     HMX_HIP(hipStreamSynchronize(c.stream));
 }
 
+void check_bucket(const HMatrix& h) {
+    HMX_CHECK(h.b == 32 || h.b == 64, HMX_EUNSUPPORTED, "the multi-RHS matvec kernels are built for bucketSize 32 or 64");
+}
+
 void check_shape(const HMatrix& h) {
     HMX_CHECK(h.n == h.S * h.b, HMX_EUNSUPPORTED,
-              "the multi-RHS matvec requires full leaves (n = numSegments * bucketSize)");
-    HMX_CHECK(h.b == 32 || h.b == 64, HMX_EUNSUPPORTED, "the multi-RHS matvec kernels are built for bucketSize 32 or 64");
+              "the mixed-precision matvec requires full leaves (n = numSegments * bucketSize)");
+    check_bucket(h);
 }
 
 // leaf-level tiles of rank above b/2 are applied as dense blocks
@@ -981,15 +1012,15 @@ bool is_near(const HMatrix& h, const TileRef& R) {
 }
 
 template <class T>
-void build_near(Context& c, HMatrix& h, MfPlan& P, const std::vector<TileRef>& tiles) {
-    const int b = (int)h.b, S = (int)h.S;
+void build_near(Context& c, const HMatrix& h, int S, MfPlan& P, const std::vector<TileRef>& tiles) {
+    const int b = (int)h.b;
     std::vector<std::vector<std::pair<int32_t, int64_t>>> perLeaf(S);
     std::vector<NearSrc> src;
     for (const TileRef& R : tiles) {
         if (!is_near(h, R)) continue;
         const double* U = R.lv->U.p + (size_t)R.lv->h_offsets[R.t] * R.m;
         const double* V = R.lv->V.p + (size_t)R.lv->h_offsets[R.t] * R.m;
-        perLeaf[R.outCl].push_back({(int32_t)R.inCl, (int64_t)src.size()});
+        perLeaf[R.leafBase + R.outCl].push_back({(int32_t)(R.leafBase + R.inCl), (int64_t)src.size()});
         src.push_back(NearSrc{U, V, (int64_t)src.size() * b * b, R.r, 0});
     }
     P.numNear = (int64_t)src.size();
@@ -1014,28 +1045,62 @@ void build_near(Context& c, HMatrix& h, MfPlan& P, const std::vector<TileRef>& t
     HMX_HIP(hipStreamSynchronize(c.stream));
 }
 
-void build_fp64_plan(Context& c, HMatrix& h) {
-    check_shape(h);
-    MfPlan& P = h.mf64;
-    std::vector<TileRef> all = all_tiles(h);
-    build_near<double>(c, h, P, all);
+// FP64 matrix-core plan over one or several H-matrices (a batch is their
+// block-diagonal operator): instance i owns leaves [leafBase_i, leafBase_i +
+// S_i) of one padded leaf space.  Vectors come in as n_i consecutive entries
+// per instance; with partial leaves they are gathered into the padded space
+// and the product scattered back.
+void build_fp64_plan_set(Context& c, const std::vector<HMatrix*>& hs, MfPlan& P) {
+    HMX_CHECK(!hs.empty(), HMX_EINVAL, "empty plan set");
+    const HMatrix& h0 = *hs[0];
+    check_bucket(h0);
+    const int b = (int)h0.b;
+    std::vector<TileRef> all;
+    std::vector<int32_t> gm;
+    int64_t leafBase = 0, vecBase = 0;
+    bool grouped = b == 64;
+    for (HMatrix* h : hs) {
+        HMX_CHECK(h->b == (uint32_t)b, HMX_EINVAL, "all instances of a batch need the same bucketSize");
+        HMX_CHECK(h->leafOffsets.size() == (size_t)h->S + 1, HMX_ESTATE, "H-matrix without leaf offsets");
+        std::vector<TileRef> t = all_tiles(*h, leafBase);
+        all.insert(all.end(), t.begin(), t.end());
+        std::vector<int32_t> g = padding_map(*h, vecBase);
+        gm.insert(gm.end(), g.begin(), g.end());
+        grouped = grouped && h->S % 4 == 0;
+        leafBase += h->S;
+        vecBase += h->n;
+    }
+    HMX_CHECK(leafBase * b < (int64_t)INT32_MAX, HMX_EUNSUPPORTED, "plan set too large for 32-bit row offsets");
+    const int S = (int)leafBase;
+    P.S = S;
+    P.b = b;
+    P.n = vecBase;
+    P.npad = leafBase * b;
+    P.gather = P.n != P.npad;
+    if (P.gather) upload(c, P.gmap, gm);
+    build_near<double>(c, h0, S, P, all);
     std::vector<GroupTile> tiles;
     for (const TileRef& R : all) {
-        if (is_near(h, R)) continue;
+        if (is_near(h0, R)) continue;
         GroupTile gt{R, {}};
         gt.cols.resize(R.r);
         std::iota(gt.cols.begin(), gt.cols.end(), 0);
         tiles.push_back(std::move(gt));
     }
-    build_group(c, h, P.g[0], 0, tiles, true, nullptr, nullptr, nullptr);
-    const int64_t tot = (int64_t)h.S * h.b * h.b;
+    build_group(c, h0, S, grouped, P.g[0], 0, tiles, true, nullptr, nullptr, nullptr);
+    const int64_t tot = (int64_t)S * b * b;
     P.Drm.alloc(tot * sizeof(double));
-    k_diag_rowmajor<double><<<(unsigned)((tot + 255) / 256), 256, 0, c.stream>>>(tot, (int)h.b, h.diag.p,
-                                                                                (double*)P.Drm.p);
-    HMX_LAUNCHED();
+    int64_t off = 0;
+    for (HMatrix* h : hs) {
+        const int64_t e = (int64_t)h->S * b * b;
+        k_diag_rowmajor<double><<<(unsigned)((e + 255) / 256), 256, 0, c.stream>>>(e, b, h->diag.p,
+                                                                                  (double*)P.Drm.p + off);
+        HMX_LAUNCHED();
+        off += e;
+    }
     HMX_HIP(hipStreamSynchronize(c.stream));
     P.dtype = 0;
-    P.bytes = P.g[0].factorBytes + (uint64_t)tot * 8 + (uint64_t)P.numNear * h.b * h.b * 8;
+    P.bytes = P.g[0].factorBytes + (uint64_t)tot * 8 + (uint64_t)P.numNear * b * b * 8;
     P.built = true;
 }
 
@@ -1075,10 +1140,10 @@ int phase1_chunk() {
 }
 
 template <class T, int CG, int NB>
-void phase1(Context& c, MfGroup& G, const T* Xb, int sh, const float* rhsScale) {
+void phase1(Context& c, MfGroup& G, const T* Xb, int sh, const float* rhsScale, int xcdChunk = -1) {
     if (G.numItems == 0) return;
     const float* cs = G.colScaleV.n ? G.colScaleV.p : nullptr;
-    const int chunk = phase1_chunk();
+    const int chunk = xcdChunk >= 0 ? xcdChunk : phase1_chunk();
     unsigned blocks = (unsigned)((G.numItems + 3) / 4);
     if (chunk > 0) blocks = (blocks + 8 * chunk - 1) / (8 * chunk) * (8 * chunk);
     k_mf_phase1<T, CG, NB><<<blocks, 256, 0, c.stream>>>(G.items.p, G.numItems, chunk, Xb, sh,
@@ -1105,38 +1170,52 @@ struct Events {
 
 int leaf_shift(const HMatrix& h) { return h.b == 64 ? 6 : 5; }
 
-// X ([s][n]) -> blocked copy, optionally divided by a per-right-hand-side scale
+// X ([s][n], full leaves) -> blocked copy, optionally divided by a
+// per-right-hand-side scale
 template <class TI, class TO>
 void to_blocked(Context& c, const HMatrix& h, int NR, const TI* X, const float* sx, TO* Xb) {
     const int64_t tot = (int64_t)h.n * NR;
-    k_x_blocked<TI, TO><<<(unsigned)((tot + 255) / 256), 256, 0, c.stream>>>(tot, (int)h.n, NR, leaf_shift(h), X, sx,
-                                                                             Xb);
+    k_x_blocked<TI, TO><<<(unsigned)((tot + 255) / 256), 256, 0, c.stream>>>(tot, (int64_t)h.n, NR, leaf_shift(h), X,
+                                                                             (int64_t)h.n, NR, nullptr, sx, Xb);
     HMX_LAUNCHED();
 }
 
+// One pass of up to 16*NB right-hand sides (kk of them real) through an FP64
+// plan: X and Y hold the caller's entries (ld ldx / ldy); with partial leaves
+// X is gathered into the padded space and the product scattered back.
 template <int NB>
-void run_fp64(Context& c, HMatrix& h, const double* X, double* Y, Events& ev) {
-    MfPlan& P = h.mf64;
+void run_fp64(Context& c, MfPlan& P, const double* X, int64_t ldx, int kk, double* Y, int64_t ldy, Events& ev) {
     MfGroup& G = P.g[0];
-    const int ldx = (int)h.n;
+    const int NR = 16 * NB, sh = P.b == 64 ? 6 : 5;
     const double* Xb = (const double*)P.Xb.p;
     ev.rec(c, 0);
-    to_blocked<double, double>(c, h, 16 * NB, X, nullptr, (double*)P.Xb.p);
-    phase1<double, 4, NB>(c, G, Xb, leaf_shift(h), nullptr);  // 4 column groups = the 64 columns of an item
+    const int64_t tot = P.npad * NR;
+    k_x_blocked<double, double><<<(unsigned)((tot + 255) / 256), 256, 0, c.stream>>>(
+        tot, P.npad, NR, sh, X, ldx, kk, P.gather ? P.gmap.p : nullptr, nullptr, (double*)P.Xb.p);
+    HMX_LAUNCHED();
+    phase1<double, 4, NB>(c, G, Xb, sh, nullptr, P.xcdChunk);  // 4 column groups = the 64 columns of an item
     ev.rec(c, 1);
-    reduce_group<double, double>(c, G, 16 * NB, nullptr);
+    reduce_group<double, double>(c, G, NR, nullptr);
     ev.rec(c, 2);
+    double* Yo = P.gather ? (double*)P.Yp.p : Y;
+    const int ldo = (int)(P.gather ? P.npad : ldy);
     if (G.grouped) {
-        k_mf_phase2g<true, double, double, double, NB, double><<<h.S / 4, 256, 0, c.stream>>>(
+        k_mf_phase2g<true, double, double, double, NB, double><<<P.S / 4, 256, 0, c.stream>>>(
             G.leafPtr.p, G.entries.p, G.grpPtr.p, G.grpEntries.p, (const double*)G.zt.p, nullptr, nullptr, nullptr,
-            nullptr, nullptr, (const double*)P.Drm.p, P.nearPtr.p, P.near.p, Xb, Y, ldx);
+            nullptr, nullptr, (const double*)P.Drm.p, P.nearPtr.p, P.near.p, Xb, Yo, ldo, kk);
     } else {
-        auto k2 = h.b == 64 ? k_mf_phase2<64, true, double, double, double, NB, double>
+        auto k2 = P.b == 64 ? k_mf_phase2<64, true, double, double, double, NB, double>
                             : k_mf_phase2<32, true, double, double, double, NB, double>;
-        k2<<<h.S, 256, 0, c.stream>>>(G.leafPtr.p, G.entries.p, (const double*)G.zt.p, G.numColsPad, nullptr,
-                                      nullptr, nullptr, 0, (const double*)P.Drm.p, P.nearPtr.p, P.near.p, Xb, Y, ldx);
+        k2<<<P.S, 256, 0, c.stream>>>(G.leafPtr.p, G.entries.p, (const double*)G.zt.p, G.numColsPad, nullptr,
+                                      nullptr, nullptr, 0, (const double*)P.Drm.p, P.nearPtr.p, P.near.p, Xb, Yo, ldo,
+                                      kk);
     }
     HMX_LAUNCHED();
+    if (P.gather) {
+        k_y_scatter<double><<<(unsigned)((P.npad * kk + 255) / 256), 256, 0, c.stream>>>(P.npad, kk, P.gmap.p,
+                                                                                        (const double*)P.Yp.p, Y, ldy);
+        HMX_LAUNCHED();
+    }
     ev.rec(c, 3);
 }
 
@@ -1165,13 +1244,13 @@ void run_mixed(Context& c, HMatrix& h, const float* X, float* Y, Events& ev) {
         k_mf_phase2g<false, float, float, _Float16, NB, float><<<h.S / 4, 256, 0, c.stream>>>(
             Ghi.leafPtr.p, Ghi.entries.p, Ghi.grpPtr.p, Ghi.grpEntries.p, (const float*)Ghi.zt.p, Glo.leafPtr.p,
             Glo.entries.p, Glo.grpPtr.p, Glo.grpEntries.p, (const _Float16*)Glo.zt.p, (const float*)P.Drm.p,
-            P.nearPtr.p, P.near.p, Xb, Y, n);
+            P.nearPtr.p, P.near.p, Xb, Y, n, NR);
     } else {
         auto k2 = h.b == 64 ? k_mf_phase2<64, false, float, float, _Float16, NB, float>
                             : k_mf_phase2<32, false, float, float, _Float16, NB, float>;
         k2<<<h.S, 256, 0, c.stream>>>(Ghi.leafPtr.p, Ghi.entries.p, (const float*)Ghi.zt.p, Ghi.numColsPad,
                                       Glo.leafPtr.p, Glo.entries.p, (const _Float16*)Glo.zt.p, Glo.numColsPad,
-                                      (const float*)P.Drm.p, P.nearPtr.p, P.near.p, Xb, Y, n);
+                                      (const float*)P.Drm.p, P.nearPtr.p, P.near.p, Xb, Y, n, NR);
     }
     HMX_LAUNCHED();
     ev.rec(c, 3);
@@ -1191,30 +1270,56 @@ void finish_timing(Events& ev, float* acc) {
 
 }  // namespace
 
+// Workspaces of an FP64 plan for passes of NR right-hand sides
+void ensure_fp64_ws(Context& c, MfPlan& P, int NR) {
+    ensure_ws(c, P.g[0], NR, sizeof(double));
+    const size_t xb = (size_t)P.npad * NR * sizeof(double);
+    if (P.Xb.n < xb) P.Xb.alloc(xb);
+    if (P.gather && P.Yp.n < xb) P.Yp.alloc(xb);
+}
+
+// Y = H X for k right-hand sides (ld = the plan's n) in passes of 64 (k a
+// multiple of 64) or 16, the last pass padded with zero columns
+void run_fp64_plan(Context& c, MfPlan& P, uint32_t k, const double* d_in, double* d_out, Events& ev) {
+    const int NB = (k % 64 == 0) ? 4 : 1, NR = 16 * NB;
+    for (uint32_t s0 = 0; s0 < k; s0 += NR) {
+        const int kk = (int)std::min<uint32_t>(NR, k - s0);
+        const double* X = d_in + (size_t)s0 * P.n;
+        double* Y = d_out + (size_t)s0 * P.n;
+        if (NB == 4) run_fp64<4>(c, P, X, P.n, kk, Y, P.n, ev);
+        else run_fp64<1>(c, P, X, P.n, kk, Y, P.n, ev);
+    }
+}
+
 void hmatvec_multi(Context& c, HMatrix& h, uint32_t k, const double* d_in, double* d_out, float* ms, int iters) {
-    HMX_CHECK(k >= 16 && k % 16 == 0, HMX_EINVAL, "multi-RHS matvec: vectorWidth must be a multiple of 16");
+    HMX_CHECK(k >= 1, HMX_EINVAL, "multi-RHS matvec: vectorWidth must be >= 1");
     HMX_CHECK(d_in && d_out && d_in != d_out, HMX_EINVAL, "bad vector pointers");
-    if (!h.mf64.built) build_fp64_plan(c, h);
+    if (!h.mf64.built) build_fp64_plan_set(c, {&h}, h.mf64);
     MfPlan& P = h.mf64;
-    const int NB = (k % 64 == 0) ? 4 : 1;
-    ensure_ws(c, P.g[0], 16 * NB, sizeof(double));
-    if (P.Xb.n < (size_t)h.n * 16 * NB * sizeof(double)) P.Xb.alloc((size_t)h.n * 16 * NB * sizeof(double));
+    ensure_fp64_ws(c, P, (k % 64 == 0) ? 64 : 16);
     Events ev;
     if (ms) ev.create();
     const int reps = ms ? std::max(1, iters) : 1;
     float acc[4] = {0, 0, 0, 0};
     for (int it = 0; it < reps; ++it) {
-        for (uint32_t s0 = 0; s0 < k; s0 += 16 * NB) {
-            const double* X = d_in + (size_t)s0 * h.n;
-            double* Y = d_out + (size_t)s0 * h.n;
-            if (NB == 4) run_fp64<4>(c, h, X, Y, ev);
-            else run_fp64<1>(c, h, X, Y, ev);
-        }
+        run_fp64_plan(c, P, k, d_in, d_out, ev);
         finish_timing(ev, acc);
     }
     if (ms)
         for (int q = 0; q < 4; ++q) ms[q] = acc[q] / reps;
 }
+
+void batch_mf_prepare(Context& c, Batch& B, uint32_t k) {
+    if (!B.mf.built) build_fp64_plan_set(c, B.hs, B.mf);
+    ensure_fp64_ws(c, B.mf, (k % 64 == 0) ? 64 : 16);
+}
+
+void batch_mf_run(Context& c, Batch& B, uint32_t k, const double* d_in, double* d_out) {
+    Events ev;
+    run_fp64_plan(c, B.mf, k, d_in, d_out, ev);
+}
+
+int default_xcd_chunk() { return phase1_chunk(); }
 
 void hmatrix_make_mixed(Context& c, HMatrix& h, double target) {
     check_shape(h);
@@ -1275,7 +1380,8 @@ void hmatrix_make_mixed(Context& c, HMatrix& h, double target) {
         su[j] = um[j] > 0 ? (float)um[j] : 1.f;
         sv[j] = vm[j] > 0 ? (float)vm[j] : 1.f;
     }
-    build_near<float>(c, h, P, tiles);
+    const bool grouped = h.b == 64 && h.S >= 4;
+    build_near<float>(c, h, (int)h.S, P, tiles);
     std::vector<GroupTile> gh, gl;
     int64_t hi = 0, lc = 0;
     for (size_t t = 0; t < tiles.size(); ++t) {
@@ -1291,8 +1397,8 @@ void hmatrix_make_mixed(Context& c, HMatrix& h, double target) {
         gh.push_back(std::move(a));
         gl.push_back(std::move(b2));
     }
-    build_group(c, h, P.g[0], 1, gh, false, nullptr, nullptr, nullptr);
-    build_group(c, h, P.g[1], 2, gl, false, &su, &sv, &colBase);
+    build_group(c, h, (int)h.S, grouped, P.g[0], 1, gh, false, nullptr, nullptr, nullptr);
+    build_group(c, h, (int)h.S, grouped, P.g[1], 2, gl, false, &su, &sv, &colBase);
     const int64_t tot = (int64_t)h.S * h.b * h.b;
     P.Drm.alloc(tot * sizeof(float));
     k_diag_rowmajor<float><<<(unsigned)((tot + 255) / 256), 256, 0, c.stream>>>(tot, (int)h.b, h.diag.p,
