@@ -311,7 +311,7 @@ def batch_main(args):
                                   "half 2D half 3D, l ~ U[0.05, 0.3], leaf 32, eps 1e-6, maxDiam, 8 RHS FP64",
                       "total_points": tot},
            "build_s": round(build_s, 3), "build_stages_s": {k2: round(v, 3) for k2, v in bp.timings.items()},
-           "factor_bytes": int(fac), "tuned_chunk": chunk, "tuned_ms": round(tune_ms, 4),
+           "factor_bytes": int(fac), "tuned_xcd_chunk" if k >= 2 else "tuned_chunk": chunk, "tuned_ms": round(tune_ms, 4),
            "rel_err_vs_dense": {"max": max(errs), "median": float(np.median(errs))}}
     print(json.dumps(out), flush=True)
 

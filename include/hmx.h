@@ -241,6 +241,13 @@ hmx_status hmx_hmatrix_mixed_info(const hmx_hmatrix* h, int64_t* hiCols, int64_t
    instead of partial sums for the reduce.  locality: phase-1 work ordered by
    right-hand-side block.  Drops the built matrix-core plans. */
 hmx_status hmx_hmatrix_set_plan_options(hmx_hmatrix* h, int32_t denseNear, int32_t directZ, int32_t locality);
+/* Storage of the FP64 matrix-core plan's factors (no counterpart in the
+   reference): 0 (default) copies them into the plan's run layout when the copy
+   fits in half the free device memory, else reads the levels in place; 1 in
+   place; 2 copy.  The copy costs the low-rank factor bytes again and makes
+   phase 1 and phase 2 work on longer runs of columns.  A batch takes the first
+   instance's choice.  Drops the built FP64 plan. */
+hmx_status hmx_hmatrix_set_fp64_layout(hmx_hmatrix* h, int32_t layout);
 /* Matrix-core plan summary (plan 0: FP64 multi-RHS, built on the first
    multi-RHS call; plan 1: mixed, built by hmx_hmatrix_make_mixed): bytes read
    per call (factors + dense blocks), leaf-level tiles applied as dense blocks
@@ -305,14 +312,17 @@ hmx_status hmx_generateHMatrixDirectBatch(hmx_context* ctx, int32_t count, const
    must outlive the batch. */
 hmx_status hmx_batch_create(hmx_context* ctx, int32_t count, hmx_hmatrix* const* hmatrices, hmx_batch** batch);
 hmx_status hmx_batch_info(const hmx_batch* batch, int64_t* totalPoints, int64_t* totalLeaves, int32_t* chunk);
-/* FP64, vectorWidth 1 or a multiple of 4/8/16.  useGraph != 0: the launch
+/* FP64, any vectorWidth >= 1 (two or more run on the matrix cores, in passes
+   of 16 or 64 columns).  useGraph != 0: the launch
    sequence is captured in a hipGraph on first use (again when d_in, d_out or
    vectorWidth change) and replayed, ordered after prior work on the context
    stream. */
 hmx_status hmx_batch_matvec(hmx_context* ctx, hmx_batch* batch, uint32_t vectorWidth, const double* d_in,
                             double* d_out, int32_t useGraph);
-/* Autotuner: times the phase-1 chunk lengths 512, 1024, 2048, 4096 at
-   vectorWidth on the batch and keeps the fastest. */
+/* Autotuner for the path vectorWidth takes; keeps the fastest candidate and
+   returns it in bestChunk.  vectorWidth 1: the streaming phase-1 chunk length
+   (512, 1024, 2048 or 4096; also reported by hmx_batch_info).  vectorWidth >= 2:
+   the matrix-core phase-1 workgroups per XCD chunk (0 = plain order, 8, 16, 32). */
 hmx_status hmx_batch_tune(hmx_context* ctx, hmx_batch* batch, uint32_t vectorWidth, int32_t* bestChunk,
                           float* bestMs);
 hmx_status hmx_batch_free(hmx_batch* batch);

@@ -60,6 +60,7 @@ void batch_matvec(Context& c, Batch& B, uint32_t k, const double* X, double* Y, 
         run_batch(c, B, k, X, Y);
         return;
     }
+    if (k >= 2 && batch_mf_stale(B)) B.drop_graph();  // the captured launches point into the old plan
     if (!B.exec || B.gX != X || B.gY != Y || B.gk != k) {
         B.drop_graph();
         // workspaces are allocated here, outside the capture; the captured
@@ -93,55 +94,47 @@ void batch_matvec(Context& c, Batch& B, uint32_t k, const double* X, double* Y, 
     HMX_HIP(hipStreamWaitEvent(c.stream, B.evOut, 0));
 }
 
-// Times the candidates of the path k right-hand sides take: the streaming
-// phase-1 chunk length (reported; it also serves one right-hand side) and,
-// for k >= 2, the matrix-core phase-1 workgroups per XCD chunk.
+// Times the candidates of the path k right-hand sides take and keeps the
+// fastest: one right-hand side, the streaming phase-1 chunk length (512, 1024,
+// 2048, 4096); two or more, the matrix-core phase-1 workgroups per XCD chunk
+// (0 = plain order, 8, 16, 32).
 void batch_tune(Context& c, Batch& B, uint32_t k, int* bestChunk, float* bestMs) {
-    const int cands[] = {512, 1024, 2048, 4096};
     DBuf<double> X((size_t)B.P.n * k), Y((size_t)B.P.n * k);
     HMX_HIP(hipMemsetAsync(X.p, 0, X.bytes(), c.stream));
     hipEvent_t e0, e1;
     HMX_HIP(hipEventCreate(&e0));
     HMX_HIP(hipEventCreate(&e1));
-    int best = 2048;
+    auto time_it = [&](auto&& run) {
+        for (int w = 0; w < 2; ++w) run();
+        HMX_HIP(hipEventRecord(e0, c.stream));
+        const int reps = 5;
+        for (int r = 0; r < reps; ++r) run();
+        HMX_HIP(hipEventRecord(e1, c.stream));
+        HMX_HIP(hipEventSynchronize(e1));
+        float t = 0;
+        HMX_HIP(hipEventElapsedTime(&t, e0, e1));
+        return t / reps;
+    };
+    int best = 0;
     float bms = 1e30f;
     try {
-        for (int ch : cands) {
-            B.drop_graph();
-            build_batch_plan(c, B.hs, ch, B.P, B.diag);
-            for (int w = 0; w < 2; ++w) run_matvec_plan(c, B.P, B.b, k, X.p, B.P.n, Y.p, nullptr);
-            HMX_HIP(hipEventRecord(e0, c.stream));
-            const int reps = 5;
-            for (int r = 0; r < reps; ++r) run_matvec_plan(c, B.P, B.b, k, X.p, B.P.n, Y.p, nullptr);
-            HMX_HIP(hipEventRecord(e1, c.stream));
-            HMX_HIP(hipEventSynchronize(e1));
-            float t = 0;
-            HMX_HIP(hipEventElapsedTime(&t, e0, e1));
-            t /= reps;
-            if (t < bms) { bms = t; best = ch; }
-        }
         B.drop_graph();
-        build_batch_plan(c, B.hs, best, B.P, B.diag);
-        if (k >= 2) {
-            batch_mf_prepare(c, B, k);
-            const int xc[] = {0, 8, 16, 32};
-            int bx = default_xcd_chunk();
-            float bmx = 1e30f;
-            for (int q : xc) {
-                B.mf.xcdChunk = q;
-                for (int w = 0; w < 2; ++w) batch_mf_run(c, B, k, X.p, Y.p);
-                HMX_HIP(hipEventRecord(e0, c.stream));
-                const int reps = 5;
-                for (int r = 0; r < reps; ++r) batch_mf_run(c, B, k, X.p, Y.p);
-                HMX_HIP(hipEventRecord(e1, c.stream));
-                HMX_HIP(hipEventSynchronize(e1));
-                float t = 0;
-                HMX_HIP(hipEventElapsedTime(&t, e0, e1));
-                t /= reps;
-                if (t < bmx) { bmx = t; bx = q; }
+        if (k == 1) {
+            for (int ch : {512, 1024, 2048, 4096}) {
+                build_batch_plan(c, B.hs, ch, B.P, B.diag);
+                ensure_matvec_ws(B.P, 1);
+                float t = time_it([&] { run_matvec_plan(c, B.P, B.b, 1, X.p, B.P.n, Y.p, nullptr); });
+                if (t < bms) { bms = t; best = ch; }
             }
-            B.mf.xcdChunk = bx;
-            bms = bmx;
+            build_batch_plan(c, B.hs, best, B.P, B.diag);
+        } else {
+            batch_mf_prepare(c, B, k);
+            for (int q : {0, 8, 16, 32}) {
+                B.mf.xcdChunk = q;
+                float t = time_it([&] { batch_mf_run(c, B, k, X.p, Y.p); });
+                if (t < bms) { bms = t; best = q; }
+            }
+            B.mf.xcdChunk = best;
         }
     } catch (...) {
         (void)hipEventDestroy(e0);

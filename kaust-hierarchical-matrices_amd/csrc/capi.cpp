@@ -596,6 +596,13 @@ hmx_status hmx_hmatrix_set_plan_options(hmx_hmatrix* h, int32_t denseNear, int32
     return HMX_OK;
 }
 
+hmx_status hmx_hmatrix_set_fp64_layout(hmx_hmatrix* h, int32_t layout) {
+    if (!h || layout < 0 || layout > 2) return HMX_EINVAL;
+    h->planFp64Layout = layout;
+    h->mf64 = MfPlan();
+    return HMX_OK;
+}
+
 hmx_status hmx_hmatrix_plan_info(const hmx_hmatrix* h, int32_t plan, uint64_t* bytes, int64_t* denseBlocks,
                                  int64_t* lowRankColumns) {
     if (!h || (plan != 0 && plan != 1)) return HMX_EINVAL;

@@ -95,6 +95,7 @@ void hmatrix_set_level(Context& c, HMatrix& h, int level, const double* U, const
     }
     HMX_HIP(hipStreamSynchronize(c.stream));
     // converted copies used by the matrix-core paths are stale now
+    ++h.factorGen;
     h.mf64 = MfPlan();
     h.mfMixed = MfPlan();
 }

@@ -225,6 +225,7 @@ struct MfNear {
 struct MfPlan {
     bool built = false;
     int dtype = 0;                 // 0: FP64 plan; 1: mixed FP32/FP16 plan
+    bool ucm = false;              // FP64 plan reads the levels' column-major U and V in place
     MfGroup g[2];                  // FP64: g[0]; mixed: g[0] = f32 hi, g[1] = f16 lo
     DBuf<unsigned char> Drm;       // diagonal blocks, row-major (f64 or f32)
     DBuf<unsigned char> nearStore; // dense leaf-level blocks, row-major (f64 or f32)
@@ -253,20 +254,19 @@ struct MfPlan {
 struct HMatrix {
     uint32_t n = 0, d = 0, b = 0, S = 0;
     int32_t numLevels = 0;
+    uint64_t factorGen = 0;            // bumped whenever factors are overwritten (plans holding copies rebuild)
     std::vector<int32_t> leafOffsets;  // KD-tree segment offsets [S+1]
     DBuf<double> diag;              // [S*b*b]
     std::vector<HLevel> levels;     // index l-1
     const Structure* structure = nullptr;
     Structure structureCopy;        // owned copy (the caller may free theirs)
-    // flattened pools for the matvec (all levels)
-    DBuf<double> Upool, Vpool;
-    std::vector<int64_t> levelPoolOff;  // element offset of each level in the pools
     MatvecPlan plan, planT;
     MfPlan mf64, mfMixed;
     // matrix-core plan choices (hmx_hmatrix_set_plan_options)
     int planDenseNear = 1;   // leaf-level tiles of rank > b/2 as dense blocks
     int planDirectZ = 1;     // phase 1 stores whole-column z itself (else partials + reduce)
     int planLocality = 1;    // phase-1 items ordered by right-hand-side block
+    int planFp64Layout = 0;  // FP64 plan factors: 0 copy when it fits in half the free memory, 1 in place, 2 copy
     uint64_t lowRankBytes = 0, denseBytes = 0, factorEntries = 0;
 };
 
@@ -276,6 +276,7 @@ struct Batch {
     std::vector<HMatrix*> hs;
     MatvecPlan P;                // streaming kernels (one right-hand side)
     MfPlan mf;                   // FP64 matrix cores (two or more), built on first use
+    std::vector<uint64_t> mfGen; // instances' factorGen when mf was built
     DBuf<double> diag;
     int b = 0;
     hipStream_t stream = nullptr;
@@ -327,6 +328,7 @@ void dense_rows(Context& c, uint32_t n, uint32_t d, const double* p, const int32
                 const int32_t* rows, uint32_t k, const double* X, double* Y);
 void hmatvec_multi(Context& c, HMatrix& h, uint32_t k, const double* d_in, double* d_out, float* ms, int iters);
 std::vector<int32_t> padding_map(const HMatrix& h, int64_t base);
+bool batch_mf_stale(const Batch& B);
 void batch_mf_prepare(Context& c, Batch& B, uint32_t k);
 void batch_mf_run(Context& c, Batch& B, uint32_t k, const double* d_in, double* d_out);
 int default_xcd_chunk();

@@ -1,5 +1,5 @@
-// Multi right-hand-side H-matvec on the matrix cores, FP64 and mixed
-// FP32/FP16 (Y = H X with X n x k, k a multiple of 16).
+// Multi right-hand-side H-matvec on the matrix cores, FP64 (any number of
+// right-hand sides, in passes of 16 or 64) and mixed FP32/FP16 (16 or 64).
 //
 //   phase 1  Z = V^T X per tile: one wave per (tile, 64 columns, 1024 rows),
 //            D(16 cols x 16 rhs) += A(16 cols x K rows) B(K rows x 16 rhs);
@@ -7,8 +7,9 @@
 //            one 16/32-byte vector per operand per four MFMAs.
 //   reduce   chunk partials summed in a fixed order, scaled, converted to the
 //            compute type and stored transposed ([rhs][column]).
-//   phase 2  one workgroup per leaf (32 or 64 rows): Y_leaf = D X_leaf + sum_t U_t[leaf rows] Z_t,
-//            A = the leaf's rows of a row-major copy of U, B = transposed Z.
+//   phase 2  one wave per leaf (32 or 64 rows): Y_leaf = D X_leaf + sum_t U_t[leaf rows] Z_t,
+//            A = the leaf's rows of U (row-major copy or the level's
+//            column-major U in place), B = transposed Z.
 //
 // Mixed precision: the columns of every tile are split by a global energy
 // budget.  A column j of tile t contributes ||u_j v_j^T||_F = ||v_j|| (U has
@@ -231,160 +232,26 @@ __global__ void __launch_bounds__(256) k_mf_reduce(int64_t nred, int64_t ldz, co
 }
 
 // ---- phase 2 -----------------------------------------------------------------
-// One workgroup of 4 waves per leaf of BL rows (BL = 32 or 64).  The leaf's
-// rows form RG = BL/16 row groups of 16; wave w owns row group w % RG and
-// every KP-th super-step of the K (column) loop, KP = 4/RG, with w / RG
-// picking which; partials are folded through LDS at the end.
-template <int BL, bool UCM, class TD, class TA, class TB, int NB, class TO>
-__global__ void __launch_bounds__(256) k_mf_phase2(const int32_t* __restrict__ ptrA, const MfEntry* __restrict__ entA,
-                                                   const TA* __restrict__ ZA, int64_t ldzA,
-                                                   const int32_t* __restrict__ ptrB, const MfEntry* __restrict__ entB,
-                                                   const TB* __restrict__ ZB, int64_t ldzB,
-                                                   const TD* __restrict__ Drm, const int32_t* __restrict__ nearPtr,
-                                                   const MfNear* __restrict__ nearB, const TD* __restrict__ Xb,
-                                                   TO* __restrict__ Y, int ldy, int kvalid) {
-    using MA = Mfma<TA>;
-    using MB = Mfma<TB>;
-    using MD = Mfma<TD>;
-    using acc_t = typename MA::acc_t;
-    using accv = typename MA::accv;
-    static_assert(sizeof(typename MB::acc_t) == sizeof(acc_t) && sizeof(typename MD::acc_t) == sizeof(acc_t),
-                  "one accumulator type per launch");
-    static_assert(BL == 32 || BL == 64, "bucketSize 32 or 64");
-    constexpr int NR = 16 * NB, RG = BL / 16, KP = 4 / RG;
-    __shared__ acc_t red[KP][BL][NR + 1];
-    const int leaf = xcd_local(blockIdx.x, gridDim.x);
-    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int rg = w % RG, kp = w / RG, lg = lane >> 4, lc = lane & 15;
-    const int row = 16 * rg + lc;
-    accv acc[NB];
-#pragma unroll
-    for (int nb = 0; nb < NB; ++nb) acc[nb] = accv{0, 0, 0, 0};
-    // dense blocks (row-major BL x BL): the diagonal block, then the leaf's
-    // dense near-field blocks.  Each wave takes BL/(16*KP) super-steps of 16
-    // columns per block; all super-steps of all blocks form one sequence whose
-    // next step's operands are loaded while the current step's MFMAs run.
-    {
-        constexpr int SPB = BL / (16 * KP);  // super-steps per block for this wave
-        const int n0 = nearPtr ? nearPtr[leaf] : 0, n1 = nearPtr ? nearPtr[leaf + 1] : 0;
-        const int total = (n1 - n0 + 1) * SPB;
-        TD a[4], bx[NB][4], an[4], bn[NB][4];
-        auto dload = [&](int idx, TD (&A)[4], TD (&B)[NB][4]) {
-            const int q = n0 - 1 + idx / SPB;
-            const int cc = 16 * kp + (idx % SPB) * 16 * KP + 4 * lg;
-            const TD* Dq = q < n0 ? Drm + (size_t)leaf * BL * BL : (const TD*)nearB[q].d;
-            const int cl = q < n0 ? leaf : nearB[q].colLeaf;
-            vload_f(Dq + row * BL + cc, A);
-#pragma unroll
-            for (int nb = 0; nb < NB; ++nb) vload(Xb + ((size_t)cl * NR + 16 * nb + lc) * BL + cc, B[nb]);
-        };
-        dload(0, a, bx);
-        for (int idx = 0; idx < total; ++idx) {
-            const bool more = idx + 1 < total;
-            if (more) dload(idx + 1, an, bn);
-#pragma unroll
-            for (int nb = 0; nb < NB; ++nb)
-#pragma unroll
-                for (int e = 0; e < 4; ++e) acc[nb] = MD::op(a[e], bx[nb][e], acc[nb]);
-            if (more) {
-#pragma unroll
-                for (int e = 0; e < 4; ++e) a[e] = an[e];
-#pragma unroll
-                for (int nb = 0; nb < NB; ++nb)
-#pragma unroll
-                    for (int e = 0; e < 4; ++e) bx[nb][e] = bn[nb][e];
-            }
-        }
-    }
-    {
-        constexpr int KL = MA::KL, VL = 4 * KL, SS = 16 * KL;
-        for (int e = ptrA[leaf]; e < ptrA[leaf + 1]; ++e) {
-            const MfEntry en = entA[e];
-            const TA* __restrict__ U = UCM ? (const TA*)en.u + row : (const TA*)en.u + (size_t)row * en.rpad;
-            const TA* __restrict__ Z = ZA;
-            for (int c0 = kp * SS; c0 < en.rpad; c0 += KP * SS) {
-                const int cc = c0 + VL * lg;
-                TA a[VL], bz[NB][VL];
-                if (cc < en.rpad) {
-                    if (UCM) {  // column-major U in place: one element per column, zero past the rank
-#pragma unroll
-                        for (int q = 0; q < VL; ++q)
-                            a[q] = cc + q < en.rank ? __builtin_nontemporal_load(U + (size_t)(cc + q) * en.ld) : (TA)0;
-                    } else {
-                        vload_f(U + cc, a);
-                    }
-#pragma unroll
-                    for (int nb = 0; nb < NB; ++nb) vload(Z + zt_index<VL, NR>(en.zcol + cc, 16 * nb + lc), bz[nb]);
-                } else {
-                    vzero(a);
-#pragma unroll
-                    for (int nb = 0; nb < NB; ++nb) vzero(bz[nb]);
-                }
-#pragma unroll
-                for (int q = 0; q < 4; ++q)
-#pragma unroll
-                    for (int nb = 0; nb < NB; ++nb)
-                        acc[nb] = MA::op(MA::make(&a[q * KL]), MA::make(&bz[nb][q * KL]), acc[nb]);
-            }
-        }
-    }
-    if (ptrB) {
-        constexpr int KL = MB::KL, VL = 4 * KL, SS = 16 * KL;
-        for (int e = ptrB[leaf]; e < ptrB[leaf + 1]; ++e) {
-            const MfEntry en = entB[e];
-            const TB* __restrict__ U = (const TB*)en.u + (size_t)row * en.rpad;
-            const TB* __restrict__ Z = ZB;
-            for (int c0 = kp * SS; c0 < en.rpad; c0 += KP * SS) {
-                const int cc = c0 + VL * lg;
-                TB a[VL], bz[NB][VL];
-                if (cc < en.rpad) {
-                    vload_f(U + cc, a);
-#pragma unroll
-                    for (int nb = 0; nb < NB; ++nb) vload(Z + zt_index<VL, NR>(en.zcol + cc, 16 * nb + lc), bz[nb]);
-                } else {
-                    vzero(a);
-#pragma unroll
-                    for (int nb = 0; nb < NB; ++nb) vzero(bz[nb]);
-                }
-#pragma unroll
-                for (int q = 0; q < 4; ++q)
-#pragma unroll
-                    for (int nb = 0; nb < NB; ++nb)
-                        acc[nb] = MB::op(MB::make(&a[q * KL]), MB::make(&bz[nb][q * KL]), acc[nb]);
-            }
-        }
-    }
-#pragma unroll
-    for (int nb = 0; nb < NB; ++nb)
-#pragma unroll
-        for (int v = 0; v < 4; ++v) red[kp][16 * rg + MA::out_row(lg, v)][16 * nb + lc] = acc[nb][v];
-    __syncthreads();
-    for (int o = threadIdx.x; o < BL * NR; o += 256) {
-        const int s = o / BL, r = o % BL;
-        acc_t v = red[0][r][s];
-#pragma unroll
-        for (int q = 1; q < KP; ++q) v += red[q][r][s];
-        if (s < kvalid) Y[(size_t)s * ldy + (size_t)leaf * BL + r] = (TO)v;
-    }
-}
-
-// ---- phase 2, grouped (64-row leaves) ------------------------------------------
-// One wave's 64 rows (one leaf) times a list of low-rank entries:
+// One wave per leaf of BL rows (RG = BL/16 row groups): all its rows, all
+// right-hand sides, every contribution, accumulated in registers and stored
+// once -- no partial sums across waves, no LDS.
+//
+// One wave's BL rows times a list of low-rank entries:
 // acc[rg][nb] += U[16rg.., cols] Z[cols, rhs 16nb..].  The operands of the
 // next 16*KL-column step (possibly of the next entry) load while the current
 // step's MFMAs run.  rowOff: the wave's first row inside the entries' U.
-template <bool UCM, class T, int NB>
-__device__ inline void lr_rows64(const MfEntry* __restrict__ ents, int e0, int e1, const T* __restrict__ Z,
-                                 int rowOff, int lg, int lc, typename Mfma<T>::accv (&acc)[4][NB]) {
+template <int BL, bool UCM, class T, int NB>
+__device__ inline void lr_rows(const MfEntry* __restrict__ ents, int e0, int e1, const T* __restrict__ Z, int rowOff,
+                               int lg, int lc, typename Mfma<T>::accv (&acc)[BL / 16][NB]) {
     using M = Mfma<T>;
-    constexpr int KL = M::KL, VL = 4 * KL, SS = 16 * KL, NR = 16 * NB;
+    constexpr int KL = M::KL, VL = 4 * KL, SS = 16 * KL, NR = 16 * NB, RG = BL / 16;
     if (e0 >= e1) return;
-    T a[4][VL], bz[NB][VL], an[4][VL], bn[NB][VL];
-    auto load = [&](const MfEntry& en, int c0, T (&A)[4][VL], T (&B)[NB][VL]) {
+    T a[RG][VL], bz[NB][VL], an[RG][VL], bn[NB][VL];
+    auto load = [&](const MfEntry& en, int c0, T (&A)[RG][VL], T (&B)[NB][VL]) {
         const int cc = c0 + VL * lg;
         if (cc < en.rpad) {
 #pragma unroll
-            for (int rg = 0; rg < 4; ++rg) {
+            for (int rg = 0; rg < RG; ++rg) {
                 const int row = rowOff + 16 * rg + lc;
                 if (UCM) {  // column-major U in place: one element per column, zero past the rank
                     const T* U = (const T*)en.u + row;
@@ -399,7 +266,7 @@ __device__ inline void lr_rows64(const MfEntry* __restrict__ ents, int e0, int e
             for (int nb = 0; nb < NB; ++nb) vload(Z + zt_index<VL, NR>(en.zcol + cc, 16 * nb + lc), B[nb]);
         } else {
 #pragma unroll
-            for (int rg = 0; rg < 4; ++rg) vzero(A[rg]);
+            for (int rg = 0; rg < RG; ++rg) vzero(A[rg]);
 #pragma unroll
             for (int nb = 0; nb < NB; ++nb) vzero(B[nb]);
         }
@@ -420,13 +287,13 @@ __device__ inline void lr_rows64(const MfEntry* __restrict__ ents, int e0, int e
 #pragma unroll
         for (int q = 0; q < 4; ++q)
 #pragma unroll
-            for (int rg = 0; rg < 4; ++rg)
+            for (int rg = 0; rg < RG; ++rg)
 #pragma unroll
                 for (int nb = 0; nb < NB; ++nb)
                     acc[rg][nb] = M::op(M::make(&a[rg][q * KL]), M::make(&bz[nb][q * KL]), acc[rg][nb]);
         if (!more) break;
 #pragma unroll
-        for (int rg = 0; rg < 4; ++rg)
+        for (int rg = 0; rg < RG; ++rg)
 #pragma unroll
             for (int q = 0; q < VL; ++q) a[rg][q] = an[rg][q];
 #pragma unroll
@@ -439,31 +306,32 @@ __device__ inline void lr_rows64(const MfEntry* __restrict__ ents, int e0, int e
     }
 }
 
-// One wave's leaf times its dense 64x64 blocks (the diagonal block, then its
-// near-field blocks), software pipelined over steps of 4*EPL columns.  A lane
-// loads EPL = 32 bytes of a block row and of an X row per step, so the four
-// lane groups of a row read one whole 128-byte line at once (an FP32 row
+// One wave's leaf times its dense BL x BL blocks (the diagonal block, then
+// its near-field blocks), software pipelined over steps of 4*EPL columns.  A
+// lane loads EPL = 32 bytes of a block row and of an X row per step, so the
+// four lane groups of a row read one whole 128-byte line at once (an FP32 row
 // read 64 bytes at a time was fetched twice).  Temporal loads: a block's
 // lines are reused by the next step.
-template <class TD, int NB>
-__device__ inline void dense_rows64(int leaf, const TD* __restrict__ Drm, const int32_t* __restrict__ nearPtr,
-                                    const MfNear* __restrict__ nearB, const TD* __restrict__ Xb, int lg, int lc,
-                                    typename Mfma<TD>::accv (&acc)[4][NB]) {
+template <int BL, class TD, int NB>
+__device__ inline void dense_rows(int leaf, const TD* __restrict__ Drm, const int32_t* __restrict__ nearPtr,
+                                  const MfNear* __restrict__ nearB, const TD* __restrict__ Xb, int lg, int lc,
+                                  typename Mfma<TD>::accv (&acc)[BL / 16][NB]) {
     using MD = Mfma<TD>;
     static_assert(MD::KL == 1, "dense blocks on the 16x16x4 shapes");
-    constexpr int NR = 16 * NB, EPL = 32 / sizeof(TD), SPB = 64 / (4 * EPL);
+    constexpr int NR = 16 * NB, EPL = 32 / sizeof(TD), SPB = BL / (4 * EPL), RG = BL / 16;
+    static_assert(SPB >= 1, "a step spans at most one block row");
     const int n0 = nearPtr ? nearPtr[leaf] : 0, n1 = nearPtr ? nearPtr[leaf + 1] : 0;
     const int total = (n1 - n0 + 1) * SPB;
-    TD a[4][EPL], bx[NB][EPL], an[4][EPL], bn[NB][EPL];
-    auto dload = [&](int idx, TD (&A)[4][EPL], TD (&B)[NB][EPL]) {
+    TD a[RG][EPL], bx[NB][EPL], an[RG][EPL], bn[NB][EPL];
+    auto dload = [&](int idx, TD (&A)[RG][EPL], TD (&B)[NB][EPL]) {
         const int q = n0 - 1 + idx / SPB;
         const int cc = (idx % SPB) * 4 * EPL + EPL * lg;
-        const TD* Dq = q < n0 ? Drm + (size_t)leaf * 64 * 64 : (const TD*)nearB[q].d;
+        const TD* Dq = q < n0 ? Drm + (size_t)leaf * BL * BL : (const TD*)nearB[q].d;
         const int cl = q < n0 ? leaf : nearB[q].colLeaf;
 #pragma unroll
-        for (int rg = 0; rg < 4; ++rg) vload(Dq + (16 * rg + lc) * 64 + cc, A[rg]);
+        for (int rg = 0; rg < RG; ++rg) vload(Dq + (16 * rg + lc) * BL + cc, A[rg]);
 #pragma unroll
-        for (int nb = 0; nb < NB; ++nb) vload(Xb + ((size_t)cl * NR + 16 * nb + lc) * 64 + cc, B[nb]);
+        for (int nb = 0; nb < NB; ++nb) vload(Xb + ((size_t)cl * NR + 16 * nb + lc) * BL + cc, B[nb]);
     };
     dload(0, a, bx);
     for (int idx = 0; idx < total; ++idx) {
@@ -472,12 +340,12 @@ __device__ inline void dense_rows64(int leaf, const TD* __restrict__ Drm, const 
 #pragma unroll
         for (int e = 0; e < EPL; ++e)
 #pragma unroll
-            for (int rg = 0; rg < 4; ++rg)
+            for (int rg = 0; rg < RG; ++rg)
 #pragma unroll
                 for (int nb = 0; nb < NB; ++nb) acc[rg][nb] = MD::op(a[rg][e], bx[nb][e], acc[rg][nb]);
         if (more) {
 #pragma unroll
-            for (int rg = 0; rg < 4; ++rg)
+            for (int rg = 0; rg < RG; ++rg)
 #pragma unroll
                 for (int e = 0; e < EPL; ++e) a[rg][e] = an[rg][e];
 #pragma unroll
@@ -488,15 +356,14 @@ __device__ inline void dense_rows64(int leaf, const TD* __restrict__ Drm, const 
     }
 }
 
-// One workgroup of 4 waves per group of 4 consecutive 64-row leaves; wave w
-// owns leaf 4g+w -- all its rows, all right-hand sides, every contribution:
-// its dense blocks and the low-rank tiles of at most two leaves (private
-// entries, per leaf), then the tiles spanning the whole group (shared
-// entries, per group).  The four waves walk the shared list in the same
-// order, so a tile's z block comes from HBM once per group and reaches the
-// other waves from the caches.
-template <bool UCM, class TD, class TA, class TB, int NB, class TO>
-__global__ void __launch_bounds__(256) k_mf_phase2g(const int32_t* __restrict__ ptrA, const MfEntry* __restrict__ entA,
+// One workgroup of 4 waves per group of 4 consecutive leaves (S leaves in
+// all); wave w owns leaf 4g+w: its dense blocks and its entries (ptr, per
+// leaf), then -- grouped plans (64-row leaves) -- the tiles spanning the
+// whole group (gptr, per group; null otherwise).  The four waves walk the
+// shared list in the same order, so a tile's z block comes from HBM once per
+// group and reaches the other waves from the caches.
+template <int BL, bool UCM, class TD, class TA, class TB, int NB, class TO>
+__global__ void __launch_bounds__(256) k_mf_phase2g(int S, const int32_t* __restrict__ ptrA, const MfEntry* __restrict__ entA,
                                                     const int32_t* __restrict__ gptrA,
                                                     const MfEntry* __restrict__ gentA, const TA* __restrict__ ZA,
                                                     const int32_t* __restrict__ ptrB, const MfEntry* __restrict__ entB,
@@ -510,29 +377,31 @@ __global__ void __launch_bounds__(256) k_mf_phase2g(const int32_t* __restrict__ 
     static_assert(sizeof(typename Mfma<TB>::acc_t) == sizeof(typename MA::acc_t) &&
                       sizeof(typename Mfma<TD>::acc_t) == sizeof(typename MA::acc_t),
                   "one accumulator type per launch");
+    constexpr int RG = BL / 16;
     const int grp = xcd_local(blockIdx.x, gridDim.x);
     const int w = threadIdx.x >> 6, lane = threadIdx.x & 63, lg = lane >> 4, lc = lane & 15;
     const int leaf = 4 * grp + w;
-    accv acc[4][NB];
+    if (leaf >= S) return;  // last group of a plan whose leaf count is not a multiple of 4 (no barriers below)
+    accv acc[RG][NB];
 #pragma unroll
-    for (int rg = 0; rg < 4; ++rg)
+    for (int rg = 0; rg < RG; ++rg)
 #pragma unroll
         for (int nb = 0; nb < NB; ++nb) acc[rg][nb] = accv{0, 0, 0, 0};
-    dense_rows64<TD, NB>(leaf, Drm, nearPtr, nearB, Xb, lg, lc, acc);
-    lr_rows64<UCM, TA, NB>(entA, ptrA[leaf], ptrA[leaf + 1], ZA, 0, lg, lc, acc);
-    lr_rows64<UCM, TA, NB>(gentA, gptrA[grp], gptrA[grp + 1], ZA, 64 * w, lg, lc, acc);
+    dense_rows<BL, TD, NB>(leaf, Drm, nearPtr, nearB, Xb, lg, lc, acc);
+    lr_rows<BL, UCM, TA, NB>(entA, ptrA[leaf], ptrA[leaf + 1], ZA, 0, lg, lc, acc);
+    if (gptrA) lr_rows<BL, UCM, TA, NB>(gentA, gptrA[grp], gptrA[grp + 1], ZA, BL * w, lg, lc, acc);
     if (ptrB) {
-        lr_rows64<false, TB, NB>(entB, ptrB[leaf], ptrB[leaf + 1], ZB, 0, lg, lc, acc);
-        lr_rows64<false, TB, NB>(gentB, gptrB[grp], gptrB[grp + 1], ZB, 64 * w, lg, lc, acc);
+        lr_rows<BL, false, TB, NB>(entB, ptrB[leaf], ptrB[leaf + 1], ZB, 0, lg, lc, acc);
+        if (gptrB) lr_rows<BL, false, TB, NB>(gentB, gptrB[grp], gptrB[grp + 1], ZB, BL * w, lg, lc, acc);
     }
 #pragma unroll
-    for (int rg = 0; rg < 4; ++rg)
+    for (int rg = 0; rg < RG; ++rg)
 #pragma unroll
         for (int nb = 0; nb < NB; ++nb)
 #pragma unroll
             for (int v = 0; v < 4; ++v)
                 if (16 * nb + lc < kvalid)
-                    Y[(size_t)(16 * nb + lc) * ldy + (size_t)leaf * 64 + 16 * rg + MA::out_row(lg, v)] = (TO)acc[rg][nb][v];
+                    Y[(size_t)(16 * nb + lc) * ldy + (size_t)leaf * BL + 16 * rg + MA::out_row(lg, v)] = (TO)acc[rg][nb][v];
 }
 
 // ---- right-hand-side conversion (mixed) ---------------------------------------
@@ -1087,7 +956,20 @@ void build_fp64_plan_set(Context& c, const std::vector<HMatrix*>& hs, MfPlan& P)
         std::iota(gt.cols.begin(), gt.cols.end(), 0);
         tiles.push_back(std::move(gt));
     }
-    build_group(c, h0, S, grouped, P.g[0], 0, tiles, true, nullptr, nullptr, nullptr);
+    // factors in place (column-major, one phase-1 item run and one phase-2
+    // entry per tile) or copied into the run layout (V by column cluster, U
+    // row-major by row cluster: fewer, longer items and entries, whole-vector
+    // U loads) when the copy fits
+    bool copy = h0.planFp64Layout == 2;
+    if (h0.planFp64Layout == 0) {
+        uint64_t est = 0;
+        for (const GroupTile& gt : tiles) est += 2ull * (uint64_t)(gt.ref.r + kPad) * gt.ref.m * sizeof(double);
+        size_t fr = 0, totMem = 0;
+        HMX_HIP(hipMemGetInfo(&fr, &totMem));
+        copy = est <= fr / 2;
+    }
+    P.ucm = !copy;
+    build_group(c, h0, S, grouped, P.g[0], 0, tiles, P.ucm, nullptr, nullptr, nullptr);
     const int64_t tot = (int64_t)S * b * b;
     P.Drm.alloc(tot * sizeof(double));
     int64_t off = 0;
@@ -1199,16 +1081,16 @@ void run_fp64(Context& c, MfPlan& P, const double* X, int64_t ldx, int kk, doubl
     ev.rec(c, 2);
     double* Yo = P.gather ? (double*)P.Yp.p : Y;
     const int ldo = (int)(P.gather ? P.npad : ldy);
-    if (G.grouped) {
-        k_mf_phase2g<true, double, double, double, NB, double><<<P.S / 4, 256, 0, c.stream>>>(
-            G.leafPtr.p, G.entries.p, G.grpPtr.p, G.grpEntries.p, (const double*)G.zt.p, nullptr, nullptr, nullptr,
-            nullptr, nullptr, (const double*)P.Drm.p, P.nearPtr.p, P.near.p, Xb, Yo, ldo, kk);
-    } else {
-        auto k2 = P.b == 64 ? k_mf_phase2<64, true, double, double, double, NB, double>
-                            : k_mf_phase2<32, true, double, double, double, NB, double>;
-        k2<<<P.S, 256, 0, c.stream>>>(G.leafPtr.p, G.entries.p, (const double*)G.zt.p, G.numColsPad, nullptr,
-                                      nullptr, nullptr, 0, (const double*)P.Drm.p, P.nearPtr.p, P.near.p, Xb, Yo, ldo,
-                                      kk);
+    {
+        const int* gp = G.grouped ? G.grpPtr.p : nullptr;
+        const MfEntry* ge = G.grouped ? G.grpEntries.p : nullptr;
+        auto k2 = P.b == 64 ? (P.ucm ? k_mf_phase2g<64, true, double, double, double, NB, double>
+                                     : k_mf_phase2g<64, false, double, double, double, NB, double>)
+                            : (P.ucm ? k_mf_phase2g<32, true, double, double, double, NB, double>
+                                     : k_mf_phase2g<32, false, double, double, double, NB, double>);
+        k2<<<(P.S + 3) / 4, 256, 0, c.stream>>>(P.S, G.leafPtr.p, G.entries.p, gp, ge, (const double*)G.zt.p,
+                                                nullptr, nullptr, nullptr, nullptr, nullptr, (const double*)P.Drm.p,
+                                                P.nearPtr.p, P.near.p, Xb, Yo, ldo, kk);
     }
     HMX_LAUNCHED();
     if (P.gather) {
@@ -1240,17 +1122,15 @@ void run_mixed(Context& c, HMatrix& h, const float* X, float* Y, Events& ev) {
     reduce_group<float, float>(c, Ghi, NR, nullptr);
     reduce_group<float, _Float16>(c, Glo, NR, P.xscale.p);
     ev.rec(c, 2);
-    if (Ghi.grouped && Glo.grouped) {
-        k_mf_phase2g<false, float, float, _Float16, NB, float><<<h.S / 4, 256, 0, c.stream>>>(
-            Ghi.leafPtr.p, Ghi.entries.p, Ghi.grpPtr.p, Ghi.grpEntries.p, (const float*)Ghi.zt.p, Glo.leafPtr.p,
-            Glo.entries.p, Glo.grpPtr.p, Glo.grpEntries.p, (const _Float16*)Glo.zt.p, (const float*)P.Drm.p,
-            P.nearPtr.p, P.near.p, Xb, Y, n, NR);
-    } else {
-        auto k2 = h.b == 64 ? k_mf_phase2<64, false, float, float, _Float16, NB, float>
-                            : k_mf_phase2<32, false, float, float, _Float16, NB, float>;
-        k2<<<h.S, 256, 0, c.stream>>>(Ghi.leafPtr.p, Ghi.entries.p, (const float*)Ghi.zt.p, Ghi.numColsPad,
-                                      Glo.leafPtr.p, Glo.entries.p, (const _Float16*)Glo.zt.p, Glo.numColsPad,
-                                      (const float*)P.Drm.p, P.nearPtr.p, P.near.p, Xb, Y, n, NR);
+    {
+        const bool gr = Ghi.grouped && Glo.grouped;
+        auto k2 = h.b == 64 ? k_mf_phase2g<64, false, float, float, _Float16, NB, float>
+                            : k_mf_phase2g<32, false, float, float, _Float16, NB, float>;
+        k2<<<(h.S + 3) / 4, 256, 0, c.stream>>>(
+            (int)h.S, Ghi.leafPtr.p, Ghi.entries.p, gr ? Ghi.grpPtr.p : nullptr, gr ? Ghi.grpEntries.p : nullptr,
+            (const float*)Ghi.zt.p, Glo.leafPtr.p, Glo.entries.p, gr ? Glo.grpPtr.p : nullptr,
+            gr ? Glo.grpEntries.p : nullptr, (const _Float16*)Glo.zt.p, (const float*)P.Drm.p, P.nearPtr.p, P.near.p,
+            Xb, Y, n, NR);
     }
     HMX_LAUNCHED();
     ev.rec(c, 3);
@@ -1309,8 +1189,26 @@ void hmatvec_multi(Context& c, HMatrix& h, uint32_t k, const double* d_in, doubl
         for (int q = 0; q < 4; ++q) ms[q] = acc[q] / reps;
 }
 
+bool batch_mf_stale(const Batch& B) {
+    if (!B.mf.built) return false;
+    for (size_t i = 0; i < B.hs.size(); ++i)
+        if (B.hs[i]->factorGen != B.mfGen[i]) return true;
+    return false;
+}
+
+// the FP64 plan of a batch, rebuilt when an instance's factors were
+// overwritten since (the plan may hold copies of them)
 void batch_mf_prepare(Context& c, Batch& B, uint32_t k) {
-    if (!B.mf.built) build_fp64_plan_set(c, B.hs, B.mf);
+    if (batch_mf_stale(B)) {
+        const int xc = B.mf.xcdChunk;
+        B.mf = MfPlan();
+        B.mf.xcdChunk = xc;
+    }
+    if (!B.mf.built) {
+        build_fp64_plan_set(c, B.hs, B.mf);
+        B.mfGen.clear();
+        for (const HMatrix* h : B.hs) B.mfGen.push_back(h->factorGen);
+    }
     ensure_fp64_ws(c, B.mf, (k % 64 == 0) ? 64 : 16);
 }
 

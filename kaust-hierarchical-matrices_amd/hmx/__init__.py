@@ -106,6 +106,7 @@ _SIGS = [
     ("hmx_hmatrix_set_level", _i32, [_vp, _vp, _i32, _vp, _vp]),
     ("hmx_hmatrix_plan_info", _i32, [_vp, _i32, _P(_u64), _P(_i64), _P(_i64)]),
     ("hmx_hmatrix_set_plan_options", _i32, [_vp, _i32, _i32, _i32]),
+    ("hmx_hmatrix_set_fp64_layout", _i32, [_vp, _i32]),
     ("hmx_hmatrix_save", _i32, [_vp, _vp, ctypes.c_char_p]),
     ("hmx_hmatrix_load", _i32, [_vp, ctypes.c_char_p, _P(_vp)]),
     ("hmx_generateHMatrixDirectBatch", _i32, [_vp, _i32, _vp, _vp, _vp, _vp, _vp]),
@@ -362,6 +363,10 @@ class HMatrix(_Obj):
 
     def set_plan_options(self, dense_near=0, direct_z=0, locality=0):
         _check(lib().hmx_hmatrix_set_plan_options(self.handle, int(dense_near), int(direct_z), int(locality)))
+
+    def set_fp64_layout(self, layout):
+        """FP64 matrix-core plan factors: 0 copy when it fits (default), 1 in place, 2 copy."""
+        _check(lib().hmx_hmatrix_set_fp64_layout(self.handle, int(layout)))
 
     def plan_info(self, plan):
         """plan 0: FP64 matrix-core plan, 1: mixed plan (bytes per call, dense blocks, low-rank columns)."""

@@ -120,8 +120,10 @@ def test_batch_tune_and_dense_error(torch_cuda, ctx, batch):
     k = 8
     X = torch_cuda.randn(k, tot, dtype=torch_cuda.float64, device="cuda")
     before = B.matvec(X)
-    chunk, ms = B.tune(k)
-    assert chunk in (512, 1024, 2048, 4096) and ms > 0
+    xc, ms = B.tune(k)
+    assert xc in (0, 8, 16, 32) and ms > 0
+    chunk, ms1 = B.tune(1)
+    assert chunk in (512, 1024, 2048, 4096) and ms1 > 0
     assert B.info()["chunk"] == chunk
     after = B.matvec(X)
     assert float(torch_cuda.linalg.norm(after - before) / torch_cuda.linalg.norm(before)) < 1e-14
@@ -188,3 +190,67 @@ def test_batched_build_matches_instances(torch_cuda, ctx):
                              X[:, a:e].contiguous(), yd)
         rel = float(torch_cuda.linalg.norm(Y[:, a:e][:, rows.long()] - yd) / torch_cuda.linalg.norm(yd))
         assert rel < 1e-5, (i, rel)
+
+
+def small_batch(torch_cuda, ctx, layout, seed=23):
+    specs = cloud_specs(4, seed, 1500, 6000)
+    pipes = []
+    for i, (n, d, ell) in enumerate(specs):
+        p = O.generate_points(n, d, 300 + i)
+        pp = hmx.Pipeline(n, d, 32, 1e-6, adm="weak", ell=ell, max_rank_policy=hmx.MAXRANK_FULL,
+                          points=to_dev(torch_cuda, p), ctx=ctx, direct=True)
+        pp.H.set_fp64_layout(layout)
+        pipes.append(pp)
+    return specs, pipes, hmx.Batch(ctx, [pp.H for pp in pipes])
+
+
+@pytest.mark.parametrize("k", [5, 64])
+def test_batch_fp64_layouts_agree(torch_cuda, ctx, k):
+    """Factors read in place (one item run and one entry per tile) and copied
+    into the run layout give the same product up to summation order; within
+    one layout the batch equals each instance bit for bit."""
+    out = {}
+    for layout in (1, 2):
+        specs, pipes, B = small_batch(torch_cuda, ctx, layout)
+        tot = B.info()["totalPoints"]
+        X = torch_cuda.randn(k, tot, dtype=torch_cuda.float64, device="cuda",
+                             generator=torch_cuda.Generator("cuda").manual_seed(4))
+        Y = B.matvec(X)
+        for i, pp in enumerate(pipes):
+            a, e = B.offsets[i], B.offsets[i + 1]
+            assert torch_cuda.equal(Y[:, a:e], pp.matvec(X[:, a:e].contiguous())), (layout, i)
+        out[layout] = Y
+    rel = float(torch_cuda.linalg.norm(out[1] - out[2]) / torch_cuda.linalg.norm(out[1]))
+    print(f"k={k}: in place vs copy rel diff {rel:.2e}")
+    assert rel < 1e-13
+
+
+def test_batch_plan_follows_factor_updates(torch_cuda, ctx):
+    """Overwriting an instance's level (hmx_hmatrix_set_level) after the
+    batch's copied FP64 plan was built: the batch, eager and replayed, applies
+    the new factors."""
+    specs, pipes, B = small_batch(torch_cuda, ctx, 2, seed=29)
+    tot = B.info()["totalPoints"]
+    k = 8
+    X = torch_cuda.randn(k, tot, dtype=torch_cuda.float64, device="cuda")
+    Y = torch_cuda.empty_like(X)
+    B.matvec(X, Y, graph=True)
+    before = Y.clone()
+    H = pipes[0].H
+    lvl = next(l for l in range(1, H.num_levels) if H.level_info(l)["rankSum"] > 0)
+    m = (1 << (H.num_levels - 1 - lvl)) * H.b
+    rs = H.level_info(lvl)["rankSum"]
+    u = torch_cuda.empty(rs * m, dtype=torch_cuda.float64, device="cuda")
+    v = torch_cuda.empty_like(u)
+    L = hmx.lib()
+    assert L.hmx_hmatrix_level_get(ctx.handle, H.handle, lvl, u.data_ptr(), v.data_ptr()) == hmx.HMX_OK
+    u.mul_(2.0)
+    assert L.hmx_hmatrix_set_level(ctx.handle, H.handle, lvl, u.data_ptr(), v.data_ptr()) == hmx.HMX_OK
+    B.matvec(X, Y, graph=True)
+    torch_cuda.cuda.synchronize()
+    a, e = B.offsets[0], B.offsets[1]
+    y0 = pipes[0].matvec(X[:, a:e].contiguous())
+    assert not torch_cuda.equal(Y[:, a:e], before[:, a:e])
+    assert torch_cuda.equal(Y[:, a:e], y0)
+    assert torch_cuda.equal(B.matvec(X), Y)
+    assert torch_cuda.equal(Y[:, e:], before[:, e:])

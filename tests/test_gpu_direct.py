@@ -129,6 +129,12 @@ def test_matvec_paths_vs_expansion(trio, ctx, torch_cuda, k):
             if opts[0] and tlr.d == 3:  # 3D leaf-level tiles have ranks above b/2
                 assert pipe.H.plan_info(0)["denseBlocks"] > 0
         pipe.H.set_plan_options(0, 0, 0)
+        # FP64 factors read in place and copied into the run layout
+        for layout in (1, 2):
+            pipe.H.set_fp64_layout(layout)
+            Y = pipe.matvec(x).cpu().numpy().T
+            assert np.linalg.norm(Y - E @ X) / np.linalg.norm(E @ X) < 1e-13, layout
+        pipe.H.set_fp64_layout(0)
 
 
 def test_b64_ranks_vs_oracle(torch_cuda, ctx):
