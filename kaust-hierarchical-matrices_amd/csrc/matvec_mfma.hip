@@ -883,15 +883,24 @@ bool is_near(const HMatrix& h, const TileRef& R) {
 template <class T>
 void build_near(Context& c, const HMatrix& h, int S, MfPlan& P, const std::vector<TileRef>& tiles) {
     const int b = (int)h.b;
-    std::vector<std::vector<std::pair<int32_t, int64_t>>> perLeaf(S);
+    // blocks stored by row leaf (a leaf's blocks are one contiguous stretch,
+    // which its phase-2 wave streams), in tile order within a leaf
+    std::vector<std::vector<const TileRef*>> perLeaf(S);
+    for (const TileRef& R : tiles)
+        if (is_near(h, R)) perLeaf[R.leafBase + R.outCl].push_back(&R);
     std::vector<NearSrc> src;
-    for (const TileRef& R : tiles) {
-        if (!is_near(h, R)) continue;
-        const double* U = R.lv->U.p + (size_t)R.lv->h_offsets[R.t] * R.m;
-        const double* V = R.lv->V.p + (size_t)R.lv->h_offsets[R.t] * R.m;
-        perLeaf[R.leafBase + R.outCl].push_back({(int32_t)(R.leafBase + R.inCl), (int64_t)src.size()});
-        src.push_back(NearSrc{U, V, (int64_t)src.size() * b * b, R.r, 0});
+    std::vector<int32_t> ptr(S + 1, 0);
+    std::vector<int32_t> colLeaf;
+    for (int leaf = 0; leaf < S; ++leaf) {
+        ptr[leaf] = (int32_t)src.size();
+        for (const TileRef* R : perLeaf[leaf]) {
+            const double* U = R->lv->U.p + (size_t)R->lv->h_offsets[R->t] * R->m;
+            const double* V = R->lv->V.p + (size_t)R->lv->h_offsets[R->t] * R->m;
+            colLeaf.push_back((int32_t)(R->leafBase + R->inCl));
+            src.push_back(NearSrc{U, V, (int64_t)src.size() * b * b, R->r, 0});
+        }
     }
+    ptr[S] = (int32_t)src.size();
     P.numNear = (int64_t)src.size();
     P.nearStore.alloc((size_t)std::max<int64_t>(1, P.numNear) * b * b * sizeof(T));
     if (P.numNear) {
@@ -901,14 +910,9 @@ void build_near(Context& c, const HMatrix& h, int S, MfPlan& P, const std::vecto
         HMX_LAUNCHED();
         HMX_HIP(hipStreamSynchronize(c.stream));
     }
-    std::vector<int32_t> ptr(S + 1, 0);
     std::vector<MfNear> list;
-    for (int leaf = 0; leaf < S; ++leaf) {
-        ptr[leaf] = (int32_t)list.size();
-        for (auto& e : perLeaf[leaf])
-            list.push_back(MfNear{(const void*)((const T*)P.nearStore.p + e.second * b * b), e.first, 0});
-    }
-    ptr[S] = (int32_t)list.size();
+    for (int64_t q = 0; q < P.numNear; ++q)
+        list.push_back(MfNear{(const void*)((const T*)P.nearStore.p + q * b * b), colLeaf[q], 0});
     upload(c, P.nearPtr, ptr);
     upload(c, P.near, list);
     HMX_HIP(hipStreamSynchronize(c.stream));
