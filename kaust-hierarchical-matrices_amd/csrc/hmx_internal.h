@@ -141,6 +141,11 @@ struct PlanHost {
     std::vector<int32_t> colNpart;
     std::vector<int32_t> leafPtr;   // [S+1]
     std::vector<MvEntry> entries;
+    // tiles of dimension >= rowBlock (0: none), per rowBlock-row block of the
+    // padded space instead of per leaf
+    int rowBlock = 0;
+    std::vector<int32_t> blkPtr;    // [S*b/rowBlock + 1]
+    std::vector<MvEntry> blkEntries;
     int64_t numCols = 0, numPart = 0;
 };
 
@@ -154,6 +159,10 @@ struct MatvecPlan {
     int32_t numLeaves = 0;
     DBuf<int32_t> leafPtr;   // [S+1] CSR over leaves
     DBuf<MvEntry> entries;
+    int rowBlock = 0;        // tiles of dimension >= rowBlock in a second phase-2 pass (0: none)
+    int32_t numBlocks = 0;
+    DBuf<int32_t> blkPtr;    // CSR over rowBlock-row blocks
+    DBuf<MvEntry> blkEntries;
     const double* diag = nullptr;  // D (or D^T for the transpose plan)
     DBuf<double> diagT;            // owned transposed copy
     int chunk = 2048;              // elements of one single-column phase-1 item
@@ -313,7 +322,7 @@ void build_dense_diagonal(Context& c, const hmx_config& cfg, const KDTree& t, co
 void hmatrix_save(Context& c, const HMatrix& h, const char* path);
 void hmatrix_load(Context& c, const char* path, HMatrix& h);
 void build_matvec_plan(Context& c, HMatrix& h, bool transpose);
-PlanHost plan_host(const HMatrix& h, bool transpose, int chunk);
+PlanHost plan_host(const HMatrix& h, bool transpose, int chunk, int rowBlock = 0);
 void upload_plan(Context& c, MatvecPlan& P, const PlanHost& H);
 void ensure_matvec_ws(MatvecPlan& P, uint32_t K);
 void run_matvec_plan(Context& c, MatvecPlan& P, int b, uint32_t k, const double* X, int64_t ldx, double* Y,

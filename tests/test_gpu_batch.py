@@ -92,6 +92,15 @@ def test_batch_equals_instances(torch_cuda, ctx, batch):
         a, e = B.offsets[i], B.offsets[i + 1]
         yi = pp.matvec(X[:, a:e].contiguous())
         assert torch_cuda.equal(Y[:, a:e], yi), i
+    # one right-hand side: the streaming kernels (per-leaf pass only in the
+    # batch; an instance's own plan may add the row-block pass, another order)
+    x1 = X[:1].contiguous()
+    y1 = B.matvec(x1)
+    for i, pp in enumerate(pipes):
+        a, e = B.offsets[i], B.offsets[i + 1]
+        yi = pp.matvec(x1[:, a:e].contiguous())
+        rel = float(torch_cuda.linalg.norm(y1[:, a:e] - yi) / torch_cuda.linalg.norm(yi))
+        assert rel < 1e-13, (i, rel)
 
 
 def test_batch_graph_replay(torch_cuda, ctx, batch):
