@@ -108,7 +108,13 @@ __global__ void __launch_bounds__(256) k_mf_phase1(const MfItem* __restrict__ it
     // over all XCDs: items differ in cost (1024-row chunks of upper levels,
     // 64-row leaf tiles), so one contiguous range per XCD would not balance.
     const int wg = chunk > 0 ? xcd_chunked(blockIdx.x, chunk) : (int)blockIdx.x;
-    const int wid = wg * 4 + (threadIdx.x >> 6);
+    // The item index is the same in every lane of the wave.  Declared so
+    // (readfirstlane), the item loads into scalar registers and the loop runs
+    // without exec masks: FP32/FP16 phase 1 8.3 -> 8.0 ms at N = 2^20, 64
+    // right-hand sides.  The FP64 kernel is scheduled worse that way (36.4 ->
+    // 40.7 ms, measured) and keeps the per-lane index.
+    const int wv = wg * 4 + (int)(threadIdx.x >> 6);
+    const int wid = sizeof(T) < 8 ? __builtin_amdgcn_readfirstlane(wv) : wv;
     if (wid >= nItems) return;
     const int lg = lane >> 4, lc = lane & 15;
     using ACC = typename M::acc_t;
@@ -379,7 +385,12 @@ __global__ void __launch_bounds__(256) k_mf_phase2g(int S, const int32_t* __rest
                   "one accumulator type per launch");
     constexpr int RG = BL / 16;
     const int grp = xcd_local(blockIdx.x, gridDim.x);
-    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63, lg = lane >> 4, lc = lane & 15;
+    // wave-uniform leaf (readfirstlane): scalar entry loads, no exec masks on
+    // the entry loops.  FP64 phase 2 39.2 -> 38.1 ms at N = 2^20, 64
+    // right-hand sides; the mixed kernel is slower that way (9.7 -> 10.2 ms,
+    // measured) and keeps the per-lane index.
+    const int w = sizeof(TA) == 8 ? __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6)) : (int)(threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63, lg = lane >> 4, lc = lane & 15;
     const int leaf = 4 * grp + w;
     if (leaf >= S) return;  // last group of a plan whose leaf count is not a multiple of 4 (no barriers below)
     accv acc[RG][NB];
