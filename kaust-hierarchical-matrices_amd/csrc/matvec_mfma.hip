@@ -111,10 +111,11 @@ __global__ void __launch_bounds__(256) k_mf_phase1(const MfItem* __restrict__ it
     // The item index is the same in every lane of the wave.  Declared so
     // (readfirstlane), the item loads into scalar registers and the loop runs
     // without exec masks: FP32/FP16 phase 1 8.3 -> 8.0 ms at N = 2^20, 64
-    // right-hand sides.  The FP64 kernel is scheduled worse that way (36.4 ->
+    // right-hand sides; FP64 with 16 (256-cloud batch) 19.15 -> 18.95 ms per
+    // call.  The FP64 kernel with 64 is scheduled worse that way (36.4 ->
     // 40.7 ms, measured) and keeps the per-lane index.
     const int wv = wg * 4 + (int)(threadIdx.x >> 6);
-    const int wid = sizeof(T) < 8 ? __builtin_amdgcn_readfirstlane(wv) : wv;
+    const int wid = (sizeof(T) < 8 || NB == 1) ? __builtin_amdgcn_readfirstlane(wv) : wv;
     if (wid >= nItems) return;
     const int lg = lane >> 4, lc = lane & 15;
     using ACC = typename M::acc_t;
