@@ -129,11 +129,13 @@ __global__ void __launch_bounds__(256) k_mf_phase1(const MfItem* __restrict__ it
     for (int g = 0; g < CG; ++g)
 #pragma unroll
         for (int nb = 0; nb < NB; ++nb) acc[g][nb] = typename M::accv{0, 0, 0, 0};
-    // FP64: software pipelined, the operands of super-step r0 + 16*KL are in
-    // flight while the MFMAs of super-step r0 run.  FP32/FP16: no register
-    // double buffer -- the registers it frees buy a third wave per SIMD,
-    // which hides the load latency better (measured).
-    constexpr bool PF = sizeof(T) == 8;
+    // FP64 with 64 right-hand sides: software pipelined, the operands of
+    // super-step r0 + 16*KL are in flight while the MFMAs of super-step r0
+    // run (36.4 ms vs 39.1 without, at N = 2^20).  FP64 with 16 and
+    // FP32/FP16: no register double buffer -- the registers it frees buy more
+    // waves per SIMD, which hide the load latency better (FP64 16 RHS 16.6 ->
+    // 15.9 ms; FP32/FP16 -0.3 ms).
+    constexpr bool PF = sizeof(T) == 8 && NB > 1;
     T a[CG][VL], bx[NB][VL], an[CG][VL], bn[NB][VL];
     auto load = [&](int r0, T (&A)[CG][VL], T (&B)[NB][VL]) {
         const int rr = r0 + VL * lg;
