@@ -1,9 +1,10 @@
 """Point clouds of uneven, non-power-of-two sizes and the batched H-matvec.
 
 Bars:
-  * partial leaves: the H-matvec of an H-matrix over n points that do not fill
-    the leaves equals the dense expansion of its own factors within 1e-13 and
-    the dense kernel product within the oracle's compression error;
+  * partial leaves (32- and 64-row leaves; 1, 4 and 16 right-hand sides,
+    transposed too): the H-matvec of an H-matrix over n points that do not
+    fill the leaves equals the dense expansion of its own factors within 1e-13
+    and the dense kernel product within the oracle's compression error;
   * a batch of independent H-matrices (2D and 3D clouds, per-cloud
     correlation length) applied in one launch per phase equals, bit for bit,
     each instance's own H-matvec; hipGraph replays equal the eager launches
@@ -34,9 +35,10 @@ def ctx(torch_cuda):
     return hmx.Context()
 
 
-@pytest.mark.parametrize("n,d,direct", [(3000, 2, False), (5001, 3, True), (2049, 2, True)])
-def test_partial_leaves(torch_cuda, ctx, n, d, direct):
-    b, tol = 32, 1e-6
+@pytest.mark.parametrize("n,d,direct,b", [(3000, 2, False, 32), (5001, 3, True, 32), (2049, 2, True, 32),
+                                          (5001, 3, True, 64)])
+def test_partial_leaves(torch_cuda, ctx, n, d, direct, b):
+    tol = 1e-6
     p = O.generate_points(n, d, 77)
     pipe = hmx.Pipeline(n, d, b, tol, adm="weak", max_rank_policy=hmx.MAXRANK_FULL, points=to_dev(torch_cuda, p),
                         ctx=ctx, direct=direct)
@@ -45,6 +47,10 @@ def test_partial_leaves(torch_cuda, ctx, n, d, direct):
     Y = pipe.matvec(x).cpu().numpy().T
     E = expand_gpu(ctx, pipe)
     assert np.linalg.norm(Y - E @ X) / np.linalg.norm(E @ X) < 1e-13
+    # one right-hand side: the streaming kernels (with 64-row leaves the
+    # per-leaf pass and the 128-row-block pass) over the padded space
+    y1 = pipe.matvec(x[:1].contiguous()).cpu().numpy()[0]
+    assert np.linalg.norm(y1 - E @ X[:, 0]) / np.linalg.norm(E @ X[:, 0]) < 1e-13
     yt = pipe.matvec(x, transpose=True).cpu().numpy().T
     assert np.linalg.norm(yt - E.T @ X) / np.linalg.norm(E.T @ X) < 1e-13
     K = O.dense_matrix(p, n, d, O.kdtree(p, n, d, b)[0], pipe.ell)
@@ -53,7 +59,7 @@ def test_partial_leaves(torch_cuda, ctx, n, d, direct):
     err_o = np.linalg.norm(ref.matvec(X) - K @ X) / np.linalg.norm(K @ X)
     print(f"n={n} d={d} direct={direct}: rel err {err:.3e} (oracle {err_o:.3e})")
     assert err <= 2 * err_o + 1e-9
-    # 16 right-hand sides with partial leaves take the streaming path
+    # 16 right-hand sides: the matrix cores over the padded leaf space
     X16 = O.generate_rhs(n, 16, 5)
     Y16 = pipe.matvec(to_dev(torch_cuda, X16.T)).cpu().numpy().T
     assert np.linalg.norm(Y16 - E @ X16) / np.linalg.norm(E @ X16) < 1e-13
