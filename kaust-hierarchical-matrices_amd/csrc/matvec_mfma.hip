@@ -99,7 +99,7 @@ __global__ void __launch_bounds__(256) k_mf_phase1(const MfItem* __restrict__ it
                                                    typename Mfma<T>::acc_t* __restrict__ zpart,
                                                    T* __restrict__ Zt, const int32_t* __restrict__ zmap,
                                                    const float* __restrict__ colScaleV,
-                                                   const float* __restrict__ rhsScale) {
+                                                   const float* __restrict__ rhsScale, int kvalid) {
     using M = Mfma<T>;
     constexpr int KL = M::KL, VL = 4 * KL, NR = 16 * NB;
     const int lane = threadIdx.x & 63;
@@ -146,9 +146,10 @@ __global__ void __launch_bounds__(256) k_mf_phase1(const MfItem* __restrict__ it
             if (rok && col < it.ncols) vload_f(V + (size_t)col * it.ld + rr, A[g]);
             else vzero(A[g]);
         }
+        // right-hand sides past kvalid are zero in Xb: not loaded
 #pragma unroll
         for (int nb = 0; nb < NB; ++nb) {
-            if (rok) vload(Xb + xb_index<NR>((int64_t)it.xoff + rr, 16 * nb + lc, sh), B[nb]);
+            if (rok && 16 * nb + lc < kvalid) vload(Xb + xb_index<NR>((int64_t)it.xoff + rr, 16 * nb + lc, sh), B[nb]);
             else vzero(B[nb]);
         }
     };
@@ -251,7 +252,7 @@ __global__ void __launch_bounds__(256) k_mf_reduce(int64_t nred, int64_t ldz, co
 // step's MFMAs run.  rowOff: the wave's first row inside the entries' U.
 template <int BL, bool UCM, class T, int NB>
 __device__ inline void lr_rows(const MfEntry* __restrict__ ents, int e0, int e1, const T* __restrict__ Z, int rowOff,
-                               int lg, int lc, typename Mfma<T>::accv (&acc)[BL / 16][NB]) {
+                               int lg, int lc, int kvalid, typename Mfma<T>::accv (&acc)[BL / 16][NB]) {
     using M = Mfma<T>;
     constexpr int KL = M::KL, VL = 4 * KL, SS = 16 * KL, NR = 16 * NB, RG = BL / 16;
     if (e0 >= e1) return;
@@ -271,8 +272,12 @@ __device__ inline void lr_rows(const MfEntry* __restrict__ ents, int e0, int e1,
                     vload_f((const T*)en.u + (size_t)row * en.rpad + cc, A[rg]);
                 }
             }
+            // z of right-hand sides past kvalid is never used: not loaded
 #pragma unroll
-            for (int nb = 0; nb < NB; ++nb) vload(Z + zt_index<VL, NR>(en.zcol + cc, 16 * nb + lc), B[nb]);
+            for (int nb = 0; nb < NB; ++nb) {
+                if (16 * nb + lc < kvalid) vload(Z + zt_index<VL, NR>(en.zcol + cc, 16 * nb + lc), B[nb]);
+                else vzero(B[nb]);
+            }
         } else {
 #pragma unroll
             for (int rg = 0; rg < RG; ++rg) vzero(A[rg]);
@@ -324,7 +329,7 @@ __device__ inline void lr_rows(const MfEntry* __restrict__ ents, int e0, int e1,
 template <int BL, class TD, int NB>
 __device__ inline void dense_rows(int leaf, const TD* __restrict__ Drm, const int32_t* __restrict__ nearPtr,
                                   const MfNear* __restrict__ nearB, const TD* __restrict__ Xb, int lg, int lc,
-                                  typename Mfma<TD>::accv (&acc)[BL / 16][NB]) {
+                                  int kvalid, typename Mfma<TD>::accv (&acc)[BL / 16][NB]) {
     using MD = Mfma<TD>;
     static_assert(MD::KL == 1, "dense blocks on the 16x16x4 shapes");
     constexpr int NR = 16 * NB, EPL = 32 / sizeof(TD), SPB = BL / (4 * EPL), RG = BL / 16;
@@ -340,7 +345,10 @@ __device__ inline void dense_rows(int leaf, const TD* __restrict__ Drm, const in
 #pragma unroll
         for (int rg = 0; rg < RG; ++rg) vload(Dq + (16 * rg + lc) * BL + cc, A[rg]);
 #pragma unroll
-        for (int nb = 0; nb < NB; ++nb) vload(Xb + ((size_t)cl * NR + 16 * nb + lc) * BL + cc, B[nb]);
+        for (int nb = 0; nb < NB; ++nb) {  // right-hand sides past kvalid are zero in Xb: not loaded
+            if (16 * nb + lc < kvalid) vload(Xb + ((size_t)cl * NR + 16 * nb + lc) * BL + cc, B[nb]);
+            else vzero(B[nb]);
+        }
     };
     dload(0, a, bx);
     for (int idx = 0; idx < total; ++idx) {
@@ -401,12 +409,12 @@ __global__ void __launch_bounds__(256) k_mf_phase2g(int S, const int32_t* __rest
     for (int rg = 0; rg < RG; ++rg)
 #pragma unroll
         for (int nb = 0; nb < NB; ++nb) acc[rg][nb] = accv{0, 0, 0, 0};
-    dense_rows<BL, TD, NB>(leaf, Drm, nearPtr, nearB, Xb, lg, lc, acc);
-    lr_rows<BL, UCM, TA, NB>(entA, ptrA[leaf], ptrA[leaf + 1], ZA, 0, lg, lc, acc);
-    if (gptrA) lr_rows<BL, UCM, TA, NB>(gentA, gptrA[grp], gptrA[grp + 1], ZA, BL * w, lg, lc, acc);
+    dense_rows<BL, TD, NB>(leaf, Drm, nearPtr, nearB, Xb, lg, lc, kvalid, acc);
+    lr_rows<BL, UCM, TA, NB>(entA, ptrA[leaf], ptrA[leaf + 1], ZA, 0, lg, lc, kvalid, acc);
+    if (gptrA) lr_rows<BL, UCM, TA, NB>(gentA, gptrA[grp], gptrA[grp + 1], ZA, BL * w, lg, lc, kvalid, acc);
     if (ptrB) {
-        lr_rows<BL, false, TB, NB>(entB, ptrB[leaf], ptrB[leaf + 1], ZB, 0, lg, lc, acc);
-        if (gptrB) lr_rows<BL, false, TB, NB>(gentB, gptrB[grp], gptrB[grp + 1], ZB, BL * w, lg, lc, acc);
+        lr_rows<BL, false, TB, NB>(entB, ptrB[leaf], ptrB[leaf + 1], ZB, 0, lg, lc, kvalid, acc);
+        if (gptrB) lr_rows<BL, false, TB, NB>(gentB, gptrB[grp], gptrB[grp + 1], ZB, BL * w, lg, lc, kvalid, acc);
     }
 #pragma unroll
     for (int rg = 0; rg < RG; ++rg)
@@ -1040,7 +1048,7 @@ int phase1_chunk() {
 }
 
 template <class T, int CG, int NB>
-void phase1(Context& c, MfGroup& G, const T* Xb, int sh, const float* rhsScale, int xcdChunk = -1) {
+void phase1(Context& c, MfGroup& G, const T* Xb, int sh, const float* rhsScale, int kvalid, int xcdChunk = -1) {
     if (G.numItems == 0) return;
     const float* cs = G.colScaleV.n ? G.colScaleV.p : nullptr;
     const int chunk = xcdChunk >= 0 ? xcdChunk : phase1_chunk();
@@ -1048,7 +1056,7 @@ void phase1(Context& c, MfGroup& G, const T* Xb, int sh, const float* rhsScale, 
     if (chunk > 0) blocks = (blocks + 8 * chunk - 1) / (8 * chunk) * (8 * chunk);
     k_mf_phase1<T, CG, NB><<<blocks, 256, 0, c.stream>>>(G.items.p, G.numItems, chunk, Xb, sh,
                                                          (typename Mfma<T>::acc_t*)G.zpart.p, (T*)G.zt.p, G.zmap.p,
-                                                         cs, rhsScale);
+                                                         cs, rhsScale, kvalid);
     HMX_LAUNCHED();
 }
 
@@ -1093,7 +1101,7 @@ void run_fp64(Context& c, MfPlan& P, const double* X, int64_t ldx, int kk, doubl
     k_x_blocked<double, double><<<(unsigned)((tot + 255) / 256), 256, 0, c.stream>>>(
         tot, P.npad, NR, sh, X, ldx, kk, P.gather ? P.gmap.p : nullptr, nullptr, (double*)P.Xb.p);
     HMX_LAUNCHED();
-    phase1<double, 4, NB>(c, G, Xb, sh, nullptr, P.xcdChunk);  // 4 column groups = the 64 columns of an item
+    phase1<double, 4, NB>(c, G, Xb, sh, nullptr, kk, P.xcdChunk);  // 4 column groups = the 64 columns of an item
     ev.rec(c, 1);
     reduce_group<double, double>(c, G, NR, nullptr);
     ev.rec(c, 2);
@@ -1134,8 +1142,8 @@ void run_mixed(Context& c, HMatrix& h, const float* X, float* Y, Events& ev) {
     const float* Xb = (const float*)P.Xb.p;
     to_blocked<float, _Float16>(c, h, NR, X, P.xscale.p, P.X16.p);
     to_blocked<float, float>(c, h, NR, X, nullptr, (float*)P.Xb.p);
-    phase1<float, 4, NB>(c, Ghi, Xb, leaf_shift(h), nullptr);
-    phase1<_Float16, 4, NB>(c, Glo, P.X16.p, leaf_shift(h), P.xscale.p);
+    phase1<float, 4, NB>(c, Ghi, Xb, leaf_shift(h), nullptr, NR);
+    phase1<_Float16, 4, NB>(c, Glo, P.X16.p, leaf_shift(h), P.xscale.p, NR);
     ev.rec(c, 1);
     reduce_group<float, float>(c, Ghi, NR, nullptr);
     reduce_group<float, _Float16>(c, Glo, NR, P.xscale.p);
