@@ -471,7 +471,8 @@ std::vector<int32_t> padding_map(const HMatrix& h, int64_t base) {
 // rows 14.4 ms, of 64 rows 14.9 ms).  32-row leaves and batches: per-leaf
 // pass only.
 int row_block(const HMatrix& h) {
-    return h.b == 64 && (int64_t)h.S * h.b >= kRowBlock ? kRowBlock : 0;
+    const int64_t npad = (int64_t)h.S * h.b;
+    return h.b == 64 && npad >= kRowBlock && npad % kRowBlock == 0 ? kRowBlock : 0;
 }
 
 void build_matvec_plan(Context& c, HMatrix& h, bool transpose) {
