@@ -208,7 +208,9 @@ hmx_status hmx_freeHMatrix(hmx_hmatrix* h);
    matrix cores (passes of 16, or 64 when vectorWidth is a multiple of 64). */
 hmx_status hmx_HMatrixVecMult(hmx_context* ctx, const hmx_hmatrix* h, uint32_t vectorWidth, const double* d_in,
                               double* d_out);
-/* out = H^T * in (no reference counterpart; the backward of HMatrixVecMult) */
+/* out = H^T * in (no reference counterpart; the backward of HMatrixVecMult);
+   any vectorWidth >= 1 and any n, on the streaming kernels in passes of 16,
+   8, 4 or 1 right-hand sides */
 hmx_status hmx_HMatrixVecMultTranspose(hmx_context* ctx, const hmx_hmatrix* h, uint32_t vectorWidth,
                                        const double* d_in, double* d_out);
 /* Times the matvec kernels with hipEvents on the context stream over iters
@@ -218,10 +220,11 @@ hmx_status hmx_matvec_profile(hmx_context* ctx, const hmx_hmatrix* h, uint32_t v
                               double* d_out, int32_t iters, float* ms);
 
 /* ---- many right-hand sides on the matrix cores ------------------------- */
-/* hmx_HMatrixVecMult with vectorWidth a multiple of 16 runs on the FP64
-   matrix cores (16 or 64 right-hand sides per pass); other widths use the
-   streaming kernels.  Times the multi-RHS kernels like hmx_matvec_profile
-   (ms[0] phase 1, ms[1] reduce, ms[2] phase 2, ms[3] call); mixed != 0 times
+/* hmx_HMatrixVecMult with vectorWidth >= 2 runs on the FP64 matrix cores
+   (passes of 64 right-hand sides when vectorWidth is a multiple of 64, else
+   of 16, the last pass padded with zero columns).  Times the multi-RHS
+   kernels like hmx_matvec_profile (ms[0] phase 1, ms[1] reduce, ms[2] phase
+   2, ms[3] call) for vectorWidth 16 or 64; mixed != 0 times
    hmx_HMatrixVecMultMixed (in/out then point to float). */
 hmx_status hmx_matvec_profile_multi(hmx_context* ctx, const hmx_hmatrix* h, uint32_t vectorWidth, const void* d_in,
                                     void* d_out, int32_t iters, int32_t mixed, float* ms);
@@ -254,8 +257,12 @@ hmx_status hmx_hmatrix_set_fp64_layout(hmx_hmatrix* h, int32_t layout);
    (rank above bucketSize/2), low-rank columns.  HMX_ESTATE if not built. */
 hmx_status hmx_hmatrix_plan_info(const hmx_hmatrix* h, int32_t plan, uint64_t* bytes, int64_t* denseBlocks,
                                  int64_t* lowRankColumns);
-/* Y = H X with X, Y n x vectorWidth column-major FP32 (vectorWidth a multiple
-   of 16); FP16/FP32 matrix cores with FP32 accumulation */
+/* Y = H X with X, Y n x vectorWidth column-major FP32, any vectorWidth >= 1
+   (passes of 64 or 16 right-hand sides, the last padded) and any n (partial
+   leaves through the padded leaf space); FP16/FP32 matrix cores with FP32
+   accumulation.  Each right-hand side is converted to FP16 divided by its
+   max-abs, and that scale is applied again in FP32, so the relative error
+   does not depend on the magnitude of the input. */
 hmx_status hmx_HMatrixVecMultMixed(hmx_context* ctx, const hmx_hmatrix* h, uint32_t vectorWidth, const float* d_in,
                                    float* d_out);
 /* One MFMA of each element type with the lane layout the kernels assume:
@@ -274,8 +281,9 @@ hmx_status hmx_stream_read_probe(hmx_context* ctx, const void* d_buf, uint64_t b
 /* Gradients of a loss with output gradient dY (Y = H X) with respect to every
    tile's factors, written into per-level buffers laid out like the level's
    U/V (dU[l-1], dV[l-1] for level l; hmx_hmatrix_level_info gives sizes).
-   X, dY: n x vectorWidth column-major FP64 device buffers.  The input
-   gradient is hmx_HMatrixVecMultTranspose(dY). */
+   X, dY: n x vectorWidth column-major FP64 device buffers, any vectorWidth
+   >= 1 and any n (rows of the padding positions of partial leaves get zero
+   gradients).  The input gradient is hmx_HMatrixVecMultTranspose(dY). */
 hmx_status hmx_hmatvec_grad_factors(hmx_context* ctx, const hmx_hmatrix* h, uint32_t vectorWidth, const double* d_X,
                                     const double* d_dY, double* const* d_dU, double* const* d_dV);
 /* Copy one level's factors into device buffers (same layout) */

@@ -38,13 +38,13 @@ void batch_create(Context& c, const std::vector<HMatrix*>& hs, Batch& B) {
 
 // One right-hand side: the streaming kernels.  Two or more: the FP64 matrix
 // cores over the batch's block-diagonal operator (passes of 16 or 64, the
-// last padded with zero columns).
+// last padded with zero columns).  Plans and workspaces are made ready here,
+// never inside a capture; when any of them was rebuilt, reallocated or
+// cleared the batch's epoch moves on, and a graph captured in an earlier
+// epoch is captured again before its next replay.
 static void prepare(Context& c, Batch& B, uint32_t k) {
-    if (k >= 2) {
-        batch_mf_prepare(c, B, k);
-    } else {
-        ensure_matvec_ws(B.P, 1);
-    }
+    const bool changed = k >= 2 ? batch_mf_prepare(c, B, k) : ensure_matvec_ws(B.P, 1);
+    if (changed) ++B.epoch;
 }
 
 static void run_batch(Context& c, Batch& B, uint32_t k, const double* X, double* Y) {
@@ -55,18 +55,15 @@ static void run_batch(Context& c, Batch& B, uint32_t k, const double* X, double*
 void batch_matvec(Context& c, Batch& B, uint32_t k, const double* X, double* Y, bool useGraph) {
     HMX_CHECK(k >= 1, HMX_EINVAL, "vectorWidth must be >= 1");
     HMX_CHECK(X && Y && X != Y, HMX_EINVAL, "bad vector pointers");
+    prepare(c, B, k);
     if (!useGraph) {
-        prepare(c, B, k);
         run_batch(c, B, k, X, Y);
         return;
     }
-    if (k >= 2 && batch_mf_stale(B)) B.drop_graph();  // the captured launches point into the old plan
-    if (!B.exec || B.gX != X || B.gY != Y || B.gk != k) {
+    if (!B.exec || B.gX != X || B.gY != Y || B.gk != k || B.gEpoch != B.epoch) {
         B.drop_graph();
-        // workspaces are allocated here, outside the capture; the captured
-        // sequence holds kernel launches only (gather or blocked copy,
-        // phase 1, reduce, phase 2, scatter per pass)
-        prepare(c, B, k);
+        // the captured sequence holds kernel launches only (gather or
+        // blocked copy, phase 1, reduce, phase 2, scatter per pass)
         HMX_HIP(hipStreamSynchronize(c.stream));
         Context cap;
         cap.stream = B.stream;
@@ -85,6 +82,7 @@ void batch_matvec(Context& c, Batch& B, uint32_t k, const double* X, double* Y, 
         B.gX = X;
         B.gY = Y;
         B.gk = k;
+        B.gEpoch = B.epoch;
     }
     // order the replay after the caller's prior work and the caller's later work after it
     HMX_HIP(hipEventRecord(B.evIn, c.stream));
@@ -143,6 +141,7 @@ void batch_tune(Context& c, Batch& B, uint32_t k, int* bestChunk, float* bestMs)
     }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
+    ++B.epoch;  // plans rebuilt, tuned launch shapes
     if (bestChunk) *bestChunk = best;
     if (bestMs) *bestMs = bms;
 }

@@ -46,7 +46,6 @@ void hmatvec_grad_factors(Context& c, HMatrix& h, uint32_t k, const double* X, c
                           double* const* dU, double* const* dV) {
     HMX_CHECK(k >= 1, HMX_EINVAL, "vectorWidth must be >= 1");
     HMX_CHECK(X && dY && dU && dV, HMX_EINVAL, "null pointer");
-    HMX_CHECK(h.n == h.S * h.b, HMX_EUNSUPPORTED, "factor gradients need full leaves (n = numSegments * bucketSize)");
     std::vector<GradTile> tiles;
     int64_t g = 0;
     for (int l = 1; l < h.numLevels; ++l) {
@@ -67,19 +66,27 @@ void hmatvec_grad_factors(Context& c, HMatrix& h, uint32_t k, const double* X, c
     HMX_HIP(hipMemcpyAsync(d_tiles.p, tiles.data(), tiles.size() * sizeof(GradTile), hipMemcpyHostToDevice, c.stream));
     int maxm = 0;
     for (auto& t : tiles) maxm = std::max(maxm, t.m);
-    // z of X (forward plan) and of dY (transposed plan), in passes of up to 16 right-hand sides
-    const uint32_t K = k == 1 ? 1 : (k <= 4 ? 4 : (k <= 8 ? 8 : 16));
-    HMX_CHECK(k == 1 || k % K == 0, HMX_EINVAL, "vectorWidth must be 1 or a multiple of 4/8/16");
-    DBuf<double> scratch((size_t)h.n * K);
-    for (uint32_t s0 = 0; s0 < k; s0 += K) {
+    // Z of X (forward plan) and G of dY (transposed plan), in passes of 16, 8,
+    // 4 or 1 right-hand sides.  Tile rows are positions of the padded leaf
+    // space: with partial leaves the outer products read the gathered copies
+    // the two matvecs leave in the plans (zero on padding rows, so the
+    // padding rows of dU and dV come out zero as the factors' own are).
+    DBuf<double> scratch((size_t)h.n * pass_width(k));
+    for (uint32_t s0 = 0; s0 < k;) {
+        const uint32_t K = pass_width(k - s0);
         const double* Xs = X + (size_t)s0 * h.n;
         const double* Ws = dY + (size_t)s0 * h.n;
         hmatvec(c, h, K, Xs, scratch.p, false, nullptr, 0);   // leaves Z in h.plan.z
         hmatvec(c, h, K, Ws, scratch.p, true, nullptr, 0);    // leaves G in h.planT.z
+        const bool pad = h.plan.gather;
+        const double* Xg = pad ? h.plan.Xp.p : Xs;
+        const double* Wg = pad ? h.planT.Xp.p : Ws;
+        const int ld = pad ? (int)h.plan.npad : (int)h.n;
         dim3 grid((unsigned)((maxm + 255) / 256), (unsigned)tiles.size());
-        k_grad_outer<<<grid, 256, 0, c.stream>>>(d_tiles.p, (int)K, (int)h.n, Xs, Ws, h.plan.z.p, h.planT.z.p,
+        k_grad_outer<<<grid, 256, 0, c.stream>>>(d_tiles.p, (int)K, ld, Xg, Wg, h.plan.z.p, h.planT.z.p,
                                                  s0 > 0 ? 1 : 0);
         HMX_LAUNCHED();
+        s0 += K;
     }
     HMX_HIP(hipStreamSynchronize(c.stream));
 }

@@ -149,6 +149,12 @@ struct PlanHost {
     int64_t numCols = 0, numPart = 0;
 };
 
+// Per-call workspaces of the streaming matvec kernels: chunk partial sums,
+// z, and (partial leaves) x and y in the padded space.
+struct MvWork {
+    double *zpart, *z, *Xp, *Yp;
+};
+
 // Device-side tables used by the matvec kernels (built once per H-matrix).
 struct MatvecPlan {
     DBuf<MvItem> items;
@@ -220,7 +226,8 @@ struct MfGroup {
     DBuf<MfEntry> grpEntries;      // U at the group's first row
     uint64_t factorBytes = 0;      // V + U bytes of this group (unpadded entries)
     DBuf<unsigned char> zpart, zt; // workspaces
-    size_t wsNR = 0;
+    size_t wsNR = 0;               // pass width the workspaces are sized for
+    int ztNR = 0;                  // pass width whose z padding columns were last cleared
 };
 
 // Leaf-level tile stored dense in a matrix-core plan (rank above b/2: the
@@ -254,6 +261,7 @@ struct MfPlan {
     DBuf<_Float16> X16;            // the same in FP16, scaled per right-hand side (mixed)
     DBuf<float> xscale, xpart;     // per right-hand-side scale and its per-slice partial maxima
     size_t x16n = 0;
+    float loMul = 1.f;             // 2^-E: FP16 z holds V^T x scaled by 2^E (mixed)
     // selection statistics (mixed)
     double frobH2 = 0, loEnergy = 0, target = 0;
     int64_t hiCols = 0, loCols = 0;
@@ -295,6 +303,8 @@ struct Batch {
     const double* gX = nullptr;  // arguments the graph was captured with
     double* gY = nullptr;
     uint32_t gk = 0;
+    uint64_t epoch = 0;          // bumped whenever a plan or workspace the launches use changes
+    uint64_t gEpoch = 0;         // epoch the graph was captured in
     void drop_graph();
     ~Batch();
 };
@@ -324,9 +334,14 @@ void hmatrix_load(Context& c, const char* path, HMatrix& h);
 void build_matvec_plan(Context& c, HMatrix& h, bool transpose);
 PlanHost plan_host(const HMatrix& h, bool transpose, int chunk, int rowBlock = 0);
 void upload_plan(Context& c, MatvecPlan& P, const PlanHost& H);
-void ensure_matvec_ws(MatvecPlan& P, uint32_t K);
+bool ensure_matvec_ws(MatvecPlan& P, uint32_t K);
+// widest streaming-kernel pass (16, 8, 4 or 1 right-hand sides) for rem columns
+uint32_t pass_width(uint32_t rem);
 void run_matvec_plan(Context& c, MatvecPlan& P, int b, uint32_t k, const double* X, int64_t ldx, double* Y,
                      hipEvent_t* ev);
+size_t matvec_ws_elems(const MatvecPlan& P, uint32_t K);
+void run_matvec_plan_ws(Context& c, const MatvecPlan& P, const MvWork& W, int b, uint32_t k, const double* X, int64_t ldx,
+                        double* Y, hipEvent_t* ev);
 void build_batch_plan(Context& c, const std::vector<HMatrix*>& hs, int chunk, MatvecPlan& P, DBuf<double>& diag);
 void batch_create(Context& c, const std::vector<HMatrix*>& hs, Batch& B);
 void batch_matvec(Context& c, Batch& B, uint32_t k, const double* X, double* Y, bool useGraph);
@@ -338,7 +353,7 @@ void dense_rows(Context& c, uint32_t n, uint32_t d, const double* p, const int32
 void hmatvec_multi(Context& c, HMatrix& h, uint32_t k, const double* d_in, double* d_out, float* ms, int iters);
 std::vector<int32_t> padding_map(const HMatrix& h, int64_t base);
 bool batch_mf_stale(const Batch& B);
-void batch_mf_prepare(Context& c, Batch& B, uint32_t k);
+bool batch_mf_prepare(Context& c, Batch& B, uint32_t k);
 void batch_mf_run(Context& c, Batch& B, uint32_t k, const double* d_in, double* d_out);
 int default_xcd_chunk();
 void hmatrix_make_mixed(Context& c, HMatrix& h, double targetRelErr);

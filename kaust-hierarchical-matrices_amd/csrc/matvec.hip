@@ -501,38 +501,41 @@ void build_matvec_plan(Context& c, HMatrix& h, bool transpose) {
 }
 
 template <int K>
-static void run_phases(Context& c, MatvecPlan& P, int b, const double* X, int64_t ldx, double* Y, hipEvent_t* ev) {
+static void run_phases(Context& c, const MatvecPlan& P, const MvWork& W, int b, const double* X, int64_t ldx, double* Y,
+                       hipEvent_t* ev) {
     if (ev) HMX_HIP(hipEventRecord(ev[0], c.stream));
     if (P.numItems > 0) {
         k_mv_phase1<K><<<(unsigned)((P.numItems + 3) / 4), 256, 0, c.stream>>>(P.items.p, P.numItems, X, (int)ldx,
-                                                                                 P.zpart.p);
+                                                                                 W.zpart);
         HMX_LAUNCHED();
     }
     if (ev) HMX_HIP(hipEventRecord(ev[1], c.stream));
     if (P.numCols > 0) {
         k_mv_reduce<<<(unsigned)((P.numCols * K + 255) / 256), 256, 0, c.stream>>>(P.numCols, K, P.colPart.p,
-                                                                                   P.colNpart.p, P.zpart.p, P.z.p);
+                                                                                   P.colNpart.p, W.zpart, W.z);
         HMX_LAUNCHED();
     }
     if (ev) HMX_HIP(hipEventRecord(ev[2], c.stream));
     if (b == 64)
-        k_mv_phase2<64, K><<<(unsigned)P.numLeaves, 256, 0, c.stream>>>(P.leafPtr.p, P.entries.p, P.diag, P.z.p, X,
+        k_mv_phase2<64, K><<<(unsigned)P.numLeaves, 256, 0, c.stream>>>(P.leafPtr.p, P.entries.p, P.diag, W.z, X,
                                                                          (int)ldx, Y);
     else
-        k_mv_phase2<32, K><<<(unsigned)P.numLeaves, 256, 0, c.stream>>>(P.leafPtr.p, P.entries.p, P.diag, P.z.p, X,
+        k_mv_phase2<32, K><<<(unsigned)P.numLeaves, 256, 0, c.stream>>>(P.leafPtr.p, P.entries.p, P.diag, W.z, X,
                                                                          (int)ldx, Y);
     HMX_LAUNCHED();
     if (P.numBlocks > 0) {  // after the per-leaf pass: it adds to Y
         HMX_CHECK(P.rowBlock == kRowBlock, HMX_ESTATE, "matvec plan with an unknown row block");
-        k_mv_phase2b<kRowBlock, K><<<(unsigned)P.numBlocks, 256, 0, c.stream>>>(P.blkPtr.p, P.blkEntries.p, P.z.p,
+        k_mv_phase2b<kRowBlock, K><<<(unsigned)P.numBlocks, 256, 0, c.stream>>>(P.blkPtr.p, P.blkEntries.p, W.z,
                                                                                  Y, (int)ldx);
         HMX_LAUNCHED();
     }
     if (ev) HMX_HIP(hipEventRecord(ev[3], c.stream));
 }
 
-void ensure_matvec_ws(MatvecPlan& P, uint32_t K) {
-    if (P.wsK >= K) return;
+// Workspaces for passes of up to K right-hand sides.  Returns true when they
+// were (re)allocated: launch sequences captured before point at the old ones.
+bool ensure_matvec_ws(MatvecPlan& P, uint32_t K) {
+    if (P.wsK >= K) return false;
     P.zpart.alloc((size_t)std::max<int64_t>(1, P.numPart) * K);
     P.z.alloc((size_t)std::max<int64_t>(1, P.numCols) * K);
     if (P.gather) {
@@ -540,42 +543,64 @@ void ensure_matvec_ws(MatvecPlan& P, uint32_t K) {
         P.Yp.alloc((size_t)P.npad * K);
     }
     P.wsK = K;
+    return true;
 }
 
-// K = kernel width (1, 4, 8 or 16); k right-hand sides in passes of K.  With
-// partial leaves every pass gathers into and scatters from the padded space.
+// K = kernel width (1, 4, 8 or 16); k (a multiple of K) right-hand sides in
+// passes of K.  With partial leaves every pass gathers into and scatters from
+// the padded space.
 template <int K>
-static void run_k(Context& c, MatvecPlan& P, int b, uint32_t k, const double* X, int64_t ldx, double* Y,
-                  hipEvent_t* ev) {
+static void run_k(Context& c, const MatvecPlan& P, const MvWork& W, int b, uint32_t k, const double* X, int64_t ldx,
+                  double* Y, hipEvent_t* ev) {
     for (uint32_t s0 = 0; s0 < k; s0 += K) {
         const double* Xs = X + (size_t)s0 * ldx;
         double* Ys = Y + (size_t)s0 * ldx;
         if (!P.gather) {
-            run_phases<K>(c, P, b, Xs, ldx, Ys, ev);
+            run_phases<K>(c, P, W, b, Xs, ldx, Ys, ev);
             continue;
         }
         const int64_t tot = P.npad * K;
-        k_gather<<<(unsigned)((tot + 255) / 256), 256, 0, c.stream>>>(P.npad, K, P.gmap.p, Xs, ldx, P.Xp.p);
+        k_gather<<<(unsigned)((tot + 255) / 256), 256, 0, c.stream>>>(P.npad, K, P.gmap.p, Xs, ldx, W.Xp);
         HMX_LAUNCHED();
-        run_phases<K>(c, P, b, P.Xp.p, P.npad, P.Yp.p, ev);
-        k_scatter<<<(unsigned)((tot + 255) / 256), 256, 0, c.stream>>>(P.npad, K, P.gmap.p, P.Yp.p, Ys, ldx);
+        run_phases<K>(c, P, W, b, W.Xp, P.npad, W.Yp, ev);
+        k_scatter<<<(unsigned)((tot + 255) / 256), 256, 0, c.stream>>>(P.npad, K, P.gmap.p, W.Yp, Ys, ldx);
         HMX_LAUNCHED();
     }
 }
 
-static uint32_t kernel_width(uint32_t k) { return k == 1 ? 1 : (k <= 4 ? 4 : (k <= 8 ? 8 : 16)); }
+uint32_t pass_width(uint32_t rem) { return rem >= 16 ? 16 : (rem >= 8 ? 8 : (rem >= 4 ? 4 : 1)); }
 
+// Workspace elements (doubles) of passes of up to K right-hand sides:
+// partial sums, z, and with partial leaves the padded copies of x and y.
+size_t matvec_ws_elems(const MatvecPlan& P, uint32_t K) {
+    return ((size_t)std::max<int64_t>(1, P.numPart) + (size_t)std::max<int64_t>(1, P.numCols) +
+            (P.gather ? 2 * (size_t)P.npad : 0)) * K;
+}
+
+// Any k: the widest kernels first (k = 21 runs 16 + 4 + 1 right-hand sides),
+// on the workspaces W (sized for passes of pass_width(k) right-hand sides).
+void run_matvec_plan_ws(Context& c, const MatvecPlan& P, const MvWork& W, int b, uint32_t k, const double* X, int64_t ldx,
+                        double* Y, hipEvent_t* ev) {
+    for (uint32_t s0 = 0; s0 < k;) {
+        const uint32_t K = pass_width(k - s0), cnt = (k - s0) / K * K;
+        const double* Xs = X + (size_t)s0 * ldx;
+        double* Ys = Y + (size_t)s0 * ldx;
+        switch (K) {
+            case 1: run_k<1>(c, P, W, b, cnt, Xs, ldx, Ys, ev); break;
+            case 4: run_k<4>(c, P, W, b, cnt, Xs, ldx, Ys, ev); break;
+            case 8: run_k<8>(c, P, W, b, cnt, Xs, ldx, Ys, ev); break;
+            default: run_k<16>(c, P, W, b, cnt, Xs, ldx, Ys, ev); break;
+        }
+        s0 += cnt;
+    }
+}
+
+// The same on the plan's own workspaces (grown here when too small).
 void run_matvec_plan(Context& c, MatvecPlan& P, int b, uint32_t k, const double* X, int64_t ldx, double* Y,
                      hipEvent_t* ev) {
-    const uint32_t K = kernel_width(k);
-    HMX_CHECK(k == 1 || k % K == 0, HMX_EINVAL, "vectorWidth must be 1 or a multiple of 4/8/16");
-    ensure_matvec_ws(P, K);
-    switch (K) {
-        case 1: run_k<1>(c, P, b, k, X, ldx, Y, ev); break;
-        case 4: run_k<4>(c, P, b, k, X, ldx, Y, ev); break;
-        case 8: run_k<8>(c, P, b, k, X, ldx, Y, ev); break;
-        default: run_k<16>(c, P, b, k, X, ldx, Y, ev); break;
-    }
+    ensure_matvec_ws(P, pass_width(k));
+    const MvWork W{P.zpart.p, P.z.p, P.Xp.p, P.Yp.p};
+    run_matvec_plan_ws(c, P, W, b, k, X, ldx, Y, ev);
 }
 
 void hmatvec(Context& c, HMatrix& h, uint32_t k, const double* d_in, double* d_out, bool transpose, float* ms,

@@ -14,10 +14,14 @@
 // Mixed precision: the columns of every tile are split by a global energy
 // budget.  A column j of tile t contributes ||u_j v_j^T||_F = ||v_j|| (U has
 // orthonormal columns).  The columns whose squared norms sum to at most
-// (target / (2*sqrt(2)*2^-11/sqrt(3)))^2 * ||H||_F^2 are stored in FP16 with
+// (target / (2*2^-11/sqrt(3)))^2 * ||H||_F^2 are stored in FP16 with
 // per-column scales (max-abs of u_j and of v_j) and multiplied on the FP16
 // matrix cores with FP32 accumulation, the rest (and the dense diagonal
-// blocks) in FP32 on the FP32 matrix cores.
+// blocks) in FP32 on the FP32 matrix cores.  The right-hand sides go to FP16
+// divided by their max-abs; FP16 z holds V^T x in those units times one
+// plan-wide power of two (no z can overflow, whatever the scale of x), and
+// phase 2 applies the right-hand-side scales to the FP32 sum of the FP16
+// terms.
 #include <algorithm>
 #include <cmath>
 #include <cstdlib>
@@ -98,8 +102,7 @@ __global__ void __launch_bounds__(256) k_mf_phase1(const MfItem* __restrict__ it
                                                    const T* __restrict__ Xb, int sh,
                                                    typename Mfma<T>::acc_t* __restrict__ zpart,
                                                    T* __restrict__ Zt, const int32_t* __restrict__ zmap,
-                                                   const float* __restrict__ colScaleV,
-                                                   const float* __restrict__ rhsScale, int kvalid) {
+                                                   const float* __restrict__ colScaleV, int kvalid) {
     using M = Mfma<T>;
     constexpr int KL = M::KL, VL = 4 * KL, NR = 16 * NB;
     const int lane = threadIdx.x & 63;
@@ -119,9 +122,6 @@ __global__ void __launch_bounds__(256) k_mf_phase1(const MfItem* __restrict__ it
     if (wid >= nItems) return;
     const int lg = lane >> 4, lc = lane & 15;
     using ACC = typename M::acc_t;
-    ACC rs[NB];  // right-hand-side scales: independent of the item, load them first
-#pragma unroll
-    for (int nb = 0; nb < NB; ++nb) rs[nb] = rhsScale ? (ACC)rhsScale[16 * nb + lc] : (ACC)1;
     const MfItem it = items[wid];
     const T* __restrict__ V = (const T*)it.v;
     typename M::accv acc[CG][NB];
@@ -188,7 +188,7 @@ __global__ void __launch_bounds__(256) k_mf_phase1(const MfItem* __restrict__ it
                     const ACC cs = colScaleV ? (ACC)colScaleV[it.zidx + col] : (ACC)1;
 #pragma unroll
                     for (int nb = 0; nb < NB; ++nb)
-                        Zt[zt_index<VL, NR>(zc, 16 * nb + lc)] = (T)(acc[g][nb][v] * cs * rs[nb]);
+                        Zt[zt_index<VL, NR>(zc, 16 * nb + lc)] = (T)(acc[g][nb][v] * cs);
                 }
             }
         return;
@@ -215,8 +215,7 @@ template <class T, class ACC, int NR>
 __global__ void __launch_bounds__(256) k_mf_reduce(int64_t nred, int64_t ldz, const int64_t* __restrict__ redCol,
                                                    const int64_t* __restrict__ colPart,
                                                    const int32_t* __restrict__ colNpart,
-                                                   const float* __restrict__ colScale,
-                                                   const float* __restrict__ rhsScale, const ACC* __restrict__ zpart,
+                                                   const float* __restrict__ colScale, const ACC* __restrict__ zpart,
                                                    T* __restrict__ Zt) {
     const int64_t g = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
     const int s = threadIdx.x & 63;
@@ -236,7 +235,6 @@ __global__ void __launch_bounds__(256) k_mf_reduce(int64_t nred, int64_t ldz, co
     ACC v = ((acc[0] + acc[1]) + (acc[2] + acc[3])) + ((acc[4] + acc[5]) + (acc[6] + acc[7]));
     const int64_t zc = redCol[g];
     if (colScale) v *= (ACC)colScale[zc];
-    if (rhsScale) v *= (ACC)rhsScale[s];
     constexpr int VL = 4 * Mfma<T>::KL;
     Zt[zt_index<VL, NR>(zc, s)] = (T)v;
 }
@@ -378,7 +376,10 @@ __device__ inline void dense_rows(int leaf, const TD* __restrict__ Drm, const in
 // leaf), then -- grouped plans (64-row leaves) -- the tiles spanning the
 // whole group (gptr, per group; null otherwise).  The four waves walk the
 // shared list in the same order, so a tile's z block comes from HBM once per
-// group and reaches the other waves from the caches.
+// group and reaches the other waves from the caches.  Mixed plans: the FP16
+// group (B) goes first into the cleared accumulators, which are then
+// multiplied by loScale[rhs] * loMul (the right-hand-side scales and the
+// plan's z exponent) before the dense blocks and the FP32 group add theirs.
 template <int BL, bool UCM, class TD, class TA, class TB, int NB, class TO>
 __global__ void __launch_bounds__(256) k_mf_phase2g(int S, const int32_t* __restrict__ ptrA, const MfEntry* __restrict__ entA,
                                                     const int32_t* __restrict__ gptrA,
@@ -388,7 +389,8 @@ __global__ void __launch_bounds__(256) k_mf_phase2g(int S, const int32_t* __rest
                                                     const MfEntry* __restrict__ gentB, const TB* __restrict__ ZB,
                                                     const TD* __restrict__ Drm, const int32_t* __restrict__ nearPtr,
                                                     const MfNear* __restrict__ nearB, const TD* __restrict__ Xb,
-                                                    TO* __restrict__ Y, int ldy, int kvalid) {
+                                                    TO* __restrict__ Y, int ldy, int kvalid,
+                                                    const float* __restrict__ loScale, float loMul) {
     using MA = Mfma<TA>;
     using accv = typename MA::accv;
     static_assert(sizeof(typename Mfma<TB>::acc_t) == sizeof(typename MA::acc_t) &&
@@ -409,13 +411,20 @@ __global__ void __launch_bounds__(256) k_mf_phase2g(int S, const int32_t* __rest
     for (int rg = 0; rg < RG; ++rg)
 #pragma unroll
         for (int nb = 0; nb < NB; ++nb) acc[rg][nb] = accv{0, 0, 0, 0};
-    dense_rows<BL, TD, NB>(leaf, Drm, nearPtr, nearB, Xb, lg, lc, kvalid, acc);
-    lr_rows<BL, UCM, TA, NB>(entA, ptrA[leaf], ptrA[leaf + 1], ZA, 0, lg, lc, kvalid, acc);
-    if (gptrA) lr_rows<BL, UCM, TA, NB>(gentA, gptrA[grp], gptrA[grp + 1], ZA, BL * w, lg, lc, kvalid, acc);
     if (ptrB) {
         lr_rows<BL, false, TB, NB>(entB, ptrB[leaf], ptrB[leaf + 1], ZB, 0, lg, lc, kvalid, acc);
         if (gptrB) lr_rows<BL, false, TB, NB>(gentB, gptrB[grp], gptrB[grp + 1], ZB, BL * w, lg, lc, kvalid, acc);
+#pragma unroll
+        for (int nb = 0; nb < NB; ++nb) {
+            const typename MA::acc_t f = 16 * nb + lc < kvalid ? (typename MA::acc_t)(loScale[16 * nb + lc] * loMul)
+                                                                : (typename MA::acc_t)0;
+#pragma unroll
+            for (int rg = 0; rg < RG; ++rg) acc[rg][nb] *= f;
+        }
     }
+    dense_rows<BL, TD, NB>(leaf, Drm, nearPtr, nearB, Xb, lg, lc, kvalid, acc);
+    lr_rows<BL, UCM, TA, NB>(entA, ptrA[leaf], ptrA[leaf + 1], ZA, 0, lg, lc, kvalid, acc);
+    if (gptrA) lr_rows<BL, UCM, TA, NB>(gentA, gptrA[grp], gptrA[grp + 1], ZA, BL * w, lg, lc, kvalid, acc);
 #pragma unroll
     for (int rg = 0; rg < RG; ++rg)
 #pragma unroll
@@ -428,12 +437,13 @@ __global__ void __launch_bounds__(256) k_mf_phase2g(int S, const int32_t* __rest
 
 // ---- right-hand-side conversion (mixed) ---------------------------------------
 // max |X[s][i]| over slice p of the rows: part[s*P + p], grid (P, right-hand sides)
-__global__ void __launch_bounds__(256) k_colmaxabs_part(int n, const float* __restrict__ X, float* __restrict__ part) {
+__global__ void __launch_bounds__(256) k_colmaxabs_part(int n, const float* __restrict__ X, int64_t ldx,
+                                                        float* __restrict__ part) {
     __shared__ float red[256];
     const int P = gridDim.x;
     const int64_t per = ((int64_t)n + P - 1) / P;
     const int64_t i0 = (int64_t)blockIdx.x * per, i1 = min((int64_t)n, i0 + per);
-    const float* x = X + (size_t)blockIdx.y * n;
+    const float* x = X + (size_t)blockIdx.y * ldx;
     float m = 0.f;
     for (int64_t i = i0 + threadIdx.x; i < i1; i += 256) m = fmaxf(m, fabsf(x[i]));
     red[threadIdx.x] = m;
@@ -643,7 +653,7 @@ struct GroupTile {
 
 void build_group(Context& c, const HMatrix& h, int S, bool grouped, MfGroup& G, int type,
                  const std::vector<GroupTile>& tiles, bool vdirect, const std::vector<float>* su,
-                 const std::vector<float>* sv, const std::vector<int64_t>* colBase) {
+                 const std::vector<float>* sv, const std::vector<int64_t>* colBase, float zmul = 1.f) {
     const bool udirect = vdirect;
     const int b = (int)h.b;
     const size_t es = tsize(type);
@@ -725,7 +735,7 @@ void build_group(Context& c, const HMatrix& h, int S, bool grouped, MfGroup& G, 
                     float fu, fv;
                     scales(t, q, fu, fv);
                     ucols.push_back(CvCol{Usrc + (size_t)tiles[t].cols[q] * m, 0, m, fu});
-                    colScale.push_back(fu * fv);
+                    colScale.push_back(fu * fv * zmul);
                 }
             }
             for (int q = cntRun; q < rpad; ++q) colScale.push_back(0.f);
@@ -891,12 +901,6 @@ void check_bucket(const HMatrix& h) {
     HMX_CHECK(h.b == 32 || h.b == 64, HMX_EUNSUPPORTED, "the multi-RHS matvec kernels are built for bucketSize 32 or 64");
 }
 
-void check_shape(const HMatrix& h) {
-    HMX_CHECK(h.n == h.S * h.b, HMX_EUNSUPPORTED,
-              "the mixed-precision matvec requires full leaves (n = numSegments * bucketSize)");
-    check_bucket(h);
-}
-
 // leaf-level tiles of rank above b/2 are applied as dense blocks
 bool is_near(const HMatrix& h, const TileRef& R) {
     return h.planDenseNear && R.m == (int)h.b && 2 * R.r > (int)h.b;
@@ -1013,29 +1017,40 @@ void build_fp64_plan_set(Context& c, const std::vector<HMatrix*>& hs, MfPlan& P)
 }
 
 template <class ACC, class T>
-void reduce_group(Context& c, MfGroup& G, int NR, const float* rhsScale) {
+void reduce_group(Context& c, MfGroup& G, int NR) {
     if (G.numRed == 0) return;
     const unsigned blocks = (unsigned)((G.numRed + 3) / 4);
     const float* cs = G.colScale.n ? G.colScale.p : nullptr;
     if (NR == 64)
         k_mf_reduce<T, ACC, 64><<<blocks, 256, 0, c.stream>>>(G.numRed, G.numColsPad, G.redCol.p, G.colPart.p,
-                                                              G.colNpart.p, cs, rhsScale, (const ACC*)G.zpart.p,
-                                                              (T*)G.zt.p);
+                                                              G.colNpart.p, cs, (const ACC*)G.zpart.p, (T*)G.zt.p);
     else
         k_mf_reduce<T, ACC, 16><<<blocks, 256, 0, c.stream>>>(G.numRed, G.numColsPad, G.redCol.p, G.colPart.p,
-                                                              G.colNpart.p, cs, rhsScale, (const ACC*)G.zpart.p,
-                                                              (T*)G.zt.p);
+                                                              G.colNpart.p, cs, (const ACC*)G.zpart.p, (T*)G.zt.p);
     HMX_LAUNCHED();
 }
 
 // Workspaces.  The padding columns of z (rank rounded up to 16) are never
-// written by the kernels and stay at the zero set here.
-void ensure_ws(Context& c, MfGroup& G, int NR, size_t accBytes) {
-    if (G.wsNR >= (size_t)NR) return;
-    G.zpart.alloc((size_t)std::max<int64_t>(1, G.numPart) * NR * accBytes);
-    G.zt.alloc((size_t)std::max<int64_t>(1, G.numColsPad) * NR * tsize(G.type));
-    HMX_HIP(hipMemsetAsync(G.zt.p, 0, G.zt.bytes(), c.stream));
-    G.wsNR = NR;
+// written by the kernels and must hold zeros: U is zero there, but 0 * NaN
+// left by an earlier call would not be.  Their positions depend on NR
+// (zt_index), so z is cleared again whenever the pass width changes.
+// Returns true when a workspace was allocated or cleared (launch sequences
+// captured before then point at memory in another state).
+bool ensure_ws(Context& c, MfGroup& G, int NR, size_t accBytes) {
+    bool changed = false;
+    if (G.wsNR < (size_t)NR) {
+        G.zpart.alloc((size_t)std::max<int64_t>(1, G.numPart) * NR * accBytes);
+        G.zt.alloc((size_t)std::max<int64_t>(1, G.numColsPad) * NR * tsize(G.type));
+        G.wsNR = NR;
+        G.ztNR = 0;
+        changed = true;
+    }
+    if (G.ztNR != NR) {
+        HMX_HIP(hipMemsetAsync(G.zt.p, 0, G.zt.bytes(), c.stream));
+        G.ztNR = NR;
+        changed = true;
+    }
+    return changed;
 }
 
 // workgroups per XCD chunk in phase 1 (0: hardware order)
@@ -1048,7 +1063,7 @@ int phase1_chunk() {
 }
 
 template <class T, int CG, int NB>
-void phase1(Context& c, MfGroup& G, const T* Xb, int sh, const float* rhsScale, int kvalid, int xcdChunk = -1) {
+void phase1(Context& c, MfGroup& G, const T* Xb, int sh, int kvalid, int xcdChunk = -1) {
     if (G.numItems == 0) return;
     const float* cs = G.colScaleV.n ? G.colScaleV.p : nullptr;
     const int chunk = xcdChunk >= 0 ? xcdChunk : phase1_chunk();
@@ -1056,7 +1071,7 @@ void phase1(Context& c, MfGroup& G, const T* Xb, int sh, const float* rhsScale, 
     if (chunk > 0) blocks = (blocks + 8 * chunk - 1) / (8 * chunk) * (8 * chunk);
     k_mf_phase1<T, CG, NB><<<blocks, 256, 0, c.stream>>>(G.items.p, G.numItems, chunk, Xb, sh,
                                                          (typename Mfma<T>::acc_t*)G.zpart.p, (T*)G.zt.p, G.zmap.p,
-                                                         cs, rhsScale, kvalid);
+                                                         cs, kvalid);
     HMX_LAUNCHED();
 }
 
@@ -1076,18 +1091,6 @@ struct Events {
     }
 };
 
-int leaf_shift(const HMatrix& h) { return h.b == 64 ? 6 : 5; }
-
-// X ([s][n], full leaves) -> blocked copy, optionally divided by a
-// per-right-hand-side scale
-template <class TI, class TO>
-void to_blocked(Context& c, const HMatrix& h, int NR, const TI* X, const float* sx, TO* Xb) {
-    const int64_t tot = (int64_t)h.n * NR;
-    k_x_blocked<TI, TO><<<(unsigned)((tot + 255) / 256), 256, 0, c.stream>>>(tot, (int64_t)h.n, NR, leaf_shift(h), X,
-                                                                             (int64_t)h.n, NR, nullptr, sx, Xb);
-    HMX_LAUNCHED();
-}
-
 // One pass of up to 16*NB right-hand sides (kk of them real) through an FP64
 // plan: X and Y hold the caller's entries (ld ldx / ldy); with partial leaves
 // X is gathered into the padded space and the product scattered back.
@@ -1101,9 +1104,9 @@ void run_fp64(Context& c, MfPlan& P, const double* X, int64_t ldx, int kk, doubl
     k_x_blocked<double, double><<<(unsigned)((tot + 255) / 256), 256, 0, c.stream>>>(
         tot, P.npad, NR, sh, X, ldx, kk, P.gather ? P.gmap.p : nullptr, nullptr, (double*)P.Xb.p);
     HMX_LAUNCHED();
-    phase1<double, 4, NB>(c, G, Xb, sh, nullptr, kk, P.xcdChunk);  // 4 column groups = the 64 columns of an item
+    phase1<double, 4, NB>(c, G, Xb, sh, kk, P.xcdChunk);  // 4 column groups = the 64 columns of an item
     ev.rec(c, 1);
-    reduce_group<double, double>(c, G, NR, nullptr);
+    reduce_group<double, double>(c, G, NR);
     ev.rec(c, 2);
     double* Yo = P.gather ? (double*)P.Yp.p : Y;
     const int ldo = (int)(P.gather ? P.npad : ldy);
@@ -1116,7 +1119,7 @@ void run_fp64(Context& c, MfPlan& P, const double* X, int64_t ldx, int kk, doubl
                                      : k_mf_phase2g<32, false, double, double, double, NB, double>);
         k2<<<(P.S + 3) / 4, 256, 0, c.stream>>>(P.S, G.leafPtr.p, G.entries.p, gp, ge, (const double*)G.zt.p,
                                                 nullptr, nullptr, nullptr, nullptr, nullptr, (const double*)P.Drm.p,
-                                                P.nearPtr.p, P.near.p, Xb, Yo, ldo, kk);
+                                                P.nearPtr.p, P.near.p, Xb, Yo, ldo, kk, nullptr, 1.f);
     }
     HMX_LAUNCHED();
     if (P.gather) {
@@ -1127,38 +1130,52 @@ void run_fp64(Context& c, MfPlan& P, const double* X, int64_t ldx, int kk, doubl
     ev.rec(c, 3);
 }
 
+// One pass of up to 16*NB right-hand sides (kk of them real) through the
+// mixed plan, X and Y FP32 with the caller's entries (ld ldx / ldy); with
+// partial leaves X is gathered into the padded space and Y scattered back.
 template <int NB>
-void run_mixed(Context& c, HMatrix& h, const float* X, float* Y, Events& ev) {
-    MfPlan& P = h.mfMixed;
+void run_mixed(Context& c, MfPlan& P, const float* X, int64_t ldx, int kk, float* Y, int64_t ldy, Events& ev) {
     MfGroup& Ghi = P.g[0];
     MfGroup& Glo = P.g[1];
-    const int n = (int)h.n, NR = 16 * NB;
+    const int NR = 16 * NB, sh = P.b == 64 ? 6 : 5;
+    const int32_t* gm = P.gather ? P.gmap.p : nullptr;
     ev.rec(c, 0);
     // right-hand sides to FP16 with one scale per column
-    k_colmaxabs_part<<<dim3(kXSlices, NR), 256, 0, c.stream>>>(n, X, P.xpart.p);
+    k_colmaxabs_part<<<dim3(kXSlices, kk), 256, 0, c.stream>>>((int)P.n, X, ldx, P.xpart.p);
     HMX_LAUNCHED();
-    k_colmaxabs_fin<<<NR, 64, 0, c.stream>>>(kXSlices, P.xpart.p, P.xscale.p);
+    k_colmaxabs_fin<<<kk, 64, 0, c.stream>>>(kXSlices, P.xpart.p, P.xscale.p);
+    HMX_LAUNCHED();
+    const int64_t tot = P.npad * NR;
+    const unsigned gx = (unsigned)((tot + 255) / 256);
+    k_x_blocked<float, _Float16><<<gx, 256, 0, c.stream>>>(tot, P.npad, NR, sh, X, ldx, kk, gm, P.xscale.p, P.X16.p);
+    HMX_LAUNCHED();
+    k_x_blocked<float, float><<<gx, 256, 0, c.stream>>>(tot, P.npad, NR, sh, X, ldx, kk, gm, nullptr, (float*)P.Xb.p);
     HMX_LAUNCHED();
     const float* Xb = (const float*)P.Xb.p;
-    to_blocked<float, _Float16>(c, h, NR, X, P.xscale.p, P.X16.p);
-    to_blocked<float, float>(c, h, NR, X, nullptr, (float*)P.Xb.p);
-    phase1<float, 4, NB>(c, Ghi, Xb, leaf_shift(h), nullptr, NR);
-    phase1<_Float16, 4, NB>(c, Glo, P.X16.p, leaf_shift(h), P.xscale.p, NR);
+    phase1<float, 4, NB>(c, Ghi, Xb, sh, kk);
+    phase1<_Float16, 4, NB>(c, Glo, P.X16.p, sh, kk);
     ev.rec(c, 1);
-    reduce_group<float, float>(c, Ghi, NR, nullptr);
-    reduce_group<float, _Float16>(c, Glo, NR, P.xscale.p);
+    reduce_group<float, float>(c, Ghi, NR);
+    reduce_group<float, _Float16>(c, Glo, NR);
     ev.rec(c, 2);
+    float* Yo = P.gather ? (float*)P.Yp.p : Y;
+    const int ldo = (int)(P.gather ? P.npad : ldy);
     {
         const bool gr = Ghi.grouped && Glo.grouped;
-        auto k2 = h.b == 64 ? k_mf_phase2g<64, false, float, float, _Float16, NB, float>
+        auto k2 = P.b == 64 ? k_mf_phase2g<64, false, float, float, _Float16, NB, float>
                             : k_mf_phase2g<32, false, float, float, _Float16, NB, float>;
-        k2<<<(h.S + 3) / 4, 256, 0, c.stream>>>(
-            (int)h.S, Ghi.leafPtr.p, Ghi.entries.p, gr ? Ghi.grpPtr.p : nullptr, gr ? Ghi.grpEntries.p : nullptr,
+        k2<<<(P.S + 3) / 4, 256, 0, c.stream>>>(
+            P.S, Ghi.leafPtr.p, Ghi.entries.p, gr ? Ghi.grpPtr.p : nullptr, gr ? Ghi.grpEntries.p : nullptr,
             (const float*)Ghi.zt.p, Glo.leafPtr.p, Glo.entries.p, gr ? Glo.grpPtr.p : nullptr,
             gr ? Glo.grpEntries.p : nullptr, (const _Float16*)Glo.zt.p, (const float*)P.Drm.p, P.nearPtr.p, P.near.p,
-            Xb, Y, n, NR);
+            Xb, Yo, ldo, kk, P.xscale.p, P.loMul);
     }
     HMX_LAUNCHED();
+    if (P.gather) {
+        k_y_scatter<float><<<(unsigned)((P.npad * kk + 255) / 256), 256, 0, c.stream>>>(P.npad, kk, P.gmap.p,
+                                                                                       (const float*)P.Yp.p, Y, ldy);
+        HMX_LAUNCHED();
+    }
     ev.rec(c, 3);
 }
 
@@ -1177,11 +1194,12 @@ void finish_timing(Events& ev, float* acc) {
 }  // namespace
 
 // Workspaces of an FP64 plan for passes of NR right-hand sides
-void ensure_fp64_ws(Context& c, MfPlan& P, int NR) {
-    ensure_ws(c, P.g[0], NR, sizeof(double));
+bool ensure_fp64_ws(Context& c, MfPlan& P, int NR) {
+    bool changed = ensure_ws(c, P.g[0], NR, sizeof(double));
     const size_t xb = (size_t)P.npad * NR * sizeof(double);
-    if (P.Xb.n < xb) P.Xb.alloc(xb);
-    if (P.gather && P.Yp.n < xb) P.Yp.alloc(xb);
+    if (P.Xb.n < xb) { P.Xb.alloc(xb); changed = true; }
+    if (P.gather && P.Yp.n < xb) { P.Yp.alloc(xb); changed = true; }
+    return changed;
 }
 
 // Y = H X for k right-hand sides (ld = the plan's n) in passes of 64 (k a
@@ -1223,8 +1241,10 @@ bool batch_mf_stale(const Batch& B) {
 }
 
 // the FP64 plan of a batch, rebuilt when an instance's factors were
-// overwritten since (the plan may hold copies of them)
-void batch_mf_prepare(Context& c, Batch& B, uint32_t k) {
+// overwritten since (the plan may hold copies of them); true when the plan
+// was rebuilt or a workspace allocated or cleared
+bool batch_mf_prepare(Context& c, Batch& B, uint32_t k) {
+    bool changed = false;
     if (batch_mf_stale(B)) {
         const int xc = B.mf.xcdChunk;
         B.mf = MfPlan();
@@ -1234,8 +1254,9 @@ void batch_mf_prepare(Context& c, Batch& B, uint32_t k) {
         build_fp64_plan_set(c, B.hs, B.mf);
         B.mfGen.clear();
         for (const HMatrix* h : B.hs) B.mfGen.push_back(h->factorGen);
+        changed = true;
     }
-    ensure_fp64_ws(c, B.mf, (k % 64 == 0) ? 64 : 16);
+    return ensure_fp64_ws(c, B.mf, (k % 64 == 0) ? 64 : 16) || changed;
 }
 
 void batch_mf_run(Context& c, Batch& B, uint32_t k, const double* d_in, double* d_out) {
@@ -1246,7 +1267,8 @@ void batch_mf_run(Context& c, Batch& B, uint32_t k, const double* d_in, double* 
 int default_xcd_chunk() { return phase1_chunk(); }
 
 void hmatrix_make_mixed(Context& c, HMatrix& h, double target) {
-    check_shape(h);
+    check_bucket(h);
+    HMX_CHECK(h.leafOffsets.size() == (size_t)h.S + 1, HMX_ESTATE, "H-matrix without leaf offsets");
     HMX_CHECK(target > 0 && target < 1, HMX_EINVAL, "target relative error must be in (0, 1)");
     MfPlan& P = h.mfMixed;
     P = MfPlan();
@@ -1279,6 +1301,8 @@ void hmatrix_make_mixed(Context& c, HMatrix& h, double target) {
     double frob = 0;
     for (double x : hd) frob += x * x;
     for (double x : v2) frob += x;
+    // partial leaves: the diagonal blocks hold 1 at each padding position
+    frob -= (double)((int64_t)h.S * h.b - (int64_t)h.n);
     // FP16 storage error of a column: rounding of u, v, x and z (rms 2^-11/sqrt(3) each)
     const double delta = std::ldexp(1.0, -11) / std::sqrt(3.0);
     const double budget = (target / (2.0 * delta)) * (target / (2.0 * delta)) * frob;
@@ -1304,6 +1328,17 @@ void hmatrix_make_mixed(Context& c, HMatrix& h, double target) {
         su[j] = um[j] > 0 ? (float)um[j] : 1.f;
         sv[j] = vm[j] > 0 ? (float)vm[j] : 1.f;
     }
+    // FP16 z of column j = (V16_j . X16) * su_j * sv_j * 2^E with |V16|, |X16|
+    // <= 1, so |z| <= m_j * su_j * sv_j * 2^E: one exponent E for the plan
+    // puts the largest such bound at 2^15 (below the FP16 maximum 65504, and
+    // the small columns as far above the subnormal range as that allows).
+    // Phase 2 multiplies the FP16 sum by 2^-E and the right-hand-side scale.
+    double zb = 0;
+    for (size_t t = 0; t < tiles.size(); ++t)
+        for (int j = 0; j < tiles[t].r; ++j)
+            if (lo[colBase[t] + j])
+                zb = std::max(zb, (double)su[colBase[t] + j] * (double)sv[colBase[t] + j] * tiles[t].m);
+    const int zexp = zb > 0 ? (int)std::floor(std::log2(32768.0 / zb)) : 0;
     const bool grouped = h.b == 64 && h.S >= 4;
     build_near<float>(c, h, (int)h.S, P, tiles);
     std::vector<GroupTile> gh, gl;
@@ -1322,7 +1357,7 @@ void hmatrix_make_mixed(Context& c, HMatrix& h, double target) {
         gl.push_back(std::move(b2));
     }
     build_group(c, h, (int)h.S, grouped, P.g[0], 1, gh, false, nullptr, nullptr, nullptr);
-    build_group(c, h, (int)h.S, grouped, P.g[1], 2, gl, false, &su, &sv, &colBase);
+    build_group(c, h, (int)h.S, grouped, P.g[1], 2, gl, false, &su, &sv, &colBase, (float)std::ldexp(1.0, zexp));
     const int64_t tot = (int64_t)h.S * h.b * h.b;
     P.Drm.alloc(tot * sizeof(float));
     k_diag_rowmajor<float><<<(unsigned)((tot + 255) / 256), 256, 0, c.stream>>>(tot, (int)h.b, h.diag.p,
@@ -1330,6 +1365,14 @@ void hmatrix_make_mixed(Context& c, HMatrix& h, double target) {
     HMX_LAUNCHED();
     HMX_HIP(hipStreamSynchronize(c.stream));
     P.dtype = 1;
+    P.loMul = (float)std::ldexp(1.0, -zexp);
+    P.S = (int)h.S;
+    P.b = (int)h.b;
+    P.n = h.n;
+    P.npad = (int64_t)h.S * h.b;
+    P.gather = P.n != P.npad;
+    if (P.gather) upload(c, P.gmap, padding_map(h, 0));
+    HMX_HIP(hipStreamSynchronize(c.stream));
     P.frobH2 = frob;
     P.loEnergy = used;
     P.target = target;
@@ -1340,32 +1383,47 @@ void hmatrix_make_mixed(Context& c, HMatrix& h, double target) {
     P.built = true;
 }
 
+// Workspaces of the mixed plan for passes of NR right-hand sides; true when
+// any was allocated or cleared
+bool ensure_mixed_ws(Context& c, MfPlan& P, int NR) {
+    bool changed = ensure_ws(c, P.g[0], NR, sizeof(float));
+    changed = ensure_ws(c, P.g[1], NR, sizeof(float)) || changed;
+    if (P.x16n < (size_t)P.npad * NR) {
+        P.X16.alloc((size_t)P.npad * NR);
+        P.Xb.alloc((size_t)P.npad * NR * sizeof(float));
+        if (P.gather) P.Yp.alloc((size_t)P.npad * NR * sizeof(float));
+        P.x16n = (size_t)P.npad * NR;
+        changed = true;
+    }
+    if (P.xscale.n < (size_t)NR) {
+        P.xscale.alloc(NR);
+        P.xpart.alloc((size_t)NR * kXSlices);
+        changed = true;
+    }
+    return changed;
+}
+
+// Y = H X for any k >= 1 (FP32, ld n): passes of 64 (k a multiple of 64) or
+// 16, the last one padded with zero columns
 void hmatvec_mixed(Context& c, HMatrix& h, uint32_t k, const float* d_in, float* d_out, float* ms, int iters) {
     HMX_CHECK(h.mfMixed.built, HMX_ESTATE, "mixed-precision factors not built (call hmx_hmatrix_make_mixed)");
-    HMX_CHECK(k >= 16 && k % 16 == 0, HMX_EINVAL, "mixed matvec: vectorWidth must be a multiple of 16");
+    HMX_CHECK(k >= 1, HMX_EINVAL, "mixed matvec: vectorWidth must be >= 1");
     HMX_CHECK(d_in && d_out && d_in != d_out, HMX_EINVAL, "bad vector pointers");
     MfPlan& P = h.mfMixed;
     const int NB = (k % 64 == 0) ? 4 : 1;
     const int NR = 16 * NB;
-    ensure_ws(c, P.g[0], NR, sizeof(float));
-    ensure_ws(c, P.g[1], NR, sizeof(float));
-    if (P.x16n < (size_t)h.n * NR) {
-        P.X16.alloc((size_t)h.n * NR);
-        P.Xb.alloc((size_t)h.n * NR * sizeof(float));
-        P.xscale.alloc(NR);
-        P.xpart.alloc((size_t)NR * kXSlices);
-        P.x16n = (size_t)h.n * NR;
-    }
+    ensure_mixed_ws(c, P, NR);
     Events ev;
     if (ms) ev.create();
     const int reps = ms ? std::max(1, iters) : 1;
     float acc[4] = {0, 0, 0, 0};
     for (int it = 0; it < reps; ++it) {
         for (uint32_t s0 = 0; s0 < k; s0 += NR) {
+            const int kk = (int)std::min<uint32_t>(NR, k - s0);
             const float* X = d_in + (size_t)s0 * h.n;
             float* Y = d_out + (size_t)s0 * h.n;
-            if (NB == 4) run_mixed<4>(c, h, X, Y, ev);
-            else run_mixed<1>(c, h, X, Y, ev);
+            if (NB == 4) run_mixed<4>(c, P, X, h.n, kk, Y, h.n, ev);
+            else run_mixed<1>(c, P, X, h.n, kk, Y, h.n, ev);
         }
         finish_timing(ev, acc);
     }

@@ -635,11 +635,12 @@ class Pipeline:
         return self.mixed_info
 
     def matvec_mixed(self, x, y=None):
-        """x: (k, n) float32 contiguous, k a multiple of 16."""
+        """x: (n,) or (k, n) float32 contiguous (an n x k column-major block)."""
         import torch
+        k = 1 if x.dim() == 1 else x.shape[0]
         if y is None:
             y = torch.empty_like(x)
-        HMatrixVecMultMixed(self.ctx, self.H, x.shape[0], x, y)
+        HMatrixVecMultMixed(self.ctx, self.H, k, x, y)
         return y
 
     def matvec(self, x, y=None, transpose=False, original_order=False):

@@ -165,7 +165,7 @@ def test_batch_rejects_mixed_bucket_sizes(torch_cuda, ctx):
 def test_batched_build_matches_instances(torch_cuda, ctx):
     """Upper levels of several clouds compressed together equal, tile for
     tile, each cloud's own direct build: same diagonal and leaf level bit for
-    bit, same ranks up to threshold-borderline columns, products within the
+    bit, the same ranks on every upper-level tile, products within the
     level tolerance; and the batch's matvec against dense kernel rows."""
     specs = cloud_specs(6, 21, nmin=2048, nmax=8192)
     pts = [to_dev(torch_cuda, O.generate_points(n, d, 300 + i)) for i, (n, d, ell) in enumerate(specs)]
@@ -192,7 +192,7 @@ def test_batched_build_matches_instances(torch_cuda, ctx):
                     assert np.linalg.norm(Ub[t] @ Vb[t].T - Us[t] @ Vs[t].T, 2) <= 2 * tol
     diffs = np.array(diffs)
     print(f"batched vs single upper-level ranks: {len(diffs)} tiles, mismatches {(diffs != 0).sum()}")
-    assert diffs.max() <= 3 and (diffs != 0).mean() <= 0.05
+    assert diffs.max() == 0
     tot = bp.batch.info()["totalPoints"]
     X = torch_cuda.randn(8, tot, dtype=torch_cuda.float64, device="cuda")
     Y = bp.matvec(X, graph=True)
@@ -269,3 +269,46 @@ def test_batch_plan_follows_factor_updates(torch_cuda, ctx):
     assert torch_cuda.equal(Y[:, a:e], y0)
     assert torch_cuda.equal(B.matvec(X), Y)
     assert torch_cuda.equal(Y[:, e:], before[:, e:])
+
+
+def test_batch_graph_recaptured_after_eager_changes(torch_cuda, ctx):
+    """A replayed graph never runs on plans or workspaces an eager call
+    rebuilt or reallocated since the capture: (1) capture at 8 right-hand
+    sides, overwrite a level, eager call (rebuilds the batch plan), replay;
+    (2) capture at 16, eager call at 64 (grows the workspaces), replay at 16.
+    Every replay equals the eager product of the same inputs bit for bit."""
+    specs, pipes, B = small_batch(torch_cuda, ctx, 2, seed=31)
+    tot = B.info()["totalPoints"]
+    g = torch_cuda.Generator("cuda").manual_seed(9)
+    X8 = torch_cuda.randn(8, tot, dtype=torch_cuda.float64, device="cuda", generator=g)
+    Y8 = torch_cuda.empty_like(X8)
+    B.matvec(X8, Y8, graph=True)
+    H = pipes[1].H
+    lvl = next(l for l in range(1, H.num_levels) if H.level_info(l)["rankSum"] > 0)
+    m = (1 << (H.num_levels - 1 - lvl)) * H.b
+    rs = H.level_info(lvl)["rankSum"]
+    u = torch_cuda.empty(rs * m, dtype=torch_cuda.float64, device="cuda")
+    v = torch_cuda.empty_like(u)
+    L = hmx.lib()
+    assert L.hmx_hmatrix_level_get(ctx.handle, H.handle, lvl, u.data_ptr(), v.data_ptr()) == hmx.HMX_OK
+    v.mul_(0.5)
+    assert L.hmx_hmatrix_set_level(ctx.handle, H.handle, lvl, u.data_ptr(), v.data_ptr()) == hmx.HMX_OK
+    eager = B.matvec(X8)
+    for rep in range(3):
+        Y8.fill_(float("nan"))
+        B.matvec(X8, Y8, graph=True)
+        torch_cuda.cuda.synchronize()
+        assert torch_cuda.equal(Y8, eager), rep
+    X16 = torch_cuda.randn(16, tot, dtype=torch_cuda.float64, device="cuda", generator=g)
+    Y16 = torch_cuda.empty_like(X16)
+    B.matvec(X16, Y16, graph=True)
+    ref16 = B.matvec(X16)
+    X64 = torch_cuda.randn(64, tot, dtype=torch_cuda.float64, device="cuda", generator=g)
+    B.matvec(X64)
+    for rep in range(3):
+        Y16.fill_(float("nan"))
+        B.matvec(X16, Y16, graph=True)
+        torch_cuda.cuda.synchronize()
+        assert torch_cuda.equal(Y16, ref16), rep
+        B.matvec(X64)  # eager work of another width between replays
+    assert torch_cuda.equal(B.matvec(X16), ref16)

@@ -9,8 +9,9 @@ Bars:
     count above tolerance/1000 (numpy SVD);
   * the direct build's expansion error against the dense kernel matrix is
     within 1.5x of the TLR path's;
-  * with b = 64 the TLR path's ranks match the CPU oracle's (up to threshold-
-    borderline columns) and both matvec paths (streaming FP64, matrix-core
+  * with b = 64 the TLR path's ranks equal the CPU oracle's and the direct
+    build's upper-level ranks equal the oracle's ARA on the dense tile with
+    the same random stream; both matvec paths (streaming FP64, matrix-core
     FP64, mixed) equal the dense expansion within their bars.
 """
 import numpy as np
@@ -150,7 +151,7 @@ def test_b64_ranks_vs_oracle(torch_cuda, ctx):
         if len(ro):
             dd = np.abs(rg.astype(int) - ro.astype(int))
             print(f"level {l}: {len(ro)} tiles, rank mismatches {(dd != 0).sum()}, max {dd.max()}")
-            assert dd.max() <= 4 and (dd != 0).mean() < 0.05
+            assert dd.max() == 0, l
 
 
 def test_b64_mixed(trio, ctx, torch_cuda):
@@ -165,20 +166,20 @@ def test_b64_mixed(trio, ctx, torch_cuda):
     y64 = direct.matvec(x32.double()).cpu().numpy().T
     rel = np.linalg.norm(y32 - y64) / np.linalg.norm(y64)
     print(f"mixed b={tlr.b} (all plan options): hi {info['hiCols']} lo {info['loCols']} rel vs FP64 {rel:.3e}")
-    assert rel < 2 * target
+    assert rel <= target
     direct.H.set_plan_options(0, 0, 0)
     info = direct.make_mixed(target)
     y32 = direct.matvec_mixed(x32).cpu().numpy().T.astype(np.float64)
     rel = np.linalg.norm(y32 - y64) / np.linalg.norm(y64)
     print(f"mixed b={tlr.b} (default plan): rel vs FP64 {rel:.3e}")
-    assert rel < 2 * target
+    assert rel <= target
 
 
 def test_direct_ranks_vs_oracle_ara(trio, ctx):
     """Upper-level tiles of the direct build against the oracle's ARA run on the
     dense kernel tile with the same random stream (seed, operation id
-    ((level+1) << 40) | tile, column index): same ranks up to threshold-
-    borderline columns, products within the level tolerance."""
+    ((level+1) << 40) | tile, column index): the same ranks on every sampled
+    tile, products within the level tolerance."""
     tlr, direct, p, K = trio
     n, b = tlr.n, tlr.b
     L = tlr.structure.num_levels()
@@ -203,4 +204,4 @@ def test_direct_ranks_vs_oracle_ara(trio, ctx):
                 assert np.linalg.norm(Us[t] @ Vs[t].T - Uo @ Vo.T, 2) <= 2 * info["tol"]
     diffs = np.array(diffs)
     print(f"direct vs oracle ARA: {len(diffs)} tiles, rank mismatches {(diffs != 0).sum()}, max {diffs.max()}")
-    assert diffs.max() <= 3 and (diffs != 0).mean() <= 0.1
+    assert diffs.max() == 0
