@@ -208,6 +208,19 @@ hmx_status hmx_freeHMatrix(hmx_hmatrix* h);
    matrix cores (passes of 16, or 64 when vectorWidth is a multiple of 64). */
 hmx_status hmx_HMatrixVecMult(hmx_context* ctx, const hmx_hmatrix* h, uint32_t vectorWidth, const double* d_in,
                               double* d_out);
+/* No counterpart in the reference: for callers that cannot allocate device
+   memory (code under HIP graph capture, tight serving loops).  out = H * in
+   as hmx_HMatrixVecMult computes it with one right-hand side (the FP64
+   streaming kernels; vectorWidth >= 2 runs them in passes of 16, 8, 4 or 1,
+   not the matrix cores), calling no hipMalloc, hipFree or other allocating
+   routine.  Its workspaces live in a fixed 64 MiB device array held per GPU
+   until the process ends; an H-matrix whose passes need more is rejected
+   with HMX_EINVAL (hmx_last_error names the size).  The H-matrix's streaming
+   plan must exist: the first hmx_HMatrixVecMult call with vectorWidth 1
+   builds it, else HMX_ESTATE.  Calls on one GPU share the array: calls that
+   overlap (other streams or threads) race. */
+hmx_status hmx_HMatrixVecMultNoAlloc(hmx_context* ctx, const hmx_hmatrix* h, uint32_t vectorWidth,
+                                     const double* d_in, double* d_out);
 /* out = H^T * in (no reference counterpart; the backward of HMatrixVecMult);
    any vectorWidth >= 1 and any n, on the streaming kernels in passes of 16,
    8, 4 or 1 right-hand sides */

@@ -57,7 +57,10 @@ hmx_status hmx_create(hmx_context** ctx, void* stream) {
     hmx_context* c = new (std::nothrow) hmx_context();
     if (!c) return HMX_ENOMEM;
     c->stream = (hipStream_t)stream;
-    hmx_status st = guard(c, [&] { HMX_HIP(hipGetDevice(&c->device)); });
+    hmx_status st = guard(c, [&] {
+        HMX_HIP(hipGetDevice(&c->device));
+        noalloc_init(c->device);  // this device's copy of the no-allocation matvec's scratch array
+    });
     if (st != HMX_OK) {
         g_noctx_error = c->last_error;
         delete c;
@@ -430,6 +433,15 @@ hmx_status hmx_HMatrixVecMult(hmx_context* ctx, const hmx_hmatrix* h, uint32_t v
             hmatvec_multi(*ctx, const_cast<hmx_hmatrix&>(*h), vectorWidth, d_in, d_out, nullptr, 0);
         else
             hmatvec(*ctx, const_cast<hmx_hmatrix&>(*h), vectorWidth, d_in, d_out, false, nullptr, 0);
+    });
+}
+
+hmx_status hmx_HMatrixVecMultNoAlloc(hmx_context* ctx, const hmx_hmatrix* h, uint32_t vectorWidth,
+                                     const double* d_in, double* d_out) {
+    if (!ctx) return HMX_EINVAL;
+    return guard(ctx, [&] {
+        NEED(h);
+        hmatvec_noalloc(*ctx, *h, vectorWidth, d_in, d_out);
     });
 }
 

@@ -362,6 +362,9 @@ void mfma_selftest(Context& c, int type, const double* A, const double* B, doubl
 void hmatvec_grad_factors(Context& c, HMatrix& h, uint32_t k, const double* X, const double* dY, double* const* dU,
                           double* const* dV);
 void hmatrix_set_level(Context& c, HMatrix& h, int level, const double* U, const double* V);
+// the no-allocation H-matvec: per-device scratch set-up (hmx_create) and the product
+void noalloc_init(int device);
+void hmatvec_noalloc(Context& c, const HMatrix& h, uint32_t k, const double* X, double* Y);
 // device exclusive scan int32 -> int64 (out has count+1 entries)
 void exclusive_scan_i32_i64(Context& c, const int32_t* in, int64_t* out, size_t count);
 

@@ -93,6 +93,7 @@ _SIGS = [
     ("hmx_freeHMatrix", _i32, [_vp]),
     ("hmx_HMatrixVecMult", _i32, [_vp, _vp, _u32, _vp, _vp]),
     ("hmx_HMatrixVecMultTranspose", _i32, [_vp, _vp, _u32, _vp, _vp]),
+    ("hmx_HMatrixVecMultNoAlloc", _i32, [_vp, _vp, _u32, _vp, _vp]),
     ("hmx_matvec_profile", _i32, [_vp, _vp, _u32, _vp, _vp, _i32, _vp]),
     ("hmx_denseRowsXVector", _i32, [_vp, _u32, _u32, _vp, _vp, _f64, _u32, _vp, _u32, _vp, _vp]),
     ("hmx_matvec_profile_multi", _i32, [_vp, _vp, _u32, _vp, _vp, _i32, _i32, _vp]),
