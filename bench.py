@@ -24,6 +24,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "kaust-hierarchical-matrices_amd"))
 
 PEAK_HBM_GBS = 8000.0
+PEAK_FP64_MFMA_TFLOPS = 78.6
 
 
 def log(msg):
@@ -43,7 +44,8 @@ def main():
     ap.add_argument("--policy", choices=["ref", "full"], default="full")
     ap.add_argument("--rows", type=int, default=2048, help="sampled rows for the dense check")
     ap.add_argument("--rhs", type=int, default=64, help="right-hand sides of the multi-RHS products")
-    ap.add_argument("--cpu-seconds", type=float, default=12.0, help="budget of the CPU baseline (0 = skip)")
+    ap.add_argument("--cpu-seconds", type=float, default=8.0,
+                    help="budget of the CPU baseline matvec loop (0 = skip the CPU baseline)")
     ap.add_argument("--cpu-stride", type=int, default=32, help="CPU baseline samples every n-th tile")
     ap.add_argument("--mixed-target", type=float, default=1e-5, help="error budget of the mixed factors")
     ap.add_argument("--batch", action="store_true",
@@ -82,6 +84,18 @@ def main():
     H = pipe.H
     nb = H.bytes()
     fac = nb["lowRankBytes"] + nb["denseBytes"]
+    # the build's dominant kernels: the upper-level samplers on the FP64 matrix cores
+    bi = H.build_info()
+    build_roofline = None
+    if bi["samplerMs"] > 0:
+        ach = bi["samplerFlops"] / (bi["samplerMs"] * 1e-3) / 1e12
+        build_roofline = {"bound": "mfma", "kernel": "k_dir_sample_mf + k_dir_sampleT_mf",
+                          "achieved": round(ach, 3), "peak": PEAK_FP64_MFMA_TFLOPS, "unit": "TFLOP/s",
+                          "frac": round(ach / PEAK_FP64_MFMA_TFLOPS, 4), "flops": bi["samplerFlops"],
+                          "kernel_s": round(bi["samplerMs"] * 1e-3, 3),
+                          "kernel_evals_per_s": round(bi["kernelEvals"] / (bi["samplerMs"] * 1e-3), 1)}
+        log(f"samplers {bi['samplerMs'] / 1e3:.2f} s, {ach:.1f} TFLOP/s FP64 MFMA "
+            f"({bi['kernelEvals'] / (bi['samplerMs'] * 1e-3):.3g} kernel evaluations/s)")
 
     # ---- FP64, one right-hand side (the plan is built on the first call, outside the timed region)
     x1 = torch.empty(n, dtype=torch.float64, device="cuda")
@@ -161,8 +175,9 @@ def main():
 
     cpu = None
     if args.cpu_seconds > 0:
-        cpu = cpu_baseline(pipe, ctx, x1, args.cpu_seconds, args.cpu_stride)
-        log(f"CPU baseline {cpu['value']} GB/s on {cpu['cores']} threads")
+        cpu = cpu_baseline(pipe, ctx, x1, args.cpu_seconds, args.cpu_stride, args)
+        log(f"CPU baseline {cpu['value']} GB/s on {cpu['cores']} threads; CPU build ~{cpu['build_s_extrapolated']} s "
+            f"({cpu['build_sample']})")
 
     # the library allocates its matrix-core plans with hipMalloc: hand back
     # what torch's caching allocator still holds from the staging copies
@@ -238,12 +253,15 @@ def main():
                                f"{args.adm} admissibility (eta 1), ARA eps={args.tol} (16-sample blocks), "
                                f"{args.policy} rank policy, FP64 factors, 1 RHS",
                    "n": n, "d": d, "bucket": b, "tol": args.tol, "admissibility": args.adm, "rhs": 1,
-                   "assembly": "direct (no S x S TLR matrix)"},
+                   "assembly": "direct (no S x S TLR matrix)",
+                   "tlr_tile": "none: every level is sampled from the kernel; the reference's TLR tile is the "
+                               "leaf (src/kDTree.cu:14), and its S x S TLR matrix at S = N/64 does not fit"},
         "ms_per_rhs": round(sec1 * 1e3, 4),
         "rel_err_vs_dense": rel_err1,
         "sampled_rows": args.rows,
         "build_s": round(build_s, 3),
         "build_stages_s": {k2: round(v, 3) for k2, v in timings.items()},
+        "build_roofline": build_roofline,
         "factor_bytes": int(fac),
         "levels": ranks_per_level,
         "kernel_ms": {"phase1_VTx": ms[0], "reduce": ms[1], "phase2_Uz_Dx": ms[2], "call": ms[3]},
@@ -333,9 +351,14 @@ def traffic_bytes(args, dom):
     return int(sum(t[k]["traffic_bytes"] for k in keys))
 
 
-def cpu_baseline(pipe, ctx, x1, budget_s, stride):
-    """Oracle OpenMP H-matvec on a sample of the GPU-built factors copied to
-    the host: every stride-th tile of every level plus all dense blocks."""
+def cpu_baseline(pipe, ctx, x1, budget_s, stride, args):
+    """The same H-matvec on the host cores: every block product in OpenBLAS
+    dgemm (numpy's bundled OpenBLAS, one thread per call) with OpenMP over
+    tiles and leaves (oracle/cpu_blas.c), on a sample of the GPU-built factors
+    copied to the host: every stride-th tile of every level plus all dense
+    blocks.  Beside it, the CPU build: the oracle's whole pipeline (TLR path)
+    at N = 2^13 and 2^14 of the same workload, extrapolated to N with the
+    growth exponent the two sizes show."""
     import numpy as np
     import torch
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
@@ -374,14 +397,29 @@ def cpu_baseline(pipe, ctx, x1, budget_s, stride):
     reps, t = 0, 0.0
     while t < budget_s and reps < 200:
         t0 = time.perf_counter()
-        O.matvec_factors(n, b, L, tiles, ranks, offs, Us, Vs, D, X)
+        O.blas_matvec_factors(n, b, L, tiles, ranks, offs, Us, Vs, D, X)
         t += time.perf_counter() - t0
         reps += 1
-    return {"value": round(sbytes * reps / t / 1e9, 3), "unit": "GB/s", "cores": O.num_threads(), "kind": "port",
-            "sample": f"every {stride}-th tile of every level ({fbytes / 1e9:.2f} GB of factors) plus all "
-                      f"{S} dense blocks, 1 RHS, {reps} reps of {t / reps * 1e3:.1f} ms "
-                      "(oracle or_matvec_factors, OpenMP)",
-            "sample_bytes": int(sbytes)}
+    out = {"value": round(sbytes * reps / t / 1e9, 3), "unit": "GB/s", "cores": O.num_threads(), "kind": "port",
+           "sample": f"every {stride}-th tile of every level ({fbytes / 1e9:.2f} GB of factors) plus all "
+                     f"{S} dense blocks, 1 RHS, {reps} reps of {t / reps * 1e3:.1f} ms (OpenBLAS dgemm per block, "
+                     "OpenMP over tiles and leaves)",
+           "sample_bytes": int(sbytes)}
+    # CPU build of the same workload at two small sizes, extrapolated
+    sizes, secs = [1 << 13, 1 << 14], []
+    adm = {"weak": 0, "boxCenter": 1, "maxDiam": 2}[args.adm]
+    for nn in sizes:
+        p = O.generate_points(nn, args.d, 1234)
+        t0 = time.perf_counter()
+        O.OracleHMatrix(p, nn, args.d, b, adm=adm, tol=args.tol,
+                        max_rank_policy=1 if args.policy == "full" else 0, seed=1234)
+        secs.append(time.perf_counter() - t0)
+    expo = float(np.log(secs[1] / secs[0]) / np.log(2.0))
+    out["build_s_extrapolated"] = round(secs[1] * (n / sizes[1]) ** expo, 1)
+    out["build_sample"] = (f"oracle pipeline (KD-tree, structure, TLR ARA, H from TLR) at N={sizes[0]}: "
+                           f"{secs[0]:.2f} s, N={sizes[1]}: {secs[1]:.2f} s on {O.num_threads()} threads; "
+                           f"extrapolated to N={n} as N^{expo:.2f}")
+    return out
 
 
 if __name__ == "__main__":

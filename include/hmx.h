@@ -29,7 +29,7 @@ typedef enum hmx_status {
     HMX_EINVAL = 1,       /* bad argument (sizes, null pointers, unsupported value) */
     HMX_ENOMEM = 2,       /* device or host allocation failed */
     HMX_EDEVICE = 3,      /* HIP runtime error */
-    HMX_ERANKCAP = 4,     /* reserved: strict mode rank-cap failure */
+    HMX_ERANKCAP = 4,     /* strictRankCap: a tile's ARA stopped at its rank cap before converging */
     HMX_EUNSUPPORTED = 5, /* valid in the reference but not provided (see hmx_last_error) */
     HMX_ESTATE = 6        /* object in the wrong state (e.g. TLR not in Morton order) */
 } hmx_status;
@@ -64,6 +64,13 @@ typedef struct hmx_config {
     int32_t araR;                    /* 10 (ARA_R) */
     int32_t maxRankPolicy;           /* HMX_MAXRANK_REF */
     uint64_t seed;                   /* 1234 */
+    /* 0 (default): a tile whose ARA reaches its rank cap before converging
+       keeps the capped factors and is counted (capHits of hmx_tlr_info and
+       hmx_hmatrix_level_info).  1: the build fails with HMX_ERANKCAP and
+       returns no object -- the reference asserts on a failed ARA instead
+       (src/constructTLRMatrix.cu:75, src/constructHMatrixFromStruct.cu:80). */
+    int32_t strictRankCap;
+    int32_t reserved;                /* 0 */
 } hmx_config;
 
 typedef struct hmx_context hmx_context;
@@ -195,6 +202,12 @@ hmx_status hmx_hmatrix_diag_copy(hmx_context* ctx, const hmx_hmatrix* h, double*
 /* bytes of factor storage read by one matvec (low-rank U+V and dense blocks) */
 hmx_status hmx_hmatrix_bytes(const hmx_hmatrix* h, uint64_t* lowRankBytes, uint64_t* denseBytes,
                              uint64_t* factorEntries);
+/* Build statistics of hmx_generateHMatrixDirect (no reference counterpart):
+   algorithmic flops of the upper-level samplers (2 m^2 per sample or basis
+   column of an m x m tile: Y = A Omega and B = A^T Q on the FP64 matrix
+   cores), kernel evaluations they made, and their kernel time in ms (hipEvents
+   on the context stream).  Zeros for the other builds. */
+hmx_status hmx_hmatrix_build_info(const hmx_hmatrix* h, double* samplerFlops, double* kernelEvals, double* samplerMs);
 /* freeHMatrix, src/HMatrix.cu:105-110 */
 hmx_status hmx_freeHMatrix(hmx_hmatrix* h);
 
@@ -347,6 +360,16 @@ hmx_status hmx_batch_matvec(hmx_context* ctx, hmx_batch* batch, uint32_t vectorW
 hmx_status hmx_batch_tune(hmx_context* ctx, hmx_batch* batch, uint32_t vectorWidth, int32_t* bestChunk,
                           float* bestMs);
 hmx_status hmx_batch_free(hmx_batch* batch);
+
+/* ---- test hooks (no reference counterpart) ----------------------------- */
+/* Forced device-allocation failure for the error-path tests: after this call
+   the failAt-th device allocation the library makes (0-based, counted from
+   this call, all contexts and threads) fails as an out-of-memory would, with
+   HMX_ENOMEM from the entry point that made it; failAt < 0 fails none and
+   only counts.  hmx_debug_alloc_count returns the allocations made since the
+   last hmx_debug_set_alloc_fault call. */
+void hmx_debug_set_alloc_fault(int64_t failAt);
+int64_t hmx_debug_alloc_count(void);
 
 #ifdef __cplusplus
 }

@@ -8,10 +8,17 @@
 
 using namespace hmx;
 
+namespace hmx {
+std::atomic<int64_t> g_allocCount{0}, g_allocFailAt{-1};
+}
+
 namespace {
 
 thread_local std::string g_noctx_error;
 
+// A failed HIP call leaves its error as the thread's last HIP error; it is
+// taken off here, so the launch checks of later calls (hipGetLastError) do
+// not report it again.
 template <class F>
 hmx_status guard(hmx_context* ctx, F&& f) {
     try {
@@ -19,13 +26,16 @@ hmx_status guard(hmx_context* ctx, F&& f) {
         if (ctx) ctx->last_error.clear();
         return HMX_OK;
     } catch (const Error& e) {
+        (void)hipGetLastError();
         if (ctx) ctx->last_error = e.what();
         else g_noctx_error = e.what();
         return e.status;
     } catch (const std::bad_alloc&) {
+        (void)hipGetLastError();
         if (ctx) ctx->last_error = "host allocation failed";
         return HMX_ENOMEM;
     } catch (const std::exception& e) {
+        (void)hipGetLastError();
         if (ctx) ctx->last_error = e.what();
         return HMX_EDEVICE;
     }
@@ -45,6 +55,19 @@ void check_config(const hmx_config* cfg) {
     HMX_CHECK(cfg->maxRankPolicy == HMX_MAXRANK_REF || cfg->maxRankPolicy == HMX_MAXRANK_FULL, HMX_EINVAL,
               "maxRankPolicy must be 0 (reference) or 1 (full)");
     HMX_CHECK(cfg->araR >= 1, HMX_EINVAL, "araR must be >= 1");
+    HMX_CHECK(cfg->strictRankCap == 0 || cfg->strictRankCap == 1, HMX_EINVAL, "strictRankCap must be 0 or 1");
+}
+
+// strictRankCap: a build whose ARA stopped at a rank cap fails
+void check_cap_hits(const hmx_config& cfg, uint64_t hits, const char* what) {
+    HMX_CHECK(!cfg.strictRankCap || hits == 0, HMX_ERANKCAP,
+              std::to_string(hits) + " " + what + " reached the rank cap before the ARA converged (strictRankCap)");
+}
+
+uint64_t level_cap_hits(const HMatrix& h) {
+    uint64_t s = 0;
+    for (const HLevel& lv : h.levels) s += (uint64_t)lv.capHits;
+    return s;
 }
 
 }  // namespace
@@ -96,6 +119,13 @@ void hmx_config_default(hmx_config* cfg) {
 }
 
 const char* hmx_version(void) { return "hmx 0.1 gfx950"; }
+
+void hmx_debug_set_alloc_fault(int64_t failAt) {
+    g_allocFailAt.store(failAt < 0 ? -1 : failAt);
+    g_allocCount.store(0);
+}
+
+int64_t hmx_debug_alloc_count(void) { return g_allocCount.load(); }
 
 hmx_status hmx_generateDataset(hmx_context* ctx, uint32_t n, uint32_t d, uint64_t seed, double* d_points) {
     if (!ctx) return HMX_EINVAL;
@@ -263,6 +293,7 @@ hmx_status hmx_createColumnMajorLRMatrix(hmx_context* ctx, const hmx_config* cfg
         hmx_tlr* m = new hmx_tlr();
         try {
             build_tlr_cm(*ctx, *cfg, *tree, d_points, *m);
+            check_cap_hits(*cfg, m->capHits, "TLR tiles");
         } catch (...) {
             delete m;
             throw;
@@ -343,6 +374,7 @@ hmx_status hmx_generateHMatrixFromStruct(hmx_context* ctx, const hmx_config* cfg
         hmx_hmatrix* h = new hmx_hmatrix();
         try {
             build_hmatrix(*ctx, *cfg, *tree, *structure, *mortonMatrix, *h);
+            check_cap_hits(*cfg, level_cap_hits(*h), "H-matrix tiles");
         } catch (...) {
             delete h;
             throw;
@@ -367,12 +399,21 @@ hmx_status hmx_generateHMatrixDirect(hmx_context* ctx, const hmx_config* cfg, co
         hmx_hmatrix* h = new hmx_hmatrix();
         try {
             build_hmatrix_direct(*ctx, *cfg, *tree, *structure, d_points, *h);
+            check_cap_hits(*cfg, level_cap_hits(*h), "H-matrix tiles");
         } catch (...) {
             delete h;
             throw;
         }
         *hmatrix = h;
     });
+}
+
+hmx_status hmx_hmatrix_build_info(const hmx_hmatrix* h, double* samplerFlops, double* kernelEvals, double* samplerMs) {
+    if (!h) return HMX_EINVAL;
+    if (samplerFlops) *samplerFlops = h->samplerFlops;
+    if (kernelEvals) *kernelEvals = h->kernelEvals;
+    if (samplerMs) *samplerMs = h->samplerMs;
+    return HMX_OK;
 }
 
 hmx_status hmx_hmatrix_level_info(const hmx_hmatrix* h, int32_t level, int32_t* numTiles, int64_t* rankSum,
@@ -660,6 +701,7 @@ hmx_status hmx_generateHMatrixDirectBatch(hmx_context* ctx, int32_t count, const
             hs.push_back(own.back().get());
         }
         build_hmatrix_direct_batch(*ctx, cf, tr, st, pt, hs);
+        for (int32_t i = 0; i < count; ++i) check_cap_hits(cfgs[i], level_cap_hits(*hs[i]), "H-matrix tiles");
         for (int32_t i = 0; i < count; ++i) hmatrices[i] = own[i].release();
     });
 }

@@ -694,6 +694,24 @@ struct AraScratch {
     DBuf<TileParam> tp;
     DBuf<int64_t> first, off;
     DBuf<int> flags;
+    // direct-mode samplers (Y = A Omega, B = A^T Q from on-the-fly kernel
+    // evaluations): algorithmic flops (2 m^2 per sample column and tile),
+    // kernel evaluations, and their kernel time on the stream
+    double samplerFlops = 0, kernelEvals = 0, samplerMs = 0;
+    hipEvent_t ev[2] = {nullptr, nullptr};
+    ~AraScratch() {
+        for (hipEvent_t e : ev)
+            if (e) (void)hipEventDestroy(e);
+    }
+    void mark(Context& c, int i) {
+        if (!ev[i]) HMX_HIP(hipEventCreate(&ev[i]));
+        HMX_HIP(hipEventRecord(ev[i], c.stream));
+    }
+    void add_time() {  // after a synchronisation: ev[0] .. ev[1]
+        float t = 0;
+        HMX_HIP(hipEventElapsedTime(&t, ev[0], ev[1]));
+        samplerMs += t;
+    }
 };
 
 // Buffer sizes of one batched ARA launch sequence.  Direct mode: one
@@ -819,8 +837,10 @@ static void compress_batch(Context& c, int araR, uint64_t seed, const LevelSourc
     const unsigned rowBlocks = (unsigned)((m + 255) / 256);
     const unsigned wgThreads = m >= 4096 ? 1024u : 256u;
     int kmax = 0, preStart = 0, preN = 0, sweeps = 0;
+    int active = T;  // tiles still iterating (the count k_lr_commit reports)
     const int maxIters = (2 * maxRank + araR) / kB16 + 2;
     for (int it = 0;; ++it) {
+        bool timedSweep = false;
         HMX_CHECK(it <= maxIters, HMX_EDEVICE, "batched ARA did not converge (non-finite input?)");
         if (kmax + kB16 > capQ && capQ < maxRank) {  // grow the basis storage
             int nc = std::min(maxRank, std::max(2 * capQ, kmax + kB16));
@@ -839,9 +859,15 @@ static void compress_batch(Context& c, int araR, uint64_t seed, const LevelSourc
                 const int nb = it == 0 ? nbFirst : nbLater;
                 k_lr_omega<<<dim3((m * kB16 * nb + 255) / 256, T), 256, 0, c.stream>>>(L, it, nb, 1);
                 HMX_LAUNCHED();
+                ws.mark(c, 0);
                 if (src.d == 1) launch_sample_direct<1>(c, L, nb, splits, ws.Ypre.p, ws.Ypart.p);
                 else if (src.d == 2) launch_sample_direct<2>(c, L, nb, splits, ws.Ypre.p, ws.Ypart.p);
                 else launch_sample_direct<3>(c, L, nb, splits, ws.Ypre.p, ws.Ypart.p);
+                ws.mark(c, 1);
+                timedSweep = true;
+                const double mm = (double)m * m;
+                ws.samplerFlops += (double)active * 2.0 * mm * kB16 * nb;
+                ws.kernelEvals += (double)active * mm;
                 preStart = it;
                 preN = nb;
                 ++sweeps;
@@ -876,7 +902,9 @@ static void compress_batch(Context& c, int araR, uint64_t seed, const LevelSourc
         int hf[2];
         HMX_HIP(hipMemcpyAsync(hf, ws.flags.p, sizeof(hf), hipMemcpyDeviceToHost, c.stream));
         HMX_HIP(hipStreamSynchronize(c.stream));
+        if (timedSweep) ws.add_time();
         kmax = hf[1];
+        active = hf[0];
         out.iters = it + 1;
         if (hf[0] == 0) break;
     }
@@ -907,9 +935,16 @@ static void compress_batch(Context& c, int araR, uint64_t seed, const LevelSourc
                                c.stream));
         L.boff = ws.off.p;
         if (direct) {
+            ws.mark(c, 0);
             if (src.d == 1) launch_sampleT_direct<1>(c, L, kmax, out.V.p);
             else if (src.d == 2) launch_sampleT_direct<2>(c, L, kmax, out.V.p);
             else launch_sampleT_direct<3>(c, L, kmax, out.V.p);
+            ws.mark(c, 1);
+            const double mm = (double)m * m;
+            for (int t = 0; t < T; ++t) {
+                ws.samplerFlops += 2.0 * mm * out.ranks[t];
+                ws.kernelEvals += mm * (double)((out.ranks[t] + 63) / 64);
+            }
         } else if (b <= 32) {
             launch_sampleT_tlr<32>(c, L, kmax, bus, T, out.V.p);
         } else {
@@ -919,6 +954,7 @@ static void compress_batch(Context& c, int araR, uint64_t seed, const LevelSourc
         k_lr_pack<<<g, 256, 0, c.stream>>>(T, m, capQ, ws.st.p, ws.off.p, ws.Q.p, out.U.p);
         HMX_LAUNCHED();
         HMX_HIP(hipStreamSynchronize(c.stream));
+        if (direct) ws.add_time();
     }
     if (log_on()) {
         auto ms = [](std::chrono::steady_clock::time_point a, std::chrono::steady_clock::time_point b) {
@@ -1027,6 +1063,9 @@ static void compress_upper_levels(Context& c, const hmx_config& cfg, const Struc
                        first_sweep_blocks(*below, cfg.araR), ws, lv);
         below = &lv.h_ranks;
     }
+    h.samplerFlops = ws.samplerFlops;
+    h.kernelEvals = ws.kernelEvals;
+    h.samplerMs = ws.samplerMs;
 }
 
 static void finish_accounting(HMatrix& h) {

@@ -58,8 +58,40 @@ __device__ inline double kernel_eval(const double* __restrict__ a, const double*
     return exp(-sqrt(diff) / ell);
 }
 
-// Same kernel with the division folded into a multiply by nil = -1/ell: for
-// sampling loops whose values are not compared bit for bit.
+// exp(x) for x <= 0 (the kernel's exponent -r/l), without the special-case
+// handling a general exp needs.  Cody-Waite argument reduction: x = k ln2 + r
+// with k = rint(x log2 e), |r| <= ln2/2, ln2 split into a head of 32
+// significant bits (k * head is exact for |k| < 2^20) and a tail, so r carries
+// no cancellation error; exp(r) by its Taylor series to degree 12 (remainder
+// below 2e-16 relative on |r| <= ln2/2), in Horner form; 2^k by ldexp, which
+// also flushes results below the double range to zero.  The split and the
+// coefficients 1/n! follow from the definitions of ln2 and exp.
+__device__ inline double exp_nonpos(double x) {
+    constexpr double kLog2e = 1.4426950408889634;
+    constexpr double kLn2Head = 0x1.62e42fee00000p-1;
+    constexpr double kLn2Tail = 0x1.a39ef35793c76p-33;
+    const double k = __builtin_rint(x * kLog2e);
+    double r = fma(-k, kLn2Head, x);
+    r = fma(-k, kLn2Tail, r);
+    double p = 2.08767569878681e-09;  // 1/12!
+    p = fma(p, r, 2.505210838544172e-08);
+    p = fma(p, r, 2.755731922398589e-07);
+    p = fma(p, r, 2.7557319223985893e-06);
+    p = fma(p, r, 2.48015873015873e-05);
+    p = fma(p, r, 0.0001984126984126984);
+    p = fma(p, r, 0.001388888888888889);
+    p = fma(p, r, 0.008333333333333333);
+    p = fma(p, r, 0.041666666666666664);
+    p = fma(p, r, 0.16666666666666666);
+    p = fma(p, r, 0.5);
+    p = fma(p, r, 1.0);
+    p = fma(p, r, 1.0);
+    return ldexp(p, (int)k);
+}
+
+// Same kernel with the division folded into a multiply by nil = -1/ell and
+// exp_nonpos for the exponential: for sampling loops whose values are not
+// compared bit for bit.
 template <int D>
 __device__ inline double kernel_eval_scaled(const double* __restrict__ a, const double* __restrict__ b, double nil) {
     double diff = 0.0;
@@ -68,7 +100,7 @@ __device__ inline double kernel_eval_scaled(const double* __restrict__ a, const 
         const double t = a[j] - b[j];
         diff = fma(t, t, diff);
     }
-    return exp(nil * sqrt(diff));
+    return exp_nonpos(nil * sqrt(diff));
 }
 
 __device__ inline double kernel_eval_d(const double* a, const double* b, int d, double ell) {

@@ -3,6 +3,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <atomic>
 #include <cstdint>
 #include <stdexcept>
 #include <string>
@@ -35,6 +36,15 @@ struct Error : std::runtime_error {
 // Launch check: catches launch-configuration errors right after a launch.
 #define HMX_LAUNCHED() HMX_HIP(hipGetLastError())
 
+// Forced allocation failure for the error-path tests (hmx_debug_set_alloc_fault):
+// allocation number g_allocFailAt (counted by g_allocCount) fails.
+extern std::atomic<int64_t> g_allocCount, g_allocFailAt;
+inline void alloc_fault_point() {
+    const int64_t k = g_allocCount.fetch_add(1);
+    if (k == g_allocFailAt.load())
+        throw Error(HMX_ENOMEM, "forced device allocation failure (allocation " + std::to_string(k) + ")");
+}
+
 // Owning device buffer (hipMalloc / hipFree), movable, not copyable.
 template <class T>
 struct DBuf {
@@ -52,8 +62,11 @@ struct DBuf {
     ~DBuf() { release(); }
     void alloc(size_t count) {
         release();
+        if (count) {
+            alloc_fault_point();
+            HMX_HIP(hipMalloc((void**)&p, count * sizeof(T)));
+        }
         n = count;
-        if (count) HMX_HIP(hipMalloc((void**)&p, count * sizeof(T)));
     }
     void release() {
         if (p) (void)hipFree(p);
@@ -285,6 +298,9 @@ struct HMatrix {
     int planLocality = 1;    // phase-1 items ordered by right-hand-side block
     int planFp64Layout = 0;  // FP64 plan factors: 0 copy when it fits in half the free memory, 1 in place, 2 copy
     uint64_t lowRankBytes = 0, denseBytes = 0, factorEntries = 0;
+    // direct assembly: the upper-level samplers' algorithmic flops, kernel
+    // evaluations and kernel time (hmx_hmatrix_build_info)
+    double samplerFlops = 0, kernelEvals = 0, samplerMs = 0;
 };
 
 // Many H-matrices applied as one block-diagonal operator: one concatenated

@@ -49,6 +49,8 @@ class Config(ctypes.Structure):
         ("araR", ctypes.c_int32),
         ("maxRankPolicy", ctypes.c_int32),
         ("seed", ctypes.c_uint64),
+        ("strictRankCap", ctypes.c_int32),
+        ("reserved", ctypes.c_int32),
     ]
 
 
@@ -116,6 +118,9 @@ _SIGS = [
     ("hmx_batch_matvec", _i32, [_vp, _vp, _u32, _vp, _vp, _i32]),
     ("hmx_batch_tune", _i32, [_vp, _vp, _u32, _P(_i32), _P(_f32)]),
     ("hmx_batch_free", _i32, [_vp]),
+    ("hmx_hmatrix_build_info", _i32, [_vp, _P(_f64), _P(_f64), _P(_f64)]),
+    ("hmx_debug_set_alloc_fault", None, [_i64]),
+    ("hmx_debug_alloc_count", _i64, []),
 ]
 
 
@@ -292,6 +297,12 @@ class HMatrix(_Obj):
     def __init__(self, handle, num_levels, n, b):
         super().__init__(handle)
         self.num_levels, self.n, self.b = num_levels, n, b
+
+    def build_info(self):
+        """Direct assembly: sampler flops, kernel evaluations, sampler kernel ms."""
+        f, e, ms = _f64(), _f64(), _f64()
+        _check(lib().hmx_hmatrix_build_info(self.handle, ctypes.byref(f), ctypes.byref(e), ctypes.byref(ms)))
+        return dict(samplerFlops=f.value, kernelEvals=e.value, samplerMs=ms.value)
 
     def level_info(self, level):
         nt, rs, mr, tol, ch = _i32(), _i64(), _i32(), _f32(), _i32()

@@ -101,6 +101,10 @@ def lib():
         L.or_dense_matrix.argtypes = [c_i, c_i, vp, vp, c_d, vp]
         L.or_matvec_factors.argtypes = [c_i, c_i, c_i, vp, vp, vp, vp, vp, vp, vp, c_i, vp, vp]
         L.or_num_threads.restype = c_i
+        L.cb_open.argtypes = [ctypes.c_char_p]
+        L.cb_open.restype = c_i
+        L.cb_matvec_factors.argtypes = [c_i, c_i, c_i, vp, vp, vp, vp, vp, vp, vp, c_i, vp, vp]
+        L.cb_matvec_factors.restype = c_i
         L.or_default_ell.argtypes = [c_i]
         L.or_default_ell.restype = c_d
         _LIB = L
@@ -378,3 +382,39 @@ def matvec_factors(n, b, num_levels, tiles, ranks, offs, U, V, diag, X):
 
 def num_threads():
     return lib().or_num_threads()
+
+
+def openblas_path():
+    """numpy's bundled ILP64 OpenBLAS (None when absent)."""
+    import glob
+    base = os.path.join(os.path.dirname(os.path.dirname(np.__file__)), "numpy.libs")
+    hits = sorted(glob.glob(os.path.join(base, "libscipy_openblas64_*.so")))
+    return hits[0] if hits else None
+
+
+def blas_matvec_factors(n, b, num_levels, tiles, ranks, offs, U, V, diag, X):
+    """matvec_factors with every block product in OpenBLAS dgemm (OpenMP over
+    tiles and leaves): the CPU baseline of the benchmark."""
+    path = openblas_path()
+    if path is None or lib().cb_open(path.encode()) != 0:
+        raise RuntimeError("OpenBLAS not found")
+    nl = num_levels - 1
+    keep = []
+
+    def arr(lst, dtype):
+        a = [np.ascontiguousarray(x, dtype=dtype) for x in lst]
+        keep.extend(a)
+        return (ctypes.c_void_p * nl)(*[x.ctypes.data for x in a])
+
+    X = np.asfortranarray(X, dtype=np.float64)
+    if X.ndim == 1:
+        X = X[:, None]
+    Y = np.zeros_like(X, order="F")
+    nt = np.array([len(t) for t in tiles], dtype=np.int32)
+    d = np.ascontiguousarray(diag, dtype=np.float64)
+    st = lib().cb_matvec_factors(n, b, num_levels, _p(nt), arr(tiles, np.int32), arr(ranks, np.int32),
+                                 arr(offs, np.int64), arr(U, np.float64), arr(V, np.float64), _p(d), X.shape[1],
+                                 _p(X), _p(Y))
+    if st != 0:
+        raise RuntimeError("OpenBLAS not loaded")
+    return Y

@@ -34,6 +34,17 @@ def test_config_defaults_match_reference_parseargs():
     assert (cfg.numberOfInputPoints, cfg.dimensionOfInputPoints, cfg.bucketSize, cfg.vectorWidth) == (1024, 2, 32, 16)
     assert cfg.lowestLevelTolerance == np.float32(1e-5)
     assert (cfg.araBlockSize, cfg.araR, cfg.eta, cfg.seed) == (16, 10, 1.0, 1234)
+    # extension: a rank-cap hit is counted, not an error, unless asked for
+    assert (cfg.strictRankCap, cfg.reserved) == (0, 0)
+
+
+def test_alloc_fault_hook_counts_without_a_device():
+    L = hmx.lib()
+    L.hmx_debug_set_alloc_fault(-1)
+    assert L.hmx_debug_alloc_count() == 0
+    L.hmx_debug_set_alloc_fault(5)
+    assert L.hmx_debug_alloc_count() == 0
+    L.hmx_debug_set_alloc_fault(-1)
 
 
 def test_generate_max_ranks():

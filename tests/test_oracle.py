@@ -406,3 +406,10 @@ def test_matvec_on_supplied_factors_matches(small_h):
     Y = O.matvec_factors(H.n, H.b, L, tiles, ranks, offs, Us, Vs, H.diag(), X)
     Yr = H.matvec(X)
     assert np.linalg.norm(Y - Yr) / np.linalg.norm(Yr) < 1e-14
+    # the OpenBLAS CPU baseline computes the same product (other summation order)
+    if O.openblas_path() is not None:
+        for k in (1, 5):
+            Xk = O.generate_rhs(H.n, k, 9 + k)
+            Yb = O.blas_matvec_factors(H.n, H.b, L, tiles, ranks, offs, Us, Vs, H.diag(), Xk)
+            Yk = H.matvec(Xk)
+            assert np.linalg.norm(Yb - Yk) / np.linalg.norm(Yk) < 1e-13
