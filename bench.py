@@ -9,7 +9,8 @@ dense diagonal blocks and leaf-level tiles by the TLR tile kernel, upper levels
 by the same batched ARA sampling the kernel on the fly.
 
 A step is one FP64 H-matvec with one right-hand side, inputs resident in HBM.
-value = algorithmic bytes (factors + dense blocks + x + y) / time, in GB/s.
+value = algorithmic bytes of that product (the factors and dense blocks its
+plan reads once each, + x + y) / time, in GB/s.
 Alongside: build seconds, accuracy on 2048 sampled rows of the dense kernel,
 and the 64-RHS product in FP64 (matrix cores) against the mixed FP32/FP16
 product (matrix cores, per-column scales), in ms per right-hand side.
@@ -48,6 +49,9 @@ def main():
                     help="budget of the CPU baseline matvec loop (0 = skip the CPU baseline)")
     ap.add_argument("--cpu-stride", type=int, default=32, help="CPU baseline samples every n-th tile")
     ap.add_argument("--mixed-target", type=float, default=1e-5, help="error budget of the mixed factors")
+    ap.add_argument("--assembly", choices=["direct", "tlr"], default="direct",
+                    help="direct: every level sampled from the kernel; tlr: the reference's path (S x S TLR "
+                         "matrix of leaf tiles, Morton order, H-matrix from the structure)")
     ap.add_argument("--batch", action="store_true",
                     help="many point clouds instead: batched build, autotuned hipGraph matvec, per-cloud dense check")
     ap.add_argument("--batch-count", type=int, default=256)
@@ -77,7 +81,7 @@ def main():
     t0 = time.perf_counter()
     pipe = hmx.Pipeline(n, d, b, args.tol, adm=args.adm,
                         max_rank_policy=hmx.MAXRANK_FULL if args.policy == "full" else hmx.MAXRANK_REF,
-                        seed=1234, ctx=ctx, timings=timings, direct=True)
+                        seed=1234, ctx=ctx, timings=timings, direct=args.assembly == "direct")
     torch.cuda.synchronize()
     build_s = time.perf_counter() - t0
     log(f"build {build_s:.2f}s stages {timings}")
@@ -139,13 +143,20 @@ def main():
         log("counters run done")
         return
     sec1 = timed(lambda: pipe.matvec(x1, y1), args.steps, args.warmup)
-    bytes1 = fac + 16 * n
+    # the streaming plan's bytes: U and V of the tiles applied as factors,
+    # the diagonal blocks and the leaf tiles applied as dense blocks
+    sp = H.plan_info(2)
+    S = pipe.tree.info()[0]
+    dense1 = 8 * (S + sp["denseBlocks"]) * b * b
+    lr1 = sp["bytes"] - dense1
+    bytes1 = sp["bytes"] + 16 * n
     gbs1 = float(bytes1 / sec1 / 1e9)
-    log(f"FP64 1-RHS {sec1 * 1e3:.3f} ms, {gbs1:.0f} GB/s")
+    log(f"FP64 1-RHS {sec1 * 1e3:.3f} ms, {gbs1:.0f} GB/s over the {bytes1 / 1e9:.2f} GB its plan reads "
+        f"({sp['denseBlocks']} leaf tiles as dense blocks; the factors hold {fac / 1e9:.2f} GB)")
     # per-kernel times: hipEvents on the library's stream around each launch
     ms = [float(v) for v in hmx.matvec_profile(ctx, H, 1, x1, y1, iters=max(10, args.steps // 2))]
-    ph1_bytes = nb["lowRankBytes"] / 2 + 8 * n           # V, x
-    ph2_bytes = nb["lowRankBytes"] / 2 + nb["denseBytes"] + 16 * n  # U, D, x, y
+    ph1_bytes = lr1 / 2 + 8 * n           # V, x
+    ph2_bytes = lr1 / 2 + dense1 + 16 * n  # U, dense blocks, x, y
     dom = 1 if ms[0] >= ms[2] else 2
     dom_ms = ms[0] if dom == 1 else ms[2]
     dom_bytes = ph1_bytes if dom == 1 else ph2_bytes
@@ -216,6 +227,31 @@ def main():
         rel_m_vs_dense = float(torch.linalg.norm(Y32.double()[:, rl] - ydM) / torch.linalg.norm(ydM))
         bytesM = int(minfo["bytes"]) + 8 * n * k
         log(f"mixed {k}-RHS {secM * 1e3:.2f} ms, rel vs FP64 {rel_m_vs_fp64:.2e}, vs dense {rel_m_vs_dense:.2e}")
+        # per-phase rooflines.  FP64: matrix-core flops 2 k per factor entry
+        # (phase 1 V^T X, phase 2 U Z; the dense blocks 2 k b^2 each) against
+        # the FP64 matrix-core peak.  Mixed: algorithmic bytes of the phase
+        # (its factors; X read once as FP16; z written once (phase 1) or read
+        # once (phase 2) as FP16/FP32; Y written once as FP32) against HBM.
+        f1, f2 = H.plan_phase_bytes(0)
+        fl1, fl2 = 2.0 * k * f1 / 8, 2.0 * k * f2 / 8
+        mfma_fp64 = {"bound": "mfma", "unit": "TFLOP/s", "peak": PEAK_FP64_MFMA_TFLOPS,
+                     "phase1": round(fl1 / (msK[0] * 1e-3) / 1e12, 3),
+                     "phase2": round(fl2 / (msK[2] * 1e-3) / 1e12, 3),
+                     "call": round((fl1 + fl2) / secK / 1e12, 3)}
+        mfma_fp64["frac"] = round(mfma_fp64["call"] / PEAK_FP64_MFMA_TFLOPS, 4)
+        m1, m2 = H.plan_phase_bytes(1)
+        zbytes = k * (2 * int(minfo["loCols"]) + 4 * int(minfo["hiCols"]))
+        mb1 = m1 + 2 * n * k + zbytes
+        mb2 = m2 + 2 * n * k + zbytes + 4 * n * k
+        mdom = 1 if msM[0] >= msM[2] else 2
+        mach = (mb1 if mdom == 1 else mb2) / ((msM[0] if mdom == 1 else msM[2]) * 1e-3) / 1e9
+        mixed_roofline = {"bound": "hbm", "unit": "GB/s", "peak": PEAK_HBM_GBS,
+                          "kernel": "k_mf_phase1 (f16, f32)" if mdom == 1 else "k_mf_phase2g (mixed)",
+                          "achieved": round(mach, 1), "frac": round(mach / PEAK_HBM_GBS, 4),
+                          "phase1_bytes": int(mb1), "phase2_bytes": int(mb2),
+                          "phase1_GBps": round(mb1 / (msM[0] * 1e-3) / 1e9, 1),
+                          "phase2_GBps": round(mb2 / (msM[2] * 1e-3) / 1e9, 1),
+                          "call_GBps": round((mb1 + mb2) / secM / 1e9, 1)}
         rhs_block = {
                 "fp64_ms_per_rhs": round(secK * 1e3 / k, 5), "fp64_call_ms": round(secK * 1e3, 3),
                 "fp64_kernel_ms": msK, "fp64_rel_err_vs_dense": rel_errK,
@@ -228,7 +264,8 @@ def main():
                 "speedup_mixed_per_rhs_vs_fp64_1rhs": round(sec1 / (secM / k), 2),
                 "mixed_target": args.mixed_target, "mixed_bytes": int(minfo["bytes"]),
                 "mixed_fp32_cols": int(minfo["hiCols"]), "mixed_fp16_cols": int(minfo["loCols"]),
-                "mixed_build_s": round(mixed_build_s, 3)}
+                "mixed_build_s": round(mixed_build_s, 3),
+                "fp64_roofline": mfma_fp64, "mixed_roofline": mixed_roofline}
 
     ranks_per_level = {}
     for l in range(1, H.num_levels):
@@ -253,9 +290,11 @@ def main():
                                f"{args.adm} admissibility (eta 1), ARA eps={args.tol} (16-sample blocks), "
                                f"{args.policy} rank policy, FP64 factors, 1 RHS",
                    "n": n, "d": d, "bucket": b, "tol": args.tol, "admissibility": args.adm, "rhs": 1,
-                   "assembly": "direct (no S x S TLR matrix)",
-                   "tlr_tile": "none: every level is sampled from the kernel; the reference's TLR tile is the "
-                               "leaf (src/kDTree.cu:14), and its S x S TLR matrix at S = N/64 does not fit"},
+                   "assembly": "direct (no S x S TLR matrix)" if args.assembly == "direct" else
+                               "TLR (S x S leaf tiles) then H-matrix from the structure",
+                   "tlr_tile": ("none: every level is sampled from the kernel; the reference's TLR tile is the "
+                                "leaf (src/kDTree.cu:14), and its S x S TLR matrix at S = N/64 does not fit")
+                               if args.assembly == "direct" else f"the leaf ({b} points)"},
         "ms_per_rhs": round(sec1 * 1e3, 4),
         "rel_err_vs_dense": rel_err1,
         "sampled_rows": args.rows,
@@ -263,6 +302,8 @@ def main():
         "build_stages_s": {k2: round(v, 3) for k2, v in timings.items()},
         "build_roofline": build_roofline,
         "factor_bytes": int(fac),
+        "plan_bytes": int(sp["bytes"]),
+        "dense_near_blocks": int(sp["denseBlocks"]),
         "levels": ranks_per_level,
         "kernel_ms": {"phase1_VTx": ms[0], "reduce": ms[1], "phase2_Uz_Dx": ms[2], "call": ms[3]},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": PEAK_HBM_GBS, "unit": "GB/s",
@@ -345,10 +386,11 @@ def traffic_bytes(args, dom):
     with open(path) as f:
         t = json.load(f)
     # phase 2 is the per-leaf pass plus the tall-tile (128-row block) pass
-    keys = ["k_mv_phase1<1>"] if dom == 1 else ["k_mv_phase2<64, 1>", "k_mv_phase2b<128, 1>"]
-    if not all(k in t for k in keys):
+    prefixes = ["k_mv_phase1<1>"] if dom == 1 else ["k_mv_phase2<64, 1>", "k_mv_phase2b<128, 1"]
+    found = [next((v for k, v in t.items() if k.startswith(p) and isinstance(v, dict)), None) for p in prefixes]
+    if any(v is None for v in found):
         return None
-    return int(sum(t[k]["traffic_bytes"] for k in keys))
+    return int(sum(v["traffic_bytes"] for v in found))
 
 
 def cpu_baseline(pipe, ctx, x1, budget_s, stride, args):

@@ -66,9 +66,11 @@ typedef struct hmx_config {
     uint64_t seed;                   /* 1234 */
     /* 0 (default): a tile whose ARA reaches its rank cap before converging
        keeps the capped factors and is counted (capHits of hmx_tlr_info and
-       hmx_hmatrix_level_info).  1: the build fails with HMX_ERANKCAP and
-       returns no object -- the reference asserts on a failed ARA instead
-       (src/constructTLRMatrix.cu:75, src/constructHMatrixFromStruct.cu:80). */
+       hmx_hmatrix_level_info).  1: with the reference's rank caps
+       (HMX_MAXRANK_REF) such a build fails with HMX_ERANKCAP and returns no
+       object -- the reference asserts on a failed ARA instead
+       (src/constructTLRMatrix.cu:75, src/constructHMatrixFromStruct.cu:80);
+       uncapped ranks end at the tile dimension, where the factors are exact. */
     int32_t strictRankCap;
     int32_t reserved;                /* 0 */
 } hmx_config;
@@ -96,6 +98,12 @@ const char* hmx_version(void);
 /* generateDataset, src/generateDataset.cu:15-19.  d_points: n*d doubles, SoA
    (p[j*n+i]); uniform [0,1) from counter-based Philox4x32-10 keyed by seed. */
 hmx_status hmx_generateDataset(hmx_context* ctx, uint32_t n, uint32_t d, uint64_t seed, double* d_points);
+/* Many clouds at once: cloud k gets n[k]*d[k] values from seed[k], exactly
+   what hmx_generateDataset gives it alone, all in one launch.  No reference
+   counterpart: the reference generates one cloud per run
+   (src/main.cu:51-56). */
+hmx_status hmx_generateDatasetBatch(hmx_context* ctx, int32_t count, const uint32_t* n, const uint32_t* d,
+                                    const uint64_t* seed, double* const* d_points);
 /* generateRandomVector, src/helperFunctions.cu:59-65: n x vectorWidth
    column-major N(0,1) doubles (Philox + Box-Muller) */
 hmx_status hmx_generateRandomVector(hmx_context* ctx, uint32_t vectorWidth, uint32_t n, uint64_t seed,
@@ -106,6 +114,16 @@ hmx_status hmx_generateRandomVector(hmx_context* ctx, uint32_t vectorWidth, uint
    src/kDTreeConstruction.cu:14-231 */
 hmx_status hmx_constructKDTree(hmx_context* ctx, uint32_t n, uint32_t d, uint32_t bucketSize, int32_t divMethod,
                                const double* d_points, hmx_kdtree** tree);
+/* KD-trees of many clouds (one bucketSize and divMethod): the clouds of each
+   point dimension are built as one forest, every level of every tree by the
+   same launches; trees[k] equals hmx_constructKDTree on cloud k bit for bit
+   and is freed by hmx_freeKDTree (its device arrays share blocks with the
+   other trees of the call; a block is released with its last tree).  No
+   reference counterpart: constructKDTree (src/kDTreeConstruction.cu:14-231)
+   builds one tree per run. */
+hmx_status hmx_constructKDTreeBatch(hmx_context* ctx, int32_t count, const uint32_t* n, const uint32_t* d,
+                                    uint32_t bucketSize, int32_t divMethod, const double* const* d_points,
+                                    hmx_kdtree** trees);
 /* numSegments / segmentSize (src/kDTree.cuh:8-14) and the number of split levels */
 hmx_status hmx_kdtree_info(const hmx_kdtree* tree, uint32_t* numSegments, uint32_t* segmentSize,
                            uint32_t* numLevels);
@@ -264,11 +282,13 @@ hmx_status hmx_hmatrix_make_mixed(hmx_context* ctx, hmx_hmatrix* h, double targe
    of the FP16 columns, bytes read per call */
 hmx_status hmx_hmatrix_mixed_info(const hmx_hmatrix* h, int64_t* hiCols, int64_t* loCols, double* frobH2,
                                   double* loEnergy, uint64_t* bytes);
-/* Matrix-core plan choices (no counterpart in the reference); each 0 or 1,
-   all 1 by default (the fastest combination measured at N = 2^20, 3D).  denseNear: leaf-level tiles of rank above bucketSize/2
-   applied as dense blocks.  directZ: phase 1 stores whole-column z itself
-   instead of partial sums for the reduce.  locality: phase-1 work ordered by
-   right-hand-side block.  Drops the built matrix-core plans. */
+/* Matvec plan choices (no counterpart in the reference); each 0 or 1, all 1
+   by default (the fastest combination measured at N = 2^20, 3D).  denseNear:
+   leaf-level tiles of rank above bucketSize/2 applied as dense blocks (the
+   streaming and the matrix-core plans).  directZ: matrix-core phase 1 stores
+   whole-column z itself instead of partial sums for the reduce.  locality:
+   matrix-core phase-1 work ordered by right-hand-side block.  Drops the built
+   plans. */
 hmx_status hmx_hmatrix_set_plan_options(hmx_hmatrix* h, int32_t denseNear, int32_t directZ, int32_t locality);
 /* Storage of the FP64 matrix-core plan's factors (no counterpart in the
    reference): 0 (default) copies them into the plan's run layout when the copy
@@ -277,12 +297,22 @@ hmx_status hmx_hmatrix_set_plan_options(hmx_hmatrix* h, int32_t denseNear, int32
    phase 1 and phase 2 work on longer runs of columns.  A batch takes the first
    instance's choice.  Drops the built FP64 plan. */
 hmx_status hmx_hmatrix_set_fp64_layout(hmx_hmatrix* h, int32_t layout);
-/* Matrix-core plan summary (plan 0: FP64 multi-RHS, built on the first
-   multi-RHS call; plan 1: mixed, built by hmx_hmatrix_make_mixed): bytes read
-   per call (factors + dense blocks), leaf-level tiles applied as dense blocks
-   (rank above bucketSize/2), low-rank columns.  HMX_ESTATE if not built. */
+/* Matvec plan summary (plan 0: FP64 matrix cores, built on the first call
+   with two or more right-hand sides; plan 1: mixed, built by
+   hmx_hmatrix_make_mixed; plan 2: the FP64 streaming kernels, built on the
+   first call with one right-hand side): bytes read per call or pass (factors
+   + dense blocks), leaf-level tiles applied as dense blocks (rank above
+   bucketSize/2), low-rank columns.  HMX_ESTATE if not built. */
 hmx_status hmx_hmatrix_plan_info(const hmx_hmatrix* h, int32_t plan, uint64_t* bytes, int64_t* denseBlocks,
                                  int64_t* lowRankColumns);
+/* The same plan's bytes split by phase: phase1Bytes = the V factors it
+   reads (Z = V^T X), phase2Bytes = the U factors, the dense diagonal and
+   near-field blocks (or, evaluated on the fly, the points) and the column
+   scales (Y = D X + near + U Z); right-hand sides and z excluded.  No
+   reference counterpart (the reference runs the two phases as CUTLASS
+   grouped GEMMs, src/cutlassHMatrixXVector.cu:152-328). */
+hmx_status hmx_hmatrix_plan_phase_bytes(const hmx_hmatrix* h, int32_t plan, uint64_t* phase1Bytes,
+                                        uint64_t* phase2Bytes);
 /* Y = H X with X, Y n x vectorWidth column-major FP32, any vectorWidth >= 1
    (passes of 64 or 16 right-hand sides, the last padded) and any n (partial
    leaves through the padded leaf space); FP16/FP32 matrix cores with FP32
@@ -330,12 +360,14 @@ hmx_status hmx_denseRowsXVector(hmx_context* ctx, uint32_t n, uint32_t d, const 
 /* ---- many point clouds (no counterpart in the reference, whose driver
    handles one point cloud per run, src/main.cu:46-173) ------------------- */
 /* Assemble `count` H-matrices directly (as hmx_generateHMatrixDirect) in one
-   go: per instance the diagonal blocks, leaf level and padded points, then the
-   upper levels of all instances together -- one batched ARA per (tile
-   dimension, point dimension) group, each tile with its own instance's
-   tolerance schedule, rank cap, random stream and correlation length.
-   cfgs, trees, structures, d_points and hmatrices are arrays of `count`;
-   araR and seed must agree across the batch. */
+   go: the padded points and the diagonal blocks of all instances in one
+   launch each, the leaf levels in one launch sequence per point dimension,
+   then the upper levels of all instances together -- one batched ARA per
+   (tile dimension, point dimension) group, each tile with its own instance's
+   tolerance schedule, rank cap, random stream and correlation length.  The
+   instances' factors share device blocks (a block is released with its last
+   instance).  cfgs, trees, structures, d_points and hmatrices are arrays of
+   `count`; araR and seed must agree across the batch. */
 hmx_status hmx_generateHMatrixDirectBatch(hmx_context* ctx, int32_t count, const hmx_config* cfgs,
                                           hmx_kdtree* const* trees, hmx_structure* const* structures,
                                           const double* const* d_points, hmx_hmatrix** hmatrices);
@@ -367,9 +399,11 @@ hmx_status hmx_batch_free(hmx_batch* batch);
    this call, all contexts and threads) fails as an out-of-memory would, with
    HMX_ENOMEM from the entry point that made it; failAt < 0 fails none and
    only counts.  hmx_debug_alloc_count returns the allocations made since the
-   last hmx_debug_set_alloc_fault call. */
+   last hmx_debug_set_alloc_fault call, hmx_debug_live_bytes the device bytes
+   the library holds right now (all objects, all contexts). */
 void hmx_debug_set_alloc_fault(int64_t failAt);
 int64_t hmx_debug_alloc_count(void);
+int64_t hmx_debug_live_bytes(void);
 
 #ifdef __cplusplus
 }

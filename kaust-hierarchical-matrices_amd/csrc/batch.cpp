@@ -26,11 +26,23 @@ Batch::~Batch() {
     if (stream) (void)hipStreamDestroy(stream);
 }
 
+static void note_gens(Batch& B) {
+    B.pGen.clear();
+    for (const HMatrix* h : B.hs) B.pGen.push_back(h->factorGen);
+}
+
+static bool p_stale(const Batch& B) {
+    for (size_t i = 0; i < B.hs.size(); ++i)
+        if (B.hs[i]->factorGen != B.pGen[i]) return true;
+    return false;
+}
+
 void batch_create(Context& c, const std::vector<HMatrix*>& hs, Batch& B) {
     for (HMatrix* h : hs) HMX_CHECK(h != nullptr, HMX_EINVAL, "null H-matrix in batch");
     B.hs = hs;
     B.b = (int)hs.at(0)->b;
     build_batch_plan(c, hs, 2048, B.P, B.diag);
+    note_gens(B);
     HMX_HIP(hipStreamCreate(&B.stream));  // default priority
     HMX_HIP(hipEventCreateWithFlags(&B.evIn, hipEventDisableTiming));
     HMX_HIP(hipEventCreateWithFlags(&B.evOut, hipEventDisableTiming));
@@ -43,7 +55,17 @@ void batch_create(Context& c, const std::vector<HMatrix*>& hs, Batch& B) {
 // cleared the batch's epoch moves on, and a graph captured in an earlier
 // epoch is captured again before its next replay.
 static void prepare(Context& c, Batch& B, uint32_t k) {
-    const bool changed = k >= 2 ? batch_mf_prepare(c, B, k) : ensure_matvec_ws(B.P, 1);
+    bool changed = false;
+    if (k >= 2) {
+        changed = batch_mf_prepare(c, B, k);
+    } else {
+        if (p_stale(B)) {  // an instance's factors were overwritten: its dense blocks are stale
+            build_batch_plan(c, B.hs, B.P.chunk, B.P, B.diag);
+            note_gens(B);
+            changed = true;
+        }
+        changed = ensure_matvec_ws(B.P, 1) || changed;
+    }
     if (changed) ++B.epoch;
 }
 
@@ -125,6 +147,7 @@ void batch_tune(Context& c, Batch& B, uint32_t k, int* bestChunk, float* bestMs)
                 if (t < bms) { bms = t; best = ch; }
             }
             build_batch_plan(c, B.hs, best, B.P, B.diag);
+            note_gens(B);
         } else {
             batch_mf_prepare(c, B, k);
             for (int q : {0, 8, 16, 32}) {

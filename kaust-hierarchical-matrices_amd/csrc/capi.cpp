@@ -9,7 +9,7 @@
 using namespace hmx;
 
 namespace hmx {
-std::atomic<int64_t> g_allocCount{0}, g_allocFailAt{-1};
+std::atomic<int64_t> g_allocCount{0}, g_allocFailAt{-1}, g_liveBytes{0};
 }
 
 namespace {
@@ -58,9 +58,12 @@ void check_config(const hmx_config* cfg) {
     HMX_CHECK(cfg->strictRankCap == 0 || cfg->strictRankCap == 1, HMX_EINVAL, "strictRankCap must be 0 or 1");
 }
 
-// strictRankCap: a build whose ARA stopped at a rank cap fails
+// strictRankCap: a build whose ARA stopped at a rank cap below the tile
+// dimension fails.  Uncapped ranks (HMX_MAXRANK_FULL) stop at the dimension
+// at most, where the factors are exact, so only the reference schedule can
+// truncate.
 void check_cap_hits(const hmx_config& cfg, uint64_t hits, const char* what) {
-    HMX_CHECK(!cfg.strictRankCap || hits == 0, HMX_ERANKCAP,
+    HMX_CHECK(!cfg.strictRankCap || cfg.maxRankPolicy != HMX_MAXRANK_REF || hits == 0, HMX_ERANKCAP,
               std::to_string(hits) + " " + what + " reached the rank cap before the ARA converged (strictRankCap)");
 }
 
@@ -127,12 +130,36 @@ void hmx_debug_set_alloc_fault(int64_t failAt) {
 
 int64_t hmx_debug_alloc_count(void) { return g_allocCount.load(); }
 
+int64_t hmx_debug_live_bytes(void) { return g_liveBytes.load(); }
+
 hmx_status hmx_generateDataset(hmx_context* ctx, uint32_t n, uint32_t d, uint64_t seed, double* d_points) {
     if (!ctx) return HMX_EINVAL;
     return guard(ctx, [&] {
         NEED(d_points);
         HMX_CHECK(d >= 1 && d <= 3, HMX_EINVAL, "dimensionOfInputPoints must be 1..3");
         generate_dataset(*ctx, n, d, seed, d_points);
+    });
+}
+
+hmx_status hmx_generateDatasetBatch(hmx_context* ctx, int32_t count, const uint32_t* n, const uint32_t* d,
+                                    const uint64_t* seed, double* const* d_points) {
+    if (!ctx) return HMX_EINVAL;
+    return guard(ctx, [&] {
+        NEED(n);
+        NEED(d);
+        NEED(seed);
+        NEED(d_points);
+        HMX_CHECK(count >= 1, HMX_EINVAL, "batch needs at least one cloud");
+        std::vector<uint64_t> counts(count), seeds(count);
+        std::vector<double*> outs(count);
+        for (int32_t k = 0; k < count; ++k) {
+            HMX_CHECK(d[k] >= 1 && d[k] <= 3, HMX_EINVAL, "dimensionOfInputPoints must be 1..3");
+            HMX_CHECK(d_points[k] != nullptr || (uint64_t)n[k] * d[k] == 0, HMX_EINVAL, "null points");
+            counts[k] = (uint64_t)n[k] * d[k];
+            seeds[k] = seed[k];
+            outs[k] = d_points[k];
+        }
+        generate_dataset_batch(*ctx, counts, seeds, outs);
     });
 }
 
@@ -178,6 +205,38 @@ hmx_status hmx_constructKDTree(hmx_context* ctx, uint32_t n, uint32_t d, uint32_
             throw;
         }
         *tree = t;
+    });
+}
+
+hmx_status hmx_constructKDTreeBatch(hmx_context* ctx, int32_t count, const uint32_t* n, const uint32_t* d,
+                                    uint32_t bucketSize, int32_t divMethod, const double* const* d_points,
+                                    hmx_kdtree** trees) {
+    if (!ctx) return HMX_EINVAL;
+    return guard(ctx, [&] {
+        NEED(n);
+        NEED(d);
+        NEED(d_points);
+        NEED(trees);
+        HMX_CHECK(count >= 1, HMX_EINVAL, "batch needs at least one cloud");
+        HMX_CHECK(divMethod == HMX_FULL_TREE || divMethod == HMX_DIVIDE_IN_HALF || divMethod == HMX_POWER_OF_TWO_ON_LEFT,
+                  HMX_EINVAL, "divMethod must be POWER_OF_TWO_ON_LEFT, DIVIDE_IN_HALF or FULL_TREE");
+        for (int32_t k = 0; k < count; ++k) trees[k] = nullptr;
+        std::vector<std::unique_ptr<hmx_kdtree>> own;
+        std::vector<KDTree*> ts;
+        std::vector<const double*> ps;
+        for (int32_t k = 0; k < count; ++k) {
+            HMX_CHECK(d_points[k] != nullptr, HMX_EINVAL, "null points");
+            own.emplace_back(new hmx_kdtree());
+            hmx_kdtree& t = *own.back();
+            t.n = n[k];
+            t.d = d[k];
+            t.b = bucketSize;
+            t.divMethod = divMethod;
+            ts.push_back(&t);
+            ps.push_back(d_points[k]);
+        }
+        build_kdtree_batch(*ctx, ts, ps);
+        for (int32_t k = 0; k < count; ++k) trees[k] = own[k].release();
     });
 }
 
@@ -646,6 +705,8 @@ hmx_status hmx_hmatrix_set_plan_options(hmx_hmatrix* h, int32_t denseNear, int32
     h->planLocality = locality;
     h->mf64 = MfPlan();
     h->mfMixed = MfPlan();
+    h->plan = MatvecPlan();
+    h->planT = MatvecPlan();
     return HMX_OK;
 }
 
@@ -658,12 +719,38 @@ hmx_status hmx_hmatrix_set_fp64_layout(hmx_hmatrix* h, int32_t layout) {
 
 hmx_status hmx_hmatrix_plan_info(const hmx_hmatrix* h, int32_t plan, uint64_t* bytes, int64_t* denseBlocks,
                                  int64_t* lowRankColumns) {
-    if (!h || (plan != 0 && plan != 1)) return HMX_EINVAL;
+    if (!h || plan < 0 || plan > 2) return HMX_EINVAL;
+    if (plan == 2) {  // the streaming (one right-hand side) plan
+        if (!h->plan.built) return HMX_ESTATE;
+        if (bytes) *bytes = h->plan.bytes;
+        if (denseBlocks) *denseBlocks = h->plan.numNear;
+        if (lowRankColumns) *lowRankColumns = h->plan.numCols;
+        return HMX_OK;
+    }
     const MfPlan& P = plan == 0 ? h->mf64 : h->mfMixed;
     if (!P.built) return HMX_ESTATE;
     if (bytes) *bytes = P.bytes;
     if (denseBlocks) *denseBlocks = P.numNear;
     if (lowRankColumns) *lowRankColumns = plan == 0 ? P.g[0].numColsPad : P.hiCols + P.loCols;
+    return HMX_OK;
+}
+
+hmx_status hmx_hmatrix_plan_phase_bytes(const hmx_hmatrix* h, int32_t plan, uint64_t* phase1Bytes,
+                                        uint64_t* phase2Bytes) {
+    if (!h || plan < 0 || plan > 2) return HMX_EINVAL;
+    uint64_t tot = 0, v = 0;
+    if (plan == 2) {
+        if (!h->plan.built) return HMX_ESTATE;
+        tot = h->plan.bytes;
+        v = h->plan.lowRankBytes / 2;
+    } else {
+        const MfPlan& P = plan == 0 ? h->mf64 : h->mfMixed;
+        if (!P.built) return HMX_ESTATE;
+        tot = P.bytes;
+        v = (P.g[0].factorBytes + (plan == 1 ? P.g[1].factorBytes : 0)) / 2;
+    }
+    if (phase1Bytes) *phase1Bytes = v;
+    if (phase2Bytes) *phase2Bytes = tot - v;
     return HMX_OK;
 }
 

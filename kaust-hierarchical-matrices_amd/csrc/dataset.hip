@@ -30,6 +30,46 @@ void generate_dataset(Context& c, uint32_t n, uint32_t d, uint64_t seed, double*
     HMX_LAUNCHED();
 }
 
+struct PointsJob {
+    uint64_t base;  // first value of the cloud among all the launch's values
+    uint64_t seed;
+    double* out;
+};
+
+// many clouds in one launch; cloud k's values are exactly k_points' for its seed
+__global__ void __launch_bounds__(256) k_points_batch(uint64_t total, const PointsJob* __restrict__ jobs, int nj) {
+    const uint64_t e = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= total) return;
+    int lo = 0, hi = nj - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (jobs[mid].base <= e) lo = mid;
+        else hi = mid - 1;
+    }
+    const PointsJob J = jobs[lo];
+    const uint64_t i = e - J.base;
+    uint32_t c[4] = {(uint32_t)i, (uint32_t)(i >> 32), 0u, kTagPoints};
+    philox4x32_10(c, (uint32_t)J.seed, (uint32_t)(J.seed >> 32));
+    J.out[i] = u53(c[0], c[1]);
+}
+
+void generate_dataset_batch(Context& c, const std::vector<uint64_t>& counts, const std::vector<uint64_t>& seeds,
+                            const std::vector<double*>& outs) {
+    std::vector<PointsJob> jobs;
+    uint64_t total = 0;
+    for (size_t k = 0; k < counts.size(); ++k) {
+        if (!counts[k]) continue;
+        jobs.push_back(PointsJob{total, seeds[k], outs[k]});
+        total += counts[k];
+    }
+    if (!total) return;
+    DBuf<PointsJob> dj(jobs.size());
+    HMX_HIP(hipMemcpyAsync(dj.p, jobs.data(), jobs.size() * sizeof(PointsJob), hipMemcpyHostToDevice, c.stream));
+    k_points_batch<<<(unsigned)((total + 255) / 256), 256, 0, c.stream>>>(total, dj.p, (int)jobs.size());
+    HMX_LAUNCHED();
+    HMX_HIP(hipStreamSynchronize(c.stream));  // the job table goes out of scope
+}
+
 // k vectors (column-major, ld n) between original point order and the
 // KD-tree order: position i of the KD order holds original point perm[i]
 __global__ void __launch_bounds__(256) k_reorder(uint32_t n, uint32_t k, int toOriginal,

@@ -71,19 +71,22 @@ void hmatvec_grad_factors(Context& c, HMatrix& h, uint32_t k, const double* X, c
     // space: with partial leaves the outer products read the gathered copies
     // the two matvecs leave in the plans (zero on padding rows, so the
     // padding rows of dU and dV come out zero as the factors' own are).
+    // the gradient plans run every tile through z, in the tile order above
+    if (!h.gplan.built) build_matvec_plan(c, h, false, h.gplan, false);
+    if (!h.gplanT.built) build_matvec_plan(c, h, true, h.gplanT, false);
     DBuf<double> scratch((size_t)h.n * pass_width(k));
     for (uint32_t s0 = 0; s0 < k;) {
         const uint32_t K = pass_width(k - s0);
         const double* Xs = X + (size_t)s0 * h.n;
         const double* Ws = dY + (size_t)s0 * h.n;
-        hmatvec(c, h, K, Xs, scratch.p, false, nullptr, 0);   // leaves Z in h.plan.z
-        hmatvec(c, h, K, Ws, scratch.p, true, nullptr, 0);    // leaves G in h.planT.z
-        const bool pad = h.plan.gather;
-        const double* Xg = pad ? h.plan.Xp.p : Xs;
-        const double* Wg = pad ? h.planT.Xp.p : Ws;
-        const int ld = pad ? (int)h.plan.npad : (int)h.n;
+        run_matvec_plan(c, h.gplan, (int)h.b, K, Xs, (int64_t)h.n, scratch.p, nullptr);   // Z in gplan.z
+        run_matvec_plan(c, h.gplanT, (int)h.b, K, Ws, (int64_t)h.n, scratch.p, nullptr);  // G in gplanT.z
+        const bool pad = h.gplan.gather;
+        const double* Xg = pad ? h.gplan.Xp.p : Xs;
+        const double* Wg = pad ? h.gplanT.Xp.p : Ws;
+        const int ld = pad ? (int)h.gplan.npad : (int)h.n;
         dim3 grid((unsigned)((maxm + 255) / 256), (unsigned)tiles.size());
-        k_grad_outer<<<grid, 256, 0, c.stream>>>(d_tiles.p, (int)K, ld, Xg, Wg, h.plan.z.p, h.planT.z.p,
+        k_grad_outer<<<grid, 256, 0, c.stream>>>(d_tiles.p, (int)K, ld, Xg, Wg, h.gplan.z.p, h.gplanT.z.p,
                                                  s0 > 0 ? 1 : 0);
         HMX_LAUNCHED();
         s0 += K;
@@ -101,10 +104,13 @@ void hmatrix_set_level(Context& c, HMatrix& h, int level, const double* U, const
         HMX_HIP(hipMemcpyAsync(lv.V.p, V, lv.V.bytes(), hipMemcpyDeviceToDevice, c.stream));
     }
     HMX_HIP(hipStreamSynchronize(c.stream));
-    // converted copies used by the matrix-core paths are stale now
+    // converted copies used by the matrix-core paths, and the streaming
+    // plans' dense near-field blocks, are stale now
     ++h.factorGen;
     h.mf64 = MfPlan();
     h.mfMixed = MfPlan();
+    h.plan = MatvecPlan();
+    h.planT = MatvecPlan();
 }
 
 }  // namespace hmx

@@ -139,20 +139,48 @@ __global__ void __launch_bounds__(256) k_lr_sample(LrLevel L) {
 // operand are staged through the wave's slice of LDS 64 at a time.
 
 // padded points: P[(leaf*b + i)*4 + j]
-__global__ void __launch_bounds__(256) k_pad_points(uint32_t n, uint32_t d, int b, uint32_t S,
-                                                    const double* __restrict__ p, const int32_t* __restrict__ perm,
-                                                    const int32_t* __restrict__ offsets, double* __restrict__ P) {
+struct PadCloud {
+    int64_t leafBase;  // first leaf of the cloud in the padded leaf space
+    const double* p;   // SoA coordinates p[j * n + i]
+    const int32_t* perm;
+    const int32_t* offsets;
+    uint32_t n, d;
+};
+
+// The padded leaf space of one or more clouds: position leaf * b + i holds
+// point i of the leaf as (x, y, z, 1), or zeros past the leaf's points; the
+// clouds' leaves follow each other (cloud k from cl[k].leafBase on)
+__global__ void __launch_bounds__(256) k_pad_points(int64_t total, int b, const PadCloud* __restrict__ cl, int ncl,
+                                                    double* __restrict__ P) {
     const int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x;
-    if (e >= (int64_t)S * b) return;
-    const int leaf = (int)(e / b), i = (int)(e % b);
-    const int a0 = offsets[leaf], cnt = offsets[leaf + 1] - a0;
+    if (e >= total) return;
+    const int64_t gl = e / b;
+    const int i = (int)(e % b);
+    int lo = 0, hi = ncl - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (cl[mid].leafBase <= gl) lo = mid;
+        else hi = mid - 1;
+    }
+    const PadCloud q = cl[lo];
+    const int leaf = (int)(gl - q.leafBase);
+    const int a0 = q.offsets[leaf], cnt = q.offsets[leaf + 1] - a0;
     double x[4] = {0.0, 0.0, 0.0, 0.0};
     if (i < cnt) {
-        for (uint32_t j = 0; j < d; ++j) x[j] = p[(size_t)j * n + perm[a0 + i]];
+        for (uint32_t j = 0; j < q.d; ++j) x[j] = q.p[(size_t)j * q.n + q.perm[a0 + i]];
         x[3] = 1.0;
     }
     double4* o = reinterpret_cast<double4*>(P + e * 4);
     *o = make_double4(x[0], x[1], x[2], x[3]);
+}
+
+static void pad_points(Context& c, int b, const std::vector<PadCloud>& cl, int64_t leaves, double* P) {
+    DBuf<PadCloud> d_cl(cl.size());
+    HMX_HIP(hipMemcpyAsync(d_cl.p, cl.data(), cl.size() * sizeof(PadCloud), hipMemcpyHostToDevice, c.stream));
+    const int64_t np = leaves * b;
+    k_pad_points<<<(unsigned)((np + 255) / 256), 256, 0, c.stream>>>(np, b, d_cl.p, (int)cl.size(), P);
+    HMX_LAUNCHED();
+    HMX_HIP(hipStreamSynchronize(c.stream));  // the cloud table goes out of scope
 }
 
 // The samplers run on the FP64 matrix cores (v_mfma_f64_16x16x4f64).  A
@@ -327,10 +355,14 @@ __global__ void __launch_bounds__(256) k_dir_sum(LrLevel L, int splits, const do
     Yout[(size_t)p * bstride + (size_t)t * per + e] = v;
 }
 
-// W[t][s*k + a] = sum_i Q_t[a*m+i] B_t[s*m+i], a < k_t:
+// W[t][s*k + a] = sum_i Q_t[a*m+i] B_t[s*m+i], a < k_t, on the FP64 matrix
+// cores: one workgroup per 16 basis columns of a tile, its four waves take
+// a quarter of the rows each (lane l holds Q^T[a0 + l%16][i + l/16] and
+// B[i + l/16][l%16] for the step at row i), and the four partial 16 x 16
+// blocks are added in wave order.
 // mode 0: B = Y (16 columns, tile still iterating); mode 2: B = accepted block (na columns)
 __global__ void __launch_bounds__(256) k_lr_dot(LrLevel L, int mode) {
-    __shared__ double sA[16][65], sB[16][65];
+    __shared__ double red[3][64][4];
     const int t = blockIdx.y;
     const LrState& S = L.st[t];
     // before the decision step the tile must still be iterating; after it,
@@ -341,29 +373,35 @@ __global__ void __launch_bounds__(256) k_lr_dot(LrLevel L, int mode) {
     const int a0 = blockIdx.x * 16;
     if (a0 >= K) return;
     const int m = L.m;
-    const double* A = L.Q + (size_t)t * m * L.capQ;
-    const double* Bm = (mode == 0 ? L.Y : L.Ya) + (size_t)t * m * kB16;
-    double* out = L.W + (size_t)t * L.capQ * kB16;
-    const int ta = threadIdx.x & 15, ts = threadIdx.x >> 4;
-    double acc = 0.0;
-    for (int i0 = 0; i0 < m; i0 += 64) {
-        for (int e = threadIdx.x; e < 16 * 64; e += 256) {
-            int c = e >> 6, x = e & 63;
-            int i = i0 + x;
-            sA[c][x] = (a0 + c < K && i < m) ? A[(size_t)(a0 + c) * m + i] : 0.0;
-            sB[c][x] = (c < C && i < m) ? Bm[(size_t)c * m + i] : 0.0;
-        }
-        __syncthreads();
-        int lim = min(64, m - i0);
-        for (int x = 0; x < lim; ++x) acc += sA[ta][x] * sB[ts][x];
-        __syncthreads();
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63, lc = lane & 15, lg = lane >> 4;
+    const bool av = a0 + lc < K, sv = lc < C;
+    const double* qa = L.Q + (size_t)t * m * L.capQ + (size_t)(av ? a0 + lc : 0) * m;
+    const double* bs = (mode == 0 ? L.Y : L.Ya) + (size_t)t * m * kB16 + (size_t)(sv ? lc : 0) * m;
+    const int part = m / 4, i0 = w * part;  // m is a multiple of 64
+    v4d_t acc = {0.0, 0.0, 0.0, 0.0};
+    for (int i = i0; i < i0 + part; i += 4) {
+        const double x = av ? qa[i + lg] : 0.0;
+        const double y = sv ? bs[i + lg] : 0.0;
+        acc = __builtin_amdgcn_mfma_f64_16x16x4f64(x, y, acc, 0, 0, 0);
     }
-    if (a0 + ta < K && ts < C) out[ts * K + a0 + ta] = acc;
+    if (w > 0)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) red[w - 1][lane][v] = acc[v];
+    __syncthreads();
+    if (w > 0) return;
+    double* out = L.W + (size_t)t * L.capQ * kB16;
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+        const double r = ((acc[v] + red[0][lane][v]) + red[1][lane][v]) + red[2][lane][v];
+        const int a = a0 + 4 * v + lg;  // output row 4v + l/16, column l%16
+        if (a < K && lc < C) out[lc * K + a] = r;
+    }
 }
 
 // Y[s*m+i] -= sum_a Q[a*m+i] W[s*k+a]  (mode 0: Y, 16 cols; mode 2: Ya, na cols)
+// on the FP64 matrix cores: one wave per 16 rows (lane l holds
+// Q[row0 + l%16][a + l/16] and -W[a + l/16][l%16] for the step at a).
 __global__ void __launch_bounds__(256) k_lr_update(LrLevel L, int mode) {
-    __shared__ double sW[64 * kB16];
     const int t = blockIdx.y;
     const LrState& S = L.st[t];
     if (mode == 0 ? !lr_active(S, L.tp[t].maxRank) : S.na == 0) return;
@@ -371,52 +409,139 @@ __global__ void __launch_bounds__(256) k_lr_update(LrLevel L, int mode) {
     const int C = mode == 0 ? kB16 : S.na;
     if (k == 0 || C == 0) return;
     const int m = L.m;
-    const int i = blockIdx.x * 256 + threadIdx.x;
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63, lc = lane & 15, lg = lane >> 4;
+    const int row0 = blockIdx.x * 64 + 16 * w;
+    if (row0 >= m) return;
     double* Y = (mode == 0 ? L.Y : L.Ya) + (size_t)t * m * kB16;
     const double* Q = L.Q + (size_t)t * m * L.capQ;
     const double* W = L.W + (size_t)t * L.capQ * kB16;
-    double acc[kB16];
+    const bool sv = lc < C;
+    v4d_t acc;
 #pragma unroll
-    for (int s = 0; s < kB16; ++s) acc[s] = (s < C && i < m) ? Y[(size_t)s * m + i] : 0.0;
-    for (int a0 = 0; a0 < k; a0 += 64) {
-        int na = min(64, k - a0);
-        __syncthreads();
-        for (int e = threadIdx.x; e < na * C; e += 256) {
-            int s = e / na, a = e % na;
-            sW[a * kB16 + s] = W[s * k + a0 + a];
-        }
-        __syncthreads();
-        if (i < m) {
-            for (int a = 0; a < na; ++a) {
-                double q = Q[(size_t)(a0 + a) * m + i];
-#pragma unroll
-                for (int s = 0; s < kB16; ++s)
-                    if (s < C) acc[s] -= q * sW[a * kB16 + s];
-            }
-        }
+    for (int v = 0; v < 4; ++v) acc[v] = sv ? Y[(size_t)lc * m + row0 + 4 * v + lg] : 0.0;
+    for (int a = 0; a < k; a += 4) {
+        const bool ok = a + lg < k;
+        const double x = ok ? Q[(size_t)(a + lg) * m + row0 + lc] : 0.0;
+        const double y = ok && sv ? -W[(size_t)lc * k + a + lg] : 0.0;
+        acc = __builtin_amdgcn_mfma_f64_16x16x4f64(x, y, acc, 0, 0, 0);
     }
-    if (i < m)
+    if (sv)
 #pragma unroll
-        for (int s = 0; s < kB16; ++s)
-            if (s < C) Y[(size_t)s * m + i] = acc[s];
+        for (int v = 0; v < 4; ++v) Y[(size_t)lc * m + row0 + 4 * v + lg] = acc[v];
 }
 
-__device__ inline double wave_sum64(double v) {
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-    return v;
-}
-
-// sum over the block; every thread receives the identical value
-__device__ inline double block_sum(double v, double* red) {
-    v = wave_sum64(v);
+// sum over the block; every thread receives the identical value.  red holds
+// two slots of 16: consecutive calls alternate (parity), so one barrier per
+// call suffices -- a thread reaches the call after next only once every
+// thread has passed the next call's barrier, i.e. has read this one's slot.
+__device__ inline double block_sum(double v, double* red, int& parity) {
+    v = wave_sum_dpp(v);
     const int w = threadIdx.x >> 6, nw = blockDim.x >> 6;
-    __syncthreads();
-    if ((threadIdx.x & 63) == 0) red[w] = v;
+    double* slot = red + 16 * parity;
+    parity ^= 1;
+    if ((threadIdx.x & 63) == 0) slot[w] = v;
     __syncthreads();
     double s = 0.0;
-    for (int q = 0; q < nw; ++q) s += red[q];
+    for (int q = 0; q < nw; ++q) s += slot[q];
     return s;
+}
+
+// One wave per tile for tiles of m <= 512 rows (R = m / 64 rows per lane):
+// the block's accepted columns and the current column stay in registers and
+// every reduction is a DPP wave sum -- no barriers, no re-reads.  The same
+// decisions as k_lr_decide.
+template <int R>
+__global__ void __launch_bounds__(64) k_lr_decide_w(LrLevel L) {
+    const int t = blockIdx.x, lane = threadIdx.x;
+    LrState& S = L.st[t];
+    if (!lr_active(S, L.tp[t].maxRank)) {
+        if (lane == 0) S.na = 0;
+        return;
+    }
+    const int m = L.m;
+    const double* Y = L.Y + (size_t)t * m * kB16;
+    double* Qb = L.Ya + (size_t)t * m * kB16;
+    double qb[kB16][R];
+    int na = 0, small = S.small_run, conv = 0, capped = 0;
+    const int capleft = L.tp[t].maxRank - S.k;
+    const double tau = L.tp[t].tau;
+    for (int j = 0; j < kB16; ++j) {
+        double y[R];
+#pragma unroll
+        for (int r = 0; r < R; ++r) y[r] = Y[(size_t)j * m + lane + 64 * r];
+#pragma unroll
+        for (int pass = 0; pass < 2; ++pass)
+#pragma unroll
+            for (int a = 0; a < kB16; ++a)
+                if (a < na) {
+                    double p = 0.0;
+#pragma unroll
+                    for (int r = 0; r < R; ++r) p += qb[a][r] * y[r];
+                    const double d = wave_sum_dpp(p);
+#pragma unroll
+                    for (int r = 0; r < R; ++r) y[r] -= qb[a][r] * d;
+                }
+        double p = 0.0;
+#pragma unroll
+        for (int r = 0; r < R; ++r) p += y[r] * y[r];
+        const double nu = sqrt(wave_sum_dpp(p));
+        if (nu > tau) {
+            if (na >= capleft) { capped = 1; break; }
+#pragma unroll
+            for (int a = 0; a < kB16; ++a)
+                if (a == na)
+#pragma unroll
+                    for (int r = 0; r < R; ++r) qb[a][r] = y[r] / nu;
+#pragma unroll
+            for (int r = 0; r < R; ++r) Qb[(size_t)na * m + lane + 64 * r] = y[r] / nu;
+            ++na;
+            small = 0;
+        } else if (++small >= L.r) {
+            conv = 1;
+            break;
+        }
+    }
+    if (lane == 0) { S.na = na; S.small_run = small; S.converged = conv; S.capped = capped; }
+}
+
+// k_lr_renorm for tiles of m <= 512 rows, one wave per tile, registers only
+template <int R>
+__global__ void __launch_bounds__(64) k_lr_renorm_w(LrLevel L) {
+    const int t = blockIdx.x, lane = threadIdx.x;
+    const LrState& S = L.st[t];
+    const int na = S.na;
+    if (na == 0) return;
+    const int m = L.m;
+    double* Qb = L.Ya + (size_t)t * m * kB16;
+    double* Q = L.Q + (size_t)t * m * L.capQ;
+    double qb[kB16][R];
+#pragma unroll
+    for (int j = 0; j < kB16; ++j)
+        if (j < na) {
+            double q[R];
+#pragma unroll
+            for (int r = 0; r < R; ++r) q[r] = Qb[(size_t)j * m + lane + 64 * r];
+#pragma unroll
+            for (int a = 0; a < j; ++a) {
+                double p = 0.0;
+#pragma unroll
+                for (int r = 0; r < R; ++r) p += qb[a][r] * q[r];
+                const double d = wave_sum_dpp(p);
+#pragma unroll
+                for (int r = 0; r < R; ++r) q[r] -= qb[a][r] * d;
+            }
+            double p = 0.0;
+#pragma unroll
+            for (int r = 0; r < R; ++r) p += q[r] * q[r];
+            const double nu = sqrt(wave_sum_dpp(p));
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                const double v = q[r] / nu;
+                qb[j][r] = v;
+                Qb[(size_t)j * m + lane + 64 * r] = v;
+                Q[(size_t)(S.k + j) * m + lane + 64 * r] = v;
+            }
+        }
 }
 
 // One workgroup per tile: column-by-column decisions on explicitly projected
@@ -424,7 +549,8 @@ __device__ inline double block_sum(double v, double* red) {
 // columns, accept when the norm exceeds tau).  Threads own fixed rows, so
 // only the reductions need the workgroup barrier.
 __global__ void __launch_bounds__(1024) k_lr_decide(LrLevel L) {
-    __shared__ double red[16];
+    __shared__ double red[32];
+    int parity = 0;
     const int t = blockIdx.x;
     LrState& S = L.st[t];
     if (!lr_active(S, L.tp[t].maxRank)) {
@@ -444,12 +570,12 @@ __global__ void __launch_bounds__(1024) k_lr_decide(LrLevel L) {
                 const double* qa = Qb + (size_t)a * m;
                 double p = 0.0;
                 for (int i = tid; i < m; i += nthr) p += qa[i] * y[i];
-                const double d = block_sum(p, red);
+                const double d = block_sum(p, red, parity);
                 for (int i = tid; i < m; i += nthr) y[i] -= qa[i] * d;
             }
         double p = 0.0;
         for (int i = tid; i < m; i += nthr) p += y[i] * y[i];
-        const double nu = sqrt(block_sum(p, red));
+        const double nu = sqrt(block_sum(p, red, parity));
         if (nu > tau) {
             if (na >= capleft) { capped = 1; break; }
             for (int i = tid; i < m; i += nthr) Qb[(size_t)na * m + i] = y[i] / nu;
@@ -466,7 +592,8 @@ __global__ void __launch_bounds__(1024) k_lr_decide(LrLevel L) {
 // After the accepted columns were projected against Q: one more modified
 // Gram-Schmidt sweep inside the block, normalise, append to Q.
 __global__ void __launch_bounds__(1024) k_lr_renorm(LrLevel L) {
-    __shared__ double red[16];
+    __shared__ double red[32];
+    int parity = 0;
     const int t = blockIdx.x;
     const LrState& S = L.st[t];
     const int na = S.na;
@@ -480,12 +607,12 @@ __global__ void __launch_bounds__(1024) k_lr_renorm(LrLevel L) {
             const double* qa = Qb + (size_t)a * m;
             double p = 0.0;
             for (int i = tid; i < m; i += nthr) p += qa[i] * q[i];
-            const double d = block_sum(p, red);
+            const double d = block_sum(p, red, parity);
             for (int i = tid; i < m; i += nthr) q[i] -= qa[i] * d;
         }
         double p = 0.0;
         for (int i = tid; i < m; i += nthr) p += q[i] * q[i];
-        const double nu = sqrt(block_sum(p, red));
+        const double nu = sqrt(block_sum(p, red, parity));
         for (int i = tid; i < m; i += nthr) {
             const double v = q[i] / nu;
             q[i] = v;
@@ -834,7 +961,7 @@ static void compress_batch(Context& c, int araR, uint64_t seed, const LevelSourc
     k_lr_init<<<(T + 255) / 256, 256, 0, c.stream>>>(T, ws.st.p);
     HMX_LAUNCHED();
     const auto tB = std::chrono::steady_clock::now();
-    const unsigned rowBlocks = (unsigned)((m + 255) / 256);
+    const unsigned rowBlocks = (unsigned)((m + 63) / 64);
     const unsigned wgThreads = m >= 4096 ? 1024u : 256u;
     int kmax = 0, preStart = 0, preN = 0, sweeps = 0;
     int active = T;  // tiles still iterating (the count k_lr_commit reports)
@@ -886,7 +1013,11 @@ static void compress_batch(Context& c, int araR, uint64_t seed, const LevelSourc
             k_lr_update<<<dim3(rowBlocks, T), 256, 0, c.stream>>>(L, 0);
             HMX_LAUNCHED();
         }
-        k_lr_decide<<<T, wgThreads, 0, c.stream>>>(L);
+        if (m == 64) k_lr_decide_w<1><<<T, 64, 0, c.stream>>>(L);
+        else if (m == 128) k_lr_decide_w<2><<<T, 64, 0, c.stream>>>(L);
+        else if (m == 256) k_lr_decide_w<4><<<T, 64, 0, c.stream>>>(L);
+        else if (m == 512) k_lr_decide_w<8><<<T, 64, 0, c.stream>>>(L);
+        else k_lr_decide<<<T, wgThreads, 0, c.stream>>>(L);
         HMX_LAUNCHED();
         if (kmax > 0) {
             k_lr_dot<<<dim3(kb, T), 256, 0, c.stream>>>(L, 2);
@@ -894,7 +1025,11 @@ static void compress_batch(Context& c, int araR, uint64_t seed, const LevelSourc
             k_lr_update<<<dim3(rowBlocks, T), 256, 0, c.stream>>>(L, 2);
             HMX_LAUNCHED();
         }
-        k_lr_renorm<<<T, wgThreads, 0, c.stream>>>(L);
+        if (m == 64) k_lr_renorm_w<1><<<T, 64, 0, c.stream>>>(L);
+        else if (m == 128) k_lr_renorm_w<2><<<T, 64, 0, c.stream>>>(L);
+        else if (m == 256) k_lr_renorm_w<4><<<T, 64, 0, c.stream>>>(L);
+        else if (m == 512) k_lr_renorm_w<8><<<T, 64, 0, c.stream>>>(L);
+        else k_lr_renorm<<<T, wgThreads, 0, c.stream>>>(L);
         HMX_LAUNCHED();
         HMX_HIP(hipMemsetAsync(ws.flags.p, 0, 2 * sizeof(int), c.stream));
         k_lr_commit<<<(T + 255) / 256, 256, 0, c.stream>>>(L, ws.flags.p);
@@ -1160,33 +1295,31 @@ void build_hmatrix_direct(Context& c, const hmx_config& cfg, const KDTree& t, co
     h.structure = &h.structureCopy;
     h.levels.clear();
     h.levels.resize(L - 1);
+    DBuf<double> P((size_t)t.numSegments * b * 4);
+    pad_points(c, b, {PadCloud{0, d_points, t.perm.p, t.offsets.p, t.n, t.d}}, t.numSegments, P.p);
     h.diag.alloc((size_t)t.numSegments * b * b);
-    build_dense_diagonal(c, cfg, t, d_points, h.diag.p);
+    HLevel& leafLevel = h.levels[L - 2];
+    leafLevel.level = L - 1;
+    const std::vector<LeafJob> job = {LeafJob{&cfg, 0, t.numSegments, &s.tileIndices[L - 2], &leafLevel}};
+    build_dense_diagonal_padded(c, t.b, t.d, P.p, job, h.diag.p);
     {
-        HLevel& lv = h.levels[L - 2];
-        lv.level = L - 1;
-        lv.maxRank = cfg.maxRankPolicy == HMX_MAXRANK_REF ? b / 2 : b;
-        lv.tol = cfg.lowestLevelTolerance;
         const auto t0 = std::chrono::steady_clock::now();
-        compress_leaf_tiles(c, cfg, t, d_points, s.tileIndices[L - 2], lv);
+        compress_leaf_tiles_padded(c, t.b, t.d, P.p, job);
         if (log_on()) {
             const double ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
-            std::fprintf(stderr, "[hmx] level %d (leaf tiles): %d tiles rank sum %lld, %.1f ms\n", L - 1, lv.numTiles,
-                         (long long)lv.rankSum, ms);
+            std::fprintf(stderr, "[hmx] level %d (leaf tiles): %d tiles rank sum %lld, %.1f ms\n", L - 1,
+                         leafLevel.numTiles, (long long)leafLevel.rankSum, ms);
             std::fflush(stderr);
         }
     }
-    DBuf<double> P((size_t)t.numSegments * b * 4);
-    const int64_t np = (int64_t)t.numSegments * b;
-    k_pad_points<<<(unsigned)((np + 255) / 256), 256, 0, c.stream>>>(t.n, t.d, b, t.numSegments, d_points, t.perm.p,
-                                                                      t.offsets.p, P.p);
-    HMX_LAUNCHED();
     LevelSource src;
     src.P = P.p;
     src.ell = cfg.correlationLength > 0 ? cfg.correlationLength : default_ell(t.d);
     src.d = (int)t.d;
     compress_upper_levels(c, cfg, s, src, b, h);
     HMX_HIP(hipStreamSynchronize(c.stream));
+    h.pts = std::move(P);
+    h.ell = src.ell;
     finish_accounting(h);
 }
 
@@ -1221,7 +1354,21 @@ void build_hmatrix_direct_batch(Context& c, const std::vector<const hmx_config*>
         pbase[i] = ptot;
         ptot += (int64_t)t.numSegments * b;
     }
-    DBuf<double> P((size_t)ptot * 4);
+    // the padded points, the diagonal blocks and the leaf level of every
+    // instance: one launch each (the leaf level one launch sequence per point
+    // dimension); every instance gets its range of the shared blocks as a view
+    const int64_t leaves = ptot / b;
+    auto P = std::make_shared<DBuf<double>>((size_t)ptot * 4);
+    auto Dg = std::make_shared<DBuf<double>>((size_t)leaves * b * b);
+    {
+        std::vector<PadCloud> pc(N);
+        for (size_t i = 0; i < N; ++i) {
+            const KDTree& t = *trees[i];
+            pc[i] = PadCloud{pbase[i] / b, pts[i], t.perm.p, t.offsets.p, t.n, t.d};
+        }
+        pad_points(c, b, pc, leaves, P->p);
+    }
+    std::vector<LeafJob> jobsByD[4];
     for (size_t i = 0; i < N; ++i) {
         const KDTree& t = *trees[i];
         const Structure& s = *ss[i];
@@ -1233,17 +1380,17 @@ void build_hmatrix_direct_batch(Context& c, const std::vector<const hmx_config*>
         h.structure = &h.structureCopy;
         h.levels.clear();
         h.levels.resize(L - 1);
-        h.diag.alloc((size_t)t.numSegments * b * b);
-        build_dense_diagonal(c, *cfgs[i], t, pts[i], h.diag.p);
-        HLevel& lv = h.levels[L - 2];
-        lv.level = L - 1;
-        lv.maxRank = cfgs[i]->maxRankPolicy == HMX_MAXRANK_REF ? b / 2 : b;
-        lv.tol = cfgs[i]->lowestLevelTolerance;
-        compress_leaf_tiles(c, *cfgs[i], t, pts[i], s.tileIndices[L - 2], lv);
-        const int64_t np = (int64_t)t.numSegments * b;
-        k_pad_points<<<(unsigned)((np + 255) / 256), 256, 0, c.stream>>>(t.n, t.d, b, t.numSegments, pts[i],
-                                                                          t.perm.p, t.offsets.p, P.p + pbase[i] * 4);
-        HMX_LAUNCHED();
+        h.levels[L - 2].level = L - 1;
+        h.diag.view(Dg, (size_t)pbase[i] * b, (size_t)t.numSegments * b * b);
+        h.pts.view(P, (size_t)pbase[i] * 4, (size_t)t.numSegments * b * 4);
+        h.ell = cfgs[i]->correlationLength > 0 ? cfgs[i]->correlationLength : default_ell(t.d);
+        jobsByD[t.d].push_back(
+            LeafJob{cfgs[i], pbase[i] / b, t.numSegments, &hs[i]->structure->tileIndices[L - 2], &h.levels[L - 2]});
+    }
+    for (uint32_t d = 1; d <= 3; ++d) {
+        if (jobsByD[d].empty()) continue;
+        build_dense_diagonal_padded(c, (uint32_t)b, d, P->p, jobsByD[d], Dg->p);
+        compress_leaf_tiles_padded(c, (uint32_t)b, d, P->p, jobsByD[d]);
     }
     // upper levels: the reference's schedule per instance, grouped across instances
     std::vector<int32_t> leafRanks;
@@ -1311,11 +1458,13 @@ void build_hmatrix_direct_batch(Context& c, const std::vector<const hmx_config*>
                 tp.push_back(tile_param((double)lv.tol, ell, pbase[j.inst], j.level, t, lv.maxRank, tiles[t]));
         }
         LevelSource src;
-        src.P = P.p;
+        src.P = P->p;
         src.d = d;
         LrOut r;
         compress_batch(c, cfgs[0]->araR, cfgs[0]->seed, src, b, m, tp, {}, first_sweep_blocks(leafRanks, cfgs[0]->araR),
                        ws, r);
+        auto gU = std::make_shared<DBuf<double>>(std::move(r.U));
+        auto gV = std::make_shared<DBuf<double>>(std::move(r.V));
         size_t t0 = 0;
         for (const Job& j : grp.second) {
             HLevel& lv = hs[j.inst]->levels[j.level - 1];
@@ -1331,15 +1480,8 @@ void build_hmatrix_direct_batch(Context& c, const std::vector<const hmx_config*>
             lv.h_offsets[T] = acc;
             lv.rankSum = acc;
             lv.capHits = caps;
-            lv.U.alloc((size_t)acc * m);
-            lv.V.alloc((size_t)acc * m);
-            if (acc > 0) {
-                const size_t o = (size_t)r.offsets[t0] * m;
-                HMX_HIP(hipMemcpyAsync(lv.U.p, r.U.p + o, (size_t)acc * m * sizeof(double), hipMemcpyDeviceToDevice,
-                                       c.stream));
-                HMX_HIP(hipMemcpyAsync(lv.V.p, r.V.p + o, (size_t)acc * m * sizeof(double), hipMemcpyDeviceToDevice,
-                                       c.stream));
-            }
+            lv.U.view(gU, (size_t)r.offsets[t0] * m, (size_t)acc * m);
+            lv.V.view(gV, (size_t)r.offsets[t0] * m, (size_t)acc * m);
             t0 += T;
         }
         HMX_HIP(hipStreamSynchronize(c.stream));

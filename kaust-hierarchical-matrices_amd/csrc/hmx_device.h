@@ -46,14 +46,17 @@ __device__ inline double gauss_at(uint64_t seed, uint32_t tag, uint64_t a, uint3
 }
 
 // Exponential covariance K(x,y) = exp(-||x-y||/ell), same operation order as
-// interaction() in the reference (src/TLRMatrixHelpers.cu:44-58).
+// interaction() in the reference (src/TLRMatrixHelpers.cu:44-58).  Its
+// diff += (x - y)*(x - y) is one fused multiply-add under nvcc's default
+// contraction (--fmad=true); it is written as fma here, so every kernel of
+// the library rounds it the same way whatever the compiler's contraction.
 template <int D>
 __device__ inline double kernel_eval(const double* __restrict__ a, const double* __restrict__ b, double ell) {
     double diff = 0.0;
 #pragma unroll
     for (int j = 0; j < D; ++j) {
-        double t = a[j] - b[j];
-        diff = __dadd_rn(diff, __dmul_rn(t, t));
+        const double t = a[j] - b[j];
+        diff = fma(t, t, diff);
     }
     return exp(-sqrt(diff) / ell);
 }
@@ -106,10 +109,40 @@ __device__ inline double kernel_eval_scaled(const double* __restrict__ a, const 
 __device__ inline double kernel_eval_d(const double* a, const double* b, int d, double ell) {
     double diff = 0.0;
     for (int j = 0; j < d; ++j) {
-        double t = a[j] - b[j];
-        diff = __dadd_rn(diff, __dmul_rn(t, t));
+        const double t = a[j] - b[j];
+        diff = fma(t, t, diff);
     }
     return exp(-sqrt(diff) / ell);
+}
+
+// lane i of a 16-lane row receives lane (i ^ 1) / (i ^ 2) / (7 - i within 8) /
+// (15 - i) of the same row (DPP quad_perm [1,0,3,2], [2,3,0,1],
+// row_half_mirror, row_mirror)
+template <int CTRL>
+__device__ inline double dpp_f64(double v) {
+    const long long u = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_update_dpp(0, (int)u, CTRL, 0xF, 0xF, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, (int)(u >> 32), CTRL, 0xF, 0xF, false);
+    return __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo));
+}
+
+__device__ inline double lane_f64(double v, int lane) {
+    const long long u = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readlane((int)u, lane);
+    const int hi = __builtin_amdgcn_readlane((int)(u >> 32), lane);
+    return __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo));
+}
+
+// Sum over the 64 lanes, the same value in every lane (wave-uniform): each
+// DPP step adds two values that every lane of the pair adds in either order
+// (a + b == b + a exactly), so every lane of a 16-lane row ends with the same
+// row sum; the four row sums are then added in a fixed order.
+__device__ inline double wave_sum_dpp(double v) {
+    v += dpp_f64<0xB1>(v);
+    v += dpp_f64<0x4E>(v);
+    v += dpp_f64<0x141>(v);
+    v += dpp_f64<0x140>(v);
+    return (lane_f64(v, 0) + lane_f64(v, 16)) + (lane_f64(v, 32) + lane_f64(v, 48));
 }
 
 // Morton helpers (src/helperKernels.cuh:87-125): column in even bits, row in odd bits

@@ -5,6 +5,7 @@
 
 #include <atomic>
 #include <cstdint>
+#include <memory>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -38,7 +39,7 @@ struct Error : std::runtime_error {
 
 // Forced allocation failure for the error-path tests (hmx_debug_set_alloc_fault):
 // allocation number g_allocFailAt (counted by g_allocCount) fails.
-extern std::atomic<int64_t> g_allocCount, g_allocFailAt;
+extern std::atomic<int64_t> g_allocCount, g_allocFailAt, g_liveBytes;
 inline void alloc_fault_point() {
     const int64_t k = g_allocCount.fetch_add(1);
     if (k == g_allocFailAt.load())
@@ -50,26 +51,48 @@ template <class T>
 struct DBuf {
     T* p = nullptr;
     size_t n = 0;
+    std::shared_ptr<void> keep;  // set for a view: the shared block this range lies in
     DBuf() = default;
     explicit DBuf(size_t count) { alloc(count); }
     DBuf(const DBuf&) = delete;
     DBuf& operator=(const DBuf&) = delete;
-    DBuf(DBuf&& o) noexcept : p(o.p), n(o.n) { o.p = nullptr; o.n = 0; }
+    DBuf(DBuf&& o) noexcept : p(o.p), n(o.n), keep(std::move(o.keep)) { o.p = nullptr; o.n = 0; }
     DBuf& operator=(DBuf&& o) noexcept {
-        if (this != &o) { release(); p = o.p; n = o.n; o.p = nullptr; o.n = 0; }
+        if (this != &o) {
+            release();
+            p = o.p; n = o.n; keep = std::move(o.keep);
+            o.p = nullptr; o.n = 0;
+        }
         return *this;
     }
     ~DBuf() { release(); }
+    // A view of count elements at offset off of a shared block: no copy; the
+    // block is freed with its last view (batched builds hand every instance
+    // its range of one allocation this way)
+    void view(const std::shared_ptr<DBuf<T>>& block, size_t off, size_t count) {
+        release();
+        if (count) {
+            keep = block;
+            p = block->p + off;
+        }
+        n = count;
+    }
     void alloc(size_t count) {
         release();
         if (count) {
             alloc_fault_point();
             HMX_HIP(hipMalloc((void**)&p, count * sizeof(T)));
+            g_liveBytes.fetch_add((int64_t)(count * sizeof(T)));
         }
         n = count;
     }
     void release() {
-        if (p) (void)hipFree(p);
+        if (keep) {
+            keep.reset();
+        } else if (p) {
+            (void)hipFree(p);
+            g_liveBytes.fetch_sub((int64_t)(n * sizeof(T)));
+        }
         p = nullptr;
         n = 0;
     }
@@ -146,9 +169,20 @@ struct MvEntry {
     int32_t pad;
 };
 
+// Leaf-level tile applied as a dense block by the streaming plan: the block
+// is U V^T (V U^T for the transposed plan), outLeaf += block * x[inLeaf].
+struct NearRef {
+    const double* U;
+    const double* V;
+    int32_t r, outLeaf, inLeaf, pad;
+};
+
 // Host form of a matvec plan over the padded vector space (position
 // leaf*b + i for point i of a leaf; padded positions hold zeros).
 struct PlanHost {
+    int b = 0;
+    std::vector<NearRef> nears;     // leaf-level tiles applied as dense blocks
+    uint64_t lowRankBytes = 0;      // U and V bytes of the tiles applied as factors
     std::vector<MvItem> items;
     std::vector<int64_t> colPart;
     std::vector<int32_t> colNpart;
@@ -180,6 +214,12 @@ struct MatvecPlan {
     DBuf<MvEntry> entries;
     int rowBlock = 0;        // tiles of dimension >= rowBlock in a second phase-2 pass (0: none)
     int32_t numBlocks = 0;
+    // leaf-level tiles of rank above b/2 as dense column-major b x b blocks
+    int64_t numNear = 0;
+    DBuf<int32_t> nearPtr, nearCol;  // CSR over output leaves; the block's input leaf
+    DBuf<double> nearD;
+    uint64_t bytes = 0;              // factor and dense-block bytes one pass reads
+    uint64_t lowRankBytes = 0;       // of which U and V of the tiles applied as factors
     DBuf<int32_t> blkPtr;    // CSR over rowBlock-row blocks
     DBuf<MvEntry> blkEntries;
     const double* diag = nullptr;  // D (or D^T for the transpose plan)
@@ -272,6 +312,11 @@ struct MfPlan {
     int xcdChunk = -1;             // phase-1 workgroups per XCD chunk (-1: the library default)
     DBuf<unsigned char> Xb;        // right-hand sides in the blocked layout (plan's dense type)
     DBuf<_Float16> X16;            // the same in FP16, scaled per right-hand side (mixed)
+    DBuf<_Float16> X16lo;          // X/scale - X16 in FP16 (mixed with near-field blocks on the fly)
+    bool nearOtf = false;          // mixed: diagonal and near-field blocks evaluated from the points
+    DBuf<float> ptsL;              // nearOtf: points relative to their leaf's first point, FP32 (x, y, z, 0)
+    DBuf<double> origin;           // nearOtf: each leaf's first point, FP64 (x, y, z, w)
+    float nlog2e = 0.f;            // -log2(e) / correlation length (nearOtf)
     DBuf<float> xscale, xpart;     // per right-hand-side scale and its per-slice partial maxima
     size_t x16n = 0;
     float loMul = 1.f;             // 2^-E: FP16 z holds V^T x scaled by 2^E (mixed)
@@ -291,6 +336,7 @@ struct HMatrix {
     const Structure* structure = nullptr;
     Structure structureCopy;        // owned copy (the caller may free theirs)
     MatvecPlan plan, planT;
+    MatvecPlan gplan, gplanT;        // factor gradients: every tile through z (no dense near-field blocks)
     MfPlan mf64, mfMixed;
     // matrix-core plan choices (hmx_hmatrix_set_plan_options)
     int planDenseNear = 1;   // leaf-level tiles of rank > b/2 as dense blocks
@@ -301,6 +347,12 @@ struct HMatrix {
     // direct assembly: the upper-level samplers' algorithmic flops, kernel
     // evaluations and kernel time (hmx_hmatrix_build_info)
     double samplerFlops = 0, kernelEvals = 0, samplerMs = 0;
+    // direct assembly: the points in the padded leaf space, (x, y, z, w) per
+    // position (w = 1 for a point, 0 for padding), and the kernel's
+    // correlation length -- what the mixed product evaluates its diagonal and
+    // near-field blocks from
+    DBuf<double> pts;
+    double ell = 0;
 };
 
 // Many H-matrices applied as one block-diagonal operator: one concatenated
@@ -308,6 +360,7 @@ struct HMatrix {
 struct Batch {
     std::vector<HMatrix*> hs;
     MatvecPlan P;                // streaming kernels (one right-hand side)
+    std::vector<uint64_t> pGen;  // instances' factorGen when P was built (its dense blocks are copies)
     MfPlan mf;                   // FP64 matrix cores (two or more), built on first use
     std::vector<uint64_t> mfGen; // instances' factorGen when mf was built
     DBuf<double> diag;
@@ -327,11 +380,14 @@ struct Batch {
 
 // ---- internal entry points (implemented in the .hip/.cpp files) ----------
 void generate_dataset(Context& c, uint32_t n, uint32_t d, uint64_t seed, double* d_points);
+void generate_dataset_batch(Context& c, const std::vector<uint64_t>& counts, const std::vector<uint64_t>& seeds,
+                            const std::vector<double*>& outs);
 void generate_random_vector(Context& c, uint32_t k, uint32_t n, uint64_t seed, double* d_out);
 void stream_read(Context& c, const void* buf, uint64_t bytes, int iters, float* ms);
 void reorder_vectors(Context& c, uint32_t n, uint32_t k, int toOriginal, const int32_t* d_perm, const double* d_in,
                      double* d_out);
 void build_kdtree(Context& c, KDTree& t, const double* d_points);
+void build_kdtree_batch(Context& c, const std::vector<KDTree*>& ts, const std::vector<const double*>& pts);
 void build_structure(Structure& s, uint32_t n, uint32_t d, uint32_t b, int adm, float eta, const double* bbmin,
                      const double* bbmax);
 void build_tlr_cm(Context& c, const hmx_config& cfg, const KDTree& t, const double* d_points, TLR& out);
@@ -342,13 +398,22 @@ void build_hmatrix_direct(Context& c, const hmx_config& cfg, const KDTree& t, co
 void build_hmatrix_direct_batch(Context& c, const std::vector<const hmx_config*>& cfgs,
                                 const std::vector<const KDTree*>& trees, const std::vector<const Structure*>& ss,
                                 const std::vector<const double*>& pts, const std::vector<HMatrix*>& hs);
-void compress_leaf_tiles(Context& c, const hmx_config& cfg, const KDTree& t, const double* d_points,
-                         const std::vector<int32_t>& tiles, HLevel& out);
-void build_dense_diagonal(Context& c, const hmx_config& cfg, const KDTree& t, const double* d_points, double* diag);
+// One cloud's share of a padded-leaf-space build: its leaves start at
+// leafBase; tiles is its leaf level's Morton tile list; out receives the level
+struct LeafJob {
+    const hmx_config* cfg;
+    int64_t leafBase;
+    uint32_t S;
+    const std::vector<int32_t>* tiles;
+    HLevel* out;
+};
+void compress_leaf_tiles_padded(Context& c, uint32_t b, uint32_t d, const double* P, const std::vector<LeafJob>& jobs);
+void build_dense_diagonal_padded(Context& c, uint32_t b, uint32_t d, const double* P, const std::vector<LeafJob>& jobs,
+                                 double* diag);
 void hmatrix_save(Context& c, const HMatrix& h, const char* path);
 void hmatrix_load(Context& c, const char* path, HMatrix& h);
-void build_matvec_plan(Context& c, HMatrix& h, bool transpose);
-PlanHost plan_host(const HMatrix& h, bool transpose, int chunk, int rowBlock = 0);
+void build_matvec_plan(Context& c, HMatrix& h, bool transpose, MatvecPlan& P, bool denseNear);
+PlanHost plan_host(const HMatrix& h, bool transpose, int chunk, int rowBlock, bool denseNear);
 void upload_plan(Context& c, MatvecPlan& P, const PlanHost& H);
 bool ensure_matvec_ws(MatvecPlan& P, uint32_t K);
 // widest streaming-kernel pass (16, 8, 4 or 1 right-hand sides) for rem columns

@@ -8,6 +8,12 @@
 // in LDS (bitonic network on the unique (key, position) pair, which equals a
 // stable sort); larger segments use a segmented LSD radix sort (8 passes of 8
 // bits, stable wave-level multisplit).
+//
+// Many clouds of the same dimension are built as one forest: their points
+// side by side, every level's segments of every cloud sorted by the same
+// launches, so a batch of clouds costs launches per level, not per cloud.
+// Each segment is processed exactly as in a one-cloud build, so every tree
+// is bitwise the tree its cloud gets alone.
 #include <algorithm>
 #include <cfloat>
 
@@ -21,7 +27,7 @@ constexpr int kSmallSeg = 2048;
 constexpr int kRadixChunk = 2048;
 
 // partial min/max of one chunk of one segment, all dims (d <= 3)
-__global__ void __launch_bounds__(256) k_bbox_partial(uint32_t n, uint32_t d, const double* __restrict__ p,
+__global__ void __launch_bounds__(256) k_bbox_partial(int64_t n, uint32_t d, const double* __restrict__ p,
                                                       const int32_t* __restrict__ perm,
                                                       const int2* __restrict__ chunks,  // {start, end}
                                                       double* __restrict__ pmin, double* __restrict__ pmax) {
@@ -58,7 +64,8 @@ __global__ void __launch_bounds__(256) k_bbox_partial(uint32_t n, uint32_t d, co
 // and the split dimension (findSpan + ArgMax, src/kDTreeHelpers.cu:52-64)
 __global__ void k_bbox_final(uint32_t nseg, uint32_t d, const int32_t* __restrict__ segChunk,
                              const double* __restrict__ pmin, const double* __restrict__ pmax,
-                             double* __restrict__ bbmin, double* __restrict__ bbmax, int32_t* __restrict__ splitDim) {
+                             const int64_t* __restrict__ node, double* __restrict__ bbmin, double* __restrict__ bbmax,
+                             int32_t* __restrict__ splitDim) {
     uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= nseg) return;
     double mn[3] = {DBL_MAX, DBL_MAX, DBL_MAX}, mx[3] = {-DBL_MAX, -DBL_MAX, -DBL_MAX};
@@ -74,20 +81,21 @@ __global__ void k_bbox_final(uint32_t nseg, uint32_t d, const int32_t* __restric
         double sp = mx[j] - mn[j];
         if (sp > best) { best = sp; dim = (int)j; }
     }
-    for (uint32_t j = 0; j < d; ++j) { bbmin[(size_t)s * d + j] = mn[j]; bbmax[(size_t)s * d + j] = mx[j]; }
+    const int64_t o = node[s] * d;  // the segment's box slot
+    for (uint32_t j = 0; j < d; ++j) { bbmin[o + j] = mn[j]; bbmax[o + j] = mx[j]; }
     splitDim[s] = dim;
 }
 
 // ---- small segments: bitonic sort in LDS ---------------------------------
-__global__ void __launch_bounds__(256) k_sort_small(uint32_t n, const double* __restrict__ p,
-                                                    int32_t* __restrict__ perm, const int32_t* __restrict__ segs,
-                                                    const int32_t* __restrict__ offsets,
+__global__ void __launch_bounds__(256) k_sort_small(int64_t n, const double* __restrict__ p,
+                                                    int32_t* __restrict__ perm, const int4* __restrict__ segs,
                                                     const int32_t* __restrict__ splitDim) {
     __shared__ uint64_t key[kSmallSeg];
     __shared__ int32_t pos[kSmallSeg];
     __shared__ int32_t val[kSmallSeg];
-    int s = segs[blockIdx.x];
-    int a = offsets[s], e = offsets[s + 1];
+    const int4 sg = segs[blockIdx.x];  // {start, end, segment, -}
+    const int s = sg.z;
+    int a = sg.x, e = sg.y;
     int size = e - a;
     if (size <= 1) return;
     int P = 1;
@@ -127,7 +135,7 @@ __global__ void __launch_bounds__(256) k_sort_small(uint32_t n, const double* __
 }
 
 // ---- large segments: segmented LSD radix sort ------------------------------
-__global__ void __launch_bounds__(256) k_make_keys(uint32_t n, const double* __restrict__ p,
+__global__ void __launch_bounds__(256) k_make_keys(int64_t n, const double* __restrict__ p,
                                                    const int32_t* __restrict__ perm, const int4* __restrict__ chunks,
                                                    const int32_t* __restrict__ splitDim, uint64_t* __restrict__ keys) {
     int4 ch = chunks[blockIdx.x];  // {start, end, segment, -}
@@ -225,90 +233,172 @@ __global__ void __launch_bounds__(256) k_radix_scatter(const uint64_t* __restric
     }
 }
 
-void build_kdtree(Context& c, KDTree& t, const double* d_points) {
-    const uint32_t n = t.n, d = t.d, b = t.b;
-    HMX_CHECK(d >= 1 && d <= 3, HMX_EINVAL, "dimensionOfInputPoints must be 1, 2 or 3");
-    HMX_CHECK(b >= 1, HMX_EINVAL, "bucketSize must be >= 1");
-    HMX_CHECK(n >= 1, HMX_EINVAL, "numberOfInputPoints must be >= 1");
-    // FULL_TREE and DIVIDE_IN_HALF run the same construction in the reference
-    // (the shared branches of src/kDTreeConstruction.cu); POWER_OF_TWO_ON_LEFT
-    // cuts every level at multiples of a power-of-two size that starts at
-    // upperPowerOfTwo(n) and halves while it exceeds b (fillOffsets,
-    // src/kDTreeHelpers.cu:29-43)
-    const bool p2l = t.divMethod == HMX_POWER_OF_TWO_ON_LEFT;
-    uint32_t L, size = 0;
-    if (p2l) {
-        size = upper_pow2(n);
-        L = 0;
-        while (size > b) { size >>= 1; ++L; }
-        t.numSegments = (n + size - 1) / size;
-    } else {
-        L = kd_num_levels(n, b);
-        t.numSegments = 1u << L;
+// the forest's points side by side: Pc[j * ntot + base_k + i] = p_k[j * n_k + i]
+struct KdSrc {
+    int64_t base;
+    const double* p;
+    uint32_t n;
+};
+
+__global__ void __launch_bounds__(256) k_forest_gather(int64_t ntot, uint32_t d, const KdSrc* __restrict__ src, int ns,
+                                                       double* __restrict__ Pc) {
+    const int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (e >= ntot) return;
+    int lo = 0, hi = ns - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (src[mid].base <= e) lo = mid;
+        else hi = mid - 1;
     }
-    t.numLevels = L;
-    const size_t nbox = ((size_t)2 << L) - 1;
-    t.perm.alloc(n);
-    t.offsets.alloc(t.numSegments + 1);
-    t.bbmin.alloc(nbox * d);
-    t.bbmax.alloc(nbox * d);
-    DBuf<int32_t> splitDim(std::max<size_t>(1, t.numSegments));
+    const KdSrc q = src[lo];
+    const int64_t i = e - q.base;
+    for (uint32_t j = 0; j < d; ++j) Pc[(size_t)j * ntot + e] = q.p[(size_t)j * q.n + i];
+}
 
-    // identity permutation (initIndexMap, src/kDTreeHelpers.cu:12-26)
-    std::vector<int32_t> h_id(n);
-    for (uint32_t i = 0; i < n; ++i) h_id[i] = (int32_t)i;
-    HMX_HIP(hipMemcpyAsync(t.perm.p, h_id.data(), n * sizeof(int32_t), hipMemcpyHostToDevice, c.stream));
+// identity permutation over the forest (initIndexMap, src/kDTreeHelpers.cu:12-26)
+__global__ void __launch_bounds__(256) k_iota(int64_t n, int32_t* __restrict__ v) {
+    const int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (e < n) v[e] = (int32_t)e;
+}
 
-    DBuf<int32_t> permB(n);
-    DBuf<uint64_t> keyA(n), keyB(n);
-    std::vector<int32_t> off = {0, (int32_t)n};
-    DBuf<int32_t> d_off(t.numSegments + 1);
+// forest positions back to each cloud's own point numbers
+__global__ void __launch_bounds__(256) k_forest_localize(int64_t ntot, const KdSrc* __restrict__ src, int ns,
+                                                         int32_t* __restrict__ perm) {
+    const int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (e >= ntot) return;
+    int lo = 0, hi = ns - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (src[mid].base <= e) lo = mid;
+        else hi = mid - 1;
+    }
+    perm[e] -= (int32_t)src[lo].base;
+}
 
-    for (uint32_t level = 0; level < L; ++level) {
-        const uint32_t lsize = p2l ? size << (L - level) : 0;
-        if (p2l) {  // segments [s*lsize, min((s+1)*lsize, n))
-            const uint32_t ns = (n + lsize - 1) / lsize;
-            off.assign(ns + 1, 0);
-            for (uint32_t q = 0; q < ns; ++q) off[q] = (int32_t)(q * lsize);
-            off[ns] = (int32_t)n;
+// One cloud of a forest: its cuts at the current level, in its own positions
+struct KdJob {
+    KDTree* t;
+    const double* p;
+    int64_t base = 0;      // first position in the forest
+    int64_t nodeBase = 0;  // first box slot in the forest's box arrays
+    uint32_t L = 0, size = 0;
+    bool p2l = false;
+    std::vector<int32_t> off;
+};
+
+// cuts of a cloud at a level: FULL_TREE / DIVIDE_IN_HALF split every segment
+// at a + ceil((e-a)/2) (fillOffsetsSort, src/kDTreeHelpers.cu:81-93);
+// POWER_OF_TWO_ON_LEFT cuts at multiples of a power-of-two size that starts
+// at upperPowerOfTwo(n) and halves while it exceeds b (fillOffsets,
+// src/kDTreeHelpers.cu:29-43)
+static void p2l_cuts(uint32_t n, uint32_t lsize, std::vector<int32_t>& off) {
+    const uint32_t ns = (n + lsize - 1) / lsize;
+    off.assign(ns + 1, 0);
+    for (uint32_t q = 0; q < ns; ++q) off[q] = (int32_t)(q * lsize);
+    off[ns] = (int32_t)n;
+}
+
+static void kd_forest(Context& c, uint32_t d, std::vector<KdJob>& jobs) {
+    const int nj = (int)jobs.size();
+    int64_t ntot = 0, nodes = 0;
+    uint32_t maxL = 0;
+    for (KdJob& J : jobs) {
+        KDTree& t = *J.t;
+        HMX_CHECK(t.d == d, HMX_EINVAL, "a forest needs one dimension");
+        HMX_CHECK(t.d >= 1 && t.d <= 3, HMX_EINVAL, "dimensionOfInputPoints must be 1, 2 or 3");
+        HMX_CHECK(t.b >= 1, HMX_EINVAL, "bucketSize must be >= 1");
+        HMX_CHECK(t.n >= 1, HMX_EINVAL, "numberOfInputPoints must be >= 1");
+        HMX_CHECK(J.p != nullptr, HMX_EINVAL, "null points");
+        J.p2l = t.divMethod == HMX_POWER_OF_TWO_ON_LEFT;
+        if (J.p2l) {
+            J.size = upper_pow2(t.n);
+            J.L = 0;
+            while (J.size > t.b) { J.size >>= 1; ++J.L; }
+            t.numSegments = (t.n + J.size - 1) / J.size;
+        } else {
+            J.L = kd_num_levels(t.n, t.b);
+            t.numSegments = 1u << J.L;
         }
-        const uint32_t nseg = (uint32_t)off.size() - 1;
+        t.numLevels = J.L;
+        J.base = ntot;
+        J.nodeBase = nodes;
+        J.off = {0, (int32_t)t.n};
+        ntot += t.n;
+        nodes += ((int64_t)2 << J.L) - 1;
+        maxL = std::max(maxL, J.L);
+    }
+    HMX_CHECK(ntot < ((int64_t)1 << 31), HMX_EINVAL, "a forest holds fewer than 2^31 points");
+    // the points: one cloud is read where it lies; several are gathered side by side
+    std::vector<KdSrc> hsrc(nj);
+    for (int k = 0; k < nj; ++k) hsrc[k] = KdSrc{jobs[k].base, jobs[k].p, jobs[k].t->n};
+    DBuf<KdSrc> dsrc(nj);
+    HMX_HIP(hipMemcpyAsync(dsrc.p, hsrc.data(), nj * sizeof(KdSrc), hipMemcpyHostToDevice, c.stream));
+    DBuf<double> Pc;
+    const double* P = jobs[0].p;
+    if (nj > 1) {
+        Pc.alloc((size_t)ntot * d);
+        k_forest_gather<<<(unsigned)((ntot + 255) / 256), 256, 0, c.stream>>>(ntot, d, dsrc.p, nj, Pc.p);
+        HMX_LAUNCHED();
+        P = Pc.p;
+    }
+    auto permF = std::make_shared<DBuf<int32_t>>((size_t)ntot);
+    auto bbminF = std::make_shared<DBuf<double>>((size_t)nodes * d);
+    auto bbmaxF = std::make_shared<DBuf<double>>((size_t)nodes * d);
+    k_iota<<<(unsigned)((ntot + 255) / 256), 256, 0, c.stream>>>(ntot, permF->p);
+    HMX_LAUNCHED();
+    int32_t* perm = permF->p;
+    DBuf<int32_t> permB((size_t)ntot);
+    DBuf<uint64_t> keyA((size_t)ntot), keyB((size_t)ntot);
+
+    for (uint32_t level = 0; level < maxL; ++level) {
+        // ---- this level's segments of every cloud still splitting
+        std::vector<int2> segs;  // forest positions [a, e)
+        std::vector<int64_t> node;
+        for (KdJob& J : jobs) {
+            if (level >= J.L) continue;
+            if (J.p2l) p2l_cuts(J.t->n, J.size << (J.L - level), J.off);
+            const int64_t nb = J.nodeBase + ((int64_t)1 << level) - 1;
+            for (size_t s = 0; s + 1 < J.off.size(); ++s) {
+                segs.push_back(make_int2((int)(J.base + J.off[s]), (int)(J.base + J.off[s + 1])));
+                node.push_back(nb + (int64_t)s);
+            }
+        }
+        const uint32_t nseg = (uint32_t)segs.size();
         // ---- bounding boxes
         std::vector<int2> bchunks;
         std::vector<int32_t> segChunk(nseg + 1);
         for (uint32_t s = 0; s < nseg; ++s) {
             segChunk[s] = (int32_t)bchunks.size();
-            for (int32_t a = off[s]; a < off[s + 1]; a += kBoxChunk)
-                bchunks.push_back(make_int2(a, std::min(a + kBoxChunk, off[s + 1])));
+            for (int32_t a = segs[s].x; a < segs[s].y; a += kBoxChunk)
+                bchunks.push_back(make_int2(a, std::min(a + kBoxChunk, segs[s].y)));
         }
         segChunk[nseg] = (int32_t)bchunks.size();
         DBuf<int2> d_bch(std::max<size_t>(1, bchunks.size()));
-        DBuf<int32_t> d_sch(nseg + 1);
+        DBuf<int32_t> d_sch(nseg + 1), splitDim(std::max<uint32_t>(1, nseg));
+        DBuf<int64_t> d_node(std::max<uint32_t>(1, nseg));
         DBuf<double> pmin(std::max<size_t>(1, bchunks.size() * d)), pmax(std::max<size_t>(1, bchunks.size() * d));
         HMX_HIP(hipMemcpyAsync(d_bch.p, bchunks.data(), bchunks.size() * sizeof(int2), hipMemcpyHostToDevice,
                                c.stream));
         HMX_HIP(hipMemcpyAsync(d_sch.p, segChunk.data(), (nseg + 1) * sizeof(int32_t), hipMemcpyHostToDevice,
                                c.stream));
-        HMX_HIP(hipMemcpyAsync(d_off.p, off.data(), (nseg + 1) * sizeof(int32_t), hipMemcpyHostToDevice, c.stream));
+        HMX_HIP(hipMemcpyAsync(d_node.p, node.data(), nseg * sizeof(int64_t), hipMemcpyHostToDevice, c.stream));
         if (!bchunks.empty()) {
-            k_bbox_partial<<<(unsigned)bchunks.size(), 256, 0, c.stream>>>(n, d, d_points, t.perm.p, d_bch.p, pmin.p,
-                                                                            pmax.p);
+            k_bbox_partial<<<(unsigned)bchunks.size(), 256, 0, c.stream>>>(ntot, d, P, perm, d_bch.p, pmin.p, pmax.p);
             HMX_LAUNCHED();
         }
-        size_t base = ((size_t)1 << level) - 1;
-        k_bbox_final<<<(nseg + 255) / 256, 256, 0, c.stream>>>(nseg, d, d_sch.p, pmin.p, pmax.p,
-                                                                t.bbmin.p + base * d, t.bbmax.p + base * d, splitDim.p);
+        k_bbox_final<<<(nseg + 255) / 256, 256, 0, c.stream>>>(nseg, d, d_sch.p, pmin.p, pmax.p, d_node.p, bbminF->p,
+                                                                bbmaxF->p, splitDim.p);
         HMX_LAUNCHED();
 
         // ---- sort each segment by its split coordinate
-        std::vector<int32_t> small;
+        std::vector<int4> small;
         std::vector<int4> rchunks;
         std::vector<int4> rsegs;
         for (uint32_t s = 0; s < nseg; ++s) {
-            int32_t a = off[s], e = off[s + 1];
+            const int32_t a = segs[s].x, e = segs[s].y;
             if (e - a <= 1) continue;
             if (e - a <= kSmallSeg) {
-                small.push_back((int32_t)s);
+                small.push_back(make_int4(a, e, (int)s, 0));
             } else {
                 int4 si = make_int4(a, (int)rchunks.size(), 0, 0);
                 for (int32_t x = a; x < e; x += kRadixChunk)
@@ -325,11 +415,11 @@ void build_kdtree(Context& c, KDTree& t, const double* d_points) {
             HMX_HIP(hipMemcpyAsync(d_rseg.p, rsegs.data(), rsegs.size() * sizeof(int4), hipMemcpyHostToDevice,
                                    c.stream));
             const unsigned nch = (unsigned)rchunks.size();
-            k_make_keys<<<nch, 256, 0, c.stream>>>(n, d_points, t.perm.p, d_rch.p, splitDim.p, keyA.p);
+            k_make_keys<<<nch, 256, 0, c.stream>>>(ntot, P, perm, d_rch.p, splitDim.p, keyA.p);
             HMX_LAUNCHED();
             uint64_t* kin = keyA.p;
             uint64_t* kout = keyB.p;
-            int32_t* vin = t.perm.p;
+            int32_t* vin = perm;
             int32_t* vout = permB.p;
             for (int pass = 0; pass < 8; ++pass) {
                 int shift = pass * 8;
@@ -342,44 +432,85 @@ void build_kdtree(Context& c, KDTree& t, const double* d_points) {
                 std::swap(kin, kout);
                 std::swap(vin, vout);
             }
-            // 8 passes: data is back in (keyA, t.perm)
+            // 8 passes: data is back in (keyA, perm)
             HMX_HIP(hipStreamSynchronize(c.stream));
         }
         if (!small.empty()) {
-            DBuf<int32_t> d_small(small.size());
-            HMX_HIP(hipMemcpyAsync(d_small.p, small.data(), small.size() * sizeof(int32_t), hipMemcpyHostToDevice,
+            DBuf<int4> d_small(small.size());
+            HMX_HIP(hipMemcpyAsync(d_small.p, small.data(), small.size() * sizeof(int4), hipMemcpyHostToDevice,
                                    c.stream));
-            k_sort_small<<<(unsigned)small.size(), 256, 0, c.stream>>>(n, d_points, t.perm.p, d_small.p, d_off.p,
-                                                                       splitDim.p);
+            k_sort_small<<<(unsigned)small.size(), 256, 0, c.stream>>>(ntot, P, perm, d_small.p, splitDim.p);
             HMX_LAUNCHED();
-            HMX_HIP(hipStreamSynchronize(c.stream));
         }
-        if (p2l) {  // the next level's cuts come from the halved size
-            HMX_HIP(hipStreamSynchronize(c.stream));
-            continue;
+        // ---- next level's cuts
+        for (KdJob& J : jobs) {
+            if (level >= J.L || J.p2l) continue;
+            const size_t ns = J.off.size() - 1;
+            std::vector<int32_t> noff(2 * ns + 1);
+            for (size_t s = 0; s < ns; ++s) {
+                noff[2 * s] = J.off[s];
+                noff[2 * s + 1] = (J.off[s + 1] - J.off[s] + 1) / 2 + J.off[s];
+            }
+            noff[2 * ns] = (int32_t)J.t->n;
+            J.off.swap(noff);
         }
-        // ---- split (fillOffsetsSort, src/kDTreeHelpers.cu:81-93)
-        std::vector<int32_t> noff(2 * nseg + 1);
-        for (uint32_t s = 0; s < nseg; ++s) {
-            noff[2 * s] = off[s];
-            noff[2 * s + 1] = (off[s + 1] - off[s] + 1) / 2 + off[s];
-        }
-        noff[2 * nseg] = (int32_t)n;
-        off.swap(noff);
         HMX_HIP(hipStreamSynchronize(c.stream));  // per-level host vectors go out of scope
     }
-    if (p2l) {
-        off.assign(t.numSegments + 1, 0);
-        for (uint32_t q = 0; q < t.numSegments; ++q) off[q] = (int32_t)(q * size);
-        off[t.numSegments] = (int32_t)n;
+    if (nj > 1) {
+        k_forest_localize<<<(unsigned)((ntot + 255) / 256), 256, 0, c.stream>>>(ntot, dsrc.p, nj, perm);
+        HMX_LAUNCHED();
     }
-    HMX_HIP(hipMemcpyAsync(t.offsets.p, off.data(), off.size() * sizeof(int32_t), hipMemcpyHostToDevice, c.stream));
-    t.h_offsets = off;
-    t.h_bbmin.resize(nbox * d);
-    t.h_bbmax.resize(nbox * d);
-    HMX_HIP(hipMemcpyAsync(t.h_bbmin.data(), t.bbmin.p, nbox * d * sizeof(double), hipMemcpyDeviceToHost, c.stream));
-    HMX_HIP(hipMemcpyAsync(t.h_bbmax.data(), t.bbmax.p, nbox * d * sizeof(double), hipMemcpyDeviceToHost, c.stream));
+    // leaf offsets of every tree in one block; boxes copied to the host once
+    std::vector<int32_t> allOff;
+    std::vector<int64_t> offBase(nj);
+    for (int k = 0; k < nj; ++k) {
+        KdJob& J = jobs[k];
+        if (J.p2l) p2l_cuts(J.t->n, J.size, J.off);
+        offBase[k] = (int64_t)allOff.size();
+        allOff.insert(allOff.end(), J.off.begin(), J.off.end());
+    }
+    auto offF = std::make_shared<DBuf<int32_t>>(allOff.size());
+    HMX_HIP(hipMemcpyAsync(offF->p, allOff.data(), allOff.size() * sizeof(int32_t), hipMemcpyHostToDevice, c.stream));
+    std::vector<double> hmin((size_t)nodes * d), hmax((size_t)nodes * d);
+    HMX_HIP(hipMemcpyAsync(hmin.data(), bbminF->p, hmin.size() * sizeof(double), hipMemcpyDeviceToHost, c.stream));
+    HMX_HIP(hipMemcpyAsync(hmax.data(), bbmaxF->p, hmax.size() * sizeof(double), hipMemcpyDeviceToHost, c.stream));
     HMX_HIP(hipStreamSynchronize(c.stream));
+    for (int k = 0; k < nj; ++k) {
+        KdJob& J = jobs[k];
+        KDTree& t = *J.t;
+        const size_t nbox = ((size_t)2 << J.L) - 1;
+        t.perm.view(permF, (size_t)J.base, t.n);
+        t.offsets.view(offF, (size_t)offBase[k], J.off.size());
+        t.bbmin.view(bbminF, (size_t)J.nodeBase * d, nbox * d);
+        t.bbmax.view(bbmaxF, (size_t)J.nodeBase * d, nbox * d);
+        t.h_offsets = J.off;
+        t.h_bbmin.assign(hmin.begin() + J.nodeBase * d, hmin.begin() + (J.nodeBase + (int64_t)nbox) * d);
+        t.h_bbmax.assign(hmax.begin() + J.nodeBase * d, hmax.begin() + (J.nodeBase + (int64_t)nbox) * d);
+    }
+}
+
+void build_kdtree(Context& c, KDTree& t, const double* d_points) {
+    HMX_CHECK(t.d >= 1 && t.d <= 3, HMX_EINVAL, "dimensionOfInputPoints must be 1, 2 or 3");
+    std::vector<KdJob> jobs(1);
+    jobs[0].t = &t;
+    jobs[0].p = d_points;
+    kd_forest(c, t.d, jobs);
+}
+
+// Trees of many clouds: one forest per point dimension
+void build_kdtree_batch(Context& c, const std::vector<KDTree*>& ts, const std::vector<const double*>& pts) {
+    HMX_CHECK(ts.size() == pts.size() && !ts.empty(), HMX_EINVAL, "batch arrays of different lengths");
+    for (uint32_t d = 1; d <= 3; ++d) {
+        std::vector<KdJob> jobs;
+        for (size_t k = 0; k < ts.size(); ++k) {
+            HMX_CHECK(ts[k]->d >= 1 && ts[k]->d <= 3, HMX_EINVAL, "dimensionOfInputPoints must be 1, 2 or 3");
+            if (ts[k]->d != d) continue;
+            jobs.emplace_back();
+            jobs.back().t = ts[k];
+            jobs.back().p = pts[k];
+        }
+        if (!jobs.empty()) kd_forest(c, d, jobs);
+    }
 }
 
 }  // namespace hmx

@@ -186,7 +186,10 @@ template <int BL, int K>
 __global__ void __launch_bounds__(256) k_mv_phase2(const int32_t* __restrict__ leafPtr,
                                                    const MvEntry* __restrict__ entries,
                                                    const double* __restrict__ diag, const double* __restrict__ z,
-                                                   const double* __restrict__ X, int ldx, double* __restrict__ Y) {
+                                                   const double* __restrict__ X, int ldx, double* __restrict__ Y,
+                                                   const int32_t* __restrict__ nearPtr,
+                                                   const int32_t* __restrict__ nearCol,
+                                                   const double* __restrict__ nearD) {
     constexpr int RPL = BL / 16;
     static_assert(RPL == 2 || RPL == 4, "bucketSize 32 or 64");
     __shared__ double red[4][BL * K];
@@ -198,19 +201,23 @@ __global__ void __launch_bounds__(256) k_mv_phase2(const int32_t* __restrict__ l
     for (int q = 0; q < RPL; ++q)
 #pragma unroll
         for (int s = 0; s < K; ++s) acc[q][s] = 0.0;
-    // dense diagonal block (column-major): columns c = cg, cg+16, ...
-    const double* D = diag + (size_t)leaf * BL * BL;
-    const double* xl = X + (size_t)leaf * BL;
+    // dense blocks (column-major): the diagonal block, then the leaf's
+    // near-field blocks; columns c = cg, cg+16, ... of each
+    const int q0 = nearPtr ? nearPtr[leaf] : 0, q1 = nearPtr ? nearPtr[leaf + 1] : 0;
+    for (int q = q0 - 1; q < q1; ++q) {
+        const double* D = q < q0 ? diag + (size_t)leaf * BL * BL : nearD + (size_t)q * BL * BL;
+        const double* xl = X + (size_t)(q < q0 ? leaf : nearCol[q]) * BL;
 #pragma unroll
-    for (int cc = 0; cc < BL / 16; ++cc) {
-        const int c = cg + 16 * cc;
-        double dv[RPL];
-        vload_d<RPL>(D + c * BL + r0, dv);
+        for (int cc = 0; cc < BL / 16; ++cc) {
+            const int c = cg + 16 * cc;
+            double dv[RPL];
+            vload_d<RPL>(D + c * BL + r0, dv);
 #pragma unroll
-        for (int s = 0; s < K; ++s) {
-            const double xv = xl[(size_t)s * ldx + c];
+            for (int s = 0; s < K; ++s) {
+                const double xv = xl[(size_t)s * ldx + c];
 #pragma unroll
-            for (int q = 0; q < RPL; ++q) acc[q][s] = fma(dv[q], xv, acc[q][s]);
+                for (int q2 = 0; q2 < RPL; ++q2) acc[q2][s] = fma(dv[q2], xv, acc[q2][s]);
+            }
         }
     }
     const int e0 = leafPtr[leaf], e1 = leafPtr[leaf + 1];
@@ -336,6 +343,19 @@ __global__ void __launch_bounds__(256) k_transpose_blocks(int S, int b, const do
     DT[leaf * per + (int64_t)q * b + c] = D[e];
 }
 
+// dense near-field blocks, column-major: out[q][col*b + row] = sum_a U[a*b + row] V[a*b + col]
+__global__ void __launch_bounds__(256) k_near_dense_cm(int b, const NearRef* __restrict__ src,
+                                                       double* __restrict__ out) {
+    const NearRef t = src[blockIdx.x];
+    double* o = out + (size_t)blockIdx.x * b * b;
+    for (int e = threadIdx.x; e < b * b; e += 256) {
+        const int col = e / b, row = e % b;
+        double v = 0.0;
+        for (int a = 0; a < t.r; ++a) v = fma(t.U[(size_t)a * b + row], t.V[(size_t)a * b + col], v);
+        o[e] = v;
+    }
+}
+
 // vectors (ldx apart) <-> the padded space (npad apart), K right-hand sides
 __global__ void __launch_bounds__(256) k_gather(int64_t npad, int K, const int32_t* __restrict__ gmap,
                                                 const double* __restrict__ X, int64_t ldx, double* __restrict__ Xp) {
@@ -358,11 +378,12 @@ __global__ void __launch_bounds__(256) k_scatter(int64_t npad, int K, const int3
 // Work lists of one H-matrix: phase-1 items over the input-side factors,
 // partial-slot bookkeeping, phase-2 entries per output leaf.  Positions are
 // in the padded space (leaf*b + i).
-PlanHost plan_host(const HMatrix& h, bool transpose, int chunk, int rowBlock) {
+PlanHost plan_host(const HMatrix& h, bool transpose, int chunk, int rowBlock, bool denseNear) {
     const int L = h.numLevels, b = (int)h.b, S = (int)h.S;
     HMX_CHECK(b == 32 || b == 64, HMX_EUNSUPPORTED, "HMatrixVecMult kernels are built for bucketSize 32 or 64");
     HMX_CHECK(chunk >= 128 && (chunk & (chunk - 1)) == 0, HMX_EINVAL, "phase-1 chunk must be a power of two >= 128");
     PlanHost H;
+    H.b = b;
     std::vector<std::vector<MvEntry>> leafEntries(S);
     const int nblk = rowBlock > 0 ? (int)((int64_t)S * b / rowBlock) : 0;
     std::vector<std::vector<MvEntry>> blkEntries(nblk);
@@ -379,6 +400,11 @@ PlanHost plan_host(const HMatrix& h, bool transpose, int chunk, int rowBlock) {
             const uint32_t inCl = transpose ? v : u, outCl = transpose ? u : v;
             const double* Pin = (transpose ? lv.U.p : lv.V.p) + (size_t)lv.h_offsets[t] * m;
             const double* Pout = (transpose ? lv.V.p : lv.U.p) + (size_t)lv.h_offsets[t] * m;
+            if (denseNear && m == b && 2 * r > b) {  // fewer bytes as a dense block, and no z
+                H.nears.push_back(NearRef{Pout, Pin, r, (int32_t)outCl, (int32_t)inCl, 0});
+                continue;
+            }
+            H.lowRankBytes += 16ull * (uint64_t)r * (uint64_t)m;
             const int nch = m >= chunk ? m / chunk : 1;
             for (int j = 0; j < r; ++j) {
                 H.colPart.push_back(slot + (int64_t)j * nch);
@@ -451,6 +477,27 @@ void upload_plan(Context& c, MatvecPlan& P, const PlanHost& H) {
         put(c, P.blkPtr, H.blkPtr);
         put(c, P.blkEntries, H.blkEntries);
     }
+    // dense near-field blocks, grouped by output leaf
+    const int S = P.numLeaves, b = H.b;
+    std::vector<NearRef> nears = H.nears;
+    std::stable_sort(nears.begin(), nears.end(), [](const NearRef& x, const NearRef& y) { return x.outLeaf < y.outLeaf; });
+    std::vector<int32_t> ptr(S + 1, 0), col(nears.size());
+    for (const NearRef& nr : nears) ptr[nr.outLeaf + 1]++;
+    for (int l = 0; l < S; ++l) ptr[l + 1] += ptr[l];
+    for (size_t q = 0; q < nears.size(); ++q) col[q] = nears[q].inLeaf;
+    P.numNear = (int64_t)nears.size();
+    put(c, P.nearPtr, ptr);
+    put(c, P.nearCol, col);
+    P.nearD.alloc((size_t)std::max<int64_t>(1, P.numNear) * b * b);
+    if (P.numNear) {
+        DBuf<NearRef> d_src;
+        put(c, d_src, nears);
+        k_near_dense_cm<<<(unsigned)P.numNear, 256, 0, c.stream>>>(b, d_src.p, P.nearD.p);
+        HMX_LAUNCHED();
+        HMX_HIP(hipStreamSynchronize(c.stream));
+    }
+    P.bytes = H.lowRankBytes + 8ull * (uint64_t)(S + P.numNear) * b * b;
+    P.lowRankBytes = H.lowRankBytes;
     P.wsK = 0;  // workspaces are sized for the previous plan
     HMX_HIP(hipStreamSynchronize(c.stream));
 }
@@ -475,11 +522,13 @@ int row_block(const HMatrix& h) {
     return h.b == 64 && npad >= kRowBlock && npad % kRowBlock == 0 ? kRowBlock : 0;
 }
 
-void build_matvec_plan(Context& c, HMatrix& h, bool transpose) {
-    MatvecPlan& P = transpose ? h.planT : h.plan;
+// denseNear: leaf-level tiles of rank above b/2 as dense blocks (the plans
+// of the products); off for the factor-gradient plans, which need z of
+// every tile
+void build_matvec_plan(Context& c, HMatrix& h, bool transpose, MatvecPlan& P, bool denseNear) {
     const int b = (int)h.b, S = (int)h.S;
     HMX_CHECK(h.leafOffsets.size() == (size_t)S + 1, HMX_ESTATE, "H-matrix without leaf offsets");
-    upload_plan(c, P, plan_host(h, transpose, P.chunk, row_block(h)));
+    upload_plan(c, P, plan_host(h, transpose, P.chunk, row_block(h), denseNear));
     P.n = h.n;
     P.npad = (int64_t)S * b;
     P.gather = h.n != (uint32_t)P.npad;
@@ -516,12 +565,13 @@ static void run_phases(Context& c, const MatvecPlan& P, const MvWork& W, int b, 
         HMX_LAUNCHED();
     }
     if (ev) HMX_HIP(hipEventRecord(ev[2], c.stream));
+    const int32_t* near = P.numNear ? P.nearPtr.p : nullptr;
     if (b == 64)
         k_mv_phase2<64, K><<<(unsigned)P.numLeaves, 256, 0, c.stream>>>(P.leafPtr.p, P.entries.p, P.diag, W.z, X,
-                                                                         (int)ldx, Y);
+                                                                         (int)ldx, Y, near, P.nearCol.p, P.nearD.p);
     else
         k_mv_phase2<32, K><<<(unsigned)P.numLeaves, 256, 0, c.stream>>>(P.leafPtr.p, P.entries.p, P.diag, W.z, X,
-                                                                         (int)ldx, Y);
+                                                                         (int)ldx, Y, near, P.nearCol.p, P.nearD.p);
     HMX_LAUNCHED();
     if (P.numBlocks > 0) {  // after the per-leaf pass: it adds to Y
         HMX_CHECK(P.rowBlock == kRowBlock, HMX_ESTATE, "matvec plan with an unknown row block");
@@ -609,7 +659,7 @@ void hmatvec(Context& c, HMatrix& h, uint32_t k, const double* d_in, double* d_o
     HMX_CHECK(d_in && d_out, HMX_EINVAL, "null vector pointer");
     HMX_CHECK(d_in != d_out, HMX_EINVAL, "in and out must not alias");
     MatvecPlan& P = transpose ? h.planT : h.plan;
-    if (!P.built) build_matvec_plan(c, h, transpose);
+    if (!P.built) build_matvec_plan(c, h, transpose, P, h.planDenseNear != 0);
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
     const bool prof = ms != nullptr;
     if (prof)
@@ -644,13 +694,14 @@ void build_batch_plan(Context& c, const std::vector<HMatrix*>& hs, int chunk, Ma
     HMX_CHECK(!hs.empty(), HMX_EINVAL, "empty batch");
     const int b = (int)hs[0]->b;
     PlanHost B;
+    B.b = b;
     B.leafPtr.push_back(0);
     std::vector<int32_t> gm;
     int64_t padBase = 0, vecBase = 0, colBase = 0, slotBase = 0, leafBase = 0;
     for (HMatrix* h : hs) {
         HMX_CHECK(h->b == (uint32_t)b, HMX_EINVAL, "all instances of a batch need the same bucketSize");
         HMX_CHECK(h->leafOffsets.size() == (size_t)h->S + 1, HMX_ESTATE, "H-matrix without leaf offsets");
-        PlanHost H = plan_host(*h, false, chunk);
+        PlanHost H = plan_host(*h, false, chunk, 0, h->planDenseNear != 0);
         for (MvItem it : H.items) {
             it.xoff = (int32_t)(it.xoff + padBase);
             it.out = (int32_t)(it.out + slotBase);
@@ -664,6 +715,12 @@ void build_batch_plan(Context& c, const std::vector<HMatrix*>& hs, int chunk, Ma
             B.entries.push_back(en);
         }
         for (size_t q = 1; q < H.leafPtr.size(); ++q) B.leafPtr.push_back(e0 + H.leafPtr[q]);
+        for (NearRef nr : H.nears) {
+            nr.outLeaf += (int32_t)leafBase;
+            nr.inLeaf += (int32_t)leafBase;
+            B.nears.push_back(nr);
+        }
+        B.lowRankBytes += H.lowRankBytes;
         std::vector<int32_t> g = padding_map(*h, vecBase);
         gm.insert(gm.end(), g.begin(), g.end());
         padBase += (int64_t)h->S * b;

@@ -26,6 +26,7 @@
 #include <cmath>
 #include <cstdlib>
 #include <numeric>
+#include <type_traits>
 
 #include "hmx_device.h"
 #include "hmx_internal.h"
@@ -174,6 +175,36 @@ __global__ void __launch_bounds__(256) k_mf_phase1(const MfItem* __restrict__ it
             for (int nb = 0; nb < NB; ++nb)
 #pragma unroll
                 for (int q = 0; q < VL; ++q) bx[nb][q] = bn[nb][q];
+        }
+    }
+    if constexpr (sizeof(T) < 8) {
+        // FP32 / FP16 groups: lane (lg, lc) holds columns 16g + 4lg + v (v < 4)
+        // for right-hand side 16nb + lc.  Their tiles' columns are padded to
+        // a multiple of 4 in both orders, so the four belong to one tile and
+        // sit at four consecutive z columns from a multiple of 4: one 8- or
+        // 16-byte store per (g, nb), and a wave's stores fill whole 512-byte
+        // stretches of z.
+        if (it.direct) {
+            typedef T tv4 __attribute__((ext_vector_type(4)));
+#pragma unroll
+            for (int g = 0; g < CG; ++g) {
+                const int col0 = 16 * g + 4 * lg;
+                if (col0 >= it.ncols) continue;
+                const int64_t zc = zmap[it.zidx + col0];
+                ACC cs[4] = {(ACC)1, (ACC)1, (ACC)1, (ACC)1};
+                if (colScaleV) {
+                    const float4 c4 = *reinterpret_cast<const float4*>(colScaleV + it.zidx + col0);
+                    cs[0] = (ACC)c4.x; cs[1] = (ACC)c4.y; cs[2] = (ACC)c4.z; cs[3] = (ACC)c4.w;
+                }
+#pragma unroll
+                for (int nb = 0; nb < NB; ++nb) {
+                    tv4 o;
+#pragma unroll
+                    for (int v = 0; v < 4; ++v) o[v] = (T)(acc[g][nb][v] * cs[v]);
+                    *reinterpret_cast<tv4*>(Zt + zt_index<VL, NR>(zc, 16 * nb + lc)) = o;
+                }
+            }
+            return;
         }
     }
     if (it.direct) {  // whole columns: scaled, converted, stored transposed ([rhs][column])
@@ -371,6 +402,87 @@ __device__ inline void dense_rows(int leaf, const TD* __restrict__ Drm, const in
     }
 }
 
+// Mixed plans with the points at hand: one wave's leaf times the kernel
+// blocks K(x_leaf, x_j) of its diagonal (j = leaf) and near-field column
+// leaves j, evaluated on the fly instead of read.  Points are held in FP32
+// relative to their leaf's first point (PL), with those origins in FP64 (O):
+// x_r - x_c = (x_r - o_i) + (o_i - o_j) - (x_c - o_j), where o_i - o_j is
+// formed in FP64 once per block and every term is small, so the differences
+// carry errors near 1e-8 absolute; sqrt and exp in FP32
+// (exp(-r/l) = 2^(r * nlog2e)).  K is split into
+// FP16 K_hi + K_lo and the right-hand sides (in units of their max-abs) into
+// X_hi + X_lo, so three FP16 16x16x16 products per step (K_hi X_hi + K_hi X_lo
+// + K_lo X_hi) carry about 22 bits; the K_lo X_lo term is below 2^-22 of the
+// product.  Lane (lg, lc) evaluates K at rows 16rg + lc and columns
+// 16ks + 4lg + e, e < 4: the A operand layout of the 16x16x16 shape.  Padding
+// positions hold the point (0, 0, 0): finite kernel values that meet zero
+// right-hand-side entries or land in discarded rows.
+template <int BL, int NB>
+__device__ inline void near_otf(int leaf, const float4* __restrict__ PL, const double4* __restrict__ O, float nlog2e,
+                                const int32_t* __restrict__ nearPtr, const MfNear* __restrict__ nearB,
+                                const _Float16* __restrict__ Xh, const _Float16* __restrict__ Xl, int lg, int lc,
+                                int kvalid, v4f (&acc)[BL / 16][NB]) {
+    constexpr int RG = BL / 16, NR = 16 * NB, SH = BL == 64 ? 6 : 5;
+    float xl[RG][3];
+#pragma unroll
+    for (int rg = 0; rg < RG; ++rg) {
+        const float4 p = PL[(size_t)leaf * BL + 16 * rg + lc];
+        xl[rg][0] = p.x; xl[rg][1] = p.y; xl[rg][2] = p.z;
+    }
+    const double4 oi = O[leaf];
+    const int n0 = nearPtr[leaf], n1 = nearPtr[leaf + 1];
+    for (int q = n0 - 1; q < n1; ++q) {
+        const int cl = q < n0 ? leaf : nearB[q].colLeaf;
+        const double4 oj = O[cl];
+        const float d0 = (float)(oi.x - oj.x), d1 = (float)(oi.y - oj.y), d2 = (float)(oi.z - oj.z);
+        float xr[RG][3];
+#pragma unroll
+        for (int rg = 0; rg < RG; ++rg) {
+            xr[rg][0] = xl[rg][0] + d0; xr[rg][1] = xl[rg][1] + d1; xr[rg][2] = xl[rg][2] + d2;
+        }
+        for (int ks = 0; ks < BL / 16; ++ks) {
+            const int c0 = cl * BL + 16 * ks + 4 * lg;  // this lane's 4 columns (padded positions)
+            float xc[4][3];
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                const float4 p = PL[c0 + e];
+                xc[e][0] = p.x; xc[e][1] = p.y; xc[e][2] = p.z;
+            }
+            v4h bh[NB], bl[NB];
+#pragma unroll
+            for (int nb = 0; nb < NB; ++nb) {
+                if (16 * nb + lc < kvalid) {
+                    const size_t o = xb_index<NR>((int64_t)c0, 16 * nb + lc, SH);
+                    bh[nb] = *reinterpret_cast<const v4h*>(Xh + o);
+                    bl[nb] = *reinterpret_cast<const v4h*>(Xl + o);
+                } else {
+                    bh[nb] = v4h{0, 0, 0, 0};
+                    bl[nb] = v4h{0, 0, 0, 0};
+                }
+            }
+#pragma unroll
+            for (int rg = 0; rg < RG; ++rg) {
+                v4h ah, al;
+#pragma unroll
+                for (int e = 0; e < 4; ++e) {
+                    const float dx = xr[rg][0] - xc[e][0], dy = xr[rg][1] - xc[e][1], dz = xr[rg][2] - xc[e][2];
+                    const float r2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
+                    const float k = exp2f(nlog2e * sqrtf(r2));
+                    const _Float16 hi = (_Float16)k;
+                    ah[e] = hi;
+                    al[e] = (_Float16)(k - (float)hi);
+                }
+#pragma unroll
+                for (int nb = 0; nb < NB; ++nb) {
+                    acc[rg][nb] = __builtin_amdgcn_mfma_f32_16x16x16f16(ah, bh[nb], acc[rg][nb], 0, 0, 0);
+                    acc[rg][nb] = __builtin_amdgcn_mfma_f32_16x16x16f16(ah, bl[nb], acc[rg][nb], 0, 0, 0);
+                    acc[rg][nb] = __builtin_amdgcn_mfma_f32_16x16x16f16(al, bh[nb], acc[rg][nb], 0, 0, 0);
+                }
+            }
+        }
+    }
+}
+
 // One workgroup of 4 waves per group of 4 consecutive leaves (S leaves in
 // all); wave w owns leaf 4g+w: its dense blocks and its entries (ptr, per
 // leaf), then -- grouped plans (64-row leaves) -- the tiles spanning the
@@ -378,8 +490,11 @@ __device__ inline void dense_rows(int leaf, const TD* __restrict__ Drm, const in
 // shared list in the same order, so a tile's z block comes from HBM once per
 // group and reaches the other waves from the caches.  Mixed plans: the FP16
 // group (B) goes first into the cleared accumulators, which are then
-// multiplied by loScale[rhs] * loMul (the right-hand-side scales and the
-// plan's z exponent) before the dense blocks and the FP32 group add theirs.
+// multiplied by loMul (2^-E, the plan's z exponent); with the points at hand
+// (P) the diagonal and near-field blocks are evaluated on the fly and added in
+// the same units (right-hand sides divided by their max-abs); then the sum is
+// multiplied by loScale[rhs] (those max-abs values) before the stored dense
+// blocks (without P) and the FP32 group add theirs.
 template <int BL, bool UCM, class TD, class TA, class TB, int NB, class TO>
 __global__ void __launch_bounds__(256) k_mf_phase2g(int S, const int32_t* __restrict__ ptrA, const MfEntry* __restrict__ entA,
                                                     const int32_t* __restrict__ gptrA,
@@ -390,7 +505,11 @@ __global__ void __launch_bounds__(256) k_mf_phase2g(int S, const int32_t* __rest
                                                     const TD* __restrict__ Drm, const int32_t* __restrict__ nearPtr,
                                                     const MfNear* __restrict__ nearB, const TD* __restrict__ Xb,
                                                     TO* __restrict__ Y, int ldy, int kvalid,
-                                                    const float* __restrict__ loScale, float loMul) {
+                                                    const float* __restrict__ loScale, float loMul,
+                                                    const float4* __restrict__ P, const double4* __restrict__ O,
+                                                    float nlog2e,
+                                                    const _Float16* __restrict__ Xh,
+                                                    const _Float16* __restrict__ Xl) {
     using MA = Mfma<TA>;
     using accv = typename MA::accv;
     static_assert(sizeof(typename Mfma<TB>::acc_t) == sizeof(typename MA::acc_t) &&
@@ -415,14 +534,23 @@ __global__ void __launch_bounds__(256) k_mf_phase2g(int S, const int32_t* __rest
         lr_rows<BL, false, TB, NB>(entB, ptrB[leaf], ptrB[leaf + 1], ZB, 0, lg, lc, kvalid, acc);
         if (gptrB) lr_rows<BL, false, TB, NB>(gentB, gptrB[grp], gptrB[grp + 1], ZB, BL * w, lg, lc, kvalid, acc);
 #pragma unroll
+        for (int rg = 0; rg < RG; ++rg)
+#pragma unroll
+            for (int nb = 0; nb < NB; ++nb) acc[rg][nb] *= (typename MA::acc_t)loMul;
+    }
+    if constexpr (std::is_same<TB, _Float16>::value && std::is_same<typename MA::accv, v4f>::value) {
+        if (P) near_otf<BL, NB>(leaf, P, O, nlog2e, nearPtr, nearB, Xh, Xl, lg, lc, kvalid, acc);
+    }
+    if (ptrB) {
+#pragma unroll
         for (int nb = 0; nb < NB; ++nb) {
-            const typename MA::acc_t f = 16 * nb + lc < kvalid ? (typename MA::acc_t)(loScale[16 * nb + lc] * loMul)
-                                                                : (typename MA::acc_t)0;
+            const typename MA::acc_t f =
+                16 * nb + lc < kvalid ? (typename MA::acc_t)loScale[16 * nb + lc] : (typename MA::acc_t)0;
 #pragma unroll
             for (int rg = 0; rg < RG; ++rg) acc[rg][nb] *= f;
         }
     }
-    dense_rows<BL, TD, NB>(leaf, Drm, nearPtr, nearB, Xb, lg, lc, kvalid, acc);
+    if (!P) dense_rows<BL, TD, NB>(leaf, Drm, nearPtr, nearB, Xb, lg, lc, kvalid, acc);
     lr_rows<BL, UCM, TA, NB>(entA, ptrA[leaf], ptrA[leaf + 1], ZA, 0, lg, lc, kvalid, acc);
     if (gptrA) lr_rows<BL, UCM, TA, NB>(gentA, gptrA[grp], gptrA[grp + 1], ZA, BL * w, lg, lc, kvalid, acc);
 #pragma unroll
@@ -486,6 +614,41 @@ __global__ void __launch_bounds__(256) k_x_blocked(int64_t total, int64_t npad, 
     Xb[o] = v;
 }
 
+// Points of the padded leaf space (x, y, z, w as doubles) relative to their
+// leaf's first position: PL[p] = (x - o, y - o, z - o) in FP32, O[leaf] = o in FP64
+__global__ void __launch_bounds__(256) k_local_points(int64_t npad, int b, const double4* __restrict__ P,
+                                                      float4* __restrict__ PL, double4* __restrict__ O) {
+    const int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (e >= npad) return;
+    const int64_t leaf = e / b;
+    const double4 o = P[leaf * b];
+    const double4 p = P[e];
+    PL[e] = make_float4((float)(p.x - o.x), (float)(p.y - o.y), (float)(p.z - o.z), 0.f);
+    if (e % b == 0) O[leaf] = o;
+}
+
+// The right-hand sides divided by their scale sx, in the blocked layout, as
+// FP16 hi = fp16(x/sx) and lo = fp16(x/sx - hi) (zero on padding and past kvalid)
+__global__ void __launch_bounds__(256) k_x_blocked_split(int64_t total, int64_t npad, int NR, int sh,
+                                                         const float* __restrict__ X, int64_t ldx, int kvalid,
+                                                         const int32_t* __restrict__ gmap,
+                                                         const float* __restrict__ sx, _Float16* __restrict__ Xh,
+                                                         _Float16* __restrict__ Xl) {
+    const int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (e >= total) return;
+    const int s = (int)(e / npad);
+    const int64_t p = e % npad;
+    const size_t o = ((((size_t)(p >> sh)) * NR + (size_t)s) << sh) + (size_t)(p & ((1 << sh) - 1));
+    float v = 0.f;
+    if (s < kvalid) {
+        const int64_t g = gmap ? (int64_t)gmap[p] : p;
+        if (g >= 0) v = X[s * ldx + g] / sx[s];
+    }
+    const _Float16 hi = (_Float16)v;
+    Xh[o] = hi;
+    Xl[o] = (_Float16)(v - (float)hi);
+}
+
 // product in the padded space (Yp, ld npad) back to the caller's entries
 template <class T>
 __global__ void __launch_bounds__(256) k_y_scatter(int64_t npad, int k, const int32_t* __restrict__ gmap,
@@ -542,7 +705,7 @@ __global__ void __launch_bounds__(256) k_cvt_cols(int64_t n, const CvCol* __rest
     const int lane = threadIdx.x & 63;
     const CvCol d = cols[c];
     const double inv = 1.0 / (double)d.scale;
-    for (int i = lane; i < d.m; i += 64) V[d.dst + i] = (T)(d.src[i] * inv);
+    for (int i = lane; i < d.m; i += 64) V[d.dst + i] = d.src ? (T)(d.src[i] * inv) : (T)0;
 }
 
 // row-major U copy with zero padding: one block per (tile, 32-row tile, 32-col tile)
@@ -562,7 +725,7 @@ __global__ void __launch_bounds__(256) k_cvt_rowmajor(int64_t n, const TrTile* _
         double v = 0.0;
         if (c < t.cnt && r < t.m) {
             const CvCol d = cols[t.colDesc + c];
-            v = d.src[r] / (double)d.scale;
+            v = d.src ? d.src[r] / (double)d.scale : 0.0;
         }
         sh[cc][tx] = v;
     }
@@ -660,6 +823,11 @@ void build_group(Context& c, const HMatrix& h, int S, bool grouped, MfGroup& G, 
     G.type = type;
     const int T = (int)tiles.size();
     auto cnt_of = [&](int t) { return (int)tiles[t].cols.size(); };
+    // FP32 / FP16 copies: every tile's columns padded to a multiple of 4 (zero
+    // columns of U and V, zero scales) in both orders, for phase 1's vector
+    // stores of z
+    const bool pad4 = !vdirect && type != 0;
+    auto cnt_pad = [&](int t) { return pad4 ? (cnt_of(t) + 3) / 4 * 4 : cnt_of(t); };
     auto scales = [&](int t, int q, float& fu, float& fv) {
         fu = fv = 1.f;
         if (su) {
@@ -723,7 +891,7 @@ void build_group(Context& c, const HMatrix& h, int S, bool grouped, MfGroup& G, 
                     ++j;
             const int m = R0.m;
             int cntRun = 0;
-            for (size_t k = i; k < j; ++k) cntRun += cnt_of(ord[k]);
+            for (size_t k = i; k < j; ++k) cntRun += cnt_pad(ord[k]);
             const int rpad = (cntRun + kPad - 1) / kPad * kPad;
             const int64_t cd0 = (int64_t)ucols.size();
             for (size_t k = i; k < j; ++k) {
@@ -736,6 +904,10 @@ void build_group(Context& c, const HMatrix& h, int S, bool grouped, MfGroup& G, 
                     scales(t, q, fu, fv);
                     ucols.push_back(CvCol{Usrc + (size_t)tiles[t].cols[q] * m, 0, m, fu});
                     colScale.push_back(fu * fv * zmul);
+                }
+                for (int q = cnt_of(t); q < cnt_pad(t); ++q) {
+                    ucols.push_back(CvCol{nullptr, 0, m, 1.f});
+                    colScale.push_back(0.f);
                 }
             }
             for (int q = cntRun; q < rpad; ++q) colScale.push_back(0.f);
@@ -810,7 +982,12 @@ void build_group(Context& c, const HMatrix& h, int S, bool grouped, MfGroup& G, 
                     zmap.push_back((int32_t)(zb[t] + q));
                     vElems += m;
                 }
-                fbytes += 2ull * (uint64_t)cnt_of(t) * m * es;
+                for (int q = cnt_of(t); q < cnt_pad(t); ++q) {
+                    vcols.push_back(CvCol{nullptr, vElems, m, 1.f});
+                    zmap.push_back((int32_t)(zb[t] + q));
+                    vElems += m;
+                }
+                fbytes += 2ull * (uint64_t)cnt_pad(t) * m * es;
             }
             if (vdirect)
                 emit_items(k0, (int64_t)zmap.size(), m, col_start(R0),
@@ -939,6 +1116,26 @@ void build_near(Context& c, const HMatrix& h, int S, MfPlan& P, const std::vecto
     std::vector<MfNear> list;
     for (int64_t q = 0; q < P.numNear; ++q)
         list.push_back(MfNear{(const void*)((const T*)P.nearStore.p + q * b * b), colLeaf[q], 0});
+    upload(c, P.nearPtr, ptr);
+    upload(c, P.near, list);
+    HMX_HIP(hipStreamSynchronize(c.stream));
+}
+
+// The near-field block list of build_near (leaf-level tiles of rank above
+// b/2, by row leaf) without the blocks themselves: for plans that evaluate
+// them from the points.
+void build_near_list(Context& c, const HMatrix& h, int S, MfPlan& P, const std::vector<TileRef>& tiles) {
+    std::vector<std::vector<int32_t>> perLeaf(S);
+    for (const TileRef& R : tiles)
+        if (is_near(h, R)) perLeaf[R.leafBase + R.outCl].push_back((int32_t)(R.leafBase + R.inCl));
+    std::vector<int32_t> ptr(S + 1, 0);
+    std::vector<MfNear> list;
+    for (int leaf = 0; leaf < S; ++leaf) {
+        ptr[leaf] = (int32_t)list.size();
+        for (int32_t cl : perLeaf[leaf]) list.push_back(MfNear{nullptr, cl, 0});
+    }
+    ptr[S] = (int32_t)list.size();
+    P.numNear = (int64_t)list.size();
     upload(c, P.nearPtr, ptr);
     upload(c, P.near, list);
     HMX_HIP(hipStreamSynchronize(c.stream));
@@ -1119,7 +1316,8 @@ void run_fp64(Context& c, MfPlan& P, const double* X, int64_t ldx, int kk, doubl
                                      : k_mf_phase2g<32, false, double, double, double, NB, double>);
         k2<<<(P.S + 3) / 4, 256, 0, c.stream>>>(P.S, G.leafPtr.p, G.entries.p, gp, ge, (const double*)G.zt.p,
                                                 nullptr, nullptr, nullptr, nullptr, nullptr, (const double*)P.Drm.p,
-                                                P.nearPtr.p, P.near.p, Xb, Yo, ldo, kk, nullptr, 1.f);
+                                                P.nearPtr.p, P.near.p, Xb, Yo, ldo, kk, nullptr, 1.f, nullptr, nullptr,
+                                                0.f, nullptr, nullptr);
     }
     HMX_LAUNCHED();
     if (P.gather) {
@@ -1147,7 +1345,12 @@ void run_mixed(Context& c, MfPlan& P, const float* X, int64_t ldx, int kk, float
     HMX_LAUNCHED();
     const int64_t tot = P.npad * NR;
     const unsigned gx = (unsigned)((tot + 255) / 256);
-    k_x_blocked<float, _Float16><<<gx, 256, 0, c.stream>>>(tot, P.npad, NR, sh, X, ldx, kk, gm, P.xscale.p, P.X16.p);
+    if (P.nearOtf)
+        k_x_blocked_split<<<gx, 256, 0, c.stream>>>(tot, P.npad, NR, sh, X, ldx, kk, gm, P.xscale.p, P.X16.p,
+                                                    P.X16lo.p);
+    else
+        k_x_blocked<float, _Float16><<<gx, 256, 0, c.stream>>>(tot, P.npad, NR, sh, X, ldx, kk, gm, P.xscale.p,
+                                                               P.X16.p);
     HMX_LAUNCHED();
     k_x_blocked<float, float><<<gx, 256, 0, c.stream>>>(tot, P.npad, NR, sh, X, ldx, kk, gm, nullptr, (float*)P.Xb.p);
     HMX_LAUNCHED();
@@ -1168,7 +1371,8 @@ void run_mixed(Context& c, MfPlan& P, const float* X, int64_t ldx, int kk, float
             P.S, Ghi.leafPtr.p, Ghi.entries.p, gr ? Ghi.grpPtr.p : nullptr, gr ? Ghi.grpEntries.p : nullptr,
             (const float*)Ghi.zt.p, Glo.leafPtr.p, Glo.entries.p, gr ? Glo.grpPtr.p : nullptr,
             gr ? Glo.grpEntries.p : nullptr, (const _Float16*)Glo.zt.p, (const float*)P.Drm.p, P.nearPtr.p, P.near.p,
-            Xb, Yo, ldo, kk, P.xscale.p, P.loMul);
+            Xb, Yo, ldo, kk, P.xscale.p, P.loMul, P.nearOtf ? reinterpret_cast<const float4*>(P.ptsL.p) : nullptr,
+            reinterpret_cast<const double4*>(P.origin.p), P.nlog2e, P.X16.p, P.X16lo.p);
     }
     HMX_LAUNCHED();
     if (P.gather) {
@@ -1340,7 +1544,22 @@ void hmatrix_make_mixed(Context& c, HMatrix& h, double target) {
                 zb = std::max(zb, (double)su[colBase[t] + j] * (double)sv[colBase[t] + j] * tiles[t].m);
     const int zexp = zb > 0 ? (int)std::floor(std::log2(32768.0 / zb)) : 0;
     const bool grouped = h.b == 64 && h.S >= 4;
-    build_near<float>(c, h, (int)h.S, P, tiles);
+    // with the points at hand (direct assembly) the diagonal and near-field
+    // blocks are evaluated in phase 2 instead of stored
+    P.nearOtf = h.pts.n >= (size_t)h.S * h.b * 4 && h.ell > 0;
+    if (P.nearOtf) {
+        build_near_list(c, h, (int)h.S, P, tiles);
+        const int64_t npad = (int64_t)h.S * h.b;
+        P.ptsL.alloc((size_t)npad * 4);
+        P.origin.alloc((size_t)h.S * 4);
+        k_local_points<<<(unsigned)((npad + 255) / 256), 256, 0, c.stream>>>(
+            npad, (int)h.b, reinterpret_cast<const double4*>(h.pts.p), reinterpret_cast<float4*>(P.ptsL.p),
+            reinterpret_cast<double4*>(P.origin.p));
+        HMX_LAUNCHED();
+        P.nlog2e = (float)(-1.4426950408889634 / h.ell);
+    } else {
+        build_near<float>(c, h, (int)h.S, P, tiles);
+    }
     std::vector<GroupTile> gh, gl;
     int64_t hi = 0, lc = 0;
     for (size_t t = 0; t < tiles.size(); ++t) {
@@ -1359,10 +1578,12 @@ void hmatrix_make_mixed(Context& c, HMatrix& h, double target) {
     build_group(c, h, (int)h.S, grouped, P.g[0], 1, gh, false, nullptr, nullptr, nullptr);
     build_group(c, h, (int)h.S, grouped, P.g[1], 2, gl, false, &su, &sv, &colBase, (float)std::ldexp(1.0, zexp));
     const int64_t tot = (int64_t)h.S * h.b * h.b;
-    P.Drm.alloc(tot * sizeof(float));
-    k_diag_rowmajor<float><<<(unsigned)((tot + 255) / 256), 256, 0, c.stream>>>(tot, (int)h.b, h.diag.p,
-                                                                               (float*)P.Drm.p);
-    HMX_LAUNCHED();
+    if (!P.nearOtf) {
+        P.Drm.alloc(tot * sizeof(float));
+        k_diag_rowmajor<float><<<(unsigned)((tot + 255) / 256), 256, 0, c.stream>>>(tot, (int)h.b, h.diag.p,
+                                                                                   (float*)P.Drm.p);
+        HMX_LAUNCHED();
+    }
     HMX_HIP(hipStreamSynchronize(c.stream));
     P.dtype = 1;
     P.loMul = (float)std::ldexp(1.0, -zexp);
@@ -1378,8 +1599,11 @@ void hmatrix_make_mixed(Context& c, HMatrix& h, double target) {
     P.target = target;
     P.hiCols = hi;
     P.loCols = lc;
-    P.bytes = P.g[0].factorBytes + P.g[1].factorBytes + (uint64_t)tot * 4 + (uint64_t)lc * 4 +
-              (uint64_t)P.numNear * h.b * h.b * 4;
+    // bytes read per call: factors, FP16 column scales, and the stored dense
+    // blocks (none when they are evaluated on the fly: then the points)
+    P.bytes = P.g[0].factorBytes + P.g[1].factorBytes + (uint64_t)lc * 4 +
+              (P.nearOtf ? (uint64_t)h.S * h.b * 16 + (uint64_t)h.S * 32
+                         : (uint64_t)tot * 4 + (uint64_t)P.numNear * h.b * h.b * 4);
     P.built = true;
 }
 
@@ -1390,6 +1614,7 @@ bool ensure_mixed_ws(Context& c, MfPlan& P, int NR) {
     changed = ensure_ws(c, P.g[1], NR, sizeof(float)) || changed;
     if (P.x16n < (size_t)P.npad * NR) {
         P.X16.alloc((size_t)P.npad * NR);
+        if (P.nearOtf) P.X16lo.alloc((size_t)P.npad * NR);
         P.Xb.alloc((size_t)P.npad * NR * sizeof(float));
         if (P.gather) P.Yp.alloc((size_t)P.npad * NR * sizeof(float));
         P.x16n = (size_t)P.npad * NR;

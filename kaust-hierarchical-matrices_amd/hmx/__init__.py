@@ -69,6 +69,8 @@ _SIGS = [
     ("hmx_generateDataset", _i32, [_vp, _u32, _u32, _u64, _vp]),
     ("hmx_generateRandomVector", _i32, [_vp, _u32, _u32, _u64, _vp]),
     ("hmx_constructKDTree", _i32, [_vp, _u32, _u32, _u32, _i32, _vp, _P(_vp)]),
+    ("hmx_generateDatasetBatch", _i32, [_vp, _i32, _P(_u32), _P(_u32), _P(_u64), _P(_vp)]),
+    ("hmx_constructKDTreeBatch", _i32, [_vp, _i32, _P(_u32), _P(_u32), _u32, _i32, _P(_vp), _P(_vp)]),
     ("hmx_kdtree_info", _i32, [_vp, _P(_u32), _P(_u32), _P(_u32)]),
     ("hmx_kdtree_copy", _i32, [_vp, _vp, _vp, _vp, _vp, _vp]),
     ("hmx_kdtree_device_perm", _vp, [_vp]),
@@ -108,6 +110,7 @@ _SIGS = [
     ("hmx_hmatrix_level_get", _i32, [_vp, _vp, _i32, _vp, _vp]),
     ("hmx_hmatrix_set_level", _i32, [_vp, _vp, _i32, _vp, _vp]),
     ("hmx_hmatrix_plan_info", _i32, [_vp, _i32, _P(_u64), _P(_i64), _P(_i64)]),
+    ("hmx_hmatrix_plan_phase_bytes", _i32, [_vp, _i32, _P(_u64), _P(_u64)]),
     ("hmx_hmatrix_set_plan_options", _i32, [_vp, _i32, _i32, _i32]),
     ("hmx_hmatrix_set_fp64_layout", _i32, [_vp, _i32]),
     ("hmx_hmatrix_save", _i32, [_vp, _vp, ctypes.c_char_p]),
@@ -121,6 +124,7 @@ _SIGS = [
     ("hmx_hmatrix_build_info", _i32, [_vp, _P(_f64), _P(_f64), _P(_f64)]),
     ("hmx_debug_set_alloc_fault", None, [_i64]),
     ("hmx_debug_alloc_count", _i64, []),
+    ("hmx_debug_live_bytes", _i64, []),
 ]
 
 
@@ -196,11 +200,15 @@ class _Obj:
     def __init__(self, handle):
         self.handle = handle
 
+    def free(self):
+        """Releases the library object now (the reference's free* call); later use raises."""
+        if getattr(self, "handle", None) and self._free:
+            h, self.handle = self.handle, None
+            getattr(lib(), self._free)(h)
+
     def __del__(self):
         try:
-            if getattr(self, "handle", None) and self._free:
-                getattr(lib(), self._free)(self.handle)
-                self.handle = None
+            self.free()
         except Exception:  # interpreter shutdown
             pass
 
@@ -381,10 +389,17 @@ class HMatrix(_Obj):
         _check(lib().hmx_hmatrix_set_fp64_layout(self.handle, int(layout)))
 
     def plan_info(self, plan):
-        """plan 0: FP64 matrix-core plan, 1: mixed plan (bytes per call, dense blocks, low-rank columns)."""
+        """plan 0: FP64 matrix-core plan, 1: mixed plan, 2: FP64 streaming plan (bytes per call or pass,
+        dense blocks, low-rank columns)."""
         by, nd, nc = _u64(), _i64(), _i64()
         _check(lib().hmx_hmatrix_plan_info(self.handle, plan, ctypes.byref(by), ctypes.byref(nd), ctypes.byref(nc)))
         return dict(bytes=by.value, denseBlocks=nd.value, lowRankColumns=nc.value)
+
+    def plan_phase_bytes(self, plan):
+        """The plan's bytes by phase: (V factors, U factors + dense/near blocks or points + scales)."""
+        p1, p2 = _u64(), _u64()
+        _check(lib().hmx_hmatrix_plan_phase_bytes(self.handle, plan, ctypes.byref(p1), ctypes.byref(p2)))
+        return p1.value, p2.value
 
     def bytes(self):
         lr, dn, fe = _u64(), _u64(), _u64()
@@ -440,6 +455,25 @@ def constructKDTree(ctx, n, d, b, points, divMethod=FULL_TREE):
     h = ctypes.c_void_p()
     _check(lib().hmx_constructKDTree(ctx.handle, n, d, b, divMethod, _ptr(points), ctypes.byref(h)), ctx)
     return KDTree(h)
+
+
+def generateDatasetBatch(ctx, ns, ds, seeds, points):
+    """Points of many clouds in one launch; cloud k as generateDataset(ns[k], ds[k], seeds[k])."""
+    c = len(ns)
+    na, da = (_u32 * c)(*ns), (_u32 * c)(*ds)
+    sa = (_u64 * c)(*seeds)
+    pa = (_vp * c)(*[_ptr(p) for p in points])
+    _check(lib().hmx_generateDatasetBatch(ctx.handle, c, na, da, sa, pa), ctx)
+
+
+def constructKDTreeBatch(ctx, ns, ds, b, points, divMethod=FULL_TREE):
+    """KD-trees of many clouds, one forest per point dimension; tree k equals constructKDTree on cloud k."""
+    c = len(ns)
+    na, da = (_u32 * c)(*ns), (_u32 * c)(*ds)
+    pa = (_vp * c)(*[_ptr(p) for p in points])
+    ha = (_vp * c)()
+    _check(lib().hmx_constructKDTreeBatch(ctx.handle, c, na, da, b, divMethod, pa, ha), ctx)
+    return [KDTree(ctypes.c_void_p(ha[k])) for k in range(c)]
 
 
 def constructHMatrixStructure(ctx, n, d, b, adm, eta, tree):
@@ -508,9 +542,10 @@ def generateHMatrixDirectBatch(ctx, cfgs, trees, structures, points):
 
 
 class BatchPipeline:
-    """Many point clouds through the whole path: per cloud the dataset,
-    KD-tree and structure, then one batched direct assembly and one batched
-    matvec operator.  clouds: list of (n, d, ell)."""
+    """Many point clouds through the whole path: the datasets in one launch,
+    the KD-trees as one forest per point dimension, the structures on the
+    host, then one batched direct assembly and one batched matvec operator.
+    clouds: list of (n, d, ell)."""
 
     def __init__(self, clouds, b=32, tol=1e-6, adm="weak", max_rank_policy=MAXRANK_FULL, seed=1234, ctx=None,
                  points=None):
@@ -520,16 +555,17 @@ class BatchPipeline:
         self.clouds = list(clouds)
         t = {}
         t0 = time.perf_counter()
-        self.points, self.trees, self.structures, self.cfgs = [], [], [], []
+        ns = [n for n, _, _ in self.clouds]
+        ds = [d for _, d, _ in self.clouds]
+        if points is None:
+            self.points = [torch.empty(n * d, dtype=torch.float64, device="cuda") for n, d in zip(ns, ds)]
+            generateDatasetBatch(self.ctx, ns, ds, [seed + i for i in range(len(ns))], self.points)
+        else:
+            self.points = list(points)
+        self.trees = constructKDTreeBatch(self.ctx, ns, ds, b, self.points)
+        self.structures, self.cfgs = [], []
         for i, (n, d, ell) in enumerate(self.clouds):
-            if points is None:
-                p = torch.empty(n * d, dtype=torch.float64, device="cuda")
-                generateDataset(self.ctx, n, d, seed + i, p)
-            else:
-                p = points[i]
-            self.points.append(p)
-            self.trees.append(constructKDTree(self.ctx, n, d, b, p))
-            self.structures.append(constructHMatrixStructure(self.ctx, n, d, b, adm, 1.0, self.trees[-1]))
+            self.structures.append(constructHMatrixStructure(self.ctx, n, d, b, adm, 1.0, self.trees[i]))
             self.cfgs.append(default_config(numberOfInputPoints=n, dimensionOfInputPoints=d, bucketSize=b,
                                             lowestLevelTolerance=tol, admissibilityCondition=ADMISSIBILITY.get(adm, adm),
                                             correlationLength=ell, maxRankPolicy=max_rank_policy, seed=seed))

@@ -406,13 +406,15 @@ OR_API int or_jds(int level, int ntiles, const int* tiles, int* rowIndices, int*
 }
 
 /* ------------------------------------------------------------------------- */
-/* Kernel function (src/TLRMatrixHelpers.cu:40-58)                           */
+/* Kernel function (src/TLRMatrixHelpers.cu:40-58); its                     */
+/* diff += (x - y)*(x - y) is one fused multiply-add under nvcc's default    */
+/* contraction (--fmad=true), so it is written as fma here                   */
 /* ------------------------------------------------------------------------- */
 static inline double kfun(const double* p, int n, int d, int a, int b2, double ell) {
     double diff = 0;
     for (int j = 0; j < d; ++j) {
         double x = p[(size_t)j * n + a], y = p[(size_t)j * n + b2];
-        diff += (x - y) * (x - y);
+        diff = fma(x - y, x - y, diff);
     }
     return exp(-sqrt(diff) / ell);
 }

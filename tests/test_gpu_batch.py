@@ -10,7 +10,10 @@ Bars:
     each instance's own H-matvec; hipGraph replays equal the eager launches
     bit for bit, with eager work between the replays; after autotuning
     (another chunk length, another summation order) within 1e-14;
-  * every instance against sampled rows of its dense FP64 kernel.
+  * every instance against sampled rows of its dense FP64 kernel;
+  * the batched datasets and KD-trees (one forest per point dimension) equal
+    each cloud's own dataset and tree bit for bit: points, permutation,
+    offsets and every bounding box the build writes.
 """
 import numpy as np
 import pytest
@@ -152,6 +155,53 @@ def test_batch_tune_and_dense_error(torch_cuda, ctx, batch):
         rel = float(torch_cuda.linalg.norm(after[:, a:e][:, rows.long()] - yd) / torch_cuda.linalg.norm(yd))
         print(f"cloud {i}: n={n} d={d} l={ell:.3f} rel err {rel:.2e}")
         assert rel < 1e-5
+
+
+def written_boxes(n, b, L, div):
+    """Box slots a tree build writes: every node of levels 0..L-1 for the
+    halving methods; the segments that exist at each level for
+    POWER_OF_TWO_ON_LEFT (cut at multiples of 2^(L-l) * size)."""
+    if div != hmx.POWER_OF_TWO_ON_LEFT:
+        return np.arange((1 << L) - 1)
+    size = 1
+    while size < n:
+        size <<= 1
+    while size > b:
+        size >>= 1
+    out = []
+    for lev in range(L):
+        lsize = size << (L - lev)
+        out.extend(range((1 << lev) - 1, (1 << lev) - 1 + (n + lsize - 1) // lsize))
+    return np.array(out, dtype=int)
+
+
+@pytest.mark.parametrize("div", [hmx.FULL_TREE, hmx.POWER_OF_TWO_ON_LEFT])
+def test_batched_datasets_and_trees_equal_single(torch_cuda, ctx, div):
+    torch = torch_cuda
+    b = 32
+    specs = [(5000, 2), (1, 3), (31, 2), (100, 1), (70000, 3), (4097, 3), (2048, 2), (33, 1), (9000, 2)]
+    ns = [n for n, _ in specs]
+    ds = [d for _, d in specs]
+    seeds = [500 + 7 * i for i in range(len(specs))]
+    pts = [torch.empty(n * d, dtype=torch.float64, device="cuda") for n, d in specs]
+    hmx.generateDatasetBatch(ctx, ns, ds, seeds, pts)
+    trees = hmx.constructKDTreeBatch(ctx, ns, ds, b, pts, divMethod=div)
+    for i, (n, d) in enumerate(specs):
+        p1 = torch.empty(n * d, dtype=torch.float64, device="cuda")
+        hmx.generateDataset(ctx, n, d, seeds[i], p1)
+        assert torch.equal(pts[i], p1), i
+        t1 = hmx.constructKDTree(ctx, n, d, b, p1, divMethod=div)
+        assert trees[i].info() == t1.info(), i
+        pb, ob, bminb, bmaxb = trees[i].copy(ctx, n, d)
+        ps, os_, bmins, bmaxs = t1.copy(ctx, n, d)
+        assert np.array_equal(pb, ps) and np.array_equal(ob, os_), i
+        w = written_boxes(n, b, t1.info()[2], div)
+        assert np.array_equal(bminb[w], bmins[w]) and np.array_equal(bmaxb[w], bmaxs[w]), i
+    # the trees outlive one another: free some, the rest stay valid
+    keep = trees[4]
+    del trees
+    pb, ob, _, _ = keep.copy(ctx, ns[4], ds[4])
+    assert np.array_equal(np.sort(pb), np.arange(ns[4]))
 
 
 def test_batch_rejects_mixed_bucket_sizes(torch_cuda, ctx):
