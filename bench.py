@@ -167,6 +167,13 @@ def main():
     del probe
     torch.cuda.empty_cache()
     log(f"streaming-read ceiling {read_ceiling:.0f} GB/s; dominant kernel {achieved:.0f} GB/s")
+    # measured FP64 arithmetic ceilings (register-only chains, 8 waves per SIMD)
+    mfma_ceiling = hmx.fp64_peak_probe(ctx, 0)
+    valu_ceiling = hmx.fp64_peak_probe(ctx, 1)
+    log(f"FP64 ceilings: matrix cores {mfma_ceiling:.1f} TFLOP/s, vector FMA {valu_ceiling:.1f} TFLOP/s")
+    if build_roofline:
+        build_roofline["measured_mfma_ceiling"] = round(mfma_ceiling, 2)
+        build_roofline["frac_of_measured_ceiling"] = round(build_roofline["achieved"] / mfma_ceiling, 4)
 
     # ---- accuracy on sampled rows of the dense FP64 kernel (EXPAND_MATRIX check)
     g = torch.Generator().manual_seed(1235)
@@ -239,6 +246,9 @@ def main():
                      "phase2": round(fl2 / (msK[2] * 1e-3) / 1e12, 3),
                      "call": round((fl1 + fl2) / secK / 1e12, 3)}
         mfma_fp64["frac"] = round(mfma_fp64["call"] / PEAK_FP64_MFMA_TFLOPS, 4)
+        mfma_fp64["measured_ceiling"] = round(mfma_ceiling, 2)
+        mfma_fp64["measured_vector_fma_ceiling"] = round(valu_ceiling, 2)
+        mfma_fp64["frac_of_measured_ceiling"] = round(mfma_fp64["call"] / mfma_ceiling, 4)
         m1, m2 = H.plan_phase_bytes(1)
         zbytes = k * (2 * int(minfo["loCols"]) + 4 * int(minfo["hiCols"]))
         mb1 = m1 + 2 * n * k + zbytes

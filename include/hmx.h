@@ -331,6 +331,12 @@ hmx_status hmx_selftest_mfma(hmx_context* ctx, int32_t type, const double* d_A, 
    16-byte non-temporal loads -- the measured streaming-read ceiling the
    H-matvec bandwidth is compared against beside the spec peak. */
 hmx_status hmx_stream_read_probe(hmx_context* ctx, const void* d_buf, uint64_t bytes, int32_t iters, float* ms);
+/* Diagnostic, no reference counterpart: measured FP64 arithmetic rate of
+   this GPU in TFLOP/s, unit 0 = the matrix cores (v_mfma_f64_16x16x4f64),
+   1 = vector FMA, from `iters` rounds of independent register-only chains in
+   8 waves per SIMD -- the ceiling the FP64 matrix-core products and the
+   samplers are compared against beside the spec peak. */
+hmx_status hmx_fp64_peak_probe(hmx_context* ctx, int32_t unit, int32_t iters, float* tflops);
 
 /* ---- differentiation (no reference counterpart: the backward of
    HMatrixVecMult, for autograd callers) ------------------------------------ */

@@ -627,6 +627,14 @@ hmx_status hmx_stream_read_probe(hmx_context* ctx, const void* d_buf, uint64_t b
     });
 }
 
+hmx_status hmx_fp64_peak_probe(hmx_context* ctx, int32_t unit, int32_t iters, float* tflops) {
+    if (!ctx) return HMX_EINVAL;
+    return guard(ctx, [&] {
+        NEED(tflops);
+        fp64_peak(*ctx, unit, iters, tflops);
+    });
+}
+
 hmx_status hmx_selftest_mfma(hmx_context* ctx, int32_t type, const double* d_A, const double* d_B, double* d_D) {
     if (!ctx) return HMX_EINVAL;
     return guard(ctx, [&] {

@@ -384,6 +384,7 @@ void generate_dataset_batch(Context& c, const std::vector<uint64_t>& counts, con
                             const std::vector<double*>& outs);
 void generate_random_vector(Context& c, uint32_t k, uint32_t n, uint64_t seed, double* d_out);
 void stream_read(Context& c, const void* buf, uint64_t bytes, int iters, float* ms);
+void fp64_peak(Context& c, int unit, int iters, float* tflops);
 void reorder_vectors(Context& c, uint32_t n, uint32_t k, int toOriginal, const int32_t* d_perm, const double* d_in,
                      double* d_out);
 void build_kdtree(Context& c, KDTree& t, const double* d_points);

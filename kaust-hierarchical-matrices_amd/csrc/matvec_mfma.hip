@@ -408,8 +408,8 @@ __device__ inline void dense_rows(int leaf, const TD* __restrict__ Drm, const in
 // relative to their leaf's first point (PL), with those origins in FP64 (O):
 // x_r - x_c = (x_r - o_i) + (o_i - o_j) - (x_c - o_j), where o_i - o_j is
 // formed in FP64 once per block and every term is small, so the differences
-// carry errors near 1e-8 absolute; sqrt and exp in FP32
-// (exp(-r/l) = 2^(r * nlog2e)).  K is split into
+// carry errors near 1e-8 absolute; sqrt and exp in FP32 on the hardware
+// instructions (exp(-r/l) = 2^(r * nlog2e)).  K is split into
 // FP16 K_hi + K_lo and the right-hand sides (in units of their max-abs) into
 // X_hi + X_lo, so three FP16 16x16x16 products per step (K_hi X_hi + K_hi X_lo
 // + K_lo X_hi) carry about 22 bits; the K_lo X_lo term is below 2^-22 of the
@@ -467,7 +467,11 @@ __device__ inline void near_otf(int leaf, const float4* __restrict__ PL, const d
                 for (int e = 0; e < 4; ++e) {
                     const float dx = xr[rg][0] - xc[e][0], dy = xr[rg][1] - xc[e][1], dz = xr[rg][2] - xc[e][2];
                     const float r2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
-                    const float k = exp2f(nlog2e * sqrtf(r2));
+                    // the hardware square root and 2^x (v_sqrt_f32, v_exp_f32:
+                    // within 1 ulp, results below 2^-126 flushed to zero), not
+                    // the correctly rounded library sequences: 4 instructions
+                    // instead of 27, and the product carries 22 bits anyway
+                    const float k = __builtin_amdgcn_exp2f(nlog2e * __builtin_amdgcn_sqrtf(r2));
                     const _Float16 hi = (_Float16)k;
                     ah[e] = hi;
                     al[e] = (_Float16)(k - (float)hi);

@@ -51,6 +51,69 @@ __global__ void __launch_bounds__(256) k_stream_read(const u32x4_t* __restrict__
     if (acc == 0x9e3779b9u) sink[0] = acc;   // keeps the loads; practically never taken
 }
 
+// FP64 arithmetic ceiling: every wave runs `iters` rounds of 8 independent
+// accumulation chains, either v_mfma_f64_16x16x4f64 (unit 0; 2048 flops
+// each) or v_fma_f64 (unit 1; 8 chains x 8 steps, 2 flops per lane each), on
+// register operands only.  The matrix-core products and the samplers are
+// quoted against the spec peak and against this measured rate.
+typedef double f64x4_t __attribute__((ext_vector_type(4)));
+template <int UNIT>
+__global__ void __launch_bounds__(256) k_fp64_peak(int iters, double seed, double* __restrict__ sink) {
+    const double x = seed + 1e-3 * (threadIdx.x & 63), y = 1.0 - 1e-9 * blockIdx.x;
+    double s = 0.0;
+    if constexpr (UNIT == 0) {
+        f64x4_t acc[8];
+#pragma unroll
+        for (int c = 0; c < 8; ++c) acc[c] = f64x4_t{0.0, 0.0, 0.0, 0.0};
+        for (int i = 0; i < iters; ++i)
+#pragma unroll
+            for (int c = 0; c < 8; ++c) acc[c] = __builtin_amdgcn_mfma_f64_16x16x4f64(x, y, acc[c], 0, 0, 0);
+#pragma unroll
+        for (int c = 0; c < 8; ++c) s += acc[c][0] + acc[c][3];
+    } else {
+        double a[8];
+#pragma unroll
+        for (int c = 0; c < 8; ++c) a[c] = x + c;
+        for (int i = 0; i < iters; ++i)
+#pragma unroll
+            for (int u = 0; u < 8; ++u)
+#pragma unroll
+                for (int c = 0; c < 8; ++c) a[c] = fma(a[c], y, x);
+#pragma unroll
+        for (int c = 0; c < 8; ++c) s += a[c];
+    }
+    if (s == 0.123456789) sink[0] = s;  // keeps the arithmetic; practically never taken
+}
+
+void fp64_peak(Context& c, int unit, int iters, float* tflops) {
+    HMX_CHECK(unit == 0 || unit == 1, HMX_EINVAL, "FP64 probe: unit 0 (matrix cores) or 1 (vector FMA)");
+    HMX_CHECK(iters >= 1 && iters <= (1 << 20), HMX_EINVAL, "FP64 probe: 1 <= iters <= 2^20");
+    DBuf<double> sink(1);
+    hipDeviceProp_t prop;
+    HMX_HIP(hipGetDeviceProperties(&prop, c.device));
+    const unsigned blocks = (unsigned)prop.multiProcessorCount * 8;  // 8 waves per SIMD
+    auto launch = [&](int it) {
+        if (unit == 0) k_fp64_peak<0><<<blocks, 256, 0, c.stream>>>(it, 0.5, sink.p);
+        else k_fp64_peak<1><<<blocks, 256, 0, c.stream>>>(it, 0.5, sink.p);
+        HMX_LAUNCHED();
+    };
+    launch(std::max(1, iters / 10));  // warm-up
+    hipEvent_t a, b;
+    HMX_HIP(hipEventCreate(&a));
+    HMX_HIP(hipEventCreate(&b));
+    HMX_HIP(hipEventRecord(a, c.stream));
+    launch(iters);
+    HMX_HIP(hipEventRecord(b, c.stream));
+    HMX_HIP(hipEventSynchronize(b));
+    float t = 0;
+    HMX_HIP(hipEventElapsedTime(&t, a, b));
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    const double waves = (double)blocks * 4;
+    const double flops = unit == 0 ? waves * iters * 8 * 2048.0 : waves * 64 * iters * 64 * 2.0;
+    *tflops = (float)(flops / (t * 1e-3) / 1e12);
+}
+
 void stream_read(Context& c, const void* buf, uint64_t bytes, int iters, float* ms) {
     HMX_CHECK(bytes >= (1u << 20) && bytes % 16 == 0, HMX_EINVAL, "stream probe: buffer of >= 1 MiB, a multiple of 16 bytes");
     HMX_CHECK(iters >= 1, HMX_EINVAL, "stream probe: iters >= 1");

@@ -106,6 +106,7 @@ _SIGS = [
     ("hmx_HMatrixVecMultMixed", _i32, [_vp, _vp, _u32, _vp, _vp]),
     ("hmx_selftest_mfma", _i32, [_vp, _i32, _vp, _vp, _vp]),
     ("hmx_stream_read_probe", _i32, [_vp, _vp, _u64, _i32, _P(_f32)]),
+    ("hmx_fp64_peak_probe", _i32, [_vp, _i32, _i32, _P(_f32)]),
     ("hmx_hmatvec_grad_factors", _i32, [_vp, _vp, _u32, _vp, _vp, _vp, _vp]),
     ("hmx_hmatrix_level_get", _i32, [_vp, _vp, _i32, _vp, _vp]),
     ("hmx_hmatrix_set_level", _i32, [_vp, _vp, _i32, _vp, _vp]),
@@ -609,6 +610,13 @@ def make_mixed(ctx, H, target=3e-6):
 
 def HMatrixVecMultMixed(ctx, H, k, x, y):
     _check(lib().hmx_HMatrixVecMultMixed(ctx.handle, H.handle, k, _ptr(x), _ptr(y)), ctx)
+
+
+def fp64_peak_probe(ctx, unit=0, iters=20000):
+    """Measured FP64 TFLOP/s of the matrix cores (unit 0) or vector FMA (unit 1)."""
+    t = ctypes.c_float(0)
+    _check(lib().hmx_fp64_peak_probe(ctx.handle, unit, iters, ctypes.byref(t)), ctx)
+    return t.value
 
 
 def stream_read_probe(ctx, buf, iters=5):
