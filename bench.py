@@ -59,8 +59,8 @@ def main():
     ap.add_argument("--batch-nmax", type=int, default=16384,
                     help="largest cloud (sizes up to 65536 need ~280 GB of FP64 factors at eps 1e-6: more than one GPU)")
     ap.add_argument("--counters-run", action="store_true",
-                    help="build, one 1 GiB device copy (HBM counter calibration), 3 FP64 1-RHS matvecs, one FP64 and "
-                         "one mixed --rhs product, exit")
+                    help="build, one 1 GiB device copy (HBM counter calibration), 3 FP64 1-RHS matvecs, the FP64 "
+                         "ceiling probes, one FP64 and one mixed --rhs product, exit")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for {args.gpus}): the library runs on one GPU; how the "
@@ -130,6 +130,9 @@ def main():
         for _ in range(3):
             pipe.matvec(x1, y1)
         torch.cuda.synchronize()
+        # the FP64 ceiling probes: the matrix-core busy counters' reference
+        hmx.fp64_peak_probe(ctx, 0, iters=4000)
+        hmx.fp64_peak_probe(ctx, 1, iters=4000)
         if args.rhs:  # one FP64 and one mixed multi-RHS product as well
             X = torch.empty(args.rhs, n, dtype=torch.float64, device="cuda")
             hmx.generateRandomVector(ctx, args.rhs, n, 777, X)

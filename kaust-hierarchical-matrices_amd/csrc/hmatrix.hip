@@ -504,6 +504,107 @@ __global__ void __launch_bounds__(64) k_lr_decide_w(LrLevel L) {
     if (lane == 0) { S.na = na; S.small_run = small; S.converged = conv; S.capped = capped; }
 }
 
+// The same decisions for tiles of 1024 and 2048 rows: four waves, thread
+// t owning rows t + 256 i (i < R = m / 256), the current column and the
+// block's accepted columns in registers, one barrier per sum (block_sum)
+template <int R>
+__global__ void __launch_bounds__(256) k_lr_decide_b(LrLevel L) {
+    __shared__ double red[32];
+    int parity = 0;
+    const int t = blockIdx.x, tid = threadIdx.x;
+    LrState& S = L.st[t];
+    if (!lr_active(S, L.tp[t].maxRank)) {
+        if (tid == 0) S.na = 0;
+        return;
+    }
+    const int m = L.m;
+    const double* Y = L.Y + (size_t)t * m * kB16;
+    double* Qb = L.Ya + (size_t)t * m * kB16;
+    double qb[kB16][R];
+    int na = 0, small = S.small_run, conv = 0, capped = 0;
+    const int capleft = L.tp[t].maxRank - S.k;
+    const double tau = L.tp[t].tau;
+    for (int j = 0; j < kB16; ++j) {
+        double y[R];
+#pragma unroll
+        for (int r = 0; r < R; ++r) y[r] = Y[(size_t)j * m + tid + 256 * r];
+#pragma unroll
+        for (int pass = 0; pass < 2; ++pass)
+#pragma unroll
+            for (int a = 0; a < kB16; ++a)
+                if (a < na) {
+                    double p = 0.0;
+#pragma unroll
+                    for (int r = 0; r < R; ++r) p += qb[a][r] * y[r];
+                    const double d = block_sum(p, red, parity);
+#pragma unroll
+                    for (int r = 0; r < R; ++r) y[r] -= qb[a][r] * d;
+                }
+        double p = 0.0;
+#pragma unroll
+        for (int r = 0; r < R; ++r) p += y[r] * y[r];
+        const double nu = sqrt(block_sum(p, red, parity));
+        if (nu > tau) {
+            if (na >= capleft) { capped = 1; break; }
+#pragma unroll
+            for (int a = 0; a < kB16; ++a)
+                if (a == na)
+#pragma unroll
+                    for (int r = 0; r < R; ++r) qb[a][r] = y[r] / nu;
+#pragma unroll
+            for (int r = 0; r < R; ++r) Qb[(size_t)na * m + tid + 256 * r] = y[r] / nu;
+            ++na;
+            small = 0;
+        } else if (++small >= L.r) {
+            conv = 1;
+            break;
+        }
+    }
+    if (tid == 0) { S.na = na; S.small_run = small; S.converged = conv; S.capped = capped; }
+}
+
+// k_lr_renorm for tiles of 1024 and 2048 rows: four waves, registers
+template <int R>
+__global__ void __launch_bounds__(256) k_lr_renorm_b(LrLevel L) {
+    __shared__ double red[32];
+    int parity = 0;
+    const int t = blockIdx.x, tid = threadIdx.x;
+    const LrState& S = L.st[t];
+    const int na = S.na;
+    if (na == 0) return;
+    const int m = L.m;
+    double* Qb = L.Ya + (size_t)t * m * kB16;
+    double* Q = L.Q + (size_t)t * m * L.capQ;
+    double qb[kB16][R];
+#pragma unroll
+    for (int j = 0; j < kB16; ++j)
+        if (j < na) {
+            double q[R];
+#pragma unroll
+            for (int r = 0; r < R; ++r) q[r] = Qb[(size_t)j * m + tid + 256 * r];
+#pragma unroll
+            for (int a = 0; a < j; ++a) {
+                double p = 0.0;
+#pragma unroll
+                for (int r = 0; r < R; ++r) p += qb[a][r] * q[r];
+                const double d = block_sum(p, red, parity);
+#pragma unroll
+                for (int r = 0; r < R; ++r) q[r] -= qb[a][r] * d;
+            }
+            double p = 0.0;
+#pragma unroll
+            for (int r = 0; r < R; ++r) p += q[r] * q[r];
+            const double nu = sqrt(block_sum(p, red, parity));
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                const double v = q[r] / nu;
+                qb[j][r] = v;
+                Qb[(size_t)j * m + tid + 256 * r] = v;
+                Q[(size_t)(S.k + j) * m + tid + 256 * r] = v;
+            }
+        }
+}
+
 // k_lr_renorm for tiles of m <= 512 rows, one wave per tile, registers only
 template <int R>
 __global__ void __launch_bounds__(64) k_lr_renorm_w(LrLevel L) {
@@ -1017,6 +1118,8 @@ static void compress_batch(Context& c, int araR, uint64_t seed, const LevelSourc
         else if (m == 128) k_lr_decide_w<2><<<T, 64, 0, c.stream>>>(L);
         else if (m == 256) k_lr_decide_w<4><<<T, 64, 0, c.stream>>>(L);
         else if (m == 512) k_lr_decide_w<8><<<T, 64, 0, c.stream>>>(L);
+        else if (m == 1024) k_lr_decide_b<4><<<T, 256, 0, c.stream>>>(L);
+        else if (m == 2048) k_lr_decide_b<8><<<T, 256, 0, c.stream>>>(L);
         else k_lr_decide<<<T, wgThreads, 0, c.stream>>>(L);
         HMX_LAUNCHED();
         if (kmax > 0) {
@@ -1029,6 +1132,8 @@ static void compress_batch(Context& c, int araR, uint64_t seed, const LevelSourc
         else if (m == 128) k_lr_renorm_w<2><<<T, 64, 0, c.stream>>>(L);
         else if (m == 256) k_lr_renorm_w<4><<<T, 64, 0, c.stream>>>(L);
         else if (m == 512) k_lr_renorm_w<8><<<T, 64, 0, c.stream>>>(L);
+        else if (m == 1024) k_lr_renorm_b<4><<<T, 256, 0, c.stream>>>(L);
+        else if (m == 2048) k_lr_renorm_b<8><<<T, 256, 0, c.stream>>>(L);
         else k_lr_renorm<<<T, wgThreads, 0, c.stream>>>(L);
         HMX_LAUNCHED();
         HMX_HIP(hipMemsetAsync(ws.flags.p, 0, 2 * sizeof(int), c.stream));
@@ -1114,13 +1219,17 @@ static TileParam tile_param(double tol, double ell, int64_t pbase, int level, in
 
 // Blocks a tile of the next level up is likely to need in its first sweep:
 // the ARA stops r samples after its last accepted column, so a tile of rank
-// k takes about (k + r)/16 blocks; the 10th-percentile rank of the level
-// below stands in for k (ranks grow slowly towards the root).
+// k takes ceil((k + r)/16) blocks; the median rank of the level below stands
+// in for k (ranks grow slowly towards the root).  A block costs one more
+// product per kernel entry, a second sweep another evaluation of every
+// entry, so covering the median tile in one sweep pays: at N = 2^20 3D the
+// samplers take 8.9 s against 9.6 s from the 10th-percentile rank rounded
+// down (14% fewer kernel evaluations; the ranks are the same).
 static int first_sweep_blocks(std::vector<int32_t> ranks, int araR) {
     if (ranks.empty()) return 2;
-    const size_t q = ranks.size() / 10;
+    const size_t q = ranks.size() / 2;
     std::nth_element(ranks.begin(), ranks.begin() + q, ranks.end());
-    return std::max(1, (ranks[q] + araR) / kB16);
+    return std::max(1, std::min(4, (ranks[q] + araR + kB16 - 1) / kB16));
 }
 
 static void compress_level(Context& c, const hmx_config& cfg, const LevelSource& src, int b,

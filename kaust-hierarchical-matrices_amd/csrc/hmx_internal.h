@@ -300,6 +300,9 @@ struct MfPlan {
     DBuf<unsigned char> nearStore; // dense leaf-level blocks, row-major (f64 or f32)
     DBuf<int32_t> nearPtr;         // [S+1] CSR over row leaves
     DBuf<MfNear> near;
+    DBuf<int32_t> gnearPtr;        // nearOtf: [S/4+1] CSR over groups of 4 leaves
+    DBuf<MfNear> gnear;            // the group's column leaves (diagonals included), pad = mask of the
+                                   // group's leaves that have the block
     int64_t numNear = 0;
     // leaf space of the plan (one H-matrix, or a batch's block-diagonal
     // operator): S leaves of b rows, npad = S*b positions for n vector

@@ -418,8 +418,8 @@ __device__ inline void dense_rows(int leaf, const TD* __restrict__ Drm, const in
 // positions hold the point (0, 0, 0): finite kernel values that meet zero
 // right-hand-side entries or land in discarded rows.
 template <int BL, int NB>
-__device__ inline void near_otf(int leaf, const float4* __restrict__ PL, const double4* __restrict__ O, float nlog2e,
-                                const int32_t* __restrict__ nearPtr, const MfNear* __restrict__ nearB,
+__device__ inline void near_otf(int leaf, int w, const float4* __restrict__ PL, const double4* __restrict__ O,
+                                float nlog2e, const int32_t* __restrict__ gnearPtr, const MfNear* __restrict__ gnear,
                                 const _Float16* __restrict__ Xh, const _Float16* __restrict__ Xl, int lg, int lc,
                                 int kvalid, v4f (&acc)[BL / 16][NB]) {
     constexpr int RG = BL / 16, NR = 16 * NB, SH = BL == 64 ? 6 : 5;
@@ -430,9 +430,12 @@ __device__ inline void near_otf(int leaf, const float4* __restrict__ PL, const d
         xl[rg][0] = p.x; xl[rg][1] = p.y; xl[rg][2] = p.z;
     }
     const double4 oi = O[leaf];
-    const int n0 = nearPtr[leaf], n1 = nearPtr[leaf + 1];
-    for (int q = n0 - 1; q < n1; ++q) {
-        const int cl = q < n0 ? leaf : nearB[q].colLeaf;
+    const int grp = leaf >> 2;
+    const int n0 = gnearPtr[grp], n1 = gnearPtr[grp + 1];
+    for (int q = n0; q < n1; ++q) {
+        const MfNear e = gnear[q];
+        if (!((__builtin_amdgcn_readfirstlane(e.pad) >> w) & 1)) continue;  // not this wave's block
+        const int cl = __builtin_amdgcn_readfirstlane(e.colLeaf);
         const double4 oj = O[cl];
         const float d0 = (float)(oi.x - oj.x), d1 = (float)(oi.y - oj.y), d2 = (float)(oi.z - oj.z);
         float xr[RG][3];
@@ -543,7 +546,7 @@ __global__ void __launch_bounds__(256) k_mf_phase2g(int S, const int32_t* __rest
             for (int nb = 0; nb < NB; ++nb) acc[rg][nb] *= (typename MA::acc_t)loMul;
     }
     if constexpr (std::is_same<TB, _Float16>::value && std::is_same<typename MA::accv, v4f>::value) {
-        if (P) near_otf<BL, NB>(leaf, P, O, nlog2e, nearPtr, nearB, Xh, Xl, lg, lc, kvalid, acc);
+        if (P) near_otf<BL, NB>(leaf, w, P, O, nlog2e, nearPtr, nearB, Xh, Xl, lg, lc, kvalid, acc);
     }
     if (ptrB) {
 #pragma unroll
@@ -1142,6 +1145,32 @@ void build_near_list(Context& c, const HMatrix& h, int S, MfPlan& P, const std::
     P.numNear = (int64_t)list.size();
     upload(c, P.nearPtr, ptr);
     upload(c, P.near, list);
+    // the same blocks merged per group of 4 leaves (one workgroup of the
+    // phase-2 kernel), diagonals included: one entry per column leaf in
+    // increasing order with the mask of the group's leaves that need it, so
+    // the four waves walk one sequence and read each neighbour's right-hand
+    // sides at about the same time
+    const int G = (S + 3) / 4;
+    std::vector<int32_t> gptr(G + 1, 0);
+    std::vector<MfNear> glist;
+    std::vector<std::pair<int32_t, int32_t>> tmp;
+    for (int g = 0; g < G; ++g) {
+        gptr[g] = (int32_t)glist.size();
+        tmp.clear();
+        for (int w = 0; w < 4 && 4 * g + w < S; ++w) {
+            const int leaf = 4 * g + w;
+            tmp.emplace_back(leaf, 1 << w);
+            for (int32_t cl : perLeaf[leaf]) tmp.emplace_back(cl, 1 << w);
+        }
+        std::sort(tmp.begin(), tmp.end());
+        for (size_t q = 0; q < tmp.size(); ++q) {
+            if (!glist.empty() && q > 0 && tmp[q].first == tmp[q - 1].first) glist.back().pad |= tmp[q].second;
+            else glist.push_back(MfNear{nullptr, tmp[q].first, tmp[q].second});
+        }
+    }
+    gptr[G] = (int32_t)glist.size();
+    upload(c, P.gnearPtr, gptr);
+    upload(c, P.gnear, glist);
     HMX_HIP(hipStreamSynchronize(c.stream));
 }
 
@@ -1374,7 +1403,8 @@ void run_mixed(Context& c, MfPlan& P, const float* X, int64_t ldx, int kk, float
         k2<<<(P.S + 3) / 4, 256, 0, c.stream>>>(
             P.S, Ghi.leafPtr.p, Ghi.entries.p, gr ? Ghi.grpPtr.p : nullptr, gr ? Ghi.grpEntries.p : nullptr,
             (const float*)Ghi.zt.p, Glo.leafPtr.p, Glo.entries.p, gr ? Glo.grpPtr.p : nullptr,
-            gr ? Glo.grpEntries.p : nullptr, (const _Float16*)Glo.zt.p, (const float*)P.Drm.p, P.nearPtr.p, P.near.p,
+            gr ? Glo.grpEntries.p : nullptr, (const _Float16*)Glo.zt.p, (const float*)P.Drm.p,
+            P.nearOtf ? P.gnearPtr.p : P.nearPtr.p, P.nearOtf ? P.gnear.p : P.near.p,
             Xb, Yo, ldo, kk, P.xscale.p, P.loMul, P.nearOtf ? reinterpret_cast<const float4*>(P.ptsL.p) : nullptr,
             reinterpret_cast<const double4*>(P.origin.p), P.nlog2e, P.X16.p, P.X16lo.p);
     }
