@@ -1170,6 +1170,8 @@ static void compress_batch(Context& c, int araR, uint64_t seed, const LevelSourc
     out.kmax = kmax;
     out.U.alloc((size_t)acc * m);
     out.V.alloc((size_t)acc * m);
+    const auto tAlloc = std::chrono::steady_clock::now();
+    double sampleTms = 0;
     if (acc > 0) {
         HMX_HIP(hipMemcpyAsync(ws.off.p, out.offsets.data(), (T + 1) * sizeof(int64_t), hipMemcpyHostToDevice,
                                c.stream));
@@ -1180,6 +1182,10 @@ static void compress_batch(Context& c, int araR, uint64_t seed, const LevelSourc
             else if (src.d == 2) launch_sampleT_direct<2>(c, L, kmax, out.V.p);
             else launch_sampleT_direct<3>(c, L, kmax, out.V.p);
             ws.mark(c, 1);
+            if (log_on()) {  // the AᵀQ sweep alone (the stream is synchronised below anyway)
+                HMX_HIP(hipStreamSynchronize(c.stream));
+                sampleTms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - tAlloc).count();
+            }
             const double mm = (double)m * m;
             for (int t = 0; t < T; ++t) {
                 ws.samplerFlops += 2.0 * mm * out.ranks[t];
@@ -1201,8 +1207,10 @@ static void compress_batch(Context& c, int araR, uint64_t seed, const LevelSourc
             return std::chrono::duration<double, std::milli>(b - a).count();
         };
         const auto tD = std::chrono::steady_clock::now();
-        std::fprintf(stderr, "[hmx]   T=%d m=%d: setup %.1f ms, ARA loop %.1f ms, A^T Q + pack %.1f ms\n", T, m,
-                     ms(tA, tB), ms(tB, tC), ms(tC, tD));
+        std::fprintf(stderr,
+                     "[hmx]   T=%d m=%d: setup %.1f ms, ARA loop %.1f ms, A^T Q + pack %.1f ms "
+                     "(ranks and factor allocation %.1f ms, A^T Q %.1f ms)\n",
+                     T, m, ms(tA, tB), ms(tB, tC), ms(tC, tD), ms(tC, tAlloc), sampleTms);
     }
 }
 
