@@ -173,6 +173,18 @@ hmx_status hmx_freeHMatrixStructure(hmx_structure* s);
    the fly; dense diagonal blocks.  rankSum may be NULL. */
 hmx_status hmx_createColumnMajorLRMatrix(hmx_context* ctx, const hmx_config* cfg, const hmx_kdtree* tree,
                                          const double* d_points, hmx_tlr** matrix, uint64_t* rankSum);
+/* One piece of a pieces x pieces partition of the TLR matrix: the tiles of
+   row leaves [rowPiece*S/pieces, (rowPiece+1)*S/pieces) and column leaves
+   [colPiece*S/pieces, ...), column-major within the piece, each equal bit for
+   bit to that tile of hmx_createColumnMajorLRMatrix (same kernel and random
+   stream); a diagonal piece (rowPiece == colPiece) carries its S/pieces
+   diagonal blocks.  numSegments of the piece is S/pieces.  For matrices whose
+   S x S TLR does not fit at once: the reference's large-matrices branch idea
+   (README.md:4; the branch is not in the snapshot).  A piece has no Morton
+   form and builds no H-matrix (HMX_EINVAL there). */
+hmx_status hmx_createColumnMajorLRMatrixPiece(hmx_context* ctx, const hmx_config* cfg, const hmx_kdtree* tree,
+                                              const double* d_points, uint32_t pieces, uint32_t rowPiece,
+                                              uint32_t colPiece, hmx_tlr** matrix, uint64_t* rankSum);
 /* convertColumnMajorToMorton, src/helperFunctions.cu:8-49 */
 hmx_status hmx_convertColumnMajorToMorton(hmx_context* ctx, const hmx_tlr* matrix, hmx_tlr** mortonMatrix);
 /* numSegments, segmentSize, ordering, rankSum, number of rank-cap hits */

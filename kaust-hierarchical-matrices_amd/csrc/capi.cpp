@@ -362,12 +362,36 @@ hmx_status hmx_createColumnMajorLRMatrix(hmx_context* ctx, const hmx_config* cfg
     });
 }
 
+hmx_status hmx_createColumnMajorLRMatrixPiece(hmx_context* ctx, const hmx_config* cfg, const hmx_kdtree* tree,
+                                              const double* d_points, uint32_t pieces, uint32_t rowPiece,
+                                              uint32_t colPiece, hmx_tlr** matrix, uint64_t* rankSum) {
+    if (!ctx) return HMX_EINVAL;
+    return guard(ctx, [&] {
+        check_config(cfg);
+        NEED(tree);
+        NEED(d_points);
+        NEED(matrix);
+        *matrix = nullptr;
+        hmx_tlr* m = new hmx_tlr();
+        try {
+            build_tlr_piece(*ctx, *cfg, *tree, d_points, pieces, rowPiece, colPiece, *m);
+            check_cap_hits(*cfg, m->capHits, "TLR tiles");
+        } catch (...) {
+            delete m;
+            throw;
+        }
+        *matrix = m;
+        if (rankSum) *rankSum = m->rankSum;
+    });
+}
+
 hmx_status hmx_convertColumnMajorToMorton(hmx_context* ctx, const hmx_tlr* matrix, hmx_tlr** mortonMatrix) {
     if (!ctx) return HMX_EINVAL;
     return guard(ctx, [&] {
         NEED(matrix);
         NEED(mortonMatrix);
         *mortonMatrix = nullptr;
+        HMX_CHECK(matrix->pieces == 1, HMX_EINVAL, "a TLR piece has no Morton form (convert the whole matrix)");
         hmx_tlr* m = new hmx_tlr();
         try {
             convert_to_morton(*ctx, *matrix, *m);
@@ -430,6 +454,7 @@ hmx_status hmx_generateHMatrixFromStruct(hmx_context* ctx, const hmx_config* cfg
         NEED(mortonMatrix);
         NEED(hmatrix);
         *hmatrix = nullptr;
+        HMX_CHECK(mortonMatrix->pieces == 1, HMX_EINVAL, "an H-matrix needs the whole TLR matrix, not a piece");
         hmx_hmatrix* h = new hmx_hmatrix();
         try {
             build_hmatrix(*ctx, *cfg, *tree, *structure, *mortonMatrix, *h);

@@ -135,6 +135,7 @@ struct Structure {
 struct TLR {
     int32_t ordering = HMX_COLUMN_MAJOR;
     uint32_t S = 0, b = 0, n = 0;
+    uint32_t pieces = 1, rowPiece = 0, colPiece = 0;  // a piece: S = leaves per piece side
     uint64_t rankSum = 0, capHits = 0;
     DBuf<int32_t> ranks;     // blockRanks [S*S]
     DBuf<int64_t> offsets;   // blockOffsets [S*S+1] (exclusive scan, in columns)
@@ -395,6 +396,8 @@ void build_kdtree_batch(Context& c, const std::vector<KDTree*>& ts, const std::v
 void build_structure(Structure& s, uint32_t n, uint32_t d, uint32_t b, int adm, float eta, const double* bbmin,
                      const double* bbmax);
 void build_tlr_cm(Context& c, const hmx_config& cfg, const KDTree& t, const double* d_points, TLR& out);
+void build_tlr_piece(Context& c, const hmx_config& cfg, const KDTree& t, const double* d_points, uint32_t pieces,
+                     uint32_t rowPiece, uint32_t colPiece, TLR& out);
 void convert_to_morton(Context& c, const TLR& cm, TLR& mo);
 void build_hmatrix(Context& c, const hmx_config& cfg, const KDTree& t, const Structure& s, const TLR& mo, HMatrix& h);
 void build_hmatrix_direct(Context& c, const hmx_config& cfg, const KDTree& t, const Structure& s,

@@ -61,6 +61,7 @@ struct TlrArgs {
     const TlrCloud* clouds;     // per-cloud constants
     const int32_t* tileCloud;   // cloud of each listed tile, or null for cloud 0
     const int32_t* tileMO;      // Morton tile list (cloud-local codes), or null for column-major enumeration
+    uint32_t diagLeaf0;         // diagonal block of leaf u goes to diag[(u - diagLeaf0) * b * b]
     int r;
     uint64_t seed;
 };
@@ -126,7 +127,7 @@ __global__ void __launch_bounds__(64) k_tlr_ara(TlrArgs g, uint64_t t0, double* 
                 } else {
                     val = c == q ? 1.0 : 0.0;
                 }
-                diag[(size_t)u * B * B + e] = val;
+                diag[(size_t)(u - g.diagLeaf0) * B * B + e] = val;
             }
         if (tid == 0) { ranks[slot] = 0; caps[slot] = 0; }
         return;
@@ -500,6 +501,58 @@ void build_tlr_cm(Context& c, const hmx_config& cfg, const KDTree& t, const doub
     g.seed = cfg.seed;
     uint64_t rankSum = 0;
     out.capHits = compress_tiles(c, b, t.d, g, q.maxRank, total, out.ranks.p, out.diagonal.p, out.U, out.V, rankSum);
+    out.rankSum = rankSum;
+    exclusive_scan_i32_i64(c, out.ranks.p, out.offsets.p, total);
+    HMX_HIP(hipStreamSynchronize(c.stream));
+}
+
+// One piece of a p x p partition of the TLR matrix (the reference's
+// large-matrices idea, README.md:4: the tiles of rows [i S/p, (i+1) S/p) and
+// columns [j S/p, (j+1) S/p), so a matrix beyond memory can be built piece
+// by piece).  Tiles in column-major order within the piece; every tile is
+// the full matrix's tile bit for bit (same kernel, same random stream
+// u*S + v); diagonal pieces carry their diagonal blocks.
+void build_tlr_piece(Context& c, const hmx_config& cfg, const KDTree& t, const double* d_points, uint32_t pieces,
+                     uint32_t rowPiece, uint32_t colPiece, TLR& out) {
+    check_tlr_args(cfg, t.b, t.numSegments);
+    const uint32_t b = t.b, S = t.numSegments;
+    HMX_CHECK(pieces >= 1 && S % pieces == 0, HMX_EINVAL, "TLR pieces: the leaf count must divide into pieces");
+    HMX_CHECK(rowPiece < pieces && colPiece < pieces, HMX_EINVAL, "TLR pieces: piece index out of range");
+    const uint32_t Sp = S / pieces;
+    out.ordering = HMX_COLUMN_MAJOR;
+    out.S = Sp;
+    out.b = b;
+    out.n = t.n;
+    out.pieces = pieces;
+    out.rowPiece = rowPiece;
+    out.colPiece = colPiece;
+    const uint64_t total = (uint64_t)Sp * Sp;
+    std::vector<int32_t> codes(total);
+    for (uint32_t uu = 0; uu < Sp; ++uu)
+        for (uint32_t vv = 0; vv < Sp; ++vv)
+            codes[(size_t)uu * Sp + vv] = (int32_t)morton_xy(colPiece * Sp + uu, rowPiece * Sp + vv);
+    out.ranks.alloc(total);
+    out.offsets.alloc(total + 1);
+    const bool onDiag = rowPiece == colPiece;
+    if (onDiag) out.diagonal.alloc((size_t)Sp * b * b);
+    const TlrCloud q = tlr_cloud(cfg, b, t.d, S, 0);
+    DBuf<TlrCloud> dq(1);
+    DBuf<int32_t> d_codes(total);
+    HMX_HIP(hipMemcpyAsync(dq.p, &q, sizeof(TlrCloud), hipMemcpyHostToDevice, c.stream));
+    HMX_HIP(hipMemcpyAsync(d_codes.p, codes.data(), total * sizeof(int32_t), hipMemcpyHostToDevice, c.stream));
+    TlrArgs g{};
+    g.n = t.n;
+    g.p = d_points;
+    g.perm = t.perm.p;
+    g.offsets = t.offsets.p;
+    g.clouds = dq.p;
+    g.tileMO = d_codes.p;
+    g.diagLeaf0 = colPiece * Sp;
+    g.r = cfg.araR;
+    g.seed = cfg.seed;
+    uint64_t rankSum = 0;
+    out.capHits = compress_tiles(c, b, t.d, g, q.maxRank, total, out.ranks.p, onDiag ? out.diagonal.p : nullptr, out.U,
+                                 out.V, rankSum);
     out.rankSum = rankSum;
     exclusive_scan_i32_i64(c, out.ranks.p, out.offsets.p, total);
     HMX_HIP(hipStreamSynchronize(c.stream));

@@ -84,6 +84,7 @@ _SIGS = [
     ("hmx_freeHMatrixStructure", _i32, [_vp]),
     ("hmx_createColumnMajorLRMatrix", _i32, [_vp, _P(Config), _vp, _vp, _P(_vp), _P(_u64)]),
     ("hmx_convertColumnMajorToMorton", _i32, [_vp, _vp, _P(_vp)]),
+    ("hmx_createColumnMajorLRMatrixPiece", _i32, [_vp, _P(Config), _vp, _vp, _u32, _u32, _u32, _P(_vp), _P(_u64)]),
     ("hmx_tlr_info", _i32, [_vp, _P(_u32), _P(_u32), _P(_i32), _P(_u64), _P(_u64)]),
     ("hmx_tlr_copy", _i32, [_vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     ("hmx_freeMatrix", _i32, [_vp]),
@@ -498,6 +499,15 @@ def createColumnMajorLRMatrix(ctx, cfg, tree, points):
     rs = _u64()
     _check(lib().hmx_createColumnMajorLRMatrix(ctx.handle, ctypes.byref(cfg), tree.handle, _ptr(points),
                                                ctypes.byref(h), ctypes.byref(rs)), ctx)
+    return TLRMatrix(h), rs.value
+
+
+def createColumnMajorLRMatrixPiece(ctx, cfg, tree, points, pieces, row_piece, col_piece):
+    """Piece (row_piece, col_piece) of a pieces x pieces partition of the TLR matrix."""
+    h = ctypes.c_void_p()
+    rs = _u64()
+    _check(lib().hmx_createColumnMajorLRMatrixPiece(ctx.handle, ctypes.byref(cfg), tree.handle, _ptr(points), pieces,
+                                                    row_piece, col_piece, ctypes.byref(h), ctypes.byref(rs)), ctx)
     return TLRMatrix(h), rs.value
 
 

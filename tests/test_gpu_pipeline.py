@@ -158,6 +158,54 @@ def test_morton_conversion_moves_factors_exactly(pair, ctx):
         assert np.array_equal(Vm[om[mo] * b:(om[mo] + r) * b], Vc[oc[i] * b:(oc[i] + r) * b])
 
 
+@pytest.mark.parametrize("pieces", [1, 2, 4])
+def test_tlr_pieces_tile_the_whole_matrix_exactly(pair, ctx, pieces):
+    """Every piece of a p x p partition holds the whole TLR matrix's tiles bit
+    for bit (ranks, U, V) and a diagonal piece its diagonal blocks."""
+    pipe, ref, _ = pair
+    S, b = ref.S, ref.b
+    rf, of, Uf, Vf, Df = pipe.tlr_cm.copy(ctx)
+    Sp = S // pieces
+    rank_total = 0
+    for i in range(pieces):
+        for j in range(pieces):
+            m, rs = hmx.createColumnMajorLRMatrixPiece(ctx, pipe.cfg, pipe.tree, pipe.points, pieces, i, j)
+            info = m.info()
+            assert info["numSegments"] == Sp and info["ordering"] == hmx.COLUMN_MAJOR
+            rp, op, Up, Vp, Dp = m.copy(ctx)
+            assert rs == info["rankSum"] == int(rp.sum())
+            rank_total += rs
+            for uu in range(Sp):
+                for vv in range(Sp):
+                    tp = uu * Sp + vv
+                    t = (j * Sp + uu) * S + (i * Sp + vv)
+                    r = rf[t]
+                    assert rp[tp] == r, (i, j, uu, vv)
+                    if r:
+                        assert np.array_equal(Up[op[tp] * b:(op[tp] + r) * b], Uf[of[t] * b:(of[t] + r) * b])
+                        assert np.array_equal(Vp[op[tp] * b:(op[tp] + r) * b], Vf[of[t] * b:(of[t] + r) * b])
+            if i == j:
+                assert np.array_equal(Dp, Df[i * Sp * b * b:(i + 1) * Sp * b * b])
+            m.free()
+    assert rank_total == int(rf.sum())
+
+
+def test_tlr_piece_rejections(pair, ctx):
+    pipe, ref, _ = pair
+    for pieces, i, j in [(3, 0, 0), (2, 2, 0), (2, 0, 2), (0, 0, 0)]:
+        with pytest.raises(hmx.HmxError) as ei:
+            hmx.createColumnMajorLRMatrixPiece(ctx, pipe.cfg, pipe.tree, pipe.points, pieces, i, j)
+        assert ei.value.status == hmx.HMX_EINVAL
+    m, _ = hmx.createColumnMajorLRMatrixPiece(ctx, pipe.cfg, pipe.tree, pipe.points, 2, 0, 0)
+    with pytest.raises(hmx.HmxError) as ei:
+        hmx.convertColumnMajorToMorton(ctx, m)
+    assert ei.value.status == hmx.HMX_EINVAL
+    with pytest.raises(hmx.HmxError) as ei:
+        hmx.generateHMatrixFromStruct(ctx, pipe.cfg, pipe.tree, pipe.structure, m)
+    assert ei.value.status == hmx.HMX_EINVAL
+    m.free()
+
+
 def test_hmatrix_levels_vs_oracle(pair, ctx):
     pipe, ref, p = pair
     L = ref.num_levels
