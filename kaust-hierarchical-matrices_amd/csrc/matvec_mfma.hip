@@ -78,14 +78,53 @@ __device__ inline void vzero(T (&out)[N]) {
     for (int i = 0; i < N; ++i) out[i] = (T)0;
 }
 
+// one 16-byte word into out[0 .. 16/sizeof(T)); the non-temporal form for
+// data read once
+template <class T>
+__device__ inline void vload16(const T* __restrict__ p, T* out) {
+    const uint4 v = *reinterpret_cast<const uint4*>(p);
+    __builtin_memcpy(out, &v, 16);
+}
+template <class T>
+__device__ inline void vload16_nt(const T* __restrict__ p, T* out) {
+    const u32x4_t v = __builtin_nontemporal_load(reinterpret_cast<const u32x4_t*>(p));
+    __builtin_memcpy(out, &v, 16);
+}
+
 // z of a precision group: column c (padded, tiles start at multiples of 16)
-// and right-hand side s at block c/VL of NR*VL elements, element s*VL + c%VL.
-// A lane's VL consecutive columns of one right-hand side (its B operand in
-// phase 2) are contiguous, and a 16-column super-step of a tile is one
-// contiguous NR*16-element block.
-template <int VL, int NR>
+// and right-hand side s.  Block c/VL of NR*VL elements holds the VL columns
+// of a lane's B operand in phase 2, split into pieces of ZP columns: piece
+// (c%VL)/ZP at piece*NR*ZP, right-hand side s at s*ZP.  FP32/FP16: 16-byte
+// pieces, so a phase-2 load instruction (one piece per lane) reads 256
+// contiguous bytes per lane group -- whole cache lines -- where one vector of
+// VL columns per lane read half of each line per instruction (mixed phase 2
+// 9.4 -> 7.2 ms with the U copies in fragment order, N = 2^20, 64
+// right-hand sides).  FP64: one 32-byte piece (ZP = VL); the FP64 kernel
+// runs one wave per SIMD and was slower with 16-byte pieces (37.9 -> 40.1 ms).
+template <class T>
+constexpr int z_piece() {
+    return sizeof(T) == 8 ? 4 * Mfma<T>::KL : 16 / (int)sizeof(T);
+}
+template <class T, int NR>
 __device__ inline size_t zt_index(int64_t c, int s) {
-    return (size_t)(c / VL) * (NR * VL) + (size_t)s * VL + (size_t)(c % VL);
+    constexpr int VL = 4 * Mfma<T>::KL, ZP = z_piece<T>();
+    return (size_t)(c / VL) * (NR * VL) + (size_t)((c % VL) / ZP) * (NR * ZP) + (size_t)s * ZP + (size_t)(c % ZP);
+}
+
+// Element (r, c) of a run's row-major U copy (rows r < m, columns c < rpad,
+// rpad a multiple of 16) in fragment order.  16-row blocks of 16*rpad
+// elements, as in plain row-major order, so a leaf's rows start at r*rpad;
+// inside a block, steps of SS = 16*KL columns (the last one w = rpad - c0
+// wide), and inside a step the 16-byte pieces ordered (piece, lane group,
+// row): lane (lg, lc)'s piece p of its VL columns c0 + VL*lg + [0, VL) of row
+// lc sits at 16*c0 + p*16*E16*(w/VL) + (16*lg + lc)*E16.  Each load
+// instruction of phase 2 then reads w*16 contiguous bytes.
+template <class T>
+__host__ __device__ inline size_t urm_index(int r, int c, int rpad) {
+    constexpr int VL = 4 * Mfma<T>::KL, SS = 16 * Mfma<T>::KL, E16 = 16 / (int)sizeof(T);
+    const int c0 = c / SS * SS, w = (rpad - c0 < SS ? rpad - c0 : SS), q = c - c0;
+    return (size_t)(r / 16) * 16 * rpad + (size_t)16 * c0 + (size_t)((q % VL) / E16) * (16 * E16 * (w / VL)) +
+           (size_t)(16 * (q / VL) + r % 16) * E16 + (size_t)(q % E16);
 }
 
 // Right-hand sides in blocks of 2^sh rows (the leaf size): block k holds the
@@ -201,7 +240,7 @@ __global__ void __launch_bounds__(256) k_mf_phase1(const MfItem* __restrict__ it
                     tv4 o;
 #pragma unroll
                     for (int v = 0; v < 4; ++v) o[v] = (T)(acc[g][nb][v] * cs[v]);
-                    *reinterpret_cast<tv4*>(Zt + zt_index<VL, NR>(zc, 16 * nb + lc)) = o;
+                    *reinterpret_cast<tv4*>(Zt + zt_index<T, NR>(zc, 16 * nb + lc)) = o;
                 }
             }
             return;
@@ -219,7 +258,7 @@ __global__ void __launch_bounds__(256) k_mf_phase1(const MfItem* __restrict__ it
                     const ACC cs = colScaleV ? (ACC)colScaleV[it.zidx + col] : (ACC)1;
 #pragma unroll
                     for (int nb = 0; nb < NB; ++nb)
-                        Zt[zt_index<VL, NR>(zc, 16 * nb + lc)] = (T)(acc[g][nb][v] * cs);
+                        Zt[zt_index<T, NR>(zc, 16 * nb + lc)] = (T)(acc[g][nb][v] * cs);
                 }
             }
         return;
@@ -266,8 +305,7 @@ __global__ void __launch_bounds__(256) k_mf_reduce(int64_t nred, int64_t ldz, co
     ACC v = ((acc[0] + acc[1]) + (acc[2] + acc[3])) + ((acc[4] + acc[5]) + (acc[6] + acc[7]));
     const int64_t zc = redCol[g];
     if (colScale) v *= (ACC)colScale[zc];
-    constexpr int VL = 4 * Mfma<T>::KL;
-    Zt[zt_index<VL, NR>(zc, s)] = (T)v;
+    Zt[zt_index<T, NR>(zc, s)] = (T)v;
 }
 
 // ---- phase 2 -----------------------------------------------------------------
@@ -283,29 +321,45 @@ template <int BL, bool UCM, class T, int NB>
 __device__ inline void lr_rows(const MfEntry* __restrict__ ents, int e0, int e1, const T* __restrict__ Z, int rowOff,
                                int lg, int lc, int kvalid, typename Mfma<T>::accv (&acc)[BL / 16][NB]) {
     using M = Mfma<T>;
-    constexpr int KL = M::KL, VL = 4 * KL, SS = 16 * KL, NR = 16 * NB, RG = BL / 16;
+    constexpr int KL = M::KL, VL = 4 * KL, SS = 16 * KL, NR = 16 * NB, RG = BL / 16, E16 = 16 / (int)sizeof(T),
+                  NP = VL / E16;
     if (e0 >= e1) return;
     T a[RG][VL], bz[NB][VL], an[RG][VL], bn[NB][VL];
     auto load = [&](const MfEntry& en, int c0, T (&A)[RG][VL], T (&B)[NB][VL]) {
         const int cc = c0 + VL * lg;
         if (cc < en.rpad) {
+            // fragment-ordered copies (urm_index): piece p of lane (lg, lc)
+            // at 16*c0 + p*16*E16*(w/VL) + (16lg + lc)*E16 of its 16-row block
+            const int pstride = 16 * E16 * (min(SS, en.rpad - c0) / VL);
 #pragma unroll
             for (int rg = 0; rg < RG; ++rg) {
-                const int row = rowOff + 16 * rg + lc;
                 if (UCM) {  // column-major U in place: one element per column, zero past the rank
-                    const T* U = (const T*)en.u + row;
+                    const T* U = (const T*)en.u + rowOff + 16 * rg + lc;
 #pragma unroll
                     for (int q = 0; q < VL; ++q)
                         A[rg][q] = cc + q < en.rank ? __builtin_nontemporal_load(U + (size_t)(cc + q) * en.ld) : (T)0;
                 } else {
-                    vload_f((const T*)en.u + (size_t)row * en.rpad + cc, A[rg]);
+                    const T* U = (const T*)en.u + (size_t)(rowOff / 16 + rg) * 16 * en.rpad + (size_t)16 * c0 +
+                                 (16 * lg + lc) * E16;
+#pragma unroll
+                    for (int p = 0; p < NP; ++p) vload16_nt(U + p * pstride, &A[rg][p * E16]);
                 }
             }
             // z of right-hand sides past kvalid is never used: not loaded
 #pragma unroll
             for (int nb = 0; nb < NB; ++nb) {
-                if (16 * nb + lc < kvalid) vload(Z + zt_index<VL, NR>(en.zcol + cc, 16 * nb + lc), B[nb]);
-                else vzero(B[nb]);
+                if (16 * nb + lc < kvalid) {
+                    constexpr int ZP = z_piece<T>();
+                    const T* zp = Z + zt_index<T, NR>(en.zcol + cc, 16 * nb + lc);
+                    if constexpr (ZP == E16) {
+#pragma unroll
+                        for (int p = 0; p < VL / ZP; ++p) vload16(zp + p * NR * ZP, &B[nb][p * ZP]);
+                    } else {
+                        vload(zp, B[nb]);
+                    }
+                } else {
+                    vzero(B[nb]);
+                }
             }
         } else {
 #pragma unroll
@@ -715,7 +769,8 @@ __global__ void __launch_bounds__(256) k_cvt_cols(int64_t n, const CvCol* __rest
     for (int i = lane; i < d.m; i += 64) V[d.dst + i] = d.src ? (T)(d.src[i] * inv) : (T)0;
 }
 
-// row-major U copy with zero padding: one block per (tile, 32-row tile, 32-col tile)
+// row-major U copy with zero padding, in fragment order (urm_index): one
+// block per (tile, 32-row tile, 32-col tile)
 struct TrTile {
     int64_t colDesc;  // first CvCol of the tile (column c of the tile = colDesc + c)
     int64_t dst;      // element offset of the tile's row-major copy
@@ -739,7 +794,7 @@ __global__ void __launch_bounds__(256) k_cvt_rowmajor(int64_t n, const TrTile* _
     __syncthreads();
     for (int rr = ty; rr < 32; rr += 8) {  // write 32 rows (contiguous columns)
         const int r = t.r0 + rr, c = t.c0 + tx;
-        if (r < t.m && c < t.rpad) U[t.dst + (int64_t)r * t.rpad + c] = (T)sh[tx][rr];
+        if (r < t.m && c < t.rpad) U[t.dst + urm_index<T>(r, c, t.rpad)] = (T)sh[tx][rr];
     }
 }
 
