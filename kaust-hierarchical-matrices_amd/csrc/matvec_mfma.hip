@@ -177,20 +177,52 @@ __global__ void __launch_bounds__(256) k_mf_phase1(const MfItem* __restrict__ it
     // 15.9 ms; FP32/FP16 -0.3 ms).
     constexpr bool PF = sizeof(T) == 8 && NB > 1;
     T a[CG][VL], bx[NB][VL], an[CG][VL], bn[NB][VL];
+    constexpr int E16 = 16 / (int)sizeof(T), NP = VL / E16;
     auto load = [&](int r0, T (&A)[CG][VL], T (&B)[NB][VL]) {
         const int rr = r0 + VL * lg;
         const bool rok = rr < it.rows;
+        if constexpr (sizeof(T) < 8) {
+            // FP32/FP16 copies in fragment order (V column c, row r at
+            // urm_index(c, r, m) of its run; the item points at its first
+            // column block and row chunk), the right-hand sides in blocks
+            // of E16 rows (sh = log2(E16)): every load instruction reads
+            // whole cache lines
+            const int pstride = 16 * E16 * (min(16 * KL, it.rows - r0) / VL);
 #pragma unroll
-        for (int g = 0; g < CG; ++g) {
-            const int col = 16 * g + lc;
-            if (rok && col < it.ncols) vload_f(V + (size_t)col * it.ld + rr, A[g]);
-            else vzero(A[g]);
-        }
-        // right-hand sides past kvalid are zero in Xb: not loaded
+            for (int g = 0; g < CG; ++g) {
+                if (rok && 16 * g + lc < it.ncols) {
+                    const T* vp = V + (size_t)g * 16 * it.ld + (size_t)16 * r0 + (16 * lg + lc) * E16;
 #pragma unroll
-        for (int nb = 0; nb < NB; ++nb) {
-            if (rok && 16 * nb + lc < kvalid) vload(Xb + xb_index<NR>((int64_t)it.xoff + rr, 16 * nb + lc, sh), B[nb]);
-            else vzero(B[nb]);
+                    for (int p = 0; p < NP; ++p) vload16_nt(vp + p * pstride, &A[g][p * E16]);
+                } else {
+                    vzero(A[g]);
+                }
+            }
+#pragma unroll
+            for (int nb = 0; nb < NB; ++nb) {
+                if (rok && 16 * nb + lc < kvalid) {
+#pragma unroll
+                    for (int p = 0; p < NP; ++p)
+                        vload16(Xb + xb_index<NR>((int64_t)it.xoff + rr + p * E16, 16 * nb + lc, sh), &B[nb][p * E16]);
+                } else {
+                    vzero(B[nb]);
+                }
+            }
+        } else {
+#pragma unroll
+            for (int g = 0; g < CG; ++g) {
+                const int col = 16 * g + lc;
+                if (rok && col < it.ncols) vload_f(V + (size_t)col * it.ld + rr, A[g]);
+                else vzero(A[g]);
+            }
+            // right-hand sides past kvalid are zero in Xb: not loaded
+#pragma unroll
+            for (int nb = 0; nb < NB; ++nb) {
+                if (rok && 16 * nb + lc < kvalid)
+                    vload(Xb + xb_index<NR>((int64_t)it.xoff + rr, 16 * nb + lc, sh), B[nb]);
+                else
+                    vzero(B[nb]);
+            }
         }
     };
     if (PF) load(0, a, bx);
@@ -429,8 +461,17 @@ __device__ inline void dense_rows(int leaf, const TD* __restrict__ Drm, const in
         for (int rg = 0; rg < RG; ++rg) vload(Dq + (16 * rg + lc) * BL + cc, A[rg]);
 #pragma unroll
         for (int nb = 0; nb < NB; ++nb) {  // right-hand sides past kvalid are zero in Xb: not loaded
-            if (16 * nb + lc < kvalid) vload(Xb + ((size_t)cl * NR + 16 * nb + lc) * BL + cc, B[nb]);
-            else vzero(B[nb]);
+            if (16 * nb + lc < kvalid) {
+                if constexpr (sizeof(TD) < 8) {  // FP32 right-hand sides in blocks of 4 rows
+#pragma unroll
+                    for (int p = 0; p < EPL / 4; ++p)
+                        vload16(Xb + xb_index<NR>((int64_t)cl * BL + cc + 4 * p, 16 * nb + lc, 2), &B[nb][4 * p]);
+                } else {  // FP64: blocks of one leaf
+                    vload(Xb + ((size_t)cl * NR + 16 * nb + lc) * BL + cc, B[nb]);
+                }
+            } else {
+                vzero(B[nb]);
+            }
         }
     };
     dload(0, a, bx);
@@ -476,7 +517,7 @@ __device__ inline void near_otf(int leaf, int w, const float4* __restrict__ PL, 
                                 float nlog2e, const int32_t* __restrict__ gnearPtr, const MfNear* __restrict__ gnear,
                                 const _Float16* __restrict__ Xh, const _Float16* __restrict__ Xl, int lg, int lc,
                                 int kvalid, v4f (&acc)[BL / 16][NB]) {
-    constexpr int RG = BL / 16, NR = 16 * NB, SH = BL == 64 ? 6 : 5;
+    constexpr int RG = BL / 16, NR = 16 * NB, SH = 3;  // right-hand sides in blocks of 8 rows
     float xl[RG][3];
 #pragma unroll
     for (int rg = 0; rg < RG; ++rg) {
@@ -753,11 +794,16 @@ __global__ void __launch_bounds__(256) k_colstats(int64_t ncols, const ColDesc* 
 }
 
 // V column gather + scale + convert: dst column c = src / scale (one wave per column)
+// FP64: dst is the column's element offset (column-major copy); FP32/FP16:
+// dst is the run's first element and the column sits at urm_index(col, i, m)
+// (fragment order, rows and columns of U's order exchanged)
 struct CvCol {
     const double* src;
     int64_t dst;   // element offset in the group's V buffer
     int32_t m;
     float scale;
+    int32_t col;   // column inside its run (fragment-ordered copies)
+    int32_t pad;
 };
 template <class T>
 __global__ void __launch_bounds__(256) k_cvt_cols(int64_t n, const CvCol* __restrict__ cols, T* __restrict__ V) {
@@ -766,7 +812,11 @@ __global__ void __launch_bounds__(256) k_cvt_cols(int64_t n, const CvCol* __rest
     const int lane = threadIdx.x & 63;
     const CvCol d = cols[c];
     const double inv = 1.0 / (double)d.scale;
-    for (int i = lane; i < d.m; i += 64) V[d.dst + i] = d.src ? (T)(d.src[i] * inv) : (T)0;
+    for (int i = lane; i < d.m; i += 64) {
+        const T v = d.src ? (T)(d.src[i] * inv) : (T)0;
+        if constexpr (sizeof(T) == 8) V[d.dst + i] = v;
+        else V[d.dst + urm_index<T>(d.col, i, d.m)] = v;
+    }
 }
 
 // row-major U copy with zero padding, in fragment order (urm_index): one
@@ -889,6 +939,7 @@ void build_group(Context& c, const HMatrix& h, int S, bool grouped, MfGroup& G, 
     // columns of U and V, zero scales) in both orders, for phase 1's vector
     // stores of z
     const bool pad4 = !vdirect && type != 0;
+    const bool vfrag = !vdirect && type != 0;  // V copies in fragment order (urm_index)
     auto cnt_pad = [&](int t) { return pad4 ? (cnt_of(t) + 3) / 4 * 4 : cnt_of(t); };
     auto scales = [&](int t, int q, float& fu, float& fv) {
         fu = fv = 1.f;
@@ -964,11 +1015,11 @@ void build_group(Context& c, const HMatrix& h, int S, bool grouped, MfGroup& G, 
                 for (int q = 0; q < cnt_of(t); ++q) {
                     float fu, fv;
                     scales(t, q, fu, fv);
-                    ucols.push_back(CvCol{Usrc + (size_t)tiles[t].cols[q] * m, 0, m, fu});
+                    ucols.push_back(CvCol{Usrc + (size_t)tiles[t].cols[q] * m, 0, m, fu, 0, 0});
                     colScale.push_back(fu * fv * zmul);
                 }
                 for (int q = cnt_of(t); q < cnt_pad(t); ++q) {
-                    ucols.push_back(CvCol{nullptr, 0, m, 1.f});
+                    ucols.push_back(CvCol{nullptr, 0, m, 1.f, 0, 0});
                     colScale.push_back(0.f);
                 }
             }
@@ -1001,7 +1052,9 @@ void build_group(Context& c, const HMatrix& h, int S, bool grouped, MfGroup& G, 
         for (int64_t c0 = 0; c0 < cnt; c0 += kMfCols)
             for (int q = 0; q < nch; ++q) {
                 MfItem it;
-                const size_t e = (size_t)c0 * m + (size_t)q * kMfRows;
+                // fragment-ordered copies: the row chunk q starts 16*q*kMfRows
+                // into its column block (urm_index(c0, q*kMfRows, m))
+                const size_t e = (size_t)c0 * m + (size_t)q * kMfRows * (vfrag ? 16 : 1);
                 it.v = vptr ? (const void*)(vptr + e * es) : (const void*)(uintptr_t)(((size_t)vOff + e) * es);
                 it.xoff = (int32_t)(colStart + q * kMfRows);
                 it.rows = rows;
@@ -1033,6 +1086,14 @@ void build_group(Context& c, const HMatrix& h, int S, bool grouped, MfGroup& G, 
                     ++j;
             const int m = R0.m;
             const int64_t k0 = (int64_t)zmap.size(), v0 = vElems;
+            int32_t runCol = 0;
+            // FP64 copies column by column; FP32/FP16 copies in fragment order
+            // from the run's first element (dst = v0, column runCol)
+            auto vcol = [&](const double* src, float scale) {
+                vcols.push_back(vfrag ? CvCol{src, v0, m, scale, runCol, 0} : CvCol{src, vElems, m, scale, 0, 0});
+                ++runCol;
+                vElems += m;
+            };
             for (size_t k = i; k < j; ++k) {
                 const int t = ord[k];
                 const TileRef& R = tiles[t].ref;
@@ -1040,17 +1101,19 @@ void build_group(Context& c, const HMatrix& h, int S, bool grouped, MfGroup& G, 
                 for (int q = 0; q < cnt_of(t); ++q) {
                     float fu, fv;
                     scales(t, q, fu, fv);
-                    if (!vdirect) vcols.push_back(CvCol{Vsrc + (size_t)tiles[t].cols[q] * m, vElems, m, fv});
+                    if (!vdirect) vcol(Vsrc + (size_t)tiles[t].cols[q] * m, fv);
+                    else vElems += m;
                     zmap.push_back((int32_t)(zb[t] + q));
-                    vElems += m;
                 }
                 for (int q = cnt_of(t); q < cnt_pad(t); ++q) {
-                    vcols.push_back(CvCol{nullptr, vElems, m, 1.f});
+                    vcol(nullptr, 1.f);
                     zmap.push_back((int32_t)(zb[t] + q));
-                    vElems += m;
                 }
                 fbytes += 2ull * (uint64_t)cnt_pad(t) * m * es;
             }
+            // a fragment-ordered run takes whole 16-column blocks (the
+            // columns past its last are never read)
+            if (vfrag) vElems = v0 + (int64_t)(runCol + 15) / 16 * 16 * m;
             if (vdirect)
                 emit_items(k0, (int64_t)zmap.size(), m, col_start(R0),
                            (const char*)(R0.lv->V.p + (size_t)R0.lv->h_offsets[R0.t] * m), 0);
@@ -1423,7 +1486,8 @@ template <int NB>
 void run_mixed(Context& c, MfPlan& P, const float* X, int64_t ldx, int kk, float* Y, int64_t ldy, Events& ev) {
     MfGroup& Ghi = P.g[0];
     MfGroup& Glo = P.g[1];
-    const int NR = 16 * NB, sh = P.b == 64 ? 6 : 5;
+    // right-hand sides in blocks of 16 bytes of rows: 8 (FP16), 4 (FP32)
+    const int NR = 16 * NB, sh16 = 3, sh32 = 2;
     const int32_t* gm = P.gather ? P.gmap.p : nullptr;
     ev.rec(c, 0);
     // right-hand sides to FP16 with one scale per column
@@ -1434,17 +1498,18 @@ void run_mixed(Context& c, MfPlan& P, const float* X, int64_t ldx, int kk, float
     const int64_t tot = P.npad * NR;
     const unsigned gx = (unsigned)((tot + 255) / 256);
     if (P.nearOtf)
-        k_x_blocked_split<<<gx, 256, 0, c.stream>>>(tot, P.npad, NR, sh, X, ldx, kk, gm, P.xscale.p, P.X16.p,
+        k_x_blocked_split<<<gx, 256, 0, c.stream>>>(tot, P.npad, NR, sh16, X, ldx, kk, gm, P.xscale.p, P.X16.p,
                                                     P.X16lo.p);
     else
-        k_x_blocked<float, _Float16><<<gx, 256, 0, c.stream>>>(tot, P.npad, NR, sh, X, ldx, kk, gm, P.xscale.p,
+        k_x_blocked<float, _Float16><<<gx, 256, 0, c.stream>>>(tot, P.npad, NR, sh16, X, ldx, kk, gm, P.xscale.p,
                                                                P.X16.p);
     HMX_LAUNCHED();
-    k_x_blocked<float, float><<<gx, 256, 0, c.stream>>>(tot, P.npad, NR, sh, X, ldx, kk, gm, nullptr, (float*)P.Xb.p);
+    k_x_blocked<float, float><<<gx, 256, 0, c.stream>>>(tot, P.npad, NR, sh32, X, ldx, kk, gm, nullptr,
+                                                        (float*)P.Xb.p);
     HMX_LAUNCHED();
     const float* Xb = (const float*)P.Xb.p;
-    phase1<float, 4, NB>(c, Ghi, Xb, sh, kk);
-    phase1<_Float16, 4, NB>(c, Glo, P.X16.p, sh, kk);
+    phase1<float, 4, NB>(c, Ghi, Xb, sh32, kk);
+    phase1<_Float16, 4, NB>(c, Glo, P.X16.p, sh16, kk);
     ev.rec(c, 1);
     reduce_group<float, float>(c, Ghi, NR);
     reduce_group<float, _Float16>(c, Glo, NR);
