@@ -334,7 +334,8 @@ hmx_status hmx_hmatrix_plan_phase_bytes(const hmx_hmatrix* h, int32_t plan, uint
 hmx_status hmx_HMatrixVecMultMixed(hmx_context* ctx, const hmx_hmatrix* h, uint32_t vectorWidth, const float* d_in,
                                    float* d_out);
 /* One MFMA of each element type with the lane layout the kernels assume:
-   D(16x16) = A(16xK) B(Kx16), K = 4 (type 0 f64, 1 f32) or 16 (type 2 f16);
+   D(16x16) = A(16xK) B(Kx16), K = 4 (type 0 f64, 1 f32), 16 (type 2 f16) or
+   32 (type 3 f16);
    A, B, D row-major FP64 device buffers.  A self-check of the matrix-core
    operand layout. */
 hmx_status hmx_selftest_mfma(hmx_context* ctx, int32_t type, const double* d_A, const double* d_B, double* d_D);

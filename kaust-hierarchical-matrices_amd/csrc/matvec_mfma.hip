@@ -3,12 +3,15 @@
 //
 //   phase 1  Z = V^T X per tile: one wave per (tile, 64 columns, 1024 rows),
 //            D(16 cols x 16 rhs) += A(16 cols x K rows) B(K rows x 16 rhs);
-//            the K index runs along V's contiguous rows, so every lane loads
-//            one 16/32-byte vector per operand per four MFMAs.
+//            the K index runs along V's rows.  FP64: V in place, one 32-byte
+//            vector per operand and lane per four MFMAs; FP32/FP16: V copied
+//            in fragment order and the right-hand sides in 16-byte row
+//            blocks, so every load instruction reads whole cache lines, and
+//            FP16 on the 16x16x32 shape.
 //   reduce   chunk partials summed in a fixed order, scaled, converted to the
 //            compute type and stored transposed ([rhs][column]).
 //   phase 2  one wave per leaf (32 or 64 rows): Y_leaf = D X_leaf + sum_t U_t[leaf rows] Z_t,
-//            A = the leaf's rows of U (row-major copy or the level's
+//            A = the leaf's rows of U (fragment-ordered copy or the level's
 //            column-major U in place), B = transposed Z.
 //
 // Mixed precision: the columns of every tile are split by a global energy
@@ -144,6 +147,10 @@ __global__ void __launch_bounds__(256) k_mf_phase1(const MfItem* __restrict__ it
                                                    T* __restrict__ Zt, const int32_t* __restrict__ zmap,
                                                    const float* __restrict__ colScaleV, int kvalid) {
     using M = Mfma<T>;
+    // the products: FP16 on the 16x16x32 shape (a lane's 16 values of a
+    // step feed two instructions instead of four; the k order inside a step
+    // is the same for both operands), FP32/FP64 on M
+    using MX = typename std::conditional<std::is_same<T, _Float16>::value, Mfma32h, M>::type;
     constexpr int KL = M::KL, VL = 4 * KL, NR = 16 * NB;
     const int lane = threadIdx.x & 63;
     // Items are sorted by the right-hand-side rows they read.  Chunks of
@@ -231,12 +238,12 @@ __global__ void __launch_bounds__(256) k_mf_phase1(const MfItem* __restrict__ it
         if (!PF) load(r0, a, bx);
         if (more) load(r0 + 16 * KL, an, bn);
 #pragma unroll
-        for (int e = 0; e < 4; ++e)
+        for (int e = 0; e < VL / MX::KL; ++e)
 #pragma unroll
             for (int g = 0; g < CG; ++g)
 #pragma unroll
                 for (int nb = 0; nb < NB; ++nb)
-                    acc[g][nb] = M::op(M::make(&a[g][e * KL]), M::make(&bx[nb][e * KL]), acc[g][nb]);
+                    acc[g][nb] = MX::op(MX::make(&a[g][e * MX::KL]), MX::make(&bx[nb][e * MX::KL]), acc[g][nb]);
         if (more) {
 #pragma unroll
             for (int g = 0; g < CG; ++g)
@@ -353,6 +360,7 @@ template <int BL, bool UCM, class T, int NB>
 __device__ inline void lr_rows(const MfEntry* __restrict__ ents, int e0, int e1, const T* __restrict__ Z, int rowOff,
                                int lg, int lc, int kvalid, typename Mfma<T>::accv (&acc)[BL / 16][NB]) {
     using M = Mfma<T>;
+    using MX = typename std::conditional<std::is_same<T, _Float16>::value, Mfma32h, M>::type;  // as in phase 1
     constexpr int KL = M::KL, VL = 4 * KL, SS = 16 * KL, NR = 16 * NB, RG = BL / 16, E16 = 16 / (int)sizeof(T),
                   NP = VL / E16;
     if (e0 >= e1) return;
@@ -414,12 +422,12 @@ __device__ inline void lr_rows(const MfEntry* __restrict__ ents, int e0, int e1,
         const bool more = ne < e1;
         if (more) load(nen, nc, an, bn);
 #pragma unroll
-        for (int q = 0; q < 4; ++q)
+        for (int q = 0; q < VL / MX::KL; ++q)
 #pragma unroll
             for (int rg = 0; rg < RG; ++rg)
 #pragma unroll
                 for (int nb = 0; nb < NB; ++nb)
-                    acc[rg][nb] = M::op(M::make(&a[rg][q * KL]), M::make(&bz[nb][q * KL]), acc[rg][nb]);
+                    acc[rg][nb] = MX::op(MX::make(&a[rg][q * MX::KL]), MX::make(&bz[nb][q * MX::KL]), acc[rg][nb]);
         if (!more) break;
 #pragma unroll
         for (int rg = 0; rg < RG; ++rg)
@@ -506,10 +514,11 @@ __device__ inline void dense_rows(int leaf, const TD* __restrict__ Drm, const in
 // carry errors near 1e-8 absolute; sqrt and exp in FP32 on the hardware
 // instructions (exp(-r/l) = 2^(r * nlog2e)).  K is split into
 // FP16 K_hi + K_lo and the right-hand sides (in units of their max-abs) into
-// X_hi + X_lo, so three FP16 16x16x16 products per step (K_hi X_hi + K_hi X_lo
+// X_hi + X_lo, so three FP16 16x16x32 products per step (K_hi X_hi + K_hi X_lo
 // + K_lo X_hi) carry about 22 bits; the K_lo X_lo term is below 2^-22 of the
 // product.  Lane (lg, lc) evaluates K at rows 16rg + lc and columns
-// 16ks + 4lg + e, e < 4: the A operand layout of the 16x16x16 shape.  Padding
+// 32ks + 8lg + e, e < 8: the A operand layout of the 16x16x32 shape; its B
+// operand is one 16-byte block of 8 right-hand-side rows.  Padding
 // positions hold the point (0, 0, 0): finite kernel values that meet zero
 // right-hand-side entries or land in discarded rows.
 template <int BL, int NB>
@@ -538,31 +547,31 @@ __device__ inline void near_otf(int leaf, int w, const float4* __restrict__ PL, 
         for (int rg = 0; rg < RG; ++rg) {
             xr[rg][0] = xl[rg][0] + d0; xr[rg][1] = xl[rg][1] + d1; xr[rg][2] = xl[rg][2] + d2;
         }
-        for (int ks = 0; ks < BL / 16; ++ks) {
-            const int c0 = cl * BL + 16 * ks + 4 * lg;  // this lane's 4 columns (padded positions)
-            float xc[4][3];
+        for (int ks = 0; ks < BL / 32; ++ks) {
+            const int c0 = cl * BL + 32 * ks + 8 * lg;  // this lane's 8 columns (padded positions)
+            float xc[8][3];
 #pragma unroll
-            for (int e = 0; e < 4; ++e) {
+            for (int e = 0; e < 8; ++e) {
                 const float4 p = PL[c0 + e];
                 xc[e][0] = p.x; xc[e][1] = p.y; xc[e][2] = p.z;
             }
-            v4h bh[NB], bl[NB];
+            v8h bh[NB], bl[NB];
 #pragma unroll
             for (int nb = 0; nb < NB; ++nb) {
                 if (16 * nb + lc < kvalid) {
                     const size_t o = xb_index<NR>((int64_t)c0, 16 * nb + lc, SH);
-                    bh[nb] = *reinterpret_cast<const v4h*>(Xh + o);
-                    bl[nb] = *reinterpret_cast<const v4h*>(Xl + o);
+                    bh[nb] = *reinterpret_cast<const v8h*>(Xh + o);
+                    bl[nb] = *reinterpret_cast<const v8h*>(Xl + o);
                 } else {
-                    bh[nb] = v4h{0, 0, 0, 0};
-                    bl[nb] = v4h{0, 0, 0, 0};
+                    bh[nb] = v8h{0, 0, 0, 0, 0, 0, 0, 0};
+                    bl[nb] = v8h{0, 0, 0, 0, 0, 0, 0, 0};
                 }
             }
 #pragma unroll
             for (int rg = 0; rg < RG; ++rg) {
-                v4h ah, al;
+                v8h ah, al;
 #pragma unroll
-                for (int e = 0; e < 4; ++e) {
+                for (int e = 0; e < 8; ++e) {
                     const float dx = xr[rg][0] - xc[e][0], dy = xr[rg][1] - xc[e][1], dz = xr[rg][2] - xc[e][2];
                     const float r2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
                     // the hardware square root and 2^x (v_sqrt_f32, v_exp_f32:
@@ -576,9 +585,9 @@ __device__ inline void near_otf(int leaf, int w, const float4* __restrict__ PL, 
                 }
 #pragma unroll
                 for (int nb = 0; nb < NB; ++nb) {
-                    acc[rg][nb] = __builtin_amdgcn_mfma_f32_16x16x16f16(ah, bh[nb], acc[rg][nb], 0, 0, 0);
-                    acc[rg][nb] = __builtin_amdgcn_mfma_f32_16x16x16f16(ah, bl[nb], acc[rg][nb], 0, 0, 0);
-                    acc[rg][nb] = __builtin_amdgcn_mfma_f32_16x16x16f16(al, bh[nb], acc[rg][nb], 0, 0, 0);
+                    acc[rg][nb] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah, bh[nb], acc[rg][nb], 0, 0, 0);
+                    acc[rg][nb] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah, bl[nb], acc[rg][nb], 0, 0, 0);
+                    acc[rg][nb] = __builtin_amdgcn_mfma_f32_16x16x32_f16(al, bh[nb], acc[rg][nb], 0, 0, 0);
                 }
             }
         }

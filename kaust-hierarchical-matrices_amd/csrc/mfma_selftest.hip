@@ -5,10 +5,9 @@
 
 namespace hmx {
 
-template <class T>
+template <class T, class M = Mfma<T>>
 __global__ void __launch_bounds__(64) k_mfma_selftest(const double* __restrict__ A, const double* __restrict__ B,
                                                      double* __restrict__ D) {
-    using M = Mfma<T>;
     constexpr int KL = M::KL, K = 4 * KL;
     const int l = threadIdx.x;
     T a[KL], b[KL];
@@ -26,7 +25,8 @@ void mfma_selftest(Context& c, int type, const double* A, const double* B, doubl
     if (type == 0) k_mfma_selftest<double><<<1, 64, 0, c.stream>>>(A, B, D);
     else if (type == 1) k_mfma_selftest<float><<<1, 64, 0, c.stream>>>(A, B, D);
     else if (type == 2) k_mfma_selftest<_Float16><<<1, 64, 0, c.stream>>>(A, B, D);
-    else throw Error(HMX_EINVAL, "mfma selftest: type must be 0 (f64), 1 (f32) or 2 (f16)");
+    else if (type == 3) k_mfma_selftest<_Float16, Mfma32h><<<1, 64, 0, c.stream>>>(A, B, D);
+    else throw Error(HMX_EINVAL, "mfma selftest: type must be 0 (f64), 1 (f32), 2 (f16) or 3 (f16, K = 32)");
     HMX_LAUNCHED();
     HMX_HIP(hipStreamSynchronize(c.stream));
 }

@@ -23,7 +23,7 @@ def ctx(torch_cuda):
     return hmx.Context()
 
 
-@pytest.mark.parametrize("typ,K", [(0, 4), (1, 4), (2, 16)])
+@pytest.mark.parametrize("typ,K", [(0, 4), (1, 4), (2, 16), (3, 32)])
 def test_mfma_operand_layout(torch_cuda, ctx, typ, K):
     rng = np.random.default_rng(typ)
     A = rng.integers(-3, 4, size=(16, K)).astype(np.float64)  # exact in every type
