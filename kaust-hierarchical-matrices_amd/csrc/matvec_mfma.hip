@@ -176,13 +176,14 @@ __global__ void __launch_bounds__(256) k_mf_phase1(const MfItem* __restrict__ it
     for (int g = 0; g < CG; ++g)
 #pragma unroll
         for (int nb = 0; nb < NB; ++nb) acc[g][nb] = typename M::accv{0, 0, 0, 0};
-    // FP64 with 64 right-hand sides: software pipelined, the operands of
-    // super-step r0 + 16*KL are in flight while the MFMAs of super-step r0
-    // run (36.4 ms vs 39.1 without, at N = 2^20).  FP64 with 16 and
-    // FP32/FP16: no register double buffer -- the registers it frees buy more
+    // FP64 with 64 right-hand sides and FP16: software pipelined, the
+    // operands of super-step r0 + 16*KL are in flight while the MFMAs of
+    // super-step r0 run (FP64 36.4 ms vs 39.1 without, at N = 2^20; FP16
+    // 4.4 -> 4.2 ms with the fragment-ordered copies).  FP64 with 16 and
+    // FP32: no register double buffer -- the registers it frees buy more
     // waves per SIMD, which hide the load latency better (FP64 16 RHS 16.6 ->
-    // 15.9 ms; FP32/FP16 -0.3 ms).
-    constexpr bool PF = sizeof(T) == 8 && NB > 1;
+    // 15.9 ms; FP32 slower with it).
+    constexpr bool PF = (sizeof(T) == 8 && NB > 1) || sizeof(T) == 2;
     T a[CG][VL], bx[NB][VL], an[CG][VL], bn[NB][VL];
     constexpr int E16 = 16 / (int)sizeof(T), NP = VL / E16;
     auto load = [&](int r0, T (&A)[CG][VL], T (&B)[NB][VL]) {
@@ -738,26 +739,44 @@ __global__ void __launch_bounds__(256) k_local_points(int64_t npad, int b, const
     if (e % b == 0) O[leaf] = o;
 }
 
-// The right-hand sides divided by their scale sx, in the blocked layout, as
-// FP16 hi = fp16(x/sx) and lo = fp16(x/sx - hi) (zero on padding and past kvalid)
-__global__ void __launch_bounds__(256) k_x_blocked_split(int64_t total, int64_t npad, int NR, int sh,
-                                                         const float* __restrict__ X, int64_t ldx, int kvalid,
-                                                         const int32_t* __restrict__ gmap,
-                                                         const float* __restrict__ sx, _Float16* __restrict__ Xh,
-                                                         _Float16* __restrict__ Xl) {
-    const int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x;
-    if (e >= total) return;
-    const int s = (int)(e / npad);
-    const int64_t p = e % npad;
-    const size_t o = ((((size_t)(p >> sh)) * NR + (size_t)s) << sh) + (size_t)(p & ((1 << sh) - 1));
-    float v = 0.f;
-    if (s < kvalid) {
-        const int64_t g = gmap ? (int64_t)gmap[p] : p;
-        if (g >= 0) v = X[s * ldx + g] / sx[s];
+// All right-hand-side copies of a mixed plan from one read of X: FP32 in
+// blocks of 4 rows (Xb), FP16 hi = fp16(x/sx) and, when Xl is given, lo =
+// fp16(x/sx - hi) in blocks of 8 rows (zero on padding and past kvalid).
+// One workgroup per 32 padded positions: X is read coalesced along the rows
+// into LDS, and each output's 32*NR-element stretch of the blocked layout
+// (contiguous for a 32-row chunk) is written in order with 16-byte stores.
+__global__ void __launch_bounds__(256) k_x_mixed(int64_t npad, int NR, const float* __restrict__ X, int64_t ldx,
+                                                 int kvalid, const int32_t* __restrict__ gmap,
+                                                 const float* __restrict__ sx, float* __restrict__ Xb,
+                                                 _Float16* __restrict__ Xh, _Float16* __restrict__ Xl) {
+    __shared__ float tile[64][33];
+    const int64_t p0 = (int64_t)blockIdx.x * 32;
+    const int t = threadIdx.x;
+    const int pp = t & 31;
+    const int64_t g = gmap ? (int64_t)gmap[p0 + pp] : p0 + pp;
+    for (int s = t >> 5; s < NR; s += 8) tile[s][pp] = (s < kvalid && g >= 0) ? X[s * ldx + g] : 0.f;
+    __syncthreads();
+    const size_t base = (size_t)p0 * NR;
+    // FP32: element idx of the stretch = block (idx/4)/NR of 4 rows, right-hand side (idx/4)%NR
+    for (int q = t; q < 8 * NR; q += 256) {
+        const int s = q % NR, r0 = 4 * (q / NR);
+        *reinterpret_cast<float4*>(Xb + base + 4 * (size_t)q) =
+            make_float4(tile[s][r0], tile[s][r0 + 1], tile[s][r0 + 2], tile[s][r0 + 3]);
     }
-    const _Float16 hi = (_Float16)v;
-    Xh[o] = hi;
-    Xl[o] = (_Float16)(v - (float)hi);
+    // FP16: blocks of 8 rows
+    for (int q = t; q < 4 * NR; q += 256) {
+        const int s = q % NR, r0 = 8 * (q / NR);
+        const float inv = s < kvalid ? sx[s] : 1.f;
+        v8h hi, lo;
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+            const float v = tile[s][r0 + j] / inv;
+            hi[j] = (_Float16)v;
+            lo[j] = (_Float16)(v - (float)hi[j]);
+        }
+        *reinterpret_cast<v8h*>(Xh + base + 8 * (size_t)q) = hi;
+        if (Xl) *reinterpret_cast<v8h*>(Xl + base + 8 * (size_t)q) = lo;
+    }
 }
 
 // product in the padded space (Yp, ld npad) back to the caller's entries
@@ -1504,17 +1523,9 @@ void run_mixed(Context& c, MfPlan& P, const float* X, int64_t ldx, int kk, float
     HMX_LAUNCHED();
     k_colmaxabs_fin<<<kk, 64, 0, c.stream>>>(kXSlices, P.xpart.p, P.xscale.p);
     HMX_LAUNCHED();
-    const int64_t tot = P.npad * NR;
-    const unsigned gx = (unsigned)((tot + 255) / 256);
-    if (P.nearOtf)
-        k_x_blocked_split<<<gx, 256, 0, c.stream>>>(tot, P.npad, NR, sh16, X, ldx, kk, gm, P.xscale.p, P.X16.p,
-                                                    P.X16lo.p);
-    else
-        k_x_blocked<float, _Float16><<<gx, 256, 0, c.stream>>>(tot, P.npad, NR, sh16, X, ldx, kk, gm, P.xscale.p,
-                                                               P.X16.p);
-    HMX_LAUNCHED();
-    k_x_blocked<float, float><<<gx, 256, 0, c.stream>>>(tot, P.npad, NR, sh32, X, ldx, kk, gm, nullptr,
-                                                        (float*)P.Xb.p);
+    HMX_CHECK(P.npad % 32 == 0 && NR <= 64, HMX_EUNSUPPORTED, "mixed plan: padded space not in 32-row chunks");
+    k_x_mixed<<<(unsigned)(P.npad / 32), 256, 0, c.stream>>>(P.npad, NR, X, ldx, kk, gm, P.xscale.p, (float*)P.Xb.p,
+                                                            P.X16.p, P.nearOtf ? P.X16lo.p : nullptr);
     HMX_LAUNCHED();
     const float* Xb = (const float*)P.Xb.p;
     phase1<float, 4, NB>(c, Ghi, Xb, sh32, kk);
