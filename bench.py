@@ -52,6 +52,9 @@ def main():
     ap.add_argument("--assembly", choices=["direct", "tlr"], default="direct",
                     help="direct: every level sampled from the kernel; tlr: the reference's path (S x S TLR "
                          "matrix of leaf tiles, Morton order, H-matrix from the structure)")
+    ap.add_argument("--symmetric", type=int, choices=[0, 1], default=1,
+                    help="direct assembly: compress only the tiles (u, v) with u < v and mirror (v, u) (the kernel "
+                         "and the admissibility are symmetric); 0: every tile on its own, as the reference")
     ap.add_argument("--batch", action="store_true",
                     help="many point clouds instead: batched build, autotuned hipGraph matvec, per-cloud dense check")
     ap.add_argument("--batch-count", type=int, default=256)
@@ -81,7 +84,8 @@ def main():
     t0 = time.perf_counter()
     pipe = hmx.Pipeline(n, d, b, args.tol, adm=args.adm,
                         max_rank_policy=hmx.MAXRANK_FULL if args.policy == "full" else hmx.MAXRANK_REF,
-                        seed=1234, ctx=ctx, timings=timings, direct=args.assembly == "direct")
+                        seed=1234, ctx=ctx, timings=timings, direct=args.assembly == "direct",
+                        symmetric=bool(args.symmetric) and args.assembly == "direct")
     torch.cuda.synchronize()
     build_s = time.perf_counter() - t0
     log(f"build {build_s:.2f}s stages {timings}")
@@ -307,7 +311,10 @@ def main():
                                "TLR (S x S leaf tiles) then H-matrix from the structure",
                    "tlr_tile": ("none: every level is sampled from the kernel; the reference's TLR tile is the "
                                 "leaf (src/kDTree.cu:14), and its S x S TLR matrix at S = N/64 does not fit")
-                               if args.assembly == "direct" else f"the leaf ({b} points)"},
+                               if args.assembly == "direct" else f"the leaf ({b} points)",
+                   "symmetric_tiles": ("tiles (u, v) with u < v compressed, (v, u) mirrored with U and V exchanged"
+                                       if args.symmetric and args.assembly == "direct"
+                                       else "every tile compressed on its own")},
         "ms_per_rhs": round(sec1 * 1e3, 4),
         "rel_err_vs_dense": rel_err1,
         "sampled_rows": args.rows,
@@ -349,7 +356,7 @@ def batch_main(args):
     ctx = hmx.Context()
     log(f"batch: {cnt} clouds, {tot} points")
     t0 = time.perf_counter()
-    bp = hmx.BatchPipeline(clouds, b=32, tol=1e-6, adm="maxDiam", ctx=ctx)
+    bp = hmx.BatchPipeline(clouds, b=32, tol=1e-6, adm="maxDiam", ctx=ctx, symmetric=bool(args.symmetric))
     torch.cuda.synchronize()
     build_s = time.perf_counter() - t0
     log(f"batch build {build_s:.2f}s {bp.timings}")
@@ -382,7 +389,8 @@ def batch_main(args):
            "dtype": "f64", "data": "synthetic",
            "config": {"workload": f"{cnt} clouds, n log-uniform in [{args.batch_nmin}, {args.batch_nmax}], "
                                   "half 2D half 3D, l ~ U[0.05, 0.3], leaf 32, eps 1e-6, maxDiam, 8 RHS FP64",
-                      "total_points": tot},
+                      "total_points": tot,
+                      "symmetric_tiles": "mirrored" if args.symmetric else "every tile compressed"},
            "build_s": round(build_s, 3), "build_stages_s": {k2: round(v, 3) for k2, v in bp.timings.items()},
            "factor_bytes": int(fac), "tuned_xcd_chunk" if k >= 2 else "tuned_chunk": chunk, "tuned_ms": round(tune_ms, 4),
            "rel_err_vs_dense": {"max": max(errs), "median": float(np.median(errs))}}

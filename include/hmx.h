@@ -72,7 +72,15 @@ typedef struct hmx_config {
        (src/constructTLRMatrix.cu:75, src/constructHMatrixFromStruct.cu:80);
        uncapped ranks end at the tile dimension, where the factors are exact. */
     int32_t strictRankCap;
-    int32_t reserved;                /* 0 */
+    /* 0 (default): every admissible tile is compressed on its own, as in the
+       reference.  1: direct assembly (hmx_generateHMatrixDirect and its batch
+       form) uses the symmetry of the kernel and of the admissibility -- tile
+       (v, u) is the transpose of tile (u, v) -- and compresses only the tiles
+       with u < v (column cluster before row cluster); tile (v, u) takes the
+       same rank with U and V exchanged (its V then has the orthonormal
+       columns).  Half the sampling work; the same error bound per tile.  No
+       counterpart in the reference. */
+    int32_t symmetricTiles;
 } hmx_config;
 
 typedef struct hmx_context hmx_context;

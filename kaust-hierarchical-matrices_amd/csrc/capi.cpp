@@ -56,6 +56,7 @@ void check_config(const hmx_config* cfg) {
               "maxRankPolicy must be 0 (reference) or 1 (full)");
     HMX_CHECK(cfg->araR >= 1, HMX_EINVAL, "araR must be >= 1");
     HMX_CHECK(cfg->strictRankCap == 0 || cfg->strictRankCap == 1, HMX_EINVAL, "strictRankCap must be 0 or 1");
+    HMX_CHECK(cfg->symmetricTiles == 0 || cfg->symmetricTiles == 1, HMX_EINVAL, "symmetricTiles must be 0 or 1");
 }
 
 // strictRankCap: a build whose ARA stopped at a rank cap below the tile

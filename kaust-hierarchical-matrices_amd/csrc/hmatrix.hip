@@ -13,6 +13,8 @@
 #include <chrono>
 #include <cstdio>
 #include <cstdlib>
+#include <numeric>
+#include <unordered_map>
 
 #include "hmx_device.h"
 #include "hmx_internal.h"
@@ -810,6 +812,19 @@ __global__ void __launch_bounds__(256) k_lr_pack(int T, int m, int cap, const Lr
     for (size_t e = (size_t)blockIdx.x * 256 + threadIdx.x; e < cnt; e += (size_t)gridDim.x * 256) d[e] = s[e];
 }
 
+// mirrored tiles: tile j takes tile i's factors exchanged, U_j = V_i and
+// V_j = U_i (k columns of m rows each); one grid row per mirrored tile
+__global__ void __launch_bounds__(256) k_lr_mirror(int m, const int64_t* __restrict__ dstOff,
+                                                   const int64_t* __restrict__ srcOff, const int32_t* __restrict__ k,
+                                                   double* __restrict__ U, double* __restrict__ V) {
+    const int j = blockIdx.y;
+    const size_t cnt = (size_t)k[j] * m, d = (size_t)dstOff[j] * m, s = (size_t)srcOff[j] * m;
+    for (size_t e = (size_t)blockIdx.x * 256 + threadIdx.x; e < cnt; e += (size_t)gridDim.x * 256) {
+        U[d + e] = V[s + e];
+        V[d + e] = U[s + e];
+    }
+}
+
 // leaf level: gather TLR tiles by Morton tile index
 __global__ void __launch_bounds__(256) k_leaf_gather(int T, int b, const int32_t* __restrict__ tiles,
                                                      const int32_t* __restrict__ ranksMO,
@@ -1013,15 +1028,21 @@ struct LrOut {
 // Batched ARA over T tiles of dimension m (bus = m/b leaves per side), each
 // with its own threshold, rank cap, random stream and (direct mode) point
 // base and correlation length.  Scratch comes from ws (grown if too small).
+//
+// outSrc (optional, one entry per output tile): output tile j holds
+// compressed tile outSrc[j], or, for outSrc[j] = ~i < 0, tile i's factors
+// with U and V exchanged (a mirrored tile).  Every compressed tile appears
+// once unmirrored.  Without outSrc the outputs are the compressed tiles.
 static void compress_batch(Context& c, int araR, uint64_t seed, const LevelSource& src, int b, int m,
                            const std::vector<TileParam>& tp, const std::vector<int64_t>& first, int firstBlocks,
-                           AraScratch& ws, LrOut& out) {
+                           AraScratch& ws, LrOut& out, const std::vector<int32_t>* outSrc = nullptr) {
     const bool direct = src.mo == nullptr;
     const int bus = m / b;
     const int T = (int)tp.size();
+    const int Tout = outSrc ? (int)outSrc->size() : T;
     const auto tA = std::chrono::steady_clock::now();
-    out.ranks.assign(T, 0);
-    out.offsets.assign(T + 1, 0);
+    out.ranks.assign(Tout, 0);
+    out.offsets.assign(Tout + 1, 0);
     if (T == 0) return;
     HMX_CHECK(!direct || m % 64 == 0, HMX_EUNSUPPORTED, "direct compression needs tile dimensions that are multiples of 64");
     int maxRank = 0;
@@ -1156,15 +1177,31 @@ static void compress_batch(Context& c, int araR, uint64_t seed, const LevelSourc
     HMX_HIP(hipStreamSynchronize(c.stream));
     int64_t acc = 0;
     int caps = 0;
-    out.capped.assign(T, 0);
-    for (int t = 0; t < T; ++t) {
-        out.ranks[t] = hst[t].k;
-        out.offsets[t] = acc;
-        acc += hst[t].k;
-        caps += hst[t].capped;
-        out.capped[t] = (char)hst[t].capped;
+    out.capped.assign(Tout, 0);
+    // offsets of the compressed tiles' own outputs (where A^T Q and the
+    // packed Q go), and the mirrored outputs to fill from them
+    std::vector<int64_t> primOff(T + 1, 0), mDst, mSrc;
+    std::vector<int32_t> mK;
+    for (int j = 0; j < Tout; ++j) {
+        const int32_t q = outSrc ? (*outSrc)[j] : j;
+        const int i = q >= 0 ? q : ~q;
+        out.ranks[j] = hst[i].k;
+        out.offsets[j] = acc;
+        caps += hst[i].capped;
+        out.capped[j] = (char)hst[i].capped;
+        if (q >= 0) primOff[i] = acc;
+        acc += hst[i].k;
     }
-    out.offsets[T] = acc;
+    for (int j = 0; j < Tout; ++j) {
+        const int32_t q = outSrc ? (*outSrc)[j] : j;
+        if (q < 0 && hst[~q].k > 0) {
+            mDst.push_back(out.offsets[j]);
+            mSrc.push_back(primOff[~q]);
+            mK.push_back(hst[~q].k);
+        }
+    }
+    primOff[T] = acc;
+    out.offsets[Tout] = acc;
     out.rankSum = acc;
     out.caps = caps;
     out.kmax = kmax;
@@ -1173,8 +1210,7 @@ static void compress_batch(Context& c, int araR, uint64_t seed, const LevelSourc
     const auto tAlloc = std::chrono::steady_clock::now();
     double sampleTms = 0;
     if (acc > 0) {
-        HMX_HIP(hipMemcpyAsync(ws.off.p, out.offsets.data(), (T + 1) * sizeof(int64_t), hipMemcpyHostToDevice,
-                               c.stream));
+        HMX_HIP(hipMemcpyAsync(ws.off.p, primOff.data(), (T + 1) * sizeof(int64_t), hipMemcpyHostToDevice, c.stream));
         L.boff = ws.off.p;
         if (direct) {
             ws.mark(c, 0);
@@ -1188,8 +1224,8 @@ static void compress_batch(Context& c, int araR, uint64_t seed, const LevelSourc
             }
             const double mm = (double)m * m;
             for (int t = 0; t < T; ++t) {
-                ws.samplerFlops += 2.0 * mm * out.ranks[t];
-                ws.kernelEvals += mm * (double)((out.ranks[t] + 63) / 64);
+                ws.samplerFlops += 2.0 * mm * hst[t].k;
+                ws.kernelEvals += mm * (double)((hst[t].k + 63) / 64);
             }
         } else if (b <= 32) {
             launch_sampleT_tlr<32>(c, L, kmax, bus, T, out.V.p);
@@ -1199,6 +1235,19 @@ static void compress_batch(Context& c, int araR, uint64_t seed, const LevelSourc
         dim3 g(std::min<unsigned>(1024, (unsigned)(((size_t)m * kmax + 255) / 256 + 1)), T);
         k_lr_pack<<<g, 256, 0, c.stream>>>(T, m, capQ, ws.st.p, ws.off.p, ws.Q.p, out.U.p);
         HMX_LAUNCHED();
+        if (!mDst.empty()) {
+            const int nm = (int)mDst.size();
+            DBuf<int64_t> dd(nm), ds(nm);
+            DBuf<int32_t> dk(nm);
+            HMX_HIP(hipMemcpyAsync(dd.p, mDst.data(), nm * sizeof(int64_t), hipMemcpyHostToDevice, c.stream));
+            HMX_HIP(hipMemcpyAsync(ds.p, mSrc.data(), nm * sizeof(int64_t), hipMemcpyHostToDevice, c.stream));
+            HMX_HIP(hipMemcpyAsync(dk.p, mK.data(), nm * sizeof(int32_t), hipMemcpyHostToDevice, c.stream));
+            const int kmaxM = *std::max_element(mK.begin(), mK.end());
+            dim3 gm(std::min<unsigned>(1024, (unsigned)(((size_t)m * kmaxM + 255) / 256)), (unsigned)nm);
+            k_lr_mirror<<<gm, 256, 0, c.stream>>>(m, dd.p, ds.p, dk.p, out.U.p, out.V.p);
+            HMX_LAUNCHED();
+            HMX_HIP(hipStreamSynchronize(c.stream));  // before dd, ds, dk go
+        }
         HMX_HIP(hipStreamSynchronize(c.stream));
         if (direct) ws.add_time();
     }
@@ -1240,6 +1289,33 @@ static int first_sweep_blocks(std::vector<int32_t> ranks, int araR) {
     return std::max(1, std::min(4, (ranks[q] + araR + kB16 - 1) / kB16));
 }
 
+// Symmetric kernels and admissibility: tile (u, v) of a level (column
+// cluster u, row cluster v) is the transpose of tile (v, u).  The tiles to
+// compress are those with u < v, those with u == v and any whose partner is
+// missing (prim: their indices in the level); outSrc maps every tile of the
+// level to its compressed tile (compress_batch), the partners mirrored.
+static void mirror_plan(const std::vector<int32_t>& tiles, std::vector<int>& prim, std::vector<int32_t>& outSrc) {
+    const int T = (int)tiles.size();
+    std::unordered_map<int32_t, int> pos;
+    pos.reserve((size_t)T * 2);
+    for (int t = 0; t < T; ++t) pos[tiles[t]] = t;
+    auto partner = [&](int t) {
+        const uint32_t code = (uint32_t)tiles[t];
+        const auto it = pos.find((int32_t)morton_xy(compact16(code >> 1), compact16(code)));
+        return it == pos.end() ? -1 : it->second;
+    };
+    std::vector<int> idx(T, -1);
+    prim.clear();
+    for (int t = 0; t < T; ++t) {
+        const uint32_t code = (uint32_t)tiles[t];
+        if (compact16(code) > compact16(code >> 1) && partner(t) >= 0) continue;  // mirrored
+        idx[t] = (int)prim.size();
+        prim.push_back(t);
+    }
+    outSrc.assign(T, 0);
+    for (int t = 0; t < T; ++t) outSrc[t] = idx[t] >= 0 ? idx[t] : ~idx[partner(t)];
+}
+
 static void compress_level(Context& c, const hmx_config& cfg, const LevelSource& src, int b,
                            const std::vector<int32_t>& tiles, int level, int numLevels, float tol, int maxRank,
                            int firstBlocks, AraScratch& ws, HLevel& out) {
@@ -1256,14 +1332,23 @@ static void compress_level(Context& c, const hmx_config& cfg, const LevelSource&
     out.h_ranks.assign(T, 0);
     out.h_offsets.assign(T + 1, 0);
     if (T == 0) return;
-    std::vector<TileParam> tp(T);
-    std::vector<int64_t> first(T);
-    for (int t = 0; t < T; ++t) {
-        tp[t] = tile_param((double)tol, src.ell, 0, level, t, maxRank, tiles[t]);
-        first[t] = (int64_t)tiles[t] * bus * bus;
+    std::vector<TileParam> tp;
+    std::vector<int64_t> first;
+    std::vector<int> prim;
+    std::vector<int32_t> outSrc;
+    const bool mirror = direct && cfg.symmetricTiles;
+    if (mirror) {
+        mirror_plan(tiles, prim, outSrc);
+    } else {
+        prim.resize(T);
+        std::iota(prim.begin(), prim.end(), 0);
+    }
+    for (int t : prim) {  // a compressed tile keeps its own random stream (its index t in the level)
+        tp.push_back(tile_param((double)tol, src.ell, 0, level, t, maxRank, tiles[t]));
+        first.push_back((int64_t)tiles[t] * bus * bus);
     }
     LrOut r;
-    compress_batch(c, cfg.araR, cfg.seed, src, b, m, tp, first, firstBlocks, ws, r);
+    compress_batch(c, cfg.araR, cfg.seed, src, b, m, tp, first, firstBlocks, ws, r, mirror ? &outSrc : nullptr);
     out.h_ranks = std::move(r.ranks);
     out.h_offsets = std::move(r.offsets);
     out.rankSum = r.rankSum;
@@ -1566,12 +1651,27 @@ void build_hmatrix_direct_batch(Context& c, const std::vector<const hmx_config*>
         const int m = grp.first.first, d = grp.first.second;
         const auto tstart = std::chrono::steady_clock::now();
         std::vector<TileParam> tp;
+        std::vector<int32_t> outSrc;  // the level tiles of all jobs, in job order
+        bool anyMirror = false;
         for (const Job& j : grp.second) {
             const hmx_config& cfg = *cfgs[j.inst];
             const HLevel& lv = hs[j.inst]->levels[j.level - 1];
             const double ell = cfg.correlationLength > 0 ? cfg.correlationLength : default_ell(trees[j.inst]->d);
             const auto& tiles = ss[j.inst]->tileIndices[j.level - 1];
-            for (int t = 0; t < (int)tiles.size(); ++t)
+            std::vector<int> prim;
+            std::vector<int32_t> src1;
+            if (cfg.symmetricTiles) {
+                mirror_plan(tiles, prim, src1);
+                anyMirror = true;
+            } else {
+                prim.resize(tiles.size());
+                std::iota(prim.begin(), prim.end(), 0);
+                src1.resize(tiles.size());
+                std::iota(src1.begin(), src1.end(), 0);
+            }
+            const int32_t cb = (int32_t)tp.size();
+            for (int32_t q : src1) outSrc.push_back(q >= 0 ? q + cb : ~(~q + cb));
+            for (int t : prim)
                 tp.push_back(tile_param((double)lv.tol, ell, pbase[j.inst], j.level, t, lv.maxRank, tiles[t]));
         }
         LevelSource src;
@@ -1579,7 +1679,7 @@ void build_hmatrix_direct_batch(Context& c, const std::vector<const hmx_config*>
         src.d = d;
         LrOut r;
         compress_batch(c, cfgs[0]->araR, cfgs[0]->seed, src, b, m, tp, {}, first_sweep_blocks(leafRanks, cfgs[0]->araR),
-                       ws, r);
+                       ws, r, anyMirror ? &outSrc : nullptr);
         auto gU = std::make_shared<DBuf<double>>(std::move(r.U));
         auto gV = std::make_shared<DBuf<double>>(std::move(r.V));
         size_t t0 = 0;

@@ -15,8 +15,9 @@
 //            column-major U in place), B = transposed Z.
 //
 // Mixed precision: the columns of every tile are split by a global energy
-// budget.  A column j of tile t contributes ||u_j v_j^T||_F = ||v_j|| (U has
-// orthonormal columns).  The columns whose squared norms sum to at most
+// budget.  A column j of tile t contributes ||u_j v_j^T||_F = ||u_j|| ||v_j||
+// (= ||v_j|| for the ARA's orthonormal U, ||u_j|| for a mirrored tile's
+// orthonormal V).  The columns whose squared norms sum to at most
 // (target / (2*2^-11/sqrt(3)))^2 * ||H||_F^2 are stored in FP16 with
 // per-column scales (max-abs of u_j and of v_j) and multiplied on the FP16
 // matrix cores with FP32 accumulation, the rest (and the dense diagonal
@@ -797,28 +798,32 @@ struct ColDesc {
     int32_t m, pad;
 };
 
-// per column: ||v||^2, max|u|, max|v|  (one wave per column)
+// per column: ||u_j v_j^T||_F^2 = ||u||^2 ||v||^2 (||v||^2 for the ARA's
+// orthonormal U, ||u||^2 for a mirrored tile's orthonormal V), max|u|,
+// max|v|  (one wave per column)
 __global__ void __launch_bounds__(256) k_colstats(int64_t ncols, const ColDesc* __restrict__ cols,
-                                                  double* __restrict__ v2, double* __restrict__ umax,
+                                                  double* __restrict__ e2, double* __restrict__ umax,
                                                   double* __restrict__ vmax) {
     const int64_t c = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
     if (c >= ncols) return;
     const int lane = threadIdx.x & 63;
     const ColDesc d = cols[c];
-    double s = 0, mu = 0, mv = 0;
+    double su = 0, sv = 0, mu = 0, mv = 0;
     for (int i = lane; i < d.m; i += 64) {
         const double u = d.u[i], v = d.v[i];
-        s += v * v;
+        su += u * u;
+        sv += v * v;
         mu = fmax(mu, fabs(u));
         mv = fmax(mv, fabs(v));
     }
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) {
-        s += __shfl_xor(s, off, 64);
+        su += __shfl_xor(su, off, 64);
+        sv += __shfl_xor(sv, off, 64);
         mu = fmax(mu, __shfl_xor(mu, off, 64));
         mv = fmax(mv, __shfl_xor(mv, off, 64));
     }
-    if (lane == 0) { v2[c] = s; umax[c] = mu; vmax[c] = mv; }
+    if (lane == 0) { e2[c] = su * sv; umax[c] = mu; vmax[c] = mv; }
 }
 
 // V column gather + scale + convert: dst column c = src / scale (one wave per column)
@@ -1673,7 +1678,8 @@ void hmatrix_make_mixed(Context& c, HMatrix& h, double target) {
         HMX_HIP(hipMemcpyAsync(vm.data(), d_vm.p, nc * 8, hipMemcpyDeviceToHost, c.stream));
         HMX_HIP(hipStreamSynchronize(c.stream));
     }
-    // ||H||_F^2 = sum of column energies + dense blocks
+    // ||H||_F^2 = sum of column energies (one factor of every tile has
+    // orthonormal columns) + dense blocks
     std::vector<double> hd(h.diag.n);
     HMX_HIP(hipMemcpy(hd.data(), h.diag.p, h.diag.bytes(), hipMemcpyDeviceToHost));
     double frob = 0;

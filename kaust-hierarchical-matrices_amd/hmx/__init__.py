@@ -50,7 +50,7 @@ class Config(ctypes.Structure):
         ("maxRankPolicy", ctypes.c_int32),
         ("seed", ctypes.c_uint64),
         ("strictRankCap", ctypes.c_int32),
-        ("reserved", ctypes.c_int32),
+        ("symmetricTiles", ctypes.c_int32),
     ]
 
 
@@ -559,7 +559,7 @@ class BatchPipeline:
     clouds: list of (n, d, ell)."""
 
     def __init__(self, clouds, b=32, tol=1e-6, adm="weak", max_rank_policy=MAXRANK_FULL, seed=1234, ctx=None,
-                 points=None):
+                 points=None, symmetric=False):
         import time
         import torch
         self.ctx = ctx or Context()
@@ -579,7 +579,8 @@ class BatchPipeline:
             self.structures.append(constructHMatrixStructure(self.ctx, n, d, b, adm, 1.0, self.trees[i]))
             self.cfgs.append(default_config(numberOfInputPoints=n, dimensionOfInputPoints=d, bucketSize=b,
                                             lowestLevelTolerance=tol, admissibilityCondition=ADMISSIBILITY.get(adm, adm),
-                                            correlationLength=ell, maxRankPolicy=max_rank_policy, seed=seed))
+                                            correlationLength=ell, maxRankPolicy=max_rank_policy, seed=seed,
+                                            symmetricTiles=int(symmetric)))
         torch.cuda.synchronize()
         t["TREES_AND_STRUCTURES"] = time.perf_counter() - t0
         t0 = time.perf_counter()
@@ -651,14 +652,15 @@ class Pipeline:
     """The reference driver sequence (src/main.cu:46-173) on the GPU."""
 
     def __init__(self, n, d, b, tol, adm="weak", eta=1.0, ell=0.0, max_rank_policy=MAXRANK_REF, seed=1234,
-                 points=None, ctx=None, timings=None, direct=False):
+                 points=None, ctx=None, timings=None, direct=False, symmetric=False):
         import time
         import torch
         self.ctx = ctx or Context()
         self.n, self.d, self.b = n, d, b
         self.cfg = default_config(numberOfInputPoints=n, dimensionOfInputPoints=d, bucketSize=b,
                                   lowestLevelTolerance=tol, admissibilityCondition=ADMISSIBILITY.get(adm, adm),
-                                  eta=eta, correlationLength=ell, maxRankPolicy=max_rank_policy, seed=seed)
+                                  eta=eta, correlationLength=ell, maxRankPolicy=max_rank_policy, seed=seed,
+                                  symmetricTiles=int(symmetric))
         self.ell = ell if ell > 0 else (0.2 if d == 3 else 0.1)
         t = {}
 

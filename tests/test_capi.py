@@ -35,7 +35,7 @@ def test_config_defaults_match_reference_parseargs():
     assert cfg.lowestLevelTolerance == np.float32(1e-5)
     assert (cfg.araBlockSize, cfg.araR, cfg.eta, cfg.seed) == (16, 10, 1.0, 1234)
     # extension: a rank-cap hit is counted, not an error, unless asked for
-    assert (cfg.strictRankCap, cfg.reserved) == (0, 0)
+    assert (cfg.strictRankCap, cfg.symmetricTiles) == (0, 0)
 
 
 def test_alloc_fault_hook_counts_without_a_device():
