@@ -273,57 +273,61 @@ __global__ void __launch_bounds__(256) k_dir_sample_mf(LrLevel L, int splits, do
 
 // B[t][a*m + c] = sum_q K(x_q, x_c) Q[t][a*m + q] for the 16NB basis columns
 // a0 = 16NB*blockIdx.y ... of every tile with k > a0; the blocks past the
-// tile's k are skipped.  A column's sum runs over q in the same order for any
-// NB, so B does not depend on NB.
-template <int D, int NB>
-__global__ void __launch_bounds__(256) k_dir_sampleT_mf(LrLevel L, double* __restrict__ Bout) {
-    constexpr int NQ = kB16 * NB;
-    __shared__ double4 sP[64];
-    __shared__ double sQ[64][NQ + 1];
+// tile's k are skipped.  A wave covers 16CB tile columns: CB = 4 with up to
+// 64 basis columns per pass, CB = 2 with 128 (the same accumulator registers;
+// a tile of rank 65..128 is evaluated once instead of twice).  Q comes
+// through LDS in blocks of QB rows.  Only tiles of rank kLo < k <= kHi are
+// processed.  A column's sum runs over q in the same order for any NB, CB and
+// QB, so B does not depend on them.
+template <int D, int NB, int CB = 4>
+__global__ void __launch_bounds__(256) k_dir_sampleT_mf(LrLevel L, double* __restrict__ Bout, int kLo, int kHi) {
+    constexpr int NQ = kB16 * NB, QB = NB > 4 ? 32 : 64, WR = 16 * CB;
+    __shared__ double4 sP[QB];
+    __shared__ double sQ[QB][NQ + 1];
     const int nw = blockDim.x >> 6;
     const int w = threadIdx.x >> 6, lane = threadIdx.x & 63, lg = lane >> 4, lc = lane & 15;
     const int m = L.m;
-    const int wgPerTile = m / (64 * nw);
+    const int wgPerTile = m / (WR * nw);
     const int t = blockIdx.x / wgPerTile;
     const int k = L.st[t].k;
     const int a0 = blockIdx.y * NQ;
-    if (a0 >= k) return;  // uniform over the workgroup
+    if (a0 >= k || k <= kLo || k > kHi) return;  // uniform over the workgroup
     const int na = min(NQ, k - a0);
-    const int cw = (blockIdx.x % wgPerTile) * 64 * nw + 64 * w;
+    const int cw = (blockIdx.x % wgPerTile) * WR * nw + WR * w;
     const TileParam tpt = L.tp[t];
     const uint32_t code = (uint32_t)tpt.code;
     const double nil = tpt.nil;
     const int64_t rowBase = tpt.pbase + (int64_t)compact16(code >> 1) * m;
     const int64_t colBase = tpt.pbase + (int64_t)compact16(code) * m;
     const double4* P4 = reinterpret_cast<const double4*>(L.P);
-    double xc[4][3], wc[4];
+    double xc[CB][3], wc[CB];
 #pragma unroll
-    for (int cb = 0; cb < 4; ++cb) {
+    for (int cb = 0; cb < CB; ++cb) {
         const double4 p = P4[colBase + cw + 16 * cb + lc];
         xc[cb][0] = p.x; xc[cb][1] = p.y; xc[cb][2] = p.z; wc[cb] = p.w;
     }
     const double* Q = L.Q + (size_t)t * m * L.capQ + (size_t)a0 * m;
-    v4d_t acc[4][NB];
+    v4d_t acc[CB][NB];
 #pragma unroll
-    for (int cb = 0; cb < 4; ++cb)
+    for (int cb = 0; cb < CB; ++cb)
 #pragma unroll
         for (int p = 0; p < NB; ++p) acc[cb][p] = v4d_t{0.0, 0.0, 0.0, 0.0};
-    for (int qb = 0; qb < m; qb += 64) {
+    for (int qb = 0; qb < m; qb += QB) {
         __syncthreads();
-        if (threadIdx.x < 64) sP[threadIdx.x] = P4[rowBase + qb + threadIdx.x];
-        for (int e = threadIdx.x; e < 64 * NQ; e += blockDim.x) {
-            const int a = e >> 6, r = e & 63;
+        if (threadIdx.x < QB) sP[threadIdx.x] = P4[rowBase + qb + threadIdx.x];
+        for (int e = threadIdx.x; e < QB * NQ; e += blockDim.x) {
+            const int a = e / QB, r = e % QB;
             sQ[r][a] = a < na ? Q[(size_t)a * m + qb + r] : 0.0;
         }
         __syncthreads();
-        for (int q4 = 0; q4 < 64; q4 += 4) {
+        for (int q4 = 0; q4 < QB; q4 += 4) {
             const double4 xr = sP[q4 + lg];
             const double xra[3] = {xr.x, xr.y, xr.z};
             double bq[NB];
 #pragma unroll
             for (int p = 0; p < NB; ++p) bq[p] = sQ[q4 + lg][16 * p + lc];
 #pragma unroll
-            for (int cb = 0; cb < 4; ++cb) {
+            for (int cb = 0; cb < CB; ++cb) {
                 const double a = kernel_eval_scaled<D>(xra, xc[cb], nil) * (xr.w * wc[cb]);
 #pragma unroll
                 for (int p = 0; p < NB; ++p)
@@ -333,7 +337,7 @@ __global__ void __launch_bounds__(256) k_dir_sampleT_mf(LrLevel L, double* __res
     }
     double* B = Bout + (size_t)L.boff[t] * m;
 #pragma unroll
-    for (int cb = 0; cb < 4; ++cb)
+    for (int cb = 0; cb < CB; ++cb)
 #pragma unroll
         for (int p = 0; p < NB; ++p)
 #pragma unroll
@@ -902,22 +906,33 @@ static void launch_sample_direct(Context& c, const LrLevel& L, int nb, int split
     }
 }
 
-template <int D, int NB>
-static void launch_sampleT_direct_nb(Context& c, const LrLevel& L, int kmax, double* B) {
+template <int D, int NB, int CB = 4>
+static void launch_sampleT_direct_nb(Context& c, const LrLevel& L, int kmax, double* B, int kLo, int kHi) {
     const int thr = sampler_threads(L.m);
-    const unsigned wgs = (unsigned)((int64_t)L.T * L.m / thr);
-    k_dir_sampleT_mf<D, NB><<<dim3(wgs, (unsigned)((kmax + kB16 * NB - 1) / (kB16 * NB))), thr, 0, c.stream>>>(L, B);
+    // a workgroup covers (thr/64) * 16CB tile columns
+    const unsigned wgs = (unsigned)((int64_t)L.T * L.m / (thr / 64 * 16 * CB));
+    k_dir_sampleT_mf<D, NB, CB>
+        <<<dim3(wgs, (unsigned)((kmax + kB16 * NB - 1) / (kB16 * NB))), thr, 0, c.stream>>>(L, B, kLo, kHi);
     HMX_LAUNCHED();
 }
 
-// one sweep carries up to 64 basis columns
+// One pass carries up to 64 basis columns.  When some tile has more, the
+// tiles of rank <= 64 take that pass and those of rank 65.. a second launch
+// with 128 columns per pass (half the tile columns per wave): one evaluation
+// of their entries instead of two, while the smaller tiles keep the shape
+// that suits them.
 template <int D>
 static void launch_sampleT_direct(Context& c, const LrLevel& L, int kmax, double* B) {
-    switch (std::min(4, (kmax + kB16 - 1) / kB16)) {
-        case 1: launch_sampleT_direct_nb<D, 1>(c, L, kmax, B); break;
-        case 2: launch_sampleT_direct_nb<D, 2>(c, L, kmax, B); break;
-        case 3: launch_sampleT_direct_nb<D, 3>(c, L, kmax, B); break;
-        default: launch_sampleT_direct_nb<D, 4>(c, L, kmax, B); break;
+    constexpr int kAll = 1 << 30;
+    switch (std::min(5, (kmax + kB16 - 1) / kB16)) {
+        case 1: launch_sampleT_direct_nb<D, 1>(c, L, kmax, B, 0, kAll); break;
+        case 2: launch_sampleT_direct_nb<D, 2>(c, L, kmax, B, 0, kAll); break;
+        case 3: launch_sampleT_direct_nb<D, 3>(c, L, kmax, B, 0, kAll); break;
+        case 4: launch_sampleT_direct_nb<D, 4>(c, L, kmax, B, 0, kAll); break;
+        default:
+            launch_sampleT_direct_nb<D, 4>(c, L, 64, B, 0, 64);
+            launch_sampleT_direct_nb<D, 8, 2>(c, L, kmax, B, 64, kAll);
+            break;
     }
 }
 
