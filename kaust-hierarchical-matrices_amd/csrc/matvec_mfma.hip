@@ -27,8 +27,11 @@
 // phase 2 applies the right-hand-side scales to the FP32 sum of the FP16
 // terms.
 #include <algorithm>
+#include <chrono>
 #include <cmath>
+#include <cstdio>
 #include <cstdlib>
+#include <cstring>
 #include <numeric>
 #include <type_traits>
 
@@ -954,6 +957,22 @@ size_t tsize(int type) { return type == 0 ? 8 : (type == 1 ? 4 : 2); }
 //     read the same right-hand-side rows are contiguous, so a phase-1 item
 //     of 64 columns spans tiles and one load of those rows serves all of
 //     them; a per-column map gives each V column its z column.
+// stage timings of the plan builds on stderr when HMX_LOG is set
+struct StageLog {
+    const char* label;
+    bool on = std::getenv("HMX_LOG") != nullptr;
+    std::chrono::steady_clock::time_point t = std::chrono::steady_clock::now();
+    explicit StageLog(const char* l) : label(l) {}
+    void mark(Context& c, const char* what) {
+        if (!on) return;
+        HMX_HIP(hipStreamSynchronize(c.stream));
+        const auto now = std::chrono::steady_clock::now();
+        std::fprintf(stderr, "[hmx] %s: %s %.1f ms\n", label, what,
+                     std::chrono::duration<double, std::milli>(now - t).count());
+        t = now;
+    }
+};
+
 struct GroupTile {
     TileRef ref;
     std::vector<int> cols;  // source column indices in the tile
@@ -962,6 +981,7 @@ struct GroupTile {
 void build_group(Context& c, const HMatrix& h, int S, bool grouped, MfGroup& G, int type,
                  const std::vector<GroupTile>& tiles, bool vdirect, const std::vector<float>* su,
                  const std::vector<float>* sv, const std::vector<int64_t>* colBase, float zmul = 1.f) {
+    StageLog lg("plan group");
     const bool udirect = vdirect;
     const int b = (int)h.b;
     const size_t es = tsize(type);
@@ -1007,6 +1027,16 @@ void build_group(Context& c, const HMatrix& h, int S, bool grouped, MfGroup& G, 
     std::vector<std::vector<MfEntry>> grpE(grouped ? S / 4 : 0);
     std::vector<CvCol> vcols, ucols;
     std::vector<TrTile> trt;
+    {
+        size_t cols = 0;
+        for (int t = 0; t < T; ++t) cols += (size_t)cnt_pad(t);
+        if (!vdirect) {
+            ucols.reserve(cols);
+            vcols.reserve(cols);
+        }
+        colScale.reserve(cols + cols / 4);
+        zmap.reserve(cols);
+    }
     std::vector<int64_t> zb(T, -1);  // z column of each tile's first column in this group
     int64_t zcol = 0, slot = 0, vElems = 0, uElems = 0;
     uint64_t fbytes = 0;
@@ -1074,6 +1104,7 @@ void build_group(Context& c, const HMatrix& h, int S, bool grouped, MfGroup& G, 
             i = j;
         }
     }
+    lg.mark(c, "U runs (host)");
     // ---- V and the phase-1 items ----
     // items over group columns [k0, k1), all of tile dimension m and column
     // cluster inCl; V at vptr (in place) or element offset vOff (copy)
@@ -1155,6 +1186,7 @@ void build_group(Context& c, const HMatrix& h, int S, bool grouped, MfGroup& G, 
             i = j;
         }
     }
+    lg.mark(c, "V runs and items (host)");
     // items that read the same right-hand-side block (same tile dimension,
     // column cluster and row chunk) next to each other (already so for the
     // copied layout)
@@ -1210,11 +1242,13 @@ void build_group(Context& c, const HMatrix& h, int S, bool grouped, MfGroup& G, 
         for (size_t k = 0; k < zmap.size(); ++k) colScaleV[k] = colScale[zmap[k]];
         upload(c, G.colScaleV, colScaleV);
     }
+    lg.mark(c, "entries and uploads");
     DBuf<CvCol> d_u, d_v;
     DBuf<TrTile> d_t;
     upload(c, d_u, ucols);
     upload(c, d_v, vcols);
     upload(c, d_t, trt);
+    lg.mark(c, "column descriptor uploads");
     if (!vcols.empty()) {
         const unsigned g = (unsigned)((vcols.size() + 3) / 4);
         if (type == 0) k_cvt_cols<double><<<g, 256, 0, c.stream>>>((int64_t)vcols.size(), d_v.p, (double*)G.V.p);
@@ -1230,6 +1264,7 @@ void build_group(Context& c, const HMatrix& h, int S, bool grouped, MfGroup& G, 
         HMX_LAUNCHED();
     }
     HMX_HIP(hipStreamSynchronize(c.stream));
+    lg.mark(c, "conversion kernels");
 }
 
 void check_bucket(const HMatrix& h) {
@@ -1650,6 +1685,7 @@ void batch_mf_run(Context& c, Batch& B, uint32_t k, const double* d_in, double* 
 int default_xcd_chunk() { return phase1_chunk(); }
 
 void hmatrix_make_mixed(Context& c, HMatrix& h, double target) {
+    StageLog lg("mixed plan");
     check_bucket(h);
     HMX_CHECK(h.leafOffsets.size() == (size_t)h.S + 1, HMX_ESTATE, "H-matrix without leaf offsets");
     HMX_CHECK(target > 0 && target < 1, HMX_EINVAL, "target relative error must be in (0, 1)");
@@ -1659,6 +1695,12 @@ void hmatrix_make_mixed(Context& c, HMatrix& h, double target) {
     // per-column statistics
     std::vector<ColDesc> cd;
     std::vector<int64_t> colBase;
+    {
+        size_t cols = 0;
+        for (const TileRef& R : tiles) cols += (size_t)R.r;
+        cd.reserve(cols);
+        colBase.reserve(tiles.size());
+    }
     for (const TileRef& R : tiles) {
         colBase.push_back((int64_t)cd.size());
         const double* U = R.lv->U.p + (size_t)R.lv->h_offsets[R.t] * R.m;
@@ -1678,6 +1720,7 @@ void hmatrix_make_mixed(Context& c, HMatrix& h, double target) {
         HMX_HIP(hipMemcpyAsync(vm.data(), d_vm.p, nc * 8, hipMemcpyDeviceToHost, c.stream));
         HMX_HIP(hipStreamSynchronize(c.stream));
     }
+    lg.mark(c, "column statistics");
     // ||H||_F^2 = sum of column energies (one factor of every tile has
     // orthonormal columns) + dense blocks
     std::vector<double> hd(h.diag.n);
@@ -1690,23 +1733,51 @@ void hmatrix_make_mixed(Context& c, HMatrix& h, double target) {
     // FP16 storage error of a column: rounding of u, v, x and z (rms 2^-11/sqrt(3) each)
     const double delta = std::ldexp(1.0, -11) / std::sqrt(3.0);
     const double budget = (target / (2.0 * delta)) * (target / (2.0 * delta)) * frob;
-    std::vector<int64_t> order(nc);
-    std::iota(order.begin(), order.end(), 0);
-    std::sort(order.begin(), order.end(), [&](int64_t a, int64_t b2) { return v2[a] < v2[b2]; });
     // columns of tiles applied as dense blocks stay out of the selection
     std::vector<char> dense(nc, 0);
     for (size_t t = 0; t < tiles.size(); ++t)
         if (is_near(h, tiles[t]))
             for (int j = 0; j < tiles[t].r; ++j) dense[colBase[t] + j] = 1;
+    // The columns in increasing energy order while the sum stays within the
+    // budget, without sorting all of them: the bit patterns of non-negative
+    // doubles order like their values, so a histogram over the top 16 bits
+    // (count and energy per bucket) gives the bucket where the running sum
+    // crosses the budget; the buckets below it are taken whole and only that
+    // bucket is sorted (by energy, then index) and taken greedily.
     std::vector<char> lo(nc, 0);
     double used = 0;
-    for (int64_t q = 0; q < nc; ++q) {
-        const int64_t j = order[q];
-        if (dense[j]) continue;
-        if (used + v2[j] > budget) break;
-        used += v2[j];
-        lo[j] = 1;
+    {
+        auto key = [&](int64_t j) {
+            uint64_t u;
+            std::memcpy(&u, &v2[j], 8);
+            return (uint32_t)(u >> 48);
+        };
+        std::vector<double> bsum(1 << 16, 0.0);
+        for (int64_t j = 0; j < nc; ++j)
+            if (!dense[j]) bsum[key(j)] += v2[j];
+        uint32_t cut = 1u << 16;  // first bucket not taken whole
+        for (uint32_t k = 0; k < (1u << 16); ++k) {
+            if (used + bsum[k] > budget) {
+                cut = k;
+                break;
+            }
+            used += bsum[k];
+        }
+        std::vector<std::pair<double, int64_t>> edge;
+        for (int64_t j = 0; j < nc; ++j) {
+            if (dense[j]) continue;
+            const uint32_t k = key(j);
+            if (k < cut) lo[j] = 1;
+            else if (k == cut) edge.emplace_back(v2[j], j);
+        }
+        std::sort(edge.begin(), edge.end());
+        for (const auto& e : edge) {
+            if (used + e.first > budget) break;
+            used += e.first;
+            lo[e.second] = 1;
+        }
     }
+    lg.mark(c, "FP16 column selection");
     std::vector<float> su(nc), sv(nc);
     for (int64_t j = 0; j < nc; ++j) {
         su[j] = um[j] > 0 ? (float)um[j] : 1.f;
@@ -1740,6 +1811,7 @@ void hmatrix_make_mixed(Context& c, HMatrix& h, double target) {
     } else {
         build_near<float>(c, h, (int)h.S, P, tiles);
     }
+    lg.mark(c, "near field");
     std::vector<GroupTile> gh, gl;
     int64_t hi = 0, lc = 0;
     for (size_t t = 0; t < tiles.size(); ++t) {
@@ -1755,8 +1827,11 @@ void hmatrix_make_mixed(Context& c, HMatrix& h, double target) {
         gh.push_back(std::move(a));
         gl.push_back(std::move(b2));
     }
+    lg.mark(c, "column split");
     build_group(c, h, (int)h.S, grouped, P.g[0], 1, gh, false, nullptr, nullptr, nullptr);
+    lg.mark(c, "FP32 group");
     build_group(c, h, (int)h.S, grouped, P.g[1], 2, gl, false, &su, &sv, &colBase, (float)std::ldexp(1.0, zexp));
+    lg.mark(c, "FP16 group");
     const int64_t tot = (int64_t)h.S * h.b * h.b;
     if (!P.nearOtf) {
         P.Drm.alloc(tot * sizeof(float));
