@@ -160,6 +160,13 @@ hmx_status hmx_freeKDTree(hmx_kdtree* tree);
 hmx_status hmx_constructHMatrixStructure(hmx_context* ctx, uint32_t n, uint32_t d, uint32_t bucketSize,
                                          int32_t admissibilityCondition, float eta, const hmx_kdtree* tree,
                                          hmx_structure** structure);
+/* The same structure from a pass on the GPU (the reference's recursion,
+   src/HMatrixStructure.cu:9-90, run level by level over the KD-tree's device
+   boxes): the same tile lists in the same order and the same JDS, bit for
+   bit. */
+hmx_status hmx_constructHMatrixStructureDevice(hmx_context* ctx, uint32_t n, uint32_t d, uint32_t bucketSize,
+                                               int32_t admissibilityCondition, float eta, const hmx_kdtree* tree,
+                                               hmx_structure** structure);
 /* Same, from host bounding boxes (host only, no device needed) */
 hmx_status hmx_structure_from_boxes(uint32_t n, uint32_t d, uint32_t bucketSize, int32_t admissibilityCondition,
                                     float eta, const double* bbmin, const double* bbmax,

@@ -292,6 +292,28 @@ hmx_status hmx_constructHMatrixStructure(hmx_context* ctx, uint32_t n, uint32_t 
     });
 }
 
+hmx_status hmx_constructHMatrixStructureDevice(hmx_context* ctx, uint32_t n, uint32_t d, uint32_t bucketSize,
+                                               int32_t admissibilityCondition, float eta, const hmx_kdtree* tree,
+                                               hmx_structure** structure) {
+    if (!ctx) return HMX_EINVAL;
+    return guard(ctx, [&] {
+        NEED(tree);
+        NEED(structure);
+        *structure = nullptr;
+        HMX_CHECK(tree->d == d && tree->b == bucketSize, HMX_EINVAL, "structure arguments do not match the tree");
+        HMX_CHECK(upper_pow2(n) == upper_pow2(tree->n), HMX_EINVAL, "structure n does not match the tree");
+        HMX_CHECK(tree->numSegments == (1u << tree->numLevels), HMX_EUNSUPPORTED,
+                  "the H-matrix structure needs 2^levels leaves (a POWER_OF_TWO_ON_LEFT tree whose last "
+                  "power-of-two block is partly empty has fewer)");
+        std::unique_ptr<hmx_structure> s(new hmx_structure());
+        build_structure_device(*ctx, *s, n, d, bucketSize, admissibilityCondition, eta, tree->bbmin.p,
+                               tree->bbmax.p);
+        HMX_CHECK((uint32_t)(1u << (s->numLevels - 1)) == tree->numSegments, HMX_EINVAL,
+                  "structure depth does not match the tree");
+        *structure = s.release();
+    });
+}
+
 hmx_status hmx_structure_from_boxes(uint32_t n, uint32_t d, uint32_t bucketSize, int32_t admissibilityCondition,
                                     float eta, const double* bbmin, const double* bbmax, hmx_structure** structure) {
     return guard(nullptr, [&] {

@@ -395,6 +395,11 @@ void build_kdtree(Context& c, KDTree& t, const double* d_points);
 void build_kdtree_batch(Context& c, const std::vector<KDTree*>& ts, const std::vector<const double*>& pts);
 void build_structure(Structure& s, uint32_t n, uint32_t d, uint32_t b, int adm, float eta, const double* bbmin,
                      const double* bbmax);
+void build_structure_device(Context& c, Structure& s, uint32_t n, uint32_t d, uint32_t b, int adm, float eta,
+                            const double* d_bbmin, const double* d_bbmax);
+void structure_shape(Structure& s, uint32_t n, uint32_t b, int adm);
+void structure_finish(Structure& s);
+void build_jds(Structure::JDS& j, int level, const std::vector<int32_t>& tiles);
 void build_tlr_cm(Context& c, const hmx_config& cfg, const KDTree& t, const double* d_points, TLR& out);
 void build_tlr_piece(Context& c, const hmx_config& cfg, const KDTree& t, const double* d_points, uint32_t pieces,
                      uint32_t rowPiece, uint32_t colPiece, TLR& out);

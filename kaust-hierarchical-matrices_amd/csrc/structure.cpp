@@ -123,8 +123,9 @@ namespace {
 
 }  // namespace
 
-void build_structure(Structure& s, uint32_t n, uint32_t d, uint32_t b, int adm, float eta, const double* bbmin,
-                     const double* bbmax) {
+// argument checks, depth and empty tile lists (constructHMatrixStructure,
+// src/HMatrixStructure.cu:92-105); shared by the host and the device pass
+void structure_shape(Structure& s, uint32_t n, uint32_t b, int adm) {
     HMX_CHECK(b >= 1 && n >= 1, HMX_EINVAL, "structure: n and bucketSize must be positive");
     HMX_CHECK(adm >= 0 && adm <= 2, HMX_EINVAL, "structure: unknown admissibility condition");
     uint32_t n2 = upper_pow2(n);
@@ -135,13 +136,23 @@ void build_structure(Structure& s, uint32_t n, uint32_t d, uint32_t b, int adm, 
     s.numLevels = ctz32(n2 / b) + 1;
     s.numTiles.assign(s.numLevels, 0);
     s.tileIndices.assign(s.numLevels, {});
+}
+
+// tile counts and the per-level JDS once the tile lists are filled
+void structure_finish(Structure& s) {
+    for (int l = 0; l < s.numLevels; ++l) s.numTiles[l] = (int32_t)s.tileIndices[l].size();
+    s.jds.assign(s.numLevels > 1 ? s.numLevels - 1 : 0, {});
+    for (int l = 1; l < s.numLevels; ++l) build_jds(s.jds[l - 1], l, s.tileIndices[l - 1]);
+}
+
+void build_structure(Structure& s, uint32_t n, uint32_t d, uint32_t b, int adm, float eta, const double* bbmin,
+                     const double* bbmax) {
+    structure_shape(s, n, b, adm);
     if (s.numLevels > 1) {
         Walker w{d, s.numLevels - 1, adm, eta, bbmin, bbmax, &s};
         w.rec(0, 0, 0);
     }
-    for (int l = 0; l < s.numLevels; ++l) s.numTiles[l] = (int32_t)s.tileIndices[l].size();
-    s.jds.assign(s.numLevels > 1 ? s.numLevels - 1 : 0, {});
-    for (int l = 1; l < s.numLevels; ++l) build_jds(s.jds[l - 1], l, s.tileIndices[l - 1]);
+    structure_finish(s);
 }
 
 }  // namespace hmx

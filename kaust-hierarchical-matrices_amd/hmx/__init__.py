@@ -77,6 +77,7 @@ _SIGS = [
     ("hmx_reorderVectors", _i32, [_vp, _vp, _u32, _i32, _vp, _vp]),
     ("hmx_freeKDTree", _i32, [_vp]),
     ("hmx_constructHMatrixStructure", _i32, [_vp, _u32, _u32, _u32, _i32, _f32, _vp, _P(_vp)]),
+    ("hmx_constructHMatrixStructureDevice", _i32, [_vp, _u32, _u32, _u32, _i32, _f32, _vp, _P(_vp)]),
     ("hmx_structure_from_boxes", _i32, [_u32, _u32, _u32, _i32, _f32, _vp, _vp, _P(_vp)]),
     ("hmx_structure_info", _i32, [_vp, _P(_i32), _vp]),
     ("hmx_structure_tiles", _i32, [_vp, _i32, _vp]),
@@ -478,10 +479,12 @@ def constructKDTreeBatch(ctx, ns, ds, b, points, divMethod=FULL_TREE):
     return [KDTree(ctypes.c_void_p(ha[k])) for k in range(c)]
 
 
-def constructHMatrixStructure(ctx, n, d, b, adm, eta, tree):
+def constructHMatrixStructure(ctx, n, d, b, adm, eta, tree, device=False):
+    """device=True: the level-synchronous pass on the GPU (the same lists, bit for bit)."""
     h = ctypes.c_void_p()
     a = ADMISSIBILITY[adm] if isinstance(adm, str) else adm
-    _check(lib().hmx_constructHMatrixStructure(ctx.handle, n, d, b, a, eta, tree.handle, ctypes.byref(h)), ctx)
+    f = lib().hmx_constructHMatrixStructureDevice if device else lib().hmx_constructHMatrixStructure
+    _check(f(ctx.handle, n, d, b, a, eta, tree.handle, ctypes.byref(h)), ctx)
     return Structure(h)
 
 

@@ -104,6 +104,39 @@ def test_structure_from_gpu_tree(torch_cuda, ctx, adm, d):
         assert np.array_equal(s.tiles(l), ref[l - 1])
 
 
+@pytest.mark.parametrize("adm,d,b,n", [("weak", 2, 32, 8192), ("boxCenter", 2, 32, 8192), ("maxDiam", 2, 32, 8192),
+                                       ("boxCenter", 3, 64, 5000), ("maxDiam", 3, 32, 70000), ("maxDiam", 1, 32, 4096),
+                                       ("weak", 3, 64, 64), ("maxDiam", 3, 64, 1 << 20)])
+def test_structure_device_pass_equals_host_pass(torch_cuda, ctx, adm, d, b, n):
+    """The level-synchronous GPU pass gives the host recursion's tile lists (same
+    order) and JDS, bit for bit; at small sizes both equal the oracle's."""
+    pts = torch_cuda.empty(n * d, dtype=torch_cuda.float64, device="cuda")
+    hmx.generateDataset(ctx, n, d, 77 + d, pts)
+    tree = hmx.constructKDTree(ctx, n, d, b, pts)
+    import time
+    t0 = time.perf_counter()
+    sh = hmx.constructHMatrixStructure(ctx, n, d, b, adm, 1.0, tree)
+    t1 = time.perf_counter()
+    sd = hmx.constructHMatrixStructure(ctx, n, d, b, adm, 1.0, tree, device=True)
+    t2 = time.perf_counter()
+    print(f"host pass {1e3 * (t1 - t0):.1f} ms, device pass {1e3 * (t2 - t1):.1f} ms")
+    L = sh.num_levels()
+    assert sd.num_levels() == L and np.array_equal(sd.num_tiles(), sh.num_tiles())
+    total = 0
+    for l in range(1, L):
+        th, td = sh.tiles(l), sd.tiles(l)
+        assert np.array_equal(th, td), (l, len(th), len(td))
+        for a, c in zip(sh.jds(l), sd.jds(l)):
+            assert np.array_equal(a, c), l
+        total += len(th)
+    print(f"{adm} d={d} b={b} n={n}: {L} levels, {total} tiles")
+    if 1 < n <= 8192:
+        _, _, rmin, rmax, _ = O.kdtree(pts.cpu().numpy(), n, d, b)
+        ref = O.structure(n, d, b, adm, 1.0, rmin, rmax)
+        for l in range(1, L):
+            assert np.array_equal(sd.tiles(l), ref[l - 1])
+
+
 # ---------------------------------------------------------- TLR + H build --
 @pytest.fixture(scope="module", params=[("weak", 1), ("weak", 0), ("boxCenter", 1)])
 def pair(request, torch_cuda, ctx):
