@@ -41,7 +41,21 @@
 
 namespace hmx {
 
-constexpr int kMfRows = 1024;  // rows per phase-1 item
+constexpr int kMfRows = 1024;  // rows per phase-1 item (FP64)
+
+// rows per phase-1 item of the converted (FP32/FP16) groups: taller items
+// write fewer partial sums of the tall tiles and the reduction reads fewer.
+// Mixed 64 RHS at N = 2^20: 12.28 / 11.85 / 11.55 / 11.89 / 12.57 ms with
+// 1024 / 2048 / 4096 / 8192 / 16384 rows (HMX_MIXED_ITEM_ROWS, a power of two
+// >= 1024, for tuning)
+static int mixed_item_rows() {
+    static const int v = [] {
+        const char* e = std::getenv("HMX_MIXED_ITEM_ROWS");
+        const int r = e ? std::atoi(e) : 4 * kMfRows;
+        return r >= kMfRows && (r & (r - 1)) == 0 ? r : kMfRows;
+    }();
+    return v;
+}
 constexpr int kMfCols = 64;    // columns per phase-1 item
 constexpr int kPad = 16;       // rank padding of the row-major U copies
 constexpr int kXSlices = 64;   // row slices of the right-hand-side max-abs scan
@@ -1109,18 +1123,19 @@ void build_group(Context& c, const HMatrix& h, int S, bool grouped, MfGroup& G, 
     // items over group columns [k0, k1), all of tile dimension m and column
     // cluster inCl; V at vptr (in place) or element offset vOff (copy)
     auto emit_items = [&](int64_t k0, int64_t k1, int m, int64_t colStart, const char* vptr, int64_t vOff) {
-        const int nch = (m + kMfRows - 1) / kMfRows;
-        const int rows = std::min(m, kMfRows);
+        const int itemRows = type == 0 ? kMfRows : mixed_item_rows();
+        const int nch = (m + itemRows - 1) / itemRows;
+        const int rows = std::min(m, itemRows);
         const bool partial = nch > 1 || !h.planDirectZ;
         const int64_t cnt = k1 - k0;
         for (int64_t c0 = 0; c0 < cnt; c0 += kMfCols)
             for (int q = 0; q < nch; ++q) {
                 MfItem it;
-                // fragment-ordered copies: the row chunk q starts 16*q*kMfRows
-                // into its column block (urm_index(c0, q*kMfRows, m))
-                const size_t e = (size_t)c0 * m + (size_t)q * kMfRows * (vfrag ? 16 : 1);
+                // fragment-ordered copies: the row chunk q starts 16*q*itemRows
+                // into its column block (urm_index(c0, q*itemRows, m))
+                const size_t e = (size_t)c0 * m + (size_t)q * itemRows * (vfrag ? 16 : 1);
                 it.v = vptr ? (const void*)(vptr + e * es) : (const void*)(uintptr_t)(((size_t)vOff + e) * es);
-                it.xoff = (int32_t)(colStart + q * kMfRows);
+                it.xoff = (int32_t)(colStart + q * itemRows);
                 it.rows = rows;
                 it.ld = m;
                 it.ncols = (int32_t)std::min<int64_t>(kMfCols, cnt - c0);
