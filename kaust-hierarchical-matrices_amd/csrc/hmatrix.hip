@@ -1309,7 +1309,7 @@ static int first_sweep_blocks(std::vector<int32_t> ranks, int araR) {
 // compress are those with u < v, those with u == v and any whose partner is
 // missing (prim: their indices in the level); outSrc maps every tile of the
 // level to its compressed tile (compress_batch), the partners mirrored.
-static void mirror_plan(const std::vector<int32_t>& tiles, std::vector<int>& prim, std::vector<int32_t>& outSrc) {
+void mirror_plan(const std::vector<int32_t>& tiles, std::vector<int>& prim, std::vector<int32_t>& outSrc) {
     const int T = (int)tiles.size();
     std::unordered_map<int32_t, int> pos;
     pos.reserve((size_t)T * 2);

@@ -420,6 +420,9 @@ struct LeafJob {
     HLevel* out;
 };
 void compress_leaf_tiles_padded(Context& c, uint32_t b, uint32_t d, const double* P, const std::vector<LeafJob>& jobs);
+// symmetric tiles: the tiles of a level to compress (prim) and, per tile, its
+// compressed tile (outSrc[t] = index into prim, or ~index for the transpose)
+void mirror_plan(const std::vector<int32_t>& tiles, std::vector<int>& prim, std::vector<int32_t>& outSrc);
 void build_dense_diagonal_padded(Context& c, uint32_t b, uint32_t d, const double* P, const std::vector<LeafJob>& jobs,
                                  double* diag);
 void hmatrix_save(Context& c, const HMatrix& h, const char* path);

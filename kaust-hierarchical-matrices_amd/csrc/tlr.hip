@@ -563,20 +563,57 @@ void build_tlr_piece(Context& c, const hmx_config& cfg, const KDTree& t, const d
 // padded leaf space P): the same factors, bit for bit, as the TLR matrix's
 // tiles at those positions.  One launch sequence for all jobs; each job's
 // level gets its range of the packed factors as a view.
+// leaf tiles of a symmetric-tiles build laid out in the level: output tile j
+// takes compressed tile src's k columns (b rows each), U and V exchanged when
+// swap is set (the transposed partner); one grid row per output tile
+struct LeafPlace {
+    int64_t dst, src;
+    int32_t k, swap;
+};
+__global__ void __launch_bounds__(256) k_leaf_place(int b, const LeafPlace* __restrict__ pl,
+                                                    const double* __restrict__ Uc, const double* __restrict__ Vc,
+                                                    double* __restrict__ U, double* __restrict__ V) {
+    const LeafPlace q = pl[blockIdx.y];
+    const size_t cnt = (size_t)q.k * b, d = (size_t)q.dst * b, s = (size_t)q.src * b;
+    const double* su = q.swap ? Vc : Uc;
+    const double* sv = q.swap ? Uc : Vc;
+    for (size_t e = (size_t)blockIdx.x * 256 + threadIdx.x; e < cnt; e += (size_t)gridDim.x * 256) {
+        U[d + e] = su[s + e];
+        V[d + e] = sv[s + e];
+    }
+}
+
 void compress_leaf_tiles_padded(Context& c, uint32_t b, uint32_t d, const double* P,
                                 const std::vector<LeafJob>& jobs) {
     const int nj = (int)jobs.size();
     if (nj == 0) return;
     std::vector<TlrCloud> cl(nj);
     std::vector<int32_t> tiles, tileCloud;
+    // symmetric-tiles jobs compress only their tiles (u, v) with u < v (the
+    // others are the transposes); outSrc maps every output tile of every job
+    // to its compressed tile (~i: exchanged)
+    std::vector<int32_t> outSrc;
+    bool anyMirror = false;
     int slotCols = 1;
     for (int j = 0; j < nj; ++j) {
         const LeafJob& J = jobs[j];
         check_tlr_args(*J.cfg, b, J.S);
         cl[j] = tlr_cloud(*J.cfg, b, d, J.S, J.leafBase);
         slotCols = std::max(slotCols, cl[j].maxRank);
-        tiles.insert(tiles.end(), J.tiles->begin(), J.tiles->end());
-        tileCloud.insert(tileCloud.end(), J.tiles->size(), j);
+        const int32_t cb = (int32_t)tiles.size();
+        if (J.cfg->symmetricTiles) {
+            std::vector<int> prim;
+            std::vector<int32_t> src1;
+            mirror_plan(*J.tiles, prim, src1);
+            for (int t : prim) tiles.push_back((*J.tiles)[t]);
+            tileCloud.insert(tileCloud.end(), prim.size(), j);
+            for (int32_t q : src1) outSrc.push_back(q >= 0 ? q + cb : ~(~q + cb));
+            anyMirror = true;
+        } else {
+            tiles.insert(tiles.end(), J.tiles->begin(), J.tiles->end());
+            tileCloud.insert(tileCloud.end(), J.tiles->size(), j);
+            for (int32_t q = 0; q < (int32_t)J.tiles->size(); ++q) outSrc.push_back(cb + q);
+        }
         HLevel& out = *J.out;
         out.numTiles = (int)J.tiles->size();
         out.m = (int)b;
@@ -609,6 +646,51 @@ void compress_leaf_tiles_padded(Context& c, uint32_t b, uint32_t d, const double
     compress_tiles(c, b, d, g, slotCols, T, d_r.p, nullptr, *U, *V, rankSum, &hcap);
     HMX_HIP(hipMemcpyAsync(hr.data(), d_r.p, T * sizeof(int32_t), hipMemcpyDeviceToHost, c.stream));
     HMX_HIP(hipStreamSynchronize(c.stream));
+    if (anyMirror) {  // lay out every job's level from the compressed tiles
+        std::vector<int64_t> coff(T + 1, 0);
+        for (uint64_t i = 0; i < T; ++i) coff[i + 1] = coff[i] + hr[i];
+        std::vector<LeafPlace> place;
+        int64_t acc = 0;
+        size_t o = 0;
+        for (int j = 0; j < nj; ++j) {
+            HLevel& out = *jobs[j].out;
+            const int64_t base = acc;
+            for (int q = 0; q < out.numTiles; ++q, ++o) {
+                const int32_t s = outSrc[o];
+                const int i = s >= 0 ? s : ~s;
+                const int32_t rk = hr[i];
+                out.h_ranks[q] = rk;
+                out.h_offsets[q] = acc - base;
+                out.capHits += hcap[i];
+                if (rk > 0) place.push_back(LeafPlace{acc, coff[i], rk, s < 0 ? 1 : 0});
+                acc += rk;
+            }
+            out.h_offsets[out.numTiles] = acc - base;
+            out.rankSum = acc - base;
+        }
+        auto U2 = std::make_shared<DBuf<double>>((size_t)std::max<int64_t>(1, acc) * b);
+        auto V2 = std::make_shared<DBuf<double>>((size_t)std::max<int64_t>(1, acc) * b);
+        if (!place.empty()) {
+            DBuf<LeafPlace> dp(place.size());
+            HMX_HIP(hipMemcpyAsync(dp.p, place.data(), place.size() * sizeof(LeafPlace), hipMemcpyHostToDevice,
+                                   c.stream));
+            for (size_t p0 = 0; p0 < place.size(); p0 += 65535) {  // grid rows per launch
+                const unsigned rows = (unsigned)std::min<size_t>(65535, place.size() - p0);
+                k_leaf_place<<<dim3(std::max(1u, (unsigned)((slotCols * b + 255) / 256)), rows), 256, 0, c.stream>>>(
+                    (int)b, dp.p + p0, U->p, V->p, U2->p, V2->p);
+                HMX_LAUNCHED();
+            }
+            HMX_HIP(hipStreamSynchronize(c.stream));
+        }
+        int64_t base = 0;
+        for (int j = 0; j < nj; ++j) {
+            HLevel& out = *jobs[j].out;
+            out.U.view(U2, (size_t)base * b, (size_t)out.rankSum * b);
+            out.V.view(V2, (size_t)base * b, (size_t)out.rankSum * b);
+            base += out.rankSum;
+        }
+        return;
+    }
     int64_t acc = 0;
     size_t t0 = 0;
     for (int j = 0; j < nj; ++j) {
