@@ -269,6 +269,10 @@ def main():
                           "phase1_GBps": round(mb1 / (msM[0] * 1e-3) / 1e9, 1),
                           "phase2_GBps": round(mb2 / (msM[2] * 1e-3) / 1e9, 1),
                           "call_GBps": round((mb1 + mb2) / secM / 1e9, 1)}
+        tm = traffic_bytes(args, "mixed")
+        if tm is not None:  # counter traffic of the same kernels (committed rocprofv3 passes)
+            mixed_roofline["traffic_phase1"], mixed_roofline["traffic_phase2"] = tm
+            mixed_roofline["traffic_over_algorithmic"] = round((tm[0] + tm[1]) / (mb1 + mb2), 3)
         rhs_block = {
                 "fp64_ms_per_rhs": round(secK * 1e3 / k, 5), "fp64_call_ms": round(secK * 1e3, 3),
                 "fp64_kernel_ms": msK, "fp64_rel_err_vs_dense": rel_errK,
@@ -406,12 +410,15 @@ def traffic_bytes(args, dom):
         return None
     with open(path) as f:
         t = json.load(f)
+    def total(prefixes):
+        found = [next((v for k, v in t.items() if k.startswith(p) and isinstance(v, dict)), None) for p in prefixes]
+        return None if any(v is None for v in found) else int(sum(v["traffic_bytes"] for v in found))
+    if dom == "mixed":  # (phase 1: the conversion and both phase-1 kernels, phase 2)
+        p1 = total(["_ZN3hmx9k_x_mixed", "k_mf_phase1<float, 4, 4>", "_ZN3hmx11k_mf_phase1IDF16_Li4ELi4E"])
+        p2 = total(["_ZN3hmx12k_mf_phase2gILi64ELb0EffDF16_Li4E"])
+        return None if p1 is None or p2 is None else (p1, p2)
     # phase 2 is the per-leaf pass plus the tall-tile (128-row block) pass
-    prefixes = ["k_mv_phase1<1>"] if dom == 1 else ["k_mv_phase2<64, 1>", "k_mv_phase2b<128, 1"]
-    found = [next((v for k, v in t.items() if k.startswith(p) and isinstance(v, dict)), None) for p in prefixes]
-    if any(v is None for v in found):
-        return None
-    return int(sum(v["traffic_bytes"] for v in found))
+    return total(["k_mv_phase1<1>"] if dom == 1 else ["k_mv_phase2<64, 1>", "k_mv_phase2b<128, 1"])
 
 
 def cpu_baseline(pipe, ctx, x1, budget_s, stride, args):

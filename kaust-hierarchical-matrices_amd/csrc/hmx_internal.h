@@ -279,6 +279,7 @@ struct MfGroup {
     DBuf<int32_t> grpPtr;          // [S/4+1]
     DBuf<MfEntry> grpEntries;      // U at the group's first row
     uint64_t factorBytes = 0;      // V + U bytes of this group (unpadded entries)
+    bool vfrag = false;            // V copied in fragment order (else FP64 V in place, column-major)
     DBuf<unsigned char> zpart, zt; // workspaces
     size_t wsNR = 0;               // pass width the workspaces are sized for
     int ztNR = 0;                  // pass width whose z padding columns were last cleared

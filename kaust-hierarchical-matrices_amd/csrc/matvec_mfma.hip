@@ -48,13 +48,16 @@ constexpr int kMfRows = 1024;  // rows per phase-1 item (FP64)
 // Mixed 64 RHS at N = 2^20: 12.28 / 11.85 / 11.55 / 11.89 / 12.57 ms with
 // 1024 / 2048 / 4096 / 8192 / 16384 rows (HMX_MIXED_ITEM_ROWS, a power of two
 // >= 1024, for tuning)
-static int mixed_item_rows() {
+static int mixed_item_rows(int64_t npad) {
     static const int v = [] {
         const char* e = std::getenv("HMX_MIXED_ITEM_ROWS");
-        const int r = e ? std::atoi(e) : 4 * kMfRows;
-        return r >= kMfRows && (r & (r - 1)) == 0 ? r : kMfRows;
+        const int r = e ? std::atoi(e) : 0;
+        return r >= kMfRows && (r & (r - 1)) == 0 ? r : 0;
     }();
-    return v;
+    // a smaller plan keeps 1024-row items, so that there are enough of them
+    // to fill the GPU (N = 65536: mixed 16 RHS 0.20 ms with 1024 rows, 0.37
+    // with 4096)
+    return v ? v : (npad >= ((int64_t)1 << 19) ? 4 * kMfRows : kMfRows);
 }
 constexpr int kMfCols = 64;    // columns per phase-1 item
 constexpr int kPad = 16;       // rank padding of the row-major U copies
@@ -158,7 +161,7 @@ __device__ inline size_t xb_index(int64_t row, int s, int sh) {
 }
 
 // ---- phase 1 ---------------------------------------------------------------
-template <class T, int CG, int NB>
+template <class T, int CG, int NB, bool VF>
 __global__ void __launch_bounds__(256) k_mf_phase1(const MfItem* __restrict__ items, int nItems, int chunk,
                                                    const T* __restrict__ Xb, int sh,
                                                    typename Mfma<T>::acc_t* __restrict__ zpart,
@@ -207,7 +210,28 @@ __global__ void __launch_bounds__(256) k_mf_phase1(const MfItem* __restrict__ it
     auto load = [&](int r0, T (&A)[CG][VL], T (&B)[NB][VL]) {
         const int rr = r0 + VL * lg;
         const bool rok = rr < it.rows;
-        if constexpr (sizeof(T) < 8) {
+        if constexpr (sizeof(T) == 8 && VF) {
+            // FP64 copies in fragment order: whole cache lines per load
+            // instruction for V; the right-hand sides in blocks of a leaf
+            const int pstride = 16 * E16 * (min(16 * KL, it.rows - r0) / VL);
+#pragma unroll
+            for (int g = 0; g < CG; ++g) {
+                if (rok && 16 * g + lc < it.ncols) {
+                    const T* vp = V + (size_t)g * 16 * it.ld + (size_t)16 * r0 + (16 * lg + lc) * E16;
+#pragma unroll
+                    for (int p = 0; p < NP; ++p) vload16_nt(vp + p * pstride, &A[g][p * E16]);
+                } else {
+                    vzero(A[g]);
+                }
+            }
+#pragma unroll
+            for (int nb = 0; nb < NB; ++nb) {
+                if (rok && 16 * nb + lc < kvalid)
+                    vload(Xb + xb_index<NR>((int64_t)it.xoff + rr, 16 * nb + lc, sh), B[nb]);
+                else
+                    vzero(B[nb]);
+            }
+        } else if constexpr (sizeof(T) < 8) {
             // FP32/FP16 copies in fragment order (V column c, row r at
             // urm_index(c, r, m) of its run; the item points at its first
             // column block and row chunk), the right-hand sides in blocks
@@ -844,7 +868,6 @@ __global__ void __launch_bounds__(256) k_colstats(int64_t ncols, const ColDesc* 
 }
 
 // V column gather + scale + convert: dst column c = src / scale (one wave per column)
-// FP64: dst is the column's element offset (column-major copy); FP32/FP16:
 // dst is the run's first element and the column sits at urm_index(col, i, m)
 // (fragment order, rows and columns of U's order exchanged)
 struct CvCol {
@@ -864,8 +887,7 @@ __global__ void __launch_bounds__(256) k_cvt_cols(int64_t n, const CvCol* __rest
     const double inv = 1.0 / (double)d.scale;
     for (int i = lane; i < d.m; i += 64) {
         const T v = d.src ? (T)(d.src[i] * inv) : (T)0;
-        if constexpr (sizeof(T) == 8) V[d.dst + i] = v;
-        else V[d.dst + urm_index<T>(d.col, i, d.m)] = v;
+        V[d.dst + urm_index<T>(d.col, i, d.m)] = v;
     }
 }
 
@@ -1006,7 +1028,7 @@ void build_group(Context& c, const HMatrix& h, int S, bool grouped, MfGroup& G, 
     // columns of U and V, zero scales) in both orders, for phase 1's vector
     // stores of z
     const bool pad4 = !vdirect && type != 0;
-    const bool vfrag = !vdirect && type != 0;  // V copies in fragment order (urm_index)
+    const bool vfrag = !vdirect;  // V copies in fragment order (urm_index)
     auto cnt_pad = [&](int t) { return pad4 ? (cnt_of(t) + 3) / 4 * 4 : cnt_of(t); };
     auto scales = [&](int t, int q, float& fu, float& fv) {
         fu = fv = 1.f;
@@ -1123,7 +1145,7 @@ void build_group(Context& c, const HMatrix& h, int S, bool grouped, MfGroup& G, 
     // items over group columns [k0, k1), all of tile dimension m and column
     // cluster inCl; V at vptr (in place) or element offset vOff (copy)
     auto emit_items = [&](int64_t k0, int64_t k1, int m, int64_t colStart, const char* vptr, int64_t vOff) {
-        const int itemRows = type == 0 ? kMfRows : mixed_item_rows();
+        const int itemRows = type == 0 ? kMfRows : mixed_item_rows((int64_t)S * b);
         const int nch = (m + itemRows - 1) / itemRows;
         const int rows = std::min(m, itemRows);
         const bool partial = nch > 1 || !h.planDirectZ;
@@ -1209,6 +1231,7 @@ void build_group(Context& c, const HMatrix& h, int S, bool grouped, MfGroup& G, 
         std::stable_sort(items.begin(), items.end(), [](const MfItem& a, const MfItem& b2) {
             return a.ld != b2.ld ? a.ld > b2.ld : a.xoff < b2.xoff;
         });
+    G.vfrag = vfrag;
     G.numColsPad = zcol;
     G.numPart = slot;
     G.factorBytes = fbytes;
@@ -1500,9 +1523,12 @@ void phase1(Context& c, MfGroup& G, const T* Xb, int sh, int kvalid, int xcdChun
     const int chunk = xcdChunk >= 0 ? xcdChunk : phase1_chunk();
     unsigned blocks = (unsigned)((G.numItems + 3) / 4);
     if (chunk > 0) blocks = (blocks + 8 * chunk - 1) / (8 * chunk) * (8 * chunk);
-    k_mf_phase1<T, CG, NB><<<blocks, 256, 0, c.stream>>>(G.items.p, G.numItems, chunk, Xb, sh,
-                                                         (typename Mfma<T>::acc_t*)G.zpart.p, (T*)G.zt.p, G.zmap.p,
-                                                         cs, kvalid);
+    // FP64 groups read V in place (column-major) or from a fragment-ordered
+    // copy: separate instantiations, so the in-place kernel's code is not
+    // touched by the other path (one wave per SIMD, sensitive to it)
+    auto k1 = G.vfrag ? k_mf_phase1<T, CG, NB, true> : k_mf_phase1<T, CG, NB, false>;
+    k1<<<blocks, 256, 0, c.stream>>>(G.items.p, G.numItems, chunk, Xb, sh, (typename Mfma<T>::acc_t*)G.zpart.p,
+                                     (T*)G.zt.p, G.zmap.p, cs, kvalid);
     HMX_LAUNCHED();
 }
 
@@ -1747,7 +1773,11 @@ void hmatrix_make_mixed(Context& c, HMatrix& h, double target) {
     frob -= (double)((int64_t)h.S * h.b - (int64_t)h.n);
     // FP16 storage error of a column: rounding of u, v, x and z (rms 2^-11/sqrt(3) each)
     const double delta = std::ldexp(1.0, -11) / std::sqrt(3.0);
-    const double budget = (target / (2.0 * delta)) * (target / (2.0 * delta)) * frob;
+    // aimed at 0.9 of the target: the budget sets the expected error, and a
+    // small plan's error scatters around it (3.004e-6 at a 3e-6 target, n =
+    // 5001, 5 right-hand sides, without the margin)
+    const double aim = 0.9 * target;
+    const double budget = (aim / (2.0 * delta)) * (aim / (2.0 * delta)) * frob;
     // columns of tiles applied as dense blocks stay out of the selection
     std::vector<char> dense(nc, 0);
     for (size_t t = 0; t < tiles.size(); ++t)
