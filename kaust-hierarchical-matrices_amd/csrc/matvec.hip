@@ -195,7 +195,23 @@ __global__ void __launch_bounds__(256) k_mv_phase2(const int32_t* __restrict__ l
     __shared__ double red[4][BL * K];
     const int leaf = blockIdx.x;
     const int l16 = threadIdx.x & 15, cg = threadIdx.x >> 4;
-    const int r0 = RPL * l16;
+    // lane l16 holds rows 2*l16 + {0, 1} (+ 32 for its second pair with
+    // 64-row leaves): each 16-byte load instruction of a column reads 256
+    // contiguous bytes (a lane reading 4 consecutive rows as two loads used
+    // half of every line per instruction)
+    const int r0 = 2 * l16;
+    auto rowq = [&](int q) { return (q >> 1) * 32 + r0 + (q & 1); };
+    auto load_col = [&](const double* __restrict__ colp, double (&out)[RPL]) {
+        double h[2];
+        vload_d<2>(colp + r0, h);
+        out[0] = h[0];
+        out[1] = h[1];
+        if constexpr (RPL == 4) {
+            vload_d<2>(colp + 32 + r0, h);
+            out[2] = h[0];
+            out[3] = h[1];
+        }
+    };
     double acc[RPL][K];
 #pragma unroll
     for (int q = 0; q < RPL; ++q)
@@ -211,7 +227,7 @@ __global__ void __launch_bounds__(256) k_mv_phase2(const int32_t* __restrict__ l
         for (int cc = 0; cc < BL / 16; ++cc) {
             const int c = cg + 16 * cc;
             double dv[RPL];
-            vload_d<RPL>(D + c * BL + r0, dv);
+            load_col(D + c * BL, dv);
 #pragma unroll
             for (int s = 0; s < K; ++s) {
                 const double xv = xl[(size_t)s * ldx + c];
@@ -235,7 +251,7 @@ __global__ void __launch_bounds__(256) k_mv_phase2(const int32_t* __restrict__ l
             for (int p = 0; p < 4; ++p) {
                 const int j = j0 + 16 * p;
                 if (j < en.rank) {
-                    vload_d<RPL>(U + (size_t)j * m + r0, u[p]);
+                    load_col(U + (size_t)j * m, u[p]);
 #pragma unroll
                     for (int s = 0; s < K; ++s) zv[p][s] = zz[(size_t)j * K + s];
                 }
@@ -262,7 +278,7 @@ __global__ void __launch_bounds__(256) k_mv_phase2(const int32_t* __restrict__ l
 #pragma unroll
         for (int s = 0; s < K; ++s)
 #pragma unroll
-            for (int q = 0; q < RPL; ++q) red[wave][s * BL + r0 + q] = acc[q][s];
+            for (int q = 0; q < RPL; ++q) red[wave][s * BL + rowq(q)] = acc[q][s];
     __syncthreads();
     for (int o = threadIdx.x; o < BL * K; o += 256) {
         const double v = (red[0][o] + red[1][o]) + (red[2][o] + red[3][o]);
