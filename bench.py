@@ -414,7 +414,7 @@ def traffic_bytes(args, dom):
         found = [next((v for k, v in t.items() if k.startswith(p) and isinstance(v, dict)), None) for p in prefixes]
         return None if any(v is None for v in found) else int(sum(v["traffic_bytes"] for v in found))
     if dom == "mixed":  # (phase 1: the conversion and both phase-1 kernels, phase 2)
-        p1 = total(["_ZN3hmx9k_x_mixed", "k_mf_phase1<float, 4, 4>", "_ZN3hmx11k_mf_phase1IDF16_Li4ELi4E"])
+        p1 = total(["_ZN3hmx9k_x_mixed", "k_mf_phase1<float, 4, 4", "_ZN3hmx11k_mf_phase1IDF16_Li4ELi4E"])
         p2 = total(["_ZN3hmx12k_mf_phase2gILi64ELb0EffDF16_Li4E"])
         return None if p1 is None or p2 is None else (p1, p2)
     # phase 2 is the per-leaf pass plus the tall-tile (128-row block) pass
