@@ -280,6 +280,8 @@ struct MfGroup {
     DBuf<MfEntry> grpEntries;      // U at the group's first row
     uint64_t factorBytes = 0;      // V + U bytes of this group (unpadded entries)
     bool vfrag = false;            // V copied in fragment order (else FP64 V in place, column-major)
+    bool split = false;            // FP32 group with V as FP16 hi + lo (phase 1 on the FP16 matrix cores)
+    int64_t vLoOff = 0;            // split: the lo copy's offset from the hi copy (FP16 elements)
     DBuf<unsigned char> zpart, zt; // workspaces
     size_t wsNR = 0;               // pass width the workspaces are sized for
     int ztNR = 0;                  // pass width whose z padding columns were last cleared

@@ -161,12 +161,23 @@ __device__ inline size_t xb_index(int64_t row, int s, int sh) {
 }
 
 // ---- phase 1 ---------------------------------------------------------------
-template <class T, int CG, int NB, bool VF>
+// SPLIT (T = FP16, TZ = FP32): the FP32 group of a mixed plan with V held as
+// FP16 hi + lo (V/scale = hi + lo, lo at V + vLoOff) and the right-hand sides
+// as FP16 hi + lo in units of their max-abs (Xb, Xlo): three FP16 products
+// per step (hi*hi + hi*lo + lo*hi, about 22 bits), z in FP32 multiplied by
+// the column and right-hand-side scales.  On the FP16 matrix cores this
+// group reads the same bytes as an FP32 copy, where the FP32 16x16x4 shape
+// at 64 right-hand sides is bound by the matrix cores.
+template <class T, int CG, int NB, bool VF, bool SPLIT = false, class TZ = T>
 __global__ void __launch_bounds__(256) k_mf_phase1(const MfItem* __restrict__ items, int nItems, int chunk,
                                                    const T* __restrict__ Xb, int sh,
                                                    typename Mfma<T>::acc_t* __restrict__ zpart,
-                                                   T* __restrict__ Zt, const int32_t* __restrict__ zmap,
-                                                   const float* __restrict__ colScaleV, int kvalid) {
+                                                   TZ* __restrict__ Zt, const int32_t* __restrict__ zmap,
+                                                   const float* __restrict__ colScaleV, int kvalid,
+                                                   const T* __restrict__ Xlo, int64_t vLoOff,
+                                                   const float* __restrict__ rhsScale) {
+    static_assert(!SPLIT || (std::is_same<T, _Float16>::value && std::is_same<TZ, float>::value),
+                  "split operands: FP16 hi + lo, FP32 z");
     using M = Mfma<T>;
     // the products: FP16 on the 16x16x32 shape (a lane's 16 values of a
     // step feed two instructions instead of four; the k order inside a step
@@ -204,7 +215,7 @@ __global__ void __launch_bounds__(256) k_mf_phase1(const MfItem* __restrict__ it
     // FP32: no register double buffer -- the registers it frees buy more
     // waves per SIMD, which hide the load latency better (FP64 16 RHS 16.6 ->
     // 15.9 ms; FP32 slower with it).
-    constexpr bool PF = (sizeof(T) == 8 && NB > 1) || sizeof(T) == 2;
+    constexpr bool PF = !SPLIT && ((sizeof(T) == 8 && NB > 1) || sizeof(T) == 2);
     T a[CG][VL], bx[NB][VL], an[CG][VL], bn[NB][VL];
     constexpr int E16 = 16 / (int)sizeof(T), NP = VL / E16;
     auto load = [&](int r0, T (&A)[CG][VL], T (&B)[NB][VL]) {
@@ -275,10 +286,38 @@ __global__ void __launch_bounds__(256) k_mf_phase1(const MfItem* __restrict__ it
             }
         }
     };
+    // SPLIT: the lo parts of V and of the right-hand sides, at the same
+    // positions as the hi parts (into an/bn, which no prefetch uses then)
+    auto load_lo = [&](int r0, T (&A)[CG][VL], T (&B)[NB][VL]) {
+        const int rr = r0 + VL * lg;
+        const bool rok = rr < it.rows;
+        const int pstride = 16 * E16 * (min(16 * KL, it.rows - r0) / VL);
+#pragma unroll
+        for (int g = 0; g < CG; ++g) {
+            if (rok && 16 * g + lc < it.ncols) {
+                const T* vp = V + vLoOff + (size_t)g * 16 * it.ld + (size_t)16 * r0 + (16 * lg + lc) * E16;
+#pragma unroll
+                for (int p = 0; p < NP; ++p) vload16_nt(vp + p * pstride, &A[g][p * E16]);
+            } else {
+                vzero(A[g]);
+            }
+        }
+#pragma unroll
+        for (int nb = 0; nb < NB; ++nb) {
+            if (rok && 16 * nb + lc < kvalid) {
+#pragma unroll
+                for (int p = 0; p < NP; ++p)
+                    vload16(Xlo + xb_index<NR>((int64_t)it.xoff + rr + p * E16, 16 * nb + lc, sh), &B[nb][p * E16]);
+            } else {
+                vzero(B[nb]);
+            }
+        }
+    };
     if (PF) load(0, a, bx);
     for (int r0 = 0; r0 < it.rows; r0 += 16 * KL) {
         const bool more = PF && r0 + 16 * KL < it.rows;
         if (!PF) load(r0, a, bx);
+        if constexpr (SPLIT) load_lo(r0, an, bn);
         if (more) load(r0 + 16 * KL, an, bn);
 #pragma unroll
         for (int e = 0; e < VL / MX::KL; ++e)
@@ -287,6 +326,22 @@ __global__ void __launch_bounds__(256) k_mf_phase1(const MfItem* __restrict__ it
 #pragma unroll
                 for (int nb = 0; nb < NB; ++nb)
                     acc[g][nb] = MX::op(MX::make(&a[g][e * MX::KL]), MX::make(&bx[nb][e * MX::KL]), acc[g][nb]);
+        if constexpr (SPLIT) {
+#pragma unroll
+            for (int e = 0; e < VL / MX::KL; ++e)
+#pragma unroll
+                for (int g = 0; g < CG; ++g)
+#pragma unroll
+                    for (int nb = 0; nb < NB; ++nb)
+                        acc[g][nb] = MX::op(MX::make(&a[g][e * MX::KL]), MX::make(&bn[nb][e * MX::KL]), acc[g][nb]);
+#pragma unroll
+            for (int e = 0; e < VL / MX::KL; ++e)
+#pragma unroll
+                for (int g = 0; g < CG; ++g)
+#pragma unroll
+                    for (int nb = 0; nb < NB; ++nb)
+                        acc[g][nb] = MX::op(MX::make(&an[g][e * MX::KL]), MX::make(&bx[nb][e * MX::KL]), acc[g][nb]);
+        }
         if (more) {
 #pragma unroll
             for (int g = 0; g < CG; ++g)
@@ -306,7 +361,7 @@ __global__ void __launch_bounds__(256) k_mf_phase1(const MfItem* __restrict__ it
         // 16-byte store per (g, nb), and a wave's stores fill whole 512-byte
         // stretches of z.
         if (it.direct) {
-            typedef T tv4 __attribute__((ext_vector_type(4)));
+            typedef TZ tv4 __attribute__((ext_vector_type(4)));
 #pragma unroll
             for (int g = 0; g < CG; ++g) {
                 const int col0 = 16 * g + 4 * lg;
@@ -320,9 +375,15 @@ __global__ void __launch_bounds__(256) k_mf_phase1(const MfItem* __restrict__ it
 #pragma unroll
                 for (int nb = 0; nb < NB; ++nb) {
                     tv4 o;
+                    if constexpr (SPLIT) {
+                        const ACC rs = 16 * nb + lc < kvalid ? (ACC)rhsScale[16 * nb + lc] : (ACC)0;
 #pragma unroll
-                    for (int v = 0; v < 4; ++v) o[v] = (T)(acc[g][nb][v] * cs[v]);
-                    *reinterpret_cast<tv4*>(Zt + zt_index<T, NR>(zc, 16 * nb + lc)) = o;
+                        for (int v = 0; v < 4; ++v) o[v] = (TZ)(acc[g][nb][v] * cs[v] * rs);
+                    } else {
+#pragma unroll
+                        for (int v = 0; v < 4; ++v) o[v] = (TZ)(acc[g][nb][v] * cs[v]);
+                    }
+                    *reinterpret_cast<tv4*>(Zt + zt_index<TZ, NR>(zc, 16 * nb + lc)) = o;
                 }
             }
             return;
@@ -340,7 +401,7 @@ __global__ void __launch_bounds__(256) k_mf_phase1(const MfItem* __restrict__ it
                     const ACC cs = colScaleV ? (ACC)colScaleV[it.zidx + col] : (ACC)1;
 #pragma unroll
                     for (int nb = 0; nb < NB; ++nb)
-                        Zt[zt_index<T, NR>(zc, 16 * nb + lc)] = (T)(acc[g][nb][v] * cs);
+                        Zt[zt_index<TZ, NR>(zc, 16 * nb + lc)] = (TZ)(acc[g][nb][v] * cs);
                 }
             }
         return;
@@ -368,7 +429,7 @@ __global__ void __launch_bounds__(256) k_mf_reduce(int64_t nred, int64_t ldz, co
                                                    const int64_t* __restrict__ colPart,
                                                    const int32_t* __restrict__ colNpart,
                                                    const float* __restrict__ colScale, const ACC* __restrict__ zpart,
-                                                   T* __restrict__ Zt) {
+                                                   T* __restrict__ Zt, const float* __restrict__ rhsScale) {
     const int64_t g = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
     const int s = threadIdx.x & 63;
     if (g >= nred || s >= NR) return;
@@ -387,6 +448,7 @@ __global__ void __launch_bounds__(256) k_mf_reduce(int64_t nred, int64_t ldz, co
     ACC v = ((acc[0] + acc[1]) + (acc[2] + acc[3])) + ((acc[4] + acc[5]) + (acc[6] + acc[7]));
     const int64_t zc = redCol[g];
     if (colScale) v *= (ACC)colScale[zc];
+    if (rhsScale) v *= (ACC)rhsScale[s];  // split FP32 group: right-hand sides in units of their max-abs
     Zt[zt_index<T, NR>(zc, s)] = (T)v;
 }
 
@@ -891,6 +953,24 @@ __global__ void __launch_bounds__(256) k_cvt_cols(int64_t n, const CvCol* __rest
     }
 }
 
+// The split copy of an FP32 group's V: hi = fp16(v/scale), lo = fp16(v/scale
+// - hi) (formed in FP64), both in the FP16 fragment order, lo at loOff
+__global__ void __launch_bounds__(256) k_cvt_cols_split(int64_t n, const CvCol* __restrict__ cols,
+                                                        _Float16* __restrict__ V, int64_t loOff) {
+    const int64_t c = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (c >= n) return;
+    const int lane = threadIdx.x & 63;
+    const CvCol d = cols[c];
+    const double inv = 1.0 / (double)d.scale;
+    for (int i = lane; i < d.m; i += 64) {
+        const double x = d.src ? d.src[i] * inv : 0.0;
+        const _Float16 hi = (_Float16)x;
+        const size_t o = (size_t)d.dst + urm_index<_Float16>(d.col, i, d.m);
+        V[o] = hi;
+        V[(size_t)loOff + o] = (_Float16)(x - (double)hi);
+    }
+}
+
 // row-major U copy with zero padding, in fragment order (urm_index): one
 // block per (tile, 32-row tile, 32-col tile)
 struct TrTile {
@@ -1016,8 +1096,10 @@ struct GroupTile {
 
 void build_group(Context& c, const HMatrix& h, int S, bool grouped, MfGroup& G, int type,
                  const std::vector<GroupTile>& tiles, bool vdirect, const std::vector<float>* su,
-                 const std::vector<float>* sv, const std::vector<int64_t>* colBase, float zmul = 1.f) {
+                 const std::vector<float>* sv, const std::vector<int64_t>* colBase, float zmul = 1.f,
+                 bool split = false) {
     StageLog lg("plan group");
+    HMX_CHECK(!split || (type == 1 && !vdirect && su), HMX_ESTATE, "split V: FP32 groups with column scales");
     const bool udirect = vdirect;
     const int b = (int)h.b;
     const size_t es = tsize(type);
@@ -1146,6 +1228,7 @@ void build_group(Context& c, const HMatrix& h, int S, bool grouped, MfGroup& G, 
     // cluster inCl; V at vptr (in place) or element offset vOff (copy)
     auto emit_items = [&](int64_t k0, int64_t k1, int m, int64_t colStart, const char* vptr, int64_t vOff) {
         const int itemRows = type == 0 ? kMfRows : mixed_item_rows((int64_t)S * b);
+        const size_t esv = split ? sizeof(_Float16) : es;  // split: offsets into the hi copy
         const int nch = (m + itemRows - 1) / itemRows;
         const int rows = std::min(m, itemRows);
         const bool partial = nch > 1 || !h.planDirectZ;
@@ -1156,7 +1239,7 @@ void build_group(Context& c, const HMatrix& h, int S, bool grouped, MfGroup& G, 
                 // fragment-ordered copies: the row chunk q starts 16*q*itemRows
                 // into its column block (urm_index(c0, q*itemRows, m))
                 const size_t e = (size_t)c0 * m + (size_t)q * itemRows * (vfrag ? 16 : 1);
-                it.v = vptr ? (const void*)(vptr + e * es) : (const void*)(uintptr_t)(((size_t)vOff + e) * es);
+                it.v = vptr ? (const void*)(vptr + e * es) : (const void*)(uintptr_t)(((size_t)vOff + e) * esv);
                 it.xoff = (int32_t)(colStart + q * itemRows);
                 it.rows = rows;
                 it.ld = m;
@@ -1232,6 +1315,8 @@ void build_group(Context& c, const HMatrix& h, int S, bool grouped, MfGroup& G, 
             return a.ld != b2.ld ? a.ld > b2.ld : a.xoff < b2.xoff;
         });
     G.vfrag = vfrag;
+    G.split = split;
+    G.vLoOff = split ? vElems : 0;  // the lo copy follows the hi copy (2 x 2 bytes per element)
     G.numColsPad = zcol;
     G.numPart = slot;
     G.factorBytes = fbytes;
@@ -1290,6 +1375,7 @@ void build_group(Context& c, const HMatrix& h, int S, bool grouped, MfGroup& G, 
     if (!vcols.empty()) {
         const unsigned g = (unsigned)((vcols.size() + 3) / 4);
         if (type == 0) k_cvt_cols<double><<<g, 256, 0, c.stream>>>((int64_t)vcols.size(), d_v.p, (double*)G.V.p);
+        else if (split) k_cvt_cols_split<<<g, 256, 0, c.stream>>>((int64_t)vcols.size(), d_v.p, (_Float16*)G.V.p, vElems);
         else if (type == 1) k_cvt_cols<float><<<g, 256, 0, c.stream>>>((int64_t)vcols.size(), d_v.p, (float*)G.V.p);
         else k_cvt_cols<_Float16><<<g, 256, 0, c.stream>>>((int64_t)vcols.size(), d_v.p, (_Float16*)G.V.p);
         HMX_LAUNCHED();
@@ -1471,16 +1557,18 @@ void build_fp64_plan_set(Context& c, const std::vector<HMatrix*>& hs, MfPlan& P)
 }
 
 template <class ACC, class T>
-void reduce_group(Context& c, MfGroup& G, int NR) {
+void reduce_group(Context& c, MfGroup& G, int NR, const float* rhsScale = nullptr) {
     if (G.numRed == 0) return;
     const unsigned blocks = (unsigned)((G.numRed + 3) / 4);
     const float* cs = G.colScale.n ? G.colScale.p : nullptr;
     if (NR == 64)
         k_mf_reduce<T, ACC, 64><<<blocks, 256, 0, c.stream>>>(G.numRed, G.numColsPad, G.redCol.p, G.colPart.p,
-                                                              G.colNpart.p, cs, (const ACC*)G.zpart.p, (T*)G.zt.p);
+                                                              G.colNpart.p, cs, (const ACC*)G.zpart.p, (T*)G.zt.p,
+                                                              rhsScale);
     else
         k_mf_reduce<T, ACC, 16><<<blocks, 256, 0, c.stream>>>(G.numRed, G.numColsPad, G.redCol.p, G.colPart.p,
-                                                              G.colNpart.p, cs, (const ACC*)G.zpart.p, (T*)G.zt.p);
+                                                              G.colNpart.p, cs, (const ACC*)G.zpart.p, (T*)G.zt.p,
+                                                              rhsScale);
     HMX_LAUNCHED();
 }
 
@@ -1528,7 +1616,22 @@ void phase1(Context& c, MfGroup& G, const T* Xb, int sh, int kvalid, int xcdChun
     // touched by the other path (one wave per SIMD, sensitive to it)
     auto k1 = G.vfrag ? k_mf_phase1<T, CG, NB, true> : k_mf_phase1<T, CG, NB, false>;
     k1<<<blocks, 256, 0, c.stream>>>(G.items.p, G.numItems, chunk, Xb, sh, (typename Mfma<T>::acc_t*)G.zpart.p,
-                                     (T*)G.zt.p, G.zmap.p, cs, kvalid);
+                                     (T*)G.zt.p, G.zmap.p, cs, kvalid, nullptr, 0, nullptr);
+    HMX_LAUNCHED();
+}
+
+// phase 1 of a mixed plan's FP32 group held as FP16 hi + lo (MfGroup::split)
+template <int NB>
+void phase1_split(Context& c, MfGroup& G, const _Float16* Xh, const _Float16* Xl, int sh, int kvalid,
+                  const float* rhsScale) {
+    if (G.numItems == 0) return;
+    HMX_CHECK(G.split && G.vfrag && G.colScaleV.n && Xl, HMX_ESTATE, "split phase 1: plan not built for it");
+    const int chunk = phase1_chunk();
+    unsigned blocks = (unsigned)((G.numItems + 3) / 4);
+    if (chunk > 0) blocks = (blocks + 8 * chunk - 1) / (8 * chunk) * (8 * chunk);
+    k_mf_phase1<_Float16, 4, NB, true, true, float><<<blocks, 256, 0, c.stream>>>(
+        G.items.p, G.numItems, chunk, Xh, sh, (float*)G.zpart.p, (float*)G.zt.p, G.zmap.p, G.colScaleV.p, kvalid, Xl,
+        G.vLoOff, rhsScale);
     HMX_LAUNCHED();
 }
 
@@ -1606,13 +1709,14 @@ void run_mixed(Context& c, MfPlan& P, const float* X, int64_t ldx, int kk, float
     HMX_LAUNCHED();
     HMX_CHECK(P.npad % 32 == 0 && NR <= 64, HMX_EUNSUPPORTED, "mixed plan: padded space not in 32-row chunks");
     k_x_mixed<<<(unsigned)(P.npad / 32), 256, 0, c.stream>>>(P.npad, NR, X, ldx, kk, gm, P.xscale.p, (float*)P.Xb.p,
-                                                            P.X16.p, P.nearOtf ? P.X16lo.p : nullptr);
+                                                            P.X16.p, P.X16lo.n ? P.X16lo.p : nullptr);
     HMX_LAUNCHED();
     const float* Xb = (const float*)P.Xb.p;
-    phase1<float, 4, NB>(c, Ghi, Xb, sh32, kk);
+    if (Ghi.split) phase1_split<NB>(c, Ghi, P.X16.p, P.X16lo.p, sh16, kk, P.xscale.p);
+    else phase1<float, 4, NB>(c, Ghi, Xb, sh32, kk);
     phase1<_Float16, 4, NB>(c, Glo, P.X16.p, sh16, kk);
     ev.rec(c, 1);
-    reduce_group<float, float>(c, Ghi, NR);
+    reduce_group<float, float>(c, Ghi, NR, Ghi.split ? P.xscale.p : nullptr);
     reduce_group<float, _Float16>(c, Glo, NR);
     ev.rec(c, 2);
     float* Yo = P.gather ? (float*)P.Yp.p : Y;
@@ -1873,7 +1977,18 @@ void hmatrix_make_mixed(Context& c, HMatrix& h, double target) {
         gl.push_back(std::move(b2));
     }
     lg.mark(c, "column split");
-    build_group(c, h, (int)h.S, grouped, P.g[0], 1, gh, false, nullptr, nullptr, nullptr);
+    {
+        // the FP32 group's V as FP16 hi + lo scaled by max|v_j| (phase 1 on the
+        // FP16 matrix cores); U stays FP32 and unscaled.  HMX_MIXED_SPLIT=0
+        // keeps an FP32 copy of V.
+        const char* e = std::getenv("HMX_MIXED_SPLIT");
+        const bool split = !e || std::atoi(e) != 0;
+        const std::vector<float> ones(split ? su.size() : 0, 1.f);
+        if (split)
+            build_group(c, h, (int)h.S, grouped, P.g[0], 1, gh, false, &ones, &sv, &colBase, 1.f, true);
+        else
+            build_group(c, h, (int)h.S, grouped, P.g[0], 1, gh, false, nullptr, nullptr, nullptr);
+    }
     lg.mark(c, "FP32 group");
     build_group(c, h, (int)h.S, grouped, P.g[1], 2, gl, false, &su, &sv, &colBase, (float)std::ldexp(1.0, zexp));
     lg.mark(c, "FP16 group");
@@ -1899,9 +2014,9 @@ void hmatrix_make_mixed(Context& c, HMatrix& h, double target) {
     P.target = target;
     P.hiCols = hi;
     P.loCols = lc;
-    // bytes read per call: factors, FP16 column scales, and the stored dense
+    // bytes read per call: factors, FP16 (and split FP32) column scales, and the stored dense
     // blocks (none when they are evaluated on the fly: then the points)
-    P.bytes = P.g[0].factorBytes + P.g[1].factorBytes + (uint64_t)lc * 4 +
+    P.bytes = P.g[0].factorBytes + P.g[1].factorBytes + (uint64_t)lc * 4 + (P.g[0].split ? (uint64_t)hi * 4 : 0) +
               (P.nearOtf ? (uint64_t)h.S * h.b * 16 + (uint64_t)h.S * 32
                          : (uint64_t)tot * 4 + (uint64_t)P.numNear * h.b * h.b * 4);
     P.built = true;
@@ -1914,7 +2029,7 @@ bool ensure_mixed_ws(Context& c, MfPlan& P, int NR) {
     changed = ensure_ws(c, P.g[1], NR, sizeof(float)) || changed;
     if (P.x16n < (size_t)P.npad * NR) {
         P.X16.alloc((size_t)P.npad * NR);
-        if (P.nearOtf) P.X16lo.alloc((size_t)P.npad * NR);
+        if (P.nearOtf || P.g[0].split) P.X16lo.alloc((size_t)P.npad * NR);
         P.Xb.alloc((size_t)P.npad * NR * sizeof(float));
         if (P.gather) P.Yp.alloc((size_t)P.npad * NR * sizeof(float));
         P.x16n = (size_t)P.npad * NR;
@@ -1923,6 +2038,9 @@ bool ensure_mixed_ws(Context& c, MfPlan& P, int NR) {
     if (P.xscale.n < (size_t)NR) {
         P.xscale.alloc(NR);
         P.xpart.alloc((size_t)NR * kXSlices);
+        // scales past a call's width are never set; the split FP32 group's
+        // reduction multiplies its (zero) partials of those columns by them
+        HMX_HIP(hipMemsetAsync(P.xscale.p, 0, P.xscale.bytes(), c.stream));
         changed = true;
     }
     return changed;
