@@ -414,8 +414,11 @@ def traffic_bytes(args, dom):
         found = [next((v for k, v in t.items() if k.startswith(p) and isinstance(v, dict)), None) for p in prefixes]
         return None if any(v is None for v in found) else int(sum(v["traffic_bytes"] for v in found))
     if dom == "mixed":  # (phase 1: the conversion and both phase-1 kernels, phase 2)
-        p1 = total(["_ZN3hmx9k_x_mixed", "k_mf_phase1<float, 4, 4", "_ZN3hmx11k_mf_phase1IDF16_Li4ELi4E"])
-        p2 = total(["_ZN3hmx12k_mf_phase2gILi64ELb0EffDF16_Li4E"])
+        # the FP32 group's V as FP16 hi + lo (the plan's default): the split
+        # phase-1 kernel (FP32 z) beside the FP16 one
+        p1 = total(["_ZN3hmx9k_x_mixed", "_ZN3hmx11k_mf_phase1IDF16_Li4ELi4ELb1ELb1EfE",
+                    "_ZN3hmx11k_mf_phase1IDF16_Li4ELi4ELb1ELb0EDF16_"])
+        p2 = total(["_ZN3hmx12k_mf_phase2gILi64ELb0EffDF16_Li4EfLb0E"])
         return None if p1 is None or p2 is None else (p1, p2)
     # phase 2 is the per-leaf pass plus the tall-tile (128-row block) pass
     return total(["k_mv_phase1<1>"] if dom == 1 else ["k_mv_phase2<64, 1>", "k_mv_phase2b<128, 1"])

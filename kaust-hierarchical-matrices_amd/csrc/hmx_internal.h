@@ -280,8 +280,9 @@ struct MfGroup {
     DBuf<MfEntry> grpEntries;      // U at the group's first row
     uint64_t factorBytes = 0;      // V + U bytes of this group (unpadded entries)
     bool vfrag = false;            // V copied in fragment order (else FP64 V in place, column-major)
-    bool split = false;            // FP32 group with V as FP16 hi + lo (phase 1 on the FP16 matrix cores)
-    int64_t vLoOff = 0;            // split: the lo copy's offset from the hi copy (FP16 elements)
+    int split = 0;                 // FP32 group as FP16 hi + lo: 1 V (phase 1 on the FP16 matrix cores),
+                                   // 2 V, U and z (both phases)
+    int64_t vLoOff = 0, uLoOff = 0; // split: the lo copies' offsets from the hi copies (FP16 elements)
     DBuf<unsigned char> zpart, zt; // workspaces
     size_t wsNR = 0;               // pass width the workspaces are sized for
     int ztNR = 0;                  // pass width whose z padding columns were last cleared
@@ -327,6 +328,7 @@ struct MfPlan {
     DBuf<float> xscale, xpart;     // per right-hand-side scale and its per-slice partial maxima
     size_t x16n = 0;
     float loMul = 1.f;             // 2^-E: FP16 z holds V^T x scaled by 2^E (mixed)
+    float hiMul = 1.f;             // 2^(E - Ea): a split FP32 group's z holds V^T x scaled by 2^Ea
     // selection statistics (mixed)
     double frobH2 = 0, loEnergy = 0, target = 0;
     int64_t hiCols = 0, loCols = 0;

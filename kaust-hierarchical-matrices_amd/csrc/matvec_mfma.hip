@@ -161,13 +161,15 @@ __device__ inline size_t xb_index(int64_t row, int s, int sh) {
 }
 
 // ---- phase 1 ---------------------------------------------------------------
-// SPLIT (T = FP16, TZ = FP32): the FP32 group of a mixed plan with V held as
-// FP16 hi + lo (V/scale = hi + lo, lo at V + vLoOff) and the right-hand sides
-// as FP16 hi + lo in units of their max-abs (Xb, Xlo): three FP16 products
-// per step (hi*hi + hi*lo + lo*hi, about 22 bits), z in FP32 multiplied by
-// the column and right-hand-side scales.  On the FP16 matrix cores this
+// SPLIT (T = FP16): the FP32 group of a mixed plan with V held as FP16 hi +
+// lo (V/scale = hi + lo, lo at V + vLoOff) and the right-hand sides as FP16
+// hi + lo in units of their max-abs (Xb, Xlo): three FP16 products per step
+// (hi*hi + hi*lo + lo*hi, about 22 bits).  On the FP16 matrix cores this
 // group reads the same bytes as an FP32 copy, where the FP32 16x16x4 shape
-// at 64 right-hand sides is bound by the matrix cores.
+// at 64 right-hand sides is bound by the matrix cores.  TZ = FP32: z
+// multiplied by the column and right-hand-side scales (true units); TZ =
+// FP16: z times the column scale (which carries the group's power of two)
+// stored as FP16 hi + lo, lo at Zt + zLoOff, in right-hand-side units.
 template <class T, int CG, int NB, bool VF, bool SPLIT = false, class TZ = T>
 __global__ void __launch_bounds__(256) k_mf_phase1(const MfItem* __restrict__ items, int nItems, int chunk,
                                                    const T* __restrict__ Xb, int sh,
@@ -175,9 +177,8 @@ __global__ void __launch_bounds__(256) k_mf_phase1(const MfItem* __restrict__ it
                                                    TZ* __restrict__ Zt, const int32_t* __restrict__ zmap,
                                                    const float* __restrict__ colScaleV, int kvalid,
                                                    const T* __restrict__ Xlo, int64_t vLoOff,
-                                                   const float* __restrict__ rhsScale) {
-    static_assert(!SPLIT || (std::is_same<T, _Float16>::value && std::is_same<TZ, float>::value),
-                  "split operands: FP16 hi + lo, FP32 z");
+                                                   const float* __restrict__ rhsScale, int64_t zLoOff) {
+    static_assert(!SPLIT || std::is_same<T, _Float16>::value, "split operands: FP16 hi + lo");
     using M = Mfma<T>;
     // the products: FP16 on the 16x16x32 shape (a lane's 16 values of a
     // step feed two instructions instead of four; the k order inside a step
@@ -375,7 +376,16 @@ __global__ void __launch_bounds__(256) k_mf_phase1(const MfItem* __restrict__ it
 #pragma unroll
                 for (int nb = 0; nb < NB; ++nb) {
                     tv4 o;
-                    if constexpr (SPLIT) {
+                    if constexpr (SPLIT && sizeof(TZ) == 2) {
+                        tv4 ol;
+#pragma unroll
+                        for (int v = 0; v < 4; ++v) {
+                            const float zv = acc[g][nb][v] * cs[v];
+                            o[v] = (TZ)zv;
+                            ol[v] = (TZ)(zv - (float)o[v]);
+                        }
+                        *reinterpret_cast<tv4*>(Zt + zLoOff + zt_index<TZ, NR>(zc, 16 * nb + lc)) = ol;
+                    } else if constexpr (SPLIT) {
                         const ACC rs = 16 * nb + lc < kvalid ? (ACC)rhsScale[16 * nb + lc] : (ACC)0;
 #pragma unroll
                         for (int v = 0; v < 4; ++v) o[v] = (TZ)(acc[g][nb][v] * cs[v] * rs);
@@ -424,12 +434,14 @@ __global__ void __launch_bounds__(256) k_mf_phase1(const MfItem* __restrict__ it
 // lane = right-hand side (NR = 64; NR = 16 uses 16 lanes): the partials are
 // summed by 8 interleaved accumulators folded in a fixed order, so the long
 // columns of the top levels (up to m/1024 partials) are not one serial chain.
-template <class T, class ACC, int NR>
+// SPL (T = FP16): z stored as FP16 hi + lo, lo at Zt + zLoOff
+template <class T, class ACC, int NR, bool SPL = false>
 __global__ void __launch_bounds__(256) k_mf_reduce(int64_t nred, int64_t ldz, const int64_t* __restrict__ redCol,
                                                    const int64_t* __restrict__ colPart,
                                                    const int32_t* __restrict__ colNpart,
                                                    const float* __restrict__ colScale, const ACC* __restrict__ zpart,
-                                                   T* __restrict__ Zt, const float* __restrict__ rhsScale) {
+                                                   T* __restrict__ Zt, const float* __restrict__ rhsScale,
+                                                   int64_t zLoOff) {
     const int64_t g = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
     const int s = threadIdx.x & 63;
     if (g >= nred || s >= NR) return;
@@ -449,7 +461,9 @@ __global__ void __launch_bounds__(256) k_mf_reduce(int64_t nred, int64_t ldz, co
     const int64_t zc = redCol[g];
     if (colScale) v *= (ACC)colScale[zc];
     if (rhsScale) v *= (ACC)rhsScale[s];  // split FP32 group: right-hand sides in units of their max-abs
-    Zt[zt_index<T, NR>(zc, s)] = (T)v;
+    const size_t zi = zt_index<T, NR>(zc, s);
+    Zt[zi] = (T)v;
+    if constexpr (SPL) Zt[(size_t)zLoOff + zi] = (T)(v - (ACC)(T)v);
 }
 
 // ---- phase 2 -----------------------------------------------------------------
@@ -545,6 +559,88 @@ __device__ inline void lr_rows(const MfEntry* __restrict__ ents, int e0, int e1,
         e = ne;
         c0 = nc;
         en = nen;
+    }
+}
+
+// lr_rows for a split group: U and z each as FP16 hi + lo (the lo parts
+// uLoOff / zLoOff elements past the hi parts, same fragment orders as an
+// FP16 group), three FP16 16x16x32 products per step (hi*hi + hi*lo +
+// lo*hi).  No register double buffer: the two operand sets take the
+// registers a prefetch would.
+template <int BL, int NB>
+__device__ inline void lr_rows_split(const MfEntry* __restrict__ ents, int e0, int e1, const _Float16* __restrict__ Z,
+                                     int64_t zLoOff, int64_t uLoOff, int rowOff, int lg, int lc, int kvalid,
+                                     v4f (&acc)[BL / 16][NB]) {
+    using M = Mfma<_Float16>;
+    using MX = Mfma32h;
+    constexpr int KL = M::KL, VL = 4 * KL, SS = 16 * KL, NR = 16 * NB, RG = BL / 16, E16 = 8, NP = VL / E16,
+                  ZP = z_piece<_Float16>();
+    static_assert(ZP == E16, "16-byte z pieces");
+    _Float16 ah[RG][VL], al[RG][VL], bh[NB][VL], bl[NB][VL];
+    for (int e = e0; e < e1; ++e) {
+        const MfEntry en = ents[e];
+        for (int c0 = 0; c0 < en.rpad; c0 += SS) {
+            const int cc = c0 + VL * lg;
+            if (cc < en.rpad) {
+                const int pstride = 16 * E16 * (min(SS, en.rpad - c0) / VL);
+#pragma unroll
+                for (int rg = 0; rg < RG; ++rg) {
+                    const _Float16* U = (const _Float16*)en.u + (size_t)(rowOff / 16 + rg) * 16 * en.rpad +
+                                        (size_t)16 * c0 + (16 * lg + lc) * E16;
+#pragma unroll
+                    for (int p = 0; p < NP; ++p) {
+                        vload16_nt(U + p * pstride, &ah[rg][p * E16]);
+                        vload16_nt(U + uLoOff + p * pstride, &al[rg][p * E16]);
+                    }
+                }
+#pragma unroll
+                for (int nb = 0; nb < NB; ++nb) {
+                    if (16 * nb + lc < kvalid) {
+                        const _Float16* zp = Z + zt_index<_Float16, NR>(en.zcol + cc, 16 * nb + lc);
+#pragma unroll
+                        for (int p = 0; p < VL / ZP; ++p) {
+                            vload16(zp + p * NR * ZP, &bh[nb][p * ZP]);
+                            vload16(zp + zLoOff + p * NR * ZP, &bl[nb][p * ZP]);
+                        }
+                    } else {
+                        vzero(bh[nb]);
+                        vzero(bl[nb]);
+                    }
+                }
+            } else {
+#pragma unroll
+                for (int rg = 0; rg < RG; ++rg) {
+                    vzero(ah[rg]);
+                    vzero(al[rg]);
+                }
+#pragma unroll
+                for (int nb = 0; nb < NB; ++nb) {
+                    vzero(bh[nb]);
+                    vzero(bl[nb]);
+                }
+            }
+#pragma unroll
+            for (int q = 0; q < VL / MX::KL; ++q)
+#pragma unroll
+                for (int rg = 0; rg < RG; ++rg)
+#pragma unroll
+                    for (int nb = 0; nb < NB; ++nb)
+                        acc[rg][nb] = MX::op(MX::make(&ah[rg][q * MX::KL]), MX::make(&bh[nb][q * MX::KL]), acc[rg][nb]);
+#pragma unroll
+            for (int q = 0; q < VL / MX::KL; ++q)
+#pragma unroll
+                for (int rg = 0; rg < RG; ++rg)
+#pragma unroll
+                    for (int nb = 0; nb < NB; ++nb)
+                        acc[rg][nb] = MX::op(MX::make(&ah[rg][q * MX::KL]), MX::make(&bl[nb][q * MX::KL]), acc[rg][nb]);
+#pragma unroll
+            for (int q = 0; q < VL / MX::KL; ++q)
+#pragma unroll
+                for (int rg = 0; rg < RG; ++rg)
+#pragma unroll
+                    for (int nb = 0; nb < NB; ++nb)
+                        acc[rg][nb] = MX::op(MX::make(&al[rg][q * MX::KL]), MX::make(&bh[nb][q * MX::KL]), acc[rg][nb]);
+        }
     }
 }
 
@@ -710,9 +806,12 @@ __device__ inline void near_otf(int leaf, int w, const float4* __restrict__ PL, 
 // (P) the diagonal and near-field blocks are evaluated on the fly and added in
 // the same units (right-hand sides divided by their max-abs); then the sum is
 // multiplied by loScale[rhs] (those max-abs values) before the stored dense
-// blocks (without P) and the FP32 group add theirs.
-template <int BL, bool UCM, class TD, class TA, class TB, int NB, class TO>
-__global__ void __launch_bounds__(256) k_mf_phase2g(int S, const int32_t* __restrict__ ptrA, const MfEntry* __restrict__ entA,
+// blocks (without P) and the FP32 group add theirs.  SPA: the FP32 group
+// held as FP16 hi + lo (U, and z in right-hand-side units times 2^Ea) goes
+// first instead, its sum multiplied by hiMul = 2^(E - Ea) into the FP16
+// group's units.
+template <int BL, bool UCM, class TD, class TA, class TB, int NB, class TO, bool SPA = false>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(SPA ? 2 : 1))) k_mf_phase2g(int S, const int32_t* __restrict__ ptrA, const MfEntry* __restrict__ entA,
                                                     const int32_t* __restrict__ gptrA,
                                                     const MfEntry* __restrict__ gentA, const TA* __restrict__ ZA,
                                                     const int32_t* __restrict__ ptrB, const MfEntry* __restrict__ entB,
@@ -725,7 +824,9 @@ __global__ void __launch_bounds__(256) k_mf_phase2g(int S, const int32_t* __rest
                                                     const float4* __restrict__ P, const double4* __restrict__ O,
                                                     float nlog2e,
                                                     const _Float16* __restrict__ Xh,
-                                                    const _Float16* __restrict__ Xl) {
+                                                    const _Float16* __restrict__ Xl, int64_t uLoA, int64_t zLoA,
+                                                    float hiMul) {
+    static_assert(!SPA || std::is_same<TA, _Float16>::value, "split group: FP16 hi + lo");
     using MA = Mfma<TA>;
     using accv = typename MA::accv;
     static_assert(sizeof(typename Mfma<TB>::acc_t) == sizeof(typename MA::acc_t) &&
@@ -746,6 +847,15 @@ __global__ void __launch_bounds__(256) k_mf_phase2g(int S, const int32_t* __rest
     for (int rg = 0; rg < RG; ++rg)
 #pragma unroll
         for (int nb = 0; nb < NB; ++nb) acc[rg][nb] = accv{0, 0, 0, 0};
+    if constexpr (SPA) {
+        lr_rows_split<BL, NB>(entA, ptrA[leaf], ptrA[leaf + 1], ZA, zLoA, uLoA, 0, lg, lc, kvalid, acc);
+        if (gptrA)
+            lr_rows_split<BL, NB>(gentA, gptrA[grp], gptrA[grp + 1], ZA, zLoA, uLoA, BL * w, lg, lc, kvalid, acc);
+#pragma unroll
+        for (int rg = 0; rg < RG; ++rg)
+#pragma unroll
+            for (int nb = 0; nb < NB; ++nb) acc[rg][nb] *= hiMul;
+    }
     if (ptrB) {
         lr_rows<BL, false, TB, NB>(entB, ptrB[leaf], ptrB[leaf + 1], ZB, 0, lg, lc, kvalid, acc);
         if (gptrB) lr_rows<BL, false, TB, NB>(gentB, gptrB[grp], gptrB[grp + 1], ZB, BL * w, lg, lc, kvalid, acc);
@@ -767,8 +877,10 @@ __global__ void __launch_bounds__(256) k_mf_phase2g(int S, const int32_t* __rest
         }
     }
     if (!P) dense_rows<BL, TD, NB>(leaf, Drm, nearPtr, nearB, Xb, lg, lc, kvalid, acc);
-    lr_rows<BL, UCM, TA, NB>(entA, ptrA[leaf], ptrA[leaf + 1], ZA, 0, lg, lc, kvalid, acc);
-    if (gptrA) lr_rows<BL, UCM, TA, NB>(gentA, gptrA[grp], gptrA[grp + 1], ZA, BL * w, lg, lc, kvalid, acc);
+    if constexpr (!SPA) {
+        lr_rows<BL, UCM, TA, NB>(entA, ptrA[leaf], ptrA[leaf + 1], ZA, 0, lg, lc, kvalid, acc);
+        if (gptrA) lr_rows<BL, UCM, TA, NB>(gentA, gptrA[grp], gptrA[grp + 1], ZA, BL * w, lg, lc, kvalid, acc);
+    }
 #pragma unroll
     for (int rg = 0; rg < RG; ++rg)
 #pragma unroll
@@ -1000,6 +1112,35 @@ __global__ void __launch_bounds__(256) k_cvt_rowmajor(int64_t n, const TrTile* _
     }
 }
 
+// the same for a split group's U: FP16 hi + lo (formed in FP64), lo at loOff
+__global__ void __launch_bounds__(256) k_cvt_rowmajor_split(int64_t n, const TrTile* __restrict__ tiles,
+                                                            const CvCol* __restrict__ cols, _Float16* __restrict__ U,
+                                                            int64_t loOff) {
+    __shared__ double sh[32][33];
+    const TrTile t = tiles[blockIdx.x];
+    const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;  // 32 x 8
+    for (int cc = ty; cc < 32; cc += 8) {
+        const int c = t.c0 + cc, r = t.r0 + tx;
+        double v = 0.0;
+        if (c < t.cnt && r < t.m) {
+            const CvCol d = cols[t.colDesc + c];
+            v = d.src ? d.src[r] / (double)d.scale : 0.0;
+        }
+        sh[cc][tx] = v;
+    }
+    __syncthreads();
+    for (int rr = ty; rr < 32; rr += 8) {
+        const int r = t.r0 + rr, c = t.c0 + tx;
+        if (r < t.m && c < t.rpad) {
+            const double x = sh[tx][rr];
+            const _Float16 hi = (_Float16)x;
+            const size_t o = (size_t)t.dst + urm_index<_Float16>(r, c, t.rpad);
+            U[o] = hi;
+            U[(size_t)loOff + o] = (_Float16)(x - (double)hi);
+        }
+    }
+}
+
 template <class T>
 __global__ void __launch_bounds__(256) k_diag_rowmajor(int64_t total, int b, const double* __restrict__ D,
                                                        T* __restrict__ Drm) {
@@ -1097,12 +1238,13 @@ struct GroupTile {
 void build_group(Context& c, const HMatrix& h, int S, bool grouped, MfGroup& G, int type,
                  const std::vector<GroupTile>& tiles, bool vdirect, const std::vector<float>* su,
                  const std::vector<float>* sv, const std::vector<int64_t>* colBase, float zmul = 1.f,
-                 bool split = false) {
+                 int split = 0) {
     StageLog lg("plan group");
     HMX_CHECK(!split || (type == 1 && !vdirect && su), HMX_ESTATE, "split V: FP32 groups with column scales");
     const bool udirect = vdirect;
     const int b = (int)h.b;
     const size_t es = tsize(type);
+    const size_t esu = split == 2 ? sizeof(_Float16) : es;  // split U: offsets into the hi copy
     G.type = type;
     const int T = (int)tiles.size();
     auto cnt_of = [&](int t) { return (int)tiles[t].cols.size(); };
@@ -1165,7 +1307,7 @@ void build_group(Context& c, const HMatrix& h, int S, bool grouped, MfGroup& G, 
         auto at = [&](int r0) {
             MfEntry e = proto;
             e.u = ucm ? (const void*)(ucm + (size_t)r0)
-                      : (const void*)(uintptr_t)(((size_t)(uintptr_t)proto.u + (size_t)r0 * proto.rpad) * es);
+                      : (const void*)(uintptr_t)(((size_t)(uintptr_t)proto.u + (size_t)r0 * proto.rpad) * esu);
             return e;
         };
         if (grouped && bus >= 4)
@@ -1316,7 +1458,8 @@ void build_group(Context& c, const HMatrix& h, int S, bool grouped, MfGroup& G, 
         });
     G.vfrag = vfrag;
     G.split = split;
-    G.vLoOff = split ? vElems : 0;  // the lo copy follows the hi copy (2 x 2 bytes per element)
+    G.vLoOff = split ? vElems : 0;  // the lo copies follow the hi copies (2 x 2 bytes per element)
+    G.uLoOff = split == 2 ? uElems : 0;
     G.numColsPad = zcol;
     G.numPart = slot;
     G.factorBytes = fbytes;
@@ -1383,6 +1526,8 @@ void build_group(Context& c, const HMatrix& h, int S, bool grouped, MfGroup& G, 
     if (!trt.empty()) {
         const unsigned g = (unsigned)trt.size();
         if (type == 0) k_cvt_rowmajor<double><<<g, 256, 0, c.stream>>>((int64_t)trt.size(), d_t.p, d_u.p, (double*)G.Urm.p);
+        else if (split == 2)
+            k_cvt_rowmajor_split<<<g, 256, 0, c.stream>>>((int64_t)trt.size(), d_t.p, d_u.p, (_Float16*)G.Urm.p, uElems);
         else if (type == 1) k_cvt_rowmajor<float><<<g, 256, 0, c.stream>>>((int64_t)trt.size(), d_t.p, d_u.p, (float*)G.Urm.p);
         else k_cvt_rowmajor<_Float16><<<g, 256, 0, c.stream>>>((int64_t)trt.size(), d_t.p, d_u.p, (_Float16*)G.Urm.p);
         HMX_LAUNCHED();
@@ -1556,19 +1701,26 @@ void build_fp64_plan_set(Context& c, const std::vector<HMatrix*>& hs, MfPlan& P)
     P.built = true;
 }
 
+// zLoOff >= 0 (T = FP16): z as FP16 hi + lo, lo at zLoOff
 template <class ACC, class T>
-void reduce_group(Context& c, MfGroup& G, int NR, const float* rhsScale = nullptr) {
+void reduce_group(Context& c, MfGroup& G, int NR, const float* rhsScale = nullptr, int64_t zLoOff = -1) {
     if (G.numRed == 0) return;
     const unsigned blocks = (unsigned)((G.numRed + 3) / 4);
     const float* cs = G.colScale.n ? G.colScale.p : nullptr;
-    if (NR == 64)
-        k_mf_reduce<T, ACC, 64><<<blocks, 256, 0, c.stream>>>(G.numRed, G.numColsPad, G.redCol.p, G.colPart.p,
-                                                              G.colNpart.p, cs, (const ACC*)G.zpart.p, (T*)G.zt.p,
-                                                              rhsScale);
-    else
-        k_mf_reduce<T, ACC, 16><<<blocks, 256, 0, c.stream>>>(G.numRed, G.numColsPad, G.redCol.p, G.colPart.p,
-                                                              G.colNpart.p, cs, (const ACC*)G.zpart.p, (T*)G.zt.p,
-                                                              rhsScale);
+    auto launch = [&](auto k) {
+        k<<<blocks, 256, 0, c.stream>>>(G.numRed, G.numColsPad, G.redCol.p, G.colPart.p, G.colNpart.p, cs,
+                                        (const ACC*)G.zpart.p, (T*)G.zt.p, rhsScale, zLoOff);
+    };
+    if constexpr (sizeof(T) == 2) {
+        if (zLoOff >= 0) {
+            if (NR == 64) launch(k_mf_reduce<T, ACC, 64, true>);
+            else launch(k_mf_reduce<T, ACC, 16, true>);
+            HMX_LAUNCHED();
+            return;
+        }
+    }
+    if (NR == 64) launch(k_mf_reduce<T, ACC, 64>);
+    else launch(k_mf_reduce<T, ACC, 16>);
     HMX_LAUNCHED();
 }
 
@@ -1616,22 +1768,29 @@ void phase1(Context& c, MfGroup& G, const T* Xb, int sh, int kvalid, int xcdChun
     // touched by the other path (one wave per SIMD, sensitive to it)
     auto k1 = G.vfrag ? k_mf_phase1<T, CG, NB, true> : k_mf_phase1<T, CG, NB, false>;
     k1<<<blocks, 256, 0, c.stream>>>(G.items.p, G.numItems, chunk, Xb, sh, (typename Mfma<T>::acc_t*)G.zpart.p,
-                                     (T*)G.zt.p, G.zmap.p, cs, kvalid, nullptr, 0, nullptr);
+                                     (T*)G.zt.p, G.zmap.p, cs, kvalid, nullptr, 0, nullptr, 0);
     HMX_LAUNCHED();
 }
 
-// phase 1 of a mixed plan's FP32 group held as FP16 hi + lo (MfGroup::split)
-template <int NB>
+// the lo half of a split group's FP16 z (MfGroup::split == 2): after the hi
+// half, whose size is fixed by the widest pass the workspace was made for
+int64_t split_zlo(const MfGroup& G) { return std::max<int64_t>(1, G.numColsPad) * (int64_t)G.wsNR; }
+
+// phase 1 of a mixed plan's FP32 group held as FP16 hi + lo (MfGroup::split):
+// TZ = float writes FP32 z in true units (rhsScale), TZ = _Float16 FP16 hi +
+// lo z in right-hand-side units
+template <int NB, class TZ>
 void phase1_split(Context& c, MfGroup& G, const _Float16* Xh, const _Float16* Xl, int sh, int kvalid,
                   const float* rhsScale) {
     if (G.numItems == 0) return;
-    HMX_CHECK(G.split && G.vfrag && G.colScaleV.n && Xl, HMX_ESTATE, "split phase 1: plan not built for it");
+    HMX_CHECK(G.split == (sizeof(TZ) == 2 ? 2 : 1) && G.vfrag && G.colScaleV.n && Xl, HMX_ESTATE,
+              "split phase 1: plan not built for it");
     const int chunk = phase1_chunk();
     unsigned blocks = (unsigned)((G.numItems + 3) / 4);
     if (chunk > 0) blocks = (blocks + 8 * chunk - 1) / (8 * chunk) * (8 * chunk);
-    k_mf_phase1<_Float16, 4, NB, true, true, float><<<blocks, 256, 0, c.stream>>>(
-        G.items.p, G.numItems, chunk, Xh, sh, (float*)G.zpart.p, (float*)G.zt.p, G.zmap.p, G.colScaleV.p, kvalid, Xl,
-        G.vLoOff, rhsScale);
+    k_mf_phase1<_Float16, 4, NB, true, true, TZ><<<blocks, 256, 0, c.stream>>>(
+        G.items.p, G.numItems, chunk, Xh, sh, (float*)G.zpart.p, (TZ*)G.zt.p, G.zmap.p, G.colScaleV.p, kvalid, Xl,
+        G.vLoOff, rhsScale, sizeof(TZ) == 2 ? split_zlo(G) : 0);
     HMX_LAUNCHED();
 }
 
@@ -1680,7 +1839,7 @@ void run_fp64(Context& c, MfPlan& P, const double* X, int64_t ldx, int kk, doubl
         k2<<<(P.S + 3) / 4, 256, 0, c.stream>>>(P.S, G.leafPtr.p, G.entries.p, gp, ge, (const double*)G.zt.p,
                                                 nullptr, nullptr, nullptr, nullptr, nullptr, (const double*)P.Drm.p,
                                                 P.nearPtr.p, P.near.p, Xb, Yo, ldo, kk, nullptr, 1.f, nullptr, nullptr,
-                                                0.f, nullptr, nullptr);
+                                                0.f, nullptr, nullptr, 0, 0, 1.f);
     }
     HMX_LAUNCHED();
     if (P.gather) {
@@ -1712,26 +1871,37 @@ void run_mixed(Context& c, MfPlan& P, const float* X, int64_t ldx, int kk, float
                                                             P.X16.p, P.X16lo.n ? P.X16lo.p : nullptr);
     HMX_LAUNCHED();
     const float* Xb = (const float*)P.Xb.p;
-    if (Ghi.split) phase1_split<NB>(c, Ghi, P.X16.p, P.X16lo.p, sh16, kk, P.xscale.p);
+    if (Ghi.split == 2) phase1_split<NB, _Float16>(c, Ghi, P.X16.p, P.X16lo.p, sh16, kk, nullptr);
+    else if (Ghi.split == 1) phase1_split<NB, float>(c, Ghi, P.X16.p, P.X16lo.p, sh16, kk, P.xscale.p);
     else phase1<float, 4, NB>(c, Ghi, Xb, sh32, kk);
     phase1<_Float16, 4, NB>(c, Glo, P.X16.p, sh16, kk);
     ev.rec(c, 1);
-    reduce_group<float, float>(c, Ghi, NR, Ghi.split ? P.xscale.p : nullptr);
+    if (Ghi.split == 2) reduce_group<float, _Float16>(c, Ghi, NR, nullptr, split_zlo(Ghi));
+    else reduce_group<float, float>(c, Ghi, NR, Ghi.split ? P.xscale.p : nullptr);
     reduce_group<float, _Float16>(c, Glo, NR);
     ev.rec(c, 2);
     float* Yo = P.gather ? (float*)P.Yp.p : Y;
     const int ldo = (int)(P.gather ? P.npad : ldy);
     {
         const bool gr = Ghi.grouped && Glo.grouped;
-        auto k2 = P.b == 64 ? k_mf_phase2g<64, false, float, float, _Float16, NB, float>
-                            : k_mf_phase2g<32, false, float, float, _Float16, NB, float>;
-        k2<<<(P.S + 3) / 4, 256, 0, c.stream>>>(
-            P.S, Ghi.leafPtr.p, Ghi.entries.p, gr ? Ghi.grpPtr.p : nullptr, gr ? Ghi.grpEntries.p : nullptr,
-            (const float*)Ghi.zt.p, Glo.leafPtr.p, Glo.entries.p, gr ? Glo.grpPtr.p : nullptr,
-            gr ? Glo.grpEntries.p : nullptr, (const _Float16*)Glo.zt.p, (const float*)P.Drm.p,
-            P.nearOtf ? P.gnearPtr.p : P.nearPtr.p, P.nearOtf ? P.gnear.p : P.near.p,
-            Xb, Yo, ldo, kk, P.xscale.p, P.loMul, P.nearOtf ? reinterpret_cast<const float4*>(P.ptsL.p) : nullptr,
-            reinterpret_cast<const double4*>(P.origin.p), P.nlog2e, P.X16.p, P.X16lo.p);
+        const bool spa = Ghi.split == 2;
+        // a split group's z is FP16 hi + lo (the kernel's ZA), else FP32
+        auto launch = [&](auto k2, const auto* za) {
+            k2<<<(P.S + 3) / 4, 256, 0, c.stream>>>(
+                P.S, Ghi.leafPtr.p, Ghi.entries.p, gr ? Ghi.grpPtr.p : nullptr, gr ? Ghi.grpEntries.p : nullptr, za,
+                Glo.leafPtr.p, Glo.entries.p, gr ? Glo.grpPtr.p : nullptr, gr ? Glo.grpEntries.p : nullptr,
+                (const _Float16*)Glo.zt.p, (const float*)P.Drm.p, P.nearOtf ? P.gnearPtr.p : P.nearPtr.p,
+                P.nearOtf ? P.gnear.p : P.near.p, Xb, Yo, ldo, kk, P.xscale.p, P.loMul,
+                P.nearOtf ? reinterpret_cast<const float4*>(P.ptsL.p) : nullptr,
+                reinterpret_cast<const double4*>(P.origin.p), P.nlog2e, P.X16.p, P.X16lo.p, spa ? Ghi.uLoOff : 0,
+                spa ? split_zlo(Ghi) : 0, P.hiMul);
+        };
+        const _Float16* zh = (const _Float16*)Ghi.zt.p;
+        const float* zf = (const float*)Ghi.zt.p;
+        if (spa && P.b == 64) launch(k_mf_phase2g<64, false, float, _Float16, _Float16, NB, float, true>, zh);
+        else if (spa) launch(k_mf_phase2g<32, false, float, _Float16, _Float16, NB, float, true>, zh);
+        else if (P.b == 64) launch(k_mf_phase2g<64, false, float, float, _Float16, NB, float>, zf);
+        else launch(k_mf_phase2g<32, false, float, float, _Float16, NB, float>, zf);
     }
     HMX_LAUNCHED();
     if (P.gather) {
@@ -1978,16 +2148,34 @@ void hmatrix_make_mixed(Context& c, HMatrix& h, double target) {
     }
     lg.mark(c, "column split");
     {
-        // the FP32 group's V as FP16 hi + lo scaled by max|v_j| (phase 1 on the
-        // FP16 matrix cores); U stays FP32 and unscaled.  HMX_MIXED_SPLIT=0
-        // keeps an FP32 copy of V.
+        // The FP32 group as FP16 hi + lo scaled per column (HMX_MIXED_SPLIT:
+        // 1, the default, V only, U and z FP32; 2 V, U and z, both phases on
+        // the FP16 matrix cores; 0 FP32 copies).  Mode 2's z holds V^T x in
+        // right-hand-side units times su_j sv_j 2^Ea, bounded by
+        // m_j su_j sv_j 2^Ea <= 2^15 as for the FP16 group; phase 2 brings
+        // the group's sum to the FP16 group's units with 2^(E - Ea).  The one
+        // exponent leaves the group's small columns with z's lo part in the
+        // FP16 subnormals (about 17 bits), an error outside the energy
+        // budget: N = 2^20 3D at a 1e-5 target, 64 right-hand sides, 7.1e-6
+        // instead of 5.9e-6 (64-RHS call 10.7 instead of 11.2 ms), so it is
+        // not the default.
         const char* e = std::getenv("HMX_MIXED_SPLIT");
-        const bool split = !e || std::atoi(e) != 0;
-        const std::vector<float> ones(split ? su.size() : 0, 1.f);
-        if (split)
-            build_group(c, h, (int)h.S, grouped, P.g[0], 1, gh, false, &ones, &sv, &colBase, 1.f, true);
-        else
+        const int split = e ? std::min(2, std::max(0, std::atoi(e))) : 1;
+        if (split == 2) {
+            double zbA = 0;
+            for (size_t t = 0; t < gh.size(); ++t)
+                for (int j : gh[t].cols)
+                    zbA = std::max(zbA, (double)su[colBase[t] + j] * (double)sv[colBase[t] + j] * tiles[t].m);
+            const int ea = zbA > 0 ? (int)std::floor(std::log2(32768.0 / zbA)) : 0;
+            build_group(c, h, (int)h.S, grouped, P.g[0], 1, gh, false, &su, &sv, &colBase, (float)std::ldexp(1.0, ea),
+                        2);
+            P.hiMul = (float)std::ldexp(1.0, zexp - ea);
+        } else if (split == 1) {
+            const std::vector<float> ones(su.size(), 1.f);
+            build_group(c, h, (int)h.S, grouped, P.g[0], 1, gh, false, &ones, &sv, &colBase, 1.f, 1);
+        } else {
             build_group(c, h, (int)h.S, grouped, P.g[0], 1, gh, false, nullptr, nullptr, nullptr);
+        }
     }
     lg.mark(c, "FP32 group");
     build_group(c, h, (int)h.S, grouped, P.g[1], 2, gl, false, &su, &sv, &colBase, (float)std::ldexp(1.0, zexp));
